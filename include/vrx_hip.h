@@ -1,0 +1,299 @@
+/*
+ * vrx_hip.h -- thin C ABI over the gfx950 HIP kernels of the visrtx_amd device.
+ *
+ * Plain pointers, sizes and opaque stream handles only; every call returns an
+ * int status (0 = success, <0 = error, message via vrx_last_error()).  All
+ * pointers named d_* are device (HBM) pointers owned by the caller.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * VisRTX source tree):
+ *   vrx_bvh_build        optix_visrtx.cpp:37-121 buildOptixBVH (OptiX accel build;
+ *                        called from scene/Group.cpp:189-266, scene/World.cpp:202-254)
+ *   vrx_tlas_*           scene/World.cpp:256-341 populateOptixInstances + TLAS build
+ *   vrx_render           frame/Frame.cu:282-289 optixLaunch of the raygen programs
+ *                        renderer/Raycast_ptx.cu:60-179 and
+ *                        renderer/AmbientOcclusion_ptx.cu:86-220
+ *   vrx_frame_clear      frame/Frame.cu:590-660 Frame::newFrame thrust::fill_n clears
+ *   vrx_stream_* / vrx_event_* / vrx_launch_host_func
+ *                        the cudaStream / cudaEvent / cudaLaunchHostFunc use in
+ *                        frame/Frame.cu:49-53,255,295-307 and optix_visrtx.h:165
+ *   vrx_diffvol_*        no reference counterpart (the reference has only the
+ *                        forward integral, gpu/volumeIntegration.h:64-165); the
+ *                        adjoint is new, for torch.autograd
+ *   vrx_batch_*          no reference counterpart (one optixLaunch per view);
+ *                        multi-view persistent launch
+ */
+#ifndef VRX_HIP_H
+#define VRX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *vrx_stream; /* hipStream_t; NULL = default stream */
+typedef void *vrx_event;  /* hipEvent_t */
+
+enum vrx_status {
+  VRX_OK = 0,
+  VRX_ERR_HIP = -1,
+  VRX_ERR_ARG = -2,
+  VRX_ERR_NOMEM = -3,
+  VRX_ERR_LIMIT = -4,
+};
+
+enum vrx_prim_type {
+  VRX_TRIANGLE = 0,
+  VRX_SPHERE = 1,
+  VRX_CYLINDER = 2,
+  VRX_BOX = 3 /* plain AABB input (used for the instance level) */
+};
+
+enum vrx_memcpy_kind { VRX_H2D = 0, VRX_D2H = 1, VRX_D2D = 2 };
+
+/* ---- runtime plumbing ---------------------------------------------------- */
+const char *vrx_last_error(void);
+int vrx_version(void);
+int vrx_device_count(int *count);
+int vrx_set_device(int device);
+int vrx_get_device(int *device);
+int vrx_device_info(int device, char *name, size_t name_len, int *num_cus, size_t *hbm_bytes);
+int vrx_malloc(void **d_ptr, size_t bytes);
+int vrx_free(void *d_ptr);
+int vrx_host_malloc(void **h_ptr, size_t bytes);
+int vrx_host_free(void *h_ptr);
+int vrx_memcpy(void *dst, const void *src, size_t bytes, int kind, vrx_stream s);
+int vrx_memcpy_sync(void *dst, const void *src, size_t bytes, int kind);
+int vrx_fill_u32(void *d_ptr, uint32_t value, size_t count, vrx_stream s);
+int vrx_stream_create(vrx_stream *s);
+int vrx_stream_destroy(vrx_stream s);
+int vrx_stream_sync(vrx_stream s);
+int vrx_device_sync(void);
+int vrx_event_create(vrx_event *e);
+int vrx_event_destroy(vrx_event e);
+int vrx_event_record(vrx_event e, vrx_stream s);
+int vrx_event_sync(vrx_event e);
+int vrx_event_query(vrx_event e); /* 1 = complete, 0 = pending, <0 error */
+int vrx_event_elapsed_ms(vrx_event start, vrx_event end, float *ms);
+int vrx_launch_host_func(vrx_stream s, void (*fn)(void *), void *user);
+
+/* ---- LBVH build ---------------------------------------------------------- */
+typedef struct vrx_prims {
+  int32_t type;           /* vrx_prim_type */
+  uint32_t n;             /* number of primitives */
+  const float *d_position;/* float3 vertices / centers (VRX_BOX: float[6] boxes) */
+  const uint32_t *d_index;/* uint3 / uint2 / uint primitive.index or NULL */
+  const float *d_radius;  /* sphere: per-vertex radius; cylinder: per-primitive; or NULL */
+  float global_radius;
+} vrx_prims;
+
+/* Output buffers of one BVH.  Sizes (bytes) from vrx_bvh_sizes. */
+typedef struct vrx_bvh {
+  void *d_nodes;          /* (n-1) * 64 B: both child boxes + children + heights */
+  void *d_leaves;         /* n * leaf record (48/16/32/0 B for tri/sphere/cyl/box) */
+  uint32_t *d_leaf_ids;   /* n: sorted leaf -> original primitive index */
+  void *d_header;         /* 64 B: root box (6 f32), height, root, n, type */
+} vrx_bvh;
+
+typedef struct vrx_bvh_sizes {
+  size_t nodes, leaves, leaf_ids, header, workspace;
+} vrx_bvh_sizes;
+
+int vrx_bvh_query_sizes(int32_t type, uint32_t n, vrx_bvh_sizes *out);
+/* Morton -> LDS-staged stable radix sort -> Karras -> atomic-flag refit. */
+int vrx_bvh_build(const vrx_prims *prims, const vrx_bvh *out, void *d_workspace,
+    size_t workspace_bytes, vrx_stream s);
+/* Debug/parity hooks: read back the sorted Morton codes (n) and the order. */
+int vrx_bvh_build_debug(const vrx_prims *prims, const vrx_bvh *out, void *d_workspace,
+    size_t workspace_bytes, uint32_t *d_codes_out, float *d_cbox_out, vrx_stream s);
+
+/* header layout (host view, 64 B) */
+typedef struct vrx_bvh_header {
+  float lo[3], hi[3];
+  uint32_t height;        /* max leaf depth + 1 (a single leaf has height 1) */
+  uint32_t root;          /* 0 or 0x80000000 (single leaf) */
+  uint32_t n;
+  int32_t type;
+  uint32_t pad[6];
+} vrx_bvh_header;
+
+/* ---- scene description for rendering -------------------------------------- */
+typedef struct vrx_geom {
+  int32_t type;
+  uint32_t n;
+  vrx_bvh bvh;
+  /* original attribute arrays (device) for hit population */
+  const float *d_position;
+  const uint32_t *d_index;
+  const float *d_radius;
+  float global_radius;
+  const uint32_t *d_primitive_id;
+  const float *d_vertex_normal;
+  int32_t caps;
+  int32_t cull_backfaces;
+  /* matte material (VALUE path) */
+  float color[3];
+  float opacity;
+  uint32_t surface_id;
+  uint32_t pad;
+} vrx_geom;
+
+typedef struct vrx_entry {
+  float xfm[12];          /* object -> world, column-major 3x4 */
+  float inv[12];          /* world -> object */
+  uint32_t identity;
+  uint32_t geom;
+  uint32_t inst_id;
+  uint32_t pad;
+} vrx_entry;
+
+typedef struct vrx_volume {
+  const float *d_data;    /* x fastest */
+  uint32_t dims[3];
+  float origin[3];
+  float spacing[3];
+  const float *d_lut;     /* 256 x RGBA (alpha already scaled by opacity) */
+  float value_range[2];
+  float unit_distance;
+  uint32_t id;
+  uint32_t inst_id;
+  uint32_t pad;
+} vrx_volume;
+
+typedef struct vrx_scene {
+  const vrx_geom *d_geoms;      /* device array */
+  uint32_t num_geoms;
+  const vrx_entry *d_entries;   /* device array */
+  uint32_t num_entries;
+  vrx_bvh tlas;                 /* over entries; unused when num_entries == 1 */
+  uint32_t tlas_height;
+  uint32_t max_blas_height;
+  const vrx_volume *d_volumes;  /* device array */
+  uint32_t num_volumes;
+  int32_t single_type;          /* prim type if every geom shares it, else -1 */
+} vrx_scene;
+
+typedef struct vrx_camera {
+  float pos[3];
+  float d00[3];
+  float du[3];
+  float dv[3];
+  float region[4];
+} vrx_camera;
+
+enum vrx_renderer_kind { VRX_RENDER_RAYCAST = 0, VRX_RENDER_AO = 1 };
+enum vrx_color_format { VRX_COLOR_F32 = 0, VRX_COLOR_SRGB8 = 1, VRX_COLOR_UNORM8 = 2 };
+
+typedef struct vrx_renderer {
+  int32_t kind;
+  float background[4];
+  float ambient_color[3];
+  float ambient_radiance;
+  int32_t pixel_samples;
+  int32_t ambient_samples;
+  float ao_distance;
+  int32_t cull_backfaces;
+  float inv_volume_sampling_rate;
+  /* launch tuning (autotuned; 0 = defaults) */
+  int32_t stack_lds;      /* LDS short-stack entries per lane: 8/16/24/32 */
+  int32_t block_pixels;   /* pixels per workgroup: 64/128/256 */
+} vrx_renderer;
+
+typedef struct vrx_frame {
+  uint32_t width, height;
+  int32_t color_format;
+  uint32_t frame_id;      /* 0 = first frame after a reset */
+  float *d_accum;         /* width*height*4 */
+  float *d_color_f32;     /* or NULL */
+  uint32_t *d_color_u32;  /* or NULL */
+  float *d_depth;         /* or NULL */
+  uint32_t *d_prim_id;
+  uint32_t *d_obj_id;
+  uint32_t *d_inst_id;
+  void *d_scratch;        /* traversal stack overflow; size from vrx_render_scratch_bytes */
+  size_t scratch_bytes;
+  uint64_t *d_counters;   /* optional [0]=rays traced, [1]=volume samples (atomics) */
+} vrx_frame;
+
+size_t vrx_render_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r,
+    uint32_t width, uint32_t height);
+/* newFrame clears (accumulation, depth = FLT_MAX, ids = 0). */
+int vrx_frame_clear(const vrx_frame *f, vrx_stream s);
+/* One frame (all pixel samples) of the raycast or AO renderer. */
+int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer *r,
+    const vrx_frame *f, vrx_stream s);
+/* Host-side camera setup (camera/Perspective.cpp:40-72). */
+int vrx_camera_perspective(const float pos[3], const float dir[3], const float up[3],
+    float fovy, float aspect, const float region[4], vrx_camera *out);
+/* Instance-level build: world boxes of entries + LBVH over them. */
+int vrx_tlas_query_sizes(uint32_t num_entries, vrx_bvh_sizes *out);
+int vrx_tlas_build(const vrx_scene *scene, const vrx_bvh *out, void *d_workspace,
+    size_t workspace_bytes, vrx_stream s);
+
+/* ---- differentiable volume (no reference counterpart) --------------------- */
+typedef struct vrx_diffvol {
+  const float *d_density;  /* dims product */
+  uint32_t dims[3];
+  float origin[3];
+  float spacing[3];
+  const float *d_lut;      /* 256 x 4 */
+  float value_range[2];
+  float unit_distance;
+  float step;              /* world-space step */
+  int32_t early_stop;      /* 1 = stop at opacity 0.99 (as the reference) */
+  int32_t checkpoint;      /* steps between transmittance checkpoints (64) */
+  uint32_t max_steps;      /* per ray upper bound (for checkpoint storage) */
+} vrx_diffvol;
+
+/* rays: n origins/directions (float3) */
+int vrx_diffvol_forward(const vrx_diffvol *v, uint32_t n, const float *d_org, const float *d_dir,
+    float *d_rgba_out, float *d_ckpt, vrx_stream s);
+int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, const float *d_dir,
+    const float *d_rgba, const float *d_grad_rgba, const float *d_ckpt,
+    float *d_grad_density, float *d_grad_lut, vrx_stream s);
+size_t vrx_diffvol_ckpt_floats(const vrx_diffvol *v, uint32_t n);
+/* pinhole rays for a set of views: cams[nviews], w x h each (row 0 = bottom) */
+int vrx_make_rays(const vrx_camera *cams, uint32_t nviews, uint32_t w, uint32_t h,
+    float *d_org, float *d_dir, vrx_stream s);
+
+/* ---- batched multi-view (no reference counterpart) ------------------------ */
+typedef struct vrx_batch {
+  uint32_t num_views;
+  uint32_t width, height;
+  const vrx_camera *d_cams;  /* device array of num_views cameras */
+  float *d_depth;            /* views*w*h or NULL */
+  uint32_t *d_prim_id;       /* views*w*h or NULL */
+  uint32_t *d_obj_id;
+  uint32_t *d_inst_id;
+  float *d_color_f32;        /* views*w*h*4 or NULL */
+  uint32_t *d_color_u32;     /* views*w*h or NULL (SRGB8) */
+  void *d_scratch;
+  size_t scratch_bytes;
+  uint32_t *d_work_counter;  /* 1 u32, persistent-launch tile queue */
+  int32_t tile_w, tile_h;    /* autotuned tile (pixels per wave = 64) */
+  int32_t persistent_blocks; /* 0 = one block per tile */
+} vrx_batch;
+
+size_t vrx_batch_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b);
+int vrx_render_batch(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b,
+    vrx_stream s);
+
+/* ---- synthetic scene generators (host, OpenMP) ---------------------------- */
+uint32_t vrx_gen_pcg32(uint64_t *state, uint64_t inc);
+void vrx_gen_spheres(uint64_t seed, uint32_t n, float rmin, float rmax, float *centers, float *radii);
+void vrx_gen_heightfield(uint64_t seed, uint32_t nx, uint32_t ny, float amplitude, float *verts,
+    uint32_t *indices);
+void vrx_gen_noise_volume(uint64_t seed, uint32_t nx, uint32_t ny, uint32_t nz, int octaves,
+    float base_freq, float *out);
+/* noise-displaced UV sphere with ~ntri triangles; returns actual counts */
+void vrx_gen_sphere_mesh_counts(uint32_t target_tris, uint32_t *nverts, uint32_t *ntris);
+void vrx_gen_sphere_mesh(uint64_t seed, uint32_t target_tris, float displacement, float *verts,
+    uint32_t *indices);
+float vrx_gen_value_noise3(uint64_t seed, float x, float y, float z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,35 @@
+"""The C-ABI libraries load and export every symbol their headers declare
+(no compute calls; runs without a GPU)."""
+import os
+import re
+
+import pytest
+
+from visrtx_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header: str, prefix: str):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(" + prefix + r"\w+)\s*\(", text))
+    return {n for n in names if not n.startswith("vrx_status")}
+
+
+def test_hip_library_exports_every_declared_symbol():
+    _native.hip()
+    syms = _native.exported_symbols(_native.HIP_LIB_PATH)
+    want = declared("vrx_hip.h", "vrx_")
+    assert want, "no declarations parsed"
+    missing = sorted(want - syms)
+    assert not missing, missing
+
+
+def test_anari_library_exports_every_declared_symbol():
+    _native.anari_lib()
+    syms = _native.exported_symbols(_native.ANARI_LIB_PATH)
+    want = declared("anari/anari.h", "anari") | declared("anari/ext/visrtx_amd/visrtx_amd.h", "")
+    want = {w for w in want if w.startswith(("anari", "make", "visrtx"))}
+    missing = sorted(want - syms)
+    assert not missing, missing

@@ -1,0 +1,180 @@
+"""Raycast / AO frames through the ANARI device on the GPU vs the CPU oracle.
+
+Bar: primitiveId / objectId / instanceId bit-exact; depth and FLOAT32_VEC4
+colour within 1e-4 relative (they are expected to be bit-identical: the
+device and the oracle use the same fp32 operation order); sRGB8 colour
+within 1 code value (GPU powf vs glibc powf).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from visrtx_amd import anari as A
+from visrtx_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def render_gpu(sc, cam, w, h, kind="raycast", color=A.FLOAT32_VEC4, **rp):
+    dev = A.Device()
+    s = S.build_anari(dev, sc)
+    c = S.make_camera(dev, s, cam)
+    r = S.make_renderer(dev, s, kind, **rp)
+    f = S.make_frame(dev, s, s.world, c, r, w, h, color=color)
+    print(f"[render] {kind} {w}x{h} geoms={len(sc.geoms)} inst={len(sc.instances)} "
+          f"vols={len(sc.volumes)}", flush=True)
+    dev.render(f)
+    dev.ready(f)
+    out = {}
+    for ch in ["color", "depth", "primitiveId", "objectId", "instanceId"]:
+        out[ch], _ = dev.map_frame(f, "channel." + ch)
+    out["duration"] = dev.get_property(f, "duration", A.FLOAT32)
+    out["rays"] = dev.get_property(f, "raysTraced", A.UINT64)
+    out["errors"] = dev.errors()
+    s.release()
+    dev.close()
+    return out
+
+
+def compare(g, o, exact_color=True):
+    assert not g["errors"], g["errors"]
+    np.testing.assert_array_equal(g["primitiveId"], o["primitiveId"])
+    np.testing.assert_array_equal(g["objectId"], o["objectId"])
+    np.testing.assert_array_equal(g["instanceId"], o["instanceId"])
+    np.testing.assert_allclose(g["depth"], o["depth"], rtol=1e-4, atol=0)
+    if g["color"].dtype == np.float32:
+        np.testing.assert_allclose(g["color"], o["color"], rtol=1e-4, atol=1e-4)
+        frac = np.mean(g["color"] == o["color"])
+        print(f"color bit-identical fraction: {frac:.6f}; depth identical "
+              f"{np.mean(g['depth'] == o['depth']):.6f}")
+    else:
+        a = g["color"].view(np.uint8).astype(np.int32)
+        b = o["color"].view(np.uint8).astype(np.int32)
+        assert np.abs(a - b).max() <= 1
+
+
+def test_heightfield_raycast_full_size(require_gpu):
+    """1024x768 raycast of the 131,072-triangle heightfield."""
+    sc, cam = S.heightfield_workload()
+    g = render_gpu(sc, cam, 1024, 768)
+    o = orc.OracleScene(sc).render(cam, 1024, 768)
+    compare(g, o)
+    assert (g["depth"] < 1e29).mean() > 0.05
+
+
+def test_heightfield_srgb(require_gpu):
+    sc, cam = S.heightfield_workload()
+    g = render_gpu(sc, cam, 256, 192, color=A.UFIXED8_RGBA_SRGB)
+    o = orc.OracleScene(sc).render(cam, 256, 192, color_format=orc.ORC_COLOR_SRGB8)
+    compare(g, o)
+
+
+def test_spheres_raycast(require_gpu):
+    sc, cam = S.spheres_workload(n=200_000, width=480, height=270, seed=5)
+    g = render_gpu(sc, cam, 480, 270)
+    o = orc.OracleScene(sc).render(cam, 480, 270)
+    compare(g, o)
+
+
+@pytest.mark.slow
+def test_10m_spheres_raycast_full_size(require_gpu):
+    sc, cam = S.spheres_workload()
+    g = render_gpu(sc, cam, 1920, 1080)
+    o = orc.OracleScene(sc).render(cam, 1920, 1080)
+    compare(g, o)
+
+
+def _cylinder_scene(caps):
+    rng = np.random.default_rng(4)
+    n = 3000
+    p0 = rng.random((n, 3), dtype=np.float32)
+    p1 = p0 + rng.normal(0, 0.05, (n, 3)).astype(np.float32)
+    pos = np.stack([p0, p1], 1).reshape(-1, 3)
+    rad = rng.uniform(0.002, 0.01, n).astype(np.float32)
+    g = S.GeomDesc("cylinder", pos, radius=rad, caps=caps, surface_id=3,
+                   primitive_id=np.arange(n, dtype=np.uint32)[::-1].copy())
+    sc = S.SceneDesc(geoms=[g], world_geoms=[0], world_id=9)
+    cam = S.CameraDesc((0.5, 0.5, -1.0), (0, 0, 1), (0, 1, 0), math.radians(50), 4 / 3)
+    return sc, cam
+
+
+@pytest.mark.parametrize("caps", [False, True])
+def test_cylinders_raycast(require_gpu, caps):
+    sc, cam = _cylinder_scene(caps)
+    g = render_gpu(sc, cam, 320, 240)
+    o = orc.OracleScene(sc).render(cam, 320, 240)
+    compare(g, o)
+
+
+def _instanced_scene():
+    """Three meshes, seven instances with rotations (two-level traversal)."""
+    geoms = []
+    for i, t in enumerate([200, 800, 3000]):
+        v, idx = S.gen_sphere_mesh(t, seed=20 + i)
+        geoms.append(S.GeomDesc("triangle", v, idx, surface_id=100 + i,
+                                color=(0.2 + 0.3 * i, 0.5, 0.7)))
+    rng = np.random.default_rng(8)
+    insts = []
+    for k in range(7):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        s = rng.uniform(0.3, 0.8)
+        t = rng.uniform(0, 4, 3)
+        M = np.zeros(12, np.float32)
+        M[0:9] = (R * s).T.reshape(-1)   # column-major 3x3
+        M[9:12] = t
+        insts.append(S.InstanceDesc(geoms=[k % 3], xfm=M, id=1000 + k))
+    sc = S.SceneDesc(geoms=geoms, instances=insts, world_id=5)
+    cam = S.CameraDesc((2.0, 2.0, -6.0), S.look_at((2.0, 2.0, -6.0), (2, 2, 2)), (0, 1, 0),
+                       math.radians(50), 1.0)
+    return sc, cam
+
+
+def test_instanced_triangles_raycast(require_gpu):
+    sc, cam = _instanced_scene()
+    g = render_gpu(sc, cam, 256, 256)
+    o = orc.OracleScene(sc).render(cam, 256, 256)
+    compare(g, o)
+    assert len(np.unique(g["instanceId"])) > 3
+
+
+def test_mixed_types_transparency_and_culling(require_gpu):
+    rng = np.random.default_rng(12)
+    c = rng.random((400, 3), dtype=np.float32)
+    sph = S.GeomDesc("sphere", c, radius=np.full(400, 0.03, np.float32), surface_id=1,
+                     opacity=0.5)
+    v, idx = S.gen_heightfield(33, seed=3)
+    tri = S.GeomDesc("triangle", v, idx, surface_id=2, cull=True)
+    cyl_p = rng.random((200, 3), dtype=np.float32)
+    cyl = S.GeomDesc("cylinder", cyl_p, global_radius=0.01, surface_id=4)
+    sc = S.SceneDesc(geoms=[sph, tri, cyl], world_geoms=[0, 1, 2], world_id=7)
+    cam = S.CameraDesc((0.5, -1.2, 1.0), S.look_at((0.5, -1.2, 1.0), (0.5, 0.5, 0.2)), (0, 0, 1),
+                       math.radians(60), 1.5)
+    g = render_gpu(sc, cam, 300, 200, background=(0.1, 0.2, 0.3, 1.0))
+    o = orc.OracleScene(sc).render(cam, 300, 200, background=(0.1, 0.2, 0.3, 1.0))
+    compare(g, o)
+
+
+def test_ao_renderer_small(require_gpu):
+    sc, cam = S.spheres_workload(n=50_000, width=160, height=90, seed=6)
+    rp = dict(pixelSamples=4, ambientSamples=2, ambientOcclusionDistance=0.05)
+    g = render_gpu(sc, cam, 160, 90, kind="ao", **rp)
+    o = orc.OracleScene(sc).render(cam, 160, 90, kind="ao", **rp)
+    compare(g, o)
+
+
+def test_empty_world_background(require_gpu):
+    sc = S.SceneDesc(world_id=3)
+    cam = S.CameraDesc((0, 0, 0), (0, 0, 1), (0, 1, 0), 1.0, 1.0)
+    g = render_gpu(sc, cam, 8, 8, background=(0.25, 0.5, 0.75, 1.0))
+    assert (g["primitiveId"] == 0).all()
+    assert (g["objectId"] == 0xFFFFFFFF).all() and (g["instanceId"] == 0xFFFFFFFF).all()
+    assert (g["depth"] == np.float32(1e30)).all()
+    o = orc.OracleScene(sc).render(cam, 8, 8, background=(0.25, 0.5, 0.75, 1.0))
+    compare(g, o)

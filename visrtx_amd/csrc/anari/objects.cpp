@@ -1,0 +1,1535 @@
+// objects.cpp -- ANARI objects of the visrtx_amd device (host C++).
+//
+// Each class cites the reference object it mirrors (paths relative to the
+// VisRTX tree).  GPU resources are created through include/vrx_hip.h only.
+#include "objects.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+namespace vrxa {
+
+// ---------------------------------------------------------------------------
+// type helpers
+// ---------------------------------------------------------------------------
+size_t sizeOf(ANARIDataType t)
+{
+  switch (t) {
+  case ANARI_BOOL:
+    return 4;
+  case ANARI_INT8:
+  case ANARI_UINT8:
+  case ANARI_UFIXED8:
+  case ANARI_UFIXED8_R_SRGB:
+    return 1;
+  case ANARI_INT16:
+  case ANARI_UINT16:
+    return 2;
+  case ANARI_INT32:
+  case ANARI_UINT32:
+  case ANARI_FLOAT32:
+  case ANARI_UFIXED8_VEC4:
+  case ANARI_UFIXED8_RGBA_SRGB:
+  case ANARI_DATA_TYPE:
+    return 4;
+  case ANARI_INT32_VEC2:
+  case ANARI_UINT32_VEC2:
+  case ANARI_FLOAT32_VEC2:
+  case ANARI_FLOAT32_BOX1:
+  case ANARI_INT64:
+  case ANARI_UINT64:
+  case ANARI_FLOAT64:
+    return 8;
+  case ANARI_INT32_VEC3:
+  case ANARI_UINT32_VEC3:
+  case ANARI_FLOAT32_VEC3:
+    return 12;
+  case ANARI_INT32_VEC4:
+  case ANARI_UINT32_VEC4:
+  case ANARI_FLOAT32_VEC4:
+  case ANARI_FLOAT32_BOX2:
+  case ANARI_FLOAT64_BOX1:
+    return 16;
+  case ANARI_FLOAT32_BOX3:
+    return 24;
+  case ANARI_FLOAT32_BOX4:
+    return 32;
+  case ANARI_FLOAT32_MAT3x4:
+    return 48;
+  case ANARI_FLOAT32_MAT4:
+    return 64;
+  case ANARI_VOID_POINTER:
+  case ANARI_FUNCTION_POINTER:
+  case ANARI_MEMORY_DELETER:
+  case ANARI_STATUS_CALLBACK:
+  case ANARI_FRAME_COMPLETION_CALLBACK:
+    return sizeof(void *);
+  default:
+    if (isObjectType(t))
+      return sizeof(void *);
+    return 0;
+  }
+}
+
+bool isObjectType(ANARIDataType t) { return t >= ANARI_OBJECT && t <= ANARI_WORLD; }
+
+const char *typeName(ANARIDataType t)
+{
+  switch (t) {
+  case ANARI_ARRAY1D:
+    return "ANARI_ARRAY1D";
+  case ANARI_ARRAY2D:
+    return "ANARI_ARRAY2D";
+  case ANARI_ARRAY3D:
+    return "ANARI_ARRAY3D";
+  case ANARI_CAMERA:
+    return "ANARI_CAMERA";
+  case ANARI_FRAME:
+    return "ANARI_FRAME";
+  case ANARI_GEOMETRY:
+    return "ANARI_GEOMETRY";
+  case ANARI_GROUP:
+    return "ANARI_GROUP";
+  case ANARI_INSTANCE:
+    return "ANARI_INSTANCE";
+  case ANARI_LIGHT:
+    return "ANARI_LIGHT";
+  case ANARI_MATERIAL:
+    return "ANARI_MATERIAL";
+  case ANARI_RENDERER:
+    return "ANARI_RENDERER";
+  case ANARI_SURFACE:
+    return "ANARI_SURFACE";
+  case ANARI_SAMPLER:
+    return "ANARI_SAMPLER";
+  case ANARI_SPATIAL_FIELD:
+    return "ANARI_SPATIAL_FIELD";
+  case ANARI_VOLUME:
+    return "ANARI_VOLUME";
+  case ANARI_WORLD:
+    return "ANARI_WORLD";
+  case ANARI_FLOAT32:
+    return "ANARI_FLOAT32";
+  case ANARI_FLOAT32_VEC3:
+    return "ANARI_FLOAT32_VEC3";
+  case ANARI_UINT32:
+    return "ANARI_UINT32";
+  default:
+    return "ANARI_TYPE";
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Object base (Object.cpp, RegisteredObject.h)
+// ---------------------------------------------------------------------------
+Object::Object(Device *d, ANARIDataType t, std::string sub) : dev(d), type(t), subtype(std::move(sub))
+{}
+
+static void dropParams(std::map<std::string, Param> &m)
+{
+  for (auto &kv : m)
+    if (kv.second.obj)
+      releaseInternal(kv.second.obj);
+  m.clear();
+}
+
+Object::~Object()
+{
+  dropParams(staged);
+  dropParams(params);
+}
+
+bool Object::getProperty(const std::string &name, ANARIDataType t, void *mem, uint64_t size, uint32_t)
+{
+  if (name == "valid" && t == ANARI_BOOL && size >= 4) {
+    int v = isValid() ? 1 : 0;
+    memcpy(mem, &v, 4);
+    return true;
+  }
+  return false;
+}
+
+const Param *Object::find(const char *name) const
+{
+  auto it = params.find(name);
+  return it == params.end() ? nullptr : &it->second;
+}
+
+bool Object::getRaw(const char *name, ANARIDataType t, void *out) const
+{
+  const Param *p = find(name);
+  if (!p || p->type != t || p->bytes.empty())
+    return false;
+  memcpy(out, p->bytes.data(), p->bytes.size());
+  return true;
+}
+
+std::string Object::getString(const char *name, const std::string &def) const
+{
+  const Param *p = find(name);
+  if (!p || p->type != ANARI_STRING)
+    return def;
+  return p->str;
+}
+
+Object *Object::getObject(const char *name, ANARIDataType t) const
+{
+  const Param *p = find(name);
+  if (!p || !p->obj)
+    return nullptr;
+  if (t != ANARI_OBJECT && p->obj->type != t)
+    return nullptr;
+  return p->obj;
+}
+
+void Object::report(int severity, int code, const char *fmt, ...) const
+{
+  va_list ap;
+  va_start(ap, fmt);
+  dev->report(const_cast<Object *>(this), severity, code, fmt, ap);
+  va_end(ap);
+}
+
+void retainInternal(Object *o)
+{
+  if (o)
+    o->intRef++;
+}
+
+void releaseInternal(Object *o)
+{
+  if (!o)
+    return;
+  o->intRef--;
+  if (o->intRef <= 0 && o->pubRef <= 0)
+    delete o;
+}
+
+// ---------------------------------------------------------------------------
+// Array (array/Array.cpp:69-119,164-197,248-297; MemoryAllocation.cpp)
+// ---------------------------------------------------------------------------
+Array::Array(Device *d, ANARIDataType arrType, const void *appMem, ANARIMemoryDeleter del,
+    const void *userData, ANARIDataType e, uint64_t n1, uint64_t n2, uint64_t n3)
+    : Object(d, arrType, ""), elem(e), dims{n1, n2, n3}, elemSize(sizeOf(e)), app(appMem),
+      deleter(del), deleterUser(userData)
+{
+  if (!appMem) {
+    owned.assign(bytes() ? bytes() : 1, 0); // managed: zeroed, app fills via map
+    app = nullptr;
+  }
+  if (isObjectType(elem))
+    refreshObjects();
+}
+
+Array::~Array()
+{
+  for (Object *o : objects)
+    releaseInternal(o);
+  objects.clear();
+  if (app && deleter)
+    deleter(deleterUser, app);
+  if (dptr)
+    vrx_free(dptr);
+}
+
+void *Array::map()
+{
+  mapped = true;
+  return (void *)host();
+}
+
+void Array::unmap()
+{
+  mapped = false;
+  version++;
+  if (isObjectType(elem))
+    refreshObjects();
+}
+
+void Array::refreshObjects()
+{
+  std::vector<Object *> fresh;
+  const Object *const *h = (const Object *const *)host();
+  for (uint64_t i = 0; h && i < count(); i++) {
+    Object *o = const_cast<Object *>(h[i]);
+    retainInternal(o);
+    fresh.push_back(o);
+  }
+  for (Object *o : objects)
+    releaseInternal(o);
+  objects.swap(fresh);
+}
+
+void Array::onPublicReleaseToZero()
+{
+  // Array.cpp:164-182: a shared array still referenced internally is copied
+  if (app && !deleter && intRef > 0) {
+    owned.assign((const uint8_t *)app, (const uint8_t *)app + bytes());
+    app = nullptr;
+  }
+}
+
+const void *Array::device()
+{
+  if (dptr && dversion == version)
+    return dptr;
+  size_t b = bytes();
+  if (b == 0)
+    return nullptr;
+  if (!dptr || dbytes < b) {
+    if (dptr)
+      vrx_free(dptr);
+    dptr = nullptr;
+    if (!dev->check(vrx_malloc(&dptr, b), this, "array allocation"))
+      return nullptr;
+    dbytes = b;
+  }
+  if (!dev->check(vrx_memcpy(dptr, host(), b, VRX_H2D, dev->stream), this, "array upload"))
+    return nullptr;
+  dversion = version;
+  return dptr;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry (scene/surface/geometry/{Geometry,Triangle,Sphere,Cylinder})
+// ---------------------------------------------------------------------------
+Geometry::Geometry(Device *d, const std::string &sub) : Object(d, ANARI_GEOMETRY, sub)
+{
+  if (sub == "triangle")
+    ptype = VRX_TRIANGLE;
+  else if (sub == "sphere")
+    ptype = VRX_SPHERE;
+  else if (sub == "cylinder")
+    ptype = VRX_CYLINDER;
+  if (ptype == VRX_CYLINDER)
+    globalRadius = 1.0f; // Cylinder.cpp:49
+}
+
+Geometry::~Geometry() { freeBVH(); }
+
+void Geometry::freeBVH()
+{
+  vrx_free(dNodes);
+  vrx_free(dLeaves);
+  vrx_free(dIds);
+  vrx_free(dHeader);
+  dNodes = dLeaves = dIds = dHeader = nullptr;
+  capNodes = capLeaves = capIds = 0;
+}
+
+static Array *arrayParam(const Object *o, const char *name)
+{
+  return (Array *)o->getObject(name, ANARI_ARRAY1D);
+}
+
+void Geometry::commit()
+{
+  valid = false;
+  bvhDirty = true;
+  if (ptype < 0) {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "unsupported geometry subtype '%s'", subtype.c_str());
+    return;
+  }
+  pos = arrayParam(this, "vertex.position");
+  index = arrayParam(this, "primitive.index");
+  primId = arrayParam(this, "primitive.id");
+  normal = ptype == VRX_TRIANGLE ? arrayParam(this, "vertex.normal") : nullptr;
+  cull = get<int>("cullBackfaces", ANARI_BOOL, 0) != 0;
+  if (ptype == VRX_SPHERE) {
+    radius = arrayParam(this, "vertex.radius");
+    globalRadius = get<float>("radius", ANARI_FLOAT32, 0.01f); // Sphere.cu:52
+  } else if (ptype == VRX_CYLINDER) {
+    radius = arrayParam(this, "primitive.radius");
+    globalRadius = get<float>("radius", ANARI_FLOAT32, 1.0f);
+    caps = getString("caps", "none") != "none"; // Cylinder.cpp:47
+  } else {
+    radius = nullptr;
+  }
+  if (!pos) {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "missing required parameter 'vertex.position' on %s geometry", subtype.c_str());
+    return;
+  }
+  if (pos->elem != ANARI_FLOAT32_VEC3) {
+    report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+        "'vertex.position' must be ANARI_FLOAT32_VEC3");
+    return;
+  }
+  const uint64_t nv = pos->count();
+  if (ptype == VRX_TRIANGLE) {
+    if (index) {
+      if (index->elem != ANARI_UINT32_VEC3) {
+        report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+            "'primitive.index' must be ANARI_UINT32_VEC3 on triangle geometry");
+        return;
+      }
+      n = (uint32_t)index->count();
+    } else {
+      if (nv % 3 != 0) {
+        // Triangle.cpp:56-69
+        report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+            "triangle soup 'vertex.position' size is not a multiple of 3");
+        return;
+      }
+      n = (uint32_t)(nv / 3);
+    }
+  } else if (ptype == VRX_SPHERE) {
+    if (index && index->elem != ANARI_UINT32) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+          "'primitive.index' must be ANARI_UINT32 on sphere geometry");
+      return;
+    }
+    n = (uint32_t)(index ? index->count() : nv);
+  } else {
+    if (index && index->elem != ANARI_UINT32_VEC2) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+          "'primitive.index' must be ANARI_UINT32_VEC2 on cylinder geometry");
+      return;
+    }
+    n = (uint32_t)(index ? index->count() : nv / 2);
+  }
+  if (index) {
+    // every index must address an existing vertex (checked once per commit)
+    const uint32_t *ix = (const uint32_t *)index->host();
+    const size_t ni = index->count() * (ptype == VRX_TRIANGLE ? 3 : (ptype == VRX_CYLINDER ? 2 : 1));
+    uint32_t mx = 0;
+    for (size_t i = 0; i < ni; i++)
+      mx = ix[i] > mx ? ix[i] : mx;
+    if (ni && (uint64_t)mx >= nv) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+          "'primitive.index' refers to vertex %u but only %llu vertices exist", mx,
+          (unsigned long long)nv);
+      return;
+    }
+  }
+  if (radius) {
+    const uint64_t need = ptype == VRX_SPHERE ? nv : n;
+    if (radius->count() < need) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+          "radius array has %llu entries, %llu needed", (unsigned long long)radius->count(),
+          (unsigned long long)need);
+      return;
+    }
+  }
+  if (radius && radius->elem != ANARI_FLOAT32) {
+    report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT, "radius array must be ANARI_FLOAT32");
+    return;
+  }
+  if (primId && (primId->elem != ANARI_UINT32 || primId->count() < n)) {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "ignoring 'primitive.id' (needs ANARI_UINT32 with one entry per primitive)");
+    primId = nullptr;
+  }
+  if (normal && (normal->elem != ANARI_FLOAT32_VEC3 || normal->count() < nv))
+    normal = nullptr;
+  valid = true;
+}
+
+int Geometry::buildBVH(vrx_stream s)
+{
+  vrx_bvh_sizes sz;
+  vrx_bvh_query_sizes(ptype, n, &sz);
+  auto grow = [&](void *&p, size_t &cap, size_t need) -> bool {
+    if (p && cap >= need)
+      return true;
+    vrx_free(p);
+    p = nullptr;
+    cap = 0;
+    if (!dev->check(vrx_malloc(&p, need), this, "BVH allocation"))
+      return false;
+    cap = need;
+    return true;
+  };
+  if (!grow(dNodes, capNodes, sz.nodes) || !grow(dLeaves, capLeaves, sz.leaves)
+      || !grow(dIds, capIds, sz.leaf_ids))
+    return -1;
+  if (!dHeader && !dev->check(vrx_malloc(&dHeader, sz.header), this, "BVH header"))
+    return -1;
+  vrx_prims p{};
+  p.type = ptype;
+  p.n = n;
+  p.d_position = (const float *)pos->device();
+  p.d_index = index ? (const uint32_t *)index->device() : nullptr;
+  p.d_radius = radius ? (const float *)radius->device() : nullptr;
+  p.global_radius = globalRadius;
+  if (primId)
+    primId->device();
+  if (normal)
+    normal->device();
+  bvh.d_nodes = dNodes;
+  bvh.d_leaves = dLeaves;
+  bvh.d_leaf_ids = (uint32_t *)dIds;
+  bvh.d_header = dHeader;
+  void *ws = dev->workspace(sz.workspace);
+  if (!ws)
+    return -1;
+  if (!dev->check(vrx_bvh_build(&p, &bvh, ws, sz.workspace, s), this, "BVH build"))
+    return -1;
+  bvhDirty = false;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Material: matte (scene/surface/material/Matte.cpp:39-74,
+// gpu/evalMaterialParameters.h:392-403 alpha modes)
+// ---------------------------------------------------------------------------
+void Material::commit()
+{
+  float c4[4] = {0.8f, 0.8f, 0.8f, 1.0f};
+  if (!getRaw("color", ANARI_FLOAT32_VEC4, c4)) {
+    float c3[3];
+    if (getRaw("color", ANARI_FLOAT32_VEC3, c3)) {
+      c4[0] = c3[0];
+      c4[1] = c3[1];
+      c4[2] = c3[2];
+    }
+  }
+  color[0] = c4[0];
+  color[1] = c4[1];
+  color[2] = c4[2];
+  float op = get<float>("opacity", ANARI_FLOAT32, 1.0f);
+  std::string mode = getString("alphaMode", "opaque");
+  float cutoff = get<float>("alphaCutoff", ANARI_FLOAT32, 0.5f);
+  float in = c4[3] * op;
+  if (mode == "blend")
+    opacity = in;
+  else if (mode == "mask")
+    opacity = in < cutoff ? 0.0f : 1.0f;
+  else
+    opacity = 1.0f;
+  if (subtype != "matte")
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "material subtype '%s' is not supported; only 'matte'", subtype.c_str());
+}
+
+// ---------------------------------------------------------------------------
+// Surface (scene/surface/Surface.cpp:42-118)
+// ---------------------------------------------------------------------------
+void Surface::commit()
+{
+  geom = (Geometry *)getObject("geometry", ANARI_GEOMETRY);
+  mat = (Material *)getObject("material", ANARI_MATERIAL);
+  id = get<uint32_t>("id", ANARI_UINT32, ~0u);
+  if (!geom)
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "missing 'geometry' on ANARISurface");
+}
+
+bool Surface::isValid() const { return geom && geom->isValid(); }
+
+// ---------------------------------------------------------------------------
+// SpatialField: structuredRegular (StructuredRegularField.cpp:90-212)
+// ---------------------------------------------------------------------------
+SpatialField::~SpatialField() { vrx_free(dData); }
+
+void SpatialField::commit()
+{
+  valid = false;
+  float o[3] = {0, 0, 0}, s[3] = {1, 1, 1};
+  getRaw("origin", ANARI_FLOAT32_VEC3, o);
+  getRaw("spacing", ANARI_FLOAT32_VEC3, s);
+  memcpy(origin, o, 12);
+  memcpy(spacing, s, 12);
+  data = (Array *)getObject("data", ANARI_ARRAY3D);
+  uploadedVersion = 0;
+  if (subtype != "structuredRegular") {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "spatial field subtype '%s' is not supported", subtype.c_str());
+    return;
+  }
+  if (!data) {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "missing required parameter 'data' on structuredRegular spatial field");
+    return;
+  }
+  if (data->elem != ANARI_FLOAT32 && data->elem != ANARI_UFIXED8 && data->elem != ANARI_UINT8) {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "invalid data array type encountered in structuredRegular spatial field(%s)",
+        typeName(data->elem));
+    return;
+  }
+  for (int k = 0; k < 3; k++)
+    dims[k] = (uint32_t)data->dims[k];
+  valid = true;
+}
+
+const float *SpatialField::deviceData()
+{
+  if (!valid)
+    return nullptr;
+  if (dData && uploadedVersion == data->version)
+    return (const float *)dData;
+  size_t n = data->count();
+  if (data->elem == ANARI_FLOAT32) {
+    // reuse the array's own device copy (no second 512 MiB copy)
+    uploadedVersion = data->version;
+    return (const float *)data->device();
+  }
+  std::vector<float> conv(n);
+  const uint8_t *src = (const uint8_t *)data->host();
+  for (size_t i = 0; i < n; i++)
+    conv[i] = data->elem == ANARI_UFIXED8 ? (float)src[i] / 255.0f : (float)src[i];
+  if (!dData && !dev->check(vrx_malloc(&dData, n * 4), this, "field allocation"))
+    return nullptr;
+  if (!dev->check(vrx_memcpy_sync(dData, conv.data(), n * 4, VRX_H2D), this, "field upload"))
+    return nullptr;
+  uploadedVersion = data->version;
+  return (const float *)dData;
+}
+
+void SpatialField::bounds(float lo[3], float hi[3]) const
+{
+  for (int k = 0; k < 3; k++) {
+    lo[k] = origin[k];
+    hi[k] = origin[k] + ((float)dims[k] - 1.0f) * spacing[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Volume: transferFunction1D (scene/volume/TransferFunction1D.cpp:46-150,
+// utility/colorMapHelpers.h:43-73)
+// ---------------------------------------------------------------------------
+Volume::~Volume() { vrx_free(dLut); }
+
+static float posf(float v, float lo, float hi)
+{
+  float c = v < lo ? lo : (v > hi ? hi : v);
+  return (c - lo) * (1.0f / (hi - lo));
+}
+
+static std::vector<float> linearPositions(size_t n, float lo, float hi)
+{
+  std::vector<float> p(n);
+  p.front() = 0.0f;
+  p.back() = 1.0f;
+  float w = 1.0f / (float)(n - 1);
+  for (size_t i = 1; i + 1 < n; i++)
+    p[i] = p[i - 1] + w;
+  for (auto &v : p)
+    v = v * (hi - lo) + lo;
+  return p;
+}
+
+static void interpolated(const float *vals, int ncomp, const std::vector<float> &positions, float lo,
+    float hi, float pos, float *out)
+{
+  size_t n = positions.size();
+  for (size_t i = 0; i + 1 < n; i++) {
+    float r0 = posf(positions[i], lo, hi), r1 = posf(positions[i + 1], lo, hi);
+    if (pos >= r0 && pos <= r1) {
+      float t = posf(pos, r0, r1);
+      for (int k = 0; k < ncomp; k++)
+        out[k] = vals[ncomp * i + k] * (1.0f - t) + vals[ncomp * (i + 1) + k] * t;
+      return;
+    }
+  }
+  size_t j = pos <= posf(positions[0], lo, hi) ? 0 : n - 1;
+  for (int k = 0; k < ncomp; k++)
+    out[k] = vals[ncomp * j + k];
+}
+
+void Volume::commit()
+{
+  id = get<uint32_t>("id", ANARI_UINT32, ~0u); // Volume.cpp:45-48
+  field = (SpatialField *)getObject("value", ANARI_SPATIAL_FIELD);
+  if (subtype != "transferFunction1D" && subtype != "scivis") {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "volume subtype '%s' is not supported", subtype.c_str());
+    field = nullptr;
+    return;
+  }
+  if (!field) {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "missing parameter 'value' on transferFunction1D ANARIVolume");
+    return;
+  }
+  Array *colorArr = arrayParam(this, "color");
+  Array *opacityArr = arrayParam(this, "opacity");
+  float uc[4] = {1, 1, 1, 1};
+  float c3[3];
+  if (getRaw("color", ANARI_FLOAT32_VEC3, c3)) {
+    uc[0] = c3[0];
+    uc[1] = c3[1];
+    uc[2] = c3[2];
+  }
+  getRaw("color", ANARI_FLOAT32_VEC4, uc);
+  float uo = get<float>("opacity", ANARI_FLOAT32, 1.0f) * uc[3];
+  unitDistance = get<float>("unitDistance", ANARI_FLOAT32, 1.0f);
+  float vr[2] = {0.0f, 1.0f};
+  getRaw("valueRange", ANARI_FLOAT32_VEC2, vr);
+  getRaw("valueRange", ANARI_FLOAT32_BOX1, vr);
+  double vrd[2];
+  if (getRaw("valueRange", ANARI_FLOAT64_BOX1, vrd)) {
+    vr[0] = (float)vrd[0];
+    vr[1] = (float)vrd[1];
+  }
+  valueRange[0] = vr[0];
+  valueRange[1] = vr[1];
+  // discritizeTFData (TransferFunction1D.cpp:101-150): 256 entries
+  std::vector<float> cpos, opos;
+  int ccomp = 0;
+  if (colorArr) {
+    if (colorArr->elem == ANARI_FLOAT32_VEC3)
+      ccomp = 3;
+    else if (colorArr->elem == ANARI_FLOAT32_VEC4)
+      ccomp = 4;
+    else
+      report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+          "unusable tf1D color array type set (%s)", typeName(colorArr->elem));
+    if (ccomp)
+      cpos = linearPositions(colorArr->count(), vr[0], vr[1]);
+  }
+  if (opacityArr && opacityArr->elem == ANARI_FLOAT32)
+    opos = linearPositions(opacityArr->count(), vr[0], vr[1]);
+  for (int i = 0; i < 256; i++) {
+    float p = (float)i / 255.0f;
+    float c[4] = {uc[0], uc[1], uc[2], uc[3]};
+    if (ccomp) {
+      float tmp[4] = {0, 0, 0, 1};
+      interpolated((const float *)colorArr->host(), ccomp, cpos, vr[0], vr[1], p, tmp);
+      c[0] = tmp[0];
+      c[1] = tmp[1];
+      c[2] = tmp[2];
+      c[3] = ccomp == 4 ? tmp[3] : 1.0f;
+    }
+    float o = uo;
+    if (!opos.empty())
+      interpolated((const float *)opacityArr->host(), 1, opos, vr[0], vr[1], p, &o);
+    lut[4 * i + 0] = c[0];
+    lut[4 * i + 1] = c[1];
+    lut[4 * i + 2] = c[2];
+    lut[4 * i + 3] = c[3] * o;
+  }
+  if (!dLut && !dev->check(vrx_malloc(&dLut, sizeof(lut)), this, "LUT allocation"))
+    return;
+  dev->check(vrx_memcpy(dLut, lut, sizeof(lut), VRX_H2D, dev->stream), this, "LUT upload");
+}
+
+// ---------------------------------------------------------------------------
+// Group / Instance / World (scene/Group.cpp, Instance.cpp, World.cpp)
+// ---------------------------------------------------------------------------
+template <typename T>
+static std::vector<T *> objectList(const Object *o, const char *name, ANARIDataType t)
+{
+  std::vector<T *> out;
+  Array *a = arrayParam(o, name);
+  if (!a || a->elem != t)
+    return out;
+  for (Object *x : a->objects)
+    if (x)
+      out.push_back((T *)x);
+  return out;
+}
+
+void Group::commit()
+{
+  surfaces = objectList<Surface>(this, "surface", ANARI_SURFACE);
+  volumes = objectList<Volume>(this, "volume", ANARI_VOLUME);
+}
+
+void Instance::commit()
+{
+  group = (Group *)getObject("group", ANARI_GROUP);
+  id = get<uint32_t>("id", ANARI_UINT32, ~0u); // Instance.cpp:42-43
+  float m4[16];
+  float m34[12];
+  if (getRaw("transform", ANARI_FLOAT32_MAT4, m4)) {
+    // column-major 4x4 -> 3x4 (drop the last row)
+    for (int c = 0; c < 4; c++)
+      for (int r = 0; r < 3; r++)
+        xfm[3 * c + r] = m4[4 * c + r];
+  } else if (getRaw("transform", ANARI_FLOAT32_MAT3x4, m34)) {
+    memcpy(xfm, m34, sizeof(xfm));
+  } else {
+    const float id3[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    memcpy(xfm, id3, sizeof(xfm));
+  }
+  if (!group)
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT, "missing 'group' on ANARIInstance");
+}
+
+World::~World() { freeScene(); }
+
+void World::freeScene()
+{
+  vrx_free(dGeoms);
+  vrx_free(dEntries);
+  vrx_free(dVolumes);
+  vrx_free(dTlasNodes);
+  vrx_free(dTlasLeaves);
+  vrx_free(dTlasIds);
+  vrx_free(dTlasHeader);
+  dGeoms = dEntries = dVolumes = dTlasNodes = dTlasLeaves = dTlasIds = dTlasHeader = nullptr;
+}
+
+void World::commit()
+{
+  instances = objectList<Instance>(this, "instance", ANARI_INSTANCE);
+  surfaces = objectList<Surface>(this, "surface", ANARI_SURFACE);
+  volumes = objectList<Volume>(this, "volume", ANARI_VOLUME);
+  id = get<uint32_t>("id", ANARI_UINT32, ~0u); // World.cpp:159 (zero instance id)
+  sceneDirty = true;
+}
+
+bool World::needsRebuild() const { return sceneDirty || builtAt < dev->lastSceneFinalize; }
+
+static bool isIdentity(const float *m)
+{
+  static const float id3[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+  return memcmp(m, id3, sizeof(id3)) == 0;
+}
+
+// inverse of a 3x4 column-major transform, computed in FP64
+static void invertXfm(const float *m, float *inv)
+{
+  double a = m[0], b = m[3], c = m[6];
+  double d = m[1], e = m[4], f = m[7];
+  double g = m[2], h = m[5], i = m[8];
+  double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+  double det = a * A + b * B + c * C;
+  double r = 1.0 / det;
+  double im[9];
+  im[0] = A * r;
+  im[3] = -(b * i - c * h) * r;
+  im[6] = (b * f - c * e) * r;
+  im[1] = B * r;
+  im[4] = (a * i - c * g) * r;
+  im[7] = -(a * f - c * d) * r;
+  im[2] = C * r;
+  im[5] = -(a * h - b * g) * r;
+  im[8] = (a * e - b * d) * r;
+  double tx = m[9], ty = m[10], tz = m[11];
+  for (int k = 0; k < 9; k++)
+    inv[k] = (float)im[k];
+  inv[9] = (float)(-(im[0] * tx + im[3] * ty + im[6] * tz));
+  inv[10] = (float)(-(im[1] * tx + im[4] * ty + im[7] * tz));
+  inv[11] = (float)(-(im[2] * tx + im[5] * ty + im[8] * tz));
+}
+
+static void xfmBox(const float *m, const float lo[3], const float hi[3], float out[6])
+{
+  out[0] = out[1] = out[2] = INFINITY;
+  out[3] = out[4] = out[5] = -INFINITY;
+  for (int c = 0; c < 8; c++) {
+    float p[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+    for (int r = 0; r < 3; r++) {
+      float q = ((m[r] * p[0] + m[3 + r] * p[1]) + m[6 + r] * p[2]) + m[9 + r];
+      out[r] = std::min(out[r], q);
+      out[3 + r] = std::max(out[3 + r], q);
+    }
+  }
+}
+
+// World::rebuildWorld (World.cpp:202-341): BLAS per geometry, instance entries,
+// instance-level BVH, volume list, world bounds.
+int World::rebuild()
+{
+  vrx_stream s = dev->stream;
+  struct Occ {
+    Surface *surf;
+    const float *xfm;
+    uint32_t inst;
+  };
+  std::vector<Occ> occ;
+  for (Surface *sf : surfaces)
+    if (sf && sf->isValid())
+      occ.push_back({sf, nullptr, id});
+  for (Instance *in : instances) {
+    if (!in || !in->group)
+      continue;
+    for (Surface *sf : in->group->surfaces)
+      if (sf && sf->isValid())
+        occ.push_back({sf, in->xfm, in->id});
+  }
+  std::vector<Volume *> vols;
+  std::vector<uint32_t> volInst;
+  for (Volume *v : volumes)
+    if (v && v->isValid()) {
+      vols.push_back(v);
+      volInst.push_back(id);
+    }
+  for (Instance *in : instances) {
+    if (!in || !in->group)
+      continue;
+    for (Volume *v : in->group->volumes) {
+      if (!v || !v->isValid())
+        continue;
+      if (!isIdentity(in->xfm)) {
+        report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
+            "instanced volumes with a non-identity transform are not supported; skipped");
+        continue;
+      }
+      vols.push_back(v);
+      volInst.push_back(in->id);
+    }
+  }
+
+  vrx_event e0 = nullptr, e1 = nullptr;
+  vrx_event_create(&e0);
+  vrx_event_create(&e1);
+  vrx_event_record(e0, s);
+  // BLAS builds for dirty geometries
+  std::vector<Geometry *> geoms;
+  for (auto &o : occ)
+    if (std::find(geoms.begin(), geoms.end(), o.surf->geom) == geoms.end())
+      geoms.push_back(o.surf->geom);
+  for (Geometry *g : geoms)
+    if (g->bvhDirty && g->buildBVH(s) != 0)
+      return -1;
+  vrx_event_record(e1, s);
+  // read back headers (heights, boxes)
+  for (Geometry *g : geoms)
+    if (!dev->check(vrx_memcpy(&g->header, g->dHeader, sizeof(vrx_bvh_header), VRX_D2H, s), this,
+            "header readback"))
+      return -1;
+  dev->check(vrx_stream_sync(s), this, "stream sync");
+  float ms = 0.0f;
+  vrx_event_elapsed_ms(e0, e1, &ms);
+  lastBuildMs = ms;
+
+  // vrx_geom per surface occurrence (deduplicated by surface)
+  std::vector<vrx_geom> hg;
+  std::vector<Surface *> gsurf;
+  std::vector<vrx_entry> he;
+  uint32_t maxH = 0;
+  int single = -2;
+  float wb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (auto &o : occ) {
+    auto it = std::find(gsurf.begin(), gsurf.end(), o.surf);
+    uint32_t gi;
+    if (it == gsurf.end()) {
+      Geometry *g = o.surf->geom;
+      vrx_geom vg{};
+      vg.type = g->ptype;
+      vg.n = g->n;
+      vg.bvh = g->bvh;
+      vg.d_position = (const float *)g->pos->device();
+      vg.d_index = g->index ? (const uint32_t *)g->index->device() : nullptr;
+      vg.d_radius = g->radius ? (const float *)g->radius->device() : nullptr;
+      vg.global_radius = g->globalRadius;
+      vg.d_primitive_id = g->primId ? (const uint32_t *)g->primId->device() : nullptr;
+      vg.d_vertex_normal = g->normal ? (const float *)g->normal->device() : nullptr;
+      vg.caps = g->caps;
+      vg.cull_backfaces = g->cull;
+      Material *m = o.surf->mat;
+      vg.color[0] = m ? m->color[0] : 0.8f; // evalShading.h:61-81 defaults
+      vg.color[1] = m ? m->color[1] : 0.8f;
+      vg.color[2] = m ? m->color[2] : 0.8f;
+      vg.opacity = m ? m->opacity : 1.0f;
+      vg.surface_id = o.surf->id;
+      gi = (uint32_t)hg.size();
+      hg.push_back(vg);
+      gsurf.push_back(o.surf);
+      maxH = std::max(maxH, g->header.height);
+      single = single == -2 ? g->ptype : (single == g->ptype ? single : -1);
+    } else {
+      gi = (uint32_t)(it - gsurf.begin());
+    }
+    vrx_entry e{};
+    static const float id3[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    memcpy(e.xfm, o.xfm ? o.xfm : id3, sizeof(e.xfm));
+    e.identity = isIdentity(e.xfm) ? 1u : 0u;
+    if (e.identity)
+      memcpy(e.inv, id3, sizeof(e.inv));
+    else
+      invertXfm(e.xfm, e.inv);
+    e.geom = gi;
+    e.inst_id = o.inst;
+    he.push_back(e);
+    const vrx_bvh_header &H = o.surf->geom->header;
+    if (H.n > 0) {
+      float bx[6];
+      if (e.identity) {
+        memcpy(bx, H.lo, 12);
+        memcpy(bx + 3, H.hi, 12);
+      } else
+        xfmBox(e.xfm, H.lo, H.hi, bx);
+      for (int k = 0; k < 3; k++) {
+        wb[k] = std::min(wb[k], bx[k]);
+        wb[3 + k] = std::max(wb[3 + k], bx[3 + k]);
+      }
+    }
+  }
+  std::vector<vrx_volume> hv;
+  for (size_t i = 0; i < vols.size(); i++) {
+    Volume *v = vols[i];
+    vrx_volume vv{};
+    vv.d_data = v->field->deviceData();
+    memcpy(vv.dims, v->field->dims, 12);
+    memcpy(vv.origin, v->field->origin, 12);
+    memcpy(vv.spacing, v->field->spacing, 12);
+    vv.d_lut = (const float *)v->dLut;
+    vv.value_range[0] = v->valueRange[0];
+    vv.value_range[1] = v->valueRange[1];
+    vv.unit_distance = v->unitDistance;
+    vv.id = v->id;
+    vv.inst_id = volInst[i];
+    hv.push_back(vv);
+    float lo[3], hi[3];
+    v->field->bounds(lo, hi);
+    for (int k = 0; k < 3; k++) {
+      wb[k] = std::min(wb[k], lo[k]);
+      wb[3 + k] = std::max(wb[3 + k], hi[k]);
+    }
+  }
+  freeScene();
+  auto upload = [&](void *&dst, const void *src, size_t bytes) -> bool {
+    if (!bytes)
+      return true;
+    if (!dev->check(vrx_malloc(&dst, bytes), this, "scene allocation"))
+      return false;
+    return dev->check(vrx_memcpy(dst, src, bytes, VRX_H2D, s), this, "scene upload");
+  };
+  if (!upload(dGeoms, hg.data(), hg.size() * sizeof(vrx_geom))
+      || !upload(dEntries, he.data(), he.size() * sizeof(vrx_entry))
+      || !upload(dVolumes, hv.data(), hv.size() * sizeof(vrx_volume)))
+    return -1;
+  memset(&scene, 0, sizeof(scene));
+  scene.d_geoms = (const vrx_geom *)dGeoms;
+  scene.num_geoms = (uint32_t)hg.size();
+  scene.d_entries = (const vrx_entry *)dEntries;
+  scene.num_entries = (uint32_t)he.size();
+  scene.d_volumes = (const vrx_volume *)dVolumes;
+  scene.num_volumes = (uint32_t)hv.size();
+  scene.max_blas_height = maxH;
+  scene.single_type = single >= 0 ? single : -1;
+  if (he.size() > 1) {
+    vrx_bvh_sizes sz;
+    vrx_tlas_query_sizes((uint32_t)he.size(), &sz);
+    if (!dev->check(vrx_malloc(&dTlasNodes, sz.nodes), this, "tlas")
+        || !dev->check(vrx_malloc(&dTlasLeaves, sz.leaves), this, "tlas")
+        || !dev->check(vrx_malloc(&dTlasIds, sz.leaf_ids), this, "tlas")
+        || !dev->check(vrx_malloc(&dTlasHeader, sz.header), this, "tlas"))
+      return -1;
+    scene.tlas.d_nodes = dTlasNodes;
+    scene.tlas.d_leaves = dTlasLeaves;
+    scene.tlas.d_leaf_ids = (uint32_t *)dTlasIds;
+    scene.tlas.d_header = dTlasHeader;
+    void *ws = dev->workspace(sz.workspace);
+    if (!ws)
+      return -1;
+    vrx_event_record(e0, s);
+    if (!dev->check(vrx_tlas_build(&scene, &scene.tlas, ws, sz.workspace, s), this, "tlas build"))
+      return -1;
+    vrx_event_record(e1, s);
+    vrx_bvh_header th;
+    dev->check(vrx_memcpy(&th, dTlasHeader, sizeof(th), VRX_D2H, s), this, "tlas header");
+    dev->check(vrx_stream_sync(s), this, "stream sync");
+    float ms2 = 0.0f;
+    vrx_event_elapsed_ms(e0, e1, &ms2);
+    lastBuildMs += ms2;
+    scene.tlas_height = th.height;
+  }
+  vrx_event_destroy(e0);
+  vrx_event_destroy(e1);
+  memcpy(bounds, wb, sizeof(bounds));
+  sceneDirty = false;
+  builtAt = dev->tick();
+  return 0;
+}
+
+bool World::getProperty(
+    const std::string &name, ANARIDataType t, void *mem, uint64_t size, uint32_t mask)
+{
+  if (name == "bounds" && t == ANARI_FLOAT32_BOX3 && size >= 24) {
+    // World.cpp:100-118: flush commits and rebuild before answering
+    dev->flushCommits();
+    if (needsRebuild() && rebuild() != 0)
+      return false;
+    memcpy(mem, bounds, 24);
+    return true;
+  }
+  if (name == "bvhBuildDuration" && t == ANARI_FLOAT32 && size >= 4) {
+    float sec = lastBuildMs / 1000.0f;
+    memcpy(mem, &sec, 4);
+    return true;
+  }
+  return Object::getProperty(name, t, mem, size, mask);
+}
+
+// ---------------------------------------------------------------------------
+// Camera: perspective (camera/Perspective.cpp:40-72, Camera.cpp:68-76)
+// ---------------------------------------------------------------------------
+void Camera::commit()
+{
+  float region[4] = {0, 0, 1, 1};
+  getRaw("imageRegion", ANARI_FLOAT32_BOX2, region);
+  float pos[3] = {0, 0, 0}, dir[3] = {0, 0, 1}, up[3] = {0, 1, 0};
+  getRaw("position", ANARI_FLOAT32_VEC3, pos);
+  getRaw("direction", ANARI_FLOAT32_VEC3, dir);
+  getRaw("up", ANARI_FLOAT32_VEC3, up);
+  float fovy = get<float>("fovy", ANARI_FLOAT32, 1.04719755f); // radians(60)
+  float aspect = get<float>("aspect", ANARI_FLOAT32, 1.0f);
+  if (subtype != "perspective")
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "camera subtype '%s' is not supported", subtype.c_str());
+  vrx_camera_perspective(pos, dir, up, fovy, aspect, region, &cam);
+}
+
+// ---------------------------------------------------------------------------
+// Renderer (renderer/Renderer.cpp:97-207, Raycast.cpp:45-50, AmbientOcclusion.cpp:44-56)
+// ---------------------------------------------------------------------------
+Renderer::Renderer(Device *d, const std::string &sub) : Object(d, ANARI_RENDERER, sub)
+{
+  if (sub == "raycast")
+    kind = VRX_RENDER_RAYCAST;
+  else if (sub == "ao" || sub == "default")
+    kind = VRX_RENDER_AO;
+}
+
+void Renderer::commit()
+{
+  memset(&r, 0, sizeof(r));
+  r.kind = kind < 0 ? VRX_RENDER_RAYCAST : kind;
+  float bg[4] = {0, 0, 0, 1};
+  getRaw("background", ANARI_FLOAT32_VEC4, bg);
+  memcpy(r.background, bg, 16);
+  float ac[3] = {1, 1, 1};
+  getRaw("ambientColor", ANARI_FLOAT32_VEC3, ac);
+  memcpy(r.ambient_color, ac, 12);
+  r.ambient_radiance = get<float>("ambientRadiance", ANARI_FLOAT32, 1.0f);
+  r.pixel_samples = get<int>("pixelSamples", ANARI_INT32, 1);
+  r.ambient_samples = std::clamp(get<int>("ambientSamples", ANARI_INT32, 1), 0, 256);
+  r.ao_distance = get<float>("ambientOcclusionDistance", ANARI_FLOAT32, 1e20f);
+  r.cull_backfaces = get<int>("cullTriangleBackfaces", ANARI_BOOL, 0);
+  float rate = std::clamp(get<float>("volumeSamplingRate", ANARI_FLOAT32, 0.125f), 1e-3f, 10.0f);
+  r.inv_volume_sampling_rate = 1.0f / rate;
+  r.stack_lds = get<int>("stackDepthLDS", ANARI_INT32, 0);
+  sampleLimit = get<int>("sampleLimit", ANARI_INT32, 128);
+  if (kind == VRX_RENDER_RAYCAST)
+    sampleLimit = 1; // single-shot renderer
+  if (kind < 0)
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "renderer subtype '%s' is not supported (raycast, ao)", subtype.c_str());
+}
+
+// ---------------------------------------------------------------------------
+// Frame (frame/Frame.cu:75-310, 312-443, 559-660)
+// ---------------------------------------------------------------------------
+Frame::Frame(Device *d) : Object(d, ANARI_FRAME, "")
+{
+  vrx_event_create(&evStart);
+  vrx_event_create(&evEnd);
+}
+
+Frame::~Frame()
+{
+  if (rendered)
+    vrx_event_sync(evEnd);
+  freeBuffers();
+  vrx_event_destroy(evStart);
+  vrx_event_destroy(evEnd);
+}
+
+void Frame::freeBuffers()
+{
+  for (void **p : {&dAccum, &dColor, &dDepth, &dPrim, &dObj, &dInst, &dScratch, &dCounters}) {
+    vrx_free(*p);
+    *p = nullptr;
+  }
+  scratchBytes = 0;
+  allocated = false;
+}
+
+bool Frame::isValid() const
+{
+  return renderer && renderer->isValid() && camera && camera->isValid() && world && world->isValid();
+}
+
+void Frame::commit()
+{
+  renderer = (Renderer *)getObject("renderer", ANARI_RENDERER);
+  camera = (Camera *)getObject("camera", ANARI_CAMERA);
+  world = (World *)getObject("world", ANARI_WORLD);
+  const Param *cb = find("frameCompletionCallback");
+  callback = cb && !cb->bytes.empty() ? *(ANARIFrameCompletionCallback *)cb->bytes.data() : nullptr;
+  const Param *cbu = find("frameCompletionCallbackUserData");
+  callbackUser = cbu && !cbu->bytes.empty() ? *(const void **)cbu->bytes.data() : nullptr;
+  uint32_t sz[2] = {10, 10};
+  getRaw("size", ANARI_UINT32_VEC2, sz);
+  size[0] = sz[0];
+  size[1] = sz[1];
+  colorType = get<ANARIDataType>("channel.color", ANARI_DATA_TYPE, ANARI_UFIXED8_RGBA_SRGB);
+  depthType = get<ANARIDataType>("channel.depth", ANARI_DATA_TYPE, ANARI_UNKNOWN);
+  primType = get<ANARIDataType>("channel.primitiveId", ANARI_DATA_TYPE, ANARI_UNKNOWN);
+  objType = get<ANARIDataType>("channel.objectId", ANARI_DATA_TYPE, ANARI_UNKNOWN);
+  instType = get<ANARIDataType>("channel.instanceId", ANARI_DATA_TYPE, ANARI_UNKNOWN);
+  // Frame.cu:119-122: any id channel implies depth
+  if (primType == ANARI_UINT32 || objType == ANARI_UINT32 || instType == ANARI_UINT32)
+    depthType = ANARI_FLOAT32;
+  freeBuffers();
+  nextReset = true;
+}
+
+void Frame::allocate()
+{
+  size_t np = (size_t)size[0] * size[1];
+  bool f32 = colorType == ANARI_FLOAT32_VEC4;
+  bool ok = dev->check(vrx_malloc(&dAccum, np * 16), this, "frame buffer")
+      && dev->check(vrx_malloc(&dColor, np * (f32 ? 16 : 4)), this, "frame buffer")
+      && dev->check(vrx_malloc(&dCounters, 16), this, "frame counters");
+  if (ok && depthType == ANARI_FLOAT32)
+    ok = dev->check(vrx_malloc(&dDepth, np * 4), this, "depth buffer");
+  if (ok && primType == ANARI_UINT32)
+    ok = dev->check(vrx_malloc(&dPrim, np * 4), this, "primitiveId buffer");
+  if (ok && objType == ANARI_UINT32)
+    ok = dev->check(vrx_malloc(&dObj, np * 4), this, "objectId buffer");
+  if (ok && instType == ANARI_UINT32)
+    ok = dev->check(vrx_malloc(&dInst, np * 4), this, "instanceId buffer");
+  allocated = ok;
+}
+
+void Frame::wait()
+{
+  if (rendered)
+    vrx_event_sync(evEnd);
+}
+
+void Frame::render()
+{
+  wait();
+  dev->flushCommits();
+  if (world && world->needsRebuild()) {
+    if (world->rebuild() != 0)
+      return;
+  }
+  if (!isValid()) {
+    const char *why = !renderer ? "missing ANARIRenderer"
+        : !renderer->isValid()  ? "invalid ANARIRenderer"
+        : !camera               ? "missing ANARICamera"
+        : !camera->isValid()    ? "invalid ANARICamera"
+        : !world                ? "missing ANARIWorld"
+                                : "invalid ANARIWorld";
+    report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_OPERATION,
+        "skipping render of incomplete or invalid frame object -- issue: %s", why);
+    return;
+  }
+  if (!allocated)
+    allocate();
+  if (!allocated)
+    return;
+  // Frame.cu:574-588 accumulation reset on any finalize since the last frame
+  if (dev->lastFinalize > lastSeenFinalize) {
+    lastSeenFinalize = dev->lastFinalize;
+    nextReset = true;
+  }
+  const int spp = renderer->spp();
+  if (!nextReset && renderer->sampleLimit > 0 && frameID >= (uint32_t)renderer->sampleLimit)
+    return; // Frame.cu:251-253
+  if (nextReset) {
+    frameID = 0;
+    nextReset = false;
+  } else {
+    frameID += (uint32_t)spp;
+  }
+  vrx_frame f{};
+  f.width = size[0];
+  f.height = size[1];
+  f.color_format = colorType == ANARI_FLOAT32_VEC4 ? VRX_COLOR_F32
+      : colorType == ANARI_UFIXED8_RGBA_SRGB        ? VRX_COLOR_SRGB8
+                                                    : VRX_COLOR_UNORM8;
+  f.frame_id = frameID;
+  f.d_accum = (float *)dAccum;
+  f.d_color_f32 = colorType == ANARI_FLOAT32_VEC4 ? (float *)dColor : nullptr;
+  f.d_color_u32 = colorType == ANARI_FLOAT32_VEC4 ? nullptr : (uint32_t *)dColor;
+  f.d_depth = (float *)dDepth;
+  f.d_prim_id = (uint32_t *)dPrim;
+  f.d_obj_id = (uint32_t *)dObj;
+  f.d_inst_id = (uint32_t *)dInst;
+  size_t need = vrx_render_scratch_bytes(&world->scene, &renderer->r, f.width, f.height);
+  if (need > scratchBytes) {
+    vrx_free(dScratch);
+    dScratch = nullptr;
+    scratchBytes = 0;
+    if (!dev->check(vrx_malloc(&dScratch, need), this, "render scratch"))
+      return;
+    scratchBytes = need;
+    vrx_fill_u32(dScratch, 0u, 64, dev->stream);
+  }
+  f.d_scratch = dScratch;
+  f.scratch_bytes = scratchBytes;
+  f.d_counters = (uint64_t *)dCounters;
+  vrx_fill_u32(dCounters, 0u, 4, dev->stream);
+  vrx_event_record(evStart, dev->stream);
+  dev->check(vrx_render(&world->scene, &camera->cam, &renderer->r, &f, dev->stream), this, "render");
+  if (callback) {
+    struct Ctx {
+      ANARIFrameCompletionCallback cb;
+      const void *user;
+      Device *dev;
+      Frame *frame;
+    };
+    static thread_local Ctx ctx;
+    ctx = Ctx{callback, callbackUser, dev, this};
+    vrx_launch_host_func(
+        dev->stream,
+        [](void *p) {
+          Ctx *c = (Ctx *)p;
+          c->cb(c->user, (ANARIDevice)c->dev, (ANARIFrame)c->frame);
+        },
+        &ctx);
+  }
+  vrx_event_record(evEnd, dev->stream);
+  vrx_memcpy(counters, dCounters, 16, VRX_D2H, dev->stream);
+  rendered = true;
+}
+
+const void *Frame::map(const std::string &channel, uint32_t *w, uint32_t *h, ANARIDataType *pt)
+{
+  wait();
+  *pt = ANARI_UNKNOWN;
+  const size_t np = (size_t)size[0] * size[1];
+  auto fetch = [&](void *d, std::vector<uint8_t> &hb, size_t bytes) -> const void * {
+    if (!d)
+      return nullptr;
+    hb.resize(bytes);
+    if (!dev->check(vrx_memcpy_sync(hb.data(), d, bytes, VRX_D2H), this, "frame download"))
+      return nullptr;
+    return hb.data();
+  };
+  const bool f32 = colorType == ANARI_FLOAT32_VEC4;
+  const void *ret = nullptr;
+  ANARIDataType t = ANARI_UNKNOWN;
+  bool gpu = false;
+  std::string ch = channel;
+  for (const char *suf : {"GPU", "CUDA", "HIP"}) {
+    size_t L = strlen(suf);
+    if (ch.size() > L && ch.compare(ch.size() - L, L, suf) == 0) {
+      ch = ch.substr(0, ch.size() - L);
+      gpu = true;
+      break;
+    }
+  }
+  if (!allocated)
+    return nullptr;
+  if (ch == "channel.color") {
+    t = colorType;
+    ret = gpu ? dColor : fetch(dColor, hColor, np * (f32 ? 16 : 4));
+  } else if (ch == "channel.depth" && dDepth) {
+    t = ANARI_FLOAT32;
+    ret = gpu ? dDepth : fetch(dDepth, hDepth, np * 4);
+  } else if (ch == "channel.primitiveId" && dPrim) {
+    t = ANARI_UINT32;
+    ret = gpu ? dPrim : fetch(dPrim, hPrim, np * 4);
+  } else if (ch == "channel.objectId" && dObj) {
+    t = ANARI_UINT32;
+    ret = gpu ? dObj : fetch(dObj, hObj, np * 4);
+  } else if (ch == "channel.instanceId" && dInst) {
+    t = ANARI_UINT32;
+    ret = gpu ? dInst : fetch(dInst, hInst, np * 4);
+  }
+  if (ret) {
+    *w = size[0];
+    *h = size[1];
+    *pt = t;
+  }
+  return ret;
+}
+
+bool Frame::getProperty(const std::string &name, ANARIDataType t, void *mem, uint64_t sz, uint32_t mask)
+{
+  if (name == "duration" && t == ANARI_FLOAT32 && sz >= 4) {
+    if (mask & ANARI_WAIT)
+      wait();
+    float ms = 0.0f;
+    if (rendered)
+      vrx_event_elapsed_ms(evStart, evEnd, &ms);
+    duration = ms / 1000.0f;
+    memcpy(mem, &duration, 4);
+    return true;
+  }
+  if (name == "numSamples" && t == ANARI_INT32 && sz >= 4) {
+    if (mask & ANARI_WAIT)
+      wait();
+    int v = (int)frameID;
+    memcpy(mem, &v, 4);
+    return true;
+  }
+  if (name == "nextFrameReset" && t == ANARI_BOOL && sz >= 4) {
+    if (mask & ANARI_WAIT)
+      wait();
+    int v = (nextReset || dev->lastFinalize > lastSeenFinalize) ? 1 : 0;
+    memcpy(mem, &v, 4);
+    return true;
+  }
+  if (name == "raysTraced" && t == ANARI_UINT64 && sz >= 8) {
+    wait();
+    memcpy(mem, &counters[0], 8);
+    return true;
+  }
+  if (name == "volumeSamples" && t == ANARI_UINT64 && sz >= 8) {
+    wait();
+    memcpy(mem, &counters[1], 8);
+    return true;
+  }
+  return Object::getProperty(name, t, mem, sz, mask);
+}
+
+// ---------------------------------------------------------------------------
+// Device (VisRTXDevice.cpp:102-762)
+// ---------------------------------------------------------------------------
+Device::Device(ANARIStatusCallback cb, const void *user) : statusCB(cb), statusUser(user) {}
+
+Device::~Device()
+{
+  if (stream)
+    vrx_stream_sync(stream);
+  for (Object *o : commitQueue)
+    releaseInternal(o);
+  commitQueue.clear();
+  vrx_free(dWorkspace);
+  if (stream)
+    vrx_stream_destroy(stream);
+}
+
+// initDevice (VisRTXDevice.cpp:435-458): lazy, latched on failure
+bool Device::init()
+{
+  if (initialized)
+    return true;
+  if (initFailed)
+    return false;
+  int count = 0;
+  if (vrx_device_count(&count) != 0 || count == 0) {
+    reportf(nullptr, ANARI_SEVERITY_FATAL_ERROR, ANARI_STATUS_UNSUPPORTED_DEVICE,
+        "no HIP device found: %s", vrx_last_error());
+    initFailed = true;
+    return false;
+  }
+  int want = 0;
+  for (const char *k : {"hipDevice", "cudaDevice"}) {
+    auto it = params.find(k);
+    if (it != params.end() && it->second.bytes.size() >= 4)
+      memcpy(&want, it->second.bytes.data(), 4);
+  }
+  if (want < 0 || want >= count) {
+    // VisRTXDevice.cpp:570-577
+    reportf(nullptr, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "device index %d out of range (%d devices), using device 0", want, count);
+    want = 0;
+  }
+  hipDevice = want;
+  if (vrx_set_device(want) != 0 || vrx_stream_create(&stream) != 0) {
+    reportf(nullptr, ANARI_SEVERITY_FATAL_ERROR, ANARI_STATUS_UNKNOWN_ERROR,
+        "device initialisation failed: %s", vrx_last_error());
+    initFailed = true;
+    return false;
+  }
+  initialized = true;
+  return true;
+}
+
+void *Device::workspace(size_t bytes)
+{
+  if (dWorkspace && workspaceBytes >= bytes)
+    return dWorkspace;
+  if (dWorkspace) {
+    vrx_stream_sync(stream);
+    vrx_free(dWorkspace);
+  }
+  dWorkspace = nullptr;
+  workspaceBytes = 0;
+  if (!check(vrx_malloc(&dWorkspace, bytes), nullptr, "workspace allocation"))
+    return nullptr;
+  workspaceBytes = bytes;
+  return dWorkspace;
+}
+
+void Device::queueCommit(Object *o)
+{
+  if (o->queued)
+    return;
+  o->queued = true;
+  retainInternal(o);
+  commitQueue.push_back(o);
+}
+
+// commitBuffer.flush (RegisteredObject.h:66-72, Frame.cu:201-208): arrays
+// and leaf objects first so that parents see finalized children.
+static int commitOrder(ANARIDataType t)
+{
+  switch (t) {
+  case ANARI_ARRAY1D:
+  case ANARI_ARRAY2D:
+  case ANARI_ARRAY3D:
+    return 0;
+  case ANARI_SPATIAL_FIELD:
+  case ANARI_GEOMETRY:
+  case ANARI_MATERIAL:
+  case ANARI_CAMERA:
+  case ANARI_RENDERER:
+    return 1;
+  case ANARI_SURFACE:
+  case ANARI_VOLUME:
+    return 2;
+  case ANARI_GROUP:
+    return 3;
+  case ANARI_INSTANCE:
+    return 4;
+  case ANARI_WORLD:
+    return 5;
+  default:
+    return 6;
+  }
+}
+
+void Device::flushCommits()
+{
+  if (commitQueue.empty())
+    return;
+  std::vector<Object *> q;
+  q.swap(commitQueue);
+  std::stable_sort(q.begin(), q.end(),
+      [](Object *a, Object *b) { return commitOrder(a->type) < commitOrder(b->type); });
+  for (Object *o : q) {
+    o->queued = false;
+    o->commit();
+    o->lastFinalized = tick();
+    lastFinalize = o->lastFinalized;
+    if (o->type != ANARI_CAMERA && o->type != ANARI_RENDERER && o->type != ANARI_FRAME)
+      lastSceneFinalize = o->lastFinalized;
+  }
+  // a changed child dirties the worlds that reference it
+  for (Object *o : q) {
+    if (o->type == ANARI_WORLD)
+      ((World *)o)->sceneDirty = true;
+  }
+  for (Object *o : q)
+    releaseInternal(o);
+}
+
+void Device::report(Object *src, int severity, int code, const char *fmt, va_list ap)
+{
+  if (!statusCB)
+    return;
+  char buf[1024];
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  statusCB(statusUser, (ANARIDevice)this, (ANARIObject)src, src ? src->type : ANARI_DEVICE,
+      severity, code, buf);
+}
+
+void Device::reportf(Object *src, int severity, int code, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  report(src, severity, code, fmt, ap);
+  va_end(ap);
+}
+
+bool Device::check(int st, Object *src, const char *what)
+{
+  if (st == VRX_OK)
+    return true;
+  reportf(src, st == VRX_ERR_NOMEM ? ANARI_SEVERITY_FATAL_ERROR : ANARI_SEVERITY_ERROR,
+      st == VRX_ERR_NOMEM ? ANARI_STATUS_OUT_OF_MEMORY : ANARI_STATUS_UNKNOWN_ERROR, "%s: %s", what,
+      vrx_last_error());
+  return false;
+}
+
+} // namespace vrxa

@@ -1,0 +1,310 @@
+// objects.h -- ANARI object model of the visrtx_amd device (host C++).
+//
+// Mirrors the reference device's object layer (devices/rtx/Object.{h,cpp},
+// RegisteredObject.h:39-92, array/, camera/, scene/, renderer/, frame/) with
+// the helium semantics the reference inherits from the ANARI-SDK:
+// parameters are staged until anariCommitParameters, commits are deferred
+// and flushed at the next renderFrame / blocking property query
+// (frame/Frame.cu:201-208), objects are reference counted (public + internal
+// references), and no exception crosses the C API -- errors go to the status
+// callback.  All GPU work goes through the C ABI in include/vrx_hip.h.
+#pragma once
+
+#include <stdarg.h>
+#include <stdint.h>
+
+#include <array>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "anari/anari.h"
+#include "vrx_hip.h"
+
+namespace vrxa {
+
+struct Device;
+struct Object;
+
+size_t sizeOf(ANARIDataType t);
+bool isObjectType(ANARIDataType t);
+const char *typeName(ANARIDataType t);
+
+struct Param {
+  ANARIDataType type{ANARI_UNKNOWN};
+  std::vector<uint8_t> bytes;
+  std::string str;
+  Object *obj{nullptr};
+};
+
+struct Object {
+  Object(Device *d, ANARIDataType t, std::string sub);
+  virtual ~Object();
+
+  Device *dev;
+  ANARIDataType type;
+  std::string subtype;
+  int pubRef{1};
+  int intRef{0};
+  std::map<std::string, Param> staged;
+  std::map<std::string, Param> params; // committed snapshot
+  bool queued{false};
+  uint64_t lastFinalized{0};
+
+  // called by the device when the deferred commit is flushed
+  virtual void commit() {}
+  virtual bool isValid() const { return true; }
+  virtual bool getProperty(const std::string &name, ANARIDataType type, void *mem, uint64_t size,
+      uint32_t mask);
+  virtual void onPublicReleaseToZero() {}
+
+  // parameter access on the committed snapshot
+  const Param *find(const char *name) const;
+  bool getRaw(const char *name, ANARIDataType t, void *out) const;
+  template <typename T>
+  T get(const char *name, ANARIDataType t, T def) const
+  {
+    T v = def;
+    getRaw(name, t, &v);
+    return v;
+  }
+  std::string getString(const char *name, const std::string &def) const;
+  Object *getObject(const char *name, ANARIDataType t) const;
+  bool has(const char *name) const { return find(name) != nullptr; }
+
+  void report(int severity, int code, const char *fmt, ...) const;
+};
+
+void retainInternal(Object *o);
+void releaseInternal(Object *o);
+
+// ---------------------------------------------------------------------------
+struct Array : Object {
+  Array(Device *d, ANARIDataType arrType, const void *appMem, ANARIMemoryDeleter del,
+      const void *userData, ANARIDataType elem, uint64_t n1, uint64_t n2, uint64_t n3);
+  ~Array() override;
+
+  ANARIDataType elem;
+  uint64_t dims[3];
+  size_t elemSize;
+  const void *app{nullptr};
+  ANARIMemoryDeleter deleter{nullptr};
+  const void *deleterUser{nullptr};
+  std::vector<uint8_t> owned; // managed or privatized storage
+  bool mapped{false};
+  uint64_t version{1};
+  // device copy
+  void *dptr{nullptr};
+  size_t dbytes{0};
+  uint64_t dversion{0};
+  std::vector<Object *> objects; // object arrays: internal references
+
+  uint64_t count() const { return dims[0] * dims[1] * dims[2]; }
+  size_t bytes() const { return count() * elemSize; }
+  const void *host() const { return owned.empty() ? app : owned.data(); }
+  void *map();
+  void unmap();
+  const void *device(); // uploads on first use / after unmap
+  void onPublicReleaseToZero() override; // privatize shared memory
+  void refreshObjects();
+};
+
+// ---------------------------------------------------------------------------
+struct Geometry : Object {
+  Geometry(Device *d, const std::string &sub);
+  ~Geometry() override;
+  void commit() override;
+  bool isValid() const override { return valid; }
+
+  int ptype{-1};
+  bool valid{false};
+  Array *pos{nullptr}, *index{nullptr}, *radius{nullptr}, *primId{nullptr}, *normal{nullptr};
+  float globalRadius{0.01f};
+  bool caps{false};
+  bool cull{false};
+  uint32_t n{0};
+  // BVH
+  bool bvhDirty{true};
+  vrx_bvh bvh{};
+  vrx_bvh_header header{};
+  void *dNodes{nullptr}, *dLeaves{nullptr}, *dIds{nullptr}, *dHeader{nullptr};
+  size_t capNodes{0}, capLeaves{0}, capIds{0};
+  int buildBVH(vrx_stream s);
+  void freeBVH();
+};
+
+struct Material : Object {
+  Material(Device *d, const std::string &sub) : Object(d, ANARI_MATERIAL, sub) {}
+  void commit() override;
+  bool isValid() const override { return subtype == "matte"; }
+  float color[3]{0.8f, 0.8f, 0.8f};
+  float opacity{1.0f};
+};
+
+struct Surface : Object {
+  Surface(Device *d) : Object(d, ANARI_SURFACE, "") {}
+  void commit() override;
+  bool isValid() const override;
+  Geometry *geom{nullptr};
+  Material *mat{nullptr};
+  uint32_t id{~0u};
+};
+
+struct SpatialField : Object {
+  SpatialField(Device *d, const std::string &sub) : Object(d, ANARI_SPATIAL_FIELD, sub) {}
+  ~SpatialField() override;
+  void commit() override;
+  bool isValid() const override { return valid; }
+  bool valid{false};
+  Array *data{nullptr};
+  float origin[3]{0, 0, 0};
+  float spacing[3]{1, 1, 1};
+  uint32_t dims[3]{0, 0, 0};
+  void *dData{nullptr}; // float32 copy on the device
+  uint64_t uploadedVersion{0};
+  const float *deviceData();
+  void bounds(float lo[3], float hi[3]) const;
+};
+
+struct Volume : Object {
+  Volume(Device *d, const std::string &sub) : Object(d, ANARI_VOLUME, sub) {}
+  ~Volume() override;
+  void commit() override;
+  bool isValid() const override { return field && field->isValid(); }
+  SpatialField *field{nullptr};
+  uint32_t id{~0u};
+  float valueRange[2]{0.0f, 1.0f};
+  float unitDistance{1.0f};
+  float lut[256 * 4];
+  void *dLut{nullptr};
+};
+
+struct Group : Object {
+  Group(Device *d) : Object(d, ANARI_GROUP, "") {}
+  void commit() override;
+  std::vector<Surface *> surfaces;
+  std::vector<Volume *> volumes;
+};
+
+struct Instance : Object {
+  Instance(Device *d, const std::string &sub) : Object(d, ANARI_INSTANCE, sub) {}
+  void commit() override;
+  Group *group{nullptr};
+  float xfm[12]{1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+  uint32_t id{~0u};
+};
+
+struct World : Object {
+  World(Device *d) : Object(d, ANARI_WORLD, "") {}
+  ~World() override;
+  void commit() override;
+  bool getProperty(const std::string &name, ANARIDataType type, void *mem, uint64_t size,
+      uint32_t mask) override;
+  std::vector<Instance *> instances;
+  std::vector<Surface *> surfaces;
+  std::vector<Volume *> volumes;
+  uint32_t id{~0u};
+
+  // device scene
+  bool sceneDirty{true};
+  uint64_t builtAt{0};
+  vrx_scene scene{};
+  void *dGeoms{nullptr}, *dEntries{nullptr}, *dVolumes{nullptr};
+  void *dTlasNodes{nullptr}, *dTlasLeaves{nullptr}, *dTlasIds{nullptr}, *dTlasHeader{nullptr};
+  float bounds[6]{0, 0, 0, 0, 0, 0};
+  float lastBuildMs{0.0f};
+  int rebuild();
+  void freeScene();
+  bool needsRebuild() const;
+};
+
+struct Camera : Object {
+  Camera(Device *d, const std::string &sub) : Object(d, ANARI_CAMERA, sub) {}
+  void commit() override;
+  bool isValid() const override { return subtype == "perspective"; }
+  vrx_camera cam{};
+};
+
+struct Renderer : Object {
+  Renderer(Device *d, const std::string &sub);
+  void commit() override;
+  bool isValid() const override { return kind >= 0; }
+  int kind{-1};
+  vrx_renderer r{};
+  int sampleLimit{128};
+  int spp() const { return r.kind == VRX_RENDER_AO ? (r.pixel_samples > 1 ? r.pixel_samples : 1) : 1; }
+};
+
+struct Frame : Object {
+  Frame(Device *d);
+  ~Frame() override;
+  void commit() override;
+  bool isValid() const override;
+  bool getProperty(const std::string &name, ANARIDataType type, void *mem, uint64_t size,
+      uint32_t mask) override;
+
+  Renderer *renderer{nullptr};
+  Camera *camera{nullptr};
+  World *world{nullptr};
+  ANARIFrameCompletionCallback callback{nullptr};
+  const void *callbackUser{nullptr};
+  uint32_t size[2]{10, 10};
+  ANARIDataType colorType{ANARI_UFIXED8_RGBA_SRGB};
+  ANARIDataType depthType{ANARI_UNKNOWN}, primType{ANARI_UNKNOWN}, objType{ANARI_UNKNOWN},
+      instType{ANARI_UNKNOWN};
+
+  // device buffers
+  void *dAccum{nullptr}, *dColor{nullptr}, *dDepth{nullptr}, *dPrim{nullptr}, *dObj{nullptr},
+      *dInst{nullptr}, *dScratch{nullptr}, *dCounters{nullptr};
+  size_t scratchBytes{0};
+  std::vector<uint8_t> hColor, hDepth, hPrim, hObj, hInst;
+  vrx_event evStart{nullptr}, evEnd{nullptr};
+  bool rendered{false};
+  bool nextReset{true};
+  uint64_t lastSeenFinalize{0};
+  uint32_t frameID{0};
+  float duration{0.0f};
+  uint64_t counters[2]{0, 0};
+  bool allocated{false};
+
+  void allocate();
+  void freeBuffers();
+  void render();
+  void wait();
+  const void *map(const std::string &channel, uint32_t *w, uint32_t *h, ANARIDataType *pt);
+};
+
+// ---------------------------------------------------------------------------
+struct Device {
+  Device(ANARIStatusCallback cb, const void *user);
+  ~Device();
+
+  std::recursive_mutex mutex;
+  ANARIStatusCallback statusCB{nullptr};
+  const void *statusUser{nullptr};
+  int refCount{1};
+  int hipDevice{0};
+  bool initialized{false};
+  bool initFailed{false};
+  vrx_stream stream{nullptr};
+  std::vector<Object *> commitQueue;
+  uint64_t finalizeClock{1};
+  uint64_t lastFinalize{0};
+  uint64_t lastSceneFinalize{0}; // last finalize of an object a world depends on
+  // BVH build workspace (grow only)
+  void *dWorkspace{nullptr};
+  size_t workspaceBytes{0};
+  std::map<std::string, Param> params;
+
+  bool init();
+  void *workspace(size_t bytes);
+  void flushCommits();
+  void queueCommit(Object *o);
+  uint64_t tick() { return ++finalizeClock; }
+  void report(Object *src, int severity, int code, const char *fmt, va_list ap);
+  void reportf(Object *src, int severity, int code, const char *fmt, ...);
+  bool check(int vrxStatus, Object *src, const char *what);
+};
+
+} // namespace vrxa

@@ -1,0 +1,673 @@
+// lbvh.hip -- software BVH build for gfx950.
+//
+// Replaces the closed OptiX acceleration build of the reference
+// (optix_visrtx.cpp:37-121, called from scene/Group.cpp:189-266 and
+// scene/World.cpp:202-254).  Pipeline, all integer work bit-exact:
+//   1. primitive bounds -> centroid bounds (exact min/max reduction)
+//   2. 30-bit Morton codes (10 bits per axis)
+//   3. stable LSD radix sort of (code, index), 4 passes of 8 bits; each pass
+//      is histogram -> single-workgroup spine scan -> LDS-staged scatter with
+//      wave-ballot stable ranking (tile = 256 lanes x 16 keys)
+//   4. Karras (HPG 2012) hierarchy, ties broken with index-augmented keys
+//   5. bottom-up refit with per-node arrival counters (acq_rel atomics)
+// Bounds inputs follow scene/surface/geometry/Sphere.cu:82-110 (c +- r),
+// Cylinder.cpp:53-102 (min/max(v0,v1) +- |r|) and Triangle.cpp:112-138.
+//
+// Node layout (64 B, one 128-byte line holds two nodes): the boxes of BOTH
+// children plus the child links, so one node fetch tests two children:
+//   a = (lo0.x lo0.y lo0.z hi0.x) b = (hi0.y hi0.z lo1.x lo1.y)
+//   c = (lo1.z hi1.x hi1.y hi1.z) d = (child0, child1, height0, height1)
+#include "vrx_internal.h"
+#include "vrx_lbvh.h"
+#include "vrx_math.h"
+
+namespace vrx {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kScanThreads = 1024;
+
+static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static uint32_t sort_blocks(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+struct Workspace {
+  uint32_t *keys_a, *vals_a, *keys_b, *vals_b;
+  uint32_t *hist;
+  uint32_t *cbox;   // 6 ordered-uint floats
+  uint32_t *parent; // 2n-1
+  uint32_t *flags;  // n-1
+};
+
+static size_t workspace_layout(uint32_t n, void *base, Workspace *w)
+{
+  size_t off = 0;
+  char *b = (char *)base;
+  auto take = [&](size_t bytes) -> void * {
+    void *p = b ? b + off : nullptr;
+    off += align256(bytes);
+    return p;
+  };
+  size_t nn = n ? n : 1;
+  Workspace ws;
+  ws.keys_a = (uint32_t *)take(4 * nn);
+  ws.vals_a = (uint32_t *)take(4 * nn);
+  ws.keys_b = (uint32_t *)take(4 * nn);
+  ws.vals_b = (uint32_t *)take(4 * nn);
+  ws.hist = (uint32_t *)take(4 * (size_t)256 * sort_blocks((uint32_t)nn));
+  ws.cbox = (uint32_t *)take(4 * 8);
+  ws.parent = (uint32_t *)take(4 * (2 * nn));
+  ws.flags = (uint32_t *)take(4 * nn);
+  if (w)
+    *w = ws;
+  return off;
+}
+
+size_t leaf_record_bytes(int type)
+{
+  switch (type) {
+  case VRX_TRIANGLE:
+    return 48;
+  case VRX_SPHERE:
+    return 16;
+  case VRX_CYLINDER:
+    return 32;
+  default:
+    return 0;
+  }
+}
+
+// --------------------------------------------------------------------------
+// ordered-uint encoding of floats for atomic min/max
+// --------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t f2o(float f)
+{
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t u)
+{
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+template <int T>
+__device__ __forceinline__ void prim_box(const vrx_prims &p, uint32_t i, v3 &lo, v3 &hi)
+{
+  if constexpr (T == VRX_SPHERE) {
+    uint32_t vi = p.d_index ? p.d_index[i] : i;
+    v3 c = ld3(p.d_position + 3 * (size_t)vi);
+    float r = p.d_radius ? p.d_radius[vi] : p.global_radius;
+    lo = mk(c.x - r, c.y - r, c.z - r);
+    hi = mk(c.x + r, c.y + r, c.z + r);
+  } else if constexpr (T == VRX_CYLINDER) {
+    uint32_t a = p.d_index ? p.d_index[2 * (size_t)i] : 2 * i;
+    uint32_t b = p.d_index ? p.d_index[2 * (size_t)i + 1] : 2 * i + 1;
+    v3 p0 = ld3(p.d_position + 3 * (size_t)a), p1 = ld3(p.d_position + 3 * (size_t)b);
+    float r = fabsf(p.d_radius ? p.d_radius[i] : p.global_radius);
+    v3 mn = vmin(p0, p1), mx = vmax(p0, p1);
+    lo = mk(mn.x - r, mn.y - r, mn.z - r);
+    hi = mk(mx.x + r, mx.y + r, mx.z + r);
+  } else if constexpr (T == VRX_TRIANGLE) {
+    uint32_t a = p.d_index ? p.d_index[3 * (size_t)i] : 3 * i;
+    uint32_t b = p.d_index ? p.d_index[3 * (size_t)i + 1] : 3 * i + 1;
+    uint32_t c = p.d_index ? p.d_index[3 * (size_t)i + 2] : 3 * i + 2;
+    v3 v0 = ld3(p.d_position + 3 * (size_t)a), v1 = ld3(p.d_position + 3 * (size_t)b),
+       v2 = ld3(p.d_position + 3 * (size_t)c);
+    lo = vmin(vmin(v0, v1), v2);
+    hi = vmax(vmax(v0, v1), v2);
+  } else {
+    const float *bx = p.d_position + 6 * (size_t)i;
+    lo = mk(bx[0], bx[1], bx[2]);
+    hi = mk(bx[3], bx[4], bx[5]);
+  }
+}
+
+__device__ __forceinline__ v3 centroid(v3 lo, v3 hi)
+{
+  return mk((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+}
+
+__global__ void k_cbox_init(uint32_t *cbox)
+{
+  if (threadIdx.x < 3)
+    cbox[threadIdx.x] = 0xFFFFFFFFu; // min slots
+  else if (threadIdx.x < 6)
+    cbox[threadIdx.x] = 0u; // max slots
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_centroid_bounds(vrx_prims p, uint32_t *cbox)
+{
+  __shared__ uint32_t red[6][256];
+  uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+    v3 lo, hi;
+    prim_box<T>(p, i, lo, hi);
+    v3 c = centroid(lo, hi);
+    float cc[3] = {c.x, c.y, c.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if (cc[k] == cc[k]) { // NaN centroids are ignored, as fminf/fmaxf do
+        uint32_t o = f2o(cc[k]);
+        mn[k] = min(mn[k], o);
+        mx[k] = max(mx[k], o);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    red[k][threadIdx.x] = mn[k];
+    red[3 + k][threadIdx.x] = mx[k];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        red[k][threadIdx.x] = min(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+        red[3 + k][threadIdx.x] = max(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3)
+    atomicMin(&cbox[threadIdx.x], red[threadIdx.x][0]);
+  else if (threadIdx.x < 6)
+    atomicMax(&cbox[threadIdx.x], red[threadIdx.x][0]);
+}
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v)
+{
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t quant10(float c, float lo, float scale)
+{
+  float x = (c - lo) * scale;
+  if (x >= 1023.0f)
+    return 1023u;
+  if (x > 0.0f)
+    return (uint32_t)x;
+  return 0u;
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_morton(
+    vrx_prims p, const uint32_t *cbox, uint32_t *keys, uint32_t *vals, float *cbox_out)
+{
+  float lo[3], hi[3], scale[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = o2f(cbox[k]);
+    hi[k] = o2f(cbox[3 + k]);
+    float ext = hi[k] - lo[k];
+    scale[k] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+  }
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cbox_out && i == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      cbox_out[k] = lo[k];
+      cbox_out[3 + k] = hi[k];
+    }
+  }
+  if (i >= p.n)
+    return;
+  v3 blo, bhi;
+  prim_box<T>(p, i, blo, bhi);
+  v3 c = centroid(blo, bhi);
+  uint32_t qx = quant10(c.x, lo[0], scale[0]);
+  uint32_t qy = quant10(c.y, lo[1], scale[1]);
+  uint32_t qz = quant10(c.z, lo[2], scale[2]);
+  keys[i] = expand_bits10(qx) * 4u + expand_bits10(qy) * 2u + expand_bits10(qz);
+  vals[i] = i;
+}
+
+// --------------------------------------------------------------------------
+// radix sort
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(
+    const uint32_t *keys, uint32_t n, uint32_t shift, uint32_t nblocks, uint32_t *hist)
+{
+  __shared__ uint32_t h[kSortThreads / 64][256];
+  const int wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < (kSortThreads / 64) * 256; i += kSortThreads)
+    (&h[0][0])[i] = 0u;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * (uint32_t)kSortTile;
+#pragma unroll
+  for (int k = 0; k < kSortItems; k++) {
+    uint32_t idx = base + (uint32_t)k * kSortThreads + threadIdx.x;
+    if (idx < n)
+      atomicAdd(&h[wave][(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  uint32_t d = threadIdx.x; // kSortThreads == 256 digits
+  uint32_t s = 0;
+#pragma unroll
+  for (int w = 0; w < kSortThreads / 64; w++)
+    s += h[w][d];
+  hist[(size_t)d * nblocks + blockIdx.x] = s;
+}
+
+// Exclusive scan, in place, of len u32 values by one workgroup.
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(uint32_t *data, uint32_t len)
+{
+  __shared__ uint32_t sums[kScanThreads];
+  const uint32_t chunk = (len + kScanThreads - 1) / kScanThreads;
+  const uint32_t beg = min(len, threadIdx.x * chunk);
+  const uint32_t end = min(len, beg + chunk);
+  uint32_t s = 0;
+  for (uint32_t i = beg; i < end; i++)
+    s += data[i];
+  sums[threadIdx.x] = s;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over 1024 sums
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    uint32_t v = (int)threadIdx.x >= off ? sums[threadIdx.x - off] : 0u;
+    __syncthreads();
+    sums[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
+  for (uint32_t i = beg; i < end; i++) {
+    uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+}
+
+// Stable LDS-staged scatter.  Element order inside a tile is (item k, wave,
+// lane), which is the original index order, so ranks are stable.
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *keys_in,
+    const uint32_t *vals_in, uint32_t *keys_out, uint32_t *vals_out, uint32_t n, uint32_t shift,
+    uint32_t nblocks, const uint32_t *offsets)
+{
+  constexpr int W = kSortThreads / 64;
+  __shared__ uint32_t digit_total[256];
+  __shared__ uint32_t round_cnt[W][256];
+  __shared__ uint32_t digit_start[256];
+  __shared__ uint32_t stage_k[kSortTile];
+  __shared__ uint32_t stage_v[kSortTile];
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t base = blockIdx.x * (uint32_t)kSortTile;
+
+  digit_total[tid] = 0u;
+#pragma unroll
+  for (int w = 0; w < W; w++)
+    round_cnt[w][tid] = 0u;
+
+  uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+#pragma unroll
+  for (int k = 0; k < kSortItems; k++) {
+    uint32_t idx = base + (uint32_t)k * kSortThreads + tid;
+    bool ok = idx < n;
+    key[k] = ok ? keys_in[idx] : 0xFFFFFFFFu;
+    val[k] = ok ? vals_in[idx] : 0u;
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int k = 0; k < kSortItems; k++) {
+    uint32_t idx = base + (uint32_t)k * kSortThreads + tid;
+    bool ok = idx < n;
+    uint32_t d = (key[k] >> shift) & 255u;
+    // lanes of this wave holding the same digit (valid ones only)
+    uint64_t m = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      uint64_t bb = __ballot((d >> b) & 1u);
+      m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t wrank = (uint32_t)__popcll(m & lt_mask);
+    uint32_t wcount = (uint32_t)__popcll(m);
+    if (ok && wrank == 0)
+      round_cnt[wave][d] = wcount;
+    __syncthreads();
+    if (ok) {
+      uint32_t pre = digit_total[d];
+      for (int w = 0; w < wave; w++)
+        pre += round_cnt[w][d];
+      rank[k] = pre + wrank;
+    }
+    __syncthreads();
+    {
+      uint32_t s = 0;
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        s += round_cnt[w][tid];
+        round_cnt[w][tid] = 0u;
+      }
+      digit_total[tid] += s;
+    }
+    __syncthreads();
+  }
+
+  // exclusive scan of the block's digit counts -> digit_start
+  if (tid < 64) {
+    uint32_t c0 = digit_total[4 * tid], c1 = digit_total[4 * tid + 1];
+    uint32_t c2 = digit_total[4 * tid + 2], c3 = digit_total[4 * tid + 3];
+    uint32_t s = c0 + c1 + c2 + c3;
+    uint32_t incl = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t v = __shfl_up(incl, off, 64);
+      if (lane >= off)
+        incl += v;
+    }
+    uint32_t ex = incl - s;
+    digit_start[4 * tid] = ex;
+    digit_start[4 * tid + 1] = ex + c0;
+    digit_start[4 * tid + 2] = ex + c0 + c1;
+    digit_start[4 * tid + 3] = ex + c0 + c1 + c2;
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int k = 0; k < kSortItems; k++) {
+    uint32_t idx = base + (uint32_t)k * kSortThreads + tid;
+    if (idx < n) {
+      uint32_t d = (key[k] >> shift) & 255u;
+      uint32_t slot = digit_start[d] + rank[k];
+      stage_k[slot] = key[k];
+      stage_v[slot] = val[k];
+    }
+  }
+  __syncthreads();
+
+  const uint32_t valid = min((uint32_t)kSortTile, n - base);
+  for (uint32_t i = tid; i < valid; i += kSortThreads) {
+    uint32_t kk = stage_k[i];
+    uint32_t d = (kk >> shift) & 255u;
+    uint32_t pos = offsets[(size_t)d * nblocks + blockIdx.x] + (i - digit_start[d]);
+    keys_out[pos] = kk;
+    vals_out[pos] = stage_v[i];
+  }
+}
+
+// --------------------------------------------------------------------------
+// Karras hierarchy
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int kdelta(const uint32_t *c, int64_t n, int64_t i, int64_t j)
+{
+  if (j < 0 || j >= n)
+    return -1;
+  uint32_t a = c[i], b = c[j];
+  if (a != b)
+    return __clz(a ^ b);
+  return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+__global__ __launch_bounds__(256) void k_karras(
+    const uint32_t *sc, uint32_t n_, Node *nodes, uint32_t *parent, uint32_t *flags)
+{
+  const int64_t n = n_;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1)
+    return;
+  int d = (kdelta(sc, n, i, i + 1) - kdelta(sc, n, i, i - 1)) > 0 ? 1 : -1;
+  int dmin = kdelta(sc, n, i, i - d);
+  int64_t lmax = 2;
+  while (kdelta(sc, n, i, i + lmax * d) > dmin)
+    lmax <<= 1;
+  int64_t l = 0;
+  for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+    if (kdelta(sc, n, i, i + (l + t) * d) > dmin)
+      l += t;
+  int64_t j = i + l * d;
+  int dnode = kdelta(sc, n, i, j);
+  int64_t s = 0, t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (kdelta(sc, n, i, i + (s + t) * d) > dnode)
+      s += t;
+  } while (t > 1);
+  int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+  int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+  uint32_t left = (lo == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
+  uint32_t right = (hi == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+  nodes[i].d = make_uint4(left, right, 0u, 0u);
+  parent[(left & kLeafBit) ? (uint32_t)(n - 1) + (left & ~kLeafBit) : left] = (uint32_t)i;
+  parent[(right & kLeafBit) ? (uint32_t)(n - 1) + (right & ~kLeafBit) : right] = (uint32_t)i;
+  flags[i] = 0u;
+  if (i == 0)
+    parent[0] = 0xFFFFFFFFu;
+}
+
+template <int T>
+__device__ __forceinline__ void write_leaf(
+    const vrx_prims &p, uint32_t orig, uint32_t k, void *leaves, v3 &lo, v3 &hi)
+{
+  if constexpr (T == VRX_SPHERE) {
+    uint32_t vi = p.d_index ? p.d_index[orig] : orig;
+    v3 c = ld3(p.d_position + 3 * (size_t)vi);
+    float r = p.d_radius ? p.d_radius[vi] : p.global_radius;
+    ((float4 *)leaves)[k] = make_float4(c.x, c.y, c.z, r);
+    lo = mk(c.x - r, c.y - r, c.z - r);
+    hi = mk(c.x + r, c.y + r, c.z + r);
+  } else if constexpr (T == VRX_CYLINDER) {
+    uint32_t a = p.d_index ? p.d_index[2 * (size_t)orig] : 2 * orig;
+    uint32_t b = p.d_index ? p.d_index[2 * (size_t)orig + 1] : 2 * orig + 1;
+    v3 p0 = ld3(p.d_position + 3 * (size_t)a), p1 = ld3(p.d_position + 3 * (size_t)b);
+    float r = fabsf(p.d_radius ? p.d_radius[orig] : p.global_radius);
+    float4 *L = (float4 *)leaves + 2 * (size_t)k;
+    L[0] = make_float4(p0.x, p0.y, p0.z, r);
+    L[1] = make_float4(p1.x, p1.y, p1.z, 0.0f);
+    v3 mn = vmin(p0, p1), mx = vmax(p0, p1);
+    lo = mk(mn.x - r, mn.y - r, mn.z - r);
+    hi = mk(mx.x + r, mx.y + r, mx.z + r);
+  } else if constexpr (T == VRX_TRIANGLE) {
+    uint32_t a = p.d_index ? p.d_index[3 * (size_t)orig] : 3 * orig;
+    uint32_t b = p.d_index ? p.d_index[3 * (size_t)orig + 1] : 3 * orig + 1;
+    uint32_t c = p.d_index ? p.d_index[3 * (size_t)orig + 2] : 3 * orig + 2;
+    v3 v0 = ld3(p.d_position + 3 * (size_t)a), v1 = ld3(p.d_position + 3 * (size_t)b),
+       v2 = ld3(p.d_position + 3 * (size_t)c);
+    float4 *L = (float4 *)leaves + 3 * (size_t)k;
+    L[0] = make_float4(v0.x, v0.y, v0.z, 0.0f);
+    L[1] = make_float4(v1.x, v1.y, v1.z, 0.0f);
+    L[2] = make_float4(v2.x, v2.y, v2.z, 0.0f);
+    lo = vmin(vmin(v0, v1), v2);
+    hi = vmax(vmax(v0, v1), v2);
+  } else {
+    prim_box<T>(p, orig, lo, hi);
+  }
+}
+
+__device__ __forceinline__ void write_header(
+    vrx_bvh_header *h, v3 lo, v3 hi, uint32_t height, uint32_t root, uint32_t n, int type)
+{
+  h->lo[0] = lo.x; h->lo[1] = lo.y; h->lo[2] = lo.z;
+  h->hi[0] = hi.x; h->hi[1] = hi.y; h->hi[2] = hi.z;
+  h->height = height;
+  h->root = root;
+  h->n = n;
+  h->type = type;
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_leaves_refit(vrx_prims p, const uint32_t *order,
+    Node *nodes, void *leaves, uint32_t *leaf_ids, const uint32_t *parent, uint32_t *flags,
+    vrx_bvh_header *header)
+{
+  const uint32_t n = p.n;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n)
+    return;
+  const uint32_t orig = order[k];
+  leaf_ids[k] = orig;
+  v3 lo, hi;
+  write_leaf<T>(p, orig, k, leaves, lo, hi);
+  if (n == 1) {
+    write_header(header, lo, hi, 1u, kLeafBit, n, T);
+    return;
+  }
+  uint32_t cur = kLeafBit | k;
+  uint32_t pidx = parent[(n - 1) + k];
+  uint32_t height = 1;
+  for (;;) {
+    Node *P = nodes + pidx;
+    const uint32_t c0 = P->d.x;
+    float *f = (float *)P;
+    if (c0 == cur) {
+      f[0] = lo.x; f[1] = lo.y; f[2] = lo.z; f[3] = hi.x; f[4] = hi.y; f[5] = hi.z;
+      ((uint32_t *)f)[14] = height;
+    } else {
+      f[6] = lo.x; f[7] = lo.y; f[8] = lo.z; f[9] = hi.x; f[10] = hi.y; f[11] = hi.z;
+      ((uint32_t *)f)[15] = height;
+    }
+    uint32_t old = __hip_atomic_fetch_add(&flags[pidx], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0u)
+      return; // sibling subtree not finished; its last thread continues
+    // second arrival: both child slots are complete
+    const float4 a = P->a, b = P->b, c = P->c;
+    const uint4 dd = P->d;
+    v3 lo0 = mk(a.x, a.y, a.z), hi0 = mk(a.w, b.x, b.y);
+    v3 lo1 = mk(b.z, b.w, c.x), hi1 = mk(c.y, c.z, c.w);
+    lo = vmin(lo0, lo1);
+    hi = vmax(hi0, hi1);
+    height = 1u + max(dd.z, dd.w);
+    cur = pidx;
+    if (pidx == 0u) {
+      write_header(header, lo, hi, height, 0u, n, T);
+      return;
+    }
+    pidx = parent[pidx];
+  }
+}
+
+__global__ void k_empty_header(vrx_bvh_header *h, int type)
+{
+  write_header(h, mk(INFINITY, INFINITY, INFINITY), mk(-INFINITY, -INFINITY, -INFINITY), 0u,
+      kLeafBit, 0u, type);
+}
+
+__global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint32_t n)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    dst[i] = src[i];
+}
+
+template <int T>
+static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &ws,
+    uint32_t *codes_out, float *cbox_out, hipStream_t s)
+{
+  const uint32_t n = p.n;
+  const unsigned g256 = (n + 255) / 256;
+  hipLaunchKernelGGL(k_cbox_init, dim3(1), dim3(64), 0, s, ws.cbox);
+  unsigned gb = min(g256, 2048u);
+  hipLaunchKernelGGL(k_centroid_bounds<T>, dim3(gb), dim3(256), 0, s, p, ws.cbox);
+  hipLaunchKernelGGL(k_morton<T>, dim3(g256), dim3(256), 0, s, p, ws.cbox, ws.keys_a, ws.vals_a,
+      cbox_out);
+  VRX_LAUNCH_CHECK();
+  const uint32_t nb = sort_blocks(n);
+  uint32_t *ki = ws.keys_a, *vi = ws.vals_a, *ko = ws.keys_b, *vo = ws.vals_b;
+  for (int pass = 0; pass < 4; pass++) {
+    uint32_t shift = 8u * (uint32_t)pass;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ki, n, shift, nb, ws.hist);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, s, ws.hist, 256u * nb);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ki, vi, ko, vo, n,
+        shift, nb, ws.hist);
+    uint32_t *t;
+    t = ki; ki = ko; ko = t;
+    t = vi; vi = vo; vo = t;
+  }
+  VRX_LAUNCH_CHECK();
+  // after 4 passes the sorted data is back in keys_a/vals_a
+  if (codes_out)
+    hipLaunchKernelGGL(k_copy_u32, dim3(g256), dim3(256), 0, s, ki, codes_out, n);
+  Node *nodes = (Node *)out.d_nodes;
+  if (n >= 2) {
+    unsigned gi = (n - 1 + 255) / 256;
+    hipLaunchKernelGGL(k_karras, dim3(gi), dim3(256), 0, s, ki, n, nodes, ws.parent, ws.flags);
+  }
+  hipLaunchKernelGGL(k_leaves_refit<T>, dim3(g256), dim3(256), 0, s, p, vi, nodes, out.d_leaves,
+      out.d_leaf_ids, ws.parent, ws.flags, (vrx_bvh_header *)out.d_header);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
+}
+
+int bvh_build_impl(const vrx_prims *p, const vrx_bvh *out, void *d_ws, size_t ws_bytes,
+    uint32_t *codes_out, float *cbox_out, hipStream_t s)
+{
+  VRX_ARG(p && out, "prims/out");
+  VRX_ARG(p->n < 0x7FFFFFFFu, "too many primitives (max 2^31-1)");
+  VRX_ARG(out->d_header, "out->d_header");
+  if (p->n == 0) {
+    hipLaunchKernelGGL(k_empty_header, dim3(1), dim3(1), 0, s, (vrx_bvh_header *)out->d_header,
+        p->type);
+    VRX_LAUNCH_CHECK();
+    return VRX_OK;
+  }
+  VRX_ARG(p->d_position, "d_position");
+  VRX_ARG(out->d_leaf_ids, "out->d_leaf_ids");
+  VRX_ARG(p->n < 2 || out->d_nodes, "out->d_nodes");
+  VRX_ARG(leaf_record_bytes(p->type) == 0 || out->d_leaves, "out->d_leaves");
+  size_t need = workspace_layout(p->n, nullptr, nullptr);
+  if (!d_ws || ws_bytes < need) {
+    set_error("workspace too small: %zu < %zu", ws_bytes, need);
+    return VRX_ERR_ARG;
+  }
+  Workspace ws;
+  workspace_layout(p->n, d_ws, &ws);
+  switch (p->type) {
+  case VRX_TRIANGLE:
+    return build_typed<VRX_TRIANGLE>(*p, *out, ws, codes_out, cbox_out, s);
+  case VRX_SPHERE:
+    return build_typed<VRX_SPHERE>(*p, *out, ws, codes_out, cbox_out, s);
+  case VRX_CYLINDER:
+    return build_typed<VRX_CYLINDER>(*p, *out, ws, codes_out, cbox_out, s);
+  case VRX_BOX:
+    return build_typed<VRX_BOX>(*p, *out, ws, codes_out, cbox_out, s);
+  default:
+    set_error("unknown primitive type %d", p->type);
+    return VRX_ERR_ARG;
+  }
+}
+
+void bvh_sizes(int32_t type, uint32_t n, vrx_bvh_sizes *o)
+{
+  o->nodes = (size_t)(n > 1 ? n - 1 : 1) * sizeof(Node);
+  o->leaves = (size_t)(n ? n : 1) * (leaf_record_bytes(type) ? leaf_record_bytes(type) : 16);
+  o->leaf_ids = (size_t)(n ? n : 1) * 4;
+  o->header = sizeof(vrx_bvh_header);
+  o->workspace = workspace_layout(n, nullptr, nullptr);
+}
+
+} // namespace vrx
+
+using namespace vrx;
+
+extern "C" {
+
+int vrx_bvh_query_sizes(int32_t type, uint32_t n, vrx_bvh_sizes *out)
+{
+  VRX_ARG(out, "out");
+  VRX_ARG(type >= 0 && type <= VRX_BOX, "type");
+  bvh_sizes(type, n, out);
+  return VRX_OK;
+}
+
+int vrx_bvh_build(const vrx_prims *prims, const vrx_bvh *out, void *d_workspace,
+    size_t workspace_bytes, vrx_stream s)
+{
+  return bvh_build_impl(prims, out, d_workspace, workspace_bytes, nullptr, nullptr, (hipStream_t)s);
+}
+
+int vrx_bvh_build_debug(const vrx_prims *prims, const vrx_bvh *out, void *d_workspace,
+    size_t workspace_bytes, uint32_t *d_codes_out, float *d_cbox_out, vrx_stream s)
+{
+  return bvh_build_impl(
+      prims, out, d_workspace, workspace_bytes, d_codes_out, d_cbox_out, (hipStream_t)s);
+}
+
+} // extern "C"

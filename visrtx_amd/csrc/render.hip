@@ -1,0 +1,655 @@
+// render.hip -- raycast and ambient-occlusion renderers, volume marching,
+// frame clears, camera setup and the instance-level BVH build.
+//
+// Per-pixel programs restated from the reference raygen programs:
+//   renderer/Raycast_ptx.cu:60-179          (raycast: first-hit ids, facing-ratio shading)
+//   renderer/AmbientOcclusion_ptx.cu:86-220 (spp loop, AO any-hit rays, gpu/computeAO.h:38-60)
+//   gpu/volumeIntegration.h:46-165,317-350  (emission-absorption march, all volumes)
+//   gpu/gpu_util.h:323-443                  (tonemap accumulation, channel writes)
+//   gpu/cameraCreateRay.h:38-81             (pinhole rays; raycast uses the pixel corner)
+// Launch shape: one 256-lane workgroup per 16x16 pixel tile, each wave an
+// 8x8 tile (coherent rays share node fetches); the traversal short stack
+// lives in dynamic LDS, laid out [slot][lane] so a wave's pushes hit 64
+// distinct banks.
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "vrx_internal.h"
+#include "vrx_trace.h"
+
+namespace vrx {
+
+constexpr int kTile = 16;
+constexpr int kBlock = kTile * kTile;
+// Hard bounds that keep every loop finite on degenerate input (a zero
+// volume step, a surface hit at the ray origin with zero epsilon, volumes
+// whose exit distance no longer grows by 1e-3 in fp32).
+constexpr int kMaxSurfaceLayers = 4096;
+constexpr uint32_t kMaxSamplesPerSegment = 1u << 24;
+constexpr int kMaxVolumeSegments = 256;
+
+// --------------------------------------------------------------------------
+// camera
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+__device__ __forceinline__ void cam_ray(const vrx_camera &c, float sx, float sy, v3 &o, v3 &d)
+{
+  sx = mixf(c.region[0], c.region[2], sx);
+  sy = mixf(c.region[1], c.region[3], sy);
+  o = ld3(c.pos);
+  d = normalize(add(add(ld3(c.d00), fmul(sx, ld3(c.du))), fmul(sy, ld3(c.dv))));
+}
+
+// --------------------------------------------------------------------------
+// volumes (fp32)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ bool vol_slab(
+    const vrx_volume &v, v3 o, v3 d, float tmin, float tmax, float &t0, float &t1)
+{
+  float lo[3] = {v.origin[0], v.origin[1], v.origin[2]};
+  float hi[3];
+  float tn = -INFINITY, tf = INFINITY;
+  float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    hi[k] = v.origin[k] + ((float)v.dims[k] - 1.0f) * v.spacing[k];
+    float inv = 1.0f / dd[k];
+    float a = (lo[k] - oo[k]) * inv, b = (hi[k] - oo[k]) * inv;
+    tn = fmaxf(tn, fminf(a, b));
+    tf = fminf(tf, fmaxf(a, b));
+  }
+  if (!(tn < tf))
+    return false;
+  if (tf < tmin || tn > tmax)
+    return false;
+  t0 = fminf(fmaxf(tn, tmin), tmax);
+  t1 = fminf(fmaxf(tf, tmin), tmax);
+  return true;
+}
+
+// software trilinear at voxel coordinates (p - origin) / spacing, edge clamp
+// (the structured-regular field's clamp-to-edge linear texture,
+// StructuredRegularField.cpp:98-159, sampleSpatialField.h:54-78)
+__device__ __forceinline__ float sample_field(const vrx_volume &v, v3 p)
+{
+  float g[3] = {(p.x - v.origin[0]) / v.spacing[0], (p.y - v.origin[1]) / v.spacing[1],
+      (p.z - v.origin[2]) / v.spacing[2]};
+  int i0[3], i1[3];
+  float f[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float gm = (float)v.dims[k] - 1.0f;
+    float x = fminf(fmaxf(g[k], 0.0f), gm);
+    int i = (int)x;
+    int imax = (int)v.dims[k] - 2;
+    if (imax < 0)
+      imax = 0;
+    i = min(i, imax);
+    i0[k] = i;
+    i1[k] = min(i + 1, (int)v.dims[k] - 1);
+    f[k] = x - (float)i;
+  }
+  const size_t sx = 1, sy = v.dims[0], sz = (size_t)v.dims[0] * v.dims[1];
+  const float *D = v.d_data;
+  size_t bz0 = (size_t)i0[2] * sz, bz1 = (size_t)i1[2] * sz;
+  size_t by0 = (size_t)i0[1] * sy, by1 = (size_t)i1[1] * sy;
+  float c000 = D[bz0 + by0 + i0[0] * sx], c100 = D[bz0 + by0 + i1[0] * sx];
+  float c010 = D[bz0 + by1 + i0[0] * sx], c110 = D[bz0 + by1 + i1[0] * sx];
+  float c001 = D[bz1 + by0 + i0[0] * sx], c101 = D[bz1 + by0 + i1[0] * sx];
+  float c011 = D[bz1 + by1 + i0[0] * sx], c111 = D[bz1 + by1 + i1[0] * sx];
+  float c00 = c000 + (c100 - c000) * f[0];
+  float c10 = c010 + (c110 - c010) * f[0];
+  float c01 = c001 + (c101 - c001) * f[0];
+  float c11 = c011 + (c111 - c011) * f[0];
+  float c0 = c00 + (c10 - c00) * f[1];
+  float c1 = c01 + (c11 - c01) * f[1];
+  return c0 + (c1 - c0) * f[2];
+}
+
+// classifySample (volumeIntegration.h:46-62): LUT texel coordinate u*256-0.5,
+// linear, clamped (TransferFunction1D.cpp:152-185).
+__device__ __forceinline__ float4 classify(const float4 *lut, float lo, float hi, float s)
+{
+  float c = fminf(fmaxf(s, lo), hi);
+  float u = (c - lo) * (1.0f / (hi - lo));
+  float x = fminf(fmaxf(u * 256.0f - 0.5f, 0.0f), 255.0f);
+  int i = min((int)x, 254);
+  float f = x - (float)i;
+  float4 a = lut[i], b = lut[i + 1];
+  return make_float4(a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, a.z + (b.z - a.z) * f,
+      a.w + (b.w - a.w) * f);
+}
+
+__device__ __forceinline__ float vol_step(const vrx_volume &v)
+{
+  return fminf(fminf(v.spacing[0] / 2.0f, v.spacing[1] / 2.0f), v.spacing[2] / 2.0f);
+}
+
+// _rayMarchVolume (volumeIntegration.h:64-103), jitter off.  Sample k is at
+// t0 + k*step (not an accumulated sum), which keeps positions within 1 ulp.
+__device__ void march(const vrx_volume &v, v3 o, v3 d, float t0, float t1, float inv_rate,
+    v3 &color, float &opacity, uint32_t &ns)
+{
+  const float step = vol_step(v) * inv_rate;
+  const float expo = step * (1.0f / v.unit_distance);
+  const float4 *lut = (const float4 *)v.d_lut;
+  const float lo = v.value_range[0], hi = v.value_range[1];
+  float T = 1.0f;
+  if (!(step > 0.0f))
+    return;
+  for (uint32_t k = 0; k < kMaxSamplesPerSegment; k++) {
+    float t = t0 + (float)k * step;
+    if (!(t <= t1) || !(opacity < 0.99f))
+      break;
+    v3 p = mk(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+    float s = sample_field(v, p);
+    ns++;
+    if (s != s)
+      continue;
+    float4 co = classify(lut, lo, hi, s);
+    float st = exp2f(expo * log2f(1.0f - co.w));
+    float w = T * (1.0f - st);
+    color.x += w * co.x;
+    color.y += w * co.y;
+    color.z += w * co.z;
+    opacity += w;
+    T *= st;
+  }
+}
+
+// rayMarchAllVolumes (volumeIntegration.h:317-350)
+__device__ float march_all(const vrx_scene &sc, v3 o, v3 d, float tlower, float tfar,
+    float inv_rate, v3 &color, float &opacity, uint32_t &obj, uint32_t &inst, uint32_t &ns)
+{
+  float depth = tfar;
+  bool first = true;
+  int last = -1;
+  float tl = tlower;
+  for (int seg = 0; seg < kMaxVolumeSegments; seg++) {
+    int best = -1;
+    float bt0 = 0.0f, bt1 = 0.0f;
+    for (uint32_t i = 0; i < sc.num_volumes; i++) {
+      if ((int)i == last)
+        continue;
+      float t0, t1;
+      if (vol_slab(sc.d_volumes[i], o, d, tl, tfar, t0, t1)) {
+        if (best < 0 || t0 < bt0) {
+          best = (int)i;
+          bt0 = t0;
+          bt1 = t1;
+        }
+      }
+    }
+    if (best < 0)
+      break;
+    const vrx_volume &V = sc.d_volumes[best];
+    if (first) {
+      obj = V.id;
+      inst = V.inst_id;
+      first = false;
+    }
+    depth = fminf(depth, bt0);
+    march(V, o, d, bt0, bt1, inv_rate, color, opacity, ns);
+    tl = bt1 + 1e-3f;
+    last = best;
+    if (!(opacity < 0.99f))
+      break;
+  }
+  return depth;
+}
+
+// --------------------------------------------------------------------------
+// channel packing (gpu_util.h:323-389, glm packUnorm4x8 / convertLinearToSRGB)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+__device__ __forceinline__ uint32_t pack_unorm4x8(float x, float y, float z, float w)
+{
+  uint32_t r = (uint32_t)(unsigned char)roundf(clamp01(x) * 255.0f);
+  r |= (uint32_t)(unsigned char)roundf(clamp01(y) * 255.0f) << 8;
+  r |= (uint32_t)(unsigned char)roundf(clamp01(z) * 255.0f) << 16;
+  r |= (uint32_t)(unsigned char)roundf(clamp01(w) * 255.0f) << 24;
+  return r;
+}
+
+__device__ __forceinline__ float to_srgb(float v)
+{
+  float c = clamp01(v);
+  return c < 0.0031308f ? c * 12.92f : powf(c, 0.41666f) * 1.055f - 0.055f;
+}
+
+__device__ __forceinline__ void write_color(const vrx_frame &f, size_t idx, float4 c)
+{
+  if (f.d_color_f32)
+    ((float4 *)f.d_color_f32)[idx] = c;
+  if (f.d_color_u32) {
+    if (f.color_format == VRX_COLOR_SRGB8)
+      f.d_color_u32[idx] = pack_unorm4x8(to_srgb(c.x), to_srgb(c.y), to_srgb(c.z), c.w);
+    else
+      f.d_color_u32[idx] = pack_unorm4x8(c.x, c.y, c.z, c.w);
+  }
+}
+
+__device__ __forceinline__ void wave_count_add(uint64_t *ctr, uint32_t v)
+{
+  // one atomic per wave
+  uint32_t s = v;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0 && s)
+    atomicAdd((unsigned long long *)ctr, (unsigned long long)s);
+}
+
+// --------------------------------------------------------------------------
+// the per-pixel program
+// --------------------------------------------------------------------------
+template <int PT, bool TWO, int KIND>
+__global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam, vrx_renderer r,
+    vrx_frame f, uint32_t sd, uint32_t ovf_cap, uint32_t *scratch)
+{
+  extern __shared__ uint32_t s_stack[];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const uint32_t px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
+  const uint32_t py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+  const bool inside = px < f.width && py < f.height;
+
+  Stack st;
+  st.lds = s_stack + tid;
+  st.stride = kBlock;
+  st.sd = sd;
+  st.sd_mask = sd - 1;
+  const uint32_t gthreads = gridDim.x * gridDim.y * kBlock;
+  const uint32_t gtid = (blockIdx.y * gridDim.x + blockIdx.x) * kBlock + tid;
+  st.ovf = scratch + 64 + gtid;
+  st.ovf_stride = gthreads;
+  st.ovf_cap = ovf_cap;
+  st.err = scratch;
+  st.cnt = 0;
+
+  uint32_t rays = 0, vsamples = 0;
+  if (inside) {
+    const size_t idx = (size_t)px + (size_t)py * f.width;
+    const float iw = 1.0f / (float)f.width, ih = 1.0f / (float)f.height;
+    const int spp = KIND == VRX_RENDER_AO ? max(r.pixel_samples, 1) : 1;
+    const int aos = min(max(r.ambient_samples, 0), 256);
+    const bool cull = r.cull_backfaces != 0;
+    Pcg rs;
+    if (KIND == VRX_RENDER_AO)
+      pcg_seed(rs, (uint64_t)idx, (uint64_t)f.frame_id);
+
+    float4 accum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float dcur = FLT_MAX;
+    uint32_t pid = 0, oid = 0, iid = 0;
+    bool ids_changed = f.frame_id == 0;
+    if (f.frame_id != 0) {
+      accum = ((const float4 *)f.d_accum)[idx];
+      if (f.d_depth)
+        dcur = f.d_depth[idx];
+    }
+    for (int i = 0; i < spp; i++) {
+      v3 o, d;
+      if (KIND == VRX_RENDER_AO) {
+        float r0 = pcg_float(rs), r1 = pcg_float(rs);
+        (void)pcg_float(rs);
+        (void)pcg_float(rs);
+        cam_ray(cam, ((float)px + r0) * iw, ((float)py + r1) * ih, o, d);
+      } else {
+        cam_ray(cam, (float)px * iw, (float)py * ih, o, d);
+      }
+      float tlower = 0.0f;
+      const float tmax = FLT_MAX;
+      v3 outc = mk(0.0f, 0.0f, 0.0f);
+      float outo = 0.0f, depth = 1e30f;
+      uint32_t prim = ~0u, obj = ~0u, inst = ~0u;
+      bool first = true;
+      for (int layer = 0; layer < kMaxSurfaceLayers && outo < 0.99f; layer++) {
+        Hit h;
+        rays++;
+        bool found = TWO ? trace<PT, true, false>(sc, o, d, tlower, tmax, cull, st, h)
+                         : trace<PT, false, false>(sc, o, d, tlower, tmax, cull, st, h);
+        v3 color = mk(0.0f, 0.0f, 0.0f);
+        float opacity = 0.0f;
+        if (found) {
+          SurfHit s;
+          populate(sc, h, o, d, s);
+          uint32_t vo = ~0u, vi = ~0u;
+          float vdepth = s.t;
+          if (sc.num_volumes)
+            vdepth = march_all(sc, o, d, tlower, s.t, r.inv_volume_sampling_rate, color, opacity,
+                vo, vi, vsamples);
+          if (first) {
+            if (vdepth < s.t) {
+              depth = vdepth;
+              prim = 0;
+              obj = vo;
+              inst = vi;
+            } else {
+              depth = s.t;
+              prim = s.prim_out;
+              obj = s.obj;
+              inst = s.inst;
+            }
+            first = false;
+          }
+          v3 lighting;
+          if (KIND == VRX_RENDER_AO) {
+            float ao = 1.0f;
+            if (aos > 0) {
+              float weights = 0.0f, hits = 0.0f;
+              for (int k = 0; k < aos; k++) {
+                v3 ao_o = add(s.hp, fmul(s.eps, s.ng));
+                float q0 = pcg_float(rs), q1 = pcg_float(rs), q2 = pcg_float(rs);
+                (void)pcg_float(rs);
+                v3 dir = normalize(mk(2.0f * q0 - 1.0f, 2.0f * q1 - 1.0f, 2.0f * q2 - 1.0f));
+                if (!(dot(dir, s.ns) > 0.0f))
+                  dir = neg(dir);
+                float wgt = fmaxf(0.0f, dot(dir, s.ns));
+                weights += wgt;
+                if (wgt != 0.0f) {
+                  Hit hh;
+                  rays++;
+                  bool occ = TWO ? trace<PT, true, true>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh)
+                                 : trace<PT, false, true>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh);
+                  hits += wgt * (occ ? 1.0f : 0.0f);
+                }
+              }
+              ao = weights > 0.0f ? 1.0f - hits / weights : 0.0f;
+            }
+            float scl = ao * r.ambient_radiance;
+            lighting = fmul(scl, ld3(r.ambient_color));
+          } else {
+            float scl = fabsf(dot(d, s.ns));
+            lighting = fmul(scl, ld3(r.ambient_color));
+          }
+          v3 bl = mulv(s.color, lighting);
+          float w = 1.0f - opacity;
+          color = mk(color.x + bl.x * w, color.y + bl.y * w, color.z + bl.z * w);
+          opacity = opacity + s.opacity * (1.0f - opacity);
+          color = mulf(color, opacity);
+          float wo = 1.0f - outo;
+          outc = mk(outc.x + color.x * wo, outc.y + color.y * wo, outc.z + color.z * wo);
+          outo = outo + opacity * (1.0f - outo);
+          tlower = s.t + s.eps;
+        } else {
+          uint32_t vo = ~0u, vi = ~0u;
+          float vdepth = tmax;
+          if (sc.num_volumes)
+            vdepth = march_all(sc, o, d, tlower, tmax, r.inv_volume_sampling_rate, color, opacity,
+                vo, vi, vsamples);
+          if (first) {
+            depth = fminf(depth, vdepth);
+            prim = 0;
+            obj = vo;
+            inst = vi;
+          }
+          color = mulf(color, opacity);
+          float w = 1.0f - opacity;
+          color = mk(color.x + r.background[0] * w, color.y + r.background[1] * w,
+              color.z + r.background[2] * w);
+          opacity = opacity + r.background[3] * (1.0f - opacity);
+          float wo = 1.0f - outo;
+          outc = mk(outc.x + color.x * wo, outc.y + color.y * wo, outc.z + color.z * wo);
+          outo = outo + opacity * (1.0f - outo);
+          break;
+        }
+      }
+      // accumResults (gpu_util.h:393-443)
+      float sc_ = 1.0f + fmaxf(0.0f, cmax3(outc));
+      accum.x = accum.x + outc.x / sc_;
+      accum.y = accum.y + outc.y / sc_;
+      accum.z = accum.z + outc.z / sc_;
+      accum.w = accum.w + outo;
+      if (depth < dcur) {
+        dcur = depth;
+        pid = prim;
+        oid = obj;
+        iid = inst;
+        ids_changed = true;
+      }
+    }
+    ((float4 *)f.d_accum)[idx] = accum;
+    float div = (float)(f.frame_id + (uint32_t)spp);
+    float4 avg = make_float4(accum.x / div, accum.y / div, accum.z / div, accum.w / div);
+    float s2 = fmaxf(1e-12f, 1.0f - cmax3(mk(avg.x, avg.y, avg.z)));
+    float4 outv = make_float4(avg.x / s2, avg.y / s2, avg.z / s2, avg.w);
+    write_color(f, idx, outv);
+    if (f.d_depth)
+      f.d_depth[idx] = dcur;
+    if (ids_changed) {
+      if (f.d_prim_id)
+        f.d_prim_id[idx] = pid;
+      if (f.d_obj_id)
+        f.d_obj_id[idx] = oid;
+      if (f.d_inst_id)
+        f.d_inst_id[idx] = iid;
+    }
+  }
+  if (f.d_counters) {
+    wave_count_add(f.d_counters, rays);
+    wave_count_add(f.d_counters + 1, vsamples);
+  }
+}
+
+__global__ void k_clear_frame(vrx_frame f)
+{
+  size_t n = (size_t)f.width * f.height;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    ((float4 *)f.d_accum)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (f.d_depth)
+      f.d_depth[i] = FLT_MAX;
+    if (f.d_prim_id)
+      f.d_prim_id[i] = 0u;
+    if (f.d_obj_id)
+      f.d_obj_id[i] = 0u;
+    if (f.d_inst_id)
+      f.d_inst_id[i] = 0u;
+  }
+}
+
+// --------------------------------------------------------------------------
+// instance level: world boxes of entries
+// --------------------------------------------------------------------------
+__global__ void k_entry_boxes(vrx_scene sc, float *boxes)
+{
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= sc.num_entries)
+    return;
+  const vrx_entry &E = sc.d_entries[e];
+  const vrx_geom &G = sc.d_geoms[E.geom];
+  const vrx_bvh_header *H = (const vrx_bvh_header *)G.bvh.d_header;
+  v3 lo = mk(H->lo[0], H->lo[1], H->lo[2]), hi = mk(H->hi[0], H->hi[1], H->hi[2]);
+  v3 wlo = lo, whi = hi;
+  if (!E.identity && G.n > 0) {
+    wlo = mk(INFINITY, INFINITY, INFINITY);
+    whi = mk(-INFINITY, -INFINITY, -INFINITY);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      v3 p = mk((c & 1) ? hi.x : lo.x, (c & 2) ? hi.y : lo.y, (c & 4) ? hi.z : lo.z);
+      v3 q = xfm_point(E.xfm, p);
+      wlo = vmin(wlo, q);
+      whi = vmax(whi, q);
+    }
+  }
+  float *b = boxes + 6 * (size_t)e;
+  b[0] = wlo.x; b[1] = wlo.y; b[2] = wlo.z;
+  b[3] = whi.x; b[4] = whi.y; b[5] = whi.z;
+}
+
+// --------------------------------------------------------------------------
+// launch helpers
+// --------------------------------------------------------------------------
+static uint32_t pick_sd(const vrx_renderer *r)
+{
+  int s = r && r->stack_lds > 0 ? r->stack_lds : 32;
+  uint32_t p = 8;
+  while (p < (uint32_t)s && p < 64)
+    p <<= 1;
+  return p;
+}
+
+static uint32_t max_stack(const vrx_scene *sc)
+{
+  uint32_t m = sc->max_blas_height + 2;
+  if (sc->num_entries > 1)
+    m += sc->tlas_height + 1;
+  return m;
+}
+
+static uint32_t ovf_cap_of(const vrx_scene *sc, uint32_t sd)
+{
+  uint32_t m = max_stack(sc);
+  return m > sd ? m - sd : 0u;
+}
+
+template <int PT, bool TWO>
+static int launch_kind(const vrx_scene &sc, const vrx_camera &cam, const vrx_renderer &r,
+    const vrx_frame &f, uint32_t sd, uint32_t cap, hipStream_t s)
+{
+  dim3 grid((f.width + kTile - 1) / kTile, (f.height + kTile - 1) / kTile);
+  size_t lds = (size_t)sd * kBlock * 4;
+  if (r.kind == VRX_RENDER_AO)
+    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_AO>), grid, dim3(kBlock), lds, s, sc, cam, r,
+        f, sd, cap, (uint32_t *)f.d_scratch);
+  else
+    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_RAYCAST>), grid, dim3(kBlock), lds, s, sc,
+        cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
+}
+
+template <int PT>
+static int launch_pt(const vrx_scene &sc, const vrx_camera &cam, const vrx_renderer &r,
+    const vrx_frame &f, uint32_t sd, uint32_t cap, hipStream_t s)
+{
+  if (sc.num_entries > 1)
+    return launch_kind<PT, true>(sc, cam, r, f, sd, cap, s);
+  return launch_kind<PT, false>(sc, cam, r, f, sd, cap, s);
+}
+
+} // namespace vrx
+
+using namespace vrx;
+
+extern "C" {
+
+size_t vrx_render_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r, uint32_t width,
+    uint32_t height)
+{
+  if (!scene)
+    return 0;
+  uint32_t sd = pick_sd(r);
+  uint32_t cap = ovf_cap_of(scene, sd);
+  size_t tiles = (size_t)((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
+  return 256 + tiles * kBlock * (size_t)cap * 4;
+}
+
+int vrx_frame_clear(const vrx_frame *f, vrx_stream s)
+{
+  VRX_ARG(f && f->d_accum, "frame/accum");
+  size_t n = (size_t)f->width * f->height;
+  unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  if (blocks == 0)
+    return VRX_OK;
+  hipLaunchKernelGGL(k_clear_frame, dim3(blocks), dim3(256), 0, (hipStream_t)s, *f);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
+}
+
+int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer *r,
+    const vrx_frame *f, vrx_stream s)
+{
+  VRX_ARG(scene && cam && r && f, "null argument");
+  VRX_ARG(f->width > 0 && f->height > 0, "frame size");
+  VRX_ARG(f->d_accum, "frame accumulation buffer");
+  VRX_ARG(scene->num_entries == 0 || (scene->d_entries && scene->d_geoms), "scene arrays");
+  VRX_ARG(scene->num_entries <= 1 || scene->tlas.d_leaf_ids, "instance BVH not built");
+  VRX_ARG(scene->num_volumes == 0 || scene->d_volumes, "volume array");
+  uint32_t sd = pick_sd(r);
+  uint32_t cap = ovf_cap_of(scene, sd);
+  size_t need = vrx_render_scratch_bytes(scene, r, f->width, f->height);
+  if (!f->d_scratch || f->scratch_bytes < need) {
+    set_error("render scratch too small: %zu < %zu", f->scratch_bytes, need);
+    return VRX_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)s;
+  switch (scene->single_type) {
+  case VRX_TRIANGLE:
+    return launch_pt<VRX_TRIANGLE>(*scene, *cam, *r, *f, sd, cap, st);
+  case VRX_SPHERE:
+    return launch_pt<VRX_SPHERE>(*scene, *cam, *r, *f, sd, cap, st);
+  case VRX_CYLINDER:
+    return launch_pt<VRX_CYLINDER>(*scene, *cam, *r, *f, sd, cap, st);
+  default:
+    return launch_pt<-1>(*scene, *cam, *r, *f, sd, cap, st);
+  }
+}
+
+int vrx_camera_perspective(const float pos[3], const float dir[3], const float up[3], float fovy,
+    float aspect, const float region[4], vrx_camera *out)
+{
+  VRX_ARG(pos && dir && up && out, "null argument");
+  v3 d = normalize(ld3(dir));
+  v3 u = normalize(ld3(up));
+  float h = 2.0f * tanf(0.5f * fovy);
+  float w = h * aspect;
+  v3 du = mulf(normalize(cross(d, u)), w);
+  v3 dv = mulf(normalize(cross(du, d)), h);
+  v3 d00 = sub(sub(d, fmul(0.5f, du)), fmul(0.5f, dv));
+  memcpy(out->pos, pos, 12);
+  out->d00[0] = d00.x; out->d00[1] = d00.y; out->d00[2] = d00.z;
+  out->du[0] = du.x; out->du[1] = du.y; out->du[2] = du.z;
+  out->dv[0] = dv.x; out->dv[1] = dv.y; out->dv[2] = dv.z;
+  if (region)
+    memcpy(out->region, region, 16);
+  else {
+    out->region[0] = 0.0f; out->region[1] = 0.0f; out->region[2] = 1.0f; out->region[3] = 1.0f;
+  }
+  return VRX_OK;
+}
+
+int vrx_tlas_query_sizes(uint32_t num_entries, vrx_bvh_sizes *out)
+{
+  VRX_ARG(out, "out");
+  bvh_sizes(VRX_BOX, num_entries, out);
+  out->workspace += ((size_t)num_entries * 24 + 255) & ~size_t(255);
+  return VRX_OK;
+}
+
+int vrx_tlas_build(const vrx_scene *scene, const vrx_bvh *out, void *d_workspace,
+    size_t workspace_bytes, vrx_stream s)
+{
+  VRX_ARG(scene && out, "null argument");
+  vrx_bvh_sizes sz;
+  vrx_tlas_query_sizes(scene->num_entries, &sz);
+  if (!d_workspace || workspace_bytes < sz.workspace) {
+    set_error("tlas workspace too small");
+    return VRX_ERR_ARG;
+  }
+  size_t box_bytes = ((size_t)scene->num_entries * 24 + 255) & ~size_t(255);
+  float *boxes = (float *)d_workspace;
+  hipStream_t st = (hipStream_t)s;
+  if (scene->num_entries) {
+    hipLaunchKernelGGL(k_entry_boxes, dim3((scene->num_entries + 255) / 256), dim3(256), 0, st,
+        *scene, boxes);
+    VRX_LAUNCH_CHECK();
+  }
+  vrx_prims p;
+  memset(&p, 0, sizeof(p));
+  p.type = VRX_BOX;
+  p.n = scene->num_entries;
+  p.d_position = boxes;
+  return bvh_build_impl(&p, out, (char *)d_workspace + box_bytes, workspace_bytes - box_bytes,
+      nullptr, nullptr, st);
+}
+
+} // extern "C"

@@ -1,0 +1,235 @@
+// runtime.hip -- device, memory, stream and event plumbing behind the C ABI.
+//
+// Replaces the CUDA runtime use of the reference device: the single stream of
+// DeviceGlobalState (optix_visrtx.h:165), the frame's start/end events and
+// completion callback (frame/Frame.cu:49-53,255,295-307), and the thrust
+// fills used for frame clears (frame/Frame.cu:590-660) -- the latter as a
+// plain fill kernel so that it can be recorded into a HIP graph.
+#include <stdarg.h>
+#include <string.h>
+
+#include "vrx_internal.h"
+
+namespace vrx {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+  set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  return e == hipErrorOutOfMemory ? VRX_ERR_NOMEM : VRX_ERR_HIP;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
+{
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride)
+    p[i] = v;
+}
+
+} // namespace vrx
+
+using namespace vrx;
+
+extern "C" {
+
+const char *vrx_last_error(void) { return g_err; }
+
+int vrx_version(void) { return 100; }
+
+int vrx_device_count(int *count)
+{
+  VRX_ARG(count, "count");
+  VRX_HIP(hipGetDeviceCount(count));
+  return VRX_OK;
+}
+
+int vrx_set_device(int device)
+{
+  VRX_HIP(hipSetDevice(device));
+  return VRX_OK;
+}
+
+int vrx_get_device(int *device)
+{
+  VRX_ARG(device, "device");
+  VRX_HIP(hipGetDevice(device));
+  return VRX_OK;
+}
+
+int vrx_device_info(int device, char *name, size_t name_len, int *num_cus, size_t *hbm_bytes)
+{
+  hipDeviceProp_t p;
+  VRX_HIP(hipGetDeviceProperties(&p, device));
+  if (name && name_len) {
+    snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (num_cus)
+    *num_cus = p.multiProcessorCount;
+  if (hbm_bytes)
+    *hbm_bytes = p.totalGlobalMem;
+  return VRX_OK;
+}
+
+int vrx_malloc(void **d_ptr, size_t bytes)
+{
+  VRX_ARG(d_ptr, "d_ptr");
+  *d_ptr = nullptr;
+  if (bytes == 0)
+    return VRX_OK;
+  VRX_HIP(hipMalloc(d_ptr, bytes));
+  return VRX_OK;
+}
+
+int vrx_free(void *d_ptr)
+{
+  if (d_ptr)
+    VRX_HIP(hipFree(d_ptr));
+  return VRX_OK;
+}
+
+int vrx_host_malloc(void **h_ptr, size_t bytes)
+{
+  VRX_ARG(h_ptr, "h_ptr");
+  *h_ptr = nullptr;
+  if (bytes == 0)
+    return VRX_OK;
+  VRX_HIP(hipHostMalloc(h_ptr, bytes, hipHostMallocDefault));
+  return VRX_OK;
+}
+
+int vrx_host_free(void *h_ptr)
+{
+  if (h_ptr)
+    VRX_HIP(hipHostFree(h_ptr));
+  return VRX_OK;
+}
+
+static hipMemcpyKind kind_of(int kind)
+{
+  return kind == VRX_H2D ? hipMemcpyHostToDevice
+                         : (kind == VRX_D2H ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+}
+
+int vrx_memcpy(void *dst, const void *src, size_t bytes, int kind, vrx_stream s)
+{
+  if (bytes == 0)
+    return VRX_OK;
+  VRX_ARG(dst && src, "dst/src");
+  VRX_HIP(hipMemcpyAsync(dst, src, bytes, kind_of(kind), (hipStream_t)s));
+  return VRX_OK;
+}
+
+int vrx_memcpy_sync(void *dst, const void *src, size_t bytes, int kind)
+{
+  if (bytes == 0)
+    return VRX_OK;
+  VRX_ARG(dst && src, "dst/src");
+  VRX_HIP(hipMemcpy(dst, src, bytes, kind_of(kind)));
+  return VRX_OK;
+}
+
+int vrx_fill_u32(void *d_ptr, uint32_t value, size_t count, vrx_stream s)
+{
+  if (count == 0)
+    return VRX_OK;
+  VRX_ARG(d_ptr, "d_ptr");
+  size_t blocks = (count + 255) / 256;
+  if (blocks > 65536)
+    blocks = 65536;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s,
+      (uint32_t *)d_ptr, value, count);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
+}
+
+int vrx_stream_create(vrx_stream *s)
+{
+  VRX_ARG(s, "s");
+  hipStream_t st;
+  VRX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  *s = (vrx_stream)st;
+  return VRX_OK;
+}
+
+int vrx_stream_destroy(vrx_stream s)
+{
+  if (s)
+    VRX_HIP(hipStreamDestroy((hipStream_t)s));
+  return VRX_OK;
+}
+
+int vrx_stream_sync(vrx_stream s)
+{
+  VRX_HIP(hipStreamSynchronize((hipStream_t)s));
+  return VRX_OK;
+}
+
+int vrx_device_sync(void)
+{
+  VRX_HIP(hipDeviceSynchronize());
+  return VRX_OK;
+}
+
+int vrx_event_create(vrx_event *e)
+{
+  VRX_ARG(e, "e");
+  hipEvent_t ev;
+  VRX_HIP(hipEventCreate(&ev));
+  *e = (vrx_event)ev;
+  return VRX_OK;
+}
+
+int vrx_event_destroy(vrx_event e)
+{
+  if (e)
+    VRX_HIP(hipEventDestroy((hipEvent_t)e));
+  return VRX_OK;
+}
+
+int vrx_event_record(vrx_event e, vrx_stream s)
+{
+  VRX_HIP(hipEventRecord((hipEvent_t)e, (hipStream_t)s));
+  return VRX_OK;
+}
+
+int vrx_event_sync(vrx_event e)
+{
+  VRX_HIP(hipEventSynchronize((hipEvent_t)e));
+  return VRX_OK;
+}
+
+int vrx_event_query(vrx_event e)
+{
+  hipError_t r = hipEventQuery((hipEvent_t)e);
+  if (r == hipSuccess)
+    return 1;
+  if (r == hipErrorNotReady)
+    return 0;
+  return hip_fail(r, "hipEventQuery", __FILE__, __LINE__);
+}
+
+int vrx_event_elapsed_ms(vrx_event start, vrx_event end, float *ms)
+{
+  VRX_ARG(ms, "ms");
+  VRX_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+  return VRX_OK;
+}
+
+int vrx_launch_host_func(vrx_stream s, void (*fn)(void *), void *user)
+{
+  VRX_ARG(fn, "fn");
+  VRX_HIP(hipLaunchHostFunc((hipStream_t)s, fn, user));
+  return VRX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,543 @@
+// vrx_trace.h -- device-side ray traversal, intersection and hit population.
+//
+// Replaces what the reference delegates to OptiX and its programs:
+//   - optixTrace + RT-core traversal (gpu/intersectRay.h:39-111) -> stackful
+//     BVH2 traversal, near child first, short stack in LDS with spill to HBM
+//   - OptiX builtin triangle test -> watertight test (Woop, Benthin, Wald,
+//     JCGT 2013)
+//   - custom intersectors scene/Intersectors_ptx.cu:74-190 (sphere, cylinder)
+//   - closest-hit population gpu/populateHit.h:150-368 and
+//     computeGeometryPrimId gpu/gpu_util.h:309-315
+//   - back-face culling gpu/populateHit.h:320-332 (+ the renderer's
+//     cullTriangleBackfaces ray flag, gpu/intersectRay.h:79-83)
+// Closest hit = lexicographic minimum of (t, entry index, primitive index)
+// over every reported intersection in [tmin, tmax]: independent of the
+// traversal order, so it is the same hit the CPU brute force finds.
+#pragma once
+
+#include "vrx_lbvh.h"
+#include "vrx_math.h"
+
+namespace vrx {
+
+constexpr uint32_t kMarker = 0xFFFFFFFFu; // "leave instance" stack marker
+
+// ---- short stack: top `sd` entries in LDS, older ones spilled to HBM ------
+struct Stack {
+  uint32_t *lds;     // this lane's column: lds[slot * stride]
+  uint32_t stride;
+  uint32_t sd_mask;  // sd - 1 (sd a power of two)
+  uint32_t sd;
+  uint32_t *ovf;     // this thread's spill column: ovf[pos * ovf_stride]
+  uint32_t ovf_stride;
+  uint32_t ovf_cap;  // spill entries available
+  uint32_t cnt;
+  uint32_t *err;     // set to 1 if the spill area would overflow
+
+  __device__ __forceinline__ void push(uint32_t x)
+  {
+    uint32_t slot = cnt & sd_mask;
+    if (cnt >= sd) {
+      uint32_t pos = cnt - sd;
+      if (pos < ovf_cap)
+        ovf[(size_t)pos * ovf_stride] = lds[slot * stride];
+      else
+        *err = 1u;
+    }
+    lds[slot * stride] = x;
+    cnt++;
+  }
+  __device__ __forceinline__ uint32_t pop()
+  {
+    cnt--;
+    uint32_t slot = cnt & sd_mask;
+    uint32_t x = lds[slot * stride];
+    if (cnt >= sd) {
+      uint32_t pos = cnt - sd;
+      lds[slot * stride] = pos < ovf_cap ? ovf[(size_t)pos * ovf_stride] : kMarker;
+    }
+    return x;
+  }
+};
+
+struct WT {
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ WT wt_setup(v3 d)
+{
+  WT w;
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  int kz = 0;
+  if (ay > ax)
+    kz = 1;
+  if (az > (kz == 0 ? ax : ay))
+    kz = 2;
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  if (comp(d, kz) < 0.0f) {
+    int tmp = kx;
+    kx = ky;
+    ky = tmp;
+  }
+  w.kx = kx;
+  w.ky = ky;
+  w.kz = kz;
+  float dz = comp(d, kz);
+  w.Sx = comp(d, kx) / dz;
+  w.Sy = comp(d, ky) / dz;
+  w.Sz = 1.0f / dz;
+  return w;
+}
+
+__device__ __forceinline__ bool isect_tri(
+    const WT &w, v3 o, v3 v0, v3 v1, v3 v2, float &t, float &bu, float &bv)
+{
+  v3 A = sub(v0, o), B = sub(v1, o), C = sub(v2, o);
+  float Akz = comp(A, w.kz), Bkz = comp(B, w.kz), Ckz = comp(C, w.kz);
+  float Ax = comp(A, w.kx) - w.Sx * Akz;
+  float Ay = comp(A, w.ky) - w.Sy * Akz;
+  float Bx = comp(B, w.kx) - w.Sx * Bkz;
+  float By = comp(B, w.ky) - w.Sy * Bkz;
+  float Cx = comp(C, w.kx) - w.Sx * Ckz;
+  float Cy = comp(C, w.ky) - w.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+    double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+    double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+    U = (float)(CxBy - CyBx);
+    V = (float)(AxCy - AyCx);
+    W = (float)(BxAy - ByAx);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+    return false;
+  float det = (U + V) + W;
+  if (det == 0.0f)
+    return false;
+  float Az = w.Sz * Akz, Bz = w.Sz * Bkz, Cz = w.Sz * Ckz;
+  float T = (U * Az + V * Bz) + W * Cz;
+  float rcp = 1.0f / det;
+  t = T * rcp;
+  bu = V * rcp;
+  bv = W * rcp;
+  return true;
+}
+
+// Intersectors_ptx.cu:74-99 (near root only)
+__device__ __forceinline__ bool isect_sphere(v3 o, v3 d, v3 c, float r, float &t)
+{
+  float rd2 = 1.0f / dot(d, d);
+  v3 CO = sub(c, o);
+  float projCO = dot(CO, d) * rd2;
+  v3 perp = sub(CO, fmul(projCO, d));
+  float l2 = dot(perp, perp);
+  float r2 = r * r;
+  if (l2 > r2)
+    return false;
+  float td = sqrtf((r2 - l2) * rd2);
+  t = projCO - td;
+  return true;
+}
+
+// Intersectors_ptx.cu:101-190: up to three reports (tin, tout, cap), in order.
+__device__ __forceinline__ int isect_cylinder(
+    v3 ro, v3 rd, v3 p0, v3 p1, float radius, bool caps, float ts[3], v3 ns[3])
+{
+  int valid = 0;
+  v3 s = sub(p1, p0);
+  v3 sxd = cross(s, rd);
+  float a = dot(sxd, sxd);
+  if (a == 0.0f)
+    return 0;
+  v3 f = sub(p0, ro);
+  v3 sxf = cross(s, f);
+  float ra = 1.0f / a;
+  float tsv = dot(sxd, sxf) * ra;
+  v3 fp = sub(f, fmul(tsv, rd));
+  float s2 = dot(s, s);
+  v3 perp = cross(s, fp);
+  float c = (radius * radius) * s2 - dot(perp, perp);
+  if (c < 0.0f)
+    return 0;
+  float td = sqrtf(c * ra);
+  float tin = tsv - td;
+  float tout = tsv + td;
+  float sf = dot(s, f);
+  float sd = dot(s, rd);
+  float inv_s2 = 1.0f / s2;
+  float u_in = (tin * sd - sf) * inv_s2;
+  v3 N_in = sub(sub(fmul(-td, rd), fp), fmul(u_in, s));
+  float u_out = (tout * sd - sf) * inv_s2;
+  v3 N_out = sub(sub(fmul(td, rd), fp), fmul(u_out, s));
+  if (sd != 0.0f) {
+    float rsd = 1.0f / sd;
+    float tA = sf * rsd;
+    float tB = tA + s2 * rsd;
+    float cap_tin = fminf(tA, tB);
+    float cap_tout = fmaxf(tA, tB);
+    if (tin > cap_tin && tin < cap_tout) {
+      ts[0] = tin;
+      ns[0] = N_in;
+      valid |= 1;
+    }
+    if (tout > cap_tin && tout < cap_tout) {
+      ts[1] = tout;
+      ns[1] = N_out;
+      valid |= 2;
+    }
+  } else if (sf <= 0.0f && sf >= -s2) {
+    ts[0] = tin;
+    ns[0] = N_in;
+    ts[1] = tout;
+    ns[1] = N_out;
+    valid |= 3;
+  }
+  if (caps) {
+    float ca = s2 - sd * sd;
+    float cb = s2 * dot(neg(f), rd) + (sf * sd);
+    float cc = s2 * dot(neg(f), neg(f)) - (sf * sf) - (radius * radius * s2);
+    float h = cb * cb - ca * cc;
+    if (h >= 0.0f) {
+      h = sqrtf(h);
+      float y = ((-cb - h) / ca) * sd - sf;
+      float dd = ((y < 0.0f ? 0.0f : s2) + sf) / sd;
+      if (fabsf(cb + ca * dd) < h) {
+        float sg = y > 0.0f ? 1.0f : (y < 0.0f ? -1.0f : 0.0f);
+        ts[2] = dd;
+        ns[2] = mk((s.x * sg) / s2, (s.y * sg) / s2, (s.z * sg) / s2);
+        valid |= 4;
+      }
+    }
+  }
+  return valid;
+}
+
+// Slab test shared with the CPU path: conservative (kBoxLo/kBoxHi slack).
+__device__ __forceinline__ bool slab(
+    float lx, float ly, float lz, float hx, float hy, float hz, v3 o, v3 inv, float tmin, float tmax,
+    float &tnear)
+{
+  float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
+  float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
+  float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+  float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+  float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+  tnear = fmaxf(tn * kBoxLo, tmin);
+  float tfar = fminf(tf * kBoxHi, tmax);
+  return tnear <= tfar;
+}
+
+struct Hit {
+  float t;
+  uint32_t entry, prim; // prim = original primitive index
+  float a0, a1, a2;     // triangle: u, v ; cylinder: object-space normal
+  bool found;
+};
+
+struct BlasCtx {
+  const Node *nodes;
+  const float4 *leaves;
+  const uint32_t *ids;
+  int type;
+  bool cull;
+  bool caps;
+  uint32_t entry;
+};
+
+__device__ __forceinline__ void load_blas(
+    const vrx_scene &sc, uint32_t e, v3 wo, v3 wd, bool cull_all, bool any, BlasCtx &b, v3 &o,
+    v3 &d, uint32_t &root, uint32_t &n)
+{
+  const vrx_entry &E = sc.d_entries[e];
+  const vrx_geom &G = sc.d_geoms[E.geom];
+  b.nodes = (const Node *)G.bvh.d_nodes;
+  b.leaves = (const float4 *)G.bvh.d_leaves;
+  b.ids = G.bvh.d_leaf_ids;
+  b.type = G.type;
+  b.cull = !any && (cull_all || G.cull_backfaces != 0);
+  b.caps = G.caps != 0;
+  b.entry = e;
+  n = G.n;
+  root = G.n == 1 ? kLeafBit : 0u;
+  if (E.identity) {
+    o = wo;
+    d = wd;
+  } else {
+    o = xfm_point(E.inv, wo);
+    d = xfm_vector(E.inv, wd);
+  }
+}
+
+template <int PT>
+__device__ __forceinline__ int blas_type(const BlasCtx &b)
+{
+  if constexpr (PT >= 0)
+    return PT;
+  else
+    return b.type;
+}
+
+// Accept a candidate t (already known to lie in [tmin, tmax]).
+__device__ __forceinline__ bool take(Hit &h, float t, float tmax, uint32_t entry, uint32_t prim)
+{
+  if (!h.found || t < tmax)
+    return true;
+  // t == tmax == h.t : lowest (entry, prim) wins
+  return entry < h.entry || (entry == h.entry && prim < h.prim);
+}
+
+// Test leaf k of the current BLAS.  Returns true for an any-hit termination.
+template <int PT, bool ANY>
+__device__ __forceinline__ bool test_leaf(
+    const BlasCtx &b, uint32_t k, v3 o, v3 d, const WT &wt, float tmin, float &tmax, Hit &h)
+{
+  const int type = blas_type<PT>(b);
+  if (type == VRX_SPHERE) {
+    float4 L = b.leaves[k];
+    float t;
+    if (isect_sphere(o, d, mk(L.x, L.y, L.z), L.w, t) && t >= tmin && t <= tmax) {
+      if (ANY)
+        return true;
+      uint32_t prim = b.ids[k];
+      if (take(h, t, tmax, b.entry, prim)) {
+        h.found = true;
+        h.t = t;
+        h.entry = b.entry;
+        h.prim = prim;
+        tmax = t;
+      }
+    }
+  } else if (type == VRX_TRIANGLE) {
+    const float4 *L = b.leaves + 3 * (size_t)k;
+    float4 A = L[0], B = L[1], C = L[2];
+    v3 v0 = mk(A.x, A.y, A.z), v1 = mk(B.x, B.y, B.z), v2 = mk(C.x, C.y, C.z);
+    float t, u, v;
+    if (isect_tri(wt, o, v0, v1, v2, t, u, v) && t >= tmin && t <= tmax) {
+      if (b.cull) {
+        v3 ng = cross(sub(v1, v0), sub(v2, v0));
+        if (!(dot(ng, d) < 0.0f))
+          return false;
+      }
+      if (ANY)
+        return true;
+      uint32_t prim = b.ids[k];
+      if (take(h, t, tmax, b.entry, prim)) {
+        h.found = true;
+        h.t = t;
+        h.entry = b.entry;
+        h.prim = prim;
+        h.a0 = u;
+        h.a1 = v;
+        tmax = t;
+      }
+    }
+  } else if (type == VRX_CYLINDER) {
+    const float4 *L = b.leaves + 2 * (size_t)k;
+    float4 P0 = L[0], P1 = L[1];
+    float ts[3];
+    v3 ns[3];
+    int vm = isect_cylinder(o, d, mk(P0.x, P0.y, P0.z), mk(P1.x, P1.y, P1.z), P0.w,
+        b.caps, ts, ns);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      if (!(vm & (1 << q)))
+        continue;
+      float t = ts[q];
+      if (t >= tmin && t <= tmax) {
+        if (ANY)
+          return true;
+        uint32_t prim = b.ids[k];
+        if (take(h, t, tmax, b.entry, prim)) {
+          h.found = true;
+          h.t = t;
+          h.entry = b.entry;
+          h.prim = prim;
+          h.a0 = ns[q].x;
+          h.a1 = ns[q].y;
+          h.a2 = ns[q].z;
+          tmax = t;
+        }
+      }
+    }
+  }
+  return false;
+}
+
+// Full traversal.  TWO_LEVEL: start in the instance BVH (sc.tlas) over
+// entries; otherwise trace entry 0 only.  Returns hit found (closest) or
+// occluded (ANY).
+template <int PT, bool TWO_LEVEL, bool ANY>
+__device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax, bool cull_all,
+    Stack &st, Hit &h)
+{
+  h.found = false;
+  h.t = tmax;
+  h.entry = 0;
+  h.prim = 0;
+  h.a0 = h.a1 = h.a2 = 0.0f;
+  if (sc.num_entries == 0)
+    return false;
+  st.cnt = 0;
+  const v3 winv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+  v3 o = wo, d = wd, inv = winv;
+  WT wt;
+  BlasCtx b;
+  bool in_blas;
+  uint32_t node;
+  const Node *nodes;
+  const uint32_t *tlas_ids = sc.tlas.d_leaf_ids;
+  const Node *tlas_nodes = (const Node *)sc.tlas.d_nodes;
+  if (TWO_LEVEL) {
+    in_blas = false;
+    nodes = tlas_nodes;
+    node = sc.num_entries == 1 ? kLeafBit : 0u;
+  } else {
+    uint32_t n, root;
+    load_blas(sc, 0u, wo, wd, cull_all, ANY, b, o, d, root, n);
+    if (n == 0)
+      return false;
+    inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    if (blas_type<PT>(b) == VRX_TRIANGLE)
+      wt = wt_setup(d);
+    in_blas = true;
+    nodes = b.nodes;
+    node = root;
+  }
+  for (;;) {
+    bool do_pop = true;
+    if (!(node & kLeafBit)) {
+      const Node nd = nodes[node];
+      float tn0, tn1;
+      bool h0 = slab(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, tmin, h.t, tn0);
+      bool h1 = slab(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, tmin, h.t, tn1);
+      if (h0 && h1) {
+        uint32_t nearc = nd.d.x, farc = nd.d.y;
+        if (tn1 < tn0) {
+          nearc = nd.d.y;
+          farc = nd.d.x;
+        }
+        st.push(farc);
+        node = nearc;
+        do_pop = false;
+      } else if (h0) {
+        node = nd.d.x;
+        do_pop = false;
+      } else if (h1) {
+        node = nd.d.y;
+        do_pop = false;
+      }
+    } else if (in_blas) {
+      float tcur = h.t;
+      if (test_leaf<PT, ANY>(b, node & ~kLeafBit, o, d, wt, tmin, tcur, h))
+        return true;
+    } else {
+      // instance leaf: enter the entry's BLAS in object space
+      uint32_t e = tlas_ids[node & ~kLeafBit];
+      uint32_t n, root;
+      load_blas(sc, e, wo, wd, cull_all, ANY, b, o, d, root, n);
+      if (n != 0) {
+        inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        if (blas_type<PT>(b) == VRX_TRIANGLE)
+          wt = wt_setup(d);
+        st.push(kMarker);
+        in_blas = true;
+        nodes = b.nodes;
+        node = root;
+        do_pop = false;
+      } else {
+        o = wo;
+        d = wd;
+      }
+    }
+    if (do_pop) {
+      for (;;) {
+        if (st.cnt == 0)
+          return ANY ? false : h.found;
+        node = st.pop();
+        if (TWO_LEVEL && node == kMarker) {
+          in_blas = false;
+          nodes = tlas_nodes;
+          o = wo;
+          d = wd;
+          inv = winv;
+          continue;
+        }
+        break;
+      }
+    }
+  }
+}
+
+// Surface hit in world space (populateHit.h:150-368).
+struct SurfHit {
+  float t, eps;
+  v3 hp, ng, ns;
+  uint32_t prim_out, obj, inst;
+  v3 color;
+  float opacity;
+};
+
+__device__ __forceinline__ void populate(
+    const vrx_scene &sc, const Hit &h, v3 wo, v3 wd, SurfHit &s)
+{
+  const vrx_entry &E = sc.d_entries[h.entry];
+  const vrx_geom &G = sc.d_geoms[E.geom];
+  v3 o = wo, d = wd;
+  const bool ident = E.identity != 0;
+  if (!ident) {
+    o = xfm_point(E.inv, wo);
+    d = xfm_vector(E.inv, wd);
+  }
+  v3 ng, ns;
+  if (G.type == VRX_TRIANGLE) {
+    uint32_t p = h.prim;
+    uint32_t ia = G.d_index ? G.d_index[3 * (size_t)p] : 3 * p;
+    uint32_t ib = G.d_index ? G.d_index[3 * (size_t)p + 1] : 3 * p + 1;
+    uint32_t ic = G.d_index ? G.d_index[3 * (size_t)p + 2] : 3 * p + 2;
+    v3 v0 = ld3(G.d_position + 3 * (size_t)ia), v1 = ld3(G.d_position + 3 * (size_t)ib),
+       v2 = ld3(G.d_position + 3 * (size_t)ic);
+    v3 raw = cross(sub(v1, v0), sub(v2, v0));
+    bool front = dot(raw, d) < 0.0f;
+    ng = normalize(raw);
+    if (!front)
+      ng = neg(ng);
+    if (G.d_vertex_normal) {
+      float bx = (1.0f - h.a0) - h.a1, by = h.a0, bz = h.a1;
+      v3 n0 = ld3(G.d_vertex_normal + 3 * (size_t)ia), n1 = ld3(G.d_vertex_normal + 3 * (size_t)ib),
+         n2 = ld3(G.d_vertex_normal + 3 * (size_t)ic);
+      ns = add(add(fmul(bx, n0), fmul(by, n1)), fmul(bz, n2));
+    } else {
+      ns = ng;
+    }
+    ns = normalize(ns);
+    if (dot(ng, ns) < 0.0f)
+      ns = neg(ns);
+  } else if (G.type == VRX_SPHERE) {
+    uint32_t vi = G.d_index ? G.d_index[h.prim] : h.prim;
+    v3 c = ld3(G.d_position + 3 * (size_t)vi);
+    v3 hp = add(o, fmul(h.t, d));
+    ng = ns = sub(hp, c);
+  } else {
+    ng = ns = mk(h.a0, h.a1, h.a2);
+  }
+  if (!ident) {
+    ng = xfm_normal(E.inv, ng);
+    ns = xfm_normal(E.inv, ns);
+  }
+  s.ng = normalize(ng);
+  s.ns = normalize(ns);
+  s.t = h.t;
+  s.hp = add(wo, fmul(h.t, wd));
+  s.eps = eps_from(s.hp, wd, h.t);
+  s.prim_out = G.d_primitive_id ? G.d_primitive_id[h.prim] : h.prim;
+  s.obj = G.surface_id;
+  s.inst = E.inst_id;
+  s.color = mk(G.color[0], G.color[1], G.color[2]);
+  s.opacity = G.opacity;
+}
+
+} // namespace vrx

@@ -1,0 +1,333 @@
+"""Seeded synthetic scenes and their ANARI construction.
+
+The scene descriptions here are plain numpy data; ``build_anari`` turns one
+into ANARI objects on a device, following the object graph of ANARI client
+code such as ``examples/simple/testApp_spheres.cpp:167-263`` in the VisRTX
+tree (geometry -> surface -> world, camera, renderer, frame).  The
+generators are the native PCG32 / value-noise routines of the HIP library
+(host code).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from . import anari as A
+
+
+# ---------------------------------------------------------------------------
+# descriptions
+# ---------------------------------------------------------------------------
+@dataclass
+class GeomDesc:
+    type: str                       # 'triangle' | 'sphere' | 'cylinder'
+    position: np.ndarray            # (V, 3) float32
+    index: np.ndarray | None = None
+    radius: np.ndarray | None = None
+    global_radius: float | None = None
+    caps: bool = False
+    primitive_id: np.ndarray | None = None
+    vertex_normal: np.ndarray | None = None
+    color: tuple = (0.8, 0.8, 0.8)
+    opacity: float = 1.0
+    surface_id: int = 0xFFFFFFFF
+    cull: bool = False
+
+    @property
+    def num_prims(self) -> int:
+        if self.type == "triangle":
+            return len(self.index) if self.index is not None else len(self.position) // 3
+        if self.type == "sphere":
+            return len(self.index) if self.index is not None else len(self.position)
+        return len(self.index) if self.index is not None else len(self.position) // 2
+
+
+@dataclass
+class InstanceDesc:
+    geoms: list
+    xfm: np.ndarray | None = None   # 12 floats, column-major 3x4
+    id: int = 0xFFFFFFFF
+
+
+@dataclass
+class VolumeDesc:
+    data: np.ndarray                # (nz, ny, nx) float32, x fastest
+    origin: tuple = (0.0, 0.0, 0.0)
+    spacing: tuple = (1.0, 1.0, 1.0)
+    color: np.ndarray | None = None  # (n, 3) or (n, 4)
+    opacity: np.ndarray | None = None
+    value_range: tuple = (0.0, 1.0)
+    unit_distance: float = 1.0
+    id: int = 0xFFFFFFFF
+
+
+@dataclass
+class CameraDesc:
+    position: tuple
+    direction: tuple
+    up: tuple
+    fovy: float                     # radians
+    aspect: float
+
+
+@dataclass
+class SceneDesc:
+    geoms: list = field(default_factory=list)
+    world_geoms: list = field(default_factory=list)   # indices into geoms
+    instances: list = field(default_factory=list)
+    volumes: list = field(default_factory=list)
+    world_id: int = 0xFFFFFFFF
+
+
+def look_at(pos, target):
+    d = np.asarray(target, np.float64) - np.asarray(pos, np.float64)
+    return tuple(float(x) for x in d)
+
+
+# ---------------------------------------------------------------------------
+# generators (native, seeded)
+# ---------------------------------------------------------------------------
+def gen_spheres(n: int, seed: int = 2, rmin: float = 1e-4, rmax: float = 3e-3):
+    L = _native.hip()
+    centers = np.empty((n, 3), np.float32)
+    radii = np.empty(n, np.float32)
+    L.vrx_gen_spheres(seed, n, rmin, rmax, centers.ctypes.data, radii.ctypes.data)
+    return centers, radii
+
+
+def gen_heightfield(n: int = 257, seed: int = 1, amplitude: float = 0.2):
+    L = _native.hip()
+    verts = np.empty((n * n, 3), np.float32)
+    idx = np.empty((2 * (n - 1) * (n - 1), 3), np.uint32)
+    L.vrx_gen_heightfield(seed, n, n, amplitude, verts.ctypes.data, idx.ctypes.data)
+    return verts, idx
+
+
+def gen_noise_volume(nx: int, ny: int | None = None, nz: int | None = None, seed: int = 3,
+                     octaves: int = 4, base_freq: float = 4.0):
+    ny = ny or nx
+    nz = nz or nx
+    L = _native.hip()
+    out = np.empty((nz, ny, nx), np.float32)
+    L.vrx_gen_noise_volume(seed, nx, ny, nz, octaves, base_freq, out.ctypes.data)
+    return out
+
+
+def gen_sphere_mesh(target_tris: int, seed: int, displacement: float = 0.3):
+    L = _native.hip()
+    nv, nt = C.c_uint32(), C.c_uint32()
+    L.vrx_gen_sphere_mesh_counts(target_tris, C.byref(nv), C.byref(nt))
+    verts = np.empty((nv.value, 3), np.float32)
+    idx = np.empty((nt.value, 3), np.uint32)
+    L.vrx_gen_sphere_mesh(seed, target_tris, displacement, verts.ctypes.data, idx.ctypes.data)
+    return verts, idx
+
+
+# a fixed 5-point colormap (blue, cyan, green, yellow, red)
+COLORMAP5 = np.array([[0.0, 0.0, 0.6], [0.0, 0.7, 0.9], [0.1, 0.8, 0.2], [0.95, 0.9, 0.1],
+                      [0.9, 0.1, 0.05]], np.float32)
+
+
+# ---------------------------------------------------------------------------
+# the benchmark workloads
+# ---------------------------------------------------------------------------
+def heightfield_workload(n: int = 257, width: int = 1024, height: int = 768, seed: int = 1):
+    """257x257 heightfield, z = 0.2 * 4-octave value noise: 131,072 triangles."""
+    verts, idx = gen_heightfield(n, seed)
+    sc = SceneDesc(geoms=[GeomDesc("triangle", verts, idx, surface_id=0)], world_geoms=[0],
+                   world_id=0)
+    cam = CameraDesc((0.5, -1.5, 1.2), look_at((0.5, -1.5, 1.2), (0.5, 0.5, 0.0)), (0, 0, 1),
+                     math.radians(45.0), width / height)
+    return sc, cam
+
+
+def spheres_workload(n: int = 10_000_000, width: int = 1920, height: int = 1080, seed: int = 2):
+    """n spheres, centers U[0,1]^3, radii log-uniform in [1e-4, 3e-3]."""
+    centers, radii = gen_spheres(n, seed)
+    sc = SceneDesc(geoms=[GeomDesc("sphere", centers, radius=radii, surface_id=0)],
+                   world_geoms=[0], world_id=0)
+    cam = CameraDesc((0.5, 0.5, -1.2), (0.0, 0.0, 1.0), (0, 1, 0), math.radians(45.0),
+                     width / height)
+    return sc, cam
+
+
+def volume_workload(n: int = 512, width: int = 1920, height: int = 1080, seed: int = 3,
+                    data: np.ndarray | None = None):
+    """n^3 fp32 fractal noise in [0,1], bounds [0,1]^3, linear opacity ramp 0..0.05."""
+    if data is None:
+        data = gen_noise_volume(n, seed=seed)
+    sp = 1.0 / (n - 1)
+    vol = VolumeDesc(data=data, origin=(0.0, 0.0, 0.0), spacing=(sp, sp, sp),
+                     color=COLORMAP5, opacity=np.array([0.0, 0.05], np.float32),
+                     value_range=(0.0, 1.0), unit_distance=sp, id=0)
+    sc = SceneDesc(volumes=[vol], world_id=0)
+    pos = (1.9, 1.5, -1.0)
+    cam = CameraDesc(pos, look_at(pos, (0.5, 0.5, 0.5)), (0, 1, 0), math.radians(35.0),
+                     width / height)
+    return sc, cam
+
+
+# ---------------------------------------------------------------------------
+# ANARI construction
+# ---------------------------------------------------------------------------
+_GEOM_SUB = {"triangle": "triangle", "sphere": "sphere", "cylinder": "cylinder"}
+
+
+class AnariScene:
+    """Handles of one scene on a device (released by ``release``)."""
+
+    def __init__(self, dev: A.Device):
+        self.dev = dev
+        self.handles = []
+        self.world = None
+
+    def keep(self, h):
+        self.handles.append(h)
+        return h
+
+    def release(self):
+        for h in reversed(self.handles):
+            self.dev.release(h)
+        self.handles.clear()
+
+
+def _surface(dev: A.Device, s: AnariScene, g: GeomDesc):
+    geom = s.keep(dev.new("Geometry", _GEOM_SUB[g.type]))
+    dev.set(geom, "vertex.position", s.keep(dev.array(g.position, A.FLOAT32_VEC3)), A.ARRAY1D)
+    if g.index is not None:
+        et = {"triangle": A.UINT32_VEC3, "sphere": A.UINT32, "cylinder": A.UINT32_VEC2}[g.type]
+        dev.set(geom, "primitive.index", s.keep(dev.array(g.index, et)), A.ARRAY1D)
+    if g.radius is not None:
+        name = "vertex.radius" if g.type == "sphere" else "primitive.radius"
+        dev.set(geom, name, s.keep(dev.array(g.radius, A.FLOAT32)), A.ARRAY1D)
+    if g.global_radius is not None:
+        dev.set(geom, "radius", g.global_radius, A.FLOAT32)
+    if g.type == "cylinder" and g.caps:
+        dev.set(geom, "caps", "both", A.STRING)
+    if g.primitive_id is not None:
+        dev.set(geom, "primitive.id", s.keep(dev.array(g.primitive_id, A.UINT32)), A.ARRAY1D)
+    if g.vertex_normal is not None:
+        dev.set(geom, "vertex.normal", s.keep(dev.array(g.vertex_normal, A.FLOAT32_VEC3)),
+                A.ARRAY1D)
+    if g.cull:
+        dev.set(geom, "cullBackfaces", 1, A.BOOL)
+    dev.commit(geom)
+    mat = s.keep(dev.new("Material", "matte"))
+    dev.set(mat, "color", g.color, A.FLOAT32_VEC3)
+    if g.opacity != 1.0:
+        dev.set(mat, "opacity", g.opacity, A.FLOAT32)
+        dev.set(mat, "alphaMode", "blend", A.STRING)
+    dev.commit(mat)
+    surf = s.keep(dev.new("Surface"))
+    dev.set(surf, "geometry", geom, A.GEOMETRY)
+    dev.set(surf, "material", mat, A.MATERIAL)
+    if g.surface_id != 0xFFFFFFFF:
+        dev.set(surf, "id", g.surface_id, A.UINT32)
+    dev.commit(surf)
+    return surf
+
+
+def _volume(dev: A.Device, s: AnariScene, v: VolumeDesc):
+    nz, ny, nx = v.data.shape
+    field_ = s.keep(dev.new("SpatialField", "structuredRegular"))
+    dev.set(field_, "data", s.keep(dev.array(v.data, A.FLOAT32, dims=(nx, ny, nz))), A.ARRAY3D)
+    dev.set(field_, "origin", v.origin, A.FLOAT32_VEC3)
+    dev.set(field_, "spacing", v.spacing, A.FLOAT32_VEC3)
+    dev.commit(field_)
+    vol = s.keep(dev.new("Volume", "transferFunction1D"))
+    dev.set(vol, "value", field_, A.SPATIAL_FIELD)
+    if v.color is not None:
+        et = A.FLOAT32_VEC3 if v.color.shape[1] == 3 else A.FLOAT32_VEC4
+        dev.set(vol, "color", s.keep(dev.array(v.color, et)), A.ARRAY1D)
+    if v.opacity is not None:
+        dev.set(vol, "opacity", s.keep(dev.array(v.opacity, A.FLOAT32)), A.ARRAY1D)
+    dev.set(vol, "valueRange", v.value_range, A.FLOAT32_VEC2)
+    dev.set(vol, "unitDistance", v.unit_distance, A.FLOAT32)
+    if v.id != 0xFFFFFFFF:
+        dev.set(vol, "id", v.id, A.UINT32)
+    dev.commit(vol)
+    return vol
+
+
+def build_anari(dev: A.Device, sc: SceneDesc) -> AnariScene:
+    s = AnariScene(dev)
+    surfs = [None] * len(sc.geoms)
+
+    def surf_of(i):
+        if surfs[i] is None:
+            surfs[i] = _surface(dev, s, sc.geoms[i])
+        return surfs[i]
+
+    world = s.keep(dev.new("World"))
+    if sc.world_geoms:
+        arr = s.keep(dev.array([surf_of(i) for i in sc.world_geoms], A.SURFACE))
+        dev.set(world, "surface", arr, A.ARRAY1D)
+    if sc.volumes:
+        vols = [_volume(dev, s, v) for v in sc.volumes]
+        dev.set(world, "volume", s.keep(dev.array(vols, A.VOLUME)), A.ARRAY1D)
+    if sc.instances:
+        insts = []
+        for inst in sc.instances:
+            grp = s.keep(dev.new("Group"))
+            arr = s.keep(dev.array([surf_of(i) for i in inst.geoms], A.SURFACE))
+            dev.set(grp, "surface", arr, A.ARRAY1D)
+            dev.commit(grp)
+            ih = s.keep(dev.new("Instance", "transform"))
+            dev.set(ih, "group", grp, A.GROUP)
+            if inst.xfm is not None:
+                dev.set(ih, "transform", np.asarray(inst.xfm, np.float32), A.FLOAT32_MAT3x4)
+            if inst.id != 0xFFFFFFFF:
+                dev.set(ih, "id", inst.id, A.UINT32)
+            dev.commit(ih)
+            insts.append(ih)
+        dev.set(world, "instance", s.keep(dev.array(insts, A.INSTANCE)), A.ARRAY1D)
+    if sc.world_id != 0xFFFFFFFF:
+        dev.set(world, "id", sc.world_id, A.UINT32)
+    dev.commit(world)
+    s.world = world
+    return s
+
+
+def make_camera(dev: A.Device, s: AnariScene, cam: CameraDesc):
+    c = s.keep(dev.new("Camera", "perspective"))
+    dev.set(c, "position", cam.position, A.FLOAT32_VEC3)
+    dev.set(c, "direction", cam.direction, A.FLOAT32_VEC3)
+    dev.set(c, "up", cam.up, A.FLOAT32_VEC3)
+    dev.set(c, "fovy", cam.fovy, A.FLOAT32)
+    dev.set(c, "aspect", cam.aspect, A.FLOAT32)
+    dev.commit(c)
+    return c
+
+
+def make_renderer(dev: A.Device, s: AnariScene, kind: str = "raycast", **params):
+    r = s.keep(dev.new("Renderer", kind))
+    types = {"background": A.FLOAT32_VEC4, "ambientColor": A.FLOAT32_VEC3,
+             "ambientRadiance": A.FLOAT32, "pixelSamples": A.INT32, "ambientSamples": A.INT32,
+             "ambientOcclusionDistance": A.FLOAT32, "cullTriangleBackfaces": A.BOOL,
+             "volumeSamplingRate": A.FLOAT32, "sampleLimit": A.INT32, "stackDepthLDS": A.INT32}
+    for k, v in params.items():
+        dev.set(r, k, v, types[k])
+    dev.commit(r)
+    return r
+
+
+def make_frame(dev: A.Device, s: AnariScene, world, camera, renderer, width: int, height: int,
+               color=A.FLOAT32_VEC4, depth=True, ids=True):
+    f = s.keep(dev.new("Frame"))
+    dev.set(f, "size", (width, height), A.UINT32_VEC2)
+    dev.set(f, "channel.color", color, A.DATA_TYPE)
+    if depth:
+        dev.set(f, "channel.depth", A.FLOAT32, A.DATA_TYPE)
+    if ids:
+        dev.set(f, "channel.primitiveId", A.UINT32, A.DATA_TYPE)
+        dev.set(f, "channel.objectId", A.UINT32, A.DATA_TYPE)
+        dev.set(f, "channel.instanceId", A.UINT32, A.DATA_TYPE)
+    dev.set(f, "world", world, A.WORLD)
+    dev.set(f, "camera", camera, A.CAMERA)
+    dev.set(f, "renderer", renderer, A.RENDERER)
+    dev.commit(f)
+    return f
