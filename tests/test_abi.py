@@ -33,3 +33,14 @@ def test_anari_library_exports_every_declared_symbol():
     want = {w for w in want if w.startswith(("anari", "make", "visrtx"))}
     missing = sorted(want - syms)
     assert not missing, missing
+
+
+def test_every_hip_entry_point_has_64bit_safe_argtypes():
+    """Each exported vrx_ function is bound with explicit argtypes, so a device
+    address can never be passed as a 32-bit C int."""
+    L = _native.hip()
+    want = declared("vrx_hip.h", "vrx_")
+    missing = sorted(n for n in want if n not in L._vrx_sigs)
+    assert not missing, missing
+    for name in want:
+        assert getattr(L, name).argtypes is not None, name

@@ -81,28 +81,62 @@ def hip() -> C.CDLL:
     global _hip
     if _hip is None:
         L = _load(HIP_LIB_PATH)
-        L.vrx_last_error.restype = C.c_char_p
-        for name in ["vrx_device_count", "vrx_set_device", "vrx_malloc", "vrx_free",
-                     "vrx_memcpy", "vrx_memcpy_sync", "vrx_fill_u32", "vrx_stream_create",
-                     "vrx_stream_sync", "vrx_device_sync", "vrx_bvh_query_sizes", "vrx_bvh_build",
-                     "vrx_bvh_build_debug", "vrx_camera_perspective", "vrx_diffvol_forward",
-                     "vrx_diffvol_backward", "vrx_make_rays", "vrx_render", "vrx_frame_clear",
-                     "vrx_tlas_build", "vrx_tlas_query_sizes", "vrx_render_batch"]:
-            getattr(L, name).restype = C.c_int
-        L.vrx_render_scratch_bytes.restype = C.c_size_t
-        L.vrx_diffvol_ckpt_floats.restype = C.c_size_t
-        L.vrx_batch_scratch_bytes.restype = C.c_size_t
-        L.vrx_gen_value_noise3.restype = C.c_float
-        L.vrx_gen_value_noise3.argtypes = [C.c_uint64, C.c_float, C.c_float, C.c_float]
-        L.vrx_gen_spheres.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.c_float, C.c_void_p,
-                                      C.c_void_p]
-        L.vrx_gen_heightfield.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_float,
-                                          C.c_void_p, C.c_void_p]
-        L.vrx_gen_noise_volume.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
-                                           C.c_int, C.c_float, C.c_void_p]
-        L.vrx_gen_sphere_mesh_counts.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p]
-        L.vrx_gen_sphere_mesh.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.c_void_p,
-                                          C.c_void_p]
+        vp, sz, u32, i32 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_int
+        # every entry point gets explicit argtypes: device addresses are 64-bit
+        # and must never be passed as a default (32-bit) C int
+        sigs = {
+            "vrx_last_error": (C.c_char_p, []),
+            "vrx_version": (i32, []),
+            "vrx_device_count": (i32, [vp]),
+            "vrx_set_device": (i32, [i32]),
+            "vrx_get_device": (i32, [vp]),
+            "vrx_device_info": (i32, [i32, vp, sz, vp, vp]),
+            "vrx_malloc": (i32, [vp, sz]),
+            "vrx_free": (i32, [vp]),
+            "vrx_host_malloc": (i32, [vp, sz]),
+            "vrx_host_free": (i32, [vp]),
+            "vrx_memcpy": (i32, [vp, vp, sz, i32, vp]),
+            "vrx_memcpy_sync": (i32, [vp, vp, sz, i32]),
+            "vrx_fill_u32": (i32, [vp, u32, sz, vp]),
+            "vrx_stream_create": (i32, [vp]),
+            "vrx_stream_destroy": (i32, [vp]),
+            "vrx_stream_sync": (i32, [vp]),
+            "vrx_device_sync": (i32, []),
+            "vrx_event_create": (i32, [vp]),
+            "vrx_event_destroy": (i32, [vp]),
+            "vrx_event_record": (i32, [vp, vp]),
+            "vrx_event_sync": (i32, [vp]),
+            "vrx_event_query": (i32, [vp]),
+            "vrx_event_elapsed_ms": (i32, [vp, vp, vp]),
+            "vrx_launch_host_func": (i32, [vp, vp, vp]),
+            "vrx_bvh_query_sizes": (i32, [i32, u32, vp]),
+            "vrx_bvh_build": (i32, [vp, vp, vp, sz, vp]),
+            "vrx_bvh_build_debug": (i32, [vp, vp, vp, sz, vp, vp, vp]),
+            "vrx_render_scratch_bytes": (sz, [vp, vp, u32, u32]),
+            "vrx_frame_clear": (i32, [vp, vp]),
+            "vrx_render": (i32, [vp, vp, vp, vp, vp]),
+            "vrx_camera_perspective": (i32, [vp, vp, vp, C.c_float, C.c_float, vp, vp]),
+            "vrx_tlas_query_sizes": (i32, [u32, vp]),
+            "vrx_tlas_build": (i32, [vp, vp, vp, sz, vp]),
+            "vrx_diffvol_forward": (i32, [vp, u32, vp, vp, vp, vp, vp]),
+            "vrx_diffvol_backward": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "vrx_diffvol_ckpt_floats": (sz, [vp, u32]),
+            "vrx_make_rays": (i32, [vp, u32, u32, u32, vp, vp, vp]),
+            "vrx_batch_scratch_bytes": (sz, [vp, vp, vp]),
+            "vrx_render_batch": (i32, [vp, vp, vp, vp]),
+            "vrx_gen_pcg32": (u32, [vp, C.c_uint64]),
+            "vrx_gen_value_noise3": (C.c_float, [C.c_uint64, C.c_float, C.c_float, C.c_float]),
+            "vrx_gen_spheres": (None, [C.c_uint64, u32, C.c_float, C.c_float, vp, vp]),
+            "vrx_gen_heightfield": (None, [C.c_uint64, u32, u32, C.c_float, vp, vp]),
+            "vrx_gen_noise_volume": (None, [C.c_uint64, u32, u32, u32, i32, C.c_float, vp]),
+            "vrx_gen_sphere_mesh_counts": (None, [u32, vp, vp]),
+            "vrx_gen_sphere_mesh": (None, [C.c_uint64, u32, C.c_float, vp, vp]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L._vrx_sigs = sigs
         _hip = L
     return _hip
 
