@@ -1,0 +1,267 @@
+"""Benchmark of the MI355X ANARI device on the synthetic workloads.
+
+Headline (``value``): primary closest-hit Mrays/s of the raycast renderer on
+the 10M-sphere scene at 1920x1080, 1 spp -- one step is one ANARI frame
+(camera commit, which resets accumulation, then anariRenderFrame) with the
+scene already resident in HBM.  Also reported under ``extra``: LBVH build ms
+of that scene, AO renderer Mrays/s (16 spp), the 131k-triangle heightfield
+raycast, and the 512^3 volume march in Gsamples/s.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public spec
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def device_sync():
+    from visrtx_amd import _native
+    _native.check(_native.hip().vrx_device_sync(), "device sync")
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def load_counts():
+    with open(os.path.join(ROOT, "tests", "golden", "workload_counts.json")) as f:
+        return json.load(f)
+
+
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+class Workload:
+    """An ANARI scene + frame on the device."""
+
+    def __init__(self, dev, sc, cam, w, h, kind="raycast", color=None, ids=True, **rp):
+        from visrtx_amd import anari as A
+        from visrtx_amd import scenes as S
+        self.dev, self.A = dev, A
+        self.s = S.build_anari(dev, sc)
+        self.cam = S.make_camera(dev, self.s, cam)
+        self.r = S.make_renderer(dev, self.s, kind, **rp)
+        self.f = S.make_frame(dev, self.s, self.s.world, self.cam, self.r, w, h,
+                              color=color if color is not None else A.UFIXED8_RGBA_SRGB,
+                              depth=True, ids=ids)
+        self.w, self.h = w, h
+
+    def step(self):
+        # a camera commit resets accumulation (Frame.cu:574-588), so every step
+        # renders a full frame even for the single-shot raycast renderer
+        self.dev.commit(self.cam)
+        self.dev.render(self.f)
+
+    def duration(self):
+        return self.dev.get_property(self.f, "duration", self.A.FLOAT32)
+
+    def rays(self):
+        return self.dev.get_property(self.f, "raysTraced", self.A.UINT64)
+
+    def samples(self):
+        return self.dev.get_property(self.f, "volumeSamples", self.A.UINT64)
+
+    def timed(self, steps, warmup):
+        for _ in range(warmup):
+            self.step()
+        device_sync()
+        durs = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step()
+            durs.append(self.duration())   # waits on the frame's end event
+        device_sync()
+        wall = time.perf_counter() - t0
+        return wall / steps, durs
+
+    def release(self):
+        self.s.release()
+
+
+def bench_lbvh(dev, work, reps=5):
+    """LBVH build time of the world (hipEvents around the BLAS build kernels)."""
+    from visrtx_amd import anari as A
+    ms = []
+    geom = work.s.geoms[0]
+    for _ in range(reps):
+        dev.commit(geom)
+        dev.get_property(work.s.world, "bounds", A.FLOAT32_BOX3)
+        ms.append(1000.0 * dev.get_property(work.s.world, "bvhBuildDuration", A.FLOAT32))
+    return statistics.median(ms), ms
+
+
+def cpu_baseline(threads):
+    """The oracle (C/OpenMP port of the same path) on a bounded sample of the
+    10M-sphere workload: same scene and camera, 640x360 rays."""
+    from oracle import oracle as orc
+    from visrtx_amd import scenes as S
+    sc, cam = S.spheres_workload()
+    t0 = time.perf_counter()
+    o = orc.OracleScene(sc)
+    build_s = time.perf_counter() - t0
+    w, h = 640, 360
+    o.render(cam, 160, 90, "raycast", threads=threads)  # warm-up
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        o.render(cam, w, h, "raycast", threads=threads)
+        times.append(time.perf_counter() - t)
+    o.close()
+    med = statistics.median(times)
+    return {"value": round(w * h / med / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+            "kind": "port",
+            "sample": f"10M-sphere scene, {w}x{h} primary raycast rays (same camera), oracle "
+                      f"LBVH + BVH2 traversal in C/OpenMP, median of 3; oracle LBVH build "
+                      f"{build_s * 1000:.0f} ms"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only-headline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this device renders on one GPU; multi-GPU sharding is "
+                                   "design-only (see DESIGN.md), run with --gpus 1"}))
+        sys.exit(2)
+
+    from visrtx_amd import anari as A
+    from visrtx_amd import scenes as S
+    counts = load_counts()
+
+    dev = A.Device()
+    extra = {}
+
+    # ---- headline: 10M spheres, 1920x1080 raycast -------------------------
+    W, H = 1920, 1080
+    log("generating 10M spheres")
+    sc, cam = S.spheres_workload(width=W, height=H)
+    work = Workload(dev, sc, cam, W, H, "raycast")
+    log("building scene")
+    t = time.perf_counter()
+    dev.get_property(work.s.world, "bounds", A.FLOAT32_BOX3)
+    log(f"scene build (upload + LBVH) {time.perf_counter() - t:.2f}s")
+    lbvh_ms, lbvh_all = bench_lbvh(dev, work)
+    extra["lbvh_build_ms_10m_spheres"] = round(lbvh_ms, 3)
+    log(f"LBVH build {lbvh_ms:.3f} ms (all {['%.3f' % x for x in lbvh_all]})")
+    sec_per_step, durs = work.timed(args.steps, args.warmup)
+    rays = W * H
+    value = rays / sec_per_step / 1e6
+    kern_ms = statistics.mean(durs) * 1000.0
+    log(f"primary raycast: {sec_per_step * 1000:.3f} ms/step wall, kernel {kern_ms:.3f} ms "
+        f"-> {value:.1f} Mrays/s (kernel-only {rays / (kern_ms / 1000) / 1e6:.1f})")
+    extra["primary_kernel_mrays_s"] = round(rays / (kern_ms / 1000) / 1e6, 2)
+    c2 = counts["spheres10m_1920x1080"]
+    bytes_per_ray = 32 + 16 + 64 * c2["nodes_per_ray"] + 16 * c2["prims_per_ray"]
+    alg_bytes = rays * bytes_per_ray
+    achieved = alg_bytes / (kern_ms / 1000) / 1e9
+    traffic = load_traffic("render_spheres10m_1920x1080")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+                "kernel": "k_render<sphere,single-level,raycast>",
+                "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
+                "avg_kernel_ms": round(kern_ms, 4)}
+
+    if not args.only_headline:
+        # AO renderer, 16 spp
+        ao = Workload(dev, sc, cam, W, H, "ao", ids=True, pixelSamples=16, ambientSamples=1,
+                      ambientOcclusionDistance=0.02)
+        sps, ad = ao.timed(3, 1)
+        nr = ao.rays()
+        extra["ao16_mrays_s"] = round(nr / sps / 1e6, 2)
+        extra["ao16_rays_per_frame"] = int(nr)
+        extra["ao16_ms_per_frame"] = round(sps * 1000, 3)
+        log(f"AO 16spp: {sps * 1000:.2f} ms/frame, {nr} rays -> {extra['ao16_mrays_s']} Mrays/s")
+        ao.release()
+    work.release()
+    del sc
+
+    if not args.only_headline:
+        # heightfield raycast
+        sc1, cam1 = S.heightfield_workload()
+        hf = Workload(dev, sc1, cam1, 1024, 768, "raycast")
+        sps, d1 = hf.timed(20, 3)
+        extra["heightfield_mrays_s"] = round(1024 * 768 / sps / 1e6, 2)
+        extra["heightfield_kernel_ms"] = round(statistics.mean(d1) * 1000, 4)
+        log(f"heightfield: {sps * 1000:.3f} ms/step -> {extra['heightfield_mrays_s']} Mrays/s")
+        hf.release()
+
+        # 512^3 volume
+        log("generating 512^3 volume")
+        sc3, cam3 = S.volume_workload()
+        vol = Workload(dev, sc3, cam3, W, H, "raycast", volumeSamplingRate=1.0)
+        sps, d3 = vol.timed(5, 1)
+        ns = vol.samples()
+        kms = statistics.mean(d3) * 1000
+        ref_ns = counts["volume512_1920x1080"]["samples"]
+        extra["volume_gsamples_s"] = round(ref_ns / (kms / 1000) / 1e9, 2)
+        extra["volume_gpu_samples"] = int(ns)
+        extra["volume_oracle_samples"] = int(ref_ns)
+        extra["volume_kernel_ms"] = round(kms, 3)
+        extra["volume_hbm_frac_32B_per_sample"] = round(ref_ns * 32 / (kms / 1000) / 1e9 /
+                                                        HBM_PEAK_GBS, 4)
+        log(f"volume: kernel {kms:.3f} ms, {ns} samples (oracle {ref_ns}) -> "
+            f"{extra['volume_gsamples_s']} Gsamples/s")
+        vol.release()
+        del sc3
+
+    dev.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"cpu baseline with {threads} threads")
+        cpu = cpu_baseline(threads)
+
+    out = {
+        "metric": "Mrays/s closest-hit, volume Gsamples/s, fwd+bwd ms/iter on 1 MI355X",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec_per_step * 1000, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded PCG32 spheres)",
+        "config": {"workload": "10M spheres (centers U[0,1]^3, radii log-uniform [1e-4,3e-3]), "
+                               "1920x1080 raycast renderer, 1 spp, primary closest-hit",
+                   "rays_per_step": rays, "renderer": "raycast",
+                   "channels": "color(SRGB8), depth, primitiveId, objectId, instanceId"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -178,3 +178,36 @@ def test_empty_world_background(require_gpu):
     assert (g["depth"] == np.float32(1e30)).all()
     o = orc.OracleScene(sc).render(cam, 8, 8, background=(0.25, 0.5, 0.75, 1.0))
     compare(g, o)
+
+
+def test_volume_raycast_vs_fp64_oracle(require_gpu):
+    """64^3 noise field, EA march at 0.5-voxel steps, early termination at 0.99:
+    colour within 1e-4 of the FP64 oracle march; ids and entry depth."""
+    data = S.gen_noise_volume(64, seed=4)
+    sc, cam = S.volume_workload(n=64, width=200, height=120, data=data)
+    g = render_gpu(sc, cam, 200, 120, volumeSamplingRate=1.0)
+    o = orc.OracleScene(sc).render(cam, 200, 120, volumeSamplingRate=1.0)
+    assert not g["errors"], g["errors"]
+    np.testing.assert_allclose(g["color"], o["color"], rtol=0, atol=1e-4)
+    np.testing.assert_array_equal(g["primitiveId"], o["primitiveId"])
+    np.testing.assert_array_equal(g["objectId"], o["objectId"])
+    np.testing.assert_array_equal(g["instanceId"], o["instanceId"])
+    np.testing.assert_allclose(g["depth"], o["depth"], rtol=1e-5)
+    hit = g["depth"] < 1e29
+    assert hit.mean() > 0.2 and (g["color"][hit][:, :3].max() > 0.05)
+
+
+def test_volume_with_surface_in_front_and_inside(require_gpu):
+    """A sphere inside the volume: march up to the surface hit, then shade."""
+    data = S.gen_noise_volume(48, seed=6)
+    sc, cam = S.volume_workload(n=48, width=160, height=100, data=data)
+    sc.geoms = [S.GeomDesc("sphere", np.array([[0.5, 0.5, 0.5], [0.2, 0.8, 0.3]], np.float32),
+                           radius=np.array([0.25, 0.1], np.float32), surface_id=11)]
+    sc.world_geoms = [0]
+    g = render_gpu(sc, cam, 160, 100, volumeSamplingRate=0.5)
+    o = orc.OracleScene(sc).render(cam, 160, 100, volumeSamplingRate=0.5)
+    assert not g["errors"], g["errors"]
+    np.testing.assert_allclose(g["color"], o["color"], rtol=0, atol=1e-4)
+    np.testing.assert_array_equal(g["primitiveId"], o["primitiveId"])
+    np.testing.assert_array_equal(g["objectId"], o["objectId"])
+    np.testing.assert_allclose(g["depth"], o["depth"], rtol=1e-5)

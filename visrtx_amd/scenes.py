@@ -183,6 +183,7 @@ class AnariScene:
     def __init__(self, dev: A.Device):
         self.dev = dev
         self.handles = []
+        self.geoms = []
         self.world = None
 
     def keep(self, h):
@@ -197,6 +198,7 @@ class AnariScene:
 
 def _surface(dev: A.Device, s: AnariScene, g: GeomDesc):
     geom = s.keep(dev.new("Geometry", _GEOM_SUB[g.type]))
+    s.geoms.append(geom)
     dev.set(geom, "vertex.position", s.keep(dev.array(g.position, A.FLOAT32_VEC3)), A.ARRAY1D)
     if g.index is not None:
         et = {"triangle": A.UINT32_VEC3, "sphere": A.UINT32, "cylinder": A.UINT32_VEC2}[g.type]
