@@ -112,6 +112,33 @@ def bench_lbvh(dev, work, reps=5):
     return statistics.median(ms), ms
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """HBM bandwidth of a plain streaming copy kernel (read + write bytes / time)."""
+    import ctypes as C
+    from visrtx_amd import _native
+    L = _native.hip()
+    src, dst = C.c_void_p(), C.c_void_p()
+    _native.check(L.vrx_malloc(C.byref(src), nbytes))
+    _native.check(L.vrx_malloc(C.byref(dst), nbytes))
+    _native.check(L.vrx_fill_u32(src, 1, nbytes // 4, None))
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    L.vrx_event_create(C.byref(e0))
+    L.vrx_event_create(C.byref(e1))
+    _native.check(L.vrx_copy_kernel(dst, src, nbytes, None))
+    L.vrx_event_record(e0, None)
+    for _ in range(reps):
+        _native.check(L.vrx_copy_kernel(dst, src, nbytes, None))
+    L.vrx_event_record(e1, None)
+    L.vrx_event_sync(e1)
+    ms = C.c_float()
+    L.vrx_event_elapsed_ms(e0, e1, C.byref(ms))
+    L.vrx_event_destroy(e0)
+    L.vrx_event_destroy(e1)
+    L.vrx_free(src)
+    L.vrx_free(dst)
+    return 2.0 * nbytes * reps / (ms.value / 1000.0) / 1e9
+
+
 def cpu_baseline(threads):
     """The oracle (C/OpenMP port of the same path) on a bounded sample of the
     10M-sphere workload: same scene and camera, 640x360 rays."""
@@ -156,6 +183,8 @@ def main():
 
     dev = A.Device()
     extra = {}
+    extra["copy_kernel_gbs"] = round(copy_bandwidth(), 1)
+    log(f"copy kernel bandwidth {extra['copy_kernel_gbs']} GB/s")
 
     # ---- headline: 10M spheres, 1920x1080 raycast -------------------------
     W, H = 1920, 1080

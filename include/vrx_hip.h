@@ -67,6 +67,9 @@ int vrx_host_free(void *h_ptr);
 int vrx_memcpy(void *dst, const void *src, size_t bytes, int kind, vrx_stream s);
 int vrx_memcpy_sync(void *dst, const void *src, size_t bytes, int kind);
 int vrx_fill_u32(void *d_ptr, uint32_t value, size_t count, vrx_stream s);
+/* plain streaming copy kernel (bytes % 16 == 0): the HBM bandwidth probe and
+   the calibration kernel for the HBM byte counters */
+int vrx_copy_kernel(void *d_dst, const void *d_src, size_t bytes, vrx_stream s);
 int vrx_stream_create(vrx_stream *s);
 int vrx_stream_destroy(vrx_stream s);
 int vrx_stream_sync(vrx_stream s);

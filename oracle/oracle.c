@@ -1092,8 +1092,30 @@ void orc_classify(const orc_volume *v, double s, double rgba[4])
   classify_lut(NULL, v->lut, v->value_range, s, rgba, NULL, NULL, NULL);
 }
 
+/* Decision margins for fp32-vs-FP64 comparisons: a pixel whose sample count
+   would change under fp32 rounding (entry/exit distances within a few ulp of
+   a sample position, or opacity within 1e-4 of the 0.99 early-termination
+   threshold) is flagged ambiguous. */
+#define AMB_T_REL 2e-6
+#define AMB_OPACITY 1e-4
+
+static inline int near_rel(double a, double b)
+{
+  double m = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+  return fabs(a - b) <= AMB_T_REL * (m > 1.0 ? m : 1.0);
+}
+
+static int vol_slab_amb(const orc_volume *v, const double o[3], const double d[3], double tmin,
+    double tmax, double *t0, double *t1, int *amb);
+
 static int vol_slab(const orc_volume *v, const double o[3], const double d[3], double tmin,
     double tmax, double *t0, double *t1)
+{
+  return vol_slab_amb(v, o, d, tmin, tmax, t0, t1, NULL);
+}
+
+static int vol_slab_amb(const orc_volume *v, const double o[3], const double d[3], double tmin,
+    double tmax, double *t0, double *t1, int *amb)
 {
   double tn = -INFINITY, tf = INFINITY;
   for (int k = 0; k < 3; k++) {
@@ -1108,6 +1130,8 @@ static int vol_slab(const orc_volume *v, const double o[3], const double d[3], d
     tn = n > tn ? n : tn;
     tf = f < tf ? f : tf;
   }
+  if (amb && (near_rel(tn, tf) || near_rel(tf, tmin) || near_rel(tn, tmax)))
+    *amb = 1;
   if (!(tn < tf))
     return 0;
   if (tf < tmin || tn > tmax)
@@ -1117,12 +1141,24 @@ static int vol_slab(const orc_volume *v, const double o[3], const double d[3], d
   return 1;
 }
 
+static double march_amb(const orc_volume *v, const double org[3], const double dir[3],
+    double tmin, double tmax, double inv_sampling_rate, int early_stop, double color[3],
+    double *opacity, uint64_t *nsamples, int *amb);
+
 double orc_volume_march(const orc_volume *v, const double org[3], const double dir[3],
     double tmin, double tmax, double inv_sampling_rate, int early_stop, double color[3],
     double *opacity, uint64_t *nsamples)
 {
+  return march_amb(v, org, dir, tmin, tmax, inv_sampling_rate, early_stop, color, opacity,
+      nsamples, NULL);
+}
+
+static double march_amb(const orc_volume *v, const double org[3], const double dir[3],
+    double tmin, double tmax, double inv_sampling_rate, int early_stop, double color[3],
+    double *opacity, uint64_t *nsamples, int *amb)
+{
   double t0, t1;
-  if (!vol_slab(v, org, dir, tmin, tmax, &t0, &t1))
+  if (!vol_slab_amb(v, org, dir, tmin, tmax, &t0, &t1, amb))
     return tmax;
   float ss = fminf(fminf(v->spacing[0] / 2.0f, v->spacing[1] / 2.0f), v->spacing[2] / 2.0f);
   double step = (double)ss * inv_sampling_rate;
@@ -1132,8 +1168,12 @@ double orc_volume_march(const orc_volume *v, const double org[3], const double d
     return t0;
   for (uint64_t k = 0; k < MAX_SAMPLES_PER_SEGMENT; k++) {
     double t = t0 + (double)k * step;
+    if (amb && near_rel(t, t1))
+      *amb = 1;
     if (!(t <= t1))
       break;
+    if (amb && early_stop && fabs(*opacity - 0.99) < AMB_OPACITY)
+      *amb = 1;
     if (early_stop && !(*opacity < 0.99))
       break;
     double p[3] = {org[0] + dir[0] * t, org[1] + dir[1] * t, org[2] + dir[2] * t};
@@ -1156,7 +1196,8 @@ double orc_volume_march(const orc_volume *v, const double org[3], const double d
 }
 
 void orc_volume_image(const orc_volume *v, const orc_camera *cam, uint32_t w, uint32_t h,
-    double inv_sampling_rate, int early_stop, double *rgba_out, uint64_t *nsamples, int32_t threads)
+    double inv_sampling_rate, int early_stop, double *rgba_out, uint64_t *nsamples, int32_t threads,
+    uint8_t *ambiguous)
 {
   cam_x cx = cam_setup(cam);
   float iw = 1.0f / (float)w, ih = 1.0f / (float)h;
@@ -1173,8 +1214,11 @@ void orc_volume_image(const orc_volume *v, const orc_camera *cam, uint32_t w, ui
       double od[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
       double c[3] = {0, 0, 0}, op = 0.0;
       uint64_t ns = 0;
-      orc_volume_march(v, od, dd, 0.0, (double)FLT_MAX, inv_sampling_rate, early_stop, c, &op, &ns);
+      int amb = 0;
+      march_amb(v, od, dd, 0.0, (double)FLT_MAX, inv_sampling_rate, early_stop, c, &op, &ns, &amb);
       total += ns;
+      if (ambiguous)
+        ambiguous[(size_t)y * w + x] = (uint8_t)amb;
       double *out = rgba_out + 4 * ((size_t)y * w + x);
       out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = op;
     }
@@ -1290,7 +1334,8 @@ static float eps_from(v3 P, v3 dir, float t)
 
 /* rayMarchAllVolumes (volumeIntegration.h:317-350), marching in FP64. */
 static float march_all(const orc_scene *s, v3 o, v3 d, float tlower, float tfar, int early,
-    float inv_rate, v3 *color, float *opacity, uint32_t *obj, uint32_t *inst, uint64_t *ns)
+    float inv_rate, v3 *color, float *opacity, uint32_t *obj, uint32_t *inst, uint64_t *ns,
+    int *amb)
 {
   float depth = tfar;
   int first = 1;
@@ -1306,7 +1351,7 @@ static float march_all(const orc_scene *s, v3 o, v3 d, float tlower, float tfar,
       if ((int)i == last)
         continue;
       double t0, t1;
-      if (vol_slab(&s->volumes[i], od, dd, tl, (double)tfar, &t0, &t1)) {
+      if (vol_slab_amb(&s->volumes[i], od, dd, tl, (double)tfar, &t0, &t1, amb)) {
         if (best < 0 || t0 < bt0) {
           best = (int)i;
           bt0 = t0;
@@ -1323,7 +1368,7 @@ static float march_all(const orc_scene *s, v3 o, v3 d, float tlower, float tfar,
     }
     if ((float)bt0 < depth)
       depth = (float)bt0;
-    orc_volume_march(&s->volumes[best], od, dd, bt0, bt1, (double)inv_rate, early, c, &op, ns);
+    march_amb(&s->volumes[best], od, dd, bt0, bt1, (double)inv_rate, early, c, &op, ns, amb);
     tl = bt1 + 1e-3;
     last = best;
     if (!(op < 0.99))
@@ -1414,6 +1459,7 @@ int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r,
       v4 accum = {mk(0, 0, 0), 0.0f};
       float dbuf = FLT_MAX;
       uint32_t pid = 0, oid = 0, iid = 0;
+      int amb = 0;
       uint64_t st[2] = {0, 0};
       for (int i = 0; i < spp; i++) {
         v3 o, d;
@@ -1440,7 +1486,7 @@ int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r,
           if (h.found) {
             uint32_t vo = ~0u, vi = ~0u;
             float vdepth = march_all(s, o, d, tlower, h.t, 1, inv_rate, &color, &opacity, &vo,
-                &vi, &s_vs);
+                &vi, &s_vs, &amb);
             if (first) {
               if (vdepth < h.t) {
                 depth = vdepth;
@@ -1495,7 +1541,7 @@ int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r,
           } else {
             uint32_t vo = ~0u, vi = ~0u;
             float vdepth = march_all(s, o, d, tlower, tmax, 1, inv_rate, &color, &opacity, &vo,
-                &vi, &s_vs);
+                &vi, &s_vs, &amb);
             if (first) {
               depth = fminf(depth, vdepth);
               prim = 0;
@@ -1540,6 +1586,8 @@ int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r,
           }
         }
       }
+      if (f->ambiguous)
+        f->ambiguous[idx] = (uint8_t)amb;
       if (f->depth)
         f->depth[idx] = dbuf;
       if (f->prim_id)

@@ -163,6 +163,8 @@ typedef struct {
   uint64_t *stats;              /* optional [0]=internal nodes visited,[1]=prims tested,
                                    [2]=rays traced, [3]=volume samples */
   int32_t threads;              /* OpenMP threads (0 = default) */
+  uint8_t *ambiguous;           /* optional: 1 where a volume sample count sits on an fp32
+                                   decision boundary (see oracle.c AMB_*) */
 } orc_frame;
 
 int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r, orc_frame *f);
@@ -209,7 +211,7 @@ double orc_volume_march(const orc_volume *v, const double org[3], const double d
 /* Whole image of the plain volume integral (C rgb, O) per pixel, FP64. */
 void orc_volume_image(const orc_volume *v, const orc_camera *cam, uint32_t w, uint32_t h,
     double inv_sampling_rate, int early_stop, double *rgba_out, uint64_t *nsamples,
-    int32_t threads);
+    int32_t threads, uint8_t *ambiguous);
 
 /* TF discretisation to 256 RGBA entries. color: ncolor*ccomp floats (3 or 4 comp),
    opacity: nopacity floats (or NULL for uniform). */

@@ -67,7 +67,7 @@ class orc_frame(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("color_format", C.c_int32),
                 ("color_f32", vp), ("color_u32", vp), ("depth", vp), ("prim_id", vp),
                 ("obj_id", vp), ("inst_id", vp), ("frame_id", C.c_uint32), ("stats", vp),
-                ("threads", C.c_int32)]
+                ("threads", C.c_int32), ("ambiguous", vp)]
 
 
 class orc_hit(C.Structure):
@@ -123,7 +123,7 @@ def lib() -> C.CDLL:
         L.orc_intersect_cylinder.argtypes = [vp, vp, vp, vp, C.c_float, C.c_int, vp, vp, vp]
         L.orc_camera_ray.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, vp, vp]
         L.orc_volume_image.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_double, C.c_int, vp,
-                                       vp, C.c_int32]
+                                       vp, C.c_int32, vp]
         L.orc_tf_discretize.argtypes = [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, vp,
                                         C.c_float, vp, vp]
         L.orc_diffvol_forward.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, C.c_double,
@@ -359,6 +359,8 @@ class OracleScene:
         out["objectId"] = np.zeros((height, width), np.uint32)
         out["instanceId"] = np.zeros((height, width), np.uint32)
         stats = np.zeros(4, np.uint64)
+        out["ambiguous"] = np.zeros((height, width), np.uint8)
+        f.ambiguous = out["ambiguous"].ctypes.data
         f.depth = out["depth"].ctypes.data
         f.prim_id = out["primitiveId"].ctypes.data
         f.obj_id = out["objectId"].ctypes.data
@@ -415,10 +417,13 @@ def volume_image(vol_desc, cam, width, height, inv_sampling_rate=1.0, early_stop
     v.value_range[:] = list(vol_desc.value_range)
     v.unit_distance = vol_desc.unit_distance
     out = np.zeros((height, width, 4), np.float64)
+    amb = np.zeros((height, width), np.uint8)
     ns = C.c_uint64()
     c = camera(cam)
     lib().orc_volume_image(C.byref(v), C.byref(c), width, height, inv_sampling_rate,
-                           int(early_stop), out.ctypes.data, C.byref(ns), threads)
+                           int(early_stop), out.ctypes.data, C.byref(ns), threads,
+                           amb.ctypes.data)
+    volume_image.last_ambiguous = amb
     return out, ns.value
 
 

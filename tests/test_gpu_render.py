@@ -188,7 +188,9 @@ def test_volume_raycast_vs_fp64_oracle(require_gpu):
     g = render_gpu(sc, cam, 200, 120, volumeSamplingRate=1.0)
     o = orc.OracleScene(sc).render(cam, 200, 120, volumeSamplingRate=1.0)
     assert not g["errors"], g["errors"]
-    np.testing.assert_allclose(g["color"], o["color"], rtol=0, atol=1e-4)
+    ok = o["ambiguous"] == 0
+    assert ok.mean() > 0.99, f"ambiguous fraction {1 - ok.mean():.4f}"
+    np.testing.assert_allclose(g["color"][ok], o["color"][ok], rtol=0, atol=1e-4)
     np.testing.assert_array_equal(g["primitiveId"], o["primitiveId"])
     np.testing.assert_array_equal(g["objectId"], o["objectId"])
     np.testing.assert_array_equal(g["instanceId"], o["instanceId"])
@@ -207,7 +209,9 @@ def test_volume_with_surface_in_front_and_inside(require_gpu):
     g = render_gpu(sc, cam, 160, 100, volumeSamplingRate=0.5)
     o = orc.OracleScene(sc).render(cam, 160, 100, volumeSamplingRate=0.5)
     assert not g["errors"], g["errors"]
-    np.testing.assert_allclose(g["color"], o["color"], rtol=0, atol=1e-4)
+    ok = o["ambiguous"] == 0
+    assert ok.mean() > 0.99, f"ambiguous fraction {1 - ok.mean():.4f}"
+    np.testing.assert_allclose(g["color"][ok], o["color"][ok], rtol=0, atol=1e-4)
     np.testing.assert_array_equal(g["primitiveId"], o["primitiveId"])
     np.testing.assert_array_equal(g["objectId"], o["objectId"])
     np.testing.assert_allclose(g["depth"], o["depth"], rtol=1e-5)

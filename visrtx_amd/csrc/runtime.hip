@@ -8,6 +8,8 @@
 #include <stdarg.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "vrx_internal.h"
 
 namespace vrx {
@@ -34,6 +36,14 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride)
     p[i] = v;
+}
+
+__global__ void k_copy_u32(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
+{
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n16; i += stride)
+    dst[i] = src[i];
 }
 
 } // namespace vrx
@@ -148,6 +158,20 @@ int vrx_fill_u32(void *d_ptr, uint32_t value, size_t count, vrx_stream s)
     blocks = 65536;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s,
       (uint32_t *)d_ptr, value, count);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
+}
+
+int vrx_copy_kernel(void *d_dst, const void *d_src, size_t bytes, vrx_stream s)
+{
+  VRX_ARG(d_dst && d_src, "dst/src");
+  VRX_ARG(bytes % 16 == 0, "bytes must be a multiple of 16");
+  size_t n16 = bytes / 16;
+  if (n16 == 0)
+    return VRX_OK;
+  size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(k_copy_u32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s,
+      (const uint4 *)d_src, (uint4 *)d_dst, n16);
   VRX_LAUNCH_CHECK();
   return VRX_OK;
 }
