@@ -6,10 +6,13 @@
 //   1. primitive bounds -> centroid bounds (exact min/max reduction)
 //   2. 30-bit Morton codes (10 bits per axis)
 //   3. stable LSD radix sort of (code, index), 4 passes of 8 bits; each pass
-//      is histogram -> single-workgroup spine scan -> LDS-staged scatter with
-//      wave-ballot stable ranking (tile = 256 lanes x 16 keys)
+//      is histogram -> per-digit row scan (256 workgroups) -> LDS-staged
+//      scatter with wave-ballot stable ranking (tile = 256 lanes x 16 keys)
 //   4. Karras (HPG 2012) hierarchy, ties broken with index-augmented keys
-//   5. bottom-up refit with per-node arrival counters (acq_rel atomics)
+//   5. refit without atomics: nodes are bucketed by the common-prefix length
+//      of their key range (a child's is always longer than its parent's) and
+//      refit one length at a time, longest first, one kernel per length, so
+//      every node is written once by one thread with plain stores
 // Bounds inputs follow scene/surface/geometry/Sphere.cu:82-110 (c +- r),
 // Cylinder.cpp:53-102 (min/max(v0,v1) +- |r|) and Triangle.cpp:112-138.
 //
@@ -34,10 +37,15 @@ static uint32_t sort_blocks(uint32_t n) { return (n + kSortTile - 1) / kSortTile
 
 struct Workspace {
   uint32_t *keys_a, *vals_a, *keys_b, *vals_b;
-  uint32_t *hist;
-  uint32_t *cbox;   // 6 ordered-uint floats
-  uint32_t *parent; // 2n-1
-  uint32_t *flags;  // n-1
+  uint32_t *hist;      // 256 x nblocks, digit-major
+  uint32_t *row_total; // 256
+  uint32_t *cbox;      // 6 ordered-uint floats
+  uint2 *child;        // n-1 child pairs from the hierarchy pass
+  uint8_t *level;      // n-1 common-prefix lengths (0..63)
+  uint32_t *bucket;    // n-1 internal nodes grouped by prefix length
+  uint32_t *lcount;    // 64 nodes per prefix length
+  uint32_t *loff;      // 64 bucket offsets
+  uint32_t *lfill;     // 64 bucket fill counters
 };
 
 static size_t workspace_layout(uint32_t n, void *base, Workspace *w)
@@ -56,9 +64,14 @@ static size_t workspace_layout(uint32_t n, void *base, Workspace *w)
   ws.keys_b = (uint32_t *)take(4 * nn);
   ws.vals_b = (uint32_t *)take(4 * nn);
   ws.hist = (uint32_t *)take(4 * (size_t)256 * sort_blocks((uint32_t)nn));
+  ws.row_total = (uint32_t *)take(4 * 256);
   ws.cbox = (uint32_t *)take(4 * 8);
-  ws.parent = (uint32_t *)take(4 * (2 * nn));
-  ws.flags = (uint32_t *)take(4 * nn);
+  ws.child = (uint2 *)take(8 * nn);
+  ws.level = (uint8_t *)take(nn);
+  ws.bucket = (uint32_t *)take(4 * nn);
+  ws.lcount = (uint32_t *)take(4 * 64);
+  ws.loff = (uint32_t *)take(4 * 64);
+  ws.lfill = (uint32_t *)take(4 * 64);
   if (w)
     *w = ws;
   return off;
@@ -255,20 +268,22 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(
   hist[(size_t)d * nblocks + blockIdx.x] = s;
 }
 
-// Exclusive scan, in place, of len u32 values by one workgroup.
-__global__ __launch_bounds__(kScanThreads) void k_scan_single(uint32_t *data, uint32_t len)
+// Row scan of the digit-major histogram: workgroup d turns row d (one count
+// per sort block) into exclusive block offsets within the digit and writes
+// the digit total.  The digit bases are scanned inside the scatter kernel.
+__global__ __launch_bounds__(256) void k_scan_rows(uint32_t *hist, uint32_t nblocks, uint32_t *row_total)
 {
-  __shared__ uint32_t sums[kScanThreads];
-  const uint32_t chunk = (len + kScanThreads - 1) / kScanThreads;
-  const uint32_t beg = min(len, threadIdx.x * chunk);
-  const uint32_t end = min(len, beg + chunk);
+  __shared__ uint32_t sums[256];
+  uint32_t *row = hist + (size_t)blockIdx.x * nblocks;
+  const uint32_t chunk = (nblocks + 255) / 256;
+  const uint32_t beg = min(nblocks, threadIdx.x * chunk);
+  const uint32_t end = min(nblocks, beg + chunk);
   uint32_t s = 0;
   for (uint32_t i = beg; i < end; i++)
-    s += data[i];
+    s += row[i];
   sums[threadIdx.x] = s;
   __syncthreads();
-  // Hillis-Steele inclusive scan over 1024 sums
-  for (int off = 1; off < kScanThreads; off <<= 1) {
+  for (int off = 1; off < 256; off <<= 1) {
     uint32_t v = (int)threadIdx.x >= off ? sums[threadIdx.x - off] : 0u;
     __syncthreads();
     sums[threadIdx.x] += v;
@@ -276,19 +291,22 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_single(uint32_t *data, ui
   }
   uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
   for (uint32_t i = beg; i < end; i++) {
-    uint32_t v = data[i];
-    data[i] = run;
+    uint32_t v = row[i];
+    row[i] = run;
     run += v;
   }
+  if (threadIdx.x == 255)
+    row_total[blockIdx.x] = sums[255];
 }
 
 // Stable LDS-staged scatter.  Element order inside a tile is (item k, wave,
 // lane), which is the original index order, so ranks are stable.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *keys_in,
     const uint32_t *vals_in, uint32_t *keys_out, uint32_t *vals_out, uint32_t n, uint32_t shift,
-    uint32_t nblocks, const uint32_t *offsets)
+    uint32_t nblocks, const uint32_t *offsets, const uint32_t *row_total)
 {
   constexpr int W = kSortThreads / 64;
+  __shared__ uint32_t digit_base[256];
   __shared__ uint32_t digit_total[256];
   __shared__ uint32_t round_cnt[W][256];
   __shared__ uint32_t digit_start[256];
@@ -305,6 +323,24 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *
 #pragma unroll
   for (int w = 0; w < W; w++)
     round_cnt[w][tid] = 0u;
+  // exclusive scan of the digit totals: global start of each digit
+  if (tid < 64) {
+    uint32_t c0 = row_total[4 * tid], c1 = row_total[4 * tid + 1];
+    uint32_t c2 = row_total[4 * tid + 2], c3 = row_total[4 * tid + 3];
+    uint32_t sm = c0 + c1 + c2 + c3;
+    uint32_t incl = sm;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t v = __shfl_up(incl, off, 64);
+      if (lane >= off)
+        incl += v;
+    }
+    uint32_t ex = incl - sm;
+    digit_base[4 * tid] = ex;
+    digit_base[4 * tid + 1] = ex + c0;
+    digit_base[4 * tid + 2] = ex + c0 + c1;
+    digit_base[4 * tid + 3] = ex + c0 + c1 + c2;
+  }
 
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
 #pragma unroll
@@ -388,7 +424,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *
   for (uint32_t i = tid; i < valid; i += kSortThreads) {
     uint32_t kk = stage_k[i];
     uint32_t d = (kk >> shift) & 255u;
-    uint32_t pos = offsets[(size_t)d * nblocks + blockIdx.x] + (i - digit_start[d]);
+    uint32_t pos = digit_base[d] + offsets[(size_t)d * nblocks + blockIdx.x] + (i - digit_start[d]);
     keys_out[pos] = kk;
     vals_out[pos] = stage_v[i];
   }
@@ -408,52 +444,92 @@ __device__ __forceinline__ int kdelta(const uint32_t *c, int64_t n, int64_t i, i
 }
 
 __global__ __launch_bounds__(256) void k_karras(
-    const uint32_t *sc, uint32_t n_, Node *nodes, uint32_t *parent, uint32_t *flags)
+    const uint32_t *sc, uint32_t n_, uint2 *child, uint8_t *level, uint32_t *lcount)
 {
+  __shared__ uint32_t hist[64];
+  if (threadIdx.x < 64)
+    hist[threadIdx.x] = 0u;
+  __syncthreads();
   const int64_t n = n_;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1)
-    return;
-  int d = (kdelta(sc, n, i, i + 1) - kdelta(sc, n, i, i - 1)) > 0 ? 1 : -1;
-  int dmin = kdelta(sc, n, i, i - d);
-  int64_t lmax = 2;
-  while (kdelta(sc, n, i, i + lmax * d) > dmin)
-    lmax <<= 1;
-  int64_t l = 0;
-  for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
-    if (kdelta(sc, n, i, i + (l + t) * d) > dmin)
-      l += t;
-  int64_t j = i + l * d;
-  int dnode = kdelta(sc, n, i, j);
-  int64_t s = 0, t = l;
-  do {
-    t = (t + 1) >> 1;
-    if (kdelta(sc, n, i, i + (s + t) * d) > dnode)
-      s += t;
-  } while (t > 1);
-  int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
-  int64_t lo = i < j ? i : j, hi = i < j ? j : i;
-  uint32_t left = (lo == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
-  uint32_t right = (hi == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
-  nodes[i].d = make_uint4(left, right, 0u, 0u);
-  parent[(left & kLeafBit) ? (uint32_t)(n - 1) + (left & ~kLeafBit) : left] = (uint32_t)i;
-  parent[(right & kLeafBit) ? (uint32_t)(n - 1) + (right & ~kLeafBit) : right] = (uint32_t)i;
-  flags[i] = 0u;
-  if (i == 0)
-    parent[0] = 0xFFFFFFFFu;
+  if (i < n - 1) {
+    int d = (kdelta(sc, n, i, i + 1) - kdelta(sc, n, i, i - 1)) > 0 ? 1 : -1;
+    int dmin = kdelta(sc, n, i, i - d);
+    int64_t lmax = 2;
+    while (kdelta(sc, n, i, i + lmax * d) > dmin)
+      lmax <<= 1;
+    int64_t l = 0;
+    for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+      if (kdelta(sc, n, i, i + (l + t) * d) > dmin)
+        l += t;
+    int64_t j = i + l * d;
+    int dnode = kdelta(sc, n, i, j);
+    int64_t s = 0, t = l;
+    do {
+      t = (t + 1) >> 1;
+      if (kdelta(sc, n, i, i + (s + t) * d) > dnode)
+        s += t;
+    } while (t > 1);
+    int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+    int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    uint32_t left = (lo == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
+    uint32_t right = (hi == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    child[i] = make_uint2(left, right);
+    // common-prefix length of the node's key range: every child range has a
+    // strictly longer one, so refitting in decreasing prefix length visits
+    // children before parents
+    uint32_t lv = (uint32_t)min(max(dnode, 0), 63);
+    level[i] = (uint8_t)lv;
+    atomicAdd(&hist[lv], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64 && hist[threadIdx.x])
+    atomicAdd(&lcount[threadIdx.x], hist[threadIdx.x]);
+}
+
+__global__ void k_level_offsets(const uint32_t *lcount, uint32_t *loff, uint32_t *lfill)
+{
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int l = 0; l < 64; l++) {
+      loff[l] = run;
+      run += lcount[l];
+      lfill[l] = 0u;
+    }
+  }
+}
+
+// group internal nodes by prefix length (block-aggregated reservations)
+__global__ __launch_bounds__(256) void k_level_scatter(
+    const uint8_t *level, uint32_t n_internal, const uint32_t *loff, uint32_t *lfill, uint32_t *bucket)
+{
+  __shared__ uint32_t cnt[64], base[64];
+  if (threadIdx.x < 64)
+    cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t lv = 0, r = 0;
+  if (i < n_internal) {
+    lv = level[i];
+    r = atomicAdd(&cnt[lv], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64 && cnt[threadIdx.x])
+    base[threadIdx.x] = loff[threadIdx.x] + atomicAdd(&lfill[threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (i < n_internal)
+    bucket[base[lv] + r] = i;
 }
 
 template <int T>
 __device__ __forceinline__ void write_leaf(
-    const vrx_prims &p, uint32_t orig, uint32_t k, void *leaves, v3 &lo, v3 &hi)
+    const vrx_prims &p, uint32_t orig, uint32_t k, void *leaves)
 {
   if constexpr (T == VRX_SPHERE) {
     uint32_t vi = p.d_index ? p.d_index[orig] : orig;
     v3 c = ld3(p.d_position + 3 * (size_t)vi);
     float r = p.d_radius ? p.d_radius[vi] : p.global_radius;
     ((float4 *)leaves)[k] = make_float4(c.x, c.y, c.z, r);
-    lo = mk(c.x - r, c.y - r, c.z - r);
-    hi = mk(c.x + r, c.y + r, c.z + r);
   } else if constexpr (T == VRX_CYLINDER) {
     uint32_t a = p.d_index ? p.d_index[2 * (size_t)orig] : 2 * orig;
     uint32_t b = p.d_index ? p.d_index[2 * (size_t)orig + 1] : 2 * orig + 1;
@@ -462,9 +538,6 @@ __device__ __forceinline__ void write_leaf(
     float4 *L = (float4 *)leaves + 2 * (size_t)k;
     L[0] = make_float4(p0.x, p0.y, p0.z, r);
     L[1] = make_float4(p1.x, p1.y, p1.z, 0.0f);
-    v3 mn = vmin(p0, p1), mx = vmax(p0, p1);
-    lo = mk(mn.x - r, mn.y - r, mn.z - r);
-    hi = mk(mx.x + r, mx.y + r, mx.z + r);
   } else if constexpr (T == VRX_TRIANGLE) {
     uint32_t a = p.d_index ? p.d_index[3 * (size_t)orig] : 3 * orig;
     uint32_t b = p.d_index ? p.d_index[3 * (size_t)orig + 1] : 3 * orig + 1;
@@ -475,10 +548,81 @@ __device__ __forceinline__ void write_leaf(
     L[0] = make_float4(v0.x, v0.y, v0.z, 0.0f);
     L[1] = make_float4(v1.x, v1.y, v1.z, 0.0f);
     L[2] = make_float4(v2.x, v2.y, v2.z, 0.0f);
+  }
+}
+
+// box of sorted leaf k, from its leaf record (same bounds rule as prim_box)
+template <int T>
+__device__ __forceinline__ void leaf_box(
+    const vrx_prims &p, const void *leaves, const uint32_t *ids, uint32_t k, v3 &lo, v3 &hi)
+{
+  if constexpr (T == VRX_SPHERE) {
+    float4 L = ((const float4 *)leaves)[k];
+    lo = mk(L.x - L.w, L.y - L.w, L.z - L.w);
+    hi = mk(L.x + L.w, L.y + L.w, L.z + L.w);
+  } else if constexpr (T == VRX_CYLINDER) {
+    const float4 *L = (const float4 *)leaves + 2 * (size_t)k;
+    float4 a = L[0], b = L[1];
+    v3 mn = vmin(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z)), mx = vmax(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
+    lo = mk(mn.x - a.w, mn.y - a.w, mn.z - a.w);
+    hi = mk(mx.x + a.w, mx.y + a.w, mx.z + a.w);
+  } else if constexpr (T == VRX_TRIANGLE) {
+    const float4 *L = (const float4 *)leaves + 3 * (size_t)k;
+    float4 a = L[0], b = L[1], c = L[2];
+    v3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
     lo = vmin(vmin(v0, v1), v2);
     hi = vmax(vmax(v0, v1), v2);
   } else {
-    prim_box<T>(p, orig, lo, hi);
+    prim_box<T>(p, ids[k], lo, hi);
+  }
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_leaves(
+    vrx_prims p, const uint32_t *order, void *leaves, uint32_t *leaf_ids)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.n)
+    return;
+  const uint32_t orig = order[k];
+  leaf_ids[k] = orig;
+  write_leaf<T>(p, orig, k, leaves);
+}
+
+// One prefix-length level of the refit: every node of the level reads its
+// children (leaf records, or internal nodes finished by earlier levels --
+// kernel boundaries make them visible) and writes its whole 64-byte node.
+template <int T>
+__global__ __launch_bounds__(256) void k_refit_level(vrx_prims p, const void *leaves,
+    const uint32_t *ids, const uint2 *child, const uint32_t *bucket, const uint32_t *loff,
+    const uint32_t *lcount, int lv, Node *nodes)
+{
+  const uint32_t cnt = lcount[lv];
+  const uint32_t beg = loff[lv];
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+    const uint32_t node = bucket[beg + q];
+    const uint2 ch = child[node];
+    v3 lo[2], hi[2];
+    uint32_t h[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const uint32_t c = s ? ch.y : ch.x;
+      if (c & kLeafBit) {
+        leaf_box<T>(p, leaves, ids, c & ~kLeafBit, lo[s], hi[s]);
+        h[s] = 1u;
+      } else {
+        const Node C = nodes[c];
+        lo[s] = vmin(mk(C.a.x, C.a.y, C.a.z), mk(C.b.z, C.b.w, C.c.x));
+        hi[s] = vmax(mk(C.a.w, C.b.x, C.b.y), mk(C.c.y, C.c.z, C.c.w));
+        h[s] = 1u + max(C.d.z, C.d.w);
+      }
+    }
+    Node N;
+    N.a = make_float4(lo[0].x, lo[0].y, lo[0].z, hi[0].x);
+    N.b = make_float4(hi[0].y, hi[0].z, lo[1].x, lo[1].y);
+    N.c = make_float4(lo[1].z, hi[1].x, hi[1].y, hi[1].z);
+    N.d = make_uint4(ch.x, ch.y, h[0], h[1]);
+    nodes[node] = N;
   }
 }
 
@@ -494,60 +638,33 @@ __device__ __forceinline__ void write_header(
 }
 
 template <int T>
-__global__ __launch_bounds__(256) void k_leaves_refit(vrx_prims p, const uint32_t *order,
-    Node *nodes, void *leaves, uint32_t *leaf_ids, const uint32_t *parent, uint32_t *flags,
-    vrx_bvh_header *header)
+__global__ void k_root_header(vrx_prims p, const void *leaves, const uint32_t *ids,
+    const Node *nodes, vrx_bvh_header *header)
 {
-  const uint32_t n = p.n;
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n)
+  if (threadIdx.x != 0)
     return;
-  const uint32_t orig = order[k];
-  leaf_ids[k] = orig;
-  v3 lo, hi;
-  write_leaf<T>(p, orig, k, leaves, lo, hi);
-  if (n == 1) {
-    write_header(header, lo, hi, 1u, kLeafBit, n, T);
+  if (p.n == 1) {
+    v3 lo, hi;
+    leaf_box<T>(p, leaves, ids, 0u, lo, hi);
+    write_header(header, lo, hi, 1u, kLeafBit, 1u, T);
     return;
   }
-  uint32_t cur = kLeafBit | k;
-  uint32_t pidx = parent[(n - 1) + k];
-  uint32_t height = 1;
-  for (;;) {
-    Node *P = nodes + pidx;
-    const uint32_t c0 = P->d.x;
-    float *f = (float *)P;
-    if (c0 == cur) {
-      f[0] = lo.x; f[1] = lo.y; f[2] = lo.z; f[3] = hi.x; f[4] = hi.y; f[5] = hi.z;
-      ((uint32_t *)f)[14] = height;
-    } else {
-      f[6] = lo.x; f[7] = lo.y; f[8] = lo.z; f[9] = hi.x; f[10] = hi.y; f[11] = hi.z;
-      ((uint32_t *)f)[15] = height;
-    }
-    uint32_t old = __hip_atomic_fetch_add(&flags[pidx], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0u)
-      return; // sibling subtree not finished; its last thread continues
-    // second arrival: both child slots are complete
-    const float4 a = P->a, b = P->b, c = P->c;
-    const uint4 dd = P->d;
-    v3 lo0 = mk(a.x, a.y, a.z), hi0 = mk(a.w, b.x, b.y);
-    v3 lo1 = mk(b.z, b.w, c.x), hi1 = mk(c.y, c.z, c.w);
-    lo = vmin(lo0, lo1);
-    hi = vmax(hi0, hi1);
-    height = 1u + max(dd.z, dd.w);
-    cur = pidx;
-    if (pidx == 0u) {
-      write_header(header, lo, hi, height, 0u, n, T);
-      return;
-    }
-    pidx = parent[pidx];
-  }
+  const Node C = nodes[0];
+  v3 lo = vmin(mk(C.a.x, C.a.y, C.a.z), mk(C.b.z, C.b.w, C.c.x));
+  v3 hi = vmax(mk(C.a.w, C.b.x, C.b.y), mk(C.c.y, C.c.z, C.c.w));
+  write_header(header, lo, hi, 1u + max(C.d.z, C.d.w), 0u, p.n, T);
 }
 
 __global__ void k_empty_header(vrx_bvh_header *h, int type)
 {
   write_header(h, mk(INFINITY, INFINITY, INFINITY), mk(-INFINITY, -INFINITY, -INFINITY), 0u,
       kLeafBit, 0u, type);
+}
+
+__global__ void k_fill_zero64(uint32_t *p)
+{
+  if (threadIdx.x < 64)
+    p[threadIdx.x] = 0u;
 }
 
 __global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint32_t n)
@@ -574,9 +691,9 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
   for (int pass = 0; pass < 4; pass++) {
     uint32_t shift = 8u * (uint32_t)pass;
     hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ki, n, shift, nb, ws.hist);
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, s, ws.hist, 256u * nb);
+    hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(256), 0, s, ws.hist, nb, ws.row_total);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ki, vi, ko, vo, n,
-        shift, nb, ws.hist);
+        shift, nb, ws.hist, ws.row_total);
     uint32_t *t;
     t = ki; ki = ko; ko = t;
     t = vi; vi = vo; vo = t;
@@ -585,13 +702,23 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
   // after 4 passes the sorted data is back in keys_a/vals_a
   if (codes_out)
     hipLaunchKernelGGL(k_copy_u32, dim3(g256), dim3(256), 0, s, ki, codes_out, n);
+  hipLaunchKernelGGL((k_leaves<T>), dim3(g256), dim3(256), 0, s, p, vi, out.d_leaves, out.d_leaf_ids);
   Node *nodes = (Node *)out.d_nodes;
   if (n >= 2) {
-    unsigned gi = (n - 1 + 255) / 256;
-    hipLaunchKernelGGL(k_karras, dim3(gi), dim3(256), 0, s, ki, n, nodes, ws.parent, ws.flags);
+    const uint32_t ni = n - 1;
+    const unsigned gi = (ni + 255) / 256;
+    hipLaunchKernelGGL(k_fill_zero64, dim3(1), dim3(64), 0, s, ws.lcount);
+    hipLaunchKernelGGL(k_karras, dim3(gi), dim3(256), 0, s, ki, n, ws.child, ws.level, ws.lcount);
+    hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, ws.lcount, ws.loff, ws.lfill);
+    hipLaunchKernelGGL(k_level_scatter, dim3(gi), dim3(256), 0, s, ws.level, ni, ws.loff, ws.lfill,
+        ws.bucket);
+    const unsigned gl = min(gi, 4096u);
+    for (int lv = 63; lv >= 0; lv--)
+      hipLaunchKernelGGL((k_refit_level<T>), dim3(gl), dim3(256), 0, s, p, out.d_leaves,
+          out.d_leaf_ids, ws.child, ws.bucket, ws.loff, ws.lcount, lv, nodes);
   }
-  hipLaunchKernelGGL(k_leaves_refit<T>, dim3(g256), dim3(256), 0, s, p, vi, nodes, out.d_leaves,
-      out.d_leaf_ids, ws.parent, ws.flags, (vrx_bvh_header *)out.d_header);
+  hipLaunchKernelGGL((k_root_header<T>), dim3(1), dim3(64), 0, s, p, out.d_leaves, out.d_leaf_ids,
+      nodes, (vrx_bvh_header *)out.d_header);
   VRX_LAUNCH_CHECK();
   return VRX_OK;
 }
