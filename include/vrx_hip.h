@@ -176,6 +176,7 @@ typedef struct vrx_scene {
   const vrx_volume *d_volumes;  /* device array */
   uint32_t num_volumes;
   int32_t single_type;          /* prim type if every geom shares it, else -1 */
+  uint64_t max_volume_voxels;   /* largest field's voxel count (>= 2^31 selects 64-bit gathers) */
 } vrx_scene;
 
 typedef struct vrx_camera {

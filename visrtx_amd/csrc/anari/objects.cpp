@@ -957,6 +957,7 @@ int World::rebuild()
     }
   }
   std::vector<vrx_volume> hv;
+  uint64_t maxVox = 0;
   for (size_t i = 0; i < vols.size(); i++) {
     Volume *v = vols[i];
     vrx_volume vv{};
@@ -971,6 +972,7 @@ int World::rebuild()
     vv.id = v->id;
     vv.inst_id = volInst[i];
     hv.push_back(vv);
+    maxVox = std::max(maxVox, (uint64_t)vv.dims[0] * vv.dims[1] * vv.dims[2]);
     float lo[3], hi[3];
     v->field->bounds(lo, hi);
     for (int k = 0; k < 3; k++) {
@@ -999,6 +1001,7 @@ int World::rebuild()
   scene.num_volumes = (uint32_t)hv.size();
   scene.max_blas_height = maxH;
   scene.single_type = single >= 0 ? single : -1;
+  scene.max_volume_voxels = maxVox;
   if (he.size() > 1) {
     vrx_bvh_sizes sz;
     vrx_tlas_query_sizes((uint32_t)he.size(), &sz);

@@ -73,36 +73,80 @@ __device__ __forceinline__ bool vol_slab(
   return true;
 }
 
-// software trilinear at voxel coordinates (p - origin) / spacing, edge clamp
-// (the structured-regular field's clamp-to-edge linear texture,
-// StructuredRegularField.cpp:98-159, sampleSpatialField.h:54-78)
-__device__ __forceinline__ float sample_field(const vrx_volume &v, v3 p)
+// Per-volume constants of the march, computed once per ray segment.
+struct VolCtx {
+  const float *data;
+  int dims[3], imax[3];
+  float gmax[3];
+  float origin[3], inv_sp[3];
+  uint32_t sy, sz;
+  float step, expo, lo, hi, inv_range;
+};
+
+__device__ __forceinline__ float vol_step(const vrx_volume &v)
 {
-  float g[3] = {(p.x - v.origin[0]) / v.spacing[0], (p.y - v.origin[1]) / v.spacing[1],
-      (p.z - v.origin[2]) / v.spacing[2]};
+  // StructuredRegularField.cpp:175-178 stepSize() = min(spacing) / 2
+  return fminf(fminf(v.spacing[0] / 2.0f, v.spacing[1] / 2.0f), v.spacing[2] / 2.0f);
+}
+
+__device__ __forceinline__ VolCtx vol_ctx(const vrx_volume &v, float inv_rate)
+{
+  VolCtx c;
+  c.data = v.d_data;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    c.dims[k] = (int)v.dims[k];
+    c.imax[k] = max((int)v.dims[k] - 2, 0);
+    c.gmax[k] = (float)v.dims[k] - 1.0f;
+    c.origin[k] = v.origin[k];
+    c.inv_sp[k] = 1.0f / v.spacing[k];
+  }
+  c.sy = v.dims[0];
+  c.sz = v.dims[0] * v.dims[1];
+  c.step = vol_step(v) * inv_rate;
+  c.expo = c.step * (1.0f / v.unit_distance);
+  c.lo = v.value_range[0];
+  c.hi = v.value_range[1];
+  c.inv_range = 1.0f / (c.hi - c.lo);
+  return c;
+}
+
+// Software trilinear at voxel coordinates g = (p - origin) / spacing with edge
+// clamp: the structured-regular field's clamp-to-edge linear texture
+// (StructuredRegularField.cpp:98-159, sampleSpatialField.h:54-78) with full
+// fp32 weights.  I64 selects 64-bit gather offsets for fields of >= 2^31 voxels.
+template <bool I64>
+__device__ __forceinline__ float trilinear(const VolCtx &c, float gx, float gy, float gz)
+{
+  float g[3] = {gx, gy, gz};
   int i0[3], i1[3];
   float f[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    float gm = (float)v.dims[k] - 1.0f;
-    float x = fminf(fmaxf(g[k], 0.0f), gm);
-    int i = (int)x;
-    int imax = (int)v.dims[k] - 2;
-    if (imax < 0)
-      imax = 0;
-    i = min(i, imax);
+    float x = fminf(fmaxf(g[k], 0.0f), c.gmax[k]);
+    int i = min((int)x, c.imax[k]);
     i0[k] = i;
-    i1[k] = min(i + 1, (int)v.dims[k] - 1);
+    i1[k] = min(i + 1, c.dims[k] - 1);
     f[k] = x - (float)i;
   }
-  const size_t sx = 1, sy = v.dims[0], sz = (size_t)v.dims[0] * v.dims[1];
-  const float *D = v.d_data;
-  size_t bz0 = (size_t)i0[2] * sz, bz1 = (size_t)i1[2] * sz;
-  size_t by0 = (size_t)i0[1] * sy, by1 = (size_t)i1[1] * sy;
-  float c000 = D[bz0 + by0 + i0[0] * sx], c100 = D[bz0 + by0 + i1[0] * sx];
-  float c010 = D[bz0 + by1 + i0[0] * sx], c110 = D[bz0 + by1 + i1[0] * sx];
-  float c001 = D[bz1 + by0 + i0[0] * sx], c101 = D[bz1 + by0 + i1[0] * sx];
-  float c011 = D[bz1 + by1 + i0[0] * sx], c111 = D[bz1 + by1 + i1[0] * sx];
+  float c000, c100, c010, c110, c001, c101, c011, c111;
+  if constexpr (I64) {
+    size_t z0 = (size_t)i0[2] * c.sz, z1 = (size_t)i1[2] * c.sz;
+    size_t y0 = (size_t)i0[1] * c.sy, y1 = (size_t)i1[1] * c.sy;
+    const float *D = c.data;
+    c000 = D[z0 + y0 + i0[0]]; c100 = D[z0 + y0 + i1[0]];
+    c010 = D[z0 + y1 + i0[0]]; c110 = D[z0 + y1 + i1[0]];
+    c001 = D[z1 + y0 + i0[0]]; c101 = D[z1 + y0 + i1[0]];
+    c011 = D[z1 + y1 + i0[0]]; c111 = D[z1 + y1 + i1[0]];
+  } else {
+    uint32_t z0 = (uint32_t)i0[2] * c.sz, z1 = (uint32_t)i1[2] * c.sz;
+    uint32_t y0 = (uint32_t)i0[1] * c.sy, y1 = (uint32_t)i1[1] * c.sy;
+    const float *D = c.data;
+    c000 = D[z0 + y0 + i0[0]]; c100 = D[z0 + y0 + i1[0]];
+    c010 = D[z0 + y1 + i0[0]]; c110 = D[z0 + y1 + i1[0]];
+    c001 = D[z1 + y0 + i0[0]]; c101 = D[z1 + y0 + i1[0]];
+    c011 = D[z1 + y1 + i0[0]]; c111 = D[z1 + y1 + i1[0]];
+  }
   float c00 = c000 + (c100 - c000) * f[0];
   float c10 = c010 + (c110 - c010) * f[0];
   float c01 = c001 + (c101 - c001) * f[0];
@@ -114,10 +158,10 @@ __device__ __forceinline__ float sample_field(const vrx_volume &v, v3 p)
 
 // classifySample (volumeIntegration.h:46-62): LUT texel coordinate u*256-0.5,
 // linear, clamped (TransferFunction1D.cpp:152-185).
-__device__ __forceinline__ float4 classify(const float4 *lut, float lo, float hi, float s)
+__device__ __forceinline__ float4 classify(const float4 *lut, const VolCtx &c, float s)
 {
-  float c = fminf(fmaxf(s, lo), hi);
-  float u = (c - lo) * (1.0f / (hi - lo));
+  float cl = fminf(fmaxf(s, c.lo), c.hi);
+  float u = (cl - c.lo) * c.inv_range;
   float x = fminf(fmaxf(u * 256.0f - 0.5f, 0.0f), 255.0f);
   int i = min((int)x, 254);
   float f = x - (float)i;
@@ -126,41 +170,43 @@ __device__ __forceinline__ float4 classify(const float4 *lut, float lo, float hi
       a.w + (b.w - a.w) * f);
 }
 
-__device__ __forceinline__ float vol_step(const vrx_volume &v)
+// _rayMarchVolume (volumeIntegration.h:64-103), jitter off.  Sample k sits at
+// t0 + k*step (not an accumulated sum, which keeps positions within an ulp of
+// the exact ones); the per-step transmittance pow(1-a, step/unitDistance) is
+// evaluated as exp2(expo * log2(1-a)) with the hardware exp2/log2.
+template <bool I64>
+__device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, v3 d, float t0,
+    float t1, v3 &color, float &opacity, uint32_t &ns)
 {
-  return fminf(fminf(v.spacing[0] / 2.0f, v.spacing[1] / 2.0f), v.spacing[2] / 2.0f);
-}
-
-// _rayMarchVolume (volumeIntegration.h:64-103), jitter off.  Sample k is at
-// t0 + k*step (not an accumulated sum), which keeps positions within 1 ulp.
-__device__ void march(const vrx_volume &v, v3 o, v3 d, float t0, float t1, float inv_rate,
-    v3 &color, float &opacity, uint32_t &ns)
-{
-  const float step = vol_step(v) * inv_rate;
-  const float expo = step * (1.0f / v.unit_distance);
-  const float4 *lut = (const float4 *)v.d_lut;
-  const float lo = v.value_range[0], hi = v.value_range[1];
-  float T = 1.0f;
-  if (!(step > 0.0f))
+  if (!(c.step > 0.0f))
     return;
+  // voxel-space ray: g(t) = go + gd * t
+  const float gox = (o.x - c.origin[0]) * c.inv_sp[0], gdx = d.x * c.inv_sp[0];
+  const float goy = (o.y - c.origin[1]) * c.inv_sp[1], gdy = d.y * c.inv_sp[1];
+  const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
+  float T = 1.0f;
   for (uint32_t k = 0; k < kMaxSamplesPerSegment; k++) {
-    float t = t0 + (float)k * step;
+    const float t = t0 + (float)k * c.step;
     if (!(t <= t1) || !(opacity < 0.99f))
       break;
-    v3 p = mk(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
-    float s = sample_field(v, p);
+    const float s = trilinear<I64>(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
     ns++;
     if (s != s)
       continue;
-    float4 co = classify(lut, lo, hi, s);
-    float st = exp2f(expo * log2f(1.0f - co.w));
-    float w = T * (1.0f - st);
+    const float4 co = classify(lut, c, s);
+    const float st = __builtin_amdgcn_exp2f(c.expo * __builtin_amdgcn_logf(1.0f - co.w));
+    const float w = T * (1.0f - st);
     color.x += w * co.x;
     color.y += w * co.y;
     color.z += w * co.z;
     opacity += w;
     T *= st;
   }
+}
+
+__device__ __forceinline__ bool vol_is_large(const vrx_volume &v)
+{
+  return (uint64_t)v.dims[0] * v.dims[1] * v.dims[2] >= (1ull << 31);
 }
 
 // rayMarchAllVolumes (volumeIntegration.h:317-350)
@@ -195,7 +241,11 @@ __device__ float march_all(const vrx_scene &sc, v3 o, v3 d, float tlower, float 
       first = false;
     }
     depth = fminf(depth, bt0);
-    march(V, o, d, bt0, bt1, inv_rate, color, opacity, ns);
+    const VolCtx c = vol_ctx(V, inv_rate);
+    if (vol_is_large(V))
+      march<true>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
+    else
+      march<false>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
     tl = bt1 + 1e-3f;
     last = best;
     if (!(opacity < 0.99f))
@@ -233,6 +283,29 @@ __device__ __forceinline__ void write_color(const vrx_frame &f, size_t idx, floa
       f.d_color_u32[idx] = pack_unorm4x8(to_srgb(c.x), to_srgb(c.y), to_srgb(c.z), c.w);
     else
       f.d_color_u32[idx] = pack_unorm4x8(c.x, c.y, c.z, c.w);
+  }
+}
+
+// accumResults / writeOutputColor (gpu_util.h:356-443): the accumulated
+// tonemapped colour, its inverse tonemap, depth and (when depth improved) ids.
+__device__ __forceinline__ void write_pixel(const vrx_frame &f, size_t idx, float4 accum, int spp,
+    float dcur, bool ids_changed, uint32_t pid, uint32_t oid, uint32_t iid)
+{
+  ((float4 *)f.d_accum)[idx] = accum;
+  float div = (float)(f.frame_id + (uint32_t)spp);
+  float4 avg = make_float4(accum.x / div, accum.y / div, accum.z / div, accum.w / div);
+  float s2 = fmaxf(1e-12f, 1.0f - cmax3(mk(avg.x, avg.y, avg.z)));
+  float4 outv = make_float4(avg.x / s2, avg.y / s2, avg.z / s2, avg.w);
+  write_color(f, idx, outv);
+  if (f.d_depth)
+    f.d_depth[idx] = dcur;
+  if (ids_changed) {
+    if (f.d_prim_id)
+      f.d_prim_id[idx] = pid;
+    if (f.d_obj_id)
+      f.d_obj_id[idx] = oid;
+    if (f.d_inst_id)
+      f.d_inst_id[idx] = iid;
   }
 }
 
@@ -415,25 +488,82 @@ __global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam,
         ids_changed = true;
       }
     }
-    ((float4 *)f.d_accum)[idx] = accum;
-    float div = (float)(f.frame_id + (uint32_t)spp);
-    float4 avg = make_float4(accum.x / div, accum.y / div, accum.z / div, accum.w / div);
-    float s2 = fmaxf(1e-12f, 1.0f - cmax3(mk(avg.x, avg.y, avg.z)));
-    float4 outv = make_float4(avg.x / s2, avg.y / s2, avg.z / s2, avg.w);
-    write_color(f, idx, outv);
-    if (f.d_depth)
-      f.d_depth[idx] = dcur;
-    if (ids_changed) {
-      if (f.d_prim_id)
-        f.d_prim_id[idx] = pid;
-      if (f.d_obj_id)
-        f.d_obj_id[idx] = oid;
-      if (f.d_inst_id)
-        f.d_inst_id[idx] = iid;
-    }
+    write_pixel(f, idx, accum, spp, dcur, ids_changed, pid, oid, iid);
   }
   if (f.d_counters) {
     wave_count_add(f.d_counters, rays);
+    wave_count_add(f.d_counters + 1, vsamples);
+  }
+}
+
+// Raycast renderer on a world holding exactly one volume and no surfaces:
+// the same per-pixel program as k_render's miss branch (the march of
+// rayMarchAllVolumes, the renderer's colour-times-opacity, background), with
+// the transfer-function LUT staged in LDS and no traversal state, so the
+// kernel needs about half the registers of the general one.
+template <bool I64>
+__global__ __launch_bounds__(kBlock) void k_render_volume(vrx_scene sc, vrx_camera cam,
+    vrx_renderer r, vrx_frame f)
+{
+  __shared__ float4 s_lut[256];
+  const vrx_volume &V = sc.d_volumes[0];
+  s_lut[threadIdx.x] = ((const float4 *)V.d_lut)[threadIdx.x];
+  __syncthreads();
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const uint32_t px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
+  const uint32_t py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+  uint32_t vsamples = 0;
+  if (px < f.width && py < f.height) {
+    const size_t idx = (size_t)px + (size_t)py * f.width;
+    const float iw = 1.0f / (float)f.width, ih = 1.0f / (float)f.height;
+    v3 o, d;
+    cam_ray(cam, (float)px * iw, (float)py * ih, o, d);
+    v3 color = mk(0.0f, 0.0f, 0.0f);
+    float opacity = 0.0f;
+    uint32_t vo = ~0u, vi = ~0u;
+    float vdepth = FLT_MAX;
+    float t0, t1;
+    if (vol_slab(V, o, d, 0.0f, FLT_MAX, t0, t1)) {
+      vo = V.id;
+      vi = V.inst_id;
+      vdepth = t0;
+      march<I64>(vol_ctx(V, r.inv_volume_sampling_rate), s_lut, o, d, t0, t1, color, opacity,
+          vsamples);
+    }
+    const float depth = fminf(1e30f, vdepth);
+    color = mulf(color, opacity);
+    float w = 1.0f - opacity;
+    color = mk(color.x + r.background[0] * w, color.y + r.background[1] * w,
+        color.z + r.background[2] * w);
+    opacity = opacity + r.background[3] * (1.0f - opacity);
+    v3 outc = mk(0.0f + color.x * 1.0f, 0.0f + color.y * 1.0f, 0.0f + color.z * 1.0f);
+    float outo = 0.0f + opacity * 1.0f;
+    float4 accum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float dcur = FLT_MAX;
+    if (f.frame_id != 0) {
+      accum = ((const float4 *)f.d_accum)[idx];
+      if (f.d_depth)
+        dcur = f.d_depth[idx];
+    }
+    float sc_ = 1.0f + fmaxf(0.0f, cmax3(outc));
+    accum.x = accum.x + outc.x / sc_;
+    accum.y = accum.y + outc.y / sc_;
+    accum.z = accum.z + outc.z / sc_;
+    accum.w = accum.w + outo;
+    bool ids_changed = f.frame_id == 0;
+    uint32_t pid = 0, oid = 0, iid = 0;
+    if (depth < dcur) {
+      dcur = depth;
+      pid = 0;
+      oid = vo;
+      iid = vi;
+      ids_changed = true;
+    }
+    write_pixel(f, idx, accum, 1, dcur, ids_changed, pid, oid, iid);
+  }
+  if (f.d_counters) {
+    wave_count_add(f.d_counters, px < f.width && py < f.height ? 1u : 0u);
     wave_count_add(f.d_counters + 1, vsamples);
   }
 }
@@ -582,6 +712,15 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
     return VRX_ERR_ARG;
   }
   hipStream_t st = (hipStream_t)s;
+  if (scene->num_entries == 0 && scene->num_volumes == 1 && r->kind == VRX_RENDER_RAYCAST) {
+    dim3 grid((f->width + kTile - 1) / kTile, (f->height + kTile - 1) / kTile);
+    if (scene->max_volume_voxels >= (1ull << 31))
+      hipLaunchKernelGGL(k_render_volume<true>, grid, dim3(kBlock), 0, st, *scene, *cam, *r, *f);
+    else
+      hipLaunchKernelGGL(k_render_volume<false>, grid, dim3(kBlock), 0, st, *scene, *cam, *r, *f);
+    VRX_LAUNCH_CHECK();
+    return VRX_OK;
+  }
   switch (scene->single_type) {
   case VRX_TRIANGLE:
     return launch_pt<VRX_TRIANGLE>(*scene, *cam, *r, *f, sd, cap, st);
