@@ -13,6 +13,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import math
 import os
 import statistics
@@ -139,6 +141,72 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
     return 2.0 * nbytes * reps / (ms.value / 1000.0) / 1e9
 
 
+def bench_diffvol(iters=200, log_every=50):
+    """128^3 density + 256-entry LUT, 32 views at 256x256: forward-only and
+    forward+adjoint times, then Adam (lr 1e-2) from a flat grid and a grey LUT."""
+    import torch
+    from visrtx_amd import diffvol as D
+    from visrtx_amd import scenes as S
+    n = 128
+    sp = 1.0 / (n - 1)
+    spec = D.GridSpec(dims=(n, n, n), spacing=(sp, sp, sp), unit_distance=sp, step=0.5 * sp,
+                      early_stop=True)
+    gt = torch.from_numpy(S.gen_noise_volume(n, seed=11)).cuda()
+    gt_lut = torch.from_numpy(D.colormap_lut(S.COLORMAP5, np.array([0.0, 0.3]))).cuda()
+    cams = D.orbit_cameras(32, seed=7, fovy=45.0)
+    org, dirs = D.make_rays(cams, 256, 256)
+    with torch.no_grad():
+        target = D.render(gt, gt_lut, org, dirs, spec)
+    dens = torch.full_like(gt, 0.5).requires_grad_(True)
+    lut = torch.from_numpy(D.linear_lut(0.2)).cuda().requires_grad_(True)
+
+    def timeit(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def fwd():
+        with torch.no_grad():
+            D.render(dens, lut, org, dirs, spec)
+
+    def fwdbwd():
+        loss = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean()
+        loss.backward()
+
+    fwd_ms = timeit(fwd, 10)
+    fb_ms = timeit(fwdbwd, 10)
+    opt = torch.optim.Adam([dens, lut], lr=1e-2)
+    dens.grad = None
+    lut.grad = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    first = last = None
+    for it in range(iters):
+        opt.zero_grad()
+        loss = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            lut.clamp_(0.0, 0.999)
+        if it == 0:
+            first = loss.item()
+        if log_every and (it + 1) % log_every == 0:
+            log(f"diffvol iter {it + 1}: loss {loss.item():.4e}")
+    torch.cuda.synchronize()
+    iter_ms = (time.perf_counter() - t0) * 1000.0 / iters
+    with torch.no_grad():
+        last = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean().item()
+    return {"fwd_ms": round(fwd_ms, 3), "fwd_bwd_ms": round(fb_ms, 3),
+            "fwd_bwd_over_fwd": round(fb_ms / fwd_ms, 3), "adam_ms_per_iter": round(iter_ms, 3),
+            "iters": iters, "mse_first": first, "mse_final": last, "rays": int(org.shape[0])}
+
+
 def cpu_baseline(threads):
     """The oracle (C/OpenMP port of the same path) on a bounded sample of the
     10M-sphere workload: same scene and camera, 640x360 rays."""
@@ -261,6 +329,14 @@ def main():
         del sc3
 
     dev.close()
+
+    if not args.only_headline:
+        log("differentiable volume")
+        dv = bench_diffvol()
+        extra["diffvol_128"] = dv
+        log(f"diffvol: fwd {dv['fwd_ms']} ms, fwd+bwd {dv['fwd_bwd_ms']} ms "
+            f"({dv['fwd_bwd_over_fwd']}x), Adam {dv['adam_ms_per_iter']} ms/iter, "
+            f"MSE {dv['mse_first']:.3e} -> {dv['mse_final']:.3e}")
 
     cpu = None
     if not args.no_cpu_baseline:

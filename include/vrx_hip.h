@@ -201,8 +201,9 @@ typedef struct vrx_renderer {
   int32_t cull_backfaces;
   float inv_volume_sampling_rate;
   /* launch tuning (autotuned; 0 = defaults) */
-  int32_t stack_lds;      /* LDS short-stack entries per lane: 8/16/24/32 */
+  int32_t stack_lds;      /* LDS short-stack entries per lane: 8/16/32/64 */
   int32_t block_pixels;   /* pixels per workgroup: 64/128/256 */
+  int32_t vol_unroll;     /* volume samples gathered per step: 1/2/4/8 */
 } vrx_renderer;
 
 typedef struct vrx_frame {

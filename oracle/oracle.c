@@ -1614,7 +1614,7 @@ int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r,
 /* ------------------------------------------------------------------------- */
 static void diff_march(const uint32_t dims[3], const float origin[3], const float spacing[3],
     const double *density, const double *lut, const double vr[2], double unit_distance,
-    double step, const double o[3], const double d[3], int early_stop, double out[4])
+    double step, const double o[3], const double d[3], int early_stop, double out[4], int *amb)
 {
   orc_volume v;
   memset(&v, 0, sizeof(v));
@@ -1623,15 +1623,19 @@ static void diff_march(const uint32_t dims[3], const float origin[3], const floa
   memcpy(v.spacing, spacing, sizeof(v.spacing));
   double t0, t1;
   out[0] = out[1] = out[2] = out[3] = 0.0;
-  if (!vol_slab(&v, o, d, 0.0, (double)FLT_MAX, &t0, &t1))
+  if (!vol_slab_amb(&v, o, d, 0.0, (double)FLT_MAX, &t0, &t1, amb))
     return;
   float vrf[2] = {(float)vr[0], (float)vr[1]};
   double expo = step / unit_distance;
   double T = 1.0;
   for (uint64_t k = 0;; k++) {
     double t = t0 + (double)k * step;
+    if (amb && near_rel(t, t1))
+      *amb = 1;
     if (!(t <= t1))
       break;
+    if (amb && early_stop && fabs(out[3] - 0.99) < AMB_OPACITY)
+      *amb = 1;
     if (early_stop && !(out[3] < 0.99))
       break;
     double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
@@ -1651,7 +1655,7 @@ static void diff_march(const uint32_t dims[3], const float origin[3], const floa
 void orc_diffvol_forward(const uint32_t dims[3], const float origin[3], const float spacing[3],
     const double *density, const double *lut, const double value_range[2], double unit_distance,
     double step, uint32_t n, const float *org, const float *dir, double *rgba_out, int early_stop,
-    int32_t threads)
+    int32_t threads, uint8_t *ambiguous)
 {
 #ifdef _OPENMP
   if (threads > 0)
@@ -1661,8 +1665,11 @@ void orc_diffvol_forward(const uint32_t dims[3], const float origin[3], const fl
   for (int64_t i = 0; i < (int64_t)n; i++) {
     double o[3] = {org[3 * i], org[3 * i + 1], org[3 * i + 2]};
     double d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+    int amb = 0;
     diff_march(dims, origin, spacing, density, lut, value_range, unit_distance, step, o, d,
-        early_stop, rgba_out + 4 * i);
+        early_stop, rgba_out + 4 * i, &amb);
+    if (ambiguous)
+      ambiguous[i] = (uint8_t)amb;
   }
 }
 
@@ -1682,7 +1689,7 @@ double orc_diffvol_loss(const uint32_t dims[3], const float origin[3], const flo
     double d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
     double out[4];
     diff_march(dims, origin, spacing, density, lut, value_range, unit_distance, step, o, d,
-        early_stop, out);
+        early_stop, out, NULL);
     total += w[4 * i] * out[0] + w[4 * i + 1] * out[1] + w[4 * i + 2] * out[2] + w[4 * i + 3] * out[3];
   }
   return total;

@@ -229,7 +229,7 @@ void orc_linear_to_srgb(const float in[4], float out[4]);
 void orc_diffvol_forward(const uint32_t dims[3], const float origin[3], const float spacing[3],
     const double *density, const double *lut, const double value_range[2], double unit_distance,
     double step, uint32_t n, const float *org, const float *dir, double *rgba_out,
-    int early_stop, int32_t threads);
+    int early_stop, int32_t threads, uint8_t *ambiguous);
 /* Loss = sum over rays and channels of w[i*4+c] * rgba[i*4+c]; returns loss. */
 double orc_diffvol_loss(const uint32_t dims[3], const float origin[3], const float spacing[3],
     const double *density, const double *lut, const double value_range[2], double unit_distance,

@@ -127,7 +127,7 @@ def lib() -> C.CDLL:
         L.orc_tf_discretize.argtypes = [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, vp,
                                         C.c_float, vp, vp]
         L.orc_diffvol_forward.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, C.c_double,
-                                          C.c_uint32, vp, vp, vp, C.c_int, C.c_int32]
+                                          C.c_uint32, vp, vp, vp, C.c_int, C.c_int32, vp]
         L.orc_diffvol_loss.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, C.c_double,
                                        C.c_uint32, vp, vp, vp, C.c_int, C.c_int32]
         _lib = L
@@ -437,3 +437,45 @@ def linear_to_srgb(v):
     o = np.empty(4, np.float32)
     lib().orc_linear_to_srgb(a.ctypes.data, o.ctypes.data)
     return o
+
+
+# ---------------------------------------------------------------------------
+# differentiable volume (FP64)
+# ---------------------------------------------------------------------------
+def _dv_args(dims, origin, spacing, density, lut, value_range):
+    d = np.asarray(dims, np.uint32)
+    o = np.asarray(origin, np.float32)
+    sp = np.asarray(spacing, np.float32)
+    den = np.ascontiguousarray(density, np.float64)
+    lt = np.ascontiguousarray(lut, np.float64)
+    vr = np.asarray(value_range, np.float64)
+    return d, o, sp, den, lt, vr
+
+
+def diffvol_forward(dims, origin, spacing, density, lut, value_range, unit_distance, step, org,
+                    dirs, early_stop=False, threads=0):
+    """FP64 (C.rgb, O) per ray and the fp32 decision-boundary mask."""
+    d, o, sp, den, lt, vr = _dv_args(dims, origin, spacing, density, lut, value_range)
+    org = np.ascontiguousarray(org, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    n = len(org)
+    out = np.zeros((n, 4), np.float64)
+    amb = np.zeros(n, np.uint8)
+    lib().orc_diffvol_forward(d.ctypes.data, o.ctypes.data, sp.ctypes.data, den.ctypes.data,
+                              lt.ctypes.data, vr.ctypes.data, unit_distance, step, n,
+                              org.ctypes.data, dirs.ctypes.data, out.ctypes.data,
+                              int(early_stop), threads, amb.ctypes.data)
+    return out, amb
+
+
+def diffvol_loss(dims, origin, spacing, density, lut, value_range, unit_distance, step, org, dirs,
+                 weights, early_stop=False, threads=0):
+    """FP64 loss sum(weights * rgba) over the given rays."""
+    d, o, sp, den, lt, vr = _dv_args(dims, origin, spacing, density, lut, value_range)
+    org = np.ascontiguousarray(org, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    w = np.ascontiguousarray(weights, np.float64)
+    return lib().orc_diffvol_loss(d.ctypes.data, o.ctypes.data, sp.ctypes.data, den.ctypes.data,
+                                  lt.ctypes.data, vr.ctypes.data, unit_distance, step, len(org),
+                                  org.ctypes.data, dirs.ctypes.data, w.ctypes.data,
+                                  int(early_stop), threads)

@@ -1104,6 +1104,7 @@ void Renderer::commit()
   float rate = std::clamp(get<float>("volumeSamplingRate", ANARI_FLOAT32, 0.125f), 1e-3f, 10.0f);
   r.inv_volume_sampling_rate = 1.0f / rate;
   r.stack_lds = get<int>("stackDepthLDS", ANARI_INT32, 0);
+  r.vol_unroll = get<int>("volumeUnroll", ANARI_INT32, 0);
   sampleLimit = get<int>("sampleLimit", ANARI_INT32, 128);
   if (kind == VRX_RENDER_RAYCAST)
     sampleLimit = 1; // single-shot renderer

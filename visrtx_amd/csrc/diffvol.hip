@@ -1,25 +1,439 @@
 // diffvol.hip -- differentiable emission-absorption volume integral.
+//
+// Forward: the plain integral of gpu/volumeIntegration.h:64-103 (jitter off)
+// over a structured-regular density grid classified by a 256-entry RGBA LUT:
+//   C = sum_k T_k (1 - tau_k) c_k,  O = sum_k T_k (1 - tau_k),
+//   T_{k+1} = T_k tau_k,  tau_k = (1 - a_k)^(step / unitDistance).
+// The reference has only this forward pass; the adjoint is new.  The forward
+// stores the transmittance every `checkpoint` (64) steps plus the sample
+// count and the final transmittance -- never every sample.
+//
+// Adjoint: with v_k = G.c_k + gO (G = dL/dC, gO = dL/dO) and the suffix
+//   R_k = sum_{j>k} (w_j / T_{k+1}) v_j,   R_{k-1} = (1 - tau_k) v_k + tau_k R_k,
+// the gradients are dL/dtau_k = T_k (R_k - v_k), dL/dc_k = w_k G.  The march is
+// replayed backward from the last sample; T_k is re-anchored at every
+// checkpoint and recovered inside a segment as T_{k+1} / tau_k (a forward
+// replay from the segment's checkpoint is used when tau_k is too small for
+// that division).  LUT gradients are pre-reduced in LDS per workgroup;
+// density gradients are accumulated per trilinear cell in registers and
+// scattered with hardware float atomics when the ray leaves the cell.
+#include <float.h>
+#include <string.h>
+
+#include <algorithm>
+
 #include "vrx_internal.h"
+#include "vrx_math.h"
+
+namespace vrx {
+
+constexpr int kDvBlock = 256;
+constexpr float kMinTauForDivision = 1e-3f;
+
+struct DvCtx {
+  const float *density;
+  int dims[3], imax[3];
+  float gmax[3];
+  float origin[3], inv_sp[3], lo_b[3], hi_b[3];
+  uint32_t sy, sz;
+  float step, expo, lo, hi, inv_range;
+  uint32_t nck;        // checkpoint slots per ray (2 + segments)
+  int ckpt;            // steps per checkpoint
+  int early;
+};
+
+static DvCtx make_ctx(const vrx_diffvol &v)
+{
+  DvCtx c;
+  c.density = v.d_density;
+  for (int k = 0; k < 3; k++) {
+    c.dims[k] = (int)v.dims[k];
+    c.imax[k] = std::max((int)v.dims[k] - 2, 0);
+    c.gmax[k] = (float)v.dims[k] - 1.0f;
+    c.origin[k] = v.origin[k];
+    c.inv_sp[k] = 1.0f / v.spacing[k];
+    c.lo_b[k] = v.origin[k];
+    c.hi_b[k] = v.origin[k] + ((float)v.dims[k] - 1.0f) * v.spacing[k];
+  }
+  c.sy = v.dims[0];
+  c.sz = v.dims[0] * v.dims[1];
+  c.step = v.step;
+  c.expo = v.step * (1.0f / v.unit_distance);
+  c.lo = v.value_range[0];
+  c.hi = v.value_range[1];
+  c.inv_range = 1.0f / (c.hi - c.lo);
+  c.ckpt = v.checkpoint > 0 ? v.checkpoint : 64;
+  c.nck = 2 + (v.max_steps + c.ckpt - 1) / c.ckpt;
+  c.early = v.early_stop;
+  return c;
+}
+
+__device__ __forceinline__ bool dv_slab(const DvCtx &c, v3 o, v3 d, float &t0, float &t1)
+{
+  float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+  float tn = -INFINITY, tf = INFINITY;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float inv = 1.0f / dd[k];
+    float a = (c.lo_b[k] - oo[k]) * inv, b = (c.hi_b[k] - oo[k]) * inv;
+    tn = fmaxf(tn, fminf(a, b));
+    tf = fminf(tf, fmaxf(a, b));
+  }
+  if (!(tn < tf) || tf < 0.0f)
+    return false;
+  t0 = fmaxf(tn, 0.0f);
+  t1 = tf;
+  return true;
+}
+
+struct Sample {
+  float s;          // density
+  float fx, fy, fz; // trilinear fractions
+  uint32_t base;    // index of the (i0, j0, k0) corner
+  uint32_t cell;    // same, used as the cell key
+};
+
+__device__ __forceinline__ Sample dv_sample(const DvCtx &c, float gx, float gy, float gz)
+{
+  float g[3] = {gx, gy, gz};
+  int i0[3];
+  float f[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float x = fminf(fmaxf(g[k], 0.0f), c.gmax[k]);
+    int i = min((int)x, c.imax[k]);
+    i0[k] = i;
+    f[k] = x - (float)i;
+  }
+  const uint32_t base = (uint32_t)i0[0] + (uint32_t)i0[1] * c.sy + (uint32_t)i0[2] * c.sz;
+  const VRX_GLOBAL float *D = (const VRX_GLOBAL float *)c.density + base;
+  // grids here have every dimension >= 2, so the +1 neighbours exist
+  float2 p00 = gld_pair(D), p10 = gld_pair(D + c.sy), p01 = gld_pair(D + c.sz),
+         p11 = gld_pair(D + c.sy + c.sz);
+  float c00 = p00.x + (p00.y - p00.x) * f[0];
+  float c10 = p10.x + (p10.y - p10.x) * f[0];
+  float c01 = p01.x + (p01.y - p01.x) * f[0];
+  float c11 = p11.x + (p11.y - p11.x) * f[0];
+  float c0 = c00 + (c10 - c00) * f[1];
+  float c1 = c01 + (c11 - c01) * f[1];
+  Sample sm;
+  sm.s = c0 + (c1 - c0) * f[2];
+  sm.fx = f[0];
+  sm.fy = f[1];
+  sm.fz = f[2];
+  sm.base = base;
+  sm.cell = base;
+  return sm;
+}
+
+struct Cls {
+  float4 co;
+  int i;
+  float f;
+  float du_ds; // d(texel coordinate)/ds, 0 where clamped
+};
+
+__device__ __forceinline__ Cls dv_classify(const float4 *lut, const DvCtx &c, float s)
+{
+  Cls r;
+  float cl = fminf(fmaxf(s, c.lo), c.hi);
+  float u = (cl - c.lo) * c.inv_range;
+  float xr = u * 256.0f - 0.5f;
+  float x = fminf(fmaxf(xr, 0.0f), 255.0f);
+  int i = min((int)x, 254);
+  float f = x - (float)i;
+  float4 a = lut[i], b = lut[i + 1];
+  r.co = make_float4(a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, a.z + (b.z - a.z) * f,
+      a.w + (b.w - a.w) * f);
+  r.i = i;
+  r.f = f;
+  const bool inside = s > c.lo && s < c.hi && xr > 0.0f && xr < 255.0f;
+  r.du_ds = inside ? 256.0f * c.inv_range : 0.0f;
+  return r;
+}
+
+__device__ __forceinline__ float dv_tau(const DvCtx &c, float a)
+{
+  return __builtin_amdgcn_exp2f(c.expo * __builtin_amdgcn_logf(1.0f - a));
+}
+
+__global__ __launch_bounds__(kDvBlock) void k_diffvol_fwd(DvCtx c, const float *lut_g, uint32_t n,
+    const float *org, const float *dir, float *rgba, float *ckpt)
+{
+  __shared__ float4 s_lut[256];
+  s_lut[threadIdx.x] = gld((const float4 *)lut_g, threadIdx.x);
+  __syncthreads();
+  const uint32_t r = blockIdx.x * kDvBlock + threadIdx.x;
+  if (r >= n)
+    return;
+  const v3 o = gld3(org, r), d = gld3(dir, r);
+  float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, O = 0.0f, T = 1.0f;
+  uint32_t N = 0;
+  float t0, t1;
+  if (dv_slab(c, o, d, t0, t1) && c.step > 0.0f) {
+    const float gox = (o.x - c.origin[0]) * c.inv_sp[0], gdx = d.x * c.inv_sp[0];
+    const float goy = (o.y - c.origin[1]) * c.inv_sp[1], gdy = d.y * c.inv_sp[1];
+    const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
+    const uint32_t kmax = (c.nck - 2) * (uint32_t)c.ckpt;
+    for (uint32_t k = 0; k < kmax; k++) {
+      const float t = t0 + (float)k * c.step;
+      if (!(t <= t1) || (c.early && !(O < 0.99f)))
+        break;
+      if (k % (uint32_t)c.ckpt == 0)
+        gst(ckpt, (size_t)(2 + k / c.ckpt) * n + r, T);
+      const Sample sm = dv_sample(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
+      const Cls cl = dv_classify(s_lut, c, sm.s);
+      const float tau = dv_tau(c, cl.co.w);
+      const float w = T * (1.0f - tau);
+      C0 += w * cl.co.x;
+      C1 += w * cl.co.y;
+      C2 += w * cl.co.z;
+      O += w;
+      T *= tau;
+      N = k + 1;
+    }
+  }
+  gst(ckpt, (size_t)0 * n + r, __uint_as_float(N));
+  gst(ckpt, (size_t)1 * n + r, T);
+  gst((float4 *)rgba, r, make_float4(C0, C1, C2, O));
+}
+
+__device__ __forceinline__ void flush_cell(const DvCtx &c, float *gd, uint32_t base, const float g[8])
+{
+  float *p = gd + base;
+  unsafeAtomicAdd(p, g[0]);
+  unsafeAtomicAdd(p + 1, g[1]);
+  unsafeAtomicAdd(p + c.sy, g[2]);
+  unsafeAtomicAdd(p + c.sy + 1, g[3]);
+  unsafeAtomicAdd(p + c.sz, g[4]);
+  unsafeAtomicAdd(p + c.sz + 1, g[5]);
+  unsafeAtomicAdd(p + c.sz + c.sy, g[6]);
+  unsafeAtomicAdd(p + c.sz + c.sy + 1, g[7]);
+}
+
+__global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *lut_g, uint32_t n,
+    const float *org, const float *dir, const float *grad_rgba, const float *ckpt,
+    float *grad_density, float *lut_partials)
+{
+  __shared__ float4 s_lut[256];
+  __shared__ float s_glut[1024];
+  s_lut[threadIdx.x] = gld((const float4 *)lut_g, threadIdx.x);
+  for (int i = threadIdx.x; i < 1024; i += kDvBlock)
+    s_glut[i] = 0.0f;
+  __syncthreads();
+  const uint32_t r = blockIdx.x * kDvBlock + threadIdx.x;
+  if (r < n) {
+    const uint32_t N = __float_as_uint(gld(ckpt, (size_t)0 * n + r));
+    float t0, t1;
+    const v3 o = gld3(org, r), d = gld3(dir, r);
+    if (N > 0 && dv_slab(c, o, d, t0, t1)) {
+      const float4 G4 = gld((const float4 *)grad_rgba, r);
+      const float gx = G4.x, gy = G4.y, gz = G4.z, gO = G4.w;
+      const float gox = (o.x - c.origin[0]) * c.inv_sp[0], gdx = d.x * c.inv_sp[0];
+      const float goy = (o.y - c.origin[1]) * c.inv_sp[1], gdy = d.y * c.inv_sp[1];
+      const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
+      float Tn = gld(ckpt, (size_t)1 * n + r); // T after the last sample
+      float R = 0.0f;
+      uint32_t cur = 0xFFFFFFFFu;
+      float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int k = (int)N - 1; k >= 0; k--) {
+        const float t = t0 + (float)k * c.step;
+        const Sample sm = dv_sample(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
+        const Cls cl = dv_classify(s_lut, c, sm.s);
+        const float tau = dv_tau(c, cl.co.w);
+        float T;
+        const uint32_t seg = (uint32_t)k / (uint32_t)c.ckpt;
+        if ((uint32_t)k % (uint32_t)c.ckpt == 0) {
+          T = gld(ckpt, (size_t)(2 + seg) * n + r);
+        } else if (tau >= kMinTauForDivision && Tn > 0.0f) {
+          T = Tn / tau;
+        } else {
+          // replay from the segment checkpoint
+          T = gld(ckpt, (size_t)(2 + seg) * n + r);
+          for (uint32_t j = seg * (uint32_t)c.ckpt; j < (uint32_t)k; j++) {
+            const float tj = t0 + (float)j * c.step;
+            const Sample sj = dv_sample(c, gox + gdx * tj, goy + gdy * tj, goz + gdz * tj);
+            T *= dv_tau(c, dv_classify(s_lut, c, sj.s).co.w);
+          }
+        }
+        const float v = (gx * cl.co.x + gy * cl.co.y + gz * cl.co.z) + gO;
+        const float w = T * (1.0f - tau);
+        const float dtau = T * (R - v);
+        const float oma = fmaxf(1.0f - cl.co.w, 1e-8f);
+        const float da = dtau * (-c.expo) * tau / oma;
+        const float dr = w * gx, dg = w * gy, db = w * gz;
+        // LUT: entries i and i+1 with weights (1-f) and f
+        const float wf0 = 1.0f - cl.f, wf1 = cl.f;
+        float *g0 = s_glut + 4 * cl.i;
+        atomicAdd(g0 + 0, wf0 * dr);
+        atomicAdd(g0 + 1, wf0 * dg);
+        atomicAdd(g0 + 2, wf0 * db);
+        atomicAdd(g0 + 3, wf0 * da);
+        atomicAdd(g0 + 4, wf1 * dr);
+        atomicAdd(g0 + 5, wf1 * dg);
+        atomicAdd(g0 + 6, wf1 * db);
+        atomicAdd(g0 + 7, wf1 * da);
+        // density: d(channel)/ds through the LUT slope, then trilinear weights
+        const float4 la = s_lut[cl.i], lb = s_lut[cl.i + 1];
+        const float ds = ((dr * (lb.x - la.x) + dg * (lb.y - la.y)) + (db * (lb.z - la.z) + da * (lb.w - la.w)))
+            * cl.du_ds;
+        if (sm.cell != cur) {
+          if (cur != 0xFFFFFFFFu)
+            flush_cell(c, grad_density, cur, acc);
+          cur = sm.cell;
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            acc[q] = 0.0f;
+        }
+        const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
+        acc[0] += ds * (ux * uy * uz);
+        acc[1] += ds * (sm.fx * uy * uz);
+        acc[2] += ds * (ux * sm.fy * uz);
+        acc[3] += ds * (sm.fx * sm.fy * uz);
+        acc[4] += ds * (ux * uy * sm.fz);
+        acc[5] += ds * (sm.fx * uy * sm.fz);
+        acc[6] += ds * (ux * sm.fy * sm.fz);
+        acc[7] += ds * (sm.fx * sm.fy * sm.fz);
+        R = (1.0f - tau) * v + tau * R;
+        Tn = T;
+      }
+      if (cur != 0xFFFFFFFFu)
+        flush_cell(c, grad_density, cur, acc);
+    }
+  }
+  __syncthreads();
+  float *dst = lut_partials + (size_t)blockIdx.x * 1024;
+  for (int i = threadIdx.x; i < 1024; i += kDvBlock)
+    gst(dst, i, s_glut[i]);
+}
+
+// grad_lut[j] += sum over workgroups of the partials (fixed order: deterministic)
+__global__ __launch_bounds__(256) void k_reduce_lut(const float *partials, uint32_t nblocks, float *grad_lut)
+{
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= 1024)
+    return;
+  float s = 0.0f;
+  for (uint32_t b = 0; b < nblocks; b++)
+    s += gld(partials, (size_t)b * 1024 + j);
+  grad_lut[j] += s;
+}
+
+__constant__ vrx_camera c_cams[256];
+
+// pinhole rays through the pixel corners (gpu/cameraCreateRay.h:74-81,
+// raycast variant), row 0 = bottom; ray index = view*w*h + y*w + x
+__global__ void k_make_rays(uint32_t nviews, uint32_t w, uint32_t h, float *org, float *dir)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t per = (size_t)w * h;
+  if (i >= per * nviews)
+    return;
+  const uint32_t view = (uint32_t)(i / per);
+  const uint32_t rem = (uint32_t)(i - (size_t)view * per);
+  const uint32_t y = rem / w, x = rem - y * w;
+  const vrx_camera &c = c_cams[view];
+  float sx = (float)x * (1.0f / (float)w), sy = (float)y * (1.0f / (float)h);
+  sx = c.region[0] * (1.0f - sx) + c.region[2] * sx;
+  sy = c.region[1] * (1.0f - sy) + c.region[3] * sy;
+  v3 dd = normalize(add(add(ld3(c.d00), fmul(sx, ld3(c.du))), fmul(sy, ld3(c.dv))));
+  gst(org, 3 * i, c.pos[0]);
+  gst(org, 3 * i + 1, c.pos[1]);
+  gst(org, 3 * i + 2, c.pos[2]);
+  gst(dir, 3 * i, dd.x);
+  gst(dir, 3 * i + 1, dd.y);
+  gst(dir, 3 * i + 2, dd.z);
+}
+
+static int check_dv(const vrx_diffvol *v)
+{
+  VRX_ARG(v && v->d_density && v->d_lut, "diffvol arguments");
+  VRX_ARG(v->dims[0] >= 2 && v->dims[1] >= 2 && v->dims[2] >= 2, "grid dims must be >= 2");
+  VRX_ARG((uint64_t)v->dims[0] * v->dims[1] * v->dims[2] < (1ull << 31), "grid too large");
+  VRX_ARG(v->step > 0.0f, "step must be > 0");
+  VRX_ARG(v->max_steps > 0, "max_steps must be > 0");
+  return VRX_OK;
+}
+
+} // namespace vrx
+
+using namespace vrx;
 
 extern "C" {
-size_t vrx_diffvol_ckpt_floats(const vrx_diffvol *v, uint32_t n) { return 0; }
+
+size_t vrx_diffvol_ckpt_floats(const vrx_diffvol *v, uint32_t n)
+{
+  if (!v)
+    return 0;
+  int ck = v->checkpoint > 0 ? v->checkpoint : 64;
+  return (size_t)(2 + (v->max_steps + ck - 1) / ck) * n;
+}
+
 int vrx_diffvol_forward(const vrx_diffvol *v, uint32_t n, const float *d_org, const float *d_dir,
     float *d_rgba_out, float *d_ckpt, vrx_stream s)
 {
-  vrx::set_error("not implemented");
-  return VRX_ERR_ARG;
+  int st = check_dv(v);
+  if (st)
+    return st;
+  if (n == 0)
+    return VRX_OK;
+  VRX_ARG(d_org && d_dir && d_rgba_out && d_ckpt, "ray/output buffers");
+  DvCtx c = make_ctx(*v);
+  hipLaunchKernelGGL(k_diffvol_fwd, dim3((n + kDvBlock - 1) / kDvBlock), dim3(kDvBlock), 0,
+      (hipStream_t)s, c, v->d_lut, n, d_org, d_dir, d_rgba_out, d_ckpt);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
 }
+
 int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, const float *d_dir,
     const float *d_rgba, const float *d_grad_rgba, const float *d_ckpt, float *d_grad_density,
     float *d_grad_lut, vrx_stream s)
 {
-  vrx::set_error("not implemented");
-  return VRX_ERR_ARG;
+  (void)d_rgba;
+  int st = check_dv(v);
+  if (st)
+    return st;
+  if (n == 0)
+    return VRX_OK;
+  VRX_ARG(d_org && d_dir && d_grad_rgba && d_ckpt && d_grad_density && d_grad_lut, "buffers");
+  DvCtx c = make_ctx(*v);
+  const uint32_t nb = (n + kDvBlock - 1) / kDvBlock;
+  // per-workgroup LUT partials live in a grow-only buffer owned by this call site
+  static thread_local void *partials = nullptr;
+  static thread_local size_t cap = 0;
+  const size_t need = (size_t)nb * 1024 * 4;
+  if (cap < need) {
+    if (partials) {
+      VRX_HIP(hipStreamSynchronize((hipStream_t)s));
+      VRX_HIP(hipFree(partials));
+    }
+    partials = nullptr;
+    cap = 0;
+    VRX_HIP(hipMalloc(&partials, need));
+    cap = need;
+  }
+  hipLaunchKernelGGL(k_diffvol_bwd, dim3(nb), dim3(kDvBlock), 0, (hipStream_t)s, c, v->d_lut, n,
+      d_org, d_dir, d_grad_rgba, d_ckpt, d_grad_density, (float *)partials);
+  hipLaunchKernelGGL(k_reduce_lut, dim3(4), dim3(256), 0, (hipStream_t)s, (const float *)partials,
+      nb, d_grad_lut);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
 }
+
 int vrx_make_rays(const vrx_camera *cams, uint32_t nviews, uint32_t w, uint32_t h, float *d_org,
     float *d_dir, vrx_stream s)
 {
-  vrx::set_error("not implemented");
-  return VRX_ERR_ARG;
+  VRX_ARG(cams && d_org && d_dir, "null argument");
+  VRX_ARG(nviews >= 1 && nviews <= 256, "1..256 views");
+  VRX_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_cams), cams, sizeof(vrx_camera) * nviews, 0,
+      hipMemcpyHostToDevice, (hipStream_t)s));
+  size_t total = (size_t)nviews * w * h;
+  if (total == 0)
+    return VRX_OK;
+  hipLaunchKernelGGL(k_make_rays, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+      (hipStream_t)s, nviews, w, h, d_org, d_dir);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
 }
-}
+
+} // extern "C"

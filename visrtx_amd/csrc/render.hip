@@ -130,22 +130,43 @@ __device__ __forceinline__ float trilinear(const VolCtx &c, float gx, float gy, 
     f[k] = x - (float)i;
   }
   float c000, c100, c010, c110, c001, c101, c011, c111;
-  if constexpr (I64) {
+  if (c.dims[0] >= 2) {
+    // x-neighbours are adjacent floats (i0 <= dims-2): one 8-byte gather each
+    const VRX_GLOBAL float *D = (const VRX_GLOBAL float *)c.data;
+    float2 p00, p10, p01, p11;
+    if constexpr (I64) {
+      size_t z0 = (size_t)i0[2] * c.sz, z1 = (size_t)i1[2] * c.sz;
+      size_t y0 = (size_t)i0[1] * c.sy, y1 = (size_t)i1[1] * c.sy;
+      p00 = gld_pair(D + z0 + y0 + i0[0]);
+      p10 = gld_pair(D + z0 + y1 + i0[0]);
+      p01 = gld_pair(D + z1 + y0 + i0[0]);
+      p11 = gld_pair(D + z1 + y1 + i0[0]);
+    } else {
+      uint32_t z0 = (uint32_t)i0[2] * c.sz, z1 = (uint32_t)i1[2] * c.sz;
+      uint32_t y0 = (uint32_t)i0[1] * c.sy, y1 = (uint32_t)i1[1] * c.sy;
+      p00 = gld_pair(D + z0 + y0 + i0[0]);
+      p10 = gld_pair(D + z0 + y1 + i0[0]);
+      p01 = gld_pair(D + z1 + y0 + i0[0]);
+      p11 = gld_pair(D + z1 + y1 + i0[0]);
+    }
+    c000 = p00.x; c100 = p00.y; c010 = p10.x; c110 = p10.y;
+    c001 = p01.x; c101 = p01.y; c011 = p11.x; c111 = p11.y;
+  } else if constexpr (I64) {
     size_t z0 = (size_t)i0[2] * c.sz, z1 = (size_t)i1[2] * c.sz;
     size_t y0 = (size_t)i0[1] * c.sy, y1 = (size_t)i1[1] * c.sy;
-    const float *D = c.data;
-    c000 = D[z0 + y0 + i0[0]]; c100 = D[z0 + y0 + i1[0]];
-    c010 = D[z0 + y1 + i0[0]]; c110 = D[z0 + y1 + i1[0]];
-    c001 = D[z1 + y0 + i0[0]]; c101 = D[z1 + y0 + i1[0]];
-    c011 = D[z1 + y1 + i0[0]]; c111 = D[z1 + y1 + i1[0]];
+    const VRX_GLOBAL float *D = (const VRX_GLOBAL float *)c.data;
+    c000 = c100 = D[z0 + y0 + i0[0]];
+    c010 = c110 = D[z0 + y1 + i0[0]];
+    c001 = c101 = D[z1 + y0 + i0[0]];
+    c011 = c111 = D[z1 + y1 + i0[0]];
   } else {
     uint32_t z0 = (uint32_t)i0[2] * c.sz, z1 = (uint32_t)i1[2] * c.sz;
     uint32_t y0 = (uint32_t)i0[1] * c.sy, y1 = (uint32_t)i1[1] * c.sy;
-    const float *D = c.data;
-    c000 = D[z0 + y0 + i0[0]]; c100 = D[z0 + y0 + i1[0]];
-    c010 = D[z0 + y1 + i0[0]]; c110 = D[z0 + y1 + i1[0]];
-    c001 = D[z1 + y0 + i0[0]]; c101 = D[z1 + y0 + i1[0]];
-    c011 = D[z1 + y1 + i0[0]]; c111 = D[z1 + y1 + i1[0]];
+    const VRX_GLOBAL float *D = (const VRX_GLOBAL float *)c.data;
+    c000 = c100 = D[z0 + y0 + i0[0]];
+    c010 = c110 = D[z0 + y1 + i0[0]];
+    c001 = c101 = D[z1 + y0 + i0[0]];
+    c011 = c111 = D[z1 + y1 + i0[0]];
   }
   float c00 = c000 + (c100 - c000) * f[0];
   float c10 = c010 + (c110 - c010) * f[0];
@@ -158,6 +179,7 @@ __device__ __forceinline__ float trilinear(const VolCtx &c, float gx, float gy, 
 
 // classifySample (volumeIntegration.h:46-62): LUT texel coordinate u*256-0.5,
 // linear, clamped (TransferFunction1D.cpp:152-185).
+template <bool LUT_GLOBAL>
 __device__ __forceinline__ float4 classify(const float4 *lut, const VolCtx &c, float s)
 {
   float cl = fminf(fmaxf(s, c.lo), c.hi);
@@ -165,7 +187,14 @@ __device__ __forceinline__ float4 classify(const float4 *lut, const VolCtx &c, f
   float x = fminf(fmaxf(u * 256.0f - 0.5f, 0.0f), 255.0f);
   int i = min((int)x, 254);
   float f = x - (float)i;
-  float4 a = lut[i], b = lut[i + 1];
+  float4 a, b;
+  if constexpr (LUT_GLOBAL) {
+    a = gld(lut, i);
+    b = gld(lut, i + 1);
+  } else {
+    a = lut[i];
+    b = lut[i + 1];
+  }
   return make_float4(a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, a.z + (b.z - a.z) * f,
       a.w + (b.w - a.w) * f);
 }
@@ -173,8 +202,12 @@ __device__ __forceinline__ float4 classify(const float4 *lut, const VolCtx &c, f
 // _rayMarchVolume (volumeIntegration.h:64-103), jitter off.  Sample k sits at
 // t0 + k*step (not an accumulated sum, which keeps positions within an ulp of
 // the exact ones); the per-step transmittance pow(1-a, step/unitDistance) is
-// evaluated as exp2(expo * log2(1-a)) with the hardware exp2/log2.
-template <bool I64>
+// evaluated as exp2(expo * log2(1-a)) with the hardware exp2/log2.  The gathers
+// of U consecutive samples are issued before any of them is composited (sample
+// positions do not depend on the accumulation), which keeps 8*U loads in
+// flight per lane; samples past the exit or the 0.99 cut-off are discarded,
+// their clamped gathers always stay inside the field.
+template <bool I64, int U, bool LUT_GLOBAL>
 __device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, v3 d, float t0,
     float t1, v3 &color, float &opacity, uint32_t &ns)
 {
@@ -185,22 +218,39 @@ __device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, 
   const float goy = (o.y - c.origin[1]) * c.inv_sp[1], gdy = d.y * c.inv_sp[1];
   const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
   float T = 1.0f;
-  for (uint32_t k = 0; k < kMaxSamplesPerSegment; k++) {
-    const float t = t0 + (float)k * c.step;
-    if (!(t <= t1) || !(opacity < 0.99f))
+  for (uint32_t k = 0; k < kMaxSamplesPerSegment; k += U) {
+    float sv[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const float t = t0 + (float)(k + u) * c.step;
+      ok[u] = t <= t1;
+      sv[u] = trilinear<I64>(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
+    }
+    bool done = false;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!done) {
+        if (!ok[u] || !(opacity < 0.99f)) {
+          done = true;
+        } else {
+          ns++;
+          const float s = sv[u];
+          if (s == s) {
+            const float4 co = classify<LUT_GLOBAL>(lut, c, s);
+            const float st = __builtin_amdgcn_exp2f(c.expo * __builtin_amdgcn_logf(1.0f - co.w));
+            const float w = T * (1.0f - st);
+            color.x += w * co.x;
+            color.y += w * co.y;
+            color.z += w * co.z;
+            opacity += w;
+            T *= st;
+          }
+        }
+      }
+    }
+    if (done)
       break;
-    const float s = trilinear<I64>(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
-    ns++;
-    if (s != s)
-      continue;
-    const float4 co = classify(lut, c, s);
-    const float st = __builtin_amdgcn_exp2f(c.expo * __builtin_amdgcn_logf(1.0f - co.w));
-    const float w = T * (1.0f - st);
-    color.x += w * co.x;
-    color.y += w * co.y;
-    color.z += w * co.z;
-    opacity += w;
-    T *= st;
   }
 }
 
@@ -243,9 +293,9 @@ __device__ float march_all(const vrx_scene &sc, v3 o, v3 d, float tlower, float 
     depth = fminf(depth, bt0);
     const VolCtx c = vol_ctx(V, inv_rate);
     if (vol_is_large(V))
-      march<true>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
+      march<true, 2, true>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
     else
-      march<false>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
+      march<false, 2, true>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
     tl = bt1 + 1e-3f;
     last = best;
     if (!(opacity < 0.99f))
@@ -501,13 +551,13 @@ __global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam,
 // rayMarchAllVolumes, the renderer's colour-times-opacity, background), with
 // the transfer-function LUT staged in LDS and no traversal state, so the
 // kernel needs about half the registers of the general one.
-template <bool I64>
+template <bool I64, int U>
 __global__ __launch_bounds__(kBlock) void k_render_volume(vrx_scene sc, vrx_camera cam,
     vrx_renderer r, vrx_frame f)
 {
   __shared__ float4 s_lut[256];
   const vrx_volume &V = sc.d_volumes[0];
-  s_lut[threadIdx.x] = ((const float4 *)V.d_lut)[threadIdx.x];
+  s_lut[threadIdx.x] = gld((const float4 *)V.d_lut, threadIdx.x);
   __syncthreads();
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -528,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void k_render_volume(vrx_scene sc, vrx_came
       vo = V.id;
       vi = V.inst_id;
       vdepth = t0;
-      march<I64>(vol_ctx(V, r.inv_volume_sampling_rate), s_lut, o, d, t0, t1, color, opacity,
+      march<I64, U, false>(vol_ctx(V, r.inv_volume_sampling_rate), s_lut, o, d, t0, t1, color, opacity,
           vsamples);
     }
     const float depth = fminf(1e30f, vdepth);
@@ -714,10 +764,21 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
   hipStream_t st = (hipStream_t)s;
   if (scene->num_entries == 0 && scene->num_volumes == 1 && r->kind == VRX_RENDER_RAYCAST) {
     dim3 grid((f->width + kTile - 1) / kTile, (f->height + kTile - 1) / kTile);
-    if (scene->max_volume_voxels >= (1ull << 31))
-      hipLaunchKernelGGL(k_render_volume<true>, grid, dim3(kBlock), 0, st, *scene, *cam, *r, *f);
+    const int u = r->vol_unroll > 0 ? r->vol_unroll : 4;
+    const bool big = scene->max_volume_voxels >= (1ull << 31);
+#define VRX_VOL_LAUNCH(B, U)                                                                    \
+  hipLaunchKernelGGL((k_render_volume<B, U>), grid, dim3(kBlock), 0, st, *scene, *cam, *r, *f)
+    if (big)
+      VRX_VOL_LAUNCH(true, 2);
+    else if (u <= 1)
+      VRX_VOL_LAUNCH(false, 1);
+    else if (u == 2)
+      VRX_VOL_LAUNCH(false, 2);
+    else if (u <= 4)
+      VRX_VOL_LAUNCH(false, 4);
     else
-      hipLaunchKernelGGL(k_render_volume<false>, grid, dim3(kBlock), 0, st, *scene, *cam, *r, *f);
+      VRX_VOL_LAUNCH(false, 8);
+#undef VRX_VOL_LAUNCH
     VRX_LAUNCH_CHECK();
     return VRX_OK;
   }

@@ -15,6 +15,22 @@ struct alignas(16) Node {
 };
 static_assert(sizeof(Node) == 64, "Node must be 64 bytes");
 
+__device__ __forceinline__ Node gld_node(const Node *nodes, size_t i)
+{
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+  const __attribute__((address_space(1))) f4v *q =
+      (const __attribute__((address_space(1))) f4v *)(nodes + i);
+  f4v a = q[0], b = q[1], c = q[2];
+  u4v d = ((const __attribute__((address_space(1))) u4v *)(nodes + i))[3];
+  Node n;
+  n.a = make_float4(a.x, a.y, a.z, a.w);
+  n.b = make_float4(b.x, b.y, b.z, b.w);
+  n.c = make_float4(c.x, c.y, c.z, c.w);
+  n.d = make_uint4(d.x, d.y, d.z, d.w);
+  return n;
+}
+
 size_t leaf_record_bytes(int type);
 void bvh_sizes(int32_t type, uint32_t n, vrx_bvh_sizes *o);
 int bvh_build_impl(const vrx_prims *p, const vrx_bvh *out, void *d_ws, size_t ws_bytes,

@@ -12,8 +12,55 @@
 #include <stdint.h>
 
 #define VRX_HD __host__ __device__ __forceinline__
+#define VRX_GLOBAL __attribute__((address_space(1)))
 
 namespace vrx {
+
+// Loads and stores through global (HBM) pointers.  Pointers fetched from
+// scene records in memory are generic to the compiler, which would otherwise
+// emit flat_* instructions (slower, and they also tie up the LDS counter the
+// traversal stack waits on).
+template <typename T>
+__device__ __forceinline__ T gld(const T *p)
+{
+  return *(const VRX_GLOBAL T *)p;
+}
+template <typename T>
+__device__ __forceinline__ T gld(const T *p, size_t i)
+{
+  return ((const VRX_GLOBAL T *)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gst(T *p, size_t i, const T &v)
+{
+  ((VRX_GLOBAL T *)p)[i] = v;
+}
+// HIP's vector types are classes whose copy constructors take generic
+// references, so 16-byte loads go through native vector types instead.
+typedef float vrx_f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int vrx_u4v __attribute__((ext_vector_type(4)));
+typedef float vrx_f2v_a4 __attribute__((ext_vector_type(2))) __attribute__((aligned(4)));
+__device__ __forceinline__ float4 gld(const float4 *p, size_t i)
+{
+  vrx_f4v v = ((const VRX_GLOBAL vrx_f4v *)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 gld(const uint4 *p, size_t i)
+{
+  vrx_u4v v = ((const VRX_GLOBAL vrx_u4v *)p)[i];
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst(float4 *p, size_t i, const float4 &v)
+{
+  vrx_f4v w = {v.x, v.y, v.z, v.w};
+  ((VRX_GLOBAL vrx_f4v *)p)[i] = w;
+}
+// two adjacent floats at 4-byte alignment (one 8-byte gather)
+__device__ __forceinline__ float2 gld_pair(const VRX_GLOBAL float *p)
+{
+  vrx_f2v_a4 v = *(const VRX_GLOBAL vrx_f2v_a4 *)p;
+  return make_float2(v.x, v.y);
+}
 
 struct v3 {
   float x, y, z;
@@ -37,6 +84,11 @@ VRX_HD v3 vmax(v3 a, v3 b) { return mk(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a
 VRX_HD float comp(v3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
 VRX_HD float cmax3(v3 v) { return fmaxf(fmaxf(v.x, v.y), v.z); }
 VRX_HD v3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
+__device__ __forceinline__ v3 gld3(const float *p, size_t i)
+{
+  const VRX_GLOBAL float *q = (const VRX_GLOBAL float *)p + 3 * i;
+  return mk(q[0], q[1], q[2]);
+}
 
 // 3x4 column-major transform (4 columns of 3): p' = M[:,0]x + M[:,1]y + M[:,2]z + M[:,3]
 VRX_HD v3 xfm_point(const float *m, v3 p)

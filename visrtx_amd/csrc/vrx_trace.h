@@ -40,9 +40,9 @@ struct Stack {
     if (cnt >= sd) {
       uint32_t pos = cnt - sd;
       if (pos < ovf_cap)
-        ovf[(size_t)pos * ovf_stride] = lds[slot * stride];
+        gst(ovf, (size_t)pos * ovf_stride, lds[slot * stride]);
       else
-        *err = 1u;
+        gst(err, 0, 1u);
     }
     lds[slot * stride] = x;
     cnt++;
@@ -54,7 +54,7 @@ struct Stack {
     uint32_t x = lds[slot * stride];
     if (cnt >= sd) {
       uint32_t pos = cnt - sd;
-      lds[slot * stride] = pos < ovf_cap ? ovf[(size_t)pos * ovf_stride] : kMarker;
+      lds[slot * stride] = pos < ovf_cap ? gld(ovf, (size_t)pos * ovf_stride) : kMarker;
     }
     return x;
   }
@@ -252,23 +252,28 @@ __device__ __forceinline__ void load_blas(
     const vrx_scene &sc, uint32_t e, v3 wo, v3 wd, bool cull_all, bool any, BlasCtx &b, v3 &o,
     v3 &d, uint32_t &root, uint32_t &n)
 {
-  const vrx_entry &E = sc.d_entries[e];
-  const vrx_geom &G = sc.d_geoms[E.geom];
-  b.nodes = (const Node *)G.bvh.d_nodes;
-  b.leaves = (const float4 *)G.bvh.d_leaves;
-  b.ids = G.bvh.d_leaf_ids;
-  b.type = G.type;
-  b.cull = !any && (cull_all || G.cull_backfaces != 0);
-  b.caps = G.caps != 0;
+  const vrx_entry *Ep = sc.d_entries + e;
+  const uint32_t gi = gld(&Ep->geom);
+  const vrx_geom *G = sc.d_geoms + gi;
+  b.nodes = (const Node *)gld(&G->bvh.d_nodes);
+  b.leaves = (const float4 *)gld(&G->bvh.d_leaves);
+  b.ids = gld(&G->bvh.d_leaf_ids);
+  b.type = gld(&G->type);
+  b.cull = !any && (cull_all || gld(&G->cull_backfaces) != 0);
+  b.caps = gld(&G->caps) != 0;
   b.entry = e;
-  n = G.n;
-  root = G.n == 1 ? kLeafBit : 0u;
-  if (E.identity) {
+  n = gld(&G->n);
+  root = n == 1 ? kLeafBit : 0u;
+  if (gld(&Ep->identity)) {
     o = wo;
     d = wd;
   } else {
-    o = xfm_point(E.inv, wo);
-    d = xfm_vector(E.inv, wd);
+    float inv[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++)
+      inv[q] = gld(&Ep->inv[q]);
+    o = xfm_point(inv, wo);
+    d = xfm_vector(inv, wd);
   }
 }
 
@@ -297,12 +302,12 @@ __device__ __forceinline__ bool test_leaf(
 {
   const int type = blas_type<PT>(b);
   if (type == VRX_SPHERE) {
-    float4 L = b.leaves[k];
+    float4 L = gld(b.leaves, k);
     float t;
     if (isect_sphere(o, d, mk(L.x, L.y, L.z), L.w, t) && t >= tmin && t <= tmax) {
       if (ANY)
         return true;
-      uint32_t prim = b.ids[k];
+      uint32_t prim = gld(b.ids, k);
       if (take(h, t, tmax, b.entry, prim)) {
         h.found = true;
         h.t = t;
@@ -313,7 +318,7 @@ __device__ __forceinline__ bool test_leaf(
     }
   } else if (type == VRX_TRIANGLE) {
     const float4 *L = b.leaves + 3 * (size_t)k;
-    float4 A = L[0], B = L[1], C = L[2];
+    float4 A = gld(L, 0), B = gld(L, 1), C = gld(L, 2);
     v3 v0 = mk(A.x, A.y, A.z), v1 = mk(B.x, B.y, B.z), v2 = mk(C.x, C.y, C.z);
     float t, u, v;
     if (isect_tri(wt, o, v0, v1, v2, t, u, v) && t >= tmin && t <= tmax) {
@@ -324,7 +329,7 @@ __device__ __forceinline__ bool test_leaf(
       }
       if (ANY)
         return true;
-      uint32_t prim = b.ids[k];
+      uint32_t prim = gld(b.ids, k);
       if (take(h, t, tmax, b.entry, prim)) {
         h.found = true;
         h.t = t;
@@ -337,7 +342,7 @@ __device__ __forceinline__ bool test_leaf(
     }
   } else if (type == VRX_CYLINDER) {
     const float4 *L = b.leaves + 2 * (size_t)k;
-    float4 P0 = L[0], P1 = L[1];
+    float4 P0 = gld(L, 0), P1 = gld(L, 1);
     float ts[3];
     v3 ns[3];
     int vm = isect_cylinder(o, d, mk(P0.x, P0.y, P0.z), mk(P1.x, P1.y, P1.z), P0.w,
@@ -350,7 +355,7 @@ __device__ __forceinline__ bool test_leaf(
       if (t >= tmin && t <= tmax) {
         if (ANY)
           return true;
-        uint32_t prim = b.ids[k];
+        uint32_t prim = gld(b.ids, k);
         if (take(h, t, tmax, b.entry, prim)) {
           h.found = true;
           h.t = t;
@@ -410,7 +415,7 @@ __device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax,
   for (;;) {
     bool do_pop = true;
     if (!(node & kLeafBit)) {
-      const Node nd = nodes[node];
+      const Node nd = gld_node(nodes, node);
       float tn0, tn1;
       bool h0 = slab(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, tmin, h.t, tn0);
       bool h1 = slab(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, tmin, h.t, tn1);
@@ -436,7 +441,7 @@ __device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax,
         return true;
     } else {
       // instance leaf: enter the entry's BLAS in object space
-      uint32_t e = tlas_ids[node & ~kLeafBit];
+      uint32_t e = gld(tlas_ids, node & ~kLeafBit);
       uint32_t n, root;
       load_blas(sc, e, wo, wd, cull_all, ANY, b, o, d, root, n);
       if (n != 0) {
@@ -484,31 +489,37 @@ struct SurfHit {
 __device__ __forceinline__ void populate(
     const vrx_scene &sc, const Hit &h, v3 wo, v3 wd, SurfHit &s)
 {
-  const vrx_entry &E = sc.d_entries[h.entry];
-  const vrx_geom &G = sc.d_geoms[E.geom];
+  const vrx_entry *Ep = sc.d_entries + h.entry;
+  const vrx_geom *Gp = sc.d_geoms + gld(&Ep->geom);
+  const int gtype = gld(&Gp->type);
+  const float *pos = gld(&Gp->d_position);
+  const uint32_t *index = gld(&Gp->d_index);
   v3 o = wo, d = wd;
-  const bool ident = E.identity != 0;
+  const bool ident = gld(&Ep->identity) != 0;
+  float inv[12];
   if (!ident) {
-    o = xfm_point(E.inv, wo);
-    d = xfm_vector(E.inv, wd);
+#pragma unroll
+    for (int q = 0; q < 12; q++)
+      inv[q] = gld(&Ep->inv[q]);
+    o = xfm_point(inv, wo);
+    d = xfm_vector(inv, wd);
   }
   v3 ng, ns;
-  if (G.type == VRX_TRIANGLE) {
+  if (gtype == VRX_TRIANGLE) {
     uint32_t p = h.prim;
-    uint32_t ia = G.d_index ? G.d_index[3 * (size_t)p] : 3 * p;
-    uint32_t ib = G.d_index ? G.d_index[3 * (size_t)p + 1] : 3 * p + 1;
-    uint32_t ic = G.d_index ? G.d_index[3 * (size_t)p + 2] : 3 * p + 2;
-    v3 v0 = ld3(G.d_position + 3 * (size_t)ia), v1 = ld3(G.d_position + 3 * (size_t)ib),
-       v2 = ld3(G.d_position + 3 * (size_t)ic);
+    uint32_t ia = index ? gld(index, 3 * (size_t)p) : 3 * p;
+    uint32_t ib = index ? gld(index, 3 * (size_t)p + 1) : 3 * p + 1;
+    uint32_t ic = index ? gld(index, 3 * (size_t)p + 2) : 3 * p + 2;
+    v3 v0 = gld3(pos, ia), v1 = gld3(pos, ib), v2 = gld3(pos, ic);
     v3 raw = cross(sub(v1, v0), sub(v2, v0));
     bool front = dot(raw, d) < 0.0f;
     ng = normalize(raw);
     if (!front)
       ng = neg(ng);
-    if (G.d_vertex_normal) {
+    const float *vn = gld(&Gp->d_vertex_normal);
+    if (vn) {
       float bx = (1.0f - h.a0) - h.a1, by = h.a0, bz = h.a1;
-      v3 n0 = ld3(G.d_vertex_normal + 3 * (size_t)ia), n1 = ld3(G.d_vertex_normal + 3 * (size_t)ib),
-         n2 = ld3(G.d_vertex_normal + 3 * (size_t)ic);
+      v3 n0 = gld3(vn, ia), n1 = gld3(vn, ib), n2 = gld3(vn, ic);
       ns = add(add(fmul(bx, n0), fmul(by, n1)), fmul(bz, n2));
     } else {
       ns = ng;
@@ -516,28 +527,29 @@ __device__ __forceinline__ void populate(
     ns = normalize(ns);
     if (dot(ng, ns) < 0.0f)
       ns = neg(ns);
-  } else if (G.type == VRX_SPHERE) {
-    uint32_t vi = G.d_index ? G.d_index[h.prim] : h.prim;
-    v3 c = ld3(G.d_position + 3 * (size_t)vi);
+  } else if (gtype == VRX_SPHERE) {
+    uint32_t vi = index ? gld(index, h.prim) : h.prim;
+    v3 c = gld3(pos, vi);
     v3 hp = add(o, fmul(h.t, d));
     ng = ns = sub(hp, c);
   } else {
     ng = ns = mk(h.a0, h.a1, h.a2);
   }
   if (!ident) {
-    ng = xfm_normal(E.inv, ng);
-    ns = xfm_normal(E.inv, ns);
+    ng = xfm_normal(inv, ng);
+    ns = xfm_normal(inv, ns);
   }
   s.ng = normalize(ng);
   s.ns = normalize(ns);
   s.t = h.t;
   s.hp = add(wo, fmul(h.t, wd));
   s.eps = eps_from(s.hp, wd, h.t);
-  s.prim_out = G.d_primitive_id ? G.d_primitive_id[h.prim] : h.prim;
-  s.obj = G.surface_id;
-  s.inst = E.inst_id;
-  s.color = mk(G.color[0], G.color[1], G.color[2]);
-  s.opacity = G.opacity;
+  const uint32_t *pid = gld(&Gp->d_primitive_id);
+  s.prim_out = pid ? gld(pid, h.prim) : h.prim;
+  s.obj = gld(&Gp->surface_id);
+  s.inst = gld(&Ep->inst_id);
+  s.color = mk(gld(&Gp->color[0]), gld(&Gp->color[1]), gld(&Gp->color[2]));
+  s.opacity = gld(&Gp->opacity);
 }
 
 } // namespace vrx
