@@ -1,0 +1,275 @@
+"""Pins the CPU oracle's intersectors, traversal, camera, channel packing and
+volume march (oracle/oracle.c) against independent references:
+
+- sphere: FP64 closed-form near root; rays starting inside report nothing
+  (only the near root is reported, Intersectors_ptx.cu:94-98);
+- triangle: FP64 Moller-Trumbore (hit/miss and t), watertightness on a fan;
+- cylinder: FP64 quadratic for the infinite cylinder clipped to the caps;
+- BVH closest hit == brute force over every primitive, including the
+  (t, entry, primitive) tie rule on duplicated primitives;
+- camera: geometric invariants (centre ray, corner ray, field of view);
+- packing: glm packUnorm4x8 / sRGB known answers;
+- volume: constant fields (closed-form transmittance) and linear fields
+  (trilinear reproduces them exactly).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from visrtx_amd import scenes as S
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+def test_sphere_matches_fp64_near_root():
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        o = rng.uniform(-3, 3, 3).astype(np.float32)
+        c = rng.uniform(-1, 1, 3).astype(np.float32)
+        r = np.float32(rng.uniform(0.05, 1.0))
+        d = _unit(c - o + rng.normal(0, 0.4, 3)).astype(np.float32)
+        t = np.zeros(1, np.float32)
+        hit = orc.lib().orc_intersect_sphere(o.ctypes.data, d.ctypes.data, c.ctypes.data,
+                                             float(r), t.ctypes.data)
+        oc = o.astype(np.float64) - c
+        b = np.dot(oc, d.astype(np.float64))
+        disc = b * b - (np.dot(oc, oc) - float(r) ** 2)
+        if disc > 1e-6:
+            assert hit
+            t_ref = -b - math.sqrt(disc)
+            assert abs(t[0] - t_ref) <= 2e-5 * max(1.0, abs(t_ref))
+        elif disc < -1e-6:
+            assert not hit
+
+
+def test_sphere_from_inside_reports_near_root_only():
+    o = np.zeros(3, np.float32)
+    d = np.array([0, 0, 1], np.float32)
+    c = np.zeros(3, np.float32)
+    t = np.zeros(1, np.float32)
+    assert orc.lib().orc_intersect_sphere(o.ctypes.data, d.ctypes.data, c.ctypes.data, 1.0,
+                                          t.ctypes.data)
+    assert t[0] == -1.0   # behind the origin: rejected by the [tmin, tmax] test
+
+
+def test_triangle_matches_fp64_moller_trumbore():
+    rng = np.random.default_rng(2)
+    bad = 0
+    for _ in range(3000):
+        v = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        o = rng.uniform(-3, 3, 3).astype(np.float32)
+        target = v.mean(0) + rng.normal(0, 0.5, 3)
+        d = _unit(target - o).astype(np.float32)
+        t, u, w = (np.zeros(1, np.float32) for _ in range(3))
+        hit = orc.lib().orc_intersect_triangle(o.ctypes.data, d.ctypes.data, v[0].ctypes.data,
+                                               v[1].ctypes.data, v[2].ctypes.data, t.ctypes.data,
+                                               u.ctypes.data, w.ctypes.data)
+        od, dd = o.astype(np.float64), d.astype(np.float64)
+        vd = v.astype(np.float64)
+        tr = orc.lib().orc_intersect_triangle_f64(od.ctypes.data, dd.ctypes.data,
+                                                  vd[0].ctypes.data, vd[1].ctypes.data,
+                                                  vd[2].ctypes.data)
+        if not math.isnan(tr):
+            if not hit:
+                bad += 1
+                continue
+            assert abs(t[0] - tr) <= 1e-4 * max(1.0, abs(tr))
+        elif hit:
+            bad += 1
+    assert bad <= 3   # edge-grazing rays where fp32 and fp64 legitimately disagree
+
+
+def test_triangle_fan_is_watertight():
+    """Rays through the shared vertex and edges of a fan hit at least one triangle."""
+    n = 12
+    verts = [np.zeros(3)] + [np.array([math.cos(2 * math.pi * k / n), math.sin(2 * math.pi * k / n),
+                                       0.0]) for k in range(n)]
+    tris = [(0, 1 + k, 1 + (k + 1) % n) for k in range(n)]
+    rng = np.random.default_rng(3)
+    for trial in range(400):
+        k = trial % n
+        a = verts[1 + k] * rng.uniform(0, 1) if trial % 3 else np.zeros(3)
+        o = np.array([a[0], a[1], 2.0], np.float32)
+        d = np.array([0, 0, -1], np.float32)
+        hits = 0
+        for (i, j, l) in tris:
+            v0, v1, v2 = (np.asarray(verts[x], np.float32) for x in (i, j, l))
+            t = np.zeros(1, np.float32)
+            u = np.zeros(1, np.float32)
+            w = np.zeros(1, np.float32)
+            hits += orc.lib().orc_intersect_triangle(o.ctypes.data, d.ctypes.data, v0.ctypes.data,
+                                                     v1.ctypes.data, v2.ctypes.data, t.ctypes.data,
+                                                     u.ctypes.data, w.ctypes.data)
+        assert hits >= 1
+
+
+def test_cylinder_matches_fp64_quadratic():
+    rng = np.random.default_rng(4)
+    checked = 0
+    for _ in range(3000):
+        p0 = rng.uniform(-1, 1, 3).astype(np.float32)
+        p1 = (p0 + rng.normal(0, 0.7, 3)).astype(np.float32)
+        r = np.float32(rng.uniform(0.05, 0.4))
+        o = rng.uniform(-3, 3, 3).astype(np.float32)
+        d = _unit((p0 + p1) / 2 - o + rng.normal(0, 0.3, 3)).astype(np.float32)
+        ts = np.zeros(3, np.float32)
+        ns = np.zeros(9, np.float32)
+        valid = np.zeros(3, np.int32)
+        orc.lib().orc_intersect_cylinder(o.ctypes.data, d.ctypes.data, p0.ctypes.data,
+                                         p1.ctypes.data, float(r), 0, ts.ctypes.data,
+                                         ns.ctypes.data, valid.ctypes.data)
+        # fp64: |(q - p0) x s|^2 = r^2 |s|^2 with q = o + t d, 0 < (q-p0).s < |s|^2
+        s = (p1 - p0).astype(np.float64)
+        f = o.astype(np.float64) - p0
+        dd = d.astype(np.float64)
+        A = np.cross(dd, s)
+        B = np.cross(f, s)
+        a = A @ A
+        b = 2 * (A @ B)
+        c = B @ B - float(r) ** 2 * (s @ s)
+        disc = b * b - 4 * a * c
+        if a < 1e-9 or abs(disc) < 1e-6:
+            continue
+        roots = []
+        if disc > 0:
+            for tr in ((-b - math.sqrt(disc)) / (2 * a), (-b + math.sqrt(disc)) / (2 * a)):
+                proj = (f + tr * dd) @ s
+                if 1e-4 * (s @ s) < proj < (1 - 1e-4) * (s @ s):
+                    roots.append(tr)
+        got = sorted(float(ts[k]) for k in range(2) if valid[k])
+        if len(roots) == len(got):
+            for g, e in zip(got, sorted(roots)):
+                assert abs(g - e) <= 2e-4 * max(1.0, abs(e))
+            checked += 1
+    assert checked > 2500
+
+
+def _random_scene(seed):
+    rng = np.random.default_rng(seed)
+    sph = S.GeomDesc("sphere", rng.random((300, 3), dtype=np.float32),
+                     radius=rng.uniform(0.005, 0.05, 300).astype(np.float32), surface_id=1)
+    tv = rng.random((3 * 200, 3), dtype=np.float32)
+    tri = S.GeomDesc("triangle", tv, surface_id=2)
+    cp = rng.random((2 * 150, 3), dtype=np.float32)
+    cyl = S.GeomDesc("cylinder", cp, radius=rng.uniform(0.003, 0.02, 150).astype(np.float32),
+                     caps=True, surface_id=3)
+    xfm = np.array([0, 1, 0, -1, 0, 0, 0, 0, 1, 0.2, 0.9, 0.1], np.float32)   # 90 deg about z
+    sc = S.SceneDesc(geoms=[sph, tri, cyl], world_geoms=[0, 1],
+                     instances=[S.InstanceDesc(geoms=[2], xfm=xfm, id=7)], world_id=4)
+    return sc, rng
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_bvh_closest_hit_equals_brute_force(seed):
+    sc, rng = _random_scene(seed)
+    os_ = orc.OracleScene(sc)
+    for _ in range(600):
+        o = rng.uniform(-0.5, 1.5, 3).astype(np.float32)
+        d = _unit(rng.uniform(0, 1, 3) - o).astype(np.float32)
+        a = os_.trace_closest(o, d)
+        b = os_.trace_brute(o, d)
+        assert (a is None) == (b is None)
+        if a is not None:
+            assert (a.t, a.entry, a.prim) == (b.t, b.entry, b.prim)
+
+
+def test_tie_rule_lowest_entry_then_primitive():
+    c = np.array([[0, 0, 0]] * 8, np.float32)
+    c[3] = c[7] = (0.5, 0.5, 0.5)
+    r = np.full(8, 0.1, np.float32)
+    g = S.GeomDesc("sphere", c, radius=r)
+    sc = S.SceneDesc(geoms=[g, g], world_geoms=[0, 1])   # the same spheres twice
+    os_ = orc.OracleScene(sc)
+    h = os_.trace_closest([0.5, 0.5, -1.0], [0, 0, 1])
+    assert (h.entry, h.prim) == (0, 3)
+    hb = os_.trace_brute([0.5, 0.5, -1.0], [0, 0, 1])
+    assert (hb.entry, hb.prim) == (0, 3)
+
+
+def test_camera_invariants():
+    fovy = math.radians(45.0)
+    cam = S.CameraDesc((0.5, -1.5, 1.2), (0.0, 2.0, -1.2), (0, 0, 1), fovy, 4 / 3)
+    W, H = 1024, 768
+    # pixel (0,0) is the bottom-left corner ray; (W/2, H/2) the centre ray
+    _, dc = orc.camera_ray(cam, W, H, W / 2, H / 2)
+    np.testing.assert_allclose(dc, _unit(cam.direction), atol=1e-6)
+    _, d_bot = orc.camera_ray(cam, W, H, W / 2, 0)
+    _, d_top = orc.camera_ray(cam, W, H, W / 2, H)
+    ang = math.acos(np.clip(np.dot(d_bot.astype(np.float64), d_top), -1, 1))
+    assert abs(ang - fovy) < 1e-5
+    _, d_l = orc.camera_ray(cam, W, H, 0, H / 2)
+    _, d_r = orc.camera_ray(cam, W, H, W, H / 2)
+    ang_x = math.acos(np.clip(np.dot(d_l.astype(np.float64), d_r), -1, 1))
+    assert abs(ang_x - 2 * math.atan(math.tan(fovy / 2) * 4 / 3)) < 1e-5
+    # bottom rows look down (z decreases with the up vector (0,0,1))
+    assert d_bot[2] < d_top[2]
+
+
+def test_packing_known_answers():
+    assert orc.pack_unorm4x8([0, 0, 0, 0]) == 0
+    assert orc.pack_unorm4x8([1, 1, 1, 1]) == 0xFFFFFFFF
+    assert orc.pack_unorm4x8([0.5, 0, 0, 0]) == 128          # round(127.5) = 128
+    assert orc.pack_unorm4x8([-1, 2, 0.25, 1]) == (0xFF << 24) | (64 << 16) | (255 << 8)
+    s = orc.linear_to_srgb([0.5, 0.001, 1.0, 0.3])
+    assert abs(s[0] - (1.055 * 0.5 ** 0.41666 - 0.055)) < 2e-7
+    assert abs(s[1] - 0.001 * 12.92) < 1e-9
+    assert abs(s[2] - 1.0) < 2e-7 and s[3] == np.float32(0.3)
+
+
+def _vol(data, lut, spacing=1.0 / 15):
+    return S.VolumeDesc(data=data, spacing=(spacing,) * 3, color=lut[0], opacity=lut[1],
+                        unit_distance=spacing)
+
+
+def test_volume_constant_field_closed_form():
+    """Constant field s, constant colour c and alpha a: after n samples
+    C = c (1 - tau^n), O = 1 - tau^n with tau = (1-a)^(step/unitDistance)."""
+    n = 16
+    data = np.full((n, n, n), 0.5, np.float32)
+    color = np.array([[0.2, 0.4, 0.8], [0.2, 0.4, 0.8]], np.float32)
+    a = 0.03
+    vol = _vol(data, (color, np.array([a, a], np.float32)))
+    cam = S.CameraDesc((0.5, 0.5, -1.0), (0, 0, 1), (0, 1, 0), math.radians(10), 1.0)
+    # 0.55-voxel step: the exit is not a multiple of the step (no knife edge)
+    img, ns = orc.volume_image(vol, cam, 8, 8, inv_sampling_rate=1.1, early_stop=False)
+    sp = np.float32(1.0 / 15)
+    step = float(sp / np.float32(2.0)) * 1.1
+    length = 15.0 * float(sp)            # axis-aligned centre ray through [0, 15*sp]
+    nsamp = int(math.floor(length / step)) + 1
+    tau = (1 - float(np.float32(a))) ** (step / float(sp))   # the LUT holds fp32 alpha
+    px = img[4, 4]
+    np.testing.assert_allclose(px[3], 1 - tau ** nsamp, rtol=1e-6)   # fp32 LUT rounding
+    np.testing.assert_allclose(px[:3], color[0] * (1 - tau ** nsamp), rtol=1e-6)
+
+
+def test_volume_linear_field_is_reproduced_exactly():
+    n = 9
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    data = (0.03 * x + 0.05 * y + 0.02 * z + 0.1).astype(np.float32)
+    v = orc.orc_volume()
+    v.dims[:] = [n, n, n]
+    v.origin[:] = [0.0, 0.0, 0.0]
+    v.spacing[:] = [0.5, 0.25, 2.0]
+    v.data = data.ctypes.data
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        p = np.array([rng.uniform(0, 4), rng.uniform(0, 2), rng.uniform(0, 16)])
+        gx, gy, gz = p[0] / 0.5, p[1] / 0.25, p[2] / 2.0
+        want = 0.03 * gx + 0.05 * gy + 0.02 * gz + 0.1
+        got = orc.lib().orc_sample_field(orc.C.byref(v), p.ctypes.data)
+        assert abs(got - want) < 1e-6
+
+
+def test_tf_discretization_matches_interpolation():
+    colors = np.array([[0, 0, 1], [0, 1, 0], [1, 0, 0]], np.float32)
+    op = np.array([0.0, 0.05], np.float32)
+    lut = orc.tf_discretize(colors, op)
+    p = np.arange(256) / 255.0
+    np.testing.assert_allclose(lut[:, 3], 0.05 * p, atol=1e-7)
+    for k in range(3):
+        np.testing.assert_allclose(lut[:, k], np.interp(p, [0, 0.5, 1], colors[:, k]), atol=1e-6)
