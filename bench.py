@@ -207,6 +207,96 @@ def bench_diffvol(iters=200, log_every=50):
             "iters": iters, "mse_first": first, "mse_final": last, "rays": int(org.shape[0])}
 
 
+def bench_batched(dev, reps=5):
+    """64 orbit views at 512x512 of 256 instances of 16 meshes (10k-2M
+    triangles): one autotuned persistent launch (eager and HIP-graph replay)
+    against 64 separate per-view ANARI frames rendered back to back."""
+    import torch
+    from visrtx_amd import anari as A
+    from visrtx_amd import batch as B
+    from visrtx_amd import scenes as S
+    t = time.perf_counter()
+    sc, cams = S.batched_workload()
+    tris = sum(g.num_prims for g in sc.geoms)
+    log(f"batched scene: {len(sc.geoms)} meshes, {tris} triangles, {len(sc.instances)} "
+        f"instances, generated in {time.perf_counter() - t:.1f}s")
+    s = S.build_anari(dev, sc)
+    r = S.make_renderer(dev, s, "raycast")
+    scene = B.scene_of(dev, s.world)
+    out = {"meshes": len(sc.geoms), "triangles": int(tris), "instances": len(sc.instances),
+           "views": len(cams), "res": 512,
+           "build_ms": round(1000.0 * dev.get_property(s.world, "bvhBuildDuration", A.FLOAT32),
+                             3)}
+    W = H = 512
+    # plain: one ANARI frame per view
+    frames = []
+    for cam in cams:
+        c = S.make_camera(dev, s, cam)
+        frames.append((c, S.make_frame(dev, s, s.world, c, r, W, H, color=A.UFIXED8_RGBA_SRGB)))
+
+    def plain():
+        for c, f in frames:
+            dev.commit(c)
+            dev.render(f)
+        for _, f in frames:
+            dev.ready(f)
+
+    plain()
+    device_sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        plain()
+    device_sync()
+    plain_ms = (time.perf_counter() - t) * 1000.0 / reps
+    kern_ms = sum(dev.get_property(f, "duration", A.FLOAT32) for _, f in frames) * 1000.0
+    for c, f in frames:
+        dev.release(f)
+        dev.release(c)
+
+    bf = B.BatchFrame(scene, B.renderer_of(dev, r), cams, W, H, srgb=True)
+    best = bf.autotune()
+    log(f"batched autotune: {best:.3f} ms, tile {bf.b.tile_w}x{bf.b.tile_h}, block "
+        f"{bf.b.block_threads}, LDS stack {bf.renderer.stack_lds}")
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        bf.render(stream)
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t = time.perf_counter()
+        e0.record(stream)
+        for _ in range(reps):
+            bf.render(stream)
+        e1.record(stream)
+        stream.synchronize()
+        eager_ms = (time.perf_counter() - t) * 1000.0 / reps
+        eager_ev = e0.elapsed_time(e1) / reps
+        bf.capture(stream)
+        bf.replay(stream)
+        stream.synchronize()
+        t = time.perf_counter()
+        e0.record(stream)
+        for _ in range(reps):
+            bf.replay(stream)
+        e1.record(stream)
+        stream.synchronize()
+        graph_ms = (time.perf_counter() - t) * 1000.0 / reps
+        graph_ev = e0.elapsed_time(e1) / reps
+    bf.release_graph()
+    rays = len(cams) * W * H
+    out.update({"rays": rays, "plain_ms": round(plain_ms, 3),
+                "plain_sum_of_kernels_ms": round(kern_ms, 3),
+                "batched_eager_ms": round(eager_ms, 3), "batched_eager_event_ms": round(eager_ev, 3),
+                "batched_graph_ms": round(graph_ms, 3), "batched_graph_event_ms": round(graph_ev, 3),
+                "speedup_vs_plain": round(plain_ms / graph_ms, 3),
+                "batched_mrays_s": round(rays / (graph_ms / 1000.0) / 1e6, 2),
+                "plain_mrays_s": round(rays / (plain_ms / 1000.0) / 1e6, 2),
+                "autotuned": {"tile": [bf.b.tile_w, bf.b.tile_h], "block": bf.b.block_threads,
+                              "stack_lds": bf.renderer.stack_lds}})
+    del bf
+    s.release()
+    return out
+
+
 def cpu_baseline(threads):
     """The oracle (C/OpenMP port of the same path) on a bounded sample of the
     10M-sphere workload: same scene and camera, 640x360 rays."""
@@ -327,6 +417,12 @@ def main():
             f"{extra['volume_gsamples_s']} Gsamples/s")
         vol.release()
         del sc3
+
+        log("batched 64 views")
+        bt = bench_batched(dev)
+        extra["batched_64x512"] = bt
+        log(f"batched: plain {bt['plain_ms']} ms, graph {bt['batched_graph_ms']} ms "
+            f"-> {bt['speedup_vs_plain']}x ({bt['batched_mrays_s']} Mrays/s)")
 
     dev.close()
 

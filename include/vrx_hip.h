@@ -35,6 +35,7 @@ extern "C" {
 
 typedef void *vrx_stream; /* hipStream_t; NULL = default stream */
 typedef void *vrx_event;  /* hipEvent_t */
+typedef void *vrx_graph;  /* hipGraphExec_t */
 
 enum vrx_status {
   VRX_OK = 0,
@@ -277,13 +278,29 @@ typedef struct vrx_batch {
   void *d_scratch;
   size_t scratch_bytes;
   uint32_t *d_work_counter;  /* 1 u32, persistent-launch tile queue */
-  int32_t tile_w, tile_h;    /* autotuned tile (pixels per wave = 64) */
-  int32_t persistent_blocks; /* 0 = one block per tile */
+  int32_t tile_w, tile_h;    /* pixels per tile, tile_w * tile_h == 64 (one wave) */
+  int32_t block_threads;     /* 64, 128 or 256 */
+  int32_t persistent_blocks; /* 0 = fill the device at the kernel's occupancy */
 } vrx_batch;
 
+/* Renders num_views views of one scene (raycast renderer: color, depth and
+ * id channels) in ONE persistent launch: every wave pulls 64-pixel tiles of
+ * any view from the d_work_counter queue.  The counter is reset by a kernel
+ * on stream s, so the call is graph-capturable.  Output index of pixel
+ * (x, y) of view v is v*w*h + y*w + x, row 0 at the bottom. */
 size_t vrx_batch_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b);
 int vrx_render_batch(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b,
     vrx_stream s);
+/* Captures vrx_render_batch on stream s (not the default stream) into an
+ * instantiated HIP graph; the graph holds the structs' values at capture. */
+int vrx_batch_graph_create(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b,
+    vrx_stream s, vrx_graph *out);
+int vrx_graph_launch(vrx_graph g, vrx_stream s);
+int vrx_graph_destroy(vrx_graph g);
+/* Times tile shape, workgroup size and LDS stack depth on this scene and
+ * writes the fastest into r->stack_lds and b->tile_w/tile_h/block_threads. */
+int vrx_batch_autotune(const vrx_scene *scene, vrx_renderer *r, vrx_batch *b, vrx_stream s,
+    float *best_ms_out);
 
 /* ---- synthetic scene generators (host, OpenMP) ---------------------------- */
 uint32_t vrx_gen_pcg32(uint64_t *state, uint64_t inc);

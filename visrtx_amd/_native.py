@@ -25,6 +25,16 @@ def _load(path: str) -> C.CDLL:
         raise VrxError(
             f"native library {os.path.basename(path)} is not built; run "
             "`python -c 'import __graft_entry__ as g; g.build()'` (or make -C visrtx_amd)")
+    # torch ships its own libamdhip64 (same soname, libamdhip64.so.7) and loads
+    # it by a different file name; importing torch first makes the dynamic
+    # linker bind this library to that one runtime, so torch's streams and
+    # allocations and ours live in a single HIP runtime.  Loaded the other way
+    # round, the process would hold two runtimes and a torch stream handle
+    # passed to vrx_* would be meaningless.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return C.CDLL(path, mode=C.RTLD_GLOBAL)
 
 
@@ -63,6 +73,32 @@ class vrx_bvh_header(C.Structure):
 
 class vrx_camera(C.Structure):
     _fields_ = [("pos", f3), ("d00", f3), ("du", f3), ("dv", f3), ("region", f4)]
+
+
+class vrx_scene(C.Structure):
+    _fields_ = [("d_geoms", C.c_void_p), ("num_geoms", C.c_uint32), ("d_entries", C.c_void_p),
+                ("num_entries", C.c_uint32), ("tlas", vrx_bvh), ("tlas_height", C.c_uint32),
+                ("max_blas_height", C.c_uint32), ("d_volumes", C.c_void_p),
+                ("num_volumes", C.c_uint32), ("single_type", C.c_int32),
+                ("max_volume_voxels", C.c_uint64)]
+
+
+class vrx_renderer(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("background", f4), ("ambient_color", f3),
+                ("ambient_radiance", C.c_float), ("pixel_samples", C.c_int32),
+                ("ambient_samples", C.c_int32), ("ao_distance", C.c_float),
+                ("cull_backfaces", C.c_int32), ("inv_volume_sampling_rate", C.c_float),
+                ("stack_lds", C.c_int32), ("block_pixels", C.c_int32), ("vol_unroll", C.c_int32)]
+
+
+class vrx_batch(C.Structure):
+    _fields_ = [("num_views", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("d_cams", C.c_void_p), ("d_depth", C.c_void_p), ("d_prim_id", C.c_void_p),
+                ("d_obj_id", C.c_void_p), ("d_inst_id", C.c_void_p), ("d_color_f32", C.c_void_p),
+                ("d_color_u32", C.c_void_p), ("d_scratch", C.c_void_p),
+                ("scratch_bytes", C.c_size_t), ("d_work_counter", C.c_void_p),
+                ("tile_w", C.c_int32), ("tile_h", C.c_int32), ("block_threads", C.c_int32),
+                ("persistent_blocks", C.c_int32)]
 
 
 class vrx_diffvol(C.Structure):
@@ -125,6 +161,10 @@ def hip() -> C.CDLL:
             "vrx_make_rays": (i32, [vp, u32, u32, u32, vp, vp, vp]),
             "vrx_batch_scratch_bytes": (sz, [vp, vp, vp]),
             "vrx_render_batch": (i32, [vp, vp, vp, vp]),
+            "vrx_batch_graph_create": (i32, [vp, vp, vp, vp, vp]),
+            "vrx_graph_launch": (i32, [vp, vp]),
+            "vrx_graph_destroy": (i32, [vp]),
+            "vrx_batch_autotune": (i32, [vp, vp, vp, vp, vp]),
             "vrx_gen_pcg32": (u32, [vp, C.c_uint64]),
             "vrx_gen_value_noise3": (C.c_float, [C.c_uint64, C.c_float, C.c_float, C.c_float]),
             "vrx_gen_spheres": (None, [C.c_uint64, u32, C.c_float, C.c_float, vp, vp]),

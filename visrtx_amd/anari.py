@@ -194,6 +194,8 @@ class Device:
             v = C.c_uint64()
         elif typ == FLOAT32_BOX3:
             v = (C.c_float * 6)()
+        elif typ == VOID_POINTER:
+            v = C.c_void_p()
         else:
             raise ValueError(typ)
         ok = self.L.anariGetProperty(self.h, obj if obj is not None else self.h, name.encode(),

@@ -1053,6 +1053,17 @@ bool World::getProperty(
     memcpy(mem, &sec, 4);
     return true;
   }
+  // extension: the device-side scene description (vrx_scene of vrx_hip.h),
+  // valid until the world is next rebuilt; lets a caller drive the batched
+  // multi-view entry points on a scene built through this API
+  if (name == "vrxScene" && t == ANARI_VOID_POINTER && size >= sizeof(void *)) {
+    dev->flushCommits();
+    if (needsRebuild() && rebuild() != 0)
+      return false;
+    const void *p = &scene;
+    memcpy(mem, &p, sizeof(p));
+    return true;
+  }
   return Object::getProperty(name, t, mem, size, mask);
 }
 
@@ -1111,6 +1122,20 @@ void Renderer::commit()
   if (kind < 0)
     report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
         "renderer subtype '%s' is not supported (raycast, ao)", subtype.c_str());
+}
+
+bool Renderer::getProperty(
+    const std::string &name, ANARIDataType t, void *mem, uint64_t size, uint32_t mask)
+{
+  // extension: the committed vrx_renderer (vrx_hip.h), valid until the next
+  // commit of this renderer
+  if (name == "vrxRenderer" && t == ANARI_VOID_POINTER && size >= sizeof(void *)) {
+    dev->flushCommits();
+    const void *p = &r;
+    memcpy(mem, &p, sizeof(p));
+    return true;
+  }
+  return Object::getProperty(name, t, mem, size, mask);
 }
 
 // ---------------------------------------------------------------------------

@@ -230,6 +230,8 @@ struct Renderer : Object {
   Renderer(Device *d, const std::string &sub);
   void commit() override;
   bool isValid() const override { return kind >= 0; }
+  bool getProperty(const std::string &name, ANARIDataType type, void *mem, uint64_t size,
+      uint32_t mask) override;
   int kind{-1};
   vrx_renderer r{};
   int sampleLimit{128};

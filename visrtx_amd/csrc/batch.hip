@@ -1,14 +1,315 @@
-// batch.hip -- batched multi-view rendering.
+// batch.hip -- many views of one scene in one persistent launch.
+//
+// The reference renders one view per optixLaunch (frame/Frame.cu:282-289);
+// it has no batched form.  Here every wave of a persistent grid pulls 64-pixel
+// tiles of any view from one atomic work queue and runs the same per-pixel
+// raycast program as the per-view kernel (vrx_shade.h), so one launch renders
+// all views with no per-view tail, and the frame can be captured once in a
+// HIP graph (the queue counter is reset by a fill kernel inside the graph,
+// never by a memset node).  vrx_batch_autotune times tile shape, workgroup
+// size and LDS stack depth on the actual scene and keeps the fastest.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
 #include "vrx_internal.h"
+#include "vrx_shade.h"
+
+namespace vrx {
+
+constexpr int kBatchMaxBlock = 256;
+constexpr uint32_t kMaxThreadsPerCU = 2048;
+
+__device__ __forceinline__ vrx_camera load_camera(const vrx_camera *cams, uint32_t view)
+{
+  vrx_camera c;
+  const float *src = (const float *)(cams + view);
+  float *dst = (float *)&c;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(vrx_camera) / 4); i++)
+    dst[i] = gld(src, i);
+  return c;
+}
+
+template <int PT, bool TWO>
+__global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, vrx_renderer r,
+    vrx_batch b, uint32_t sd, uint32_t cap, uint32_t *scratch, uint32_t tiles_x, uint32_t tiles_y)
+{
+  extern __shared__ uint32_t s_stack[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  Stack st;
+  st.lds = s_stack + tid;
+  st.stride = blockDim.x;
+  st.sd = sd;
+  st.sd_mask = sd - 1;
+  const uint32_t gthreads = gridDim.x * blockDim.x;
+  const uint32_t gtid = blockIdx.x * blockDim.x + tid;
+  st.ovf = scratch + 64 + gtid;
+  st.ovf_stride = gthreads;
+  st.ovf_cap = cap;
+  st.err = scratch;
+  st.cnt = 0;
+
+  vrx_frame f = {};
+  f.width = b.width;
+  f.height = b.height;
+  f.color_format = b.d_color_f32 ? VRX_COLOR_F32 : VRX_COLOR_SRGB8;
+  f.d_color_f32 = b.d_color_f32;
+  f.d_color_u32 = b.d_color_u32;
+  f.d_depth = b.d_depth;
+  f.d_prim_id = b.d_prim_id;
+  f.d_obj_id = b.d_obj_id;
+  f.d_inst_id = b.d_inst_id;
+
+  const uint32_t per_view = tiles_x * tiles_y;
+  const uint32_t total = per_view * b.num_views;
+  const uint32_t tw = (uint32_t)b.tile_w;
+  uint32_t rays = 0, vsamples = 0;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0)
+      t = atomicAdd(b.d_work_counter, 1u);
+    t = __shfl(t, 0, 64);
+    if (t >= total)
+      break;
+    const uint32_t view = t / per_view;
+    const uint32_t rem = t - view * per_view;
+    const uint32_t ty = rem / tiles_x, tx = rem - ty * tiles_x;
+    const uint32_t px = tx * tw + (uint32_t)lane % tw;
+    const uint32_t py = ty * (uint32_t)b.tile_h + (uint32_t)lane / tw;
+    if (px < b.width && py < b.height) {
+      const vrx_camera cam = load_camera(b.d_cams, view);
+      const size_t idx = (size_t)view * b.width * b.height + (size_t)py * b.width + px;
+      render_pixel<PT, TWO, VRX_RENDER_RAYCAST>(sc, cam, r, f, px, py, idx, st, rays, vsamples);
+    }
+  }
+}
+
+static int g_num_cus = 0;
+
+static int num_cus()
+{
+  if (!g_num_cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+      return 256;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess)
+      return 256;
+    g_num_cus = p.multiProcessorCount;
+  }
+  return g_num_cus;
+}
+
+static uint32_t pick_sd(int want)
+{
+  uint32_t p = 8;
+  while (p < (uint32_t)(want > 0 ? want : 32) && p < 64)
+    p <<= 1;
+  return p;
+}
+
+static uint32_t cap_for(const vrx_scene &sc, uint32_t sd)
+{
+  uint32_t m = sc.max_blas_height + 2;
+  if (sc.num_entries > 1)
+    m += sc.tlas_height + 1;
+  return m > sd ? m - sd : 0u;
+}
+
+template <int PT, bool TWO>
+static const void *kernel_ptr()
+{
+  return (const void *)&k_render_batch<PT, TWO>;
+}
+
+static const void *pick_kernel(const vrx_scene &sc)
+{
+  const bool two = sc.num_entries > 1;
+  switch (sc.single_type) {
+  case VRX_TRIANGLE:
+    return two ? kernel_ptr<VRX_TRIANGLE, true>() : kernel_ptr<VRX_TRIANGLE, false>();
+  case VRX_SPHERE:
+    return two ? kernel_ptr<VRX_SPHERE, true>() : kernel_ptr<VRX_SPHERE, false>();
+  case VRX_CYLINDER:
+    return two ? kernel_ptr<VRX_CYLINDER, true>() : kernel_ptr<VRX_CYLINDER, false>();
+  default:
+    return two ? kernel_ptr<-1, true>() : kernel_ptr<-1, false>();
+  }
+}
+
+__global__ void k_reset_counter(uint32_t *c)
+{
+  if (threadIdx.x == 0)
+    *c = 0u;
+}
+
+static int launch_batch(const vrx_scene &sc, const vrx_renderer &r, const vrx_batch &b, hipStream_t s)
+{
+  const int block = std::min(std::max(b.block_threads > 0 ? b.block_threads : 256, 64), 256) & ~63;
+  const uint32_t sd = pick_sd(r.stack_lds);
+  const uint32_t cap = cap_for(sc, sd);
+  const size_t lds = (size_t)sd * block * 4;
+  const void *kern = pick_kernel(sc);
+  int per_cu = 0;
+  VRX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds));
+  if (per_cu < 1)
+    per_cu = 1;
+  uint32_t blocks = b.persistent_blocks > 0 ? (uint32_t)b.persistent_blocks
+                                            : (uint32_t)(num_cus() * per_cu);
+  const uint32_t max_blocks = (uint32_t)num_cus() * kMaxThreadsPerCU / (uint32_t)block;
+  blocks = std::min(blocks, max_blocks);
+  const uint32_t tw = (uint32_t)b.tile_w, th = (uint32_t)b.tile_h;
+  const uint32_t tiles_x = (b.width + tw - 1) / tw, tiles_y = (b.height + th - 1) / th;
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, b.d_work_counter);
+  void *args[] = {(void *)&sc, (void *)&r, (void *)&b, (void *)&sd, (void *)&cap,
+      (void *)&b.d_scratch, (void *)&tiles_x, (void *)&tiles_y};
+  VRX_HIP(hipLaunchKernel(kern, dim3(blocks), dim3(block), args, lds, s));
+  return VRX_OK;
+}
+
+static int validate(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b)
+{
+  VRX_ARG(scene && r && b, "null argument");
+  VRX_ARG(b->num_views > 0 && b->width > 0 && b->height > 0, "batch dimensions");
+  VRX_ARG(b->d_cams && b->d_work_counter, "cameras / work counter");
+  VRX_ARG(b->tile_w > 0 && b->tile_h > 0 && b->tile_w * b->tile_h == 64, "tile must hold 64 pixels");
+  VRX_ARG(scene->num_entries == 0 || (scene->d_entries && scene->d_geoms), "scene arrays");
+  VRX_ARG(scene->num_entries <= 1 || scene->tlas.d_leaf_ids, "instance BVH not built");
+  VRX_ARG(scene->num_volumes == 0 || scene->d_volumes, "volume array");
+  size_t need = vrx_batch_scratch_bytes(scene, r, b);
+  if (!b->d_scratch || b->scratch_bytes < need) {
+    set_error("batch scratch too small: %zu < %zu", b->scratch_bytes, need);
+    return VRX_ERR_ARG;
+  }
+  return VRX_OK;
+}
+
+} // namespace vrx
+
+using namespace vrx;
 
 extern "C" {
+
 size_t vrx_batch_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b)
 {
-  return 0;
+  (void)b;
+  if (!scene)
+    return 0;
+  // sized for the smallest LDS stack the autotuner tries (16) and a grid of
+  // at most 2048 resident lanes per CU
+  const uint32_t sd = std::min(pick_sd(r ? r->stack_lds : 0), 16u);
+  const uint32_t cap = cap_for(*scene, sd);
+  return 256 + (size_t)num_cus() * kMaxThreadsPerCU * cap * 4;
 }
+
 int vrx_render_batch(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b, vrx_stream s)
 {
-  vrx::set_error("not implemented");
-  return VRX_ERR_ARG;
+  int st = validate(scene, r, b);
+  if (st)
+    return st;
+  st = launch_batch(*scene, *r, *b, (hipStream_t)s);
+  if (st)
+    return st;
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
 }
+
+int vrx_batch_graph_create(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b,
+    vrx_stream s, vrx_graph *out)
+{
+  VRX_ARG(out && s, "a non-default stream and an output handle are required");
+  int st = validate(scene, r, b);
+  if (st)
+    return st;
+  hipStream_t hs = (hipStream_t)s;
+  hipGraph_t g = nullptr;
+  VRX_HIP(hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal));
+  int lst = launch_batch(*scene, *r, *b, hs);
+  hipError_t e = hipStreamEndCapture(hs, &g);
+  if (lst)
+    return lst;
+  if (e != hipSuccess)
+    return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  hipGraphExec_t ge = nullptr;
+  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  *out = (vrx_graph)ge;
+  return VRX_OK;
 }
+
+int vrx_graph_launch(vrx_graph g, vrx_stream s)
+{
+  VRX_ARG(g, "graph");
+  VRX_HIP(hipGraphLaunch((hipGraphExec_t)g, (hipStream_t)s));
+  return VRX_OK;
+}
+
+int vrx_graph_destroy(vrx_graph g)
+{
+  if (g)
+    VRX_HIP(hipGraphExecDestroy((hipGraphExec_t)g));
+  return VRX_OK;
+}
+
+int vrx_batch_autotune(const vrx_scene *scene, vrx_renderer *r, vrx_batch *b, vrx_stream s,
+    float *best_ms_out)
+{
+  int st = validate(scene, r, b);
+  if (st)
+    return st;
+  hipStream_t hs = (hipStream_t)s;
+  struct Cand {
+    int tw, th, block, sd;
+  };
+  std::vector<Cand> cands;
+  const int tiles[][2] = {{8, 8}, {16, 4}, {32, 2}, {64, 1}};
+  for (auto &t : tiles)
+    for (int block : {64, 128, 256})
+      for (int sd : {16, 32})
+        cands.push_back({t[0], t[1], block, sd});
+  hipEvent_t e0, e1;
+  VRX_HIP(hipEventCreate(&e0));
+  VRX_HIP(hipEventCreate(&e1));
+  float best = 3.4e38f;
+  Cand bestc = cands[0];
+  vrx_renderer rr = *r;
+  vrx_batch bb = *b;
+  bb.persistent_blocks = 0;
+  for (const Cand &c : cands) {
+    rr.stack_lds = c.sd;
+    bb.tile_w = c.tw;
+    bb.tile_h = c.th;
+    bb.block_threads = c.block;
+    if ((st = launch_batch(*scene, rr, bb, hs)))
+      return st; // warm-up
+    VRX_HIP(hipEventRecord(e0, hs));
+    for (int rep = 0; rep < 3; rep++)
+      if ((st = launch_batch(*scene, rr, bb, hs)))
+        return st;
+    VRX_HIP(hipEventRecord(e1, hs));
+    VRX_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    VRX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    ms /= 3.0f;
+    if (ms < best) {
+      best = ms;
+      bestc = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  r->stack_lds = bestc.sd;
+  b->tile_w = bestc.tw;
+  b->tile_h = bestc.th;
+  b->block_threads = bestc.block;
+  b->persistent_blocks = 0;
+  if (best_ms_out)
+    *best_ms_out = best;
+  return VRX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,512 @@
+// vrx_shade.h -- device-side per-pixel programs shared by the per-view and
+// the batched multi-view kernels: pinhole rays, the emission-absorption
+// volume march, channel packing and the raycast / AO pixel program.
+#pragma once
+
+#include <float.h>
+#include <math.h>
+
+#include "vrx_trace.h"
+
+namespace vrx {
+
+constexpr int kTile = 16;
+constexpr int kBlock = kTile * kTile;
+// Hard bounds that keep every loop finite on degenerate input (a zero
+// volume step, a surface hit at the ray origin with zero epsilon, volumes
+// whose exit distance no longer grows by 1e-3 in fp32).
+constexpr int kMaxSurfaceLayers = 4096;
+constexpr uint32_t kMaxSamplesPerSegment = 1u << 24;
+constexpr int kMaxVolumeSegments = 256;
+
+// --------------------------------------------------------------------------
+// camera
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+__device__ __forceinline__ void cam_ray(const vrx_camera &c, float sx, float sy, v3 &o, v3 &d)
+{
+  sx = mixf(c.region[0], c.region[2], sx);
+  sy = mixf(c.region[1], c.region[3], sy);
+  o = ld3(c.pos);
+  d = normalize(add(add(ld3(c.d00), fmul(sx, ld3(c.du))), fmul(sy, ld3(c.dv))));
+}
+
+// --------------------------------------------------------------------------
+// volumes (fp32)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ bool vol_slab(
+    const vrx_volume &v, v3 o, v3 d, float tmin, float tmax, float &t0, float &t1)
+{
+  float lo[3] = {v.origin[0], v.origin[1], v.origin[2]};
+  float hi[3];
+  float tn = -INFINITY, tf = INFINITY;
+  float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    hi[k] = v.origin[k] + ((float)v.dims[k] - 1.0f) * v.spacing[k];
+    float inv = 1.0f / dd[k];
+    float a = (lo[k] - oo[k]) * inv, b = (hi[k] - oo[k]) * inv;
+    tn = fmaxf(tn, fminf(a, b));
+    tf = fminf(tf, fmaxf(a, b));
+  }
+  if (!(tn < tf))
+    return false;
+  if (tf < tmin || tn > tmax)
+    return false;
+  t0 = fminf(fmaxf(tn, tmin), tmax);
+  t1 = fminf(fmaxf(tf, tmin), tmax);
+  return true;
+}
+
+// Per-volume constants of the march, computed once per ray segment.
+struct VolCtx {
+  const float *data;
+  int dims[3], imax[3];
+  float gmax[3];
+  float origin[3], inv_sp[3];
+  uint32_t sy, sz;
+  float step, expo, lo, hi, inv_range;
+};
+
+__device__ __forceinline__ float vol_step(const vrx_volume &v)
+{
+  // StructuredRegularField.cpp:175-178 stepSize() = min(spacing) / 2
+  return fminf(fminf(v.spacing[0] / 2.0f, v.spacing[1] / 2.0f), v.spacing[2] / 2.0f);
+}
+
+__device__ __forceinline__ VolCtx vol_ctx(const vrx_volume &v, float inv_rate)
+{
+  VolCtx c;
+  c.data = v.d_data;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    c.dims[k] = (int)v.dims[k];
+    c.imax[k] = max((int)v.dims[k] - 2, 0);
+    c.gmax[k] = (float)v.dims[k] - 1.0f;
+    c.origin[k] = v.origin[k];
+    c.inv_sp[k] = 1.0f / v.spacing[k];
+  }
+  c.sy = v.dims[0];
+  c.sz = v.dims[0] * v.dims[1];
+  c.step = vol_step(v) * inv_rate;
+  c.expo = c.step * (1.0f / v.unit_distance);
+  c.lo = v.value_range[0];
+  c.hi = v.value_range[1];
+  c.inv_range = 1.0f / (c.hi - c.lo);
+  return c;
+}
+
+// Software trilinear at voxel coordinates g = (p - origin) / spacing with edge
+// clamp: the structured-regular field's clamp-to-edge linear texture
+// (StructuredRegularField.cpp:98-159, sampleSpatialField.h:54-78) with full
+// fp32 weights.  I64 selects 64-bit gather offsets for fields of >= 2^31 voxels.
+template <bool I64>
+__device__ __forceinline__ float trilinear(const VolCtx &c, float gx, float gy, float gz)
+{
+  float g[3] = {gx, gy, gz};
+  int i0[3], i1[3];
+  float f[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float x = fminf(fmaxf(g[k], 0.0f), c.gmax[k]);
+    int i = min((int)x, c.imax[k]);
+    i0[k] = i;
+    i1[k] = min(i + 1, c.dims[k] - 1);
+    f[k] = x - (float)i;
+  }
+  float c000, c100, c010, c110, c001, c101, c011, c111;
+  if (c.dims[0] >= 2) {
+    // x-neighbours are adjacent floats (i0 <= dims-2): one 8-byte gather each
+    const VRX_GLOBAL float *D = (const VRX_GLOBAL float *)c.data;
+    float2 p00, p10, p01, p11;
+    if constexpr (I64) {
+      size_t z0 = (size_t)i0[2] * c.sz, z1 = (size_t)i1[2] * c.sz;
+      size_t y0 = (size_t)i0[1] * c.sy, y1 = (size_t)i1[1] * c.sy;
+      p00 = gld_pair(D + z0 + y0 + i0[0]);
+      p10 = gld_pair(D + z0 + y1 + i0[0]);
+      p01 = gld_pair(D + z1 + y0 + i0[0]);
+      p11 = gld_pair(D + z1 + y1 + i0[0]);
+    } else {
+      uint32_t z0 = (uint32_t)i0[2] * c.sz, z1 = (uint32_t)i1[2] * c.sz;
+      uint32_t y0 = (uint32_t)i0[1] * c.sy, y1 = (uint32_t)i1[1] * c.sy;
+      p00 = gld_pair(D + z0 + y0 + i0[0]);
+      p10 = gld_pair(D + z0 + y1 + i0[0]);
+      p01 = gld_pair(D + z1 + y0 + i0[0]);
+      p11 = gld_pair(D + z1 + y1 + i0[0]);
+    }
+    c000 = p00.x; c100 = p00.y; c010 = p10.x; c110 = p10.y;
+    c001 = p01.x; c101 = p01.y; c011 = p11.x; c111 = p11.y;
+  } else if constexpr (I64) {
+    size_t z0 = (size_t)i0[2] * c.sz, z1 = (size_t)i1[2] * c.sz;
+    size_t y0 = (size_t)i0[1] * c.sy, y1 = (size_t)i1[1] * c.sy;
+    const VRX_GLOBAL float *D = (const VRX_GLOBAL float *)c.data;
+    c000 = c100 = D[z0 + y0 + i0[0]];
+    c010 = c110 = D[z0 + y1 + i0[0]];
+    c001 = c101 = D[z1 + y0 + i0[0]];
+    c011 = c111 = D[z1 + y1 + i0[0]];
+  } else {
+    uint32_t z0 = (uint32_t)i0[2] * c.sz, z1 = (uint32_t)i1[2] * c.sz;
+    uint32_t y0 = (uint32_t)i0[1] * c.sy, y1 = (uint32_t)i1[1] * c.sy;
+    const VRX_GLOBAL float *D = (const VRX_GLOBAL float *)c.data;
+    c000 = c100 = D[z0 + y0 + i0[0]];
+    c010 = c110 = D[z0 + y1 + i0[0]];
+    c001 = c101 = D[z1 + y0 + i0[0]];
+    c011 = c111 = D[z1 + y1 + i0[0]];
+  }
+  float c00 = c000 + (c100 - c000) * f[0];
+  float c10 = c010 + (c110 - c010) * f[0];
+  float c01 = c001 + (c101 - c001) * f[0];
+  float c11 = c011 + (c111 - c011) * f[0];
+  float c0 = c00 + (c10 - c00) * f[1];
+  float c1 = c01 + (c11 - c01) * f[1];
+  return c0 + (c1 - c0) * f[2];
+}
+
+// classifySample (volumeIntegration.h:46-62): LUT texel coordinate u*256-0.5,
+// linear, clamped (TransferFunction1D.cpp:152-185).
+template <bool LUT_GLOBAL>
+__device__ __forceinline__ float4 classify(const float4 *lut, const VolCtx &c, float s)
+{
+  float cl = fminf(fmaxf(s, c.lo), c.hi);
+  float u = (cl - c.lo) * c.inv_range;
+  float x = fminf(fmaxf(u * 256.0f - 0.5f, 0.0f), 255.0f);
+  int i = min((int)x, 254);
+  float f = x - (float)i;
+  float4 a, b;
+  if constexpr (LUT_GLOBAL) {
+    a = gld(lut, i);
+    b = gld(lut, i + 1);
+  } else {
+    a = lut[i];
+    b = lut[i + 1];
+  }
+  return make_float4(a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, a.z + (b.z - a.z) * f,
+      a.w + (b.w - a.w) * f);
+}
+
+// _rayMarchVolume (volumeIntegration.h:64-103), jitter off.  Sample k sits at
+// t0 + k*step (not an accumulated sum, which keeps positions within an ulp of
+// the exact ones); the per-step transmittance pow(1-a, step/unitDistance) is
+// evaluated as exp2(expo * log2(1-a)) with the hardware exp2/log2.  The gathers
+// of U consecutive samples are issued before any of them is composited (sample
+// positions do not depend on the accumulation), which keeps 8*U loads in
+// flight per lane; samples past the exit or the 0.99 cut-off are discarded,
+// their clamped gathers always stay inside the field.
+template <bool I64, int U, bool LUT_GLOBAL>
+__device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, v3 d, float t0,
+    float t1, v3 &color, float &opacity, uint32_t &ns)
+{
+  if (!(c.step > 0.0f))
+    return;
+  // voxel-space ray: g(t) = go + gd * t
+  const float gox = (o.x - c.origin[0]) * c.inv_sp[0], gdx = d.x * c.inv_sp[0];
+  const float goy = (o.y - c.origin[1]) * c.inv_sp[1], gdy = d.y * c.inv_sp[1];
+  const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
+  float T = 1.0f;
+  for (uint32_t k = 0; k < kMaxSamplesPerSegment; k += U) {
+    float sv[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const float t = t0 + (float)(k + u) * c.step;
+      ok[u] = t <= t1;
+      sv[u] = trilinear<I64>(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
+    }
+    bool done = false;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!done) {
+        if (!ok[u] || !(opacity < 0.99f)) {
+          done = true;
+        } else {
+          ns++;
+          const float s = sv[u];
+          if (s == s) {
+            const float4 co = classify<LUT_GLOBAL>(lut, c, s);
+            const float st = __builtin_amdgcn_exp2f(c.expo * __builtin_amdgcn_logf(1.0f - co.w));
+            const float w = T * (1.0f - st);
+            color.x += w * co.x;
+            color.y += w * co.y;
+            color.z += w * co.z;
+            opacity += w;
+            T *= st;
+          }
+        }
+      }
+    }
+    if (done)
+      break;
+  }
+}
+
+__device__ __forceinline__ bool vol_is_large(const vrx_volume &v)
+{
+  return (uint64_t)v.dims[0] * v.dims[1] * v.dims[2] >= (1ull << 31);
+}
+
+// rayMarchAllVolumes (volumeIntegration.h:317-350)
+__device__ float march_all(const vrx_scene &sc, v3 o, v3 d, float tlower, float tfar,
+    float inv_rate, v3 &color, float &opacity, uint32_t &obj, uint32_t &inst, uint32_t &ns)
+{
+  float depth = tfar;
+  bool first = true;
+  int last = -1;
+  float tl = tlower;
+  for (int seg = 0; seg < kMaxVolumeSegments; seg++) {
+    int best = -1;
+    float bt0 = 0.0f, bt1 = 0.0f;
+    for (uint32_t i = 0; i < sc.num_volumes; i++) {
+      if ((int)i == last)
+        continue;
+      float t0, t1;
+      if (vol_slab(sc.d_volumes[i], o, d, tl, tfar, t0, t1)) {
+        if (best < 0 || t0 < bt0) {
+          best = (int)i;
+          bt0 = t0;
+          bt1 = t1;
+        }
+      }
+    }
+    if (best < 0)
+      break;
+    const vrx_volume &V = sc.d_volumes[best];
+    if (first) {
+      obj = V.id;
+      inst = V.inst_id;
+      first = false;
+    }
+    depth = fminf(depth, bt0);
+    const VolCtx c = vol_ctx(V, inv_rate);
+    if (vol_is_large(V))
+      march<true, 2, true>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
+    else
+      march<false, 2, true>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
+    tl = bt1 + 1e-3f;
+    last = best;
+    if (!(opacity < 0.99f))
+      break;
+  }
+  return depth;
+}
+
+// --------------------------------------------------------------------------
+// channel packing (gpu_util.h:323-389, glm packUnorm4x8 / convertLinearToSRGB)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+__device__ __forceinline__ uint32_t pack_unorm4x8(float x, float y, float z, float w)
+{
+  uint32_t r = (uint32_t)(unsigned char)roundf(clamp01(x) * 255.0f);
+  r |= (uint32_t)(unsigned char)roundf(clamp01(y) * 255.0f) << 8;
+  r |= (uint32_t)(unsigned char)roundf(clamp01(z) * 255.0f) << 16;
+  r |= (uint32_t)(unsigned char)roundf(clamp01(w) * 255.0f) << 24;
+  return r;
+}
+
+__device__ __forceinline__ float to_srgb(float v)
+{
+  float c = clamp01(v);
+  return c < 0.0031308f ? c * 12.92f : powf(c, 0.41666f) * 1.055f - 0.055f;
+}
+
+__device__ __forceinline__ void write_color(const vrx_frame &f, size_t idx, float4 c)
+{
+  if (f.d_color_f32)
+    ((float4 *)f.d_color_f32)[idx] = c;
+  if (f.d_color_u32) {
+    if (f.color_format == VRX_COLOR_SRGB8)
+      f.d_color_u32[idx] = pack_unorm4x8(to_srgb(c.x), to_srgb(c.y), to_srgb(c.z), c.w);
+    else
+      f.d_color_u32[idx] = pack_unorm4x8(c.x, c.y, c.z, c.w);
+  }
+}
+
+// accumResults / writeOutputColor (gpu_util.h:356-443): the accumulated
+// tonemapped colour, its inverse tonemap, depth and (when depth improved) ids.
+__device__ __forceinline__ void write_pixel(const vrx_frame &f, size_t idx, float4 accum, int spp,
+    float dcur, bool ids_changed, uint32_t pid, uint32_t oid, uint32_t iid)
+{
+  if (f.d_accum)
+    ((float4 *)f.d_accum)[idx] = accum;
+  float div = (float)(f.frame_id + (uint32_t)spp);
+  float4 avg = make_float4(accum.x / div, accum.y / div, accum.z / div, accum.w / div);
+  float s2 = fmaxf(1e-12f, 1.0f - cmax3(mk(avg.x, avg.y, avg.z)));
+  float4 outv = make_float4(avg.x / s2, avg.y / s2, avg.z / s2, avg.w);
+  write_color(f, idx, outv);
+  if (f.d_depth)
+    f.d_depth[idx] = dcur;
+  if (ids_changed) {
+    if (f.d_prim_id)
+      f.d_prim_id[idx] = pid;
+    if (f.d_obj_id)
+      f.d_obj_id[idx] = oid;
+    if (f.d_inst_id)
+      f.d_inst_id[idx] = iid;
+  }
+}
+
+__device__ __forceinline__ void wave_count_add(uint64_t *ctr, uint32_t v)
+{
+  // one atomic per wave
+  uint32_t s = v;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0 && s)
+    atomicAdd((unsigned long long *)ctr, (unsigned long long)s);
+}
+
+// --------------------------------------------------------------------------
+// the per-pixel program
+// --------------------------------------------------------------------------
+// One pixel of the raycast / AO program (Raycast_ptx.cu:60-179,
+// AmbientOcclusion_ptx.cu:86-220): all pixel samples, the surface/volume
+// layer loop, accumulation and channel writes at output index idx.
+template <int PT, bool TWO, int KIND>
+__device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_camera &cam,
+    const vrx_renderer &r, const vrx_frame &f, uint32_t px, uint32_t py, size_t idx, Stack &st,
+    uint32_t &rays, uint32_t &vsamples)
+{
+  const float iw = 1.0f / (float)f.width, ih = 1.0f / (float)f.height;
+  const int spp = KIND == VRX_RENDER_AO ? max(r.pixel_samples, 1) : 1;
+  const int aos = min(max(r.ambient_samples, 0), 256);
+  const bool cull = r.cull_backfaces != 0;
+  Pcg rs;
+  if (KIND == VRX_RENDER_AO)
+    pcg_seed(rs, (uint64_t)idx, (uint64_t)f.frame_id);
+
+  float4 accum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float dcur = FLT_MAX;
+  uint32_t pid = 0, oid = 0, iid = 0;
+  bool ids_changed = f.frame_id == 0;
+  if (f.frame_id != 0) {
+    accum = ((const float4 *)f.d_accum)[idx];
+    if (f.d_depth)
+      dcur = f.d_depth[idx];
+  }
+  for (int i = 0; i < spp; i++) {
+    v3 o, d;
+    if (KIND == VRX_RENDER_AO) {
+      float r0 = pcg_float(rs), r1 = pcg_float(rs);
+      (void)pcg_float(rs);
+      (void)pcg_float(rs);
+      cam_ray(cam, ((float)px + r0) * iw, ((float)py + r1) * ih, o, d);
+    } else {
+      cam_ray(cam, (float)px * iw, (float)py * ih, o, d);
+    }
+    float tlower = 0.0f;
+    const float tmax = FLT_MAX;
+    v3 outc = mk(0.0f, 0.0f, 0.0f);
+    float outo = 0.0f, depth = 1e30f;
+    uint32_t prim = ~0u, obj = ~0u, inst = ~0u;
+    bool first = true;
+    for (int layer = 0; layer < kMaxSurfaceLayers && outo < 0.99f; layer++) {
+      Hit h;
+      rays++;
+      bool found = TWO ? trace<PT, true, false>(sc, o, d, tlower, tmax, cull, st, h)
+                       : trace<PT, false, false>(sc, o, d, tlower, tmax, cull, st, h);
+      v3 color = mk(0.0f, 0.0f, 0.0f);
+      float opacity = 0.0f;
+      if (found) {
+        SurfHit s;
+        populate(sc, h, o, d, s);
+        uint32_t vo = ~0u, vi = ~0u;
+        float vdepth = s.t;
+        if (sc.num_volumes)
+          vdepth = march_all(sc, o, d, tlower, s.t, r.inv_volume_sampling_rate, color, opacity,
+              vo, vi, vsamples);
+        if (first) {
+          if (vdepth < s.t) {
+            depth = vdepth;
+            prim = 0;
+            obj = vo;
+            inst = vi;
+          } else {
+            depth = s.t;
+            prim = s.prim_out;
+            obj = s.obj;
+            inst = s.inst;
+          }
+          first = false;
+        }
+        v3 lighting;
+        if (KIND == VRX_RENDER_AO) {
+          float ao = 1.0f;
+          if (aos > 0) {
+            float weights = 0.0f, hits = 0.0f;
+            for (int k = 0; k < aos; k++) {
+              v3 ao_o = add(s.hp, fmul(s.eps, s.ng));
+              float q0 = pcg_float(rs), q1 = pcg_float(rs), q2 = pcg_float(rs);
+              (void)pcg_float(rs);
+              v3 dir = normalize(mk(2.0f * q0 - 1.0f, 2.0f * q1 - 1.0f, 2.0f * q2 - 1.0f));
+              if (!(dot(dir, s.ns) > 0.0f))
+                dir = neg(dir);
+              float wgt = fmaxf(0.0f, dot(dir, s.ns));
+              weights += wgt;
+              if (wgt != 0.0f) {
+                Hit hh;
+                rays++;
+                bool occ = TWO ? trace<PT, true, true>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh)
+                               : trace<PT, false, true>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh);
+                hits += wgt * (occ ? 1.0f : 0.0f);
+              }
+            }
+            ao = weights > 0.0f ? 1.0f - hits / weights : 0.0f;
+          }
+          float scl = ao * r.ambient_radiance;
+          lighting = fmul(scl, ld3(r.ambient_color));
+        } else {
+          float scl = fabsf(dot(d, s.ns));
+          lighting = fmul(scl, ld3(r.ambient_color));
+        }
+        v3 bl = mulv(s.color, lighting);
+        float w = 1.0f - opacity;
+        color = mk(color.x + bl.x * w, color.y + bl.y * w, color.z + bl.z * w);
+        opacity = opacity + s.opacity * (1.0f - opacity);
+        color = mulf(color, opacity);
+        float wo = 1.0f - outo;
+        outc = mk(outc.x + color.x * wo, outc.y + color.y * wo, outc.z + color.z * wo);
+        outo = outo + opacity * (1.0f - outo);
+        tlower = s.t + s.eps;
+      } else {
+        uint32_t vo = ~0u, vi = ~0u;
+        float vdepth = tmax;
+        if (sc.num_volumes)
+          vdepth = march_all(sc, o, d, tlower, tmax, r.inv_volume_sampling_rate, color, opacity,
+              vo, vi, vsamples);
+        if (first) {
+          depth = fminf(depth, vdepth);
+          prim = 0;
+          obj = vo;
+          inst = vi;
+        }
+        color = mulf(color, opacity);
+        float w = 1.0f - opacity;
+        color = mk(color.x + r.background[0] * w, color.y + r.background[1] * w,
+            color.z + r.background[2] * w);
+        opacity = opacity + r.background[3] * (1.0f - opacity);
+        float wo = 1.0f - outo;
+        outc = mk(outc.x + color.x * wo, outc.y + color.y * wo, outc.z + color.z * wo);
+        outo = outo + opacity * (1.0f - outo);
+        break;
+      }
+    }
+    // accumResults (gpu_util.h:393-443)
+    float sc_ = 1.0f + fmaxf(0.0f, cmax3(outc));
+    accum.x = accum.x + outc.x / sc_;
+    accum.y = accum.y + outc.y / sc_;
+    accum.z = accum.z + outc.z / sc_;
+    accum.w = accum.w + outo;
+    if (depth < dcur) {
+      dcur = depth;
+      pid = prim;
+      oid = obj;
+      iid = inst;
+      ids_changed = true;
+    }
+  }
+  write_pixel(f, idx, accum, spp, dcur, ids_changed, pid, oid, iid);
+}
+
+} // namespace vrx

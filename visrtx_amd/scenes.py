@@ -171,6 +171,59 @@ def volume_workload(n: int = 512, width: int = 1920, height: int = 1080, seed: i
     return sc, cam
 
 
+def random_rotation(rng) -> np.ndarray:
+    """Uniformly distributed rotation (normalised Gaussian quaternion), 3x3 float64."""
+    q = rng.normal(size=4)
+    w, x, y, z = q / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def orbit_poses(n: int, center, radius: float, seed: int, fovy_deg: float = 45.0,
+                aspect: float = 1.0) -> list:
+    """n seeded camera poses on an orbit around ``center``, looking at it."""
+    rng = np.random.default_rng(seed)
+    c = np.asarray(center, np.float64)
+    cams = []
+    for i in range(n):
+        phi = 2.0 * math.pi * (i + rng.uniform(0.0, 0.5)) / n
+        z = rng.uniform(-0.5, 0.5)
+        r = math.sqrt(1.0 - z * z)
+        pos = tuple(float(v) for v in c + radius * np.array([r * math.cos(phi), z,
+                                                              r * math.sin(phi)]))
+        cams.append(CameraDesc(pos, look_at(pos, c), (0, 1, 0), math.radians(fovy_deg), aspect))
+    return cams
+
+
+def batched_workload(views: int = 64, res: int = 512, instances: int = 256, meshes: int = 16,
+                     tri_min: int = 10_000, tri_max: int = 2_000_000, box: float = 10.0,
+                     seed: int = 9):
+    """``meshes`` noise-displaced spheres (radius ~1) with triangle counts
+    log-uniform in [tri_min, tri_max], ``instances`` instances of them with
+    uniformly random rotations and translations in [0, box]^3, and ``views``
+    orbit poses at res x res looking at the box centre."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(tri_min), math.log(tri_max), meshes)).astype(np.int64)
+    geoms = []
+    for i, t in enumerate(counts):
+        v, idx = gen_sphere_mesh(int(t), seed=seed * 1000 + i)
+        geoms.append(GeomDesc("triangle", v, idx, surface_id=i,
+                              color=tuple(float(c) for c in rng.uniform(0.2, 0.9, 3))))
+    insts = []
+    for k in range(instances):
+        R = random_rotation(rng)
+        t = rng.uniform(0.0, box, 3)
+        M = np.zeros(12, np.float32)
+        M[0:9] = R.T.reshape(-1)        # column-major 3x3
+        M[9:12] = t
+        insts.append(InstanceDesc(geoms=[int(rng.integers(meshes))], xfm=M, id=k))
+    sc = SceneDesc(geoms=geoms, instances=insts, world_id=0)
+    c = 0.5 * box
+    cams = orbit_poses(views, (c, c, c), 1.9 * box, seed + 1)
+    return sc, cams
+
+
 # ---------------------------------------------------------------------------
 # ANARI construction
 # ---------------------------------------------------------------------------
