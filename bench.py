@@ -250,8 +250,8 @@ def bench_batched(dev, reps=5):
     plain_ms = (time.perf_counter() - t) * 1000.0 / reps
     kern_ms = sum(dev.get_property(f, "duration", A.FLOAT32) for _, f in frames) * 1000.0
     for c, f in frames:
-        dev.release(f)
-        dev.release(c)
+        s.drop(f)
+        s.drop(c)
 
     bf = B.BatchFrame(scene, B.renderer_of(dev, r), cams, W, H, srgb=True)
     best = bf.autotune()
@@ -322,6 +322,9 @@ def cpu_baseline(threads):
                       f"{build_s * 1000:.0f} ms"}
 
 
+SECTIONS = ("headline", "ao", "heightfield", "volume", "batched", "diffvol")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,11 +332,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-headline", action="store_true")
+    ap.add_argument("--sections", default=",".join(SECTIONS),
+                    help="comma-separated subset of " + ",".join(SECTIONS) +
+                         " (profiling one kernel); the JSON value needs 'headline'")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this device renders on one GPU; multi-GPU sharding is "
                                    "design-only (see DESIGN.md), run with --gpus 1"}))
         sys.exit(2)
+    sec = {"headline"} if args.only_headline else set(args.sections.split(","))
+    unknown = sec - set(SECTIONS)
+    if unknown:
+        ap.error(f"unknown sections {sorted(unknown)}")
 
     from visrtx_amd import anari as A
     from visrtx_amd import scenes as S
@@ -344,38 +354,40 @@ def main():
     extra["copy_kernel_gbs"] = round(copy_bandwidth(), 1)
     log(f"copy kernel bandwidth {extra['copy_kernel_gbs']} GB/s")
 
-    # ---- headline: 10M spheres, 1920x1080 raycast -------------------------
     W, H = 1920, 1080
-    log("generating 10M spheres")
-    sc, cam = S.spheres_workload(width=W, height=H)
-    work = Workload(dev, sc, cam, W, H, "raycast")
-    log("building scene")
-    t = time.perf_counter()
-    dev.get_property(work.s.world, "bounds", A.FLOAT32_BOX3)
-    log(f"scene build (upload + LBVH) {time.perf_counter() - t:.2f}s")
-    lbvh_ms, lbvh_all = bench_lbvh(dev, work)
-    extra["lbvh_build_ms_10m_spheres"] = round(lbvh_ms, 3)
-    log(f"LBVH build {lbvh_ms:.3f} ms (all {['%.3f' % x for x in lbvh_all]})")
-    sec_per_step, durs = work.timed(args.steps, args.warmup)
+    value = sec_per_step = roofline = None
     rays = W * H
-    value = rays / sec_per_step / 1e6
-    kern_ms = statistics.mean(durs) * 1000.0
-    log(f"primary raycast: {sec_per_step * 1000:.3f} ms/step wall, kernel {kern_ms:.3f} ms "
-        f"-> {value:.1f} Mrays/s (kernel-only {rays / (kern_ms / 1000) / 1e6:.1f})")
-    extra["primary_kernel_mrays_s"] = round(rays / (kern_ms / 1000) / 1e6, 2)
-    c2 = counts["spheres10m_1920x1080"]
-    bytes_per_ray = 32 + 16 + 64 * c2["nodes_per_ray"] + 16 * c2["prims_per_ray"]
-    alg_bytes = rays * bytes_per_ray
-    achieved = alg_bytes / (kern_ms / 1000) / 1e9
-    traffic = load_traffic("render_spheres10m_1920x1080")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
-                "kernel": "k_render<sphere,single-level,raycast>",
-                "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
-                "avg_kernel_ms": round(kern_ms, 4)}
-
-    if not args.only_headline:
+    if "headline" in sec or "ao" in sec:
+        # ---- headline: 10M spheres, 1920x1080 raycast ---------------------
+        log("generating 10M spheres")
+        sc, cam = S.spheres_workload(width=W, height=H)
+        work = Workload(dev, sc, cam, W, H, "raycast")
+        log("building scene")
+        t = time.perf_counter()
+        dev.get_property(work.s.world, "bounds", A.FLOAT32_BOX3)
+        log(f"scene build (upload + LBVH) {time.perf_counter() - t:.2f}s")
+    if "headline" in sec:
+        lbvh_ms, lbvh_all = bench_lbvh(dev, work)
+        extra["lbvh_build_ms_10m_spheres"] = round(lbvh_ms, 3)
+        log(f"LBVH build {lbvh_ms:.3f} ms (all {['%.3f' % x for x in lbvh_all]})")
+        sec_per_step, durs = work.timed(args.steps, args.warmup)
+        value = rays / sec_per_step / 1e6
+        kern_ms = statistics.mean(durs) * 1000.0
+        log(f"primary raycast: {sec_per_step * 1000:.3f} ms/step wall, kernel {kern_ms:.3f} ms "
+            f"-> {value:.1f} Mrays/s (kernel-only {rays / (kern_ms / 1000) / 1e6:.1f})")
+        extra["primary_kernel_mrays_s"] = round(rays / (kern_ms / 1000) / 1e6, 2)
+        c2 = counts["spheres10m_1920x1080"]
+        bytes_per_ray = 32 + 16 + 64 * c2["nodes_per_ray"] + 16 * c2["prims_per_ray"]
+        alg_bytes = rays * bytes_per_ray
+        achieved = alg_bytes / (kern_ms / 1000) / 1e9
+        traffic = load_traffic("render_spheres10m_1920x1080")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+                    "kernel": "k_render<sphere,single-level,raycast>",
+                    "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
+                    "avg_kernel_ms": round(kern_ms, 4)}
+    if "ao" in sec:
         # AO renderer, 16 spp
         ao = Workload(dev, sc, cam, W, H, "ao", ids=True, pixelSamples=16, ambientSamples=1,
                       ambientOcclusionDistance=0.02)
@@ -386,11 +398,11 @@ def main():
         extra["ao16_ms_per_frame"] = round(sps * 1000, 3)
         log(f"AO 16spp: {sps * 1000:.2f} ms/frame, {nr} rays -> {extra['ao16_mrays_s']} Mrays/s")
         ao.release()
-    work.release()
-    del sc
+    if "headline" in sec or "ao" in sec:
+        work.release()
+        del sc
 
-    if not args.only_headline:
-        # heightfield raycast
+    if "heightfield" in sec:
         sc1, cam1 = S.heightfield_workload()
         hf = Workload(dev, sc1, cam1, 1024, 768, "raycast")
         sps, d1 = hf.timed(20, 3)
@@ -399,7 +411,7 @@ def main():
         log(f"heightfield: {sps * 1000:.3f} ms/step -> {extra['heightfield_mrays_s']} Mrays/s")
         hf.release()
 
-        # 512^3 volume
+    if "volume" in sec:
         log("generating 512^3 volume")
         sc3, cam3 = S.volume_workload()
         vol = Workload(dev, sc3, cam3, W, H, "raycast", volumeSamplingRate=1.0)
@@ -418,6 +430,7 @@ def main():
         vol.release()
         del sc3
 
+    if "batched" in sec:
         log("batched 64 views")
         bt = bench_batched(dev)
         extra["batched_64x512"] = bt
@@ -426,7 +439,7 @@ def main():
 
     dev.close()
 
-    if not args.only_headline:
+    if "diffvol" in sec:
         log("differentiable volume")
         dv = bench_diffvol()
         extra["diffvol_128"] = dv
@@ -442,12 +455,12 @@ def main():
 
     out = {
         "metric": "Mrays/s closest-hit, volume Gsamples/s, fwd+bwd ms/iter on 1 MI355X",
-        "value": round(value, 2),
+        "value": round(value, 2) if value is not None else None,
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(sec_per_step * 1000, 4),
+        "ms_per_step": round(sec_per_step * 1000, 4) if sec_per_step else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -43,8 +43,8 @@ def _per_view(dev, s, r, cam, w, h):
     dev.render(f)
     dev.ready(f)
     out = {ch: dev.map_frame(f, "channel." + ch)[0].copy() for ch in CHANNELS}
-    dev.release(f)
-    dev.release(c)
+    s.drop(f)
+    s.drop(c)
     return out
 
 

@@ -243,6 +243,11 @@ class AnariScene:
         self.handles.append(h)
         return h
 
+    def drop(self, h):
+        """Release one kept handle now (it is then not released again)."""
+        self.handles.remove(h)
+        self.dev.release(h)
+
     def release(self):
         for h in reversed(self.handles):
             self.dev.release(h)
