@@ -318,6 +318,12 @@ class OracleScene:
                                    int(cull), C.byref(h))
         return h if ok else None
 
+    def trace_any(self, org, dir_, tmin=0.0, tmax=3.4e38) -> bool:
+        o = np.asarray(org, np.float32)
+        d = np.asarray(dir_, np.float32)
+        return bool(lib().orc_trace_any(C.byref(self.s), o.ctypes.data, d.ctypes.data, tmin, tmax,
+                                        None))
+
     def trace_rays(self, org, dirs, tmin=0.0, tmax=3.4e38):
         n = len(org)
         o = np.ascontiguousarray(org, np.float32)

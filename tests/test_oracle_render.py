@@ -273,3 +273,78 @@ def test_tf_discretization_matches_interpolation():
     np.testing.assert_allclose(lut[:, 3], 0.05 * p, atol=1e-7)
     for k in range(3):
         np.testing.assert_allclose(lut[:, k], np.interp(p, [0, 0.5, 1], colors[:, k]), atol=1e-6)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_any_hit_equals_closest_hit_within_range(seed):
+    """Opaque geometry: the AO any-hit query is 'a closest hit exists in [tmin, tmax]'."""
+    sc, rng = _random_scene(seed)
+    os_ = orc.OracleScene(sc)
+    agree = 0
+    for _ in range(600):
+        o = rng.uniform(-0.5, 1.5, 3).astype(np.float32)
+        d = _unit(rng.uniform(0, 1, 3) - o).astype(np.float32)
+        tmax = float(rng.uniform(0.05, 2.0))
+        h = os_.trace_closest(o, d, 0.0, tmax)
+        assert os_.trace_any(o, d, 0.0, tmax) == (h is not None)
+        agree += h is not None
+    assert agree > 50
+
+
+def _cube_mesh():
+    v = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1)], np.float32)
+    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    idx = []
+    for a, b, c, d in quads:
+        idx += [(a, b, c), (a, c, d)]
+    return v, np.array(idx, np.uint32)
+
+
+def test_ao_closed_forms_unoccluded_and_enclosed():
+    """AO = 1 with nothing above a floor (colour = base * radiance * ambientColor);
+    AO = 0 inside a closed, index-shared cube (every AO ray hits a wall)."""
+    floor_v = np.array([[-5, 0, -5], [5, 0, -5], [5, 0, 5], [-5, 0, 5]], np.float32)
+    floor_i = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    sc = S.SceneDesc(geoms=[S.GeomDesc("triangle", floor_v, floor_i, color=(0.8, 0.6, 0.4))],
+                     world_geoms=[0])
+    cam = S.CameraDesc((0.0, 2.0, -3.0), (0.0, -2.0, 3.0), (0, 1, 0), math.radians(40), 1.0)
+    o = orc.OracleScene(sc).render(cam, 24, 24, "ao", pixelSamples=1, ambientSamples=8,
+                                   ambientRadiance=0.5, ambientColor=(1.0, 0.5, 0.25))
+    hit = o["depth"] < 1e29
+    assert hit.mean() > 0.3
+    want = np.array([0.8 * 0.5 * 1.0, 0.6 * 0.5 * 0.5, 0.4 * 0.5 * 0.25], np.float32)
+    np.testing.assert_allclose(o["color"][hit][:, :3], np.broadcast_to(want, (hit.sum(), 3)),
+                               rtol=2e-6)
+    np.testing.assert_allclose(o["color"][hit][:, 3], 1.0, rtol=1e-6)
+
+    v, idx = _cube_mesh()
+    sc = S.SceneDesc(geoms=[S.GeomDesc("triangle", v, idx)], world_geoms=[0])
+    cam = S.CameraDesc((0.5, 0.5, 0.5), (0.3, 0.2, 1.0), (0, 1, 0), math.radians(60), 1.0)
+    o = orc.OracleScene(sc).render(cam, 24, 24, "ao", pixelSamples=2, ambientSamples=16)
+    assert (o["depth"] < 1e29).all()
+    assert (o["color"][..., :3] == 0.0).all()
+    np.testing.assert_allclose(o["color"][..., 3], 1.0, rtol=1e-6)
+
+
+def test_diffvol_forward_equals_volume_march():
+    """The FP64 march of the differentiable path (explicit rays, explicit LUT)
+    equals the FP64 march of the renderer path on the same field and rays."""
+    n = 12
+    data = S.gen_noise_volume(n, seed=9)
+    vol = _vol(data, (S.COLORMAP5, np.array([0.0, 0.4], np.float32)), spacing=1.0 / (n - 1))
+    W = H = 10
+    cam = S.CameraDesc((1.9, 1.5, -1.0), S.look_at((1.9, 1.5, -1.0), (0.5, 0.5, 0.5)), (0, 1, 0),
+                       math.radians(35), 1.0)
+    img, _ = orc.volume_image(vol, cam, W, H, inv_sampling_rate=1.0, early_stop=True)
+    rays = [orc.camera_ray(cam, W, H, x, y) for y in range(H) for x in range(W)]
+    org = np.array([r[0] for r in rays], np.float32)
+    dirs = np.array([r[1] for r in rays], np.float32)
+    sp = np.float32(vol.spacing[0])
+    step = float(sp / np.float32(2.0))
+    lut = orc.tf_discretize(vol.color, vol.opacity, vol.value_range)
+    got, _ = orc.diffvol_forward((n, n, n), vol.origin, vol.spacing, data, lut, vol.value_range,
+                                 vol.unit_distance, step, org, dirs, early_stop=True)
+    # the renderer path derives its step in fp32 (min(spacing)/2 * 1/rate): differences
+    # stay at the level of that one rounding
+    np.testing.assert_allclose(got, img.reshape(-1, 4), rtol=1e-7, atol=1e-9)
+    assert img[..., 3].max() > 0.3
