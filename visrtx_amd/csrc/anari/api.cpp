@@ -233,7 +233,9 @@ ANARIFrame anariNewFrame(ANARIDevice d)
 // ---------------------------------------------------------------------------
 void anariSetParameter(ANARIDevice d, ANARIObject o, const char *name, ANARIDataType t, const void *mem)
 {
-  if (!d || !name)
+  // a null object (a failed creation) is ignored; the device handle itself
+  // addresses device parameters
+  if (!d || !o || !name)
     return;
   std::lock_guard<std::recursive_mutex> lock(D(d)->mutex);
   Param p;
@@ -252,7 +254,7 @@ void anariSetParameter(ANARIDevice d, ANARIObject o, const char *name, ANARIData
     }
     p.bytes.assign((const uint8_t *)mem, (const uint8_t *)mem + sz);
   }
-  auto &m = (!o || (void *)o == (void *)d) ? D(d)->params : O(o)->staged;
+  auto &m = (void *)o == (void *)d ? D(d)->params : O(o)->staged;
   auto it = m.find(name);
   if (it != m.end())
     freeParam(it->second);
@@ -261,10 +263,10 @@ void anariSetParameter(ANARIDevice d, ANARIObject o, const char *name, ANARIData
 
 void anariUnsetParameter(ANARIDevice d, ANARIObject o, const char *name)
 {
-  if (!d || !name)
+  if (!d || !o || !name)
     return;
   std::lock_guard<std::recursive_mutex> lock(D(d)->mutex);
-  auto &m = (!o || (void *)o == (void *)d) ? D(d)->params : O(o)->staged;
+  auto &m = (void *)o == (void *)d ? D(d)->params : O(o)->staged;
   auto it = m.find(name);
   if (it != m.end()) {
     freeParam(it->second);
@@ -274,10 +276,10 @@ void anariUnsetParameter(ANARIDevice d, ANARIObject o, const char *name)
 
 void anariUnsetAllParameters(ANARIDevice d, ANARIObject o)
 {
-  if (!d)
+  if (!d || !o)
     return;
   std::lock_guard<std::recursive_mutex> lock(D(d)->mutex);
-  auto &m = (!o || (void *)o == (void *)d) ? D(d)->params : O(o)->staged;
+  auto &m = (void *)o == (void *)d ? D(d)->params : O(o)->staged;
   for (auto &kv : m)
     freeParam(kv.second);
   m.clear();
@@ -285,10 +287,10 @@ void anariUnsetAllParameters(ANARIDevice d, ANARIObject o)
 
 void anariCommitParameters(ANARIDevice d, ANARIObject o)
 {
-  if (!d)
+  if (!d || !o)
     return;
   std::lock_guard<std::recursive_mutex> lock(D(d)->mutex);
-  if (!o || (void *)o == (void *)d) {
+  if ((void *)o == (void *)d) {
     // device parameters (cudaDevice / hipDevice, forceInit): VisRTXDevice.cpp:460-518
     auto it = D(d)->params.find("forceInit");
     if (it != D(d)->params.end())
@@ -312,9 +314,11 @@ void anariCommitParameters(ANARIDevice d, ANARIObject o)
 
 void anariRelease(ANARIDevice d, ANARIObject o)
 {
-  if (!d)
+  // a null object is a no-op (a failed creation's handle); only the device
+  // handle itself releases the device
+  if (!d || !o)
     return;
-  if (!o || (void *)o == (void *)d) {
+  if ((void *)o == (void *)d) {
     Device *dev = D(d);
     bool last;
     {
@@ -337,10 +341,10 @@ void anariRelease(ANARIDevice d, ANARIObject o)
 
 void anariRetain(ANARIDevice d, ANARIObject o)
 {
-  if (!d)
+  if (!d || !o)
     return;
   std::lock_guard<std::recursive_mutex> lock(D(d)->mutex);
-  if (!o || (void *)o == (void *)d)
+  if ((void *)o == (void *)d)
     D(d)->refCount++;
   else
     O(o)->pubRef++;
