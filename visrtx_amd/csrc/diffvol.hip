@@ -90,7 +90,7 @@ struct Sample {
   float s;          // density
   float fx, fy, fz; // trilinear fractions
   uint32_t base;    // index of the (i0, j0, k0) corner
-  uint32_t cell;    // same, used as the cell key
+  int ci, cj, ck;   // the cell (i0, j0, k0)
 };
 
 __device__ __forceinline__ Sample dv_sample(const DvCtx &c, float gx, float gy, float gz)
@@ -122,7 +122,9 @@ __device__ __forceinline__ Sample dv_sample(const DvCtx &c, float gx, float gy, 
   sm.fy = f[1];
   sm.fz = f[2];
   sm.base = base;
-  sm.cell = base;
+  sm.ci = i0[0];
+  sm.cj = i0[1];
+  sm.ck = i0[2];
   return sm;
 }
 
@@ -198,17 +200,140 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_fwd(DvCtx c, const float *
   gst((float4 *)rgba, r, make_float4(C0, C1, C2, O));
 }
 
-__device__ __forceinline__ void flush_cell(const DvCtx &c, float *gd, uint32_t base, const float g[8])
+// ---------------------------------------------------------------------------
+// Backward.  Two register caches keep the scatter traffic down:
+//  * density: the 8 corner gradients of the current trilinear cell.  When the
+//    ray moves to a neighbouring cell the corners it shares with the old cell
+//    stay in registers (one axis at a time: a face step keeps 4 of 8); only
+//    the corners left behind are added to the grid, with one hardware float
+//    atomic each, so every grid voxel receives about one atomic per ray that
+//    passes it instead of one per cell visit;
+//  * LUT: the gradients of the two entries (i, i+1) of the current LUT
+//    interval, carried the same way when the interval index moves by one,
+//    and flushed into the workgroup's LDS copy of the LUT gradient.
+// ---------------------------------------------------------------------------
+struct CornerAcc {
+  float a[8];    // corner q = x + 2y + 4z of the current cell
+  uint32_t base; // grid index of corner 0
+  int ci, cj, ck;
+  bool live;
+};
+
+__device__ __forceinline__ void put(float *gd, uint32_t idx, float v)
 {
-  float *p = gd + base;
-  unsafeAtomicAdd(p, g[0]);
-  unsafeAtomicAdd(p + 1, g[1]);
-  unsafeAtomicAdd(p + c.sy, g[2]);
-  unsafeAtomicAdd(p + c.sy + 1, g[3]);
-  unsafeAtomicAdd(p + c.sz, g[4]);
-  unsafeAtomicAdd(p + c.sz + 1, g[5]);
-  unsafeAtomicAdd(p + c.sz + c.sy, g[6]);
-  unsafeAtomicAdd(p + c.sz + c.sy + 1, g[7]);
+  if (v != 0.0f)
+    unsafeAtomicAdd(gd + idx, v);
+}
+
+__device__ __forceinline__ void corner_offsets(const DvCtx &c, uint32_t off[8])
+{
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+    off[q] = (uint32_t)(q & 1) + (uint32_t)((q >> 1) & 1) * c.sy + (uint32_t)((q >> 2) & 1) * c.sz;
+}
+
+// move the cell by d (-1, 0, +1) along AXIS, flushing the face left behind
+template <int AXIS>
+__device__ __forceinline__ void shift_cell(const DvCtx &c, float *gd, CornerAcc &A, int d)
+{
+  constexpr int bit = 1 << AXIS;
+  const uint32_t stride = AXIS == 0 ? 1u : (AXIS == 1 ? c.sy : c.sz);
+  uint32_t off[8];
+  corner_offsets(c, off);
+  if (d > 0) {
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if ((q & bit) == 0) {
+        put(gd, A.base + off[q], A.a[q]);
+        A.a[q] = A.a[q | bit];
+        A.a[q | bit] = 0.0f;
+      }
+    A.base += stride;
+  } else if (d < 0) {
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if ((q & bit) != 0) {
+        put(gd, A.base + off[q], A.a[q]);
+        A.a[q] = A.a[q ^ bit];
+        A.a[q ^ bit] = 0.0f;
+      }
+    A.base -= stride;
+  }
+}
+
+__device__ __forceinline__ void flush_all(const DvCtx &c, float *gd, CornerAcc &A)
+{
+  if (!A.live)
+    return;
+  uint32_t off[8];
+  corner_offsets(c, off);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    put(gd, A.base + off[q], A.a[q]);
+    A.a[q] = 0.0f;
+  }
+}
+
+__device__ __forceinline__ void enter_cell(const DvCtx &c, float *gd, CornerAcc &A, const Sample &sm)
+{
+  if (A.live && sm.ci == A.ci && sm.cj == A.cj && sm.ck == A.ck)
+    return;
+  const int dx = sm.ci - A.ci, dy = sm.cj - A.cj, dz = sm.ck - A.ck;
+  if (A.live && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
+    shift_cell<0>(c, gd, A, dx);
+    shift_cell<1>(c, gd, A, dy);
+    shift_cell<2>(c, gd, A, dz);
+  } else {
+    flush_all(c, gd, A);
+    A.base = sm.base;
+    A.live = true;
+  }
+  A.ci = sm.ci;
+  A.cj = sm.cj;
+  A.ck = sm.ck;
+}
+
+struct LutAcc {
+  float lo[4], hi[4]; // gradients of entries i and i+1
+  int i;
+};
+
+__device__ __forceinline__ void lut_put(float *s_glut, int e, const float g[4])
+{
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++)
+    if (g[ch] != 0.0f)
+      atomicAdd(s_glut + 4 * e + ch, g[ch]);
+}
+
+__device__ __forceinline__ void lut_enter(float *s_glut, LutAcc &L, int i)
+{
+  if (i == L.i)
+    return;
+  if (L.i >= 0 && i == L.i + 1) {
+    lut_put(s_glut, L.i, L.lo);
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+      L.lo[ch] = L.hi[ch];
+      L.hi[ch] = 0.0f;
+    }
+  } else if (L.i >= 0 && i == L.i - 1) {
+    lut_put(s_glut, L.i + 1, L.hi);
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+      L.hi[ch] = L.lo[ch];
+      L.lo[ch] = 0.0f;
+    }
+  } else {
+    if (L.i >= 0) {
+      lut_put(s_glut, L.i, L.lo);
+      lut_put(s_glut, L.i + 1, L.hi);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++)
+      L.lo[ch] = L.hi[ch] = 0.0f;
+  }
+  L.i = i;
 }
 
 __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *lut_g, uint32_t n,
@@ -234,8 +359,18 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
       const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
       float Tn = gld(ckpt, (size_t)1 * n + r); // T after the last sample
       float R = 0.0f;
-      uint32_t cur = 0xFFFFFFFFu;
-      float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      CornerAcc A;
+      A.live = false;
+      A.base = 0;
+      A.ci = A.cj = A.ck = 0;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        A.a[q] = 0.0f;
+      LutAcc L;
+      L.i = -1;
+#pragma unroll
+      for (int ch = 0; ch < 4; ch++)
+        L.lo[ch] = L.hi[ch] = 0.0f;
       for (int k = (int)N - 1; k >= 0; k--) {
         const float t = t0 + (float)k * c.step;
         const Sample sm = dv_sample(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
@@ -246,7 +381,7 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
         if ((uint32_t)k % (uint32_t)c.ckpt == 0) {
           T = gld(ckpt, (size_t)(2 + seg) * n + r);
         } else if (tau >= kMinTauForDivision && Tn > 0.0f) {
-          T = Tn / tau;
+          T = Tn * __builtin_amdgcn_rcpf(tau);
         } else {
           // replay from the segment checkpoint
           T = gld(ckpt, (size_t)(2 + seg) * n + r);
@@ -260,45 +395,41 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
         const float w = T * (1.0f - tau);
         const float dtau = T * (R - v);
         const float oma = fmaxf(1.0f - cl.co.w, 1e-8f);
-        const float da = dtau * (-c.expo) * tau / oma;
+        const float da = dtau * (-c.expo) * tau * __builtin_amdgcn_rcpf(oma);
         const float dr = w * gx, dg = w * gy, db = w * gz;
         // LUT: entries i and i+1 with weights (1-f) and f
+        lut_enter(s_glut, L, cl.i);
         const float wf0 = 1.0f - cl.f, wf1 = cl.f;
-        float *g0 = s_glut + 4 * cl.i;
-        atomicAdd(g0 + 0, wf0 * dr);
-        atomicAdd(g0 + 1, wf0 * dg);
-        atomicAdd(g0 + 2, wf0 * db);
-        atomicAdd(g0 + 3, wf0 * da);
-        atomicAdd(g0 + 4, wf1 * dr);
-        atomicAdd(g0 + 5, wf1 * dg);
-        atomicAdd(g0 + 6, wf1 * db);
-        atomicAdd(g0 + 7, wf1 * da);
+        L.lo[0] += wf0 * dr;
+        L.lo[1] += wf0 * dg;
+        L.lo[2] += wf0 * db;
+        L.lo[3] += wf0 * da;
+        L.hi[0] += wf1 * dr;
+        L.hi[1] += wf1 * dg;
+        L.hi[2] += wf1 * db;
+        L.hi[3] += wf1 * da;
         // density: d(channel)/ds through the LUT slope, then trilinear weights
         const float4 la = s_lut[cl.i], lb = s_lut[cl.i + 1];
         const float ds = ((dr * (lb.x - la.x) + dg * (lb.y - la.y)) + (db * (lb.z - la.z) + da * (lb.w - la.w)))
             * cl.du_ds;
-        if (sm.cell != cur) {
-          if (cur != 0xFFFFFFFFu)
-            flush_cell(c, grad_density, cur, acc);
-          cur = sm.cell;
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-            acc[q] = 0.0f;
-        }
+        enter_cell(c, grad_density, A, sm);
         const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
-        acc[0] += ds * (ux * uy * uz);
-        acc[1] += ds * (sm.fx * uy * uz);
-        acc[2] += ds * (ux * sm.fy * uz);
-        acc[3] += ds * (sm.fx * sm.fy * uz);
-        acc[4] += ds * (ux * uy * sm.fz);
-        acc[5] += ds * (sm.fx * uy * sm.fz);
-        acc[6] += ds * (ux * sm.fy * sm.fz);
-        acc[7] += ds * (sm.fx * sm.fy * sm.fz);
+        A.a[0] += ds * (ux * uy * uz);
+        A.a[1] += ds * (sm.fx * uy * uz);
+        A.a[2] += ds * (ux * sm.fy * uz);
+        A.a[3] += ds * (sm.fx * sm.fy * uz);
+        A.a[4] += ds * (ux * uy * sm.fz);
+        A.a[5] += ds * (sm.fx * uy * sm.fz);
+        A.a[6] += ds * (ux * sm.fy * sm.fz);
+        A.a[7] += ds * (sm.fx * sm.fy * sm.fz);
         R = (1.0f - tau) * v + tau * R;
         Tn = T;
       }
-      if (cur != 0xFFFFFFFFu)
-        flush_cell(c, grad_density, cur, acc);
+      flush_all(c, grad_density, A);
+      if (L.i >= 0) {
+        lut_put(s_glut, L.i, L.lo);
+        lut_put(s_glut, L.i + 1, L.hi);
+      }
     }
   }
   __syncthreads();
@@ -307,15 +438,30 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
     gst(dst, i, s_glut[i]);
 }
 
-// grad_lut[j] += sum over workgroups of the partials (fixed order: deterministic)
-__global__ __launch_bounds__(256) void k_reduce_lut(const float *partials, uint32_t nblocks, float *grad_lut)
+// LUT gradient: the per-workgroup partials are summed in two fixed-order
+// stages (slices of workgroups, then the slices), so the result is
+// deterministic and the reduction runs on kLutSlices x 4 workgroups.
+constexpr uint32_t kLutSlices = 128;
+
+__global__ __launch_bounds__(256) void k_reduce_lut_slices(const float *partials, uint32_t nblocks,
+    float *slices)
+{
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x; // LUT scalar 0..1023
+  const uint32_t sl = blockIdx.y;
+  const uint32_t per = (nblocks + kLutSlices - 1) / kLutSlices;
+  const uint32_t b0 = sl * per, b1 = min(nblocks, b0 + per);
+  float s = 0.0f;
+  for (uint32_t b = b0; b < b1; b++)
+    s += gld(partials, (size_t)b * 1024 + j);
+  gst(slices, (size_t)sl * 1024 + j, s);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_lut(const float *slices, float *grad_lut)
 {
   const uint32_t j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= 1024)
-    return;
   float s = 0.0f;
-  for (uint32_t b = 0; b < nblocks; b++)
-    s += gld(partials, (size_t)b * 1024 + j);
+  for (uint32_t sl = 0; sl < kLutSlices; sl++)
+    s += gld(slices, (size_t)sl * 1024 + j);
   grad_lut[j] += s;
 }
 
@@ -401,7 +547,7 @@ int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, c
   // per-workgroup LUT partials live in a grow-only buffer owned by this call site
   static thread_local void *partials = nullptr;
   static thread_local size_t cap = 0;
-  const size_t need = (size_t)nb * 1024 * 4;
+  const size_t need = ((size_t)nb + kLutSlices) * 1024 * 4;
   if (cap < need) {
     if (partials) {
       VRX_HIP(hipStreamSynchronize((hipStream_t)s));
@@ -414,8 +560,11 @@ int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, c
   }
   hipLaunchKernelGGL(k_diffvol_bwd, dim3(nb), dim3(kDvBlock), 0, (hipStream_t)s, c, v->d_lut, n,
       d_org, d_dir, d_grad_rgba, d_ckpt, d_grad_density, (float *)partials);
-  hipLaunchKernelGGL(k_reduce_lut, dim3(4), dim3(256), 0, (hipStream_t)s, (const float *)partials,
-      nb, d_grad_lut);
+  float *slices = (float *)partials + (size_t)nb * 1024;
+  hipLaunchKernelGGL(k_reduce_lut_slices, dim3(4, kLutSlices), dim3(256), 0, (hipStream_t)s,
+      (const float *)partials, nb, slices);
+  hipLaunchKernelGGL(k_reduce_lut, dim3(4), dim3(256), 0, (hipStream_t)s, (const float *)slices,
+      d_grad_lut);
   VRX_LAUNCH_CHECK();
   return VRX_OK;
 }
