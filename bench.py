@@ -150,7 +150,7 @@ def bench_diffvol(iters=200, log_every=50):
     n = 128
     sp = 1.0 / (n - 1)
     spec = D.GridSpec(dims=(n, n, n), spacing=(sp, sp, sp), unit_distance=sp, step=0.5 * sp,
-                      early_stop=True)
+                      early_stop=True, image=(256, 256))
     gt = torch.from_numpy(S.gen_noise_volume(n, seed=11)).cuda()
     gt_lut = torch.from_numpy(D.colormap_lut(S.COLORMAP5, np.array([0.0, 0.3]))).cuda()
     cams = D.orbit_cameras(32, seed=7, fovy=45.0)

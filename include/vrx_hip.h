@@ -251,6 +251,11 @@ typedef struct vrx_diffvol {
   int32_t early_stop;      /* 1 = stop at opacity 0.99 (as the reference) */
   int32_t checkpoint;      /* steps between transmittance checkpoints (64) */
   uint32_t max_steps;      /* per ray upper bound (for checkpoint storage) */
+  /* optional layout hint: rays are images of image_w x image_h, row-major,
+   * one after another (as vrx_make_rays writes them); the adjoint then gives
+   * each workgroup a 16x16 pixel tile so neighbouring rays share gradient
+   * scatter.  0 = rays in no particular order. */
+  uint32_t image_w, image_h;
 } vrx_diffvol;
 
 /* rays: n origins/directions (float3) */

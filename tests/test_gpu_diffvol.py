@@ -19,12 +19,12 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _setup(n, views, res, lut, early, seed=3):
+def _setup(n, views, res, lut, early, seed=3, tiled=True):
     from visrtx_amd import diffvol as D
     dens = S.gen_noise_volume(n, seed=seed)
     sp = 1.0 / (n - 1)
     spec = D.GridSpec(dims=(n, n, n), spacing=(sp, sp, sp), unit_distance=sp, step=0.5 * sp,
-                      early_stop=early)
+                      early_stop=early, image=(res, res) if tiled else None)
     cams = D.orbit_cameras(views, seed=seed, fovy=45.0)
     org, dirs = D.make_rays(cams, res, res)
     td = torch.from_numpy(dens).cuda()
@@ -63,6 +63,17 @@ def test_adjoint_vs_fp64_central_differences(require_gpu):
     torch.cuda.synchronize()
     gd = td.grad.cpu().numpy().reshape(-1)
     gl = tl.grad.cpu().numpy().reshape(-1)
+    # the same gradients with rays treated as unstructured (no tile mapping)
+    td.grad = None
+    tl.grad = None
+    spec_u = D.GridSpec(**{**spec.__dict__, "image": None})
+    rgba_u = D.render(td, tl, org, dirs, spec_u)
+    (rgba_u * torch.from_numpy(w).float().cuda()).sum().backward()
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(td.grad.cpu().numpy().reshape(-1), gd, rtol=1e-4,
+                               atol=1e-6 * np.abs(gd).max())
+    np.testing.assert_allclose(tl.grad.cpu().numpy().reshape(-1), gl, rtol=1e-4,
+                               atol=1e-6 * np.abs(gl).max())
     o, d = org.cpu().numpy(), dirs.cpu().numpy()
     args = (spec.dims, spec.origin, spec.spacing)
     dd = dens.astype(np.float64).reshape(-1).copy()

@@ -40,6 +40,9 @@ struct DvCtx {
   uint32_t nck;        // checkpoint slots per ray (2 + segments)
   int ckpt;            // steps per checkpoint
   int early;
+  // ray layout of the adjoint's workgroups: 16x16 pixel tiles of images of
+  // img_w x img_h when img_w > 0, else 256 consecutive rays
+  uint32_t img_w, img_h, tiles_x, tiles_per_img;
 };
 
 static DvCtx make_ctx(const vrx_diffvol &v)
@@ -65,6 +68,10 @@ static DvCtx make_ctx(const vrx_diffvol &v)
   c.ckpt = v.checkpoint > 0 ? v.checkpoint : 64;
   c.nck = 2 + (v.max_steps + c.ckpt - 1) / c.ckpt;
   c.early = v.early_stop;
+  c.img_w = v.image_w;
+  c.img_h = v.image_h;
+  c.tiles_x = (v.image_w + 15) / 16;
+  c.tiles_per_img = c.tiles_x * ((v.image_h + 15) / 16);
   return c;
 }
 
@@ -219,10 +226,41 @@ struct CornerAcc {
   bool live;
 };
 
-__device__ __forceinline__ void put(float *gd, uint32_t idx, float v)
+// Workgroup voxel cache in LDS: a direct-mapped table of (voxel, gradient)
+// pairs, updated with 64-bit compare-and-swap.  Adding to a slot held by
+// another voxel evicts that voxel's sum to the grid (one global float atomic);
+// the table is drained at the end of the workgroup.  Neighbouring rays of a
+// pixel tile pass the same voxels at about the same time, so most corner
+// updates merge in LDS instead of reaching the grid.
+constexpr int kCacheBits = 12;
+constexpr int kCacheSlots = 1 << kCacheBits;
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t cache_slot(uint32_t v)
 {
-  if (v != 0.0f)
-    unsafeAtomicAdd(gd + idx, v);
+  return (v * 2654435761u) >> (32 - kCacheBits);
+}
+
+__device__ __forceinline__ void put(unsigned long long *cache, float *gd, uint32_t idx, float x)
+{
+  if (x == 0.0f)
+    return;
+  unsigned long long *slot = cache + cache_slot(idx);
+  unsigned long long old = *slot;
+  for (;;) {
+    const uint32_t key = (uint32_t)old;
+    const float val = __uint_as_float((uint32_t)(old >> 32));
+    const bool same = key == idx;
+    const float nv = same ? val + x : x;
+    const unsigned long long nw = ((unsigned long long)__float_as_uint(nv) << 32) | idx;
+    const unsigned long long seen = atomicCAS(slot, old, nw);
+    if (seen == old) {
+      if (!same && key != kEmptyKey && val != 0.0f)
+        unsafeAtomicAdd(gd + key, val);
+      return;
+    }
+    old = seen;
+  }
 }
 
 __device__ __forceinline__ void corner_offsets(const DvCtx &c, uint32_t off[8])
@@ -234,7 +272,8 @@ __device__ __forceinline__ void corner_offsets(const DvCtx &c, uint32_t off[8])
 
 // move the cell by d (-1, 0, +1) along AXIS, flushing the face left behind
 template <int AXIS>
-__device__ __forceinline__ void shift_cell(const DvCtx &c, float *gd, CornerAcc &A, int d)
+__device__ __forceinline__ void shift_cell(
+    const DvCtx &c, unsigned long long *cache, float *gd, CornerAcc &A, int d)
 {
   constexpr int bit = 1 << AXIS;
   const uint32_t stride = AXIS == 0 ? 1u : (AXIS == 1 ? c.sy : c.sz);
@@ -244,7 +283,7 @@ __device__ __forceinline__ void shift_cell(const DvCtx &c, float *gd, CornerAcc 
 #pragma unroll
     for (int q = 0; q < 8; q++)
       if ((q & bit) == 0) {
-        put(gd, A.base + off[q], A.a[q]);
+        put(cache, gd, A.base + off[q], A.a[q]);
         A.a[q] = A.a[q | bit];
         A.a[q | bit] = 0.0f;
       }
@@ -253,7 +292,7 @@ __device__ __forceinline__ void shift_cell(const DvCtx &c, float *gd, CornerAcc 
 #pragma unroll
     for (int q = 0; q < 8; q++)
       if ((q & bit) != 0) {
-        put(gd, A.base + off[q], A.a[q]);
+        put(cache, gd, A.base + off[q], A.a[q]);
         A.a[q] = A.a[q ^ bit];
         A.a[q ^ bit] = 0.0f;
       }
@@ -261,7 +300,8 @@ __device__ __forceinline__ void shift_cell(const DvCtx &c, float *gd, CornerAcc 
   }
 }
 
-__device__ __forceinline__ void flush_all(const DvCtx &c, float *gd, CornerAcc &A)
+__device__ __forceinline__ void flush_all(
+    const DvCtx &c, unsigned long long *cache, float *gd, CornerAcc &A)
 {
   if (!A.live)
     return;
@@ -269,22 +309,23 @@ __device__ __forceinline__ void flush_all(const DvCtx &c, float *gd, CornerAcc &
   corner_offsets(c, off);
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    put(gd, A.base + off[q], A.a[q]);
+    put(cache, gd, A.base + off[q], A.a[q]);
     A.a[q] = 0.0f;
   }
 }
 
-__device__ __forceinline__ void enter_cell(const DvCtx &c, float *gd, CornerAcc &A, const Sample &sm)
+__device__ __forceinline__ void enter_cell(
+    const DvCtx &c, unsigned long long *cache, float *gd, CornerAcc &A, const Sample &sm)
 {
   if (A.live && sm.ci == A.ci && sm.cj == A.cj && sm.ck == A.ck)
     return;
   const int dx = sm.ci - A.ci, dy = sm.cj - A.cj, dz = sm.ck - A.ck;
   if (A.live && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
-    shift_cell<0>(c, gd, A, dx);
-    shift_cell<1>(c, gd, A, dy);
-    shift_cell<2>(c, gd, A, dz);
+    shift_cell<0>(c, cache, gd, A, dx);
+    shift_cell<1>(c, cache, gd, A, dy);
+    shift_cell<2>(c, cache, gd, A, dz);
   } else {
-    flush_all(c, gd, A);
+    flush_all(c, cache, gd, A);
     A.base = sm.base;
     A.live = true;
   }
@@ -342,12 +383,27 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
 {
   __shared__ float4 s_lut[256];
   __shared__ float s_glut[1024];
+  __shared__ unsigned long long s_cache[kCacheSlots];
   s_lut[threadIdx.x] = gld((const float4 *)lut_g, threadIdx.x);
   for (int i = threadIdx.x; i < 1024; i += kDvBlock)
     s_glut[i] = 0.0f;
+  for (int i = threadIdx.x; i < kCacheSlots; i += kDvBlock)
+    s_cache[i] = (unsigned long long)kEmptyKey;
   __syncthreads();
-  const uint32_t r = blockIdx.x * kDvBlock + threadIdx.x;
-  if (r < n) {
+  uint32_t r;
+  bool valid;
+  if (c.img_w > 0) {
+    const uint32_t img = blockIdx.x / c.tiles_per_img, t = blockIdx.x - img * c.tiles_per_img;
+    const uint32_t ty = t / c.tiles_x, tx = t - ty * c.tiles_x;
+    const uint32_t x = tx * 16 + (threadIdx.x & 15), y = ty * 16 + (threadIdx.x >> 4);
+    valid = x < c.img_w && y < c.img_h;
+    r = (uint32_t)((size_t)img * c.img_w * c.img_h + (size_t)y * c.img_w + x);
+    valid = valid && r < n;
+  } else {
+    r = blockIdx.x * kDvBlock + threadIdx.x;
+    valid = r < n;
+  }
+  if (valid) {
     const uint32_t N = __float_as_uint(gld(ckpt, (size_t)0 * n + r));
     float t0, t1;
     const v3 o = gld3(org, r), d = gld3(dir, r);
@@ -412,7 +468,7 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
         const float4 la = s_lut[cl.i], lb = s_lut[cl.i + 1];
         const float ds = ((dr * (lb.x - la.x) + dg * (lb.y - la.y)) + (db * (lb.z - la.z) + da * (lb.w - la.w)))
             * cl.du_ds;
-        enter_cell(c, grad_density, A, sm);
+        enter_cell(c, s_cache, grad_density, A, sm);
         const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
         A.a[0] += ds * (ux * uy * uz);
         A.a[1] += ds * (sm.fx * uy * uz);
@@ -425,7 +481,7 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
         R = (1.0f - tau) * v + tau * R;
         Tn = T;
       }
-      flush_all(c, grad_density, A);
+      flush_all(c, s_cache, grad_density, A);
       if (L.i >= 0) {
         lut_put(s_glut, L.i, L.lo);
         lut_put(s_glut, L.i + 1, L.hi);
@@ -433,6 +489,13 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
     }
   }
   __syncthreads();
+  for (int i = threadIdx.x; i < kCacheSlots; i += kDvBlock) {
+    const unsigned long long e = s_cache[i];
+    const uint32_t key = (uint32_t)e;
+    const float val = __uint_as_float((uint32_t)(e >> 32));
+    if (key != kEmptyKey && val != 0.0f)
+      unsafeAtomicAdd(grad_density + key, val);
+  }
   float *dst = lut_partials + (size_t)blockIdx.x * 1024;
   for (int i = threadIdx.x; i < 1024; i += kDvBlock)
     gst(dst, i, s_glut[i]);
@@ -543,7 +606,12 @@ int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, c
     return VRX_OK;
   VRX_ARG(d_org && d_dir && d_grad_rgba && d_ckpt && d_grad_density && d_grad_lut, "buffers");
   DvCtx c = make_ctx(*v);
-  const uint32_t nb = (n + kDvBlock - 1) / kDvBlock;
+  if (c.img_w > 0) {
+    const size_t per = (size_t)c.img_w * c.img_h;
+    VRX_ARG(c.img_h > 0 && n % per == 0, "image_w/image_h do not divide the ray count");
+  }
+  const uint32_t nb = c.img_w > 0 ? (uint32_t)(n / ((size_t)c.img_w * c.img_h)) * c.tiles_per_img
+                                  : (n + kDvBlock - 1) / kDvBlock;
   // per-workgroup LUT partials live in a grow-only buffer owned by this call site
   static thread_local void *partials = nullptr;
   static thread_local size_t cap = 0;

@@ -30,6 +30,9 @@ class GridSpec:
     step: float = 0.5           # world units
     early_stop: bool = True     # stop at opacity 0.99, as the reference does
     checkpoint: int = 64
+    # rays are images of image[0] x image[1] pixels (vrx_make_rays order);
+    # lets the adjoint tile its workgroups.  None = unstructured rays.
+    image: tuple | None = None
 
     @property
     def max_steps(self) -> int:
@@ -50,6 +53,8 @@ class GridSpec:
         v.early_stop = 1 if self.early_stop else 0
         v.checkpoint = self.checkpoint
         v.max_steps = self.max_steps
+        if self.image is not None:
+            v.image_w, v.image_h = int(self.image[0]), int(self.image[1])
         return v
 
 
