@@ -425,6 +425,8 @@ def main():
         extra["volume_kernel_ms"] = round(kms, 3)
         extra["volume_hbm_frac_32B_per_sample"] = round(ref_ns * 32 / (kms / 1000) / 1e9 /
                                                         HBM_PEAK_GBS, 4)
+        vt = load_traffic("render_volume512_1920x1080")
+        extra["volume_traffic_bytes_per_launch"] = vt["hbm_bytes_per_launch"] if vt else None
         log(f"volume: kernel {kms:.3f} ms, {ns} samples (oracle {ref_ns}) -> "
             f"{extra['volume_gsamples_s']} Gsamples/s")
         vol.release()
