@@ -226,40 +226,57 @@ struct CornerAcc {
   bool live;
 };
 
-// Workgroup voxel cache in LDS: a direct-mapped table of (voxel, gradient)
-// pairs, updated with 64-bit compare-and-swap.  Adding to a slot held by
-// another voxel evicts that voxel's sum to the grid (one global float atomic);
-// the table is drained at the end of the workgroup.  Neighbouring rays of a
-// pixel tile pass the same voxels at about the same time, so most corner
-// updates merge in LDS instead of reaching the grid.
+// Workgroup voxel cache in LDS: a direct-mapped table of voxel keys and
+// gradient sums.  A corner update goes to its slot when the slot is free or
+// already holds that voxel (LDS float add), else straight to the grid (one
+// global float atomic).  The workgroup's rays step in lockstep and the table
+// is drained to the grid every kDrainEvery steps, so it holds the voxels that
+// the pixel tile's rays are passing; neighbouring rays visit the same voxels
+// at about the same step, so most updates merge in LDS.  Keys never change
+// between drains, which keeps the add race-free without a lock.
 constexpr int kCacheBits = 12;
 constexpr int kCacheSlots = 1 << kCacheBits;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+constexpr uint32_t kDrainEvery = 32;
+
+struct VoxCache {
+  uint32_t *key;
+  float *val;
+};
 
 __device__ __forceinline__ uint32_t cache_slot(uint32_t v)
 {
   return (v * 2654435761u) >> (32 - kCacheBits);
 }
 
-__device__ __forceinline__ void put(unsigned long long *cache, float *gd, uint32_t idx, float x)
+__device__ __forceinline__ void put(const VoxCache &vc, float *gd, uint32_t idx, float x)
 {
   if (x == 0.0f)
     return;
-  unsigned long long *slot = cache + cache_slot(idx);
-  unsigned long long old = *slot;
-  for (;;) {
-    const uint32_t key = (uint32_t)old;
-    const float val = __uint_as_float((uint32_t)(old >> 32));
-    const bool same = key == idx;
-    const float nv = same ? val + x : x;
-    const unsigned long long nw = ((unsigned long long)__float_as_uint(nv) << 32) | idx;
-    const unsigned long long seen = atomicCAS(slot, old, nw);
-    if (seen == old) {
-      if (!same && key != kEmptyKey && val != 0.0f)
-        unsafeAtomicAdd(gd + key, val);
-      return;
+  const uint32_t h = cache_slot(idx);
+  uint32_t k = vc.key[h];
+  if (k == kEmptyKey) {
+    const uint32_t old = atomicCAS(vc.key + h, kEmptyKey, idx);
+    k = old == kEmptyKey ? idx : old;
+  }
+  if (k == idx)
+    atomicAdd(vc.val + h, x);
+  else
+    unsafeAtomicAdd(gd + idx, x);
+}
+
+// all threads of the workgroup, between barriers
+__device__ __forceinline__ void drain(const VoxCache &vc, float *gd)
+{
+  for (int i = threadIdx.x; i < kCacheSlots; i += kDvBlock) {
+    const uint32_t k = vc.key[i];
+    if (k != kEmptyKey) {
+      const float v = vc.val[i];
+      if (v != 0.0f)
+        unsafeAtomicAdd(gd + k, v);
+      vc.key[i] = kEmptyKey;
+      vc.val[i] = 0.0f;
     }
-    old = seen;
   }
 }
 
@@ -273,7 +290,7 @@ __device__ __forceinline__ void corner_offsets(const DvCtx &c, uint32_t off[8])
 // move the cell by d (-1, 0, +1) along AXIS, flushing the face left behind
 template <int AXIS>
 __device__ __forceinline__ void shift_cell(
-    const DvCtx &c, unsigned long long *cache, float *gd, CornerAcc &A, int d)
+    const DvCtx &c, const VoxCache &cache, float *gd, CornerAcc &A, int d)
 {
   constexpr int bit = 1 << AXIS;
   const uint32_t stride = AXIS == 0 ? 1u : (AXIS == 1 ? c.sy : c.sz);
@@ -301,7 +318,7 @@ __device__ __forceinline__ void shift_cell(
 }
 
 __device__ __forceinline__ void flush_all(
-    const DvCtx &c, unsigned long long *cache, float *gd, CornerAcc &A)
+    const DvCtx &c, const VoxCache &cache, float *gd, CornerAcc &A)
 {
   if (!A.live)
     return;
@@ -315,7 +332,7 @@ __device__ __forceinline__ void flush_all(
 }
 
 __device__ __forceinline__ void enter_cell(
-    const DvCtx &c, unsigned long long *cache, float *gd, CornerAcc &A, const Sample &sm)
+    const DvCtx &c, const VoxCache &cache, float *gd, CornerAcc &A, const Sample &sm)
 {
   if (A.live && sm.ci == A.ci && sm.cj == A.cj && sm.ck == A.ck)
     return;
@@ -383,13 +400,21 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
 {
   __shared__ float4 s_lut[256];
   __shared__ float s_glut[1024];
-  __shared__ unsigned long long s_cache[kCacheSlots];
+  __shared__ uint32_t s_key[kCacheSlots];
+  __shared__ float s_val[kCacheSlots];
+  __shared__ uint32_t s_itmax;
+  const VoxCache vc{s_key, s_val};
   s_lut[threadIdx.x] = gld((const float4 *)lut_g, threadIdx.x);
   for (int i = threadIdx.x; i < 1024; i += kDvBlock)
     s_glut[i] = 0.0f;
-  for (int i = threadIdx.x; i < kCacheSlots; i += kDvBlock)
-    s_cache[i] = (unsigned long long)kEmptyKey;
+  for (int i = threadIdx.x; i < kCacheSlots; i += kDvBlock) {
+    s_key[i] = kEmptyKey;
+    s_val[i] = 0.0f;
+  }
+  if (threadIdx.x == 0)
+    s_itmax = 0;
   __syncthreads();
+
   uint32_t r;
   bool valid;
   if (c.img_w > 0) {
@@ -403,99 +428,119 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
     r = blockIdx.x * kDvBlock + threadIdx.x;
     valid = r < n;
   }
+
+  // per-ray state
+  uint32_t N = 0;
+  float t0 = 0.0f, t1 = 0.0f;
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f, gO = 0.0f;
+  float gox = 0.0f, gdx = 0.0f, goy = 0.0f, gdy = 0.0f, goz = 0.0f, gdz = 0.0f;
+  float Tn = 0.0f, R = 0.0f;
   if (valid) {
-    const uint32_t N = __float_as_uint(gld(ckpt, (size_t)0 * n + r));
-    float t0, t1;
+    N = __float_as_uint(gld(ckpt, (size_t)0 * n + r));
     const v3 o = gld3(org, r), d = gld3(dir, r);
     if (N > 0 && dv_slab(c, o, d, t0, t1)) {
       const float4 G4 = gld((const float4 *)grad_rgba, r);
-      const float gx = G4.x, gy = G4.y, gz = G4.z, gO = G4.w;
-      const float gox = (o.x - c.origin[0]) * c.inv_sp[0], gdx = d.x * c.inv_sp[0];
-      const float goy = (o.y - c.origin[1]) * c.inv_sp[1], gdy = d.y * c.inv_sp[1];
-      const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
-      float Tn = gld(ckpt, (size_t)1 * n + r); // T after the last sample
-      float R = 0.0f;
-      CornerAcc A;
-      A.live = false;
-      A.base = 0;
-      A.ci = A.cj = A.ck = 0;
-#pragma unroll
-      for (int q = 0; q < 8; q++)
-        A.a[q] = 0.0f;
-      LutAcc L;
-      L.i = -1;
-#pragma unroll
-      for (int ch = 0; ch < 4; ch++)
-        L.lo[ch] = L.hi[ch] = 0.0f;
-      for (int k = (int)N - 1; k >= 0; k--) {
-        const float t = t0 + (float)k * c.step;
-        const Sample sm = dv_sample(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
-        const Cls cl = dv_classify(s_lut, c, sm.s);
-        const float tau = dv_tau(c, cl.co.w);
-        float T;
-        const uint32_t seg = (uint32_t)k / (uint32_t)c.ckpt;
-        if ((uint32_t)k % (uint32_t)c.ckpt == 0) {
-          T = gld(ckpt, (size_t)(2 + seg) * n + r);
-        } else if (tau >= kMinTauForDivision && Tn > 0.0f) {
-          T = Tn * __builtin_amdgcn_rcpf(tau);
-        } else {
-          // replay from the segment checkpoint
-          T = gld(ckpt, (size_t)(2 + seg) * n + r);
-          for (uint32_t j = seg * (uint32_t)c.ckpt; j < (uint32_t)k; j++) {
-            const float tj = t0 + (float)j * c.step;
-            const Sample sj = dv_sample(c, gox + gdx * tj, goy + gdy * tj, goz + gdz * tj);
-            T *= dv_tau(c, dv_classify(s_lut, c, sj.s).co.w);
-          }
-        }
-        const float v = (gx * cl.co.x + gy * cl.co.y + gz * cl.co.z) + gO;
-        const float w = T * (1.0f - tau);
-        const float dtau = T * (R - v);
-        const float oma = fmaxf(1.0f - cl.co.w, 1e-8f);
-        const float da = dtau * (-c.expo) * tau * __builtin_amdgcn_rcpf(oma);
-        const float dr = w * gx, dg = w * gy, db = w * gz;
-        // LUT: entries i and i+1 with weights (1-f) and f
-        lut_enter(s_glut, L, cl.i);
-        const float wf0 = 1.0f - cl.f, wf1 = cl.f;
-        L.lo[0] += wf0 * dr;
-        L.lo[1] += wf0 * dg;
-        L.lo[2] += wf0 * db;
-        L.lo[3] += wf0 * da;
-        L.hi[0] += wf1 * dr;
-        L.hi[1] += wf1 * dg;
-        L.hi[2] += wf1 * db;
-        L.hi[3] += wf1 * da;
-        // density: d(channel)/ds through the LUT slope, then trilinear weights
-        const float4 la = s_lut[cl.i], lb = s_lut[cl.i + 1];
-        const float ds = ((dr * (lb.x - la.x) + dg * (lb.y - la.y)) + (db * (lb.z - la.z) + da * (lb.w - la.w)))
-            * cl.du_ds;
-        enter_cell(c, s_cache, grad_density, A, sm);
-        const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
-        A.a[0] += ds * (ux * uy * uz);
-        A.a[1] += ds * (sm.fx * uy * uz);
-        A.a[2] += ds * (ux * sm.fy * uz);
-        A.a[3] += ds * (sm.fx * sm.fy * uz);
-        A.a[4] += ds * (ux * uy * sm.fz);
-        A.a[5] += ds * (sm.fx * uy * sm.fz);
-        A.a[6] += ds * (ux * sm.fy * sm.fz);
-        A.a[7] += ds * (sm.fx * sm.fy * sm.fz);
-        R = (1.0f - tau) * v + tau * R;
-        Tn = T;
-      }
-      flush_all(c, s_cache, grad_density, A);
-      if (L.i >= 0) {
-        lut_put(s_glut, L.i, L.lo);
-        lut_put(s_glut, L.i + 1, L.hi);
-      }
+      gx = G4.x;
+      gy = G4.y;
+      gz = G4.z;
+      gO = G4.w;
+      gox = (o.x - c.origin[0]) * c.inv_sp[0];
+      gdx = d.x * c.inv_sp[0];
+      goy = (o.y - c.origin[1]) * c.inv_sp[1];
+      gdy = d.y * c.inv_sp[1];
+      goz = (o.z - c.origin[2]) * c.inv_sp[2];
+      gdz = d.z * c.inv_sp[2];
+      Tn = gld(ckpt, (size_t)1 * n + r); // T after the last sample
+    } else {
+      N = 0;
     }
   }
+  if (N > 0)
+    atomicMax(&s_itmax, N);
+  CornerAcc A;
+  A.live = false;
+  A.base = 0;
+  A.ci = A.cj = A.ck = 0;
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+    A.a[q] = 0.0f;
+  LutAcc L;
+  L.i = -1;
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++)
+    L.lo[ch] = L.hi[ch] = 0.0f;
   __syncthreads();
-  for (int i = threadIdx.x; i < kCacheSlots; i += kDvBlock) {
-    const unsigned long long e = s_cache[i];
-    const uint32_t key = (uint32_t)e;
-    const float val = __uint_as_float((uint32_t)(e >> 32));
-    if (key != kEmptyKey && val != 0.0f)
-      unsafeAtomicAdd(grad_density + key, val);
+  const uint32_t itmax = s_itmax;
+
+  // lockstep over the tile: step j processes sample N-1-j of every ray
+  for (uint32_t j0 = 0; j0 < itmax; j0 += kDrainEvery) {
+    const uint32_t j1 = min(itmax, j0 + kDrainEvery);
+    for (uint32_t j = j0; j < j1 && j < N; j++) {
+      const int k = (int)(N - 1 - j);
+      const float t = t0 + (float)k * c.step;
+      const Sample sm = dv_sample(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
+      const Cls cl = dv_classify(s_lut, c, sm.s);
+      const float tau = dv_tau(c, cl.co.w);
+      float T;
+      const uint32_t seg = (uint32_t)k / (uint32_t)c.ckpt;
+      if ((uint32_t)k % (uint32_t)c.ckpt == 0) {
+        T = gld(ckpt, (size_t)(2 + seg) * n + r);
+      } else if (tau >= kMinTauForDivision && Tn > 0.0f) {
+        T = Tn * __builtin_amdgcn_rcpf(tau);
+      } else {
+        // replay from the segment checkpoint
+        T = gld(ckpt, (size_t)(2 + seg) * n + r);
+        for (uint32_t q = seg * (uint32_t)c.ckpt; q < (uint32_t)k; q++) {
+          const float tq = t0 + (float)q * c.step;
+          const Sample sq = dv_sample(c, gox + gdx * tq, goy + gdy * tq, goz + gdz * tq);
+          T *= dv_tau(c, dv_classify(s_lut, c, sq.s).co.w);
+        }
+      }
+      const float v = (gx * cl.co.x + gy * cl.co.y + gz * cl.co.z) + gO;
+      const float w = T * (1.0f - tau);
+      const float dtau = T * (R - v);
+      const float oma = fmaxf(1.0f - cl.co.w, 1e-8f);
+      const float da = dtau * (-c.expo) * tau * __builtin_amdgcn_rcpf(oma);
+      const float dr = w * gx, dg = w * gy, db = w * gz;
+      // LUT: entries i and i+1 with weights (1-f) and f
+      lut_enter(s_glut, L, cl.i);
+      const float wf0 = 1.0f - cl.f, wf1 = cl.f;
+      L.lo[0] += wf0 * dr;
+      L.lo[1] += wf0 * dg;
+      L.lo[2] += wf0 * db;
+      L.lo[3] += wf0 * da;
+      L.hi[0] += wf1 * dr;
+      L.hi[1] += wf1 * dg;
+      L.hi[2] += wf1 * db;
+      L.hi[3] += wf1 * da;
+      // density: d(channel)/ds through the LUT slope, then trilinear weights
+      const float4 la = s_lut[cl.i], lb = s_lut[cl.i + 1];
+      const float ds = ((dr * (lb.x - la.x) + dg * (lb.y - la.y)) + (db * (lb.z - la.z) + da * (lb.w - la.w)))
+          * cl.du_ds;
+      enter_cell(c, vc, grad_density, A, sm);
+      const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
+      A.a[0] += ds * (ux * uy * uz);
+      A.a[1] += ds * (sm.fx * uy * uz);
+      A.a[2] += ds * (ux * sm.fy * uz);
+      A.a[3] += ds * (sm.fx * sm.fy * uz);
+      A.a[4] += ds * (ux * uy * sm.fz);
+      A.a[5] += ds * (sm.fx * uy * sm.fz);
+      A.a[6] += ds * (ux * sm.fy * sm.fz);
+      A.a[7] += ds * (sm.fx * sm.fy * sm.fz);
+      R = (1.0f - tau) * v + tau * R;
+      Tn = T;
+    }
+    __syncthreads();
+    drain(vc, grad_density);
+    __syncthreads();
   }
+  flush_all(c, vc, grad_density, A);
+  if (L.i >= 0) {
+    lut_put(s_glut, L.i, L.lo);
+    lut_put(s_glut, L.i + 1, L.hi);
+  }
+  __syncthreads();
+  drain(vc, grad_density);
   float *dst = lut_partials + (size_t)blockIdx.x * 1024;
   for (int i = threadIdx.x; i < 1024; i += kDvBlock)
     gst(dst, i, s_glut[i]);
