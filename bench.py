@@ -338,7 +338,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this device renders on one GPU; multi-GPU sharding is "
-                                   "design-only (see DESIGN.md), run with --gpus 1"}))
+                                   "described but not built (one GPU per process); run with --gpus 1"}))
         sys.exit(2)
     sec = {"headline"} if args.only_headline else set(args.sections.split(","))
     unknown = sec - set(SECTIONS)

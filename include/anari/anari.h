@@ -9,7 +9,7 @@
  * values below are restated from the specification's type list; those for
  * the box/matrix types (2004+) could not be checked against an SDK install
  * offline.  An application compiled against the real SDK headers must use
- * the same values for the types it passes (see INTEGRATION.md).
+ * the same values for the types it passes.
  */
 #ifndef VRX_ANARI_H
 #define VRX_ANARI_H
@@ -45,6 +45,12 @@ typedef ANARIObject ANARIVolume;
 typedef ANARIObject ANARIWorld;
 
 typedef int ANARIDataType;
+/* one entry of an ANARI_PARAMETER_LIST (anariGetObjectInfo "parameter"),
+   terminated by {NULL, ANARI_UNKNOWN} */
+typedef struct {
+  const char *name;
+  ANARIDataType type;
+} ANARIParameter;
 typedef int ANARIStatusSeverity;
 typedef int ANARIStatusCode;
 typedef int ANARIWaitMask;

@@ -64,6 +64,12 @@ void freeParam(Param &p)
 
 } // namespace
 
+namespace vrx_anari {
+const void *objectInfo(ANARIDataType type, const char *subtype, const char *info, ANARIDataType infoType);
+const void *parameterInfo(ANARIDataType type, const char *subtype, const char *param,
+    ANARIDataType paramType, const char *info, ANARIDataType infoType);
+} // namespace vrx_anari
+
 extern "C" {
 
 // ---------------------------------------------------------------------------
@@ -373,15 +379,16 @@ const char **anariGetObjectSubtypes(ANARIDevice, ANARIDataType t)
   }
 }
 
-const void *anariGetObjectInfo(ANARIDevice, ANARIDataType, const char *, const char *, ANARIDataType)
+const void *anariGetObjectInfo(ANARIDevice, ANARIDataType objectType, const char *subtype,
+    const char *infoName, ANARIDataType infoType)
 {
-  return nullptr;
+  return vrx_anari::objectInfo(objectType, subtype, infoName, infoType);
 }
 
-const void *anariGetParameterInfo(ANARIDevice, ANARIDataType, const char *, const char *,
-    ANARIDataType, const char *, ANARIDataType)
+const void *anariGetParameterInfo(ANARIDevice, ANARIDataType objectType, const char *subtype,
+    const char *paramName, ANARIDataType paramType, const char *infoName, ANARIDataType infoType)
 {
-  return nullptr;
+  return vrx_anari::parameterInfo(objectType, subtype, paramName, paramType, infoName, infoType);
 }
 
 int anariGetProperty(ANARIDevice d, ANARIObject o, const char *name, ANARIDataType t, void *mem,

@@ -1,0 +1,223 @@
+// info.cpp -- ANARI object and parameter introspection (anariGetObjectInfo /
+// anariGetParameterInfo) for the objects this device implements.
+//
+// The reference answers these queries from tables generated out of
+// devices/rtx/visrtx_device.json by the ANARI-SDK code generator
+// (devices/rtx/CMakeLists.txt:385-391).  Here the tables are written out
+// directly for the subtypes on this device's path, with the defaults and
+// clamps the device applies when it commits each object (objects.cpp).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <anari/anari.h>
+
+namespace vrx_anari {
+
+namespace {
+
+struct ParamDef {
+  const char *name;
+  ANARIDataType type;
+  bool required;
+  const void *def; // default value of `type`, or null
+  const void *min;
+  const void *max;
+  const char *desc;
+};
+
+struct ObjDef {
+  ANARIDataType type;
+  const char *subtype; // null: the object type has no subtypes
+  const char *desc;
+  std::vector<ParamDef> params;
+  std::vector<ANARIParameter> list; // null-terminated, built on first use
+};
+
+const float kZero3[3] = {0.0f, 0.0f, 0.0f};
+const float kOne3[3] = {1.0f, 1.0f, 1.0f};
+const float kGrey3[3] = {0.8f, 0.8f, 0.8f};
+const float kDirZ[3] = {0.0f, 0.0f, 1.0f};
+const float kUpY[3] = {0.0f, 1.0f, 0.0f};
+const float kBlack4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+const float kRegion[4] = {0.0f, 0.0f, 1.0f, 1.0f};
+const float kRange01[2] = {0.0f, 1.0f};
+const float kOne = 1.0f;
+const float kSphereRadius = 0.01f;
+const float kFovy = 1.04719755f; // 60 degrees
+const float kRate = 0.125f, kRateMin = 1e-3f, kRateMax = 10.0f;
+const float kAoDist = 1e20f;
+const float kHalf = 0.5f;
+const int32_t kInt1 = 1, kInt0 = 0, kInt256 = 256, kLimit = 128;
+const int32_t kFalse = 0;
+const uint32_t kNoId = 0xFFFFFFFFu;
+const ANARIDataType kSrgb = ANARI_UFIXED8_RGBA_SRGB;
+const float kIdentity4[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+
+std::vector<ObjDef> &tables()
+{
+  static std::vector<ObjDef> t = {
+      {ANARI_GEOMETRY, "sphere", "spheres given by centres and radii (Sphere.cu)",
+          {{"vertex.position", ANARI_ARRAY1D, true, nullptr, nullptr, nullptr, "FLOAT32_VEC3 centres"},
+              {"vertex.radius", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "FLOAT32 per-centre radii"},
+              {"radius", ANARI_FLOAT32, false, &kSphereRadius, nullptr, nullptr, "radius when no vertex.radius"},
+              {"primitive.index", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32 centre indices"},
+              {"primitive.id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32 primitiveId values"}}},
+      {ANARI_GEOMETRY, "cylinder", "cylinders between vertex pairs (Cylinder.cpp)",
+          {{"vertex.position", ANARI_ARRAY1D, true, nullptr, nullptr, nullptr, "FLOAT32_VEC3 end points"},
+              {"primitive.index", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32_VEC2 end point pairs"},
+              {"primitive.radius", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "FLOAT32 per-cylinder radii"},
+              {"radius", ANARI_FLOAT32, false, &kOne, nullptr, nullptr, "radius when no primitive.radius"},
+              {"caps", ANARI_STRING, false, "none", nullptr, nullptr, "\"none\" or \"both\""},
+              {"primitive.id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32 primitiveId values"}}},
+      {ANARI_GEOMETRY, "triangle", "triangle soup or indexed mesh (Triangle.cpp)",
+          {{"vertex.position", ANARI_ARRAY1D, true, nullptr, nullptr, nullptr, "FLOAT32_VEC3 vertices"},
+              {"primitive.index", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32_VEC3 triangles"},
+              {"vertex.normal", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "FLOAT32_VEC3 shading normals"},
+              {"primitive.id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32 primitiveId values"},
+              {"cullBackfaces", ANARI_BOOL, false, &kFalse, nullptr, nullptr, "skip back-facing hits"}}},
+      {ANARI_MATERIAL, "matte", "constant-colour matte material (Matte.cpp)",
+          {{"color", ANARI_FLOAT32_VEC3, false, kGrey3, nullptr, nullptr, "base colour"},
+              {"opacity", ANARI_FLOAT32, false, &kOne, nullptr, nullptr, "opacity"},
+              {"alphaMode", ANARI_STRING, false, "opaque", nullptr, nullptr, "opaque | blend | mask"},
+              {"alphaCutoff", ANARI_FLOAT32, false, &kHalf, nullptr, nullptr, "mask threshold"}}},
+      {ANARI_SURFACE, nullptr, "a geometry with a material",
+          {{"geometry", ANARI_GEOMETRY, true, nullptr, nullptr, nullptr, "the geometry"},
+              {"material", ANARI_MATERIAL, false, nullptr, nullptr, nullptr, "the material"},
+              {"id", ANARI_UINT32, false, &kNoId, nullptr, nullptr, "objectId value"}}},
+      {ANARI_SPATIAL_FIELD, "structuredRegular", "regular grid of samples (StructuredRegularField.cpp)",
+          {{"data", ANARI_ARRAY3D, true, nullptr, nullptr, nullptr, "FLOAT32 / UINT8 / UFIXED8 samples, x fastest"},
+              {"origin", ANARI_FLOAT32_VEC3, false, kZero3, nullptr, nullptr, "position of sample (0,0,0)"},
+              {"spacing", ANARI_FLOAT32_VEC3, false, kOne3, nullptr, nullptr, "sample spacing"},
+              {"filter", ANARI_STRING, false, "linear", nullptr, nullptr, "reconstruction filter"}}},
+      {ANARI_VOLUME, "transferFunction1D", "emission-absorption volume (TransferFunction1D.cpp)",
+          {{"value", ANARI_SPATIAL_FIELD, true, nullptr, nullptr, nullptr, "the field"},
+              {"color", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "FLOAT32_VEC3/VEC4 colour map"},
+              {"opacity", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "FLOAT32 opacity map"},
+              {"valueRange", ANARI_FLOAT32_BOX1, false, kRange01, nullptr, nullptr, "field values mapped to the maps"},
+              {"unitDistance", ANARI_FLOAT32, false, &kOne, nullptr, nullptr, "distance of one opacity unit"},
+              {"id", ANARI_UINT32, false, &kNoId, nullptr, nullptr, "objectId value"}}},
+      {ANARI_GROUP, nullptr, "surfaces and volumes instanced together",
+          {{"surface", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "SURFACE array"},
+              {"volume", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "VOLUME array"}}},
+      {ANARI_INSTANCE, "transform", "a group under an affine transform (Instance.cpp)",
+          {{"group", ANARI_GROUP, true, nullptr, nullptr, nullptr, "the group"},
+              {"transform", ANARI_FLOAT32_MAT4, false, kIdentity4, nullptr, nullptr, "object to world"},
+              {"id", ANARI_UINT32, false, &kNoId, nullptr, nullptr, "instanceId value"}}},
+      {ANARI_WORLD, nullptr, "the scene",
+          {{"surface", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "SURFACE array"},
+              {"volume", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "VOLUME array"},
+              {"instance", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "INSTANCE array"},
+              {"id", ANARI_UINT32, false, &kNoId, nullptr, nullptr, "instanceId of world-level objects"}}},
+      {ANARI_CAMERA, "perspective", "pinhole camera (Perspective.cpp)",
+          {{"position", ANARI_FLOAT32_VEC3, false, kZero3, nullptr, nullptr, "eye position"},
+              {"direction", ANARI_FLOAT32_VEC3, false, kDirZ, nullptr, nullptr, "view direction"},
+              {"up", ANARI_FLOAT32_VEC3, false, kUpY, nullptr, nullptr, "up vector"},
+              {"fovy", ANARI_FLOAT32, false, &kFovy, nullptr, nullptr, "vertical field of view (radians)"},
+              {"aspect", ANARI_FLOAT32, false, &kOne, nullptr, nullptr, "width / height"},
+              {"imageRegion", ANARI_FLOAT32_BOX2, false, kRegion, nullptr, nullptr, "rendered part of the image plane"}}},
+      {ANARI_FRAME, nullptr, "frame buffer and render target",
+          {{"world", ANARI_WORLD, true, nullptr, nullptr, nullptr, "the scene"},
+              {"camera", ANARI_CAMERA, true, nullptr, nullptr, nullptr, "the camera"},
+              {"renderer", ANARI_RENDERER, true, nullptr, nullptr, nullptr, "the renderer"},
+              {"size", ANARI_UINT32_VEC2, true, nullptr, nullptr, nullptr, "width, height"},
+              {"channel.color", ANARI_DATA_TYPE, false, &kSrgb, nullptr, nullptr, "UFIXED8_RGBA_SRGB | UFIXED8_VEC4 | FLOAT32_VEC4"},
+              {"channel.depth", ANARI_DATA_TYPE, false, nullptr, nullptr, nullptr, "FLOAT32"},
+              {"channel.primitiveId", ANARI_DATA_TYPE, false, nullptr, nullptr, nullptr, "UINT32"},
+              {"channel.objectId", ANARI_DATA_TYPE, false, nullptr, nullptr, nullptr, "UINT32"},
+              {"channel.instanceId", ANARI_DATA_TYPE, false, nullptr, nullptr, nullptr, "UINT32"},
+              {"frameCompletionCallback", ANARI_FRAME_COMPLETION_CALLBACK, false, nullptr, nullptr, nullptr, "called when a frame completes"},
+              {"frameCompletionCallbackUserData", ANARI_VOID_POINTER, false, nullptr, nullptr, nullptr, "callback argument"}}},
+  };
+  return t;
+}
+
+std::vector<ParamDef> rendererParams(bool ao)
+{
+  std::vector<ParamDef> p = {
+      {"background", ANARI_FLOAT32_VEC4, false, kBlack4, nullptr, nullptr, "colour of rays that miss"},
+      {"ambientColor", ANARI_FLOAT32_VEC3, false, kOne3, nullptr, nullptr, "ambient light colour"},
+      {"ambientRadiance", ANARI_FLOAT32, false, &kOne, nullptr, nullptr, "ambient light intensity"},
+      {"cullTriangleBackfaces", ANARI_BOOL, false, &kFalse, nullptr, nullptr, "skip back-facing triangles"},
+      {"volumeSamplingRate", ANARI_FLOAT32, false, &kRate, &kRateMin, &kRateMax, "samples per half voxel"},
+      {"sampleLimit", ANARI_INT32, false, &kLimit, &kInt0, nullptr, "accumulation limit (raycast: 1)"},
+      {"stackDepthLDS", ANARI_INT32, false, &kInt0, &kInt0, nullptr, "traversal stack entries kept in LDS (0: default)"},
+      {"volumeUnroll", ANARI_INT32, false, &kInt0, &kInt0, nullptr, "volume samples gathered per step (0: default)"}};
+  if (ao) {
+    p.push_back({"pixelSamples", ANARI_INT32, false, &kInt1, &kInt1, nullptr, "samples per pixel per frame"});
+    p.push_back({"ambientSamples", ANARI_INT32, false, &kInt1, &kInt0, &kInt256, "occlusion rays per hit"});
+    p.push_back({"ambientOcclusionDistance", ANARI_FLOAT32, false, &kAoDist, nullptr, nullptr, "occlusion ray length"});
+  }
+  return p;
+}
+
+ObjDef *find(ANARIDataType type, const char *subtype)
+{
+  static std::vector<ObjDef> renderers;
+  if (type == ANARI_RENDERER) {
+    if (renderers.empty()) {
+      renderers.push_back({ANARI_RENDERER, "raycast", "first-hit shading (Raycast_ptx.cu)", rendererParams(false), {}});
+      renderers.push_back({ANARI_RENDERER, "ao", "ambient occlusion (AmbientOcclusion_ptx.cu)", rendererParams(true), {}});
+      renderers.push_back({ANARI_RENDERER, "default", "ambient occlusion", rendererParams(true), {}});
+    }
+    for (auto &o : renderers)
+      if (subtype && std::strcmp(subtype, o.subtype) == 0)
+        return &o;
+    return nullptr;
+  }
+  for (auto &o : tables()) {
+    if (o.type != type)
+      continue;
+    if (!o.subtype || (subtype && std::strcmp(subtype, o.subtype) == 0))
+      return &o;
+  }
+  return nullptr;
+}
+
+} // namespace
+
+const void *objectInfo(ANARIDataType type, const char *subtype, const char *info, ANARIDataType infoType)
+{
+  ObjDef *o = find(type, subtype);
+  if (!o || !info)
+    return nullptr;
+  if (std::strcmp(info, "description") == 0 && infoType == ANARI_STRING)
+    return o->desc;
+  if (std::strcmp(info, "parameter") == 0 && infoType == ANARI_PARAMETER_LIST) {
+    if (o->list.empty()) {
+      for (const ParamDef &p : o->params)
+        o->list.push_back({p.name, p.type});
+      o->list.push_back({nullptr, ANARI_UNKNOWN});
+    }
+    return o->list.data();
+  }
+  return nullptr;
+}
+
+const void *parameterInfo(ANARIDataType type, const char *subtype, const char *param,
+    ANARIDataType paramType, const char *info, ANARIDataType infoType)
+{
+  static const int32_t kTrue = 1;
+  ObjDef *o = find(type, subtype);
+  if (!o || !param || !info)
+    return nullptr;
+  for (const ParamDef &p : o->params) {
+    if (std::strcmp(p.name, param) != 0 || p.type != paramType)
+      continue;
+    if (std::strcmp(info, "required") == 0 && infoType == ANARI_BOOL)
+      return p.required ? &kTrue : &kFalse;
+    if (std::strcmp(info, "description") == 0 && infoType == ANARI_STRING)
+      return p.desc;
+    if (std::strcmp(info, "default") == 0 && infoType == p.type)
+      return p.def;
+    if (std::strcmp(info, "minimum") == 0 && infoType == p.type)
+      return p.min;
+    if (std::strcmp(info, "maximum") == 0 && infoType == p.type)
+      return p.max;
+    return nullptr;
+  }
+  return nullptr;
+}
+
+} // namespace vrx_anari
