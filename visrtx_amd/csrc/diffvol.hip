@@ -234,10 +234,10 @@ struct CornerAcc {
 // the pixel tile's rays are passing; neighbouring rays visit the same voxels
 // at about the same step, so most updates merge in LDS.  Keys never change
 // between drains, which keeps the add race-free without a lock.
-constexpr int kCacheBits = 12;
+constexpr int kCacheBits = 11;
 constexpr int kCacheSlots = 1 << kCacheBits;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
-constexpr uint32_t kDrainEvery = 32;
+constexpr uint32_t kDrainEvery = 16;
 
 struct VoxCache {
   uint32_t *key;
@@ -351,6 +351,14 @@ __device__ __forceinline__ void enter_cell(
   A.ck = sm.ck;
 }
 
+// The workgroup keeps kLutCopies copies of the LUT gradient in LDS, lane l
+// adding into copy l % kLutCopies: lanes of a tile sample similar densities,
+// so they hit the same entries, and the copies cut those same-address
+// conflicts.  The copy stride is padded by one float so that copies of one
+// entry fall in different banks.
+constexpr int kLutCopies = 4;
+constexpr int kGlutStride = 1025;
+
 struct LutAcc {
   float lo[4], hi[4]; // gradients of entries i and i+1
   int i;
@@ -399,14 +407,15 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
     float *grad_density, float *lut_partials)
 {
   __shared__ float4 s_lut[256];
-  __shared__ float s_glut[1024];
+  __shared__ float s_glut[kLutCopies * kGlutStride];
   __shared__ uint32_t s_key[kCacheSlots];
   __shared__ float s_val[kCacheSlots];
   __shared__ uint32_t s_itmax;
   const VoxCache vc{s_key, s_val};
   s_lut[threadIdx.x] = gld((const float4 *)lut_g, threadIdx.x);
-  for (int i = threadIdx.x; i < 1024; i += kDvBlock)
+  for (int i = threadIdx.x; i < kLutCopies * kGlutStride; i += kDvBlock)
     s_glut[i] = 0.0f;
+  float *my_glut = s_glut + (threadIdx.x % kLutCopies) * kGlutStride;
   for (int i = threadIdx.x; i < kCacheSlots; i += kDvBlock) {
     s_key[i] = kEmptyKey;
     s_val[i] = 0.0f;
@@ -503,7 +512,7 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
       const float da = dtau * (-c.expo) * tau * __builtin_amdgcn_rcpf(oma);
       const float dr = w * gx, dg = w * gy, db = w * gz;
       // LUT: entries i and i+1 with weights (1-f) and f
-      lut_enter(s_glut, L, cl.i);
+      lut_enter(my_glut, L, cl.i);
       const float wf0 = 1.0f - cl.f, wf1 = cl.f;
       L.lo[0] += wf0 * dr;
       L.lo[1] += wf0 * dg;
@@ -536,14 +545,19 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
   }
   flush_all(c, vc, grad_density, A);
   if (L.i >= 0) {
-    lut_put(s_glut, L.i, L.lo);
-    lut_put(s_glut, L.i + 1, L.hi);
+    lut_put(my_glut, L.i, L.lo);
+    lut_put(my_glut, L.i + 1, L.hi);
   }
   __syncthreads();
   drain(vc, grad_density);
   float *dst = lut_partials + (size_t)blockIdx.x * 1024;
-  for (int i = threadIdx.x; i < 1024; i += kDvBlock)
-    gst(dst, i, s_glut[i]);
+  for (int i = threadIdx.x; i < 1024; i += kDvBlock) {
+    float g = 0.0f;
+#pragma unroll
+    for (int cp = 0; cp < kLutCopies; cp++)
+      g += s_glut[cp * kGlutStride + i];
+    gst(dst, i, g);
+  }
 }
 
 // LUT gradient: the per-workgroup partials are summed in two fixed-order
