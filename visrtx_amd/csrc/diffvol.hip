@@ -292,29 +292,25 @@ template <int AXIS>
 __device__ __forceinline__ void shift_cell(
     const DvCtx &c, const VoxCache &cache, float *gd, CornerAcc &A, int d)
 {
+  if (d == 0)
+    return;
+  // branch-free in the sign of d, so lanes moving either way share one path
   constexpr int bit = 1 << AXIS;
   const uint32_t stride = AXIS == 0 ? 1u : (AXIS == 1 ? c.sy : c.sz);
+  const bool up = d > 0;
   uint32_t off[8];
   corner_offsets(c, off);
-  if (d > 0) {
 #pragma unroll
-    for (int q = 0; q < 8; q++)
-      if ((q & bit) == 0) {
-        put(cache, gd, A.base + off[q], A.a[q]);
-        A.a[q] = A.a[q | bit];
-        A.a[q | bit] = 0.0f;
-      }
-    A.base += stride;
-  } else if (d < 0) {
-#pragma unroll
-    for (int q = 0; q < 8; q++)
-      if ((q & bit) != 0) {
-        put(cache, gd, A.base + off[q], A.a[q]);
-        A.a[q] = A.a[q ^ bit];
-        A.a[q ^ bit] = 0.0f;
-      }
-    A.base -= stride;
+  for (int q0 = 0; q0 < 8; q0++) {
+    if (q0 & bit)
+      continue;
+    const int q1 = q0 | bit;
+    put(cache, gd, A.base + (up ? off[q0] : off[q1]), up ? A.a[q0] : A.a[q1]);
+    const float keep = up ? A.a[q1] : A.a[q0];
+    A.a[q0] = up ? keep : 0.0f;
+    A.a[q1] = up ? 0.0f : keep;
   }
+  A.base = up ? A.base + stride : A.base - stride;
 }
 
 __device__ __forceinline__ void flush_all(
@@ -356,7 +352,7 @@ __device__ __forceinline__ void enter_cell(
 // so they hit the same entries, and the copies cut those same-address
 // conflicts.  The copy stride is padded by one float so that copies of one
 // entry fall in different banks.
-constexpr int kLutCopies = 4;
+constexpr int kLutCopies = 2;
 constexpr int kGlutStride = 1025;
 
 struct LutAcc {
@@ -366,38 +362,34 @@ struct LutAcc {
 
 __device__ __forceinline__ void lut_put(float *s_glut, int e, const float g[4])
 {
+  if (g[0] != 0.0f || g[1] != 0.0f || g[2] != 0.0f || g[3] != 0.0f) {
 #pragma unroll
-  for (int ch = 0; ch < 4; ch++)
-    if (g[ch] != 0.0f)
+    for (int ch = 0; ch < 4; ch++)
       atomicAdd(s_glut + 4 * e + ch, g[ch]);
+  }
 }
 
 __device__ __forceinline__ void lut_enter(float *s_glut, LutAcc &L, int i)
 {
   if (i == L.i)
     return;
-  if (L.i >= 0 && i == L.i + 1) {
-    lut_put(s_glut, L.i, L.lo);
+  const bool up = L.i >= 0 && i == L.i + 1;   // entry L.i+1 is kept as the new lo
+  const bool down = L.i >= 0 && i == L.i - 1; // entry L.i is kept as the new hi
+  if (L.i >= 0) {
+    float flo[4], fhi[4];
 #pragma unroll
     for (int ch = 0; ch < 4; ch++) {
-      L.lo[ch] = L.hi[ch];
-      L.hi[ch] = 0.0f;
+      flo[ch] = down ? 0.0f : L.lo[ch];
+      fhi[ch] = up ? 0.0f : L.hi[ch];
     }
-  } else if (L.i >= 0 && i == L.i - 1) {
-    lut_put(s_glut, L.i + 1, L.hi);
+    lut_put(s_glut, L.i, flo);
+    lut_put(s_glut, L.i + 1, fhi);
+  }
 #pragma unroll
-    for (int ch = 0; ch < 4; ch++) {
-      L.hi[ch] = L.lo[ch];
-      L.lo[ch] = 0.0f;
-    }
-  } else {
-    if (L.i >= 0) {
-      lut_put(s_glut, L.i, L.lo);
-      lut_put(s_glut, L.i + 1, L.hi);
-    }
-#pragma unroll
-    for (int ch = 0; ch < 4; ch++)
-      L.lo[ch] = L.hi[ch] = 0.0f;
+  for (int ch = 0; ch < 4; ch++) {
+    const float nlo = up ? L.hi[ch] : 0.0f, nhi = down ? L.lo[ch] : 0.0f;
+    L.lo[ch] = nlo;
+    L.hi[ch] = nhi;
   }
   L.i = i;
 }
