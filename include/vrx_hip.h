@@ -99,6 +99,7 @@ typedef struct vrx_bvh {
   void *d_leaves;         /* n * leaf record (48/16/32/0 B for tri/sphere/cyl/box) */
   uint32_t *d_leaf_ids;   /* n: sorted leaf -> original primitive index */
   void *d_header;         /* 64 B: root box (6 f32), height, root, n, type */
+  void *d_wide;           /* optional (n-1) * 128 B 4-wide nodes (vrx_bvh_widen), or NULL */
 } vrx_bvh;
 
 typedef struct vrx_bvh_sizes {
@@ -106,9 +107,16 @@ typedef struct vrx_bvh_sizes {
 } vrx_bvh_sizes;
 
 int vrx_bvh_query_sizes(int32_t type, uint32_t n, vrx_bvh_sizes *out);
-/* Morton -> LDS-staged stable radix sort -> Karras -> atomic-flag refit. */
+/* Morton -> LDS-staged stable radix sort -> Karras -> refit by common-prefix
+ * level (no atomics). */
 int vrx_bvh_build(const vrx_prims *prims, const vrx_bvh *out, void *d_workspace,
     size_t workspace_bytes, vrx_stream s);
+/* 4-wide nodes from a built BVH2 (no reference counterpart: OptiX picks its
+ * own node format).  Record i holds the up-to-4 grandchildren of BVH2 node i
+ * (a leaf child stays one slot), so the 4-wide tree shares the BVH2's leaves
+ * and root.  Writes bvh->d_wide, which must hold vrx_bvh_wide_bytes(n). */
+size_t vrx_bvh_wide_bytes(uint32_t n);
+int vrx_bvh_widen(const vrx_bvh *bvh, uint32_t n, vrx_stream s);
 /* Debug/parity hooks: read back the sorted Morton codes (n) and the order. */
 int vrx_bvh_build_debug(const vrx_prims *prims, const vrx_bvh *out, void *d_workspace,
     size_t workspace_bytes, uint32_t *d_codes_out, float *d_cbox_out, vrx_stream s);
@@ -178,6 +186,8 @@ typedef struct vrx_scene {
   uint32_t num_volumes;
   int32_t single_type;          /* prim type if every geom shares it, else -1 */
   uint64_t max_volume_voxels;   /* largest field's voxel count (>= 2^31 selects 64-bit gathers) */
+  uint32_t wide;                /* 1 = every BLAS (and the TLAS if any) has d_wide */
+  uint32_t pad;
 } vrx_scene;
 
 typedef struct vrx_camera {
@@ -205,6 +215,7 @@ typedef struct vrx_renderer {
   int32_t stack_lds;      /* LDS short-stack entries per lane: 8/16/32/64 */
   int32_t block_pixels;   /* pixels per workgroup: 64/128/256 */
   int32_t vol_unroll;     /* volume samples gathered per step: 1/2/4/8 */
+  int32_t node_width;     /* BVH node width traversed: 2 (default) or 4 (needs scene->wide) */
 } vrx_renderer;
 
 typedef struct vrx_frame {
@@ -302,8 +313,9 @@ int vrx_batch_graph_create(const vrx_scene *scene, const vrx_renderer *r, const 
     vrx_stream s, vrx_graph *out);
 int vrx_graph_launch(vrx_graph g, vrx_stream s);
 int vrx_graph_destroy(vrx_graph g);
-/* Times tile shape, workgroup size and LDS stack depth on this scene and
- * writes the fastest into r->stack_lds and b->tile_w/tile_h/block_threads. */
+/* Times tile shape, workgroup size, LDS stack depth and BVH node width (4 only
+ * when scene->wide) on this scene and writes the fastest into r->stack_lds,
+ * r->node_width and b->tile_w/tile_h/block_threads. */
 int vrx_batch_autotune(const vrx_scene *scene, vrx_renderer *r, vrx_batch *b, vrx_stream s,
     float *best_ms_out);
 

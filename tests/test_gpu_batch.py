@@ -131,3 +131,28 @@ def test_batched_rejects_bad_tiles_and_small_scratch(require_gpu):
     assert b"scratch" in L.vrx_last_error()
     s.release()
     dev.close()
+
+
+def test_batched_wide_nodes_equal_per_view(require_gpu):
+    """Node width 4 in the persistent launch: same channels as the BVH2
+    per-view frames; the autotuner may pick either width."""
+    w, h = 64, 48
+    sc, cams = _small_batched(4, (w, h))
+    B, dev, s, r = _setup(sc)
+    scene = B.scene_of(dev, s.world, wide=True)
+    assert scene.wide == 1
+    bf = B.BatchFrame(scene, B.renderer_of(dev, r), cams, w, h)
+    bf.renderer.node_width = 4
+    bf.render()
+    torch.cuda.synchronize()
+    got = bf.channels()
+    for v, cam in enumerate(cams):
+        ref = _per_view(dev, s, r, cam, w, h)
+        _assert_same({k: got[k][v] for k in CHANNELS}, ref, f"view {v} width 4")
+    bf.autotune()
+    assert bf.renderer.node_width in (2, 4)
+    bf.render()
+    torch.cuda.synchronize()
+    _assert_same(bf.channels(), got, "autotuned")
+    s.release()
+    dev.close()

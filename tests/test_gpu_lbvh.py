@@ -141,3 +141,47 @@ def test_10m_spheres_bit_exact(require_gpu):
     g = gpu_build(N.VRX_SPHERE, c, radius=r)
     o = oracle_build(orc.ORC_SPHERE, c, radius=r)
     check_equal(g, o, 10_000_000)
+
+
+def test_wide_nodes_match_bvh2_collapse(require_gpu):
+    """vrx_bvh_widen: record i holds node i's grandchildren (a leaf child keeps
+    its slot), boxes copied bit for bit from the BVH2 nodes."""
+    from tests.devmem import DeviceBuffer as DB
+    c, r = S.gen_spheres(5000, seed=12)
+    L = N.hip()
+    n = len(c)
+    sz = N.vrx_bvh_sizes()
+    N.check(L.vrx_bvh_query_sizes(N.VRX_SPHERE, C.c_uint32(n), C.byref(sz)))
+    dpos, drad = DB.from_array(c), DB.from_array(r)
+    nodes, leaves, ids = DB(sz.nodes), DB(sz.leaves), DB(sz.leaf_ids)
+    header, ws = DB(sz.header), DB(sz.workspace)
+    wide = DB(L.vrx_bvh_wide_bytes(n))
+    p = N.vrx_prims(N.VRX_SPHERE, n, dpos.addr, None, drad.addr, 0.0)
+    b = N.vrx_bvh(nodes.addr, leaves.addr, ids.addr, header.addr, wide.addr)
+    N.check(L.vrx_bvh_build(C.byref(p), C.byref(b), ws.addr, C.c_size_t(sz.workspace), None))
+    N.check(L.vrx_bvh_widen(C.byref(b), n, None), "vrx_bvh_widen")
+    N.check(L.vrx_device_sync())
+    raw = nodes.to_array(np.uint32, 16 * (n - 1)).reshape(-1, 16)
+    w = wide.to_array(np.uint32, 32 * (n - 1)).reshape(-1, 32)
+    boxes = raw[:, 0:12]
+    child = raw[:, 12:14]
+
+    def kids(i):
+        out = []
+        for j in range(2):
+            bx = boxes[i, 6 * j:6 * j + 6]
+            lk = int(child[i, j])
+            if lk & LEAF:
+                out.append((bx, lk))
+            else:
+                for g in range(2):
+                    out.append((boxes[lk, 6 * g:6 * g + 6], int(child[lk, g])))
+        return out
+
+    for i in range(n - 1):
+        k = kids(i)
+        assert w[i, 28] == len(k)
+        for s_, (bx, lk) in enumerate(k):
+            got = [w[i, 4 * comp + s_] for comp in range(6)]
+            np.testing.assert_array_equal(np.array(got, np.uint32), bx, err_msg=f"node {i}")
+            assert w[i, 24 + s_] == lk

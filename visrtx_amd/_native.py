@@ -58,7 +58,7 @@ class vrx_prims(C.Structure):
 
 class vrx_bvh(C.Structure):
     _fields_ = [("d_nodes", C.c_void_p), ("d_leaves", C.c_void_p), ("d_leaf_ids", C.c_void_p),
-                ("d_header", C.c_void_p)]
+                ("d_header", C.c_void_p), ("d_wide", C.c_void_p)]
 
 
 class vrx_bvh_sizes(C.Structure):
@@ -80,7 +80,7 @@ class vrx_scene(C.Structure):
                 ("num_entries", C.c_uint32), ("tlas", vrx_bvh), ("tlas_height", C.c_uint32),
                 ("max_blas_height", C.c_uint32), ("d_volumes", C.c_void_p),
                 ("num_volumes", C.c_uint32), ("single_type", C.c_int32),
-                ("max_volume_voxels", C.c_uint64)]
+                ("max_volume_voxels", C.c_uint64), ("wide", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class vrx_renderer(C.Structure):
@@ -88,7 +88,8 @@ class vrx_renderer(C.Structure):
                 ("ambient_radiance", C.c_float), ("pixel_samples", C.c_int32),
                 ("ambient_samples", C.c_int32), ("ao_distance", C.c_float),
                 ("cull_backfaces", C.c_int32), ("inv_volume_sampling_rate", C.c_float),
-                ("stack_lds", C.c_int32), ("block_pixels", C.c_int32), ("vol_unroll", C.c_int32)]
+                ("stack_lds", C.c_int32), ("block_pixels", C.c_int32), ("vol_unroll", C.c_int32),
+                ("node_width", C.c_int32)]
 
 
 class vrx_batch(C.Structure):
@@ -150,6 +151,8 @@ def hip() -> C.CDLL:
             "vrx_bvh_query_sizes": (i32, [i32, u32, vp]),
             "vrx_bvh_build": (i32, [vp, vp, vp, sz, vp]),
             "vrx_bvh_build_debug": (i32, [vp, vp, vp, sz, vp, vp, vp]),
+            "vrx_bvh_wide_bytes": (sz, [u32]),
+            "vrx_bvh_widen": (i32, [vp, u32, vp]),
             "vrx_render_scratch_bytes": (sz, [vp, vp, u32, u32]),
             "vrx_frame_clear": (i32, [vp, vp]),
             "vrx_render": (i32, [vp, vp, vp, vp, vp]),

@@ -19,9 +19,10 @@ from . import _native as N
 from . import anari as A
 
 
-def scene_of(dev: A.Device, world) -> N.vrx_scene:
-    """Copy of the world's device-side scene description (rebuilds if needed)."""
-    p = dev.get_property(world, "vrxScene", A.VOID_POINTER)
+def scene_of(dev: A.Device, world, wide: bool = False) -> N.vrx_scene:
+    """Copy of the world's device-side scene description (rebuilds if needed);
+    ``wide`` also builds the 4-wide BVH nodes so node width 4 can be used."""
+    p = dev.get_property(world, "vrxSceneWide" if wide else "vrxScene", A.VOID_POINTER)
     if not p:
         raise N.VrxError("world has no scene (vrxScene property unavailable)")
     return N.vrx_scene.from_buffer_copy(C.string_at(p, C.sizeof(N.vrx_scene)))
@@ -104,8 +105,9 @@ class BatchFrame:
                 "vrx_render_batch")
 
     def autotune(self, stream=None) -> float:
-        """Times tile shape x workgroup size x LDS stack depth on this scene and
-        keeps the fastest; returns its time per launch in ms."""
+        """Times tile shape x workgroup size x LDS stack depth x BVH node width
+        (4 only when the scene has 4-wide nodes) on this scene and keeps the
+        fastest; returns its time per launch in ms."""
         ms = C.c_float()
         N.check(N.hip().vrx_batch_autotune(C.byref(self.scene), C.byref(self.renderer),
                                            C.byref(self.b), self._stream(stream), C.byref(ms)),

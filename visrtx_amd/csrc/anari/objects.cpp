@@ -318,8 +318,11 @@ void Geometry::freeBVH()
   vrx_free(dLeaves);
   vrx_free(dIds);
   vrx_free(dHeader);
-  dNodes = dLeaves = dIds = dHeader = nullptr;
-  capNodes = capLeaves = capIds = 0;
+  vrx_free(dWide);
+  dNodes = dLeaves = dIds = dHeader = dWide = nullptr;
+  capNodes = capLeaves = capIds = capWide = 0;
+  wideValid = false;
+  bvh.d_wide = nullptr;
 }
 
 static Array *arrayParam(const Object *o, const char *name)
@@ -469,9 +472,31 @@ int Geometry::buildBVH(vrx_stream s)
   void *ws = dev->workspace(sz.workspace);
   if (!ws)
     return -1;
+  wideValid = false;
+  bvh.d_wide = nullptr;
   if (!dev->check(vrx_bvh_build(&p, &bvh, ws, sz.workspace, s), this, "BVH build"))
     return -1;
   bvhDirty = false;
+  return 0;
+}
+
+int Geometry::buildWide(vrx_stream s)
+{
+  if (wideValid)
+    return 0;
+  const size_t need = vrx_bvh_wide_bytes(n);
+  if (need > capWide) {
+    vrx_free(dWide);
+    dWide = nullptr;
+    capWide = 0;
+    if (!dev->check(vrx_malloc(&dWide, need), this, "4-wide node allocation"))
+      return -1;
+    capWide = need;
+  }
+  bvh.d_wide = dWide;
+  if (!dev->check(vrx_bvh_widen(&bvh, n, s), this, "4-wide nodes"))
+    return -1;
+  wideValid = true;
   return 0;
 }
 
@@ -765,7 +790,9 @@ void World::freeScene()
   vrx_free(dTlasLeaves);
   vrx_free(dTlasIds);
   vrx_free(dTlasHeader);
+  vrx_free(dTlasWide);
   dGeoms = dEntries = dVolumes = dTlasNodes = dTlasLeaves = dTlasIds = dTlasHeader = nullptr;
+  dTlasWide = nullptr;
 }
 
 void World::commit()
@@ -896,6 +923,7 @@ int World::rebuild()
   // vrx_geom per surface occurrence (deduplicated by surface)
   std::vector<vrx_geom> hg;
   std::vector<Surface *> gsurf;
+  std::vector<Geometry *> geomOf;
   std::vector<vrx_entry> he;
   uint32_t maxH = 0;
   int single = -2;
@@ -909,6 +937,7 @@ int World::rebuild()
       vg.type = g->ptype;
       vg.n = g->n;
       vg.bvh = g->bvh;
+      vg.bvh.d_wide = g->wideValid ? g->dWide : nullptr;
       vg.d_position = (const float *)g->pos->device();
       vg.d_index = g->index ? (const uint32_t *)g->index->device() : nullptr;
       vg.d_radius = g->radius ? (const float *)g->radius->device() : nullptr;
@@ -926,6 +955,7 @@ int World::rebuild()
       gi = (uint32_t)hg.size();
       hg.push_back(vg);
       gsurf.push_back(o.surf);
+      geomOf.push_back(g);
       maxH = std::max(maxH, g->header.height);
       single = single == -2 ? g->ptype : (single == g->ptype ? single : -1);
     } else {
@@ -1032,8 +1062,50 @@ int World::rebuild()
   vrx_event_destroy(e0);
   vrx_event_destroy(e1);
   memcpy(bounds, wb, sizeof(bounds));
+  hostGeoms = std::move(hg);
+  hostGeomOf = std::move(geomOf);
   sceneDirty = false;
   builtAt = dev->tick();
+  return 0;
+}
+
+int World::ensureWide()
+{
+  if (scene.wide || scene.num_entries == 0)
+    return 0;
+  vrx_stream s = dev->stream;
+  vrx_event e0 = nullptr, e1 = nullptr;
+  vrx_event_create(&e0);
+  vrx_event_create(&e1);
+  vrx_event_record(e0, s);
+  for (size_t i = 0; i < hostGeoms.size(); i++) {
+    Geometry *g = hostGeomOf[i];
+    if (g->buildWide(s) != 0)
+      return -1;
+    hostGeoms[i].bvh.d_wide = g->dWide;
+  }
+  if (scene.num_entries > 1) {
+    if (!dTlasWide
+        && !dev->check(vrx_malloc(&dTlasWide, vrx_bvh_wide_bytes(scene.num_entries)), this,
+            "4-wide instance nodes"))
+      return -1;
+    scene.tlas.d_wide = dTlasWide;
+    if (!dev->check(vrx_bvh_widen(&scene.tlas, scene.num_entries, s), this, "4-wide instance nodes"))
+      return -1;
+  }
+  vrx_event_record(e1, s);
+  if (!hostGeoms.empty()
+      && !dev->check(vrx_memcpy(dGeoms, hostGeoms.data(), hostGeoms.size() * sizeof(vrx_geom),
+                         VRX_H2D, s),
+          this, "scene upload"))
+    return -1;
+  dev->check(vrx_stream_sync(s), this, "stream sync");
+  float ms = 0.0f;
+  vrx_event_elapsed_ms(e0, e1, &ms);
+  lastWideMs = ms;
+  vrx_event_destroy(e0);
+  vrx_event_destroy(e1);
+  scene.wide = 1;
   return 0;
 }
 
@@ -1056,9 +1128,18 @@ bool World::getProperty(
   // extension: the device-side scene description (vrx_scene of vrx_hip.h),
   // valid until the world is next rebuilt; lets a caller drive the batched
   // multi-view entry points on a scene built through this API
-  if (name == "vrxScene" && t == ANARI_VOID_POINTER && size >= sizeof(void *)) {
+  if (name == "bvhWidenDuration" && t == ANARI_FLOAT32 && size >= 4) {
+    float sec = lastWideMs / 1000.0f;
+    memcpy(mem, &sec, 4);
+    return true;
+  }
+  // "vrxSceneWide" also builds the 4-wide nodes (node width 4)
+  if ((name == "vrxScene" || name == "vrxSceneWide") && t == ANARI_VOID_POINTER
+      && size >= sizeof(void *)) {
     dev->flushCommits();
     if (needsRebuild() && rebuild() != 0)
+      return false;
+    if (name == "vrxSceneWide" && ensureWide() != 0)
       return false;
     const void *p = &scene;
     memcpy(mem, &p, sizeof(p));
@@ -1116,6 +1197,7 @@ void Renderer::commit()
   r.inv_volume_sampling_rate = 1.0f / rate;
   r.stack_lds = get<int>("stackDepthLDS", ANARI_INT32, 0);
   r.vol_unroll = get<int>("volumeUnroll", ANARI_INT32, 0);
+  r.node_width = get<int>("nodeWidth", ANARI_INT32, 2) == 4 ? 4 : 2;
   sampleLimit = get<int>("sampleLimit", ANARI_INT32, 128);
   if (kind == VRX_RENDER_RAYCAST)
     sampleLimit = 1; // single-shot renderer
@@ -1242,6 +1324,8 @@ void Frame::render()
   if (!allocated)
     allocate();
   if (!allocated)
+    return;
+  if (renderer->r.node_width == 4 && world->ensureWide() != 0)
     return;
   // Frame.cu:574-588 accumulation reset on any finalize since the last frame
   if (dev->lastFinalize > lastSeenFinalize) {

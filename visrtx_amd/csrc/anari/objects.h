@@ -130,7 +130,12 @@ struct Geometry : Object {
   vrx_bvh_header header{};
   void *dNodes{nullptr}, *dLeaves{nullptr}, *dIds{nullptr}, *dHeader{nullptr};
   size_t capNodes{0}, capLeaves{0}, capIds{0};
+  // 4-wide nodes of the current BVH, built on demand (node width 4)
+  void *dWide{nullptr};
+  size_t capWide{0};
+  bool wideValid{false};
   int buildBVH(vrx_stream s);
+  int buildWide(vrx_stream s);
   void freeBVH();
 };
 
@@ -212,9 +217,14 @@ struct World : Object {
   vrx_scene scene{};
   void *dGeoms{nullptr}, *dEntries{nullptr}, *dVolumes{nullptr};
   void *dTlasNodes{nullptr}, *dTlasLeaves{nullptr}, *dTlasIds{nullptr}, *dTlasHeader{nullptr};
+  void *dTlasWide{nullptr};
   float bounds[6]{0, 0, 0, 0, 0, 0};
-  float lastBuildMs{0.0f};
+  float lastBuildMs{0.0f}, lastWideMs{0.0f};
+  // host copies of the scene tables (for re-upload when 4-wide nodes appear)
+  std::vector<vrx_geom> hostGeoms;
+  std::vector<Geometry *> hostGeomOf;
   int rebuild();
+  int ensureWide(); // 4-wide nodes for every BLAS and the TLAS; sets scene.wide
   void freeScene();
   bool needsRebuild() const;
 };

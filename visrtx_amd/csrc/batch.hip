@@ -32,7 +32,7 @@ __device__ __forceinline__ vrx_camera load_camera(const vrx_camera *cams, uint32
   return c;
 }
 
-template <int PT, bool TWO>
+template <int PT, bool TWO, bool WIDE>
 __global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, vrx_renderer r,
     vrx_batch b, uint32_t sd, uint32_t cap, uint32_t *scratch, uint32_t tiles_x, uint32_t tiles_y)
 {
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, v
     if (px < b.width && py < b.height) {
       const vrx_camera cam = load_camera(b.d_cams, view);
       const size_t idx = (size_t)view * b.width * b.height + (size_t)py * b.width + px;
-      render_pixel<PT, TWO, VRX_RENDER_RAYCAST>(sc, cam, r, f, px, py, idx, st, rays, vsamples);
+      render_pixel<PT, TWO, VRX_RENDER_RAYCAST, WIDE>(sc, cam, r, f, px, py, idx, st, rays, vsamples);
     }
   }
 }
@@ -110,33 +110,42 @@ static uint32_t pick_sd(int want)
   return p;
 }
 
-static uint32_t cap_for(const vrx_scene &sc, uint32_t sd)
+static uint32_t cap_for(const vrx_scene &sc, uint32_t sd, bool wide)
 {
-  uint32_t m = sc.max_blas_height + 2;
+  // as the per-view renderer: one push per BVH2 level, up to three per
+  // 4-wide level (which spans two BVH2 levels)
+  uint32_t m = wide ? 3 * ((sc.max_blas_height + 1) / 2) + 2 : sc.max_blas_height + 2;
   if (sc.num_entries > 1)
-    m += sc.tlas_height + 1;
+    m += wide ? 3 * ((sc.tlas_height + 1) / 2) + 1 : sc.tlas_height + 1;
   return m > sd ? m - sd : 0u;
 }
 
-template <int PT, bool TWO>
-static const void *kernel_ptr()
+template <int PT, bool WIDE>
+static const void *kernel_ptr(bool two)
 {
-  return (const void *)&k_render_batch<PT, TWO>;
+  return two ? (const void *)&k_render_batch<PT, true, WIDE>
+             : (const void *)&k_render_batch<PT, false, WIDE>;
 }
 
-static const void *pick_kernel(const vrx_scene &sc)
+template <bool WIDE>
+static const void *pick_kernel_w(const vrx_scene &sc)
 {
   const bool two = sc.num_entries > 1;
   switch (sc.single_type) {
   case VRX_TRIANGLE:
-    return two ? kernel_ptr<VRX_TRIANGLE, true>() : kernel_ptr<VRX_TRIANGLE, false>();
+    return kernel_ptr<VRX_TRIANGLE, WIDE>(two);
   case VRX_SPHERE:
-    return two ? kernel_ptr<VRX_SPHERE, true>() : kernel_ptr<VRX_SPHERE, false>();
+    return kernel_ptr<VRX_SPHERE, WIDE>(two);
   case VRX_CYLINDER:
-    return two ? kernel_ptr<VRX_CYLINDER, true>() : kernel_ptr<VRX_CYLINDER, false>();
+    return kernel_ptr<VRX_CYLINDER, WIDE>(two);
   default:
-    return two ? kernel_ptr<-1, true>() : kernel_ptr<-1, false>();
+    return kernel_ptr<-1, WIDE>(two);
   }
+}
+
+static const void *pick_kernel(const vrx_scene &sc, bool wide)
+{
+  return wide ? pick_kernel_w<true>(sc) : pick_kernel_w<false>(sc);
 }
 
 __global__ void k_reset_counter(uint32_t *c)
@@ -148,10 +157,11 @@ __global__ void k_reset_counter(uint32_t *c)
 static int launch_batch(const vrx_scene &sc, const vrx_renderer &r, const vrx_batch &b, hipStream_t s)
 {
   const int block = std::min(std::max(b.block_threads > 0 ? b.block_threads : 256, 64), 256) & ~63;
+  const bool wide = r.node_width == 4;
   const uint32_t sd = pick_sd(r.stack_lds);
-  const uint32_t cap = cap_for(sc, sd);
+  const uint32_t cap = cap_for(sc, sd, wide);
   const size_t lds = (size_t)sd * block * 4;
-  const void *kern = pick_kernel(sc);
+  const void *kern = pick_kernel(sc, wide);
   int per_cu = 0;
   VRX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds));
   if (per_cu < 1)
@@ -178,6 +188,9 @@ static int validate(const vrx_scene *scene, const vrx_renderer *r, const vrx_bat
   VRX_ARG(scene->num_entries == 0 || (scene->d_entries && scene->d_geoms), "scene arrays");
   VRX_ARG(scene->num_entries <= 1 || scene->tlas.d_leaf_ids, "instance BVH not built");
   VRX_ARG(scene->num_volumes == 0 || scene->d_volumes, "volume array");
+  VRX_ARG(r->node_width == 0 || r->node_width == 2 || r->node_width == 4, "node width 2 or 4");
+  VRX_ARG(r->node_width != 4 || scene->num_entries == 0 || scene->wide,
+      "node width 4 needs 4-wide nodes for every BVH (scene->wide)");
   size_t need = vrx_batch_scratch_bytes(scene, r, b);
   if (!b->d_scratch || b->scratch_bytes < need) {
     set_error("batch scratch too small: %zu < %zu", b->scratch_bytes, need);
@@ -197,10 +210,12 @@ size_t vrx_batch_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r, co
   (void)b;
   if (!scene)
     return 0;
-  // sized for the smallest LDS stack the autotuner tries (16) and a grid of
+  // sized for the smallest LDS stack the autotuner tries (16), the deeper
+  // stack of 4-wide traversal when the scene has 4-wide nodes, and a grid of
   // at most 2048 resident lanes per CU
   const uint32_t sd = std::min(pick_sd(r ? r->stack_lds : 0), 16u);
-  const uint32_t cap = cap_for(*scene, sd);
+  const bool wide = scene->wide || (r && r->node_width == 4);
+  const uint32_t cap = std::max(cap_for(*scene, sd, false), wide ? cap_for(*scene, sd, true) : 0u);
   return 256 + (size_t)num_cus() * kMaxThreadsPerCU * cap * 4;
 }
 
@@ -263,14 +278,19 @@ int vrx_batch_autotune(const vrx_scene *scene, vrx_renderer *r, vrx_batch *b, vr
     return st;
   hipStream_t hs = (hipStream_t)s;
   struct Cand {
-    int tw, th, block, sd;
+    int tw, th, block, sd, width;
   };
   std::vector<Cand> cands;
   const int tiles[][2] = {{8, 8}, {16, 4}, {32, 2}, {64, 1}};
-  for (auto &t : tiles)
-    for (int block : {64, 128, 256})
-      for (int sd : {16, 32})
-        cands.push_back({t[0], t[1], block, sd});
+  const bool can_wide = scene->wide || scene->num_entries == 0;
+  for (int width : {2, 4}) {
+    if (width == 4 && !can_wide)
+      continue;
+    for (auto &t : tiles)
+      for (int block : {64, 128, 256})
+        for (int sd : {16, 32})
+          cands.push_back({t[0], t[1], block, sd, width});
+  }
   hipEvent_t e0, e1;
   VRX_HIP(hipEventCreate(&e0));
   VRX_HIP(hipEventCreate(&e1));
@@ -281,6 +301,7 @@ int vrx_batch_autotune(const vrx_scene *scene, vrx_renderer *r, vrx_batch *b, vr
   bb.persistent_blocks = 0;
   for (const Cand &c : cands) {
     rr.stack_lds = c.sd;
+    rr.node_width = c.width;
     bb.tile_w = c.tw;
     bb.tile_h = c.th;
     bb.block_threads = c.block;
@@ -303,6 +324,7 @@ int vrx_batch_autotune(const vrx_scene *scene, vrx_renderer *r, vrx_batch *b, vr
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   r->stack_lds = bestc.sd;
+  r->node_width = bestc.width;
   b->tile_w = bestc.tw;
   b->tile_h = bestc.th;
   b->block_threads = bestc.block;

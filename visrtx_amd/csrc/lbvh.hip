@@ -655,6 +655,70 @@ __global__ void k_root_header(vrx_prims p, const void *leaves, const uint32_t *i
   write_header(header, lo, hi, 1u + max(C.d.z, C.d.w), 0u, p.n, T);
 }
 
+// 4-wide record i from BVH2 node i: each child that is an internal node is
+// replaced by its two children (boxes taken from that node), a leaf child
+// keeps its slot.  Every BVH2 node gets a record; traversal from the root
+// only reaches those two BVH2 levels apart.
+__device__ __forceinline__ void node_child(const Node &nd, int j, float b[6], uint32_t &link)
+{
+  if (j == 0) {
+    b[0] = nd.a.x; b[1] = nd.a.y; b[2] = nd.a.z; b[3] = nd.a.w; b[4] = nd.b.x; b[5] = nd.b.y;
+    link = nd.d.x;
+  } else {
+    b[0] = nd.b.z; b[1] = nd.b.w; b[2] = nd.c.x; b[3] = nd.c.y; b[4] = nd.c.z; b[5] = nd.c.w;
+    link = nd.d.y;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_widen(const Node *nodes, uint32_t ninner, Node4 *wide)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninner)
+    return;
+  const Node nd = gld_node(nodes, i);
+  float bx[4][6];
+  uint32_t lk[4];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    float b[6];
+    uint32_t link;
+    node_child(nd, j, b, link);
+    if (link & kLeafBit) {
+#pragma unroll
+      for (int q = 0; q < 6; q++)
+        bx[cnt][q] = b[q];
+      lk[cnt] = link;
+      cnt++;
+    } else {
+      const Node cn = gld_node(nodes, link);
+#pragma unroll
+      for (int g = 0; g < 2; g++) {
+        uint32_t gl;
+        node_child(cn, g, bx[cnt], gl);
+        lk[cnt] = gl;
+        cnt++;
+      }
+    }
+  }
+  for (uint32_t c = cnt; c < 4; c++) {
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+      bx[c][q] = 0.0f;
+    lk[c] = kLeafBit;
+  }
+  float4 *w = (float4 *)(wide + i);
+  gst(w, 0, make_float4(bx[0][0], bx[1][0], bx[2][0], bx[3][0]));
+  gst(w, 1, make_float4(bx[0][1], bx[1][1], bx[2][1], bx[3][1]));
+  gst(w, 2, make_float4(bx[0][2], bx[1][2], bx[2][2], bx[3][2]));
+  gst(w, 3, make_float4(bx[0][3], bx[1][3], bx[2][3], bx[3][3]));
+  gst(w, 4, make_float4(bx[0][4], bx[1][4], bx[2][4], bx[3][4]));
+  gst(w, 5, make_float4(bx[0][5], bx[1][5], bx[2][5], bx[3][5]));
+  gst(w, 6, make_float4(__uint_as_float(lk[0]), __uint_as_float(lk[1]), __uint_as_float(lk[2]),
+                __uint_as_float(lk[3])));
+  gst(w, 7, make_float4(__uint_as_float(cnt), 0.0f, 0.0f, 0.0f));
+}
+
 __global__ void k_empty_header(vrx_bvh_header *h, int type)
 {
   write_header(h, mk(INFINITY, INFINITY, INFINITY), mk(-INFINITY, -INFINITY, -INFINITY), 0u,
@@ -795,6 +859,24 @@ int vrx_bvh_build_debug(const vrx_prims *prims, const vrx_bvh *out, void *d_work
 {
   return bvh_build_impl(
       prims, out, d_workspace, workspace_bytes, d_codes_out, d_cbox_out, (hipStream_t)s);
+}
+
+size_t vrx_bvh_wide_bytes(uint32_t n)
+{
+  return n > 1 ? (size_t)(n - 1) * sizeof(Node4) : 0;
+}
+
+int vrx_bvh_widen(const vrx_bvh *bvh, uint32_t n, vrx_stream s)
+{
+  VRX_ARG(bvh, "bvh");
+  if (n < 2)
+    return VRX_OK; // a single leaf (or nothing): no inner nodes
+  VRX_ARG(bvh->d_nodes && bvh->d_wide, "bvh nodes / wide buffer");
+  const uint32_t ninner = n - 1;
+  hipLaunchKernelGGL(k_widen, dim3((ninner + 255) / 256), dim3(256), 0, (hipStream_t)s,
+      (const Node *)bvh->d_nodes, ninner, (Node4 *)bvh->d_wide);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
 }
 
 } // extern "C"

@@ -24,7 +24,7 @@
 
 namespace vrx {
 
-template <int PT, bool TWO, int KIND>
+template <int PT, bool TWO, int KIND, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam, vrx_renderer r,
     vrx_frame f, uint32_t sd, uint32_t ovf_cap, uint32_t *scratch)
 {
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam,
 
   uint32_t rays = 0, vsamples = 0;
   if (inside)
-    render_pixel<PT, TWO, KIND>(sc, cam, r, f, px, py, (size_t)px + (size_t)py * f.width, st, rays,
+    render_pixel<PT, TWO, KIND, WIDE>(sc, cam, r, f, px, py, (size_t)px + (size_t)py * f.width, st, rays,
         vsamples);
   if (f.d_counters) {
     wave_count_add(f.d_counters, rays);
@@ -189,32 +189,49 @@ static uint32_t pick_sd(const vrx_renderer *r)
   return p;
 }
 
-static uint32_t max_stack(const vrx_scene *sc)
+static bool wide_of(const vrx_renderer *r)
 {
-  uint32_t m = sc->max_blas_height + 2;
+  return r && r->node_width == 4;
+}
+
+static uint32_t max_stack(const vrx_scene *sc, bool wide)
+{
+  // BVH2: one push per level (+2); 4-wide: up to three pushes per 4-wide
+  // level, and a 4-wide level spans two BVH2 levels
+  uint32_t m = wide ? 3 * ((sc->max_blas_height + 1) / 2) + 2 : sc->max_blas_height + 2;
   if (sc->num_entries > 1)
-    m += sc->tlas_height + 1;
+    m += wide ? 3 * ((sc->tlas_height + 1) / 2) + 1 : sc->tlas_height + 1;
   return m;
 }
 
-static uint32_t ovf_cap_of(const vrx_scene *sc, uint32_t sd)
+static uint32_t ovf_cap_of(const vrx_scene *sc, uint32_t sd, bool wide)
 {
-  uint32_t m = max_stack(sc);
+  uint32_t m = max_stack(sc, wide);
   return m > sd ? m - sd : 0u;
+}
+
+template <int PT, bool TWO, bool WIDE>
+static void launch_kw(const vrx_scene &sc, const vrx_camera &cam, const vrx_renderer &r,
+    const vrx_frame &f, uint32_t sd, uint32_t cap, hipStream_t s)
+{
+  dim3 grid((f.width + kTile - 1) / kTile, (f.height + kTile - 1) / kTile);
+  size_t lds = (size_t)sd * kBlock * 4;
+  if (r.kind == VRX_RENDER_AO)
+    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_AO, WIDE>), grid, dim3(kBlock), lds, s, sc,
+        cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
+  else
+    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_RAYCAST, WIDE>), grid, dim3(kBlock), lds, s,
+        sc, cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
 }
 
 template <int PT, bool TWO>
 static int launch_kind(const vrx_scene &sc, const vrx_camera &cam, const vrx_renderer &r,
     const vrx_frame &f, uint32_t sd, uint32_t cap, hipStream_t s)
 {
-  dim3 grid((f.width + kTile - 1) / kTile, (f.height + kTile - 1) / kTile);
-  size_t lds = (size_t)sd * kBlock * 4;
-  if (r.kind == VRX_RENDER_AO)
-    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_AO>), grid, dim3(kBlock), lds, s, sc, cam, r,
-        f, sd, cap, (uint32_t *)f.d_scratch);
+  if (wide_of(&r))
+    launch_kw<PT, TWO, true>(sc, cam, r, f, sd, cap, s);
   else
-    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_RAYCAST>), grid, dim3(kBlock), lds, s, sc,
-        cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
+    launch_kw<PT, TWO, false>(sc, cam, r, f, sd, cap, s);
   VRX_LAUNCH_CHECK();
   return VRX_OK;
 }
@@ -240,7 +257,7 @@ size_t vrx_render_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r, u
   if (!scene)
     return 0;
   uint32_t sd = pick_sd(r);
-  uint32_t cap = ovf_cap_of(scene, sd);
+  uint32_t cap = ovf_cap_of(scene, sd, wide_of(r));
   size_t tiles = (size_t)((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
   return 256 + tiles * kBlock * (size_t)cap * 4;
 }
@@ -266,8 +283,11 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
   VRX_ARG(scene->num_entries == 0 || (scene->d_entries && scene->d_geoms), "scene arrays");
   VRX_ARG(scene->num_entries <= 1 || scene->tlas.d_leaf_ids, "instance BVH not built");
   VRX_ARG(scene->num_volumes == 0 || scene->d_volumes, "volume array");
+  VRX_ARG(r->node_width == 0 || r->node_width == 2 || r->node_width == 4, "node width 2 or 4");
+  VRX_ARG(!wide_of(r) || scene->num_entries == 0 || scene->wide,
+      "node width 4 needs 4-wide nodes for every BVH (vrx_bvh_widen, scene->wide)");
   uint32_t sd = pick_sd(r);
-  uint32_t cap = ovf_cap_of(scene, sd);
+  uint32_t cap = ovf_cap_of(scene, sd, wide_of(r));
   size_t need = vrx_render_scratch_bytes(scene, r, f->width, f->height);
   if (!f->d_scratch || f->scratch_bytes < need) {
     set_error("render scratch too small: %zu < %zu", f->scratch_bytes, need);

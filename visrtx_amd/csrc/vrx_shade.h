@@ -363,7 +363,7 @@ __device__ __forceinline__ void wave_count_add(uint64_t *ctr, uint32_t v)
 // One pixel of the raycast / AO program (Raycast_ptx.cu:60-179,
 // AmbientOcclusion_ptx.cu:86-220): all pixel samples, the surface/volume
 // layer loop, accumulation and channel writes at output index idx.
-template <int PT, bool TWO, int KIND>
+template <int PT, bool TWO, int KIND, bool WIDE = false>
 __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_camera &cam,
     const vrx_renderer &r, const vrx_frame &f, uint32_t px, uint32_t py, size_t idx, Stack &st,
     uint32_t &rays, uint32_t &vsamples)
@@ -404,8 +404,8 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
     for (int layer = 0; layer < kMaxSurfaceLayers && outo < 0.99f; layer++) {
       Hit h;
       rays++;
-      bool found = TWO ? trace<PT, true, false>(sc, o, d, tlower, tmax, cull, st, h)
-                       : trace<PT, false, false>(sc, o, d, tlower, tmax, cull, st, h);
+      bool found = TWO ? trace<PT, true, false, WIDE>(sc, o, d, tlower, tmax, cull, st, h)
+                       : trace<PT, false, false, WIDE>(sc, o, d, tlower, tmax, cull, st, h);
       v3 color = mk(0.0f, 0.0f, 0.0f);
       float opacity = 0.0f;
       if (found) {
@@ -447,8 +447,8 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
               if (wgt != 0.0f) {
                 Hit hh;
                 rays++;
-                bool occ = TWO ? trace<PT, true, true>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh)
-                               : trace<PT, false, true>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh);
+                bool occ = TWO ? trace<PT, true, true, WIDE>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh)
+                               : trace<PT, false, true, WIDE>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh);
                 hits += wgt * (occ ? 1.0f : 0.0f);
               }
             }

@@ -240,6 +240,7 @@ struct Hit {
 
 struct BlasCtx {
   const Node *nodes;
+  const Node4 *wnodes; // 4-wide nodes (traversal of width 4 only)
   const float4 *leaves;
   const uint32_t *ids;
   int type;
@@ -256,6 +257,7 @@ __device__ __forceinline__ void load_blas(
   const uint32_t gi = gld(&Ep->geom);
   const vrx_geom *G = sc.d_geoms + gi;
   b.nodes = (const Node *)gld(&G->bvh.d_nodes);
+  b.wnodes = (const Node4 *)gld(&G->bvh.d_wide);
   b.leaves = (const float4 *)gld(&G->bvh.d_leaves);
   b.ids = gld(&G->bvh.d_leaf_ids);
   b.type = gld(&G->type);
@@ -375,7 +377,57 @@ __device__ __forceinline__ bool test_leaf(
 // Full traversal.  TWO_LEVEL: start in the instance BVH (sc.tlas) over
 // entries; otherwise trace entry 0 only.  Returns hit found (closest) or
 // occluded (ANY).
-template <int PT, bool TWO_LEVEL, bool ANY>
+// order (t, link) pairs by t: a 5-comparator sorting network
+__device__ __forceinline__ void cswap(float &ta, uint32_t &la, float &tb, uint32_t &lb)
+{
+  if (tb < ta) {
+    const float t = ta;
+    ta = tb;
+    tb = t;
+    const uint32_t l = la;
+    la = lb;
+    lb = l;
+  }
+}
+
+// Visit a 4-wide node: test the child boxes, continue with the nearest hit
+// child and push the others far-to-near.  Returns false when no child is hit.
+__device__ __forceinline__ bool visit4(const Node4 &nd, v3 o, v3 inv, float tmin, float tmax,
+    Stack &st, uint32_t &node)
+{
+  const float lx[4] = {nd.lox.x, nd.lox.y, nd.lox.z, nd.lox.w};
+  const float ly[4] = {nd.loy.x, nd.loy.y, nd.loy.z, nd.loy.w};
+  const float lz[4] = {nd.loz.x, nd.loz.y, nd.loz.z, nd.loz.w};
+  const float hx[4] = {nd.hix.x, nd.hix.y, nd.hix.z, nd.hix.w};
+  const float hy[4] = {nd.hiy.x, nd.hiy.y, nd.hiy.z, nd.hiy.w};
+  const float hz[4] = {nd.hiz.x, nd.hiz.y, nd.hiz.z, nd.hiz.w};
+  uint32_t l[4] = {nd.link.x, nd.link.y, nd.link.z, nd.link.w};
+  float t[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    float tn;
+    const bool hit = (uint32_t)c < nd.meta.x && slab(lx[c], ly[c], lz[c], hx[c], hy[c], hz[c], o,
+                                                     inv, tmin, tmax, tn);
+    t[c] = hit ? tn : INFINITY;
+  }
+  cswap(t[0], l[0], t[1], l[1]);
+  cswap(t[2], l[2], t[3], l[3]);
+  cswap(t[0], l[0], t[2], l[2]);
+  cswap(t[1], l[1], t[3], l[3]);
+  cswap(t[1], l[1], t[2], l[2]);
+  if (!(t[0] < INFINITY))
+    return false;
+  if (t[3] < INFINITY)
+    st.push(l[3]);
+  if (t[2] < INFINITY)
+    st.push(l[2]);
+  if (t[1] < INFINITY)
+    st.push(l[1]);
+  node = l[0];
+  return true;
+}
+
+template <int PT, bool TWO_LEVEL, bool ANY, bool WIDE = false>
 __device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax, bool cull_all,
     Stack &st, Hit &h)
 {
@@ -394,11 +446,14 @@ __device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax,
   bool in_blas;
   uint32_t node;
   const Node *nodes;
+  const Node4 *wnodes = nullptr;
   const uint32_t *tlas_ids = sc.tlas.d_leaf_ids;
   const Node *tlas_nodes = (const Node *)sc.tlas.d_nodes;
+  const Node4 *tlas_wnodes = (const Node4 *)sc.tlas.d_wide;
   if (TWO_LEVEL) {
     in_blas = false;
     nodes = tlas_nodes;
+    wnodes = tlas_wnodes;
     node = sc.num_entries == 1 ? kLeafBit : 0u;
   } else {
     uint32_t n, root;
@@ -410,11 +465,15 @@ __device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax,
       wt = wt_setup(d);
     in_blas = true;
     nodes = b.nodes;
+    wnodes = b.wnodes;
     node = root;
   }
   for (;;) {
     bool do_pop = true;
-    if (!(node & kLeafBit)) {
+    if (WIDE && !(node & kLeafBit)) {
+      const Node4 nd = gld_node4(wnodes, node);
+      do_pop = !visit4(nd, o, inv, tmin, h.t, st, node);
+    } else if (!(node & kLeafBit)) {
       const Node nd = gld_node(nodes, node);
       float tn0, tn1;
       bool h0 = slab(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, tmin, h.t, tn0);
@@ -451,6 +510,7 @@ __device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax,
         st.push(kMarker);
         in_blas = true;
         nodes = b.nodes;
+        wnodes = b.wnodes;
         node = root;
         do_pop = false;
       } else {
@@ -466,6 +526,7 @@ __device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax,
         if (TWO_LEVEL && node == kMarker) {
           in_blas = false;
           nodes = tlas_nodes;
+          wnodes = tlas_wnodes;
           o = wo;
           d = wd;
           inv = winv;

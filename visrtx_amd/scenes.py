@@ -369,7 +369,7 @@ def make_renderer(dev: A.Device, s: AnariScene, kind: str = "raycast", **params)
              "ambientRadiance": A.FLOAT32, "pixelSamples": A.INT32, "ambientSamples": A.INT32,
              "ambientOcclusionDistance": A.FLOAT32, "cullTriangleBackfaces": A.BOOL,
              "volumeSamplingRate": A.FLOAT32, "sampleLimit": A.INT32, "stackDepthLDS": A.INT32,
-             "volumeUnroll": A.INT32}
+             "volumeUnroll": A.INT32, "nodeWidth": A.INT32}
     for k, v in params.items():
         dev.set(r, k, v, types[k])
     dev.commit(r)
