@@ -387,6 +387,20 @@ def main():
                     "kernel": "k_render<sphere,single-level,raycast>",
                     "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
                     "avg_kernel_ms": round(kern_ms, 4)}
+        # the same frames traversing 4-wide nodes (renderer nodeWidth = 4)
+        dev.set(work.r, "nodeWidth", 4, A.INT32)
+        dev.commit(work.r)
+        sps4, durs4 = work.timed(args.steps, args.warmup)
+        k4 = statistics.mean(durs4) * 1000.0
+        extra["primary_kernel_mrays_s_width4"] = round(rays / (k4 / 1000) / 1e6, 2)
+        extra["primary_wall_mrays_s_width4"] = round(rays / sps4 / 1e6, 2)
+        extra["widen_ms_10m_spheres"] = round(
+            1000.0 * dev.get_property(work.s.world, "bvhWidenDuration", A.FLOAT32), 3)
+        log(f"primary raycast, 4-wide nodes: kernel {k4:.3f} ms -> "
+            f"{extra['primary_kernel_mrays_s_width4']} Mrays/s (widen "
+            f"{extra['widen_ms_10m_spheres']} ms)")
+        dev.set(work.r, "nodeWidth", 2, A.INT32)
+        dev.commit(work.r)
     if "ao" in sec:
         # AO renderer, 16 spp
         ao = Workload(dev, sc, cam, W, H, "ao", ids=True, pixelSamples=16, ambientSamples=1,
@@ -408,6 +422,10 @@ def main():
         sps, d1 = hf.timed(20, 3)
         extra["heightfield_mrays_s"] = round(1024 * 768 / sps / 1e6, 2)
         extra["heightfield_kernel_ms"] = round(statistics.mean(d1) * 1000, 4)
+        dev.set(hf.r, "nodeWidth", 4, A.INT32)
+        dev.commit(hf.r)
+        _, d14 = hf.timed(20, 3)
+        extra["heightfield_kernel_ms_width4"] = round(statistics.mean(d14) * 1000, 4)
         log(f"heightfield: {sps * 1000:.3f} ms/step -> {extra['heightfield_mrays_s']} Mrays/s")
         hf.release()
 
