@@ -222,7 +222,7 @@ def bench_batched(dev, reps=5):
         f"instances, generated in {time.perf_counter() - t:.1f}s")
     s = S.build_anari(dev, sc)
     r = S.make_renderer(dev, s, "raycast")
-    scene = B.scene_of(dev, s.world)
+    scene = B.scene_of(dev, s.world, wide=True)
     out = {"meshes": len(sc.geoms), "triangles": int(tris), "instances": len(sc.instances),
            "views": len(cams), "res": 512,
            "build_ms": round(1000.0 * dev.get_property(s.world, "bvhBuildDuration", A.FLOAT32),
@@ -256,7 +256,8 @@ def bench_batched(dev, reps=5):
     bf = B.BatchFrame(scene, B.renderer_of(dev, r), cams, W, H, srgb=True)
     best = bf.autotune()
     log(f"batched autotune: {best:.3f} ms, tile {bf.b.tile_w}x{bf.b.tile_h}, block "
-        f"{bf.b.block_threads}, LDS stack {bf.renderer.stack_lds}")
+        f"{bf.b.block_threads}, LDS stack {bf.renderer.stack_lds}, node width "
+        f"{bf.renderer.node_width}")
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         bf.render(stream)
@@ -291,7 +292,8 @@ def bench_batched(dev, reps=5):
                 "batched_mrays_s": round(rays / (graph_ms / 1000.0) / 1e6, 2),
                 "plain_mrays_s": round(rays / (plain_ms / 1000.0) / 1e6, 2),
                 "autotuned": {"tile": [bf.b.tile_w, bf.b.tile_h], "block": bf.b.block_threads,
-                              "stack_lds": bf.renderer.stack_lds}})
+                              "stack_lds": bf.renderer.stack_lds,
+                              "node_width": bf.renderer.node_width}})
     del bf
     s.release()
     return out
@@ -411,6 +413,12 @@ def main():
         extra["ao16_rays_per_frame"] = int(nr)
         extra["ao16_ms_per_frame"] = round(sps * 1000, 3)
         log(f"AO 16spp: {sps * 1000:.2f} ms/frame, {nr} rays -> {extra['ao16_mrays_s']} Mrays/s")
+        dev.set(ao.r, "nodeWidth", 4, A.INT32)
+        dev.commit(ao.r)
+        sps4, _ = ao.timed(3, 1)
+        extra["ao16_mrays_s_width4"] = round(ao.rays() / sps4 / 1e6, 2)
+        log(f"AO 16spp, 4-wide nodes: {sps4 * 1000:.2f} ms/frame -> "
+            f"{extra['ao16_mrays_s_width4']} Mrays/s")
         ao.release()
     if "headline" in sec or "ao" in sec:
         work.release()

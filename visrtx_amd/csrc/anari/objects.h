@@ -135,6 +135,7 @@ struct Geometry : Object {
   size_t capWide{0};
   bool wideValid{false};
   int buildBVH(vrx_stream s);
+  int allocWide();
   int buildWide(vrx_stream s);
   void freeBVH();
 };
