@@ -386,22 +386,22 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
-                    "kernel": "k_render<sphere,single-level,raycast>",
+                    "kernel": "k_render<sphere,single-level,raycast,4-wide nodes>",
                     "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
                     "avg_kernel_ms": round(kern_ms, 4)}
-        # the same frames traversing 4-wide nodes (renderer nodeWidth = 4)
-        dev.set(work.r, "nodeWidth", 4, A.INT32)
-        dev.commit(work.r)
-        sps4, durs4 = work.timed(args.steps, args.warmup)
-        k4 = statistics.mean(durs4) * 1000.0
-        extra["primary_kernel_mrays_s_width4"] = round(rays / (k4 / 1000) / 1e6, 2)
-        extra["primary_wall_mrays_s_width4"] = round(rays / sps4 / 1e6, 2)
         extra["widen_ms_10m_spheres"] = round(
             1000.0 * dev.get_property(work.s.world, "bvhWidenDuration", A.FLOAT32), 3)
-        log(f"primary raycast, 4-wide nodes: kernel {k4:.3f} ms -> "
-            f"{extra['primary_kernel_mrays_s_width4']} Mrays/s (widen "
-            f"{extra['widen_ms_10m_spheres']} ms)")
+        # the same frames traversing the BVH2 directly (renderer nodeWidth = 2)
         dev.set(work.r, "nodeWidth", 2, A.INT32)
+        dev.commit(work.r)
+        sps2, durs2 = work.timed(args.steps, args.warmup)
+        k2 = statistics.mean(durs2) * 1000.0
+        extra["primary_kernel_mrays_s_width2"] = round(rays / (k2 / 1000) / 1e6, 2)
+        extra["primary_wall_mrays_s_width2"] = round(rays / sps2 / 1e6, 2)
+        log(f"primary raycast, BVH2 nodes: kernel {k2:.3f} ms -> "
+            f"{extra['primary_kernel_mrays_s_width2']} Mrays/s (4-wide widen "
+            f"{extra['widen_ms_10m_spheres']} ms)")
+        dev.set(work.r, "nodeWidth", 4, A.INT32)
         dev.commit(work.r)
     if "ao" in sec:
         # AO renderer, 16 spp
@@ -413,12 +413,12 @@ def main():
         extra["ao16_rays_per_frame"] = int(nr)
         extra["ao16_ms_per_frame"] = round(sps * 1000, 3)
         log(f"AO 16spp: {sps * 1000:.2f} ms/frame, {nr} rays -> {extra['ao16_mrays_s']} Mrays/s")
-        dev.set(ao.r, "nodeWidth", 4, A.INT32)
+        dev.set(ao.r, "nodeWidth", 2, A.INT32)
         dev.commit(ao.r)
-        sps4, _ = ao.timed(3, 1)
-        extra["ao16_mrays_s_width4"] = round(ao.rays() / sps4 / 1e6, 2)
-        log(f"AO 16spp, 4-wide nodes: {sps4 * 1000:.2f} ms/frame -> "
-            f"{extra['ao16_mrays_s_width4']} Mrays/s")
+        sps2, _ = ao.timed(3, 1)
+        extra["ao16_mrays_s_width2"] = round(ao.rays() / sps2 / 1e6, 2)
+        log(f"AO 16spp, BVH2 nodes: {sps2 * 1000:.2f} ms/frame -> "
+            f"{extra['ao16_mrays_s_width2']} Mrays/s")
         ao.release()
     if "headline" in sec or "ao" in sec:
         work.release()
@@ -430,10 +430,10 @@ def main():
         sps, d1 = hf.timed(20, 3)
         extra["heightfield_mrays_s"] = round(1024 * 768 / sps / 1e6, 2)
         extra["heightfield_kernel_ms"] = round(statistics.mean(d1) * 1000, 4)
-        dev.set(hf.r, "nodeWidth", 4, A.INT32)
+        dev.set(hf.r, "nodeWidth", 2, A.INT32)
         dev.commit(hf.r)
-        _, d14 = hf.timed(20, 3)
-        extra["heightfield_kernel_ms_width4"] = round(statistics.mean(d14) * 1000, 4)
+        _, d12 = hf.timed(20, 3)
+        extra["heightfield_kernel_ms_width2"] = round(statistics.mean(d12) * 1000, 4)
         log(f"heightfield: {sps * 1000:.3f} ms/step -> {extra['heightfield_mrays_s']} Mrays/s")
         hf.release()
 

@@ -58,6 +58,8 @@ class BatchFrame:
         self.scene = scene
         self.renderer = N.vrx_renderer.from_buffer_copy(renderer)
         self.renderer.kind = N.VRX_RENDER_RAYCAST
+        if self.renderer.node_width == 4 and not scene.wide:
+            self.renderer.node_width = 2   # the scene was fetched without 4-wide nodes
         structs = [camera_struct(c) for c in cams]
         raw = b"".join(bytes(s) for s in structs)
         self.cams = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)

@@ -1208,7 +1208,9 @@ void Renderer::commit()
   r.inv_volume_sampling_rate = 1.0f / rate;
   r.stack_lds = get<int>("stackDepthLDS", ANARI_INT32, 0);
   r.vol_unroll = get<int>("volumeUnroll", ANARI_INT32, 0);
-  r.node_width = get<int>("nodeWidth", ANARI_INT32, 2) == 4 ? 4 : 2;
+  // 4-wide nodes by default: on MI355X they trace 6-22% faster than BVH2
+  // (the world builds them once, lazily, from the BVH2)
+  r.node_width = get<int>("nodeWidth", ANARI_INT32, 4) == 2 ? 2 : 4;
   sampleLimit = get<int>("sampleLimit", ANARI_INT32, 128);
   if (kind == VRX_RENDER_RAYCAST)
     sampleLimit = 1; // single-shot renderer

@@ -451,8 +451,10 @@ __global__ __launch_bounds__(256) void k_karras(
     hist[threadIdx.x] = 0u;
   __syncthreads();
   const int64_t n = n_;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n - 1) {
+  // grid-stride: a bounded grid keeps the per-block level histograms few, so
+  // the contended global level counters see one add per block and level
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n - 1;
+       i += (int64_t)gridDim.x * blockDim.x) {
     int d = (kdelta(sc, n, i, i + 1) - kdelta(sc, n, i, i - 1)) > 0 ? 1 : -1;
     int dmin = kdelta(sc, n, i, i - d);
     int64_t lmax = 2;
@@ -500,6 +502,12 @@ __global__ void k_level_offsets(const uint32_t *lcount, uint32_t *loff, uint32_t
 }
 
 // group internal nodes by prefix length (block-aggregated reservations)
+// Each workgroup buckets a chunk of kLevelChunk nodes: ranks within the chunk
+// come from LDS counters, and the chunk claims its slice of every level with
+// one global add per level, so the 64 level counters see few, large adds.
+constexpr int kLevelItems = 16;
+constexpr uint32_t kLevelChunk = 256 * kLevelItems;
+
 __global__ __launch_bounds__(256) void k_level_scatter(
     const uint8_t *level, uint32_t n_internal, const uint32_t *loff, uint32_t *lfill, uint32_t *bucket)
 {
@@ -507,18 +515,28 @@ __global__ __launch_bounds__(256) void k_level_scatter(
   if (threadIdx.x < 64)
     cnt[threadIdx.x] = 0u;
   __syncthreads();
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t lv = 0, r = 0;
-  if (i < n_internal) {
-    lv = level[i];
-    r = atomicAdd(&cnt[lv], 1u);
+  const uint32_t c0 = blockIdx.x * kLevelChunk;
+  uint32_t lv[kLevelItems], r[kLevelItems];
+#pragma unroll
+  for (int k = 0; k < kLevelItems; k++) {
+    const uint32_t i = c0 + (uint32_t)k * 256 + threadIdx.x;
+    lv[k] = 0;
+    r[k] = 0;
+    if (i < n_internal) {
+      lv[k] = level[i];
+      r[k] = atomicAdd(&cnt[lv[k]], 1u);
+    }
   }
   __syncthreads();
   if (threadIdx.x < 64 && cnt[threadIdx.x])
     base[threadIdx.x] = loff[threadIdx.x] + atomicAdd(&lfill[threadIdx.x], cnt[threadIdx.x]);
   __syncthreads();
-  if (i < n_internal)
-    bucket[base[lv] + r] = i;
+#pragma unroll
+  for (int k = 0; k < kLevelItems; k++) {
+    const uint32_t i = c0 + (uint32_t)k * 256 + threadIdx.x;
+    if (i < n_internal)
+      bucket[base[lv[k]] + r[k]] = i;
+  }
 }
 
 template <int T>
@@ -772,11 +790,14 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
     const uint32_t ni = n - 1;
     const unsigned gi = (ni + 255) / 256;
     hipLaunchKernelGGL(k_fill_zero64, dim3(1), dim3(64), 0, s, ws.lcount);
-    hipLaunchKernelGGL(k_karras, dim3(gi), dim3(256), 0, s, ki, n, ws.child, ws.level, ws.lcount);
+    hipLaunchKernelGGL(k_karras, dim3(min(gi, 4096u)), dim3(256), 0, s, ki, n, ws.child, ws.level,
+        ws.lcount);
     hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, ws.lcount, ws.loff, ws.lfill);
-    hipLaunchKernelGGL(k_level_scatter, dim3(gi), dim3(256), 0, s, ws.level, ni, ws.loff, ws.lfill,
-        ws.bucket);
-    const unsigned gl = min(gi, 4096u);
+    hipLaunchKernelGGL(k_level_scatter, dim3((ni + kLevelChunk - 1) / kLevelChunk), dim3(256), 0, s,
+        ws.level, ni, ws.loff, ws.lfill, ws.bucket);
+    // grid-stride refit: most prefix-length levels are small or empty, so a
+    // modest grid keeps their launches cheap
+    const unsigned gl = min(gi, 1024u);
     for (int lv = 63; lv >= 0; lv--)
       hipLaunchKernelGGL((k_refit_level<T>), dim3(gl), dim3(256), 0, s, p, out.d_leaves,
           out.d_leaf_ids, ws.child, ws.bucket, ws.loff, ws.lcount, lv, nodes);
