@@ -1,0 +1,93 @@
+"""Frame semantics through the ANARI API on the GPU: the completion callback
+fires once per rendered frame with that frame's handle (Frame.cu:295-304),
+frameReady(NO_WAIT) turns true once the frame is done, accumulation resets on
+a camera commit and the raycast renderer stops at one sample (Raycast.cpp:
+45-50), ray counters and the traversal stack-overflow check stay clean."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from visrtx_amd import anari as A
+from visrtx_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
+
+
+def _scene(dev):
+    c, r = S.gen_spheres(20_000, seed=5, rmin=2e-3, rmax=8e-3)
+    sc = S.SceneDesc(geoms=[S.GeomDesc("sphere", c, radius=r, surface_id=1)], world_geoms=[0])
+    s = S.build_anari(dev, sc)
+    cam = S.CameraDesc((0.5, 0.5, -1.2), (0.0, 0.0, 1.0), (0, 1, 0), math.radians(45.0), 1.0)
+    return s, cam
+
+
+def test_completion_callbacks_and_ready(require_gpu):
+    dev = A.Device()
+    s, cam = _scene(dev)
+    r = S.make_renderer(dev, s, "ao", pixelSamples=1, ambientSamples=1)
+    seen = []
+
+    @CB
+    def on_done(user, device, frame):
+        seen.append((user, frame))
+
+    frames = []
+    for k in range(3):
+        c = S.make_camera(dev, s, cam)
+        f = S.make_frame(dev, s, s.world, c, r, 64, 64)
+        dev.set(f, "frameCompletionCallback", C.cast(on_done, C.c_void_p).value,
+                A.FRAME_COMPLETION_CALLBACK)
+        dev.set(f, "frameCompletionCallbackUserData", 1000 + k, A.VOID_POINTER)
+        dev.commit(f)
+        frames.append(f)
+    print("[frame] 3 frames with callbacks", flush=True)
+    for f in frames:
+        dev.render(f)
+    for f in frames:
+        assert dev.ready(f, wait=True)
+        assert dev.ready(f, wait=False)
+    got = sorted((u, fr) for u, fr in seen)
+    assert got == sorted((1000 + k, frames[k]) for k in range(3)), got
+    for f in frames:
+        assert dev.get_property(f, "raysTraced", A.UINT64) >= 64 * 64
+    assert not dev.errors(), dev.errors()
+    s.release()
+    dev.close()
+
+
+def test_accumulation_and_sample_limit(require_gpu):
+    dev = A.Device()
+    s, cam = _scene(dev)
+    c = S.make_camera(dev, s, cam)
+    ao = S.make_renderer(dev, s, "ao", pixelSamples=1, ambientSamples=2, sampleLimit=3)
+    f = S.make_frame(dev, s, s.world, c, ao, 48, 48)
+    ns = []
+    for _ in range(5):
+        dev.render(f)
+        dev.ready(f)
+        ns.append(dev.get_property(f, "numSamples", A.INT32))
+    # frameID counts accumulated samples; once it reaches sampleLimit the
+    # frame is not rendered again (Frame.cu:251-253)
+    assert ns == [0, 1, 2, 3, 3], ns
+    dev.commit(c)                       # a camera commit resets accumulation
+    dev.render(f)
+    dev.ready(f)
+    assert dev.get_property(f, "numSamples", A.INT32) == 0
+    rc = S.make_renderer(dev, s, "raycast")
+    dev.set(f, "renderer", rc, A.RENDERER)
+    dev.commit(f)
+    a = None
+    for _ in range(3):
+        dev.render(f)
+        dev.ready(f)
+        b = dev.map_frame(f, "channel.color")[0].copy()
+        if a is not None:
+            np.testing.assert_array_equal(a, b)   # single-shot renderer
+        a = b
+    assert not dev.errors(), dev.errors()
+    s.release()
+    dev.close()

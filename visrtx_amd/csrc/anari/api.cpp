@@ -470,7 +470,7 @@ int anariFrameReady(ANARIDevice d, ANARIFrame f, ANARIWaitMask m)
     return 0;
   Frame *fr = (Frame *)f;
   if (m == ANARI_NO_WAIT)
-    return fr->rendered ? (vrx_event_query(fr->evEnd) == 1 ? 1 : 0) : 1;
+    return fr->rendered ? (vrx_event_query(fr->evDone) == 1 ? 1 : 0) : 1;
   fr->wait();
   return 1;
 }

@@ -1240,15 +1240,23 @@ Frame::Frame(Device *d) : Object(d, ANARI_FRAME, "")
 {
   vrx_event_create(&evStart);
   vrx_event_create(&evEnd);
+  vrx_event_create(&evDone);
+  void *p = nullptr;
+  if (vrx_host_malloc(&p, 32) == 0) {
+    hReadback = (uint64_t *)p;
+    memset(hReadback, 0, 32);
+  }
 }
 
 Frame::~Frame()
 {
   if (rendered)
-    vrx_event_sync(evEnd);
+    vrx_event_sync(evDone);
   freeBuffers();
   vrx_event_destroy(evStart);
   vrx_event_destroy(evEnd);
+  vrx_event_destroy(evDone);
+  vrx_host_free(hReadback);
 }
 
 void Frame::freeBuffers()
@@ -1311,8 +1319,16 @@ void Frame::allocate()
 
 void Frame::wait()
 {
-  if (rendered)
-    vrx_event_sync(evEnd);
+  if (!rendered)
+    return;
+  vrx_event_sync(evDone);
+  // the traversal stacks are sized from the BVH heights, so this flag stays
+  // clear; it is checked so that a violated bound can never pass silently
+  if (hReadback && (uint32_t)hReadback[2] != 0 && !stackReported) {
+    report(ANARI_SEVERITY_ERROR, ANARI_STATUS_UNKNOWN_ERROR,
+        "traversal stack overflow in the last frame: results are incomplete");
+    stackReported = true;
+  }
 }
 
 void Frame::render()
@@ -1384,25 +1400,22 @@ void Frame::render()
   vrx_fill_u32(dCounters, 0u, 4, dev->stream);
   vrx_event_record(evStart, dev->stream);
   dev->check(vrx_render(&world->scene, &camera->cam, &renderer->r, &f, dev->stream), this, "render");
+  vrx_event_record(evEnd, dev->stream);
+  if (hReadback) {
+    vrx_memcpy(hReadback, dCounters, 16, VRX_D2H, dev->stream);
+    vrx_memcpy(hReadback + 2, dScratch, 4, VRX_D2H, dev->stream);
+  }
   if (callback) {
-    struct Ctx {
-      ANARIFrameCompletionCallback cb;
-      const void *user;
-      Device *dev;
-      Frame *frame;
-    };
-    static thread_local Ctx ctx;
-    ctx = Ctx{callback, callbackUser, dev, this};
+    cbctx = CallbackCtx{callback, callbackUser, dev, this};
     vrx_launch_host_func(
         dev->stream,
         [](void *p) {
-          Ctx *c = (Ctx *)p;
+          CallbackCtx *c = (CallbackCtx *)p;
           c->cb(c->user, (ANARIDevice)c->dev, (ANARIFrame)c->frame);
         },
-        &ctx);
+        &cbctx);
   }
-  vrx_event_record(evEnd, dev->stream);
-  vrx_memcpy(counters, dCounters, 16, VRX_D2H, dev->stream);
+  vrx_event_record(evDone, dev->stream);
   rendered = true;
 }
 
@@ -1460,6 +1473,7 @@ const void *Frame::map(const std::string &channel, uint32_t *w, uint32_t *h, ANA
 
 bool Frame::getProperty(const std::string &name, ANARIDataType t, void *mem, uint64_t sz, uint32_t mask)
 {
+  static const uint64_t kZero64 = 0;
   if (name == "duration" && t == ANARI_FLOAT32 && sz >= 4) {
     if (mask & ANARI_WAIT)
       wait();
@@ -1486,12 +1500,12 @@ bool Frame::getProperty(const std::string &name, ANARIDataType t, void *mem, uin
   }
   if (name == "raysTraced" && t == ANARI_UINT64 && sz >= 8) {
     wait();
-    memcpy(mem, &counters[0], 8);
+    memcpy(mem, hReadback ? &hReadback[0] : &kZero64, 8);
     return true;
   }
   if (name == "volumeSamples" && t == ANARI_UINT64 && sz >= 8) {
     wait();
-    memcpy(mem, &counters[1], 8);
+    memcpy(mem, hReadback ? &hReadback[1] : &kZero64, 8);
     return true;
   }
   return Object::getProperty(name, t, mem, sz, mask);

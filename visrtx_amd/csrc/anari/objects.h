@@ -273,12 +273,22 @@ struct Frame : Object {
   size_t scratchBytes{0};
   std::vector<uint8_t> hColor, hDepth, hPrim, hObj, hInst;
   vrx_event evStart{nullptr}, evEnd{nullptr};
+  vrx_event evDone{nullptr}; // after the result read-backs that follow evEnd
   bool rendered{false};
+  bool stackReported{false};
+  struct CallbackCtx {
+    ANARIFrameCompletionCallback cb;
+    const void *user;
+    void *dev;
+    void *frame;
+  } cbctx{}; // one outstanding render per frame, so one context per frame
   bool nextReset{true};
   uint64_t lastSeenFinalize{0};
   uint32_t frameID{0};
   float duration{0.0f};
-  uint64_t counters[2]{0, 0};
+  // pinned read-back slots: [0] rays traced, [1] volume samples, [2] the
+  // stack overflow flag (pinned, so the copies stay asynchronous)
+  uint64_t *hReadback{nullptr};
   bool allocated{false};
 
   void allocate();
