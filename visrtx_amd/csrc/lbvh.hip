@@ -46,9 +46,11 @@ struct Workspace {
   uint32_t *lcount;    // 64 nodes per prefix length
   uint32_t *loff;      // 64 bucket offsets
   uint32_t *lfill;     // 64 bucket fill counters
+  void *rec;           // n leaf records in input order (written coalesced by
+                       // the Morton pass, gathered into sorted order later)
 };
 
-static size_t workspace_layout(uint32_t n, void *base, Workspace *w)
+static size_t workspace_layout(int type, uint32_t n, void *base, Workspace *w)
 {
   size_t off = 0;
   char *b = (char *)base;
@@ -72,6 +74,7 @@ static size_t workspace_layout(uint32_t n, void *base, Workspace *w)
   ws.lcount = (uint32_t *)take(4 * 64);
   ws.loff = (uint32_t *)take(4 * 64);
   ws.lfill = (uint32_t *)take(4 * 64);
+  ws.rec = take(leaf_record_bytes(type) * nn);
   if (w)
     *w = ws;
   return off;
@@ -210,8 +213,12 @@ __device__ __forceinline__ uint32_t quant10(float c, float lo, float scale)
 }
 
 template <int T>
+__device__ __forceinline__ void write_leaf(
+    const vrx_prims &p, uint32_t orig, uint32_t k, void *leaves);
+
+template <int T>
 __global__ __launch_bounds__(256) void k_morton(
-    vrx_prims p, const uint32_t *cbox, uint32_t *keys, uint32_t *vals, float *cbox_out)
+    vrx_prims p, const uint32_t *cbox, uint32_t *keys, uint32_t *vals, float *cbox_out, void *rec)
 {
   float lo[3], hi[3], scale[3];
 #pragma unroll
@@ -239,6 +246,8 @@ __global__ __launch_bounds__(256) void k_morton(
   uint32_t qz = quant10(c.z, lo[2], scale[2]);
   keys[i] = expand_bits10(qx) * 4u + expand_bits10(qy) * 2u + expand_bits10(qz);
   vals[i] = i;
+  if constexpr (T != VRX_BOX)
+    write_leaf<T>(p, i, i, rec); // input order: a coalesced write
 }
 
 // --------------------------------------------------------------------------
@@ -597,14 +606,22 @@ __device__ __forceinline__ void leaf_box(
 
 template <int T>
 __global__ __launch_bounds__(256) void k_leaves(
-    vrx_prims p, const uint32_t *order, void *leaves, uint32_t *leaf_ids)
+    vrx_prims p, const uint32_t *order, const void *rec, void *leaves, uint32_t *leaf_ids)
 {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= p.n)
     return;
   const uint32_t orig = order[k];
   leaf_ids[k] = orig;
-  write_leaf<T>(p, orig, k, leaves);
+  // gather the record the Morton pass wrote for primitive `orig`: one
+  // contiguous 16/32/48-byte read instead of separate index, vertex and
+  // radius gathers
+  constexpr int nq = T == VRX_SPHERE ? 1 : (T == VRX_CYLINDER ? 2 : (T == VRX_TRIANGLE ? 3 : 0));
+  const float4 *src = (const float4 *)rec + (size_t)nq * orig;
+  float4 *dst = (float4 *)leaves + (size_t)nq * k;
+#pragma unroll
+  for (int q = 0; q < nq; q++)
+    gst(dst, q, gld(src, q));
 }
 
 // One prefix-length level of the refit: every node of the level reads its
@@ -690,51 +707,66 @@ __device__ __forceinline__ void node_child(const Node &nd, int j, float b[6], ui
 
 __global__ __launch_bounds__(256) void k_widen(const Node *nodes, uint32_t ninner, Node4 *wide)
 {
+  // records are staged in LDS and written out lane-contiguous, so each store
+  // instruction covers whole 128-byte lines
+  __shared__ float4 s_rec[256 * 8];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ninner)
-    return;
-  const Node nd = gld_node(nodes, i);
-  float bx[4][6];
-  uint32_t lk[4];
-  uint32_t cnt = 0;
+  const uint32_t b0 = blockIdx.x * blockDim.x;
+  const uint32_t nrec = min(256u, ninner - b0);
+  if (i < ninner) {
+    const Node nd = gld_node(nodes, i);
+    // the one or two slots each child contributes (fixed register indices:
+    // a dynamically indexed slot array would spill to scratch)
+    float cb[2][2][6];
+    uint32_t cl[2][2], cc[2];
 #pragma unroll
-  for (int j = 0; j < 2; j++) {
-    float b[6];
-    uint32_t link;
-    node_child(nd, j, b, link);
-    if (link & kLeafBit) {
+    for (int j = 0; j < 2; j++) {
+      uint32_t link;
+      node_child(nd, j, cb[j][0], link);
+      if (link & kLeafBit) {
+        cl[j][0] = link;
+        cl[j][1] = kLeafBit;
 #pragma unroll
-      for (int q = 0; q < 6; q++)
-        bx[cnt][q] = b[q];
-      lk[cnt] = link;
-      cnt++;
-    } else {
-      const Node cn = gld_node(nodes, link);
-#pragma unroll
-      for (int g = 0; g < 2; g++) {
-        uint32_t gl;
-        node_child(cn, g, bx[cnt], gl);
-        lk[cnt] = gl;
-        cnt++;
+        for (int q = 0; q < 6; q++)
+          cb[j][1][q] = 0.0f;
+        cc[j] = 1;
+      } else {
+        const Node cn = gld_node(nodes, link);
+        node_child(cn, 0, cb[j][0], cl[j][0]);
+        node_child(cn, 1, cb[j][1], cl[j][1]);
+        cc[j] = 2;
       }
     }
-  }
-  for (uint32_t c = cnt; c < 4; c++) {
+    const bool two = cc[0] == 2;
+    const uint32_t cnt = cc[0] + cc[1];
+    float bx[4][6];
+    uint32_t lk[4];
 #pragma unroll
-    for (int q = 0; q < 6; q++)
-      bx[c][q] = 0.0f;
-    lk[c] = kLeafBit;
+    for (int q = 0; q < 6; q++) {
+      bx[0][q] = cb[0][0][q];
+      bx[1][q] = two ? cb[0][1][q] : cb[1][0][q];
+      bx[2][q] = two ? cb[1][0][q] : cb[1][1][q];
+      bx[3][q] = two ? cb[1][1][q] : 0.0f;
+    }
+    lk[0] = cl[0][0];
+    lk[1] = two ? cl[0][1] : cl[1][0];
+    lk[2] = two ? cl[1][0] : cl[1][1];
+    lk[3] = two ? cl[1][1] : kLeafBit;
+    float4 *w = s_rec + threadIdx.x * 8;
+    w[0] = make_float4(bx[0][0], bx[1][0], bx[2][0], bx[3][0]);
+    w[1] = make_float4(bx[0][1], bx[1][1], bx[2][1], bx[3][1]);
+    w[2] = make_float4(bx[0][2], bx[1][2], bx[2][2], bx[3][2]);
+    w[3] = make_float4(bx[0][3], bx[1][3], bx[2][3], bx[3][3]);
+    w[4] = make_float4(bx[0][4], bx[1][4], bx[2][4], bx[3][4]);
+    w[5] = make_float4(bx[0][5], bx[1][5], bx[2][5], bx[3][5]);
+    w[6] = make_float4(__uint_as_float(lk[0]), __uint_as_float(lk[1]), __uint_as_float(lk[2]),
+        __uint_as_float(lk[3]));
+    w[7] = make_float4(__uint_as_float(cnt), 0.0f, 0.0f, 0.0f);
   }
-  float4 *w = (float4 *)(wide + i);
-  gst(w, 0, make_float4(bx[0][0], bx[1][0], bx[2][0], bx[3][0]));
-  gst(w, 1, make_float4(bx[0][1], bx[1][1], bx[2][1], bx[3][1]));
-  gst(w, 2, make_float4(bx[0][2], bx[1][2], bx[2][2], bx[3][2]));
-  gst(w, 3, make_float4(bx[0][3], bx[1][3], bx[2][3], bx[3][3]));
-  gst(w, 4, make_float4(bx[0][4], bx[1][4], bx[2][4], bx[3][4]));
-  gst(w, 5, make_float4(bx[0][5], bx[1][5], bx[2][5], bx[3][5]));
-  gst(w, 6, make_float4(__uint_as_float(lk[0]), __uint_as_float(lk[1]), __uint_as_float(lk[2]),
-                __uint_as_float(lk[3])));
-  gst(w, 7, make_float4(__uint_as_float(cnt), 0.0f, 0.0f, 0.0f));
+  __syncthreads();
+  float4 *out = (float4 *)(wide + b0);
+  for (uint32_t q = threadIdx.x; q < nrec * 8; q += blockDim.x)
+    gst(out, q, s_rec[q]);
 }
 
 __global__ void k_empty_header(vrx_bvh_header *h, int type)
@@ -766,7 +798,7 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
   unsigned gb = min(g256, 2048u);
   hipLaunchKernelGGL(k_centroid_bounds<T>, dim3(gb), dim3(256), 0, s, p, ws.cbox);
   hipLaunchKernelGGL(k_morton<T>, dim3(g256), dim3(256), 0, s, p, ws.cbox, ws.keys_a, ws.vals_a,
-      cbox_out);
+      cbox_out, ws.rec);
   VRX_LAUNCH_CHECK();
   const uint32_t nb = sort_blocks(n);
   uint32_t *ki = ws.keys_a, *vi = ws.vals_a, *ko = ws.keys_b, *vo = ws.vals_b;
@@ -784,7 +816,8 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
   // after 4 passes the sorted data is back in keys_a/vals_a
   if (codes_out)
     hipLaunchKernelGGL(k_copy_u32, dim3(g256), dim3(256), 0, s, ki, codes_out, n);
-  hipLaunchKernelGGL((k_leaves<T>), dim3(g256), dim3(256), 0, s, p, vi, out.d_leaves, out.d_leaf_ids);
+  hipLaunchKernelGGL((k_leaves<T>), dim3(g256), dim3(256), 0, s, p, vi, ws.rec, out.d_leaves,
+      out.d_leaf_ids);
   Node *nodes = (Node *)out.d_nodes;
   if (n >= 2) {
     const uint32_t ni = n - 1;
@@ -824,13 +857,13 @@ int bvh_build_impl(const vrx_prims *p, const vrx_bvh *out, void *d_ws, size_t ws
   VRX_ARG(out->d_leaf_ids, "out->d_leaf_ids");
   VRX_ARG(p->n < 2 || out->d_nodes, "out->d_nodes");
   VRX_ARG(leaf_record_bytes(p->type) == 0 || out->d_leaves, "out->d_leaves");
-  size_t need = workspace_layout(p->n, nullptr, nullptr);
+  size_t need = workspace_layout(p->type, p->n, nullptr, nullptr);
   if (!d_ws || ws_bytes < need) {
     set_error("workspace too small: %zu < %zu", ws_bytes, need);
     return VRX_ERR_ARG;
   }
   Workspace ws;
-  workspace_layout(p->n, d_ws, &ws);
+  workspace_layout(p->type, p->n, d_ws, &ws);
   switch (p->type) {
   case VRX_TRIANGLE:
     return build_typed<VRX_TRIANGLE>(*p, *out, ws, codes_out, cbox_out, s);
@@ -852,7 +885,7 @@ void bvh_sizes(int32_t type, uint32_t n, vrx_bvh_sizes *o)
   o->leaves = (size_t)(n ? n : 1) * (leaf_record_bytes(type) ? leaf_record_bytes(type) : 16);
   o->leaf_ids = (size_t)(n ? n : 1) * 4;
   o->header = sizeof(vrx_bvh_header);
-  o->workspace = workspace_layout(n, nullptr, nullptr);
+  o->workspace = workspace_layout(type, n, nullptr, nullptr);
 }
 
 } // namespace vrx
