@@ -131,7 +131,7 @@ template <bool WIDE>
 static const void *pick_kernel_w(const vrx_scene &sc)
 {
   const bool two = sc.num_entries > 1;
-  switch (sc.single_type) {
+  switch (sc.num_volumes ? -1 : sc.single_type) { // volumes: generic kernel
   case VRX_TRIANGLE:
     return kernel_ptr<VRX_TRIANGLE, WIDE>(two);
   case VRX_SPHERE:

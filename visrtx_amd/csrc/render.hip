@@ -314,7 +314,9 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
     VRX_LAUNCH_CHECK();
     return VRX_OK;
   }
-  switch (scene->single_type) {
+  // the primitive-specialised kernels carry no volume code: scenes with
+  // volumes take the generic one
+  switch (scene->num_volumes ? -1 : scene->single_type) {
   case VRX_TRIANGLE:
     return launch_pt<VRX_TRIANGLE>(*scene, *cam, *r, *f, sd, cap, st);
   case VRX_SPHERE:

@@ -413,7 +413,7 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
         populate(sc, h, o, d, s);
         uint32_t vo = ~0u, vi = ~0u;
         float vdepth = s.t;
-        if (sc.num_volumes)
+        if (PT < 0 && sc.num_volumes) // scenes with volumes use the generic kernel
           vdepth = march_all(sc, o, d, tlower, s.t, r.inv_volume_sampling_rate, color, opacity,
               vo, vi, vsamples);
         if (first) {
@@ -472,7 +472,7 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
       } else {
         uint32_t vo = ~0u, vi = ~0u;
         float vdepth = tmax;
-        if (sc.num_volumes)
+        if (PT < 0 && sc.num_volumes) // scenes with volumes use the generic kernel
           vdepth = march_all(sc, o, d, tlower, tmax, r.inv_volume_sampling_rate, color, opacity,
               vo, vi, vsamples);
         if (first) {
