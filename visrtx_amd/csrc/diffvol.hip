@@ -100,58 +100,6 @@ struct Sample {
   int ci, cj, ck;   // the cell (i0, j0, k0)
 };
 
-// A sample split in two: dv_fetch computes the cell and issues the four
-// corner-pair loads, dv_finish interpolates.  The adjoint fetches sample k-1
-// before it scatters the gradients of sample k, so the gathers are in flight
-// during the scatter (which the compiler could not reorder them across).
-struct Fetch {
-  float2 p00, p10, p01, p11;
-  float f[3];
-  uint32_t base;
-  int i0[3];
-};
-
-__device__ __forceinline__ Fetch dv_fetch(const DvCtx &c, float gx, float gy, float gz)
-{
-  Fetch q;
-  float g[3] = {gx, gy, gz};
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    float x = fminf(fmaxf(g[k], 0.0f), c.gmax[k]);
-    int i = min((int)x, c.imax[k]);
-    q.i0[k] = i;
-    q.f[k] = x - (float)i;
-  }
-  q.base = (uint32_t)q.i0[0] + (uint32_t)q.i0[1] * c.sy + (uint32_t)q.i0[2] * c.sz;
-  const VRX_GLOBAL float *D = (const VRX_GLOBAL float *)c.density + q.base;
-  q.p00 = gld_pair(D);
-  q.p10 = gld_pair(D + c.sy);
-  q.p01 = gld_pair(D + c.sz);
-  q.p11 = gld_pair(D + c.sy + c.sz);
-  return q;
-}
-
-__device__ __forceinline__ Sample dv_finish(const Fetch &q)
-{
-  const float *f = q.f;
-  float c00 = q.p00.x + (q.p00.y - q.p00.x) * f[0];
-  float c10 = q.p10.x + (q.p10.y - q.p10.x) * f[0];
-  float c01 = q.p01.x + (q.p01.y - q.p01.x) * f[0];
-  float c11 = q.p11.x + (q.p11.y - q.p11.x) * f[0];
-  float c0 = c00 + (c10 - c00) * f[1];
-  float c1 = c01 + (c11 - c01) * f[1];
-  Sample sm;
-  sm.s = c0 + (c1 - c0) * f[2];
-  sm.fx = f[0];
-  sm.fy = f[1];
-  sm.fz = f[2];
-  sm.base = q.base;
-  sm.ci = q.i0[0];
-  sm.cj = q.i0[1];
-  sm.ck = q.i0[2];
-  return sm;
-}
-
 __device__ __forceinline__ Sample dv_sample(const DvCtx &c, float gx, float gy, float gz)
 {
   float g[3] = {gx, gy, gz};
@@ -526,20 +474,12 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
   const uint32_t itmax = s_itmax;
 
   // lockstep over the tile: step j processes sample N-1-j of every ray
-  Fetch pre{};
-  if (N > 0) {
-    const float t = t0 + (float)(N - 1) * c.step;
-    pre = dv_fetch(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
-  }
   for (uint32_t j0 = 0; j0 < itmax; j0 += kDrainEvery) {
     const uint32_t j1 = min(itmax, j0 + kDrainEvery);
     for (uint32_t j = j0; j < j1 && j < N; j++) {
       const int k = (int)(N - 1 - j);
-      const Sample sm = dv_finish(pre);
-      if (k > 0) {
-        const float tn = t0 + (float)(k - 1) * c.step;
-        pre = dv_fetch(c, gox + gdx * tn, goy + gdy * tn, goz + gdz * tn);
-      }
+      const float t = t0 + (float)k * c.step;
+      const Sample sm = dv_sample(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
       const Cls cl = dv_classify(s_lut, c, sm.s);
       const float tau = dv_tau(c, cl.co.w);
       float T;
