@@ -161,15 +161,19 @@ def bench_diffvol(iters=200, log_every=50):
     lut = torch.from_numpy(D.linear_lut(0.2)).cuda().requires_grad_(True)
 
     def timeit(fn, reps):
+        """median over reps of one call each, bracketed by HIP events"""
+        fn()
         fn()
         torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
+        ts = []
         for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
             fn()
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return statistics.median(ts)
 
     def fwd():
         with torch.no_grad():
@@ -179,8 +183,8 @@ def bench_diffvol(iters=200, log_every=50):
         loss = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean()
         loss.backward()
 
-    fwd_ms = timeit(fwd, 10)
-    fb_ms = timeit(fwdbwd, 10)
+    fwd_ms = timeit(fwd, 20)
+    fb_ms = timeit(fwdbwd, 20)
     opt = torch.optim.Adam([dens, lut], lr=1e-2)
     dens.grad = None
     lut.grad = None
