@@ -380,7 +380,11 @@ def main():
         value = rays / sec_per_step / 1e6
         kern_ms = statistics.mean(durs) * 1000.0
         log(f"primary raycast: {sec_per_step * 1000:.3f} ms/step wall, kernel {kern_ms:.3f} ms "
-            f"-> {value:.1f} Mrays/s (kernel-only {rays / (kern_ms / 1000) / 1e6:.1f})")
+            f"-> {value:.1f} Mrays/s (kernel-only {rays / (kern_ms / 1000) / 1e6:.1f}); tuned "
+            f"node width {dev.get_property(work.f, 'tunedNodeWidth', A.INT32)}, LDS stack "
+            f"{dev.get_property(work.f, 'tunedStackDepth', A.INT32)}")
+        extra["tuned_launch"] = {"node_width": dev.get_property(work.f, "tunedNodeWidth", A.INT32),
+                                 "stack_lds": dev.get_property(work.f, "tunedStackDepth", A.INT32)}
         extra["primary_kernel_mrays_s"] = round(rays / (kern_ms / 1000) / 1e6, 2)
         c2 = counts["spheres10m_1920x1080"]
         bytes_per_ray = 32 + 16 + 64 * c2["nodes_per_ray"] + 16 * c2["prims_per_ray"]
@@ -411,7 +415,7 @@ def main():
         # AO renderer, 16 spp
         ao = Workload(dev, sc, cam, W, H, "ao", ids=True, pixelSamples=16, ambientSamples=1,
                       ambientOcclusionDistance=0.02)
-        sps, ad = ao.timed(3, 1)
+        sps, ad = ao.timed(3, 5)   # warm-up covers the frame's launch autotuning
         nr = ao.rays()
         extra["ao16_mrays_s"] = round(nr / sps / 1e6, 2)
         extra["ao16_rays_per_frame"] = int(nr)
@@ -419,7 +423,7 @@ def main():
         log(f"AO 16spp: {sps * 1000:.2f} ms/frame, {nr} rays -> {extra['ao16_mrays_s']} Mrays/s")
         dev.set(ao.r, "nodeWidth", 2, A.INT32)
         dev.commit(ao.r)
-        sps2, _ = ao.timed(3, 1)
+        sps2, _ = ao.timed(3, 3)
         extra["ao16_mrays_s_width2"] = round(ao.rays() / sps2 / 1e6, 2)
         log(f"AO 16spp, BVH2 nodes: {sps2 * 1000:.2f} ms/frame -> "
             f"{extra['ao16_mrays_s_width2']} Mrays/s")

@@ -91,3 +91,35 @@ def test_accumulation_and_sample_limit(require_gpu):
     assert not dev.errors(), dev.errors()
     s.release()
     dev.close()
+
+
+def test_launch_autotuning_keeps_frames_identical(require_gpu):
+    """The frame times node width x LDS stack depth candidates on its first
+    frames and keeps the fastest; every frame's channels stay bit-identical."""
+    dev = A.Device()
+    s, cam = _scene(dev)
+    c = S.make_camera(dev, s, cam)
+    r = S.make_renderer(dev, s, "raycast")
+    f = S.make_frame(dev, s, s.world, c, r, 96, 80)
+    outs = []
+    for _ in range(7):
+        dev.commit(c)
+        dev.render(f)
+        dev.ready(f)
+        outs.append({ch: dev.map_frame(f, "channel." + ch)[0].copy()
+                     for ch in ("color", "depth", "primitiveId")})
+    for o in outs[1:]:
+        for ch in o:
+            np.testing.assert_array_equal(o[ch], outs[0][ch], err_msg=ch)
+    assert dev.get_property(f, "tunedNodeWidth", A.INT32) in (2, 4)
+    assert dev.get_property(f, "tunedStackDepth", A.INT32) in (16, 32)
+    # an explicit parameter is honoured, not tuned
+    dev.set(r, "nodeWidth", 2, A.INT32)
+    dev.commit(r)
+    for _ in range(4):
+        dev.render(f)
+        dev.ready(f)
+    assert dev.get_property(f, "tunedNodeWidth", A.INT32) == 2
+    assert not dev.errors(), dev.errors()
+    s.release()
+    dev.close()

@@ -1211,6 +1211,8 @@ void Renderer::commit()
   // 4-wide nodes by default: on MI355X they trace 6-22% faster than BVH2
   // (the world builds them once, lazily, from the BVH2)
   r.node_width = get<int>("nodeWidth", ANARI_INT32, 4) == 2 ? 2 : 4;
+  tuneWidth = find("nodeWidth") == nullptr;
+  tuneStack = find("stackDepthLDS") == nullptr;
   sampleLimit = get<int>("sampleLimit", ANARI_INT32, 128);
   if (kind == VRX_RENDER_RAYCAST)
     sampleLimit = 1; // single-shot renderer
@@ -1331,6 +1333,42 @@ void Frame::wait()
   }
 }
 
+void Frame::pickLaunch(vrx_renderer &rr)
+{
+  uint64_t key = (uint64_t)(uintptr_t)world ^ (world->builtAt << 20) ^ (uint64_t)(uintptr_t)renderer
+      ^ (renderer->lastFinalized << 40) ^ ((uint64_t)size[0] << 8) ^ ((uint64_t)size[1] << 28);
+  if (key != tune.key) {
+    tune = Tune{};
+    tune.key = key;
+    const int widths[2] = {4, 2}, sds[2] = {16, 32};
+    const int nw = renderer->tuneWidth ? 2 : 1, ns = renderer->tuneStack ? 2 : 1;
+    for (int a = 0; a < ns; a++)
+      for (int b = 0; b < nw; b++) {
+        tune.width[tune.n] = renderer->tuneWidth ? widths[b] : rr.node_width;
+        tune.sd[tune.n] = renderer->tuneStack ? sds[a] : rr.stack_lds;
+        tune.n++;
+      }
+  } else if (tune.best < 0 && tune.last >= 0) {
+    // the previous render used candidate `last`; wait() has made its events
+    // complete
+    float ms = 0.0f;
+    if (vrx_event_elapsed_ms(evStart, evEnd, &ms) == 0)
+      tune.ms[tune.last] = ms;
+    if (tune.next >= tune.n) {
+      tune.best = 0;
+      for (int i = 1; i < tune.n; i++)
+        if (tune.ms[i] < tune.ms[tune.best])
+          tune.best = i;
+    }
+  }
+  const int use = tune.best >= 0 ? tune.best : tune.next;
+  if (tune.best < 0)
+    tune.next++;
+  tune.last = tune.best >= 0 ? -1 : use;
+  rr.node_width = tune.width[use];
+  rr.stack_lds = tune.sd[use];
+}
+
 void Frame::render()
 {
   wait();
@@ -1354,8 +1392,6 @@ void Frame::render()
     allocate();
   if (!allocated)
     return;
-  if (renderer->r.node_width == 4 && world->ensureWide() != 0)
-    return;
   // Frame.cu:574-588 accumulation reset on any finalize since the last frame
   if (dev->lastFinalize > lastSeenFinalize) {
     lastSeenFinalize = dev->lastFinalize;
@@ -1370,6 +1406,10 @@ void Frame::render()
   } else {
     frameID += (uint32_t)spp;
   }
+  vrx_renderer rr = renderer->r;
+  pickLaunch(rr);
+  if (rr.node_width == 4 && world->ensureWide() != 0)
+    return;
   vrx_frame f{};
   f.width = size[0];
   f.height = size[1];
@@ -1384,7 +1424,7 @@ void Frame::render()
   f.d_prim_id = (uint32_t *)dPrim;
   f.d_obj_id = (uint32_t *)dObj;
   f.d_inst_id = (uint32_t *)dInst;
-  size_t need = vrx_render_scratch_bytes(&world->scene, &renderer->r, f.width, f.height);
+  size_t need = vrx_render_scratch_bytes(&world->scene, &rr, f.width, f.height);
   if (need > scratchBytes) {
     vrx_free(dScratch);
     dScratch = nullptr;
@@ -1399,7 +1439,7 @@ void Frame::render()
   f.d_counters = (uint64_t *)dCounters;
   vrx_fill_u32(dCounters, 0u, 4, dev->stream);
   vrx_event_record(evStart, dev->stream);
-  dev->check(vrx_render(&world->scene, &camera->cam, &renderer->r, &f, dev->stream), this, "render");
+  dev->check(vrx_render(&world->scene, &camera->cam, &rr, &f, dev->stream), this, "render");
   vrx_event_record(evEnd, dev->stream);
   if (hReadback) {
     vrx_memcpy(hReadback, dCounters, 16, VRX_D2H, dev->stream);
@@ -1495,6 +1535,14 @@ bool Frame::getProperty(const std::string &name, ANARIDataType t, void *mem, uin
     if (mask & ANARI_WAIT)
       wait();
     int v = (nextReset || dev->lastFinalize > lastSeenFinalize) ? 1 : 0;
+    memcpy(mem, &v, 4);
+    return true;
+  }
+  if ((name == "tunedNodeWidth" || name == "tunedStackDepth") && t == ANARI_INT32 && sz >= 4) {
+    // the launch configuration the autotuner settled on (0 while tuning)
+    int v = 0;
+    if (tune.best >= 0)
+      v = name == "tunedNodeWidth" ? tune.width[tune.best] : tune.sd[tune.best];
     memcpy(mem, &v, 4);
     return true;
   }

@@ -246,6 +246,8 @@ struct Renderer : Object {
   int kind{-1};
   vrx_renderer r{};
   int sampleLimit{128};
+  // launch parameters left to the frame's autotuner (not set explicitly)
+  bool tuneWidth{true}, tuneStack{true};
   int spp() const { return r.kind == VRX_RENDER_AO ? (r.pixel_samples > 1 ? r.pixel_samples : 1) : 1; }
 };
 
@@ -276,6 +278,17 @@ struct Frame : Object {
   vrx_event evDone{nullptr}; // after the result read-backs that follow evEnd
   bool rendered{false};
   bool stackReported{false};
+  // Online launch autotuning: the first frames of a (world, renderer, size)
+  // configuration each use one candidate (node width x LDS stack depth), timed
+  // by the frame's events; then the fastest is kept.  Every candidate gives
+  // bit-identical channels, so tuning only changes the time.
+  struct Tune {
+    uint64_t key{0};
+    int n{0}, next{0}, best{-1}, last{-1};
+    int width[4]{}, sd[4]{};
+    float ms[4]{};
+  } tune;
+  void pickLaunch(vrx_renderer &rr);
   struct CallbackCtx {
     ANARIFrameCompletionCallback cb;
     const void *user;
