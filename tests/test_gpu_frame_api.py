@@ -117,6 +117,7 @@ def test_launch_autotuning_keeps_frames_identical(require_gpu):
     dev.set(r, "nodeWidth", 2, A.INT32)
     dev.commit(r)
     for _ in range(4):
+        dev.commit(c)                   # raycast frames render once per camera commit
         dev.render(f)
         dev.ready(f)
     assert dev.get_property(f, "tunedNodeWidth", A.INT32) == 2
