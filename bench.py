@@ -85,7 +85,34 @@ class Workload:
     def samples(self):
         return self.dev.get_property(self.f, "volumeSamples", self.A.UINT64)
 
+    def tune(self, max_frames=12):
+        """Untimed frames until the frame's launch autotuner (node width x LDS
+        stack depth, one candidate per frame) has settled."""
+        # the property describes the latest render, so a parameter change shows
+        # only after one frame has been rendered with it
+        self.step()
+        for _ in range(max_frames):
+            if self.dev.get_property(self.f, "tunedNodeWidth", self.A.INT32):
+                break
+            self.step()
+        device_sync()
+
+    def tuned(self):
+        return (self.dev.get_property(self.f, "tunedNodeWidth", self.A.INT32),
+                self.dev.get_property(self.f, "tunedStackDepth", self.A.INT32))
+
+    def pin_other_width(self):
+        """Renderer parameters for the node width the autotuner did not pick,
+        at the tuned LDS stack depth; returns that width."""
+        w, sd = self.tuned()
+        other = 2 if w == 4 else 4
+        self.dev.set(self.r, "nodeWidth", other, self.A.INT32)
+        self.dev.set(self.r, "stackDepthLDS", sd, self.A.INT32)
+        self.dev.commit(self.r)
+        return other
+
     def timed(self, steps, warmup):
+        self.tune()
         for _ in range(warmup):
             self.step()
         device_sync()
@@ -379,54 +406,56 @@ def main():
         sec_per_step, durs = work.timed(args.steps, args.warmup)
         value = rays / sec_per_step / 1e6
         kern_ms = statistics.mean(durs) * 1000.0
+        tw, tsd = work.tuned()
         log(f"primary raycast: {sec_per_step * 1000:.3f} ms/step wall, kernel {kern_ms:.3f} ms "
             f"-> {value:.1f} Mrays/s (kernel-only {rays / (kern_ms / 1000) / 1e6:.1f}); tuned "
-            f"node width {dev.get_property(work.f, 'tunedNodeWidth', A.INT32)}, LDS stack "
-            f"{dev.get_property(work.f, 'tunedStackDepth', A.INT32)}")
-        extra["tuned_launch"] = {"node_width": dev.get_property(work.f, "tunedNodeWidth", A.INT32),
-                                 "stack_lds": dev.get_property(work.f, "tunedStackDepth", A.INT32)}
+            f"node width {tw}, LDS stack {tsd}")
+        extra["tuned_launch"] = {"node_width": tw, "stack_lds": tsd}
         extra["primary_kernel_mrays_s"] = round(rays / (kern_ms / 1000) / 1e6, 2)
         c2 = counts["spheres10m_1920x1080"]
         bytes_per_ray = 32 + 16 + 64 * c2["nodes_per_ray"] + 16 * c2["prims_per_ray"]
         alg_bytes = rays * bytes_per_ray
         achieved = alg_bytes / (kern_ms / 1000) / 1e9
         traffic = load_traffic("render_spheres10m_1920x1080")
+        # HBM bytes of the kernel instance the autotuner picked (both node
+        # widths are profiled)
+        inst = f"k_render<1, false, 0, {'true' if tw == 4 else 'false'}>"
+        tb = [v for k, v in (traffic or {}).get("per_kernel", {}).items() if inst in k]
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
-                    "kernel": "k_render<sphere,single-level,raycast,4-wide nodes>",
+                    "traffic": tb[0] if tb else None,
+                    "kernel": f"k_render<sphere,single-level,raycast,{tw}-wide nodes>",
+                    "stack_lds": tsd,
                     "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
                     "avg_kernel_ms": round(kern_ms, 4)}
         extra["widen_ms_10m_spheres"] = round(
             1000.0 * dev.get_property(work.s.world, "bvhWidenDuration", A.FLOAT32), 3)
-        # the same frames traversing the BVH2 directly (renderer nodeWidth = 2)
-        dev.set(work.r, "nodeWidth", 2, A.INT32)
-        dev.commit(work.r)
+        # the same frames with the node width the autotuner did not pick
+        ow = work.pin_other_width()
         sps2, durs2 = work.timed(args.steps, args.warmup)
         k2 = statistics.mean(durs2) * 1000.0
-        extra["primary_kernel_mrays_s_width2"] = round(rays / (k2 / 1000) / 1e6, 2)
-        extra["primary_wall_mrays_s_width2"] = round(rays / sps2 / 1e6, 2)
-        log(f"primary raycast, BVH2 nodes: kernel {k2:.3f} ms -> "
-            f"{extra['primary_kernel_mrays_s_width2']} Mrays/s (4-wide widen "
+        extra[f"primary_kernel_mrays_s_width{ow}"] = round(rays / (k2 / 1000) / 1e6, 2)
+        extra[f"primary_wall_mrays_s_width{ow}"] = round(rays / sps2 / 1e6, 2)
+        log(f"primary raycast, {ow}-wide nodes: kernel {k2:.3f} ms -> "
+            f"{extra[f'primary_kernel_mrays_s_width{ow}']} Mrays/s (4-wide widen "
             f"{extra['widen_ms_10m_spheres']} ms)")
-        dev.set(work.r, "nodeWidth", 4, A.INT32)
-        dev.commit(work.r)
     if "ao" in sec:
         # AO renderer, 16 spp
         ao = Workload(dev, sc, cam, W, H, "ao", ids=True, pixelSamples=16, ambientSamples=1,
                       ambientOcclusionDistance=0.02)
-        sps, ad = ao.timed(3, 5)   # warm-up covers the frame's launch autotuning
+        sps, ad = ao.timed(3, 1)
         nr = ao.rays()
         extra["ao16_mrays_s"] = round(nr / sps / 1e6, 2)
         extra["ao16_rays_per_frame"] = int(nr)
         extra["ao16_ms_per_frame"] = round(sps * 1000, 3)
-        log(f"AO 16spp: {sps * 1000:.2f} ms/frame, {nr} rays -> {extra['ao16_mrays_s']} Mrays/s")
-        dev.set(ao.r, "nodeWidth", 2, A.INT32)
-        dev.commit(ao.r)
-        sps2, _ = ao.timed(3, 3)
-        extra["ao16_mrays_s_width2"] = round(ao.rays() / sps2 / 1e6, 2)
-        log(f"AO 16spp, BVH2 nodes: {sps2 * 1000:.2f} ms/frame -> "
-            f"{extra['ao16_mrays_s_width2']} Mrays/s")
+        extra["ao16_tuned_launch"] = dict(zip(("node_width", "stack_lds"), ao.tuned()))
+        log(f"AO 16spp: {sps * 1000:.2f} ms/frame, {nr} rays -> {extra['ao16_mrays_s']} Mrays/s "
+            f"(tuned {extra['ao16_tuned_launch']})")
+        ow = ao.pin_other_width()
+        sps2, _ = ao.timed(3, 1)
+        extra[f"ao16_mrays_s_width{ow}"] = round(ao.rays() / sps2 / 1e6, 2)
+        log(f"AO 16spp, {ow}-wide nodes: {sps2 * 1000:.2f} ms/frame -> "
+            f"{extra[f'ao16_mrays_s_width{ow}']} Mrays/s")
         ao.release()
     if "headline" in sec or "ao" in sec:
         work.release()
@@ -438,10 +467,10 @@ def main():
         sps, d1 = hf.timed(20, 3)
         extra["heightfield_mrays_s"] = round(1024 * 768 / sps / 1e6, 2)
         extra["heightfield_kernel_ms"] = round(statistics.mean(d1) * 1000, 4)
-        dev.set(hf.r, "nodeWidth", 2, A.INT32)
-        dev.commit(hf.r)
+        extra["heightfield_tuned_launch"] = dict(zip(("node_width", "stack_lds"), hf.tuned()))
+        ow = hf.pin_other_width()
         _, d12 = hf.timed(20, 3)
-        extra["heightfield_kernel_ms_width2"] = round(statistics.mean(d12) * 1000, 4)
+        extra[f"heightfield_kernel_ms_width{ow}"] = round(statistics.mean(d12) * 1000, 4)
         log(f"heightfield: {sps * 1000:.3f} ms/step -> {extra['heightfield_mrays_s']} Mrays/s")
         hf.release()
 
