@@ -212,6 +212,13 @@ def bench_diffvol(iters=200, log_every=50):
 
     fwd_ms = timeit(fwd, 20)
     fb_ms = timeit(fwdbwd, 20)
+    # the same step with the backward replayed from the transmittance checkpoints
+    spec_suffix = spec
+    spec = D.GridSpec(**{**spec_suffix.__dict__, "adjoint": "replay"})
+    fb_replay_ms = timeit(fwdbwd, 20)
+    spec = spec_suffix
+    dens.grad = None
+    lut.grad = None
     opt = torch.optim.Adam([dens, lut], lr=1e-2)
     dens.grad = None
     lut.grad = None
@@ -234,7 +241,10 @@ def bench_diffvol(iters=200, log_every=50):
     with torch.no_grad():
         last = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean().item()
     return {"fwd_ms": round(fwd_ms, 3), "fwd_bwd_ms": round(fb_ms, 3),
-            "fwd_bwd_over_fwd": round(fb_ms / fwd_ms, 3), "adam_ms_per_iter": round(iter_ms, 3),
+            "fwd_bwd_over_fwd": round(fb_ms / fwd_ms, 3), "adjoint": "suffix",
+            "fwd_bwd_ms_replay_adjoint": round(fb_replay_ms, 3),
+            "fwd_bwd_over_fwd_replay_adjoint": round(fb_replay_ms / fwd_ms, 3),
+            "adam_ms_per_iter": round(iter_ms, 3),
             "iters": iters, "mse_first": first, "mse_final": last, "rays": int(org.shape[0])}
 
 
@@ -509,7 +519,8 @@ def main():
         dv = bench_diffvol()
         extra["diffvol_128"] = dv
         log(f"diffvol: fwd {dv['fwd_ms']} ms, fwd+bwd {dv['fwd_bwd_ms']} ms "
-            f"({dv['fwd_bwd_over_fwd']}x), Adam {dv['adam_ms_per_iter']} ms/iter, "
+            f"({dv['fwd_bwd_over_fwd']}x; checkpoint-replay adjoint "
+            f"{dv['fwd_bwd_ms_replay_adjoint']} ms), Adam {dv['adam_ms_per_iter']} ms/iter, "
             f"MSE {dv['mse_first']:.3e} -> {dv['mse_final']:.3e}")
 
     cpu = None

@@ -267,7 +267,15 @@ typedef struct vrx_diffvol {
    * each workgroup a 16x16 pixel tile so neighbouring rays share gradient
    * scatter.  0 = rays in no particular order. */
   uint32_t image_w, image_h;
+  int32_t adjoint;         /* vrx_adjoint: backward schedule */
 } vrx_diffvol;
+
+/* Backward schedules of vrx_diffvol_backward (same gradients):
+ *  SUFFIX  the march is replayed forward; the suffix sum that the adjoint
+ *          needs is the forward's FP64 total minus an FP64 prefix (default);
+ *  REPLAY  the march is replayed backward from the transmittance stored
+ *          every `checkpoint` steps. */
+enum vrx_adjoint { VRX_ADJOINT_SUFFIX = 0, VRX_ADJOINT_REPLAY = 1 };
 
 /* rays: n origins/directions (float3) */
 int vrx_diffvol_forward(const vrx_diffvol *v, uint32_t n, const float *d_org, const float *d_dir,

@@ -19,12 +19,12 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _setup(n, views, res, lut, early, seed=3, tiled=True):
+def _setup(n, views, res, lut, early, seed=3, tiled=True, adjoint="suffix"):
     from visrtx_amd import diffvol as D
     dens = S.gen_noise_volume(n, seed=seed)
     sp = 1.0 / (n - 1)
     spec = D.GridSpec(dims=(n, n, n), spacing=(sp, sp, sp), unit_distance=sp, step=0.5 * sp,
-                      early_stop=early, image=(res, res) if tiled else None)
+                      early_stop=early, image=(res, res) if tiled else None, adjoint=adjoint)
     cams = D.orbit_cameras(views, seed=seed, fovy=45.0)
     org, dirs = D.make_rays(cams, res, res)
     td = torch.from_numpy(dens).cuda()
@@ -48,11 +48,14 @@ def test_forward_matches_fp64_oracle(require_gpu):
     assert ref[:, 3].max() > 0.5
 
 
-def test_adjoint_vs_fp64_central_differences(require_gpu):
-    """1000 parameters (600 voxels, 400 LUT scalars): relative error <= 1e-3."""
+@pytest.mark.parametrize("adjoint", ["suffix", "replay"])
+def test_adjoint_vs_fp64_central_differences(require_gpu, adjoint):
+    """1000 parameters (600 voxels, 400 LUT scalars): relative error <= 1e-3,
+    for both backward schedules."""
     from visrtx_amd import diffvol as D
     lut = D.linear_lut(alpha_max=0.5)
-    D, spec, td, tl, org, dirs, dens = _setup(24, 2, 32, lut, early=False, seed=4)
+    D, spec, td, tl, org, dirs, dens = _setup(24, 2, 32, lut, early=False, seed=4, adjoint=adjoint)
+    print(f"[diffvol] FD check, adjoint={adjoint}", flush=True)
     n = org.shape[0]
     rng = np.random.default_rng(0)
     w = rng.normal(size=(n, 4))
@@ -126,3 +129,48 @@ def test_adam_reduces_loss(require_gpu):
         losses.append(loss.item())
     print(f"[diffvol] loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     assert losses[-1] < 0.5 * losses[0]
+
+
+def _grads(D, spec, td, tl, org, dirs, w):
+    td = td.detach().clone().requires_grad_(True)
+    tl = tl.detach().clone().requires_grad_(True)
+    rgba = D.render(td, tl, org, dirs, spec)
+    (rgba * w).sum().backward()
+    torch.cuda.synchronize()
+    return td.grad.cpu().numpy().reshape(-1), tl.grad.cpu().numpy().reshape(-1)
+
+
+def _close(a, b, tol):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30) <= tol
+
+
+def test_suffix_and_replay_adjoints_agree(require_gpu):
+    """Both schedules give the same gradients (early termination on, a 40x40
+    image that leaves partial workgroup tiles, rays that miss the grid), and
+    the tiled and untiled ray layouts agree."""
+    from visrtx_amd import diffvol as D
+    lut = orc.tf_discretize(S.COLORMAP5, np.array([0.0, 0.6], np.float32))
+    D, spec, td, tl, org, dirs, dens = _setup(32, 3, 40, lut, early=True, seed=8)
+    print(f"[diffvol] suffix vs replay, {org.shape[0]} rays", flush=True)
+    w = torch.from_numpy(np.random.default_rng(1).normal(size=(org.shape[0], 4))).float().cuda()
+    gd_s, gl_s = _grads(D, spec, td, tl, org, dirs, w)
+    gd_r, gl_r = _grads(D, D.GridSpec(**{**spec.__dict__, "adjoint": "replay"}), td, tl, org, dirs, w)
+    gd_u, gl_u = _grads(D, D.GridSpec(**{**spec.__dict__, "image": None}), td, tl, org, dirs, w)
+    assert np.abs(gd_s).max() > 0 and np.abs(gl_s).max() > 0
+    assert _close(gd_s, gd_r, 1e-5) and _close(gl_s, gl_r, 1e-5)
+    assert _close(gd_u, gd_s, 1e-6) and _close(gl_u, gl_s, 1e-6)
+
+
+def test_suffix_adjoint_fixed_point_scale(require_gpu):
+    """The suffix schedule sums gradients in 64-bit fixed point whose scale
+    follows the workgroup's output gradients: scaling the loss by 1e-12 or
+    1e+12 scales the gradients by the same factor, with no loss of digits."""
+    from visrtx_amd import diffvol as D
+    lut = D.linear_lut(alpha_max=0.5)
+    D, spec, td, tl, org, dirs, dens = _setup(24, 2, 32, lut, early=True, seed=9)
+    w = torch.from_numpy(np.random.default_rng(2).normal(size=(org.shape[0], 4))).float().cuda()
+    gd, gl = _grads(D, spec, td, tl, org, dirs, w)
+    for f in (1e-12, 1e12):
+        print(f"[diffvol] fixed-point scale, loss x {f:g}", flush=True)
+        gdf, glf = _grads(D, spec, td, tl, org, dirs, w * f)
+        assert _close(gdf / f, gd, 1e-6) and _close(glf / f, gl, 1e-6)

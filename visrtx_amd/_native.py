@@ -102,12 +102,15 @@ class vrx_batch(C.Structure):
                 ("persistent_blocks", C.c_int32)]
 
 
+VRX_ADJOINT_SUFFIX, VRX_ADJOINT_REPLAY = 0, 1
+
+
 class vrx_diffvol(C.Structure):
     _fields_ = [("d_density", C.c_void_p), ("dims", C.c_uint32 * 3), ("origin", f3),
                 ("spacing", f3), ("d_lut", C.c_void_p), ("value_range", C.c_float * 2),
                 ("unit_distance", C.c_float), ("step", C.c_float), ("early_stop", C.c_int32),
                 ("checkpoint", C.c_int32), ("max_steps", C.c_uint32),
-                ("image_w", C.c_uint32), ("image_h", C.c_uint32)]
+                ("image_w", C.c_uint32), ("image_h", C.c_uint32), ("adjoint", C.c_int32)]
 
 
 _hip = None

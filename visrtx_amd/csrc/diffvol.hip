@@ -5,19 +5,26 @@
 //   C = sum_k T_k (1 - tau_k) c_k,  O = sum_k T_k (1 - tau_k),
 //   T_{k+1} = T_k tau_k,  tau_k = (1 - a_k)^(step / unitDistance).
 // The reference has only this forward pass; the adjoint is new.  The forward
-// stores the transmittance every `checkpoint` (64) steps plus the sample
-// count and the final transmittance -- never every sample.
+// stores, per ray, the sample count, the final transmittance, the
+// transmittance every `checkpoint` (64) steps and the FP64 sums of w_k c_k
+// and w_k -- never every sample.
 //
 // Adjoint: with v_k = G.c_k + gO (G = dL/dC, gO = dL/dO) and the suffix
 //   R_k = sum_{j>k} (w_j / T_{k+1}) v_j,   R_{k-1} = (1 - tau_k) v_k + tau_k R_k,
-// the gradients are dL/dtau_k = T_k (R_k - v_k), dL/dc_k = w_k G.  The march is
-// replayed backward from the last sample; T_k is re-anchored at every
-// checkpoint and recovered inside a segment as T_{k+1} / tau_k (a forward
-// replay from the segment's checkpoint is used when tau_k is too small for
-// that division).  LUT gradients are pre-reduced in LDS per workgroup;
-// density gradients are accumulated per trilinear cell in registers and
-// scattered with hardware float atomics when the ray leaves the cell.
+// the gradients are dL/dtau_k = T_k (R_k - v_k), dL/dc_k = w_k G.  Two
+// schedules compute them (vrx_diffvol.adjoint):
+//  * replay: the march is replayed backward from the last sample; T_k is
+//    re-anchored at every checkpoint and recovered inside a segment as
+//    T_{k+1} / tau_k (a forward replay from the segment's checkpoint is used
+//    when tau_k is too small for that division);
+//  * suffix (default): the march is replayed forward, in the forward's own
+//    order, and T_{k+1} R_k = sum_{j>k} w_j v_j is the forward's FP64 total
+//    minus the FP64 prefix, so no division by tau and no checkpoint reads.
+// Both accumulate the density gradient per trilinear cell in registers and
+// flush it when the ray leaves the cell; LUT gradients are pre-reduced in LDS
+// per workgroup.
 #include <float.h>
+#include <limits.h>
 #include <string.h>
 
 #include <algorithm>
@@ -37,7 +44,8 @@ struct DvCtx {
   float origin[3], inv_sp[3], lo_b[3], hi_b[3];
   uint32_t sy, sz;
   float step, expo, lo, hi, inv_range;
-  uint32_t nck;        // checkpoint slots per ray (2 + segments)
+  uint32_t nck;        // checkpoint slots per ray (2 + segments); the FP64
+                       // totals follow in slots nck .. nck + 7
   int ckpt;            // steps per checkpoint
   int early;
   // ray layout of the adjoint's workgroups: 16x16 pixel tiles of images of
@@ -45,7 +53,7 @@ struct DvCtx {
   uint32_t img_w, img_h, tiles_x, tiles_per_img;
 };
 
-static DvCtx make_ctx(const vrx_diffvol &v)
+static DvCtx make_ctx(const vrx_diffvol &v, uint32_t tile_w = 16)
 {
   DvCtx c;
   c.density = v.d_density;
@@ -70,7 +78,7 @@ static DvCtx make_ctx(const vrx_diffvol &v)
   c.early = v.early_stop;
   c.img_w = v.image_w;
   c.img_h = v.image_h;
-  c.tiles_x = (v.image_w + 15) / 16;
+  c.tiles_x = (v.image_w + tile_w - 1) / tile_w;
   c.tiles_per_img = c.tiles_x * ((v.image_h + 15) / 16);
   return c;
 }
@@ -177,6 +185,7 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_fwd(DvCtx c, const float *
     return;
   const v3 o = gld3(org, r), d = gld3(dir, r);
   float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, O = 0.0f, T = 1.0f;
+  double D0 = 0.0, D1 = 0.0, D2 = 0.0, DO = 0.0; // FP64 sums of w c and w
   uint32_t N = 0;
   float t0, t1;
   if (dv_slab(c, o, d, t0, t1) && c.step > 0.0f) {
@@ -198,12 +207,24 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_fwd(DvCtx c, const float *
       C1 += w * cl.co.y;
       C2 += w * cl.co.z;
       O += w;
+      const double wd = (double)w;
+      D0 = fma(wd, (double)cl.co.x, D0);
+      D1 = fma(wd, (double)cl.co.y, D1);
+      D2 = fma(wd, (double)cl.co.z, D2);
+      DO += wd;
       T *= tau;
       N = k + 1;
     }
   }
   gst(ckpt, (size_t)0 * n + r, __uint_as_float(N));
   gst(ckpt, (size_t)1 * n + r, T);
+  const double tot[4] = {D0, D1, D2, DO};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(tot[q]);
+    gst(ckpt, (size_t)(c.nck + 2 * q) * n + r, __uint_as_float((uint32_t)b));
+    gst(ckpt, (size_t)(c.nck + 2 * q + 1) * n + r, __uint_as_float((uint32_t)(b >> 32)));
+  }
   gst((float4 *)rgba, r, make_float4(C0, C1, C2, O));
 }
 
@@ -369,7 +390,9 @@ __device__ __forceinline__ void lut_put(float *s_glut, int e, const float g[4])
   }
 }
 
-__device__ __forceinline__ void lut_enter(float *s_glut, LutAcc &L, int i)
+// move the carried interval to i; Flush(e, g[4]) adds g to LUT entry e
+template <class Flush>
+__device__ __forceinline__ void lut_enter(Flush flush, LutAcc &L, int i)
 {
   if (i == L.i)
     return;
@@ -382,8 +405,8 @@ __device__ __forceinline__ void lut_enter(float *s_glut, LutAcc &L, int i)
       flo[ch] = down ? 0.0f : L.lo[ch];
       fhi[ch] = up ? 0.0f : L.hi[ch];
     }
-    lut_put(s_glut, L.i, flo);
-    lut_put(s_glut, L.i + 1, fhi);
+    flush(L.i, flo);
+    flush(L.i + 1, fhi);
   }
 #pragma unroll
   for (int ch = 0; ch < 4; ch++) {
@@ -394,7 +417,7 @@ __device__ __forceinline__ void lut_enter(float *s_glut, LutAcc &L, int i)
   L.i = i;
 }
 
-__global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *lut_g, uint32_t n,
+__global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd_replay(DvCtx c, const float *lut_g, uint32_t n,
     const float *org, const float *dir, const float *grad_rgba, const float *ckpt,
     float *grad_density, float *lut_partials)
 {
@@ -504,7 +527,7 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
       const float da = dtau * (-c.expo) * tau * __builtin_amdgcn_rcpf(oma);
       const float dr = w * gx, dg = w * gy, db = w * gz;
       // LUT: entries i and i+1 with weights (1-f) and f
-      lut_enter(my_glut, L, cl.i);
+      lut_enter([&](int e, const float g[4]) { lut_put(my_glut, e, g); }, L, cl.i);
       const float wf0 = 1.0f - cl.f, wf1 = cl.f;
       L.lo[0] += wf0 * dr;
       L.lo[1] += wf0 * dg;
@@ -550,6 +573,509 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd(DvCtx c, const float *
       g += s_glut[cp * kGlutStride + i];
     gst(dst, i, g);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Suffix adjoint.  A workgroup is kSfxWaves waves; each wave traces one 8x8
+// pixel tile and steps its rays in lockstep (sample k of a ray at step
+// off + k; off = 0, or the entry distance in steps when the rays' entry cells
+// spread over more than kAlignBox voxels, as on a face seen at a grazing
+// angle).  Every chunk of up to kSfxChunk steps, the wave takes the box of
+// voxels its samples touch; if the box fits its LDS window (kWinCap
+// entries), the carried corner gradients it flushes are added there, as
+// 64-bit fixed-point integers -- LDS integer atomics are cheap, LDS float
+// atomics are not -- and the window is added to the grid at the end of the
+// chunk with row-contiguous float atomics.  A box that does not fit is
+// halved in steps, down to one step; beyond that the flushes go to the grid.
+// The fixed-point scale of each workgroup comes from a bound on any window
+// or LUT sum built from its rays' largest output gradient and the LUT range,
+// so sums cannot overflow; the resolution is 2^-61 of that bound.
+// ---------------------------------------------------------------------------
+constexpr int kSfxWaves = 8;
+constexpr int kSfxThreads = 64 * kSfxWaves;
+constexpr int kWinCap = 1024;
+constexpr int kSfxChunk = 16;
+constexpr int kAlignBox = 256;
+
+// |x| 2^e rounded to an integer, as a 64-bit two's complement word.  With
+// |x| 2^e < 2^62 the integer has at most 24 significant bits, so its two
+// 32-bit halves are split exactly in float arithmetic.
+__device__ __forceinline__ unsigned long long to_fix(float x, float scale)
+{
+  const float y = rintf(fabsf(x) * scale);
+  const float hf = truncf(y * 2.3283064365386963e-10f); // 2^-32
+  const uint32_t hi = (uint32_t)hf;
+  const uint32_t lo = (uint32_t)(y - hf * 4294967296.0f);
+  const unsigned long long v = ((unsigned long long)hi << 32) | lo;
+  return x < 0.0f ? (unsigned long long)(-(long long)v) : v;
+}
+
+__device__ __forceinline__ float from_fix(unsigned long long q, double inv_scale)
+{
+  return (float)((double)(long long)q * inv_scale);
+}
+
+struct WinSink {
+  unsigned long long *win; // this wave's window
+  float *gd;
+  float scale;             // 2^e of the fixed-point words
+  int ox, oy, oz;          // box origin (voxel coordinates)
+  int sx, sxy;             // box row and slice pitch
+  bool lds;                // the box fits the window
+};
+
+struct WinCarry {
+  float a[8];     // corner q = x + 2y + 4z of the current cell
+  uint32_t base;  // grid index of corner 0
+  int lbase;      // window index of corner 0 (while the sink is the window)
+  int ci, cj, ck; // the cell
+  bool live;
+};
+
+__device__ __forceinline__ int win_index(const WinSink &S, int i, int j, int k)
+{
+  return (i - S.ox) + (j - S.oy) * S.sx + (k - S.oz) * S.sxy;
+}
+
+template <bool LDS>
+__device__ __forceinline__ void win_put(const WinSink &S, int lidx, uint32_t gidx, float x)
+{
+  if (LDS)
+    __hip_atomic_fetch_add(S.win + lidx, to_fix(x, S.scale), __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_WORKGROUP);
+  else if (x != 0.0f)
+    unsafeAtomicAdd(S.gd + gidx, x);
+}
+
+// move the cell one step along AXIS (UP: +1, else -1), flushing the face left
+// behind; rays of a tile are nearly parallel, so a wave rarely takes both
+template <int AXIS, bool UP, bool LDS>
+__device__ __forceinline__ void win_shift(const DvCtx &c, const WinSink &S, WinCarry &A)
+{
+  constexpr int bit = 1 << AXIS;
+  const uint32_t stride = AXIS == 0 ? 1u : (AXIS == 1 ? c.sy : c.sz);
+  const int lstride = AXIS == 0 ? 1 : (AXIS == 1 ? S.sx : S.sxy);
+#pragma unroll
+  for (int q0 = 0; q0 < 8; q0++) {
+    if (q0 & bit)
+      continue;
+    const int q1 = q0 | bit;
+    const int qf = UP ? q0 : q1; // the corner left behind
+    const uint32_t go = (uint32_t)(qf & 1) + (uint32_t)((qf >> 1) & 1) * c.sy
+        + (uint32_t)((qf >> 2) & 1) * c.sz;
+    const int lo = (qf & 1) + ((qf >> 1) & 1) * S.sx + ((qf >> 2) & 1) * S.sxy;
+    win_put<LDS>(S, A.lbase + lo, A.base + go, A.a[qf]);
+    if (UP) {
+      A.a[q0] = A.a[q1];
+      A.a[q1] = 0.0f;
+    } else {
+      A.a[q1] = A.a[q0];
+      A.a[q0] = 0.0f;
+    }
+  }
+  A.base = UP ? A.base + stride : A.base - stride;
+  A.lbase = UP ? A.lbase + lstride : A.lbase - lstride;
+}
+
+template <int AXIS, bool LDS>
+__device__ __forceinline__ void win_step(const DvCtx &c, const WinSink &S, WinCarry &A, int d)
+{
+  if (d > 0)
+    win_shift<AXIS, true, LDS>(c, S, A);
+  else if (d < 0)
+    win_shift<AXIS, false, LDS>(c, S, A);
+}
+
+template <bool LDS>
+__device__ __forceinline__ void win_flush_cell(const DvCtx &c, const WinSink &S, WinCarry &A)
+{
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const uint32_t go = (uint32_t)(q & 1) + (uint32_t)((q >> 1) & 1) * c.sy
+        + (uint32_t)((q >> 2) & 1) * c.sz;
+    const int lo = (q & 1) + ((q >> 1) & 1) * S.sx + ((q >> 2) & 1) * S.sxy;
+    win_put<LDS>(S, A.lbase + lo, A.base + go, A.a[q]);
+    A.a[q] = 0.0f;
+  }
+}
+
+__device__ __forceinline__ void win_flush(const DvCtx &c, const WinSink &S, WinCarry &A)
+{
+  if (!A.live)
+    return;
+  if (S.lds)
+    win_flush_cell<true>(c, S, A);
+  else
+    win_flush_cell<false>(c, S, A);
+  A.live = false;
+}
+
+template <bool LDS>
+__device__ __forceinline__ void win_move(const DvCtx &c, const WinSink &S, WinCarry &A, const Sample &sm)
+{
+  const int dx = sm.ci - A.ci, dy = sm.cj - A.cj, dz = sm.ck - A.ck;
+  if (A.live && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
+    win_step<0, LDS>(c, S, A, dx);
+    win_step<1, LDS>(c, S, A, dy);
+    win_step<2, LDS>(c, S, A, dz);
+  } else {
+    if (A.live)
+      win_flush_cell<LDS>(c, S, A);
+    A.base = sm.base;
+    A.lbase = win_index(S, sm.ci, sm.cj, sm.ck);
+    A.live = true;
+  }
+  A.ci = sm.ci;
+  A.cj = sm.cj;
+  A.ck = sm.ck;
+}
+
+__device__ __forceinline__ void win_enter(const DvCtx &c, const WinSink &S, WinCarry &A, const Sample &sm)
+{
+  if (A.live && sm.ci == A.ci && sm.cj == A.cj && sm.ck == A.ck)
+    return;
+  if (S.lds)
+    win_move<true>(c, S, A, sm);
+  else
+    win_move<false>(c, S, A, sm);
+}
+
+// the trilinear cell of a sample position, exactly as dv_sample finds it
+__device__ __forceinline__ void cell_of(const DvCtx &c, float gx, float gy, float gz, int cell[3])
+{
+  const float g[3] = {gx, gy, gz};
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const float x = fminf(fmaxf(g[q], 0.0f), c.gmax[q]);
+    cell[q] = min((int)x, c.imax[q]);
+  }
+}
+
+__device__ __forceinline__ void wave_box(int bl[3], int bh[3])
+{
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      bl[q] = min(bl[q], __shfl_xor(bl[q], m));
+      bh[q] = max(bh[q], __shfl_xor(bh[q], m));
+    }
+  }
+}
+
+__device__ __forceinline__ int wave_min(int x)
+{
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1)
+    x = min(x, __shfl_xor(x, m));
+  return x;
+}
+
+__device__ __forceinline__ int wave_max(int x)
+{
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1)
+    x = max(x, __shfl_xor(x, m));
+  return x;
+}
+
+__device__ __forceinline__ float wave_fmax(float x)
+{
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1)
+    x = fmaxf(x, __shfl_xor(x, m));
+  return x;
+}
+
+// LDS operations of this wave complete before the next ones are issued
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// exponent e with bound * 2^e < 2^61 (bound 0: e = 0)
+__device__ __forceinline__ int fix_exponent(float bound)
+{
+  if (!(bound > 0.0f))
+    return 0;
+  if (!(bound < 3.0e38f))
+    return -120;
+  int e;
+  frexpf(bound, &e); // bound < 2^e
+  return max(-120, min(120, 61 - e));
+}
+
+__global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, const float *lut_g,
+    uint32_t n, const float *org, const float *dir, const float *grad_rgba, const float *ckpt,
+    float *grad_density, float *lut_partials)
+{
+  __shared__ float4 s_lut[256];
+  __shared__ unsigned long long s_glut[1024];
+  __shared__ unsigned long long s_win[kSfxWaves * kWinCap];
+  __shared__ uint32_t s_stat[4]; // max |G|, max |LUT|, max alpha, max(-alpha, 0) as float bits
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x < 4)
+    s_stat[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < 1024; i += kSfxThreads)
+    s_glut[i] = 0;
+  for (int i = threadIdx.x; i < kSfxWaves * kWinCap; i += kSfxThreads)
+    s_win[i] = 0;
+  unsigned long long *win = s_win + wave * kWinCap;
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    const float4 e = gld((const float4 *)lut_g, threadIdx.x);
+    s_lut[threadIdx.x] = e;
+    const float lm = wave_fmax(fmaxf(fmaxf(fabsf(e.x), fabsf(e.y)), fmaxf(fabsf(e.z), fabsf(e.w))));
+    const float am = wave_fmax(e.w), an = wave_fmax(-e.w);
+    if (lane == 0) {
+      atomicMax(&s_stat[1], __float_as_uint(lm));
+      atomicMax(&s_stat[2], __float_as_uint(fmaxf(am, 0.0f)));
+      atomicMax(&s_stat[3], __float_as_uint(fmaxf(an, 0.0f)));
+    }
+  }
+
+  uint32_t r;
+  bool valid;
+  if (c.img_w > 0) {
+    // workgroup tile 32x16 pixels: 4x2 wave tiles of 8x8
+    const uint32_t img = blockIdx.x / c.tiles_per_img, t = blockIdx.x - img * c.tiles_per_img;
+    const uint32_t ty = t / c.tiles_x, tx = t - ty * c.tiles_x;
+    const uint32_t x = tx * 32 + (wave & 3) * 8 + (lane & 7), y = ty * 16 + (wave >> 2) * 8 + (lane >> 3);
+    valid = x < c.img_w && y < c.img_h;
+    r = (uint32_t)((size_t)img * c.img_w * c.img_h + (size_t)y * c.img_w + x);
+    valid = valid && r < n;
+  } else {
+    r = blockIdx.x * kSfxThreads + threadIdx.x;
+    valid = r < n;
+  }
+
+  uint32_t N = 0;
+  float t0 = 0.0f, t1 = 0.0f;
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f, gO = 0.0f;
+  float gox = 0.0f, gdx = 0.0f, goy = 0.0f, gdy = 0.0f, goz = 0.0f, gdz = 0.0f;
+  float T = 1.0f;
+  double Ssuf = 0.0; // sum over the samples not yet replayed of w_j v_j
+  if (valid) {
+    N = __float_as_uint(gld(ckpt, (size_t)0 * n + r));
+    const v3 o = gld3(org, r), d = gld3(dir, r);
+    if (N > 0 && dv_slab(c, o, d, t0, t1)) {
+      const float4 G4 = gld((const float4 *)grad_rgba, r);
+      gx = G4.x;
+      gy = G4.y;
+      gz = G4.z;
+      gO = G4.w;
+      gox = (o.x - c.origin[0]) * c.inv_sp[0];
+      gdx = d.x * c.inv_sp[0];
+      goy = (o.y - c.origin[1]) * c.inv_sp[1];
+      gdy = d.y * c.inv_sp[1];
+      goz = (o.z - c.origin[2]) * c.inv_sp[2];
+      gdz = d.z * c.inv_sp[2];
+      double tot[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t lo = __float_as_uint(gld(ckpt, (size_t)(c.nck + 2 * q) * n + r));
+        const uint32_t hi = __float_as_uint(gld(ckpt, (size_t)(c.nck + 2 * q + 1) * n + r));
+        tot[q] = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+      }
+      Ssuf = ((double)gx * tot[0] + (double)gy * tot[1]) + ((double)gz * tot[2] + (double)gO * tot[3]);
+    } else {
+      N = 0;
+    }
+  }
+  {
+    const float gm = wave_fmax(fmaxf(fmaxf(fabsf(gx), fabsf(gy)), fmaxf(fabsf(gz), fabsf(gO))));
+    if (lane == 0)
+      atomicMax(&s_stat[0], __float_as_uint(gm));
+  }
+  __syncthreads();
+
+  // Fixed-point scales.  With Gm = max |G|, Lm = max |LUT|, alpha in
+  // [-an, am]: |w| <= Wm = Tm (1 + (1 + an)^expo), Tm = (1 + an)^(expo kmax)
+  // bounds the transmittance, |v| <= Vm = Gm (3 Lm + 1), |suffix| and
+  // |T v| <= Wm Vm, |da| <= Dm = 2 expo Wm Vm / max(1 - am, 1e-8); a sample
+  // adds at most Pm = max(Gm Wm, Dm) to a LUT scalar and at most
+  // 256/range 2 Lm (3 Gm Wm + Dm) to ds.  A carried corner holds at most the
+  // samples of the 8 cells around it, each cell at most spc samples; a window
+  // entry gets one flush per ray of the wave.
+  float dscale, lscale;
+  double dinv, linv;
+  {
+    const float Gm = __uint_as_float(s_stat[0]), Lm = __uint_as_float(s_stat[1]);
+    const float am = __uint_as_float(s_stat[2]), an = __uint_as_float(s_stat[3]);
+    const float kmax = (float)((c.nck - 2) * (uint32_t)c.ckpt);
+    const float Tm = an > 0.0f ? powf(1.0f + an, c.expo * kmax) : 1.0f;
+    const float Wm = Tm * (1.0f + powf(1.0f + an, c.expo));
+    const float Vm = Gm * (3.0f * Lm + 1.0f);
+    const float Dm = 2.0f * c.expo * Wm * Vm / fmaxf(1.0f - am, 1e-8f);
+    const float Pm = fmaxf(Gm * Wm, Dm);
+    const float dsm = 256.0f * fabsf(c.inv_range) * 2.0f * Lm * (3.0f * Gm * Wm + Dm);
+    const float stepg = c.step * fminf(fminf(c.inv_sp[0], c.inv_sp[1]), c.inv_sp[2]);
+    const float spc = floorf(1.7320508f / stepg) + 2.0f;
+    const int ed = fix_exponent(dsm * 8.0f * spc * 64.0f * 2.0f);
+    const int el = fix_exponent(Pm * kmax * (float)kSfxThreads * 2.0f);
+    dscale = ldexpf(1.0f, ed);
+    lscale = ldexpf(1.0f, el);
+    dinv = ldexp(1.0, -ed);
+    linv = ldexp(1.0, -el);
+  }
+
+  // lockstep offset of this wave's rays
+  int off = 0;
+  {
+    int bl[3] = {INT_MAX, INT_MAX, INT_MAX}, bh[3] = {INT_MIN, INT_MIN, INT_MIN};
+    if (N > 0) {
+      int ce[3];
+      cell_of(c, gox + gdx * t0, goy + gdy * t0, goz + gdz * t0, ce);
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        bl[q] = ce[q];
+        bh[q] = ce[q] + 1;
+      }
+    }
+    wave_box(bl, bh);
+    const int ev = bh[0] >= bl[0] ? (bh[0] - bl[0] + 1) * (bh[1] - bl[1] + 1) * (bh[2] - bl[2] + 1) : 0;
+    if (ev > kAlignBox && N > 0)
+      off = (int)(t0 / c.step);
+  }
+  const int gbeg = wave_min(N > 0 ? off : INT_MAX), gend = wave_max(N > 0 ? off + (int)N : INT_MIN);
+
+  WinCarry A;
+  A.live = false;
+  A.base = 0;
+  A.lbase = 0;
+  A.ci = A.cj = A.ck = 0;
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+    A.a[q] = 0.0f;
+  LutAcc L;
+  L.i = -1;
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++)
+    L.lo[ch] = L.hi[ch] = 0.0f;
+  const auto lut_flush = [&](int e, const float g[4]) {
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++)
+      __hip_atomic_fetch_add(s_glut + 4 * e + ch, to_fix(g[ch], lscale), __ATOMIC_RELAXED,
+          __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+
+  int CL = kSfxChunk;
+  for (int g0 = gbeg; g0 < gend; g0 += CL) {
+    WinSink S;
+    S.win = win;
+    S.gd = grad_density;
+    S.scale = dscale;
+    int vol;
+    for (CL = kSfxChunk;; CL >>= 1) {
+      const int g1 = min(gend, g0 + CL);
+      // cells this ray's samples of the chunk touch, with the sample before
+      // the chunk (the carried corners it flushes in this chunk are there)
+      const int ka = max(g0 - 1 - off, 0), kb = min((int)N - 1, g1 - 1 - off);
+      int bl[3] = {INT_MAX, INT_MAX, INT_MAX}, bh[3] = {INT_MIN, INT_MIN, INT_MIN};
+      if (N > 0 && kb >= ka) {
+        const float ta = t0 + (float)ka * c.step, tb = t0 + (float)kb * c.step;
+        int ca[3], cb[3];
+        cell_of(c, gox + gdx * ta, goy + gdy * ta, goz + gdz * ta, ca);
+        cell_of(c, gox + gdx * tb, goy + gdy * tb, goz + gdz * tb, cb);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          bl[q] = min(ca[q], cb[q]);
+          bh[q] = max(ca[q], cb[q]) + 1;
+        }
+      }
+      wave_box(bl, bh);
+      const bool any = bh[0] >= bl[0];
+      S.ox = bl[0];
+      S.oy = bl[1];
+      S.oz = bl[2];
+      S.sx = any ? bh[0] - bl[0] + 1 : 0;
+      S.sxy = any ? S.sx * (bh[1] - bl[1] + 1) : 0;
+      vol = any ? S.sxy * (bh[2] - bl[2] + 1) : 0;
+      S.lds = any && vol <= kWinCap;
+      if (S.lds || !any || CL == 1)
+        break;
+    }
+    const int g1 = min(gend, g0 + CL);
+    if (A.live)
+      A.lbase = win_index(S, A.ci, A.cj, A.ck);
+    for (int g = max(g0, off); g < g1 && g < off + (int)N; g++) {
+      const int k = g - off;
+      const float t = t0 + (float)k * c.step;
+      const Sample sm = dv_sample(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
+      const Cls cl = dv_classify(s_lut, c, sm.s);
+      const float tau = dv_tau(c, cl.co.w);
+      const float w = T * (1.0f - tau);
+      // sum over the later samples of w_j v_j = T_{k+1} R_k, from the FP64
+      // total minus the FP64 prefix
+      const double vd = ((double)gx * (double)cl.co.x + (double)gy * (double)cl.co.y)
+          + ((double)gz * (double)cl.co.z + (double)gO);
+      Ssuf -= (double)w * vd;
+      const float v = (gx * cl.co.x + gy * cl.co.y + gz * cl.co.z) + gO;
+      const float Tn = T * tau;
+      const float oma = fmaxf(1.0f - cl.co.w, 1e-8f);
+      const float da = ((float)Ssuf - Tn * v) * (-c.expo) * __builtin_amdgcn_rcpf(oma);
+      const float dr = w * gx, dg = w * gy, db = w * gz;
+      lut_enter(lut_flush, L, cl.i);
+      const float wf0 = 1.0f - cl.f, wf1 = cl.f;
+      L.lo[0] += wf0 * dr;
+      L.lo[1] += wf0 * dg;
+      L.lo[2] += wf0 * db;
+      L.lo[3] += wf0 * da;
+      L.hi[0] += wf1 * dr;
+      L.hi[1] += wf1 * dg;
+      L.hi[2] += wf1 * db;
+      L.hi[3] += wf1 * da;
+      // density: d(channel)/ds through the LUT slope, then trilinear weights
+      const float4 la = s_lut[cl.i], lb = s_lut[cl.i + 1];
+      const float ds = ((dr * (lb.x - la.x) + dg * (lb.y - la.y)) + (db * (lb.z - la.z) + da * (lb.w - la.w)))
+          * cl.du_ds;
+      win_enter(c, S, A, sm);
+      const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
+      A.a[0] += ds * (ux * uy * uz);
+      A.a[1] += ds * (sm.fx * uy * uz);
+      A.a[2] += ds * (ux * sm.fy * uz);
+      A.a[3] += ds * (sm.fx * sm.fy * uz);
+      A.a[4] += ds * (ux * uy * sm.fz);
+      A.a[5] += ds * (sm.fx * uy * sm.fz);
+      A.a[6] += ds * (ux * sm.fy * sm.fz);
+      A.a[7] += ds * (sm.fx * sm.fy * sm.fz);
+      T = Tn;
+      if (k == (int)N - 1)
+        win_flush(c, S, A); // the ray ends in this chunk
+    }
+    if (S.lds) {
+      // add the window to the grid, row by row (x fastest)
+      wave_lds_sync();
+      const float inv_sxy = 1.0f / (float)S.sxy, inv_sx = 1.0f / (float)S.sx;
+      for (int i = lane; i < vol; i += 64) {
+        const unsigned long long q = win[i];
+        if (q != 0) {
+          // i, sxy < 2^12: the float quotients are off by at most one
+          int lz = (int)((float)i * inv_sxy);
+          lz += (lz + 1) * S.sxy <= i ? 1 : 0;
+          lz -= lz * S.sxy > i ? 1 : 0;
+          const int rem = i - lz * S.sxy;
+          int ly = (int)((float)rem * inv_sx);
+          ly += (ly + 1) * S.sx <= rem ? 1 : 0;
+          ly -= ly * S.sx > rem ? 1 : 0;
+          const int lx = rem - ly * S.sx;
+          unsafeAtomicAdd(grad_density + (uint32_t)(S.ox + lx) + (uint32_t)(S.oy + ly) * c.sy
+              + (uint32_t)(S.oz + lz) * c.sz, from_fix(q, dinv));
+          win[i] = 0;
+        }
+      }
+      wave_lds_sync();
+    }
+  }
+  if (L.i >= 0) {
+    lut_flush(L.i, L.lo);
+    lut_flush(L.i + 1, L.hi);
+  }
+  __syncthreads();
+  float *dst = lut_partials + (size_t)blockIdx.x * 1024;
+  for (int i = threadIdx.x; i < 1024; i += kSfxThreads)
+    gst(dst, i, from_fix(s_glut[i], linv));
 }
 
 // LUT gradient: the per-workgroup partials are summed in two fixed-order
@@ -626,7 +1152,8 @@ size_t vrx_diffvol_ckpt_floats(const vrx_diffvol *v, uint32_t n)
   if (!v)
     return 0;
   int ck = v->checkpoint > 0 ? v->checkpoint : 64;
-  return (size_t)(2 + (v->max_steps + ck - 1) / ck) * n;
+  // count, final T, T every ck steps, then the FP64 totals (8 words)
+  return (size_t)(2 + (v->max_steps + ck - 1) / ck + 8) * n;
 }
 
 int vrx_diffvol_forward(const vrx_diffvol *v, uint32_t n, const float *d_org, const float *d_dir,
@@ -656,13 +1183,17 @@ int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, c
   if (n == 0)
     return VRX_OK;
   VRX_ARG(d_org && d_dir && d_grad_rgba && d_ckpt && d_grad_density && d_grad_lut, "buffers");
-  DvCtx c = make_ctx(*v);
+  VRX_ARG(v->adjoint == VRX_ADJOINT_SUFFIX || v->adjoint == VRX_ADJOINT_REPLAY, "unknown adjoint schedule");
+  const bool suffix = v->adjoint == VRX_ADJOINT_SUFFIX;
+  // workgroup tiles: 32x16 pixels (suffix), 16x16 (replay)
+  DvCtx c = make_ctx(*v, suffix ? 32 : 16);
   if (c.img_w > 0) {
     const size_t per = (size_t)c.img_w * c.img_h;
     VRX_ARG(c.img_h > 0 && n % per == 0, "image_w/image_h do not divide the ray count");
   }
+  const uint32_t threads = suffix ? (uint32_t)kSfxThreads : (uint32_t)kDvBlock;
   const uint32_t nb = c.img_w > 0 ? (uint32_t)(n / ((size_t)c.img_w * c.img_h)) * c.tiles_per_img
-                                  : (n + kDvBlock - 1) / kDvBlock;
+                                  : (n + threads - 1) / threads;
   // per-workgroup LUT partials live in a grow-only buffer owned by this call site
   static thread_local void *partials = nullptr;
   static thread_local size_t cap = 0;
@@ -677,8 +1208,12 @@ int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, c
     VRX_HIP(hipMalloc(&partials, need));
     cap = need;
   }
-  hipLaunchKernelGGL(k_diffvol_bwd, dim3(nb), dim3(kDvBlock), 0, (hipStream_t)s, c, v->d_lut, n,
-      d_org, d_dir, d_grad_rgba, d_ckpt, d_grad_density, (float *)partials);
+  if (suffix)
+    hipLaunchKernelGGL(k_diffvol_bwd_suffix, dim3(nb), dim3(kSfxThreads), 0, (hipStream_t)s, c,
+        v->d_lut, n, d_org, d_dir, d_grad_rgba, d_ckpt, d_grad_density, (float *)partials);
+  else
+    hipLaunchKernelGGL(k_diffvol_bwd_replay, dim3(nb), dim3(kDvBlock), 0, (hipStream_t)s, c,
+        v->d_lut, n, d_org, d_dir, d_grad_rgba, d_ckpt, d_grad_density, (float *)partials);
   float *slices = (float *)partials + (size_t)nb * 1024;
   hipLaunchKernelGGL(k_reduce_lut_slices, dim3(4, kLutSlices), dim3(256), 0, (hipStream_t)s,
       (const float *)partials, nb, slices);

@@ -3,8 +3,10 @@
 ``VolumeIntegral`` wraps the HIP forward/adjoint pair of ``include/vrx_hip.h``
 (``vrx_diffvol_forward`` / ``vrx_diffvol_backward``): the forward is the
 volume integral of the reference's ray march (gpu/volumeIntegration.h:64-103
-in the VisRTX tree, jitter off), the backward is a hand-written adjoint that
-replays the march from transmittance checkpoints stored every 64 steps.
+in the VisRTX tree, jitter off), the backward is a hand-written adjoint.  By
+default it replays the march in forward order and takes the suffix sums it
+needs from the forward's FP64 totals; ``adjoint="replay"`` replays the march
+backward from transmittance checkpoints stored every 64 steps.
 Gradients flow to the density grid and to the 256-entry RGBA transfer
 function LUT.  Torch provides device memory and the stream only.
 """
@@ -33,6 +35,9 @@ class GridSpec:
     # rays are images of image[0] x image[1] pixels (vrx_make_rays order);
     # lets the adjoint tile its workgroups.  None = unstructured rays.
     image: tuple | None = None
+    # backward schedule: "suffix" (forward-order replay, FP64 suffix sums) or
+    # "replay" (backward replay from the transmittance checkpoints)
+    adjoint: str = "suffix"
 
     @property
     def max_steps(self) -> int:
@@ -55,6 +60,9 @@ class GridSpec:
         v.max_steps = self.max_steps
         if self.image is not None:
             v.image_w, v.image_h = int(self.image[0]), int(self.image[1])
+        if self.adjoint not in ("suffix", "replay"):
+            raise ValueError(f"adjoint must be 'suffix' or 'replay', not {self.adjoint!r}")
+        v.adjoint = N.VRX_ADJOINT_SUFFIX if self.adjoint == "suffix" else N.VRX_ADJOINT_REPLAY
         return v
 
 
