@@ -587,9 +587,11 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd_replay(DvCtx c, const 
 // atomics are not -- and the window is added to the grid at the end of the
 // chunk with row-contiguous float atomics.  A box that does not fit is
 // halved in steps, down to one step; beyond that the flushes go to the grid.
-// The fixed-point scale of each workgroup comes from a bound on any window
-// or LUT sum built from its rays' largest output gradient and the LUT range,
-// so sums cannot overflow; the resolution is 2^-61 of that bound.
+// The fixed-point scale of each workgroup comes from a bound on any single
+// flush built from its rays' largest output gradient and the LUT range: a
+// flush is below 2^50 units, so sums of any number of flushes that can meet
+// in one word stay far from overflow, and the resolution is 2^-50 of the
+// bound.
 // ---------------------------------------------------------------------------
 constexpr int kSfxWaves = 8;
 constexpr int kSfxThreads = 64 * kSfxWaves;
@@ -597,17 +599,14 @@ constexpr int kWinCap = 1024;
 constexpr int kSfxChunk = 16;
 constexpr int kAlignBox = 256;
 
-// |x| 2^e rounded to an integer, as a 64-bit two's complement word.  With
-// |x| 2^e < 2^62 the integer has at most 24 significant bits, so its two
-// 32-bit halves are split exactly in float arithmetic.
-__device__ __forceinline__ unsigned long long to_fix(float x, float scale)
+// x scale (a power of two) rounded to an integer, as a 64-bit two's
+// complement word.  For |x scale| < 2^51, adding 1.5 2^52 leaves the integer
+// in the low bits of the double's mantissa, so the word is the double's bits
+// minus those of 1.5 2^52.
+__device__ __forceinline__ unsigned long long to_fix(float x, double scale)
 {
-  const float y = rintf(fabsf(x) * scale);
-  const float hf = truncf(y * 2.3283064365386963e-10f); // 2^-32
-  const uint32_t hi = (uint32_t)hf;
-  const uint32_t lo = (uint32_t)(y - hf * 4294967296.0f);
-  const unsigned long long v = ((unsigned long long)hi << 32) | lo;
-  return x < 0.0f ? (unsigned long long)(-(long long)v) : v;
+  const double d = fma((double)x, scale, 6755399441055744.0);
+  return (unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull;
 }
 
 __device__ __forceinline__ float from_fix(unsigned long long q, double inv_scale)
@@ -618,7 +617,7 @@ __device__ __forceinline__ float from_fix(unsigned long long q, double inv_scale
 struct WinSink {
   unsigned long long *win; // this wave's window
   float *gd;
-  float scale;             // 2^e of the fixed-point words
+  double scale;            // 2^e of the fixed-point words
   int ox, oy, oz;          // box origin (voxel coordinates)
   int sx, sxy;             // box row and slice pitch
   bool lds;                // the box fits the window
@@ -794,7 +793,7 @@ __device__ __forceinline__ void wave_lds_sync()
   __builtin_amdgcn_wave_barrier();
 }
 
-// exponent e with bound * 2^e < 2^61 (bound 0: e = 0)
+// exponent e with bound * 2^e < 2^50 (bound 0: e = 0)
 __device__ __forceinline__ int fix_exponent(float bound)
 {
   if (!(bound > 0.0f))
@@ -803,7 +802,7 @@ __device__ __forceinline__ int fix_exponent(float bound)
     return -120;
   int e;
   frexpf(bound, &e); // bound < 2^e
-  return max(-120, min(120, 61 - e));
+  return max(-120, min(120, 50 - e));
 }
 
 __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, const float *lut_g,
@@ -896,9 +895,11 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
   // |T v| <= Wm Vm, |da| <= Dm = 2 expo Wm Vm / max(1 - am, 1e-8); a sample
   // adds at most Pm = max(Gm Wm, Dm) to a LUT scalar and at most
   // 256/range 2 Lm (3 Gm Wm + Dm) to ds.  A carried corner holds at most the
-  // samples of the 8 cells around it, each cell at most spc samples; a window
-  // entry gets one flush per ray of the wave.
-  float dscale, lscale;
+  // samples of the 8 cells around it, each cell at most spc samples; a
+  // carried LUT scalar at most the ray's kmax samples.  A window word gets
+  // one flush per ray of the wave, a LUT word at most kmax Pm per ray of the
+  // workgroup: with flushes below 2^50 units, no word exceeds 2^60.
+  double dscale, lscale;
   double dinv, linv;
   {
     const float Gm = __uint_as_float(s_stat[0]), Lm = __uint_as_float(s_stat[1]);
@@ -912,10 +913,10 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
     const float dsm = 256.0f * fabsf(c.inv_range) * 2.0f * Lm * (3.0f * Gm * Wm + Dm);
     const float stepg = c.step * fminf(fminf(c.inv_sp[0], c.inv_sp[1]), c.inv_sp[2]);
     const float spc = floorf(1.7320508f / stepg) + 2.0f;
-    const int ed = fix_exponent(dsm * 8.0f * spc * 64.0f * 2.0f);
-    const int el = fix_exponent(Pm * kmax * (float)kSfxThreads * 2.0f);
-    dscale = ldexpf(1.0f, ed);
-    lscale = ldexpf(1.0f, el);
+    const int ed = fix_exponent(dsm * 8.0f * spc);
+    const int el = fix_exponent(Pm * kmax);
+    dscale = ldexp(1.0, ed);
+    lscale = ldexp(1.0, el);
     dinv = ldexp(1.0, -ed);
     linv = ldexp(1.0, -el);
   }
