@@ -145,9 +145,10 @@ __device__ __forceinline__ Sample dv_sample(const DvCtx &c, float gx, float gy, 
 
 struct Cls {
   float4 co;
+  float4 slope; // lut[i + 1] - lut[i]
   int i;
   float f;
-  float du_ds; // d(texel coordinate)/ds, 0 where clamped
+  float du_ds;  // d(texel coordinate)/ds, 0 where clamped
 };
 
 __device__ __forceinline__ Cls dv_classify(const float4 *lut, const DvCtx &c, float s)
@@ -166,6 +167,7 @@ __device__ __forceinline__ Cls dv_classify(const float4 *lut, const DvCtx &c, fl
   r.f = f;
   const bool inside = s > c.lo && s < c.hi && xr > 0.0f && xr < 255.0f;
   r.du_ds = inside ? 256.0f * c.inv_range : 0.0f;
+  r.slope = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, b.w - a.w);
   return r;
 }
 
@@ -538,9 +540,8 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd_replay(DvCtx c, const 
       L.hi[2] += wf1 * db;
       L.hi[3] += wf1 * da;
       // density: d(channel)/ds through the LUT slope, then trilinear weights
-      const float4 la = s_lut[cl.i], lb = s_lut[cl.i + 1];
-      const float ds = ((dr * (lb.x - la.x) + dg * (lb.y - la.y)) + (db * (lb.z - la.z) + da * (lb.w - la.w)))
-          * cl.du_ds;
+      const float4 sl = cl.slope;
+      const float ds = ((dr * sl.x + dg * sl.y) + (db * sl.z + da * sl.w)) * cl.du_ds;
       enter_cell(c, vc, grad_density, A, sm);
       const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
       A.a[0] += ds * (ux * uy * uz);
@@ -1009,9 +1010,9 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
       const float w = T * (1.0f - tau);
       // sum over the later samples of w_j v_j = T_{k+1} R_k, from the FP64
       // total minus the FP64 prefix
-      const double vd = ((double)gx * (double)cl.co.x + (double)gy * (double)cl.co.y)
-          + ((double)gz * (double)cl.co.z + (double)gO);
-      Ssuf -= (double)w * vd;
+      const double vd = fma((double)gx, (double)cl.co.x,
+          fma((double)gy, (double)cl.co.y, fma((double)gz, (double)cl.co.z, (double)gO)));
+      Ssuf = fma(-(double)w, vd, Ssuf);
       const float v = (gx * cl.co.x + gy * cl.co.y + gz * cl.co.z) + gO;
       const float Tn = T * tau;
       const float oma = fmaxf(1.0f - cl.co.w, 1e-8f);
@@ -1019,28 +1020,29 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
       const float dr = w * gx, dg = w * gy, db = w * gz;
       lut_enter(lut_flush, L, cl.i);
       const float wf0 = 1.0f - cl.f, wf1 = cl.f;
-      L.lo[0] += wf0 * dr;
-      L.lo[1] += wf0 * dg;
-      L.lo[2] += wf0 * db;
-      L.lo[3] += wf0 * da;
-      L.hi[0] += wf1 * dr;
-      L.hi[1] += wf1 * dg;
-      L.hi[2] += wf1 * db;
-      L.hi[3] += wf1 * da;
+      L.lo[0] = fmaf(wf0, dr, L.lo[0]);
+      L.lo[1] = fmaf(wf0, dg, L.lo[1]);
+      L.lo[2] = fmaf(wf0, db, L.lo[2]);
+      L.lo[3] = fmaf(wf0, da, L.lo[3]);
+      L.hi[0] = fmaf(wf1, dr, L.hi[0]);
+      L.hi[1] = fmaf(wf1, dg, L.hi[1]);
+      L.hi[2] = fmaf(wf1, db, L.hi[2]);
+      L.hi[3] = fmaf(wf1, da, L.hi[3]);
       // density: d(channel)/ds through the LUT slope, then trilinear weights
-      const float4 la = s_lut[cl.i], lb = s_lut[cl.i + 1];
-      const float ds = ((dr * (lb.x - la.x) + dg * (lb.y - la.y)) + (db * (lb.z - la.z) + da * (lb.w - la.w)))
-          * cl.du_ds;
+      const float4 sl = cl.slope;
+      const float ds = fmaf(dr, sl.x, fmaf(dg, sl.y, fmaf(db, sl.z, da * sl.w))) * cl.du_ds;
       win_enter(c, S, A, sm);
       const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
-      A.a[0] += ds * (ux * uy * uz);
-      A.a[1] += ds * (sm.fx * uy * uz);
-      A.a[2] += ds * (ux * sm.fy * uz);
-      A.a[3] += ds * (sm.fx * sm.fy * uz);
-      A.a[4] += ds * (ux * uy * sm.fz);
-      A.a[5] += ds * (sm.fx * uy * sm.fz);
-      A.a[6] += ds * (ux * sm.fy * sm.fz);
-      A.a[7] += ds * (sm.fx * sm.fy * sm.fz);
+      const float d0 = ds * ux, d1 = ds * sm.fx;
+      const float w00 = uy * uz, w10 = sm.fy * uz, w01 = uy * sm.fz, w11 = sm.fy * sm.fz;
+      A.a[0] = fmaf(d0, w00, A.a[0]);
+      A.a[1] = fmaf(d1, w00, A.a[1]);
+      A.a[2] = fmaf(d0, w10, A.a[2]);
+      A.a[3] = fmaf(d1, w10, A.a[3]);
+      A.a[4] = fmaf(d0, w01, A.a[4]);
+      A.a[5] = fmaf(d1, w01, A.a[5]);
+      A.a[6] = fmaf(d0, w11, A.a[6]);
+      A.a[7] = fmaf(d1, w11, A.a[7]);
       T = Tn;
       if (k == (int)N - 1)
         win_flush(c, S, A); // the ray ends in this chunk
