@@ -210,11 +210,24 @@ def bench_diffvol(iters=200, log_every=50):
         loss = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean()
         loss.backward()
 
+    # the integral alone: forward, then backward from a fixed upstream gradient
+    # (the MSE gradient at the start point); no loss kernels in the timing
+    with torch.no_grad():
+        rgba0 = D.render(dens, lut, org, dirs, spec)
+        g_up = (2.0 / rgba0.numel()) * (rgba0 - target)
+
+    def fwdbwd_integral():
+        dens.grad = None
+        lut.grad = None
+        D.render(dens, lut, org, dirs, spec).backward(g_up)
+
     fwd_ms = timeit(fwd, 20)
+    fbi_ms = timeit(fwdbwd_integral, 20)
     fb_ms = timeit(fwdbwd, 20)
-    # the same step with the backward replayed from the transmittance checkpoints
+    # the same with the backward replayed from the transmittance checkpoints
     spec_suffix = spec
     spec = D.GridSpec(**{**spec_suffix.__dict__, "adjoint": "replay"})
+    fbi_replay_ms = timeit(fwdbwd_integral, 20)
     fb_replay_ms = timeit(fwdbwd, 20)
     spec = spec_suffix
     dens.grad = None
@@ -240,10 +253,14 @@ def bench_diffvol(iters=200, log_every=50):
     iter_ms = (time.perf_counter() - t0) * 1000.0 / iters
     with torch.no_grad():
         last = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean().item()
-    return {"fwd_ms": round(fwd_ms, 3), "fwd_bwd_ms": round(fb_ms, 3),
-            "fwd_bwd_over_fwd": round(fb_ms / fwd_ms, 3), "adjoint": "suffix",
-            "fwd_bwd_ms_replay_adjoint": round(fb_replay_ms, 3),
-            "fwd_bwd_over_fwd_replay_adjoint": round(fb_replay_ms / fwd_ms, 3),
+    return {"fwd_ms": round(fwd_ms, 3), "adjoint": "suffix",
+            "fwd_bwd_integral_ms": round(fbi_ms, 3),
+            "fwd_bwd_integral_over_fwd": round(fbi_ms / fwd_ms, 3),
+            "fwd_bwd_with_mse_loss_ms": round(fb_ms, 3),
+            "fwd_bwd_with_mse_loss_over_fwd": round(fb_ms / fwd_ms, 3),
+            "replay_adjoint": {"fwd_bwd_integral_ms": round(fbi_replay_ms, 3),
+                               "fwd_bwd_integral_over_fwd": round(fbi_replay_ms / fwd_ms, 3),
+                               "fwd_bwd_with_mse_loss_ms": round(fb_replay_ms, 3)},
             "adam_ms_per_iter": round(iter_ms, 3),
             "iters": iters, "mse_first": first, "mse_final": last, "rays": int(org.shape[0])}
 
@@ -518,9 +535,11 @@ def main():
         log("differentiable volume")
         dv = bench_diffvol()
         extra["diffvol_128"] = dv
-        log(f"diffvol: fwd {dv['fwd_ms']} ms, fwd+bwd {dv['fwd_bwd_ms']} ms "
-            f"({dv['fwd_bwd_over_fwd']}x; checkpoint-replay adjoint "
-            f"{dv['fwd_bwd_ms_replay_adjoint']} ms), Adam {dv['adam_ms_per_iter']} ms/iter, "
+        log(f"diffvol: fwd {dv['fwd_ms']} ms, fwd+bwd {dv['fwd_bwd_integral_ms']} ms "
+            f"({dv['fwd_bwd_integral_over_fwd']}x; with the MSE loss "
+            f"{dv['fwd_bwd_with_mse_loss_ms']} ms = {dv['fwd_bwd_with_mse_loss_over_fwd']}x; "
+            f"checkpoint-replay adjoint {dv['replay_adjoint']['fwd_bwd_integral_ms']} ms), "
+            f"Adam {dv['adam_ms_per_iter']} ms/iter, "
             f"MSE {dv['mse_first']:.3e} -> {dv['mse_final']:.3e}")
 
     cpu = None
