@@ -811,7 +811,7 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
     float *grad_density, float *lut_partials)
 {
   __shared__ float4 s_lut[256];
-  __shared__ unsigned long long s_glut[1024];
+  __shared__ unsigned long long s_glut[1024]; // channel-major: [ch * 256 + entry]
   __shared__ unsigned long long s_win[kSfxWaves * kWinCap];
   __shared__ uint32_t s_stat[4]; // max |G|, max |LUT|, max alpha, max(-alpha, 0) as float bits
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -958,7 +958,7 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
   const auto lut_flush = [&](int e, const float g[4]) {
 #pragma unroll
     for (int ch = 0; ch < 4; ch++)
-      __hip_atomic_fetch_add(s_glut + 4 * e + ch, to_fix(g[ch], lscale), __ATOMIC_RELAXED,
+      __hip_atomic_fetch_add(s_glut + ch * 256 + e, to_fix(g[ch], lscale), __ATOMIC_RELAXED,
           __HIP_MEMORY_SCOPE_WORKGROUP);
   };
 
@@ -1078,7 +1078,7 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
   __syncthreads();
   float *dst = lut_partials + (size_t)blockIdx.x * 1024;
   for (int i = threadIdx.x; i < 1024; i += kSfxThreads)
-    gst(dst, i, from_fix(s_glut[i], linv));
+    gst(dst, i, from_fix(s_glut[(i & 3) * 256 + (i >> 2)], linv));
 }
 
 // LUT gradient: the per-workgroup partials are summed in two fixed-order
