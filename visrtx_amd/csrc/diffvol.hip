@@ -231,166 +231,14 @@ __global__ __launch_bounds__(kDvBlock) void k_diffvol_fwd(DvCtx c, const float *
 }
 
 // ---------------------------------------------------------------------------
-// Backward.  Two register caches keep the scatter traffic down:
-//  * density: the 8 corner gradients of the current trilinear cell.  When the
-//    ray moves to a neighbouring cell the corners it shares with the old cell
-//    stay in registers (one axis at a time: a face step keeps 4 of 8); only
-//    the corners left behind are added to the grid, with one hardware float
-//    atomic each, so every grid voxel receives about one atomic per ray that
-//    passes it instead of one per cell visit;
-//  * LUT: the gradients of the two entries (i, i+1) of the current LUT
-//    interval, carried the same way when the interval index moves by one,
-//    and flushed into the workgroup's LDS copy of the LUT gradient.
+// Backward.  The LUT gradient of a ray is carried in registers for the two
+// entries (i, i+1) of the current LUT interval and moves with the interval
+// when it changes by one; other changes flush the carried entries.
 // ---------------------------------------------------------------------------
-struct CornerAcc {
-  float a[8];    // corner q = x + 2y + 4z of the current cell
-  uint32_t base; // grid index of corner 0
-  int ci, cj, ck;
-  bool live;
-};
-
-// Workgroup voxel cache in LDS: a direct-mapped table of voxel keys and
-// gradient sums.  A corner update goes to its slot when the slot is free or
-// already holds that voxel (LDS float add), else straight to the grid (one
-// global float atomic).  The workgroup's rays step in lockstep and the table
-// is drained to the grid every kDrainEvery steps, so it holds the voxels that
-// the pixel tile's rays are passing; neighbouring rays visit the same voxels
-// at about the same step, so most updates merge in LDS.  Keys never change
-// between drains, which keeps the add race-free without a lock.
-constexpr int kCacheBits = 11;
-constexpr int kCacheSlots = 1 << kCacheBits;
-constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
-constexpr uint32_t kDrainEvery = 16;
-
-struct VoxCache {
-  uint32_t *key;
-  float *val;
-};
-
-__device__ __forceinline__ uint32_t cache_slot(uint32_t v)
-{
-  return (v * 2654435761u) >> (32 - kCacheBits);
-}
-
-__device__ __forceinline__ void put(const VoxCache &vc, float *gd, uint32_t idx, float x)
-{
-  if (x == 0.0f)
-    return;
-  const uint32_t h = cache_slot(idx);
-  uint32_t k = vc.key[h];
-  if (k == kEmptyKey) {
-    const uint32_t old = atomicCAS(vc.key + h, kEmptyKey, idx);
-    k = old == kEmptyKey ? idx : old;
-  }
-  if (k == idx)
-    atomicAdd(vc.val + h, x);
-  else
-    unsafeAtomicAdd(gd + idx, x);
-}
-
-// all threads of the workgroup, between barriers
-__device__ __forceinline__ void drain(const VoxCache &vc, float *gd)
-{
-  for (int i = threadIdx.x; i < kCacheSlots; i += kDvBlock) {
-    const uint32_t k = vc.key[i];
-    if (k != kEmptyKey) {
-      const float v = vc.val[i];
-      if (v != 0.0f)
-        unsafeAtomicAdd(gd + k, v);
-      vc.key[i] = kEmptyKey;
-      vc.val[i] = 0.0f;
-    }
-  }
-}
-
-__device__ __forceinline__ void corner_offsets(const DvCtx &c, uint32_t off[8])
-{
-#pragma unroll
-  for (int q = 0; q < 8; q++)
-    off[q] = (uint32_t)(q & 1) + (uint32_t)((q >> 1) & 1) * c.sy + (uint32_t)((q >> 2) & 1) * c.sz;
-}
-
-// move the cell by d (-1, 0, +1) along AXIS, flushing the face left behind
-template <int AXIS>
-__device__ __forceinline__ void shift_cell(
-    const DvCtx &c, const VoxCache &cache, float *gd, CornerAcc &A, int d)
-{
-  if (d == 0)
-    return;
-  // branch-free in the sign of d, so lanes moving either way share one path
-  constexpr int bit = 1 << AXIS;
-  const uint32_t stride = AXIS == 0 ? 1u : (AXIS == 1 ? c.sy : c.sz);
-  const bool up = d > 0;
-  uint32_t off[8];
-  corner_offsets(c, off);
-#pragma unroll
-  for (int q0 = 0; q0 < 8; q0++) {
-    if (q0 & bit)
-      continue;
-    const int q1 = q0 | bit;
-    put(cache, gd, A.base + (up ? off[q0] : off[q1]), up ? A.a[q0] : A.a[q1]);
-    const float keep = up ? A.a[q1] : A.a[q0];
-    A.a[q0] = up ? keep : 0.0f;
-    A.a[q1] = up ? 0.0f : keep;
-  }
-  A.base = up ? A.base + stride : A.base - stride;
-}
-
-__device__ __forceinline__ void flush_all(
-    const DvCtx &c, const VoxCache &cache, float *gd, CornerAcc &A)
-{
-  if (!A.live)
-    return;
-  uint32_t off[8];
-  corner_offsets(c, off);
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    put(cache, gd, A.base + off[q], A.a[q]);
-    A.a[q] = 0.0f;
-  }
-}
-
-__device__ __forceinline__ void enter_cell(
-    const DvCtx &c, const VoxCache &cache, float *gd, CornerAcc &A, const Sample &sm)
-{
-  if (A.live && sm.ci == A.ci && sm.cj == A.cj && sm.ck == A.ck)
-    return;
-  const int dx = sm.ci - A.ci, dy = sm.cj - A.cj, dz = sm.ck - A.ck;
-  if (A.live && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
-    shift_cell<0>(c, cache, gd, A, dx);
-    shift_cell<1>(c, cache, gd, A, dy);
-    shift_cell<2>(c, cache, gd, A, dz);
-  } else {
-    flush_all(c, cache, gd, A);
-    A.base = sm.base;
-    A.live = true;
-  }
-  A.ci = sm.ci;
-  A.cj = sm.cj;
-  A.ck = sm.ck;
-}
-
-// The workgroup keeps kLutCopies copies of the LUT gradient in LDS, lane l
-// adding into copy l % kLutCopies: lanes of a tile sample similar densities,
-// so they hit the same entries, and the copies cut those same-address
-// conflicts.  The copy stride is padded by one float so that copies of one
-// entry fall in different banks.
-constexpr int kLutCopies = 2;
-constexpr int kGlutStride = 1025;
-
 struct LutAcc {
   float lo[4], hi[4]; // gradients of entries i and i+1
   int i;
 };
-
-__device__ __forceinline__ void lut_put(float *s_glut, int e, const float g[4])
-{
-  if (g[0] != 0.0f || g[1] != 0.0f || g[2] != 0.0f || g[3] != 0.0f) {
-#pragma unroll
-    for (int ch = 0; ch < 4; ch++)
-      atomicAdd(s_glut + 4 * e + ch, g[ch]);
-  }
-}
 
 // move the carried interval to i; Flush(e, g[4]) adds g to LUT entry e
 template <class Flush>
@@ -419,169 +267,14 @@ __device__ __forceinline__ void lut_enter(Flush flush, LutAcc &L, int i)
   L.i = i;
 }
 
-__global__ __launch_bounds__(kDvBlock) void k_diffvol_bwd_replay(DvCtx c, const float *lut_g, uint32_t n,
-    const float *org, const float *dir, const float *grad_rgba, const float *ckpt,
-    float *grad_density, float *lut_partials)
-{
-  __shared__ float4 s_lut[256];
-  __shared__ float s_glut[kLutCopies * kGlutStride];
-  __shared__ uint32_t s_key[kCacheSlots];
-  __shared__ float s_val[kCacheSlots];
-  __shared__ uint32_t s_itmax;
-  const VoxCache vc{s_key, s_val};
-  s_lut[threadIdx.x] = gld((const float4 *)lut_g, threadIdx.x);
-  for (int i = threadIdx.x; i < kLutCopies * kGlutStride; i += kDvBlock)
-    s_glut[i] = 0.0f;
-  float *my_glut = s_glut + (threadIdx.x % kLutCopies) * kGlutStride;
-  for (int i = threadIdx.x; i < kCacheSlots; i += kDvBlock) {
-    s_key[i] = kEmptyKey;
-    s_val[i] = 0.0f;
-  }
-  if (threadIdx.x == 0)
-    s_itmax = 0;
-  __syncthreads();
-
-  uint32_t r;
-  bool valid;
-  if (c.img_w > 0) {
-    const uint32_t img = blockIdx.x / c.tiles_per_img, t = blockIdx.x - img * c.tiles_per_img;
-    const uint32_t ty = t / c.tiles_x, tx = t - ty * c.tiles_x;
-    const uint32_t x = tx * 16 + (threadIdx.x & 15), y = ty * 16 + (threadIdx.x >> 4);
-    valid = x < c.img_w && y < c.img_h;
-    r = (uint32_t)((size_t)img * c.img_w * c.img_h + (size_t)y * c.img_w + x);
-    valid = valid && r < n;
-  } else {
-    r = blockIdx.x * kDvBlock + threadIdx.x;
-    valid = r < n;
-  }
-
-  // per-ray state
-  uint32_t N = 0;
-  float t0 = 0.0f, t1 = 0.0f;
-  float gx = 0.0f, gy = 0.0f, gz = 0.0f, gO = 0.0f;
-  float gox = 0.0f, gdx = 0.0f, goy = 0.0f, gdy = 0.0f, goz = 0.0f, gdz = 0.0f;
-  float Tn = 0.0f, R = 0.0f;
-  if (valid) {
-    N = __float_as_uint(gld(ckpt, (size_t)0 * n + r));
-    const v3 o = gld3(org, r), d = gld3(dir, r);
-    if (N > 0 && dv_slab(c, o, d, t0, t1)) {
-      const float4 G4 = gld((const float4 *)grad_rgba, r);
-      gx = G4.x;
-      gy = G4.y;
-      gz = G4.z;
-      gO = G4.w;
-      gox = (o.x - c.origin[0]) * c.inv_sp[0];
-      gdx = d.x * c.inv_sp[0];
-      goy = (o.y - c.origin[1]) * c.inv_sp[1];
-      gdy = d.y * c.inv_sp[1];
-      goz = (o.z - c.origin[2]) * c.inv_sp[2];
-      gdz = d.z * c.inv_sp[2];
-      Tn = gld(ckpt, (size_t)1 * n + r); // T after the last sample
-    } else {
-      N = 0;
-    }
-  }
-  if (N > 0)
-    atomicMax(&s_itmax, N);
-  CornerAcc A;
-  A.live = false;
-  A.base = 0;
-  A.ci = A.cj = A.ck = 0;
-#pragma unroll
-  for (int q = 0; q < 8; q++)
-    A.a[q] = 0.0f;
-  LutAcc L;
-  L.i = -1;
-#pragma unroll
-  for (int ch = 0; ch < 4; ch++)
-    L.lo[ch] = L.hi[ch] = 0.0f;
-  __syncthreads();
-  const uint32_t itmax = s_itmax;
-
-  // lockstep over the tile: step j processes sample N-1-j of every ray
-  for (uint32_t j0 = 0; j0 < itmax; j0 += kDrainEvery) {
-    const uint32_t j1 = min(itmax, j0 + kDrainEvery);
-    for (uint32_t j = j0; j < j1 && j < N; j++) {
-      const int k = (int)(N - 1 - j);
-      const float t = t0 + (float)k * c.step;
-      const Sample sm = dv_sample(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
-      const Cls cl = dv_classify(s_lut, c, sm.s);
-      const float tau = dv_tau(c, cl.co.w);
-      float T;
-      const uint32_t seg = (uint32_t)k / (uint32_t)c.ckpt;
-      if ((uint32_t)k % (uint32_t)c.ckpt == 0) {
-        T = gld(ckpt, (size_t)(2 + seg) * n + r);
-      } else if (tau >= kMinTauForDivision && Tn > 0.0f) {
-        T = Tn * __builtin_amdgcn_rcpf(tau);
-      } else {
-        // replay from the segment checkpoint
-        T = gld(ckpt, (size_t)(2 + seg) * n + r);
-        for (uint32_t q = seg * (uint32_t)c.ckpt; q < (uint32_t)k; q++) {
-          const float tq = t0 + (float)q * c.step;
-          const Sample sq = dv_sample(c, gox + gdx * tq, goy + gdy * tq, goz + gdz * tq);
-          T *= dv_tau(c, dv_classify(s_lut, c, sq.s).co.w);
-        }
-      }
-      const float v = (gx * cl.co.x + gy * cl.co.y + gz * cl.co.z) + gO;
-      const float w = T * (1.0f - tau);
-      const float dtau = T * (R - v);
-      const float oma = fmaxf(1.0f - cl.co.w, 1e-8f);
-      const float da = dtau * (-c.expo) * tau * __builtin_amdgcn_rcpf(oma);
-      const float dr = w * gx, dg = w * gy, db = w * gz;
-      // LUT: entries i and i+1 with weights (1-f) and f
-      lut_enter([&](int e, const float g[4]) { lut_put(my_glut, e, g); }, L, cl.i);
-      const float wf0 = 1.0f - cl.f, wf1 = cl.f;
-      L.lo[0] += wf0 * dr;
-      L.lo[1] += wf0 * dg;
-      L.lo[2] += wf0 * db;
-      L.lo[3] += wf0 * da;
-      L.hi[0] += wf1 * dr;
-      L.hi[1] += wf1 * dg;
-      L.hi[2] += wf1 * db;
-      L.hi[3] += wf1 * da;
-      // density: d(channel)/ds through the LUT slope, then trilinear weights
-      const float4 sl = cl.slope;
-      const float ds = ((dr * sl.x + dg * sl.y) + (db * sl.z + da * sl.w)) * cl.du_ds;
-      enter_cell(c, vc, grad_density, A, sm);
-      const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
-      A.a[0] += ds * (ux * uy * uz);
-      A.a[1] += ds * (sm.fx * uy * uz);
-      A.a[2] += ds * (ux * sm.fy * uz);
-      A.a[3] += ds * (sm.fx * sm.fy * uz);
-      A.a[4] += ds * (ux * uy * sm.fz);
-      A.a[5] += ds * (sm.fx * uy * sm.fz);
-      A.a[6] += ds * (ux * sm.fy * sm.fz);
-      A.a[7] += ds * (sm.fx * sm.fy * sm.fz);
-      R = (1.0f - tau) * v + tau * R;
-      Tn = T;
-    }
-    __syncthreads();
-    drain(vc, grad_density);
-    __syncthreads();
-  }
-  flush_all(c, vc, grad_density, A);
-  if (L.i >= 0) {
-    lut_put(my_glut, L.i, L.lo);
-    lut_put(my_glut, L.i + 1, L.hi);
-  }
-  __syncthreads();
-  drain(vc, grad_density);
-  float *dst = lut_partials + (size_t)blockIdx.x * 1024;
-  for (int i = threadIdx.x; i < 1024; i += kDvBlock) {
-    float g = 0.0f;
-#pragma unroll
-    for (int cp = 0; cp < kLutCopies; cp++)
-      g += s_glut[cp * kGlutStride + i];
-    gst(dst, i, g);
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Suffix adjoint.  A workgroup is kSfxWaves waves; each wave traces one 8x8
-// pixel tile and steps its rays in lockstep (sample k of a ray at step
-// off + k; off = 0, or the entry distance in steps when the rays' entry cells
-// spread over more than kAlignBox voxels, as on a face seen at a grazing
-// angle).  Every chunk of up to kSfxChunk steps, the wave takes the box of
+// The adjoint kernel, both schedules.  A workgroup is kSfxWaves waves; each
+// wave traces one 8x8 pixel tile and steps its rays in lockstep: the j-th
+// sample a ray processes (k = j forward, k = N-1-j for the replay) runs at
+// step off + j.  off = 0 (rays start together at their first / last sample)
+// unless those samples' cells spread over more than kAlignBox voxels, as on
+// a face seen at a grazing angle; then the rays are aligned by distance from
+// the camera.  Every chunk of up to kSfxChunk steps, the wave takes the box of
 // voxels its samples touch; if the box fits its LDS window (kWinCap
 // entries), the carried corner gradients it flushes are added there, as
 // 64-bit fixed-point integers -- LDS integer atomics are cheap, LDS float
@@ -806,7 +499,9 @@ __device__ __forceinline__ int fix_exponent(float bound)
   return max(-120, min(120, 50 - e));
 }
 
-__global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, const float *lut_g,
+// REPLAY = false: suffix schedule; true: replay from the T checkpoints
+template <bool REPLAY>
+__global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd(DvCtx c, const float *lut_g,
     uint32_t n, const float *org, const float *dir, const float *grad_rgba, const float *ckpt,
     float *grad_density, float *lut_partials)
 {
@@ -854,8 +549,10 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
   float t0 = 0.0f, t1 = 0.0f;
   float gx = 0.0f, gy = 0.0f, gz = 0.0f, gO = 0.0f;
   float gox = 0.0f, gdx = 0.0f, goy = 0.0f, gdy = 0.0f, goz = 0.0f, gdz = 0.0f;
-  float T = 1.0f;
-  double Ssuf = 0.0; // sum over the samples not yet replayed of w_j v_j
+  float T = 1.0f;     // suffix: T before the next sample
+  double Ssuf = 0.0;  // suffix: sum over the samples not yet replayed of w_j v_j
+  float Tn = 0.0f;    // replay: T after the sample replayed last
+  float R = 0.0f;     // replay: R of the sample replayed last
   if (valid) {
     N = __float_as_uint(gld(ckpt, (size_t)0 * n + r));
     const v3 o = gld3(org, r), d = gld3(dir, r);
@@ -871,14 +568,18 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
       gdy = d.y * c.inv_sp[1];
       goz = (o.z - c.origin[2]) * c.inv_sp[2];
       gdz = d.z * c.inv_sp[2];
-      double tot[4];
+      if (REPLAY) {
+        Tn = gld(ckpt, (size_t)1 * n + r); // T after the last sample
+      } else {
+        double tot[4];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const uint32_t lo = __float_as_uint(gld(ckpt, (size_t)(c.nck + 2 * q) * n + r));
-        const uint32_t hi = __float_as_uint(gld(ckpt, (size_t)(c.nck + 2 * q + 1) * n + r));
-        tot[q] = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+        for (int q = 0; q < 4; q++) {
+          const uint32_t lo = __float_as_uint(gld(ckpt, (size_t)(c.nck + 2 * q) * n + r));
+          const uint32_t hi = __float_as_uint(gld(ckpt, (size_t)(c.nck + 2 * q + 1) * n + r));
+          tot[q] = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+        }
+        Ssuf = ((double)gx * tot[0] + (double)gy * tot[1]) + ((double)gz * tot[2] + (double)gO * tot[3]);
       }
-      Ssuf = ((double)gx * tot[0] + (double)gy * tot[1]) + ((double)gz * tot[2] + (double)gO * tot[3]);
     } else {
       N = 0;
     }
@@ -892,8 +593,8 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
 
   // Fixed-point scales.  With Gm = max |G|, Lm = max |LUT|, alpha in
   // [-an, am]: |w| <= Wm = Tm (1 + (1 + an)^expo), Tm = (1 + an)^(expo kmax)
-  // bounds the transmittance, |v| <= Vm = Gm (3 Lm + 1), |suffix| and
-  // |T v| <= Wm Vm, |da| <= Dm = 2 expo Wm Vm / max(1 - am, 1e-8); a sample
+  // bounds the transmittance, |v| <= Vm = Gm (3 Lm + 1), |suffix|, |T R|
+  // and |T v| <= Wm Vm, |da| <= Dm = 2 expo Wm Vm / max(1 - am, 1e-8); a sample
   // adds at most Pm = max(Gm Wm, Dm) to a LUT scalar and at most
   // 256/range 2 Lm (3 Gm Wm + Dm) to ds.  A carried corner holds at most the
   // samples of the 8 cells around it, each cell at most spc samples; a
@@ -926,9 +627,10 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
   int off = 0;
   {
     int bl[3] = {INT_MAX, INT_MAX, INT_MAX}, bh[3] = {INT_MIN, INT_MIN, INT_MIN};
+    const float tf = t0 + (float)(REPLAY ? (int)N - 1 : 0) * c.step; // first sample replayed
     if (N > 0) {
       int ce[3];
-      cell_of(c, gox + gdx * t0, goy + gdy * t0, goz + gdz * t0, ce);
+      cell_of(c, gox + gdx * tf, goy + gdy * tf, goz + gdz * tf, ce);
 #pragma unroll
       for (int q = 0; q < 3; q++) {
         bl[q] = ce[q];
@@ -937,8 +639,16 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
     }
     wave_box(bl, bh);
     const int ev = bh[0] >= bl[0] ? (bh[0] - bl[0] + 1) * (bh[1] - bl[1] + 1) * (bh[2] - bl[2] + 1) : 0;
-    if (ev > kAlignBox && N > 0)
-      off = (int)(t0 / c.step);
+    if (ev > kAlignBox) {
+      // distance of the first replayed sample, in steps
+      const int e = N > 0 ? (int)(t0 / c.step) + (REPLAY ? (int)N - 1 : 0) : 0;
+      if (REPLAY) {
+        const int emax = wave_max(N > 0 ? e : INT_MIN); // the farthest starts first
+        off = N > 0 ? emax - e : 0;
+      } else {
+        off = e;
+      }
+    }
   }
   const int gbeg = wave_min(N > 0 ? off : INT_MAX), gend = wave_max(N > 0 ? off + (int)N : INT_MIN);
 
@@ -971,11 +681,13 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
     int vol;
     for (CL = kSfxChunk;; CL >>= 1) {
       const int g1 = min(gend, g0 + CL);
-      // cells this ray's samples of the chunk touch, with the sample before
-      // the chunk (the carried corners it flushes in this chunk are there)
-      const int ka = max(g0 - 1 - off, 0), kb = min((int)N - 1, g1 - 1 - off);
+      // cells this ray's samples of the chunk touch (replay indices j), with
+      // the sample before the chunk (the carried corners it flushes in this
+      // chunk are there)
+      const int ja = max(g0 - 1 - off, 0), jb = min((int)N - 1, g1 - 1 - off);
       int bl[3] = {INT_MAX, INT_MAX, INT_MAX}, bh[3] = {INT_MIN, INT_MIN, INT_MIN};
-      if (N > 0 && kb >= ka) {
+      if (N > 0 && jb >= ja) {
+        const int ka = REPLAY ? (int)N - 1 - jb : ja, kb = REPLAY ? (int)N - 1 - ja : jb;
         const float ta = t0 + (float)ka * c.step, tb = t0 + (float)kb * c.step;
         int ca[3], cb[3];
         cell_of(c, gox + gdx * ta, goy + gdy * ta, goz + gdz * ta, ca);
@@ -1002,21 +714,47 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
     if (A.live)
       A.lbase = win_index(S, A.ci, A.cj, A.ck);
     for (int g = max(g0, off); g < g1 && g < off + (int)N; g++) {
-      const int k = g - off;
+      const int j = g - off;
+      const int k = REPLAY ? (int)N - 1 - j : j;
       const float t = t0 + (float)k * c.step;
       const Sample sm = dv_sample(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
       const Cls cl = dv_classify(s_lut, c, sm.s);
       const float tau = dv_tau(c, cl.co.w);
-      const float w = T * (1.0f - tau);
-      // sum over the later samples of w_j v_j = T_{k+1} R_k, from the FP64
-      // total minus the FP64 prefix
-      const double vd = fma((double)gx, (double)cl.co.x,
-          fma((double)gy, (double)cl.co.y, fma((double)gz, (double)cl.co.z, (double)gO)));
-      Ssuf = fma(-(double)w, vd, Ssuf);
       const float v = (gx * cl.co.x + gy * cl.co.y + gz * cl.co.z) + gO;
-      const float Tn = T * tau;
       const float oma = fmaxf(1.0f - cl.co.w, 1e-8f);
-      const float da = ((float)Ssuf - Tn * v) * (-c.expo) * __builtin_amdgcn_rcpf(oma);
+      float w, da;
+      if (REPLAY) {
+        // T_k: the checkpoint, T_{k+1} / tau_k, or a forward replay from the
+        // segment's checkpoint where tau_k is too small for the division
+        const uint32_t seg = (uint32_t)k / (uint32_t)c.ckpt;
+        if ((uint32_t)k % (uint32_t)c.ckpt == 0) {
+          T = gld(ckpt, (size_t)(2 + seg) * n + r);
+        } else if (tau >= kMinTauForDivision && Tn > 0.0f) {
+          T = Tn * __builtin_amdgcn_rcpf(tau);
+        } else {
+          T = gld(ckpt, (size_t)(2 + seg) * n + r);
+          for (uint32_t q = seg * (uint32_t)c.ckpt; q < (uint32_t)k; q++) {
+            const float tq = t0 + (float)q * c.step;
+            const Sample sq = dv_sample(c, gox + gdx * tq, goy + gdy * tq, goz + gdz * tq);
+            T *= dv_tau(c, dv_classify(s_lut, c, sq.s).co.w);
+          }
+        }
+        w = T * (1.0f - tau);
+        const float dtau = T * (R - v);
+        da = dtau * (-c.expo) * tau * __builtin_amdgcn_rcpf(oma);
+        R = (1.0f - tau) * v + tau * R;
+        Tn = T;
+      } else {
+        w = T * (1.0f - tau);
+        // sum over the later samples of w_j v_j = T_{k+1} R_k, from the FP64
+        // total minus the FP64 prefix
+        const double vd = fma((double)gx, (double)cl.co.x,
+            fma((double)gy, (double)cl.co.y, fma((double)gz, (double)cl.co.z, (double)gO)));
+        Ssuf = fma(-(double)w, vd, Ssuf);
+        const float Tk1 = T * tau;
+        da = ((float)Ssuf - Tk1 * v) * (-c.expo) * __builtin_amdgcn_rcpf(oma);
+        T = Tk1;
+      }
       const float dr = w * gx, dg = w * gy, db = w * gz;
       lut_enter(lut_flush, L, cl.i);
       const float wf0 = 1.0f - cl.f, wf1 = cl.f;
@@ -1043,8 +781,7 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd_suffix(DvCtx c, con
       A.a[5] = fmaf(d1, w01, A.a[5]);
       A.a[6] = fmaf(d0, w11, A.a[6]);
       A.a[7] = fmaf(d1, w11, A.a[7]);
-      T = Tn;
-      if (k == (int)N - 1)
+      if (j == (int)N - 1)
         win_flush(c, S, A); // the ray ends in this chunk
     }
     if (S.lds) {
@@ -1188,13 +925,12 @@ int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, c
   VRX_ARG(d_org && d_dir && d_grad_rgba && d_ckpt && d_grad_density && d_grad_lut, "buffers");
   VRX_ARG(v->adjoint == VRX_ADJOINT_SUFFIX || v->adjoint == VRX_ADJOINT_REPLAY, "unknown adjoint schedule");
   const bool suffix = v->adjoint == VRX_ADJOINT_SUFFIX;
-  // workgroup tiles: 32x16 pixels (suffix), 16x16 (replay)
-  DvCtx c = make_ctx(*v, suffix ? 32 : 16);
+  DvCtx c = make_ctx(*v, 32); // workgroup tiles of 32x16 pixels
   if (c.img_w > 0) {
     const size_t per = (size_t)c.img_w * c.img_h;
     VRX_ARG(c.img_h > 0 && n % per == 0, "image_w/image_h do not divide the ray count");
   }
-  const uint32_t threads = suffix ? (uint32_t)kSfxThreads : (uint32_t)kDvBlock;
+  const uint32_t threads = (uint32_t)kSfxThreads;
   const uint32_t nb = c.img_w > 0 ? (uint32_t)(n / ((size_t)c.img_w * c.img_h)) * c.tiles_per_img
                                   : (n + threads - 1) / threads;
   // per-workgroup LUT partials live in a grow-only buffer owned by this call site
@@ -1212,10 +948,10 @@ int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, c
     cap = need;
   }
   if (suffix)
-    hipLaunchKernelGGL(k_diffvol_bwd_suffix, dim3(nb), dim3(kSfxThreads), 0, (hipStream_t)s, c,
+    hipLaunchKernelGGL(k_diffvol_bwd<false>, dim3(nb), dim3(kSfxThreads), 0, (hipStream_t)s, c,
         v->d_lut, n, d_org, d_dir, d_grad_rgba, d_ckpt, d_grad_density, (float *)partials);
   else
-    hipLaunchKernelGGL(k_diffvol_bwd_replay, dim3(nb), dim3(kDvBlock), 0, (hipStream_t)s, c,
+    hipLaunchKernelGGL(k_diffvol_bwd<true>, dim3(nb), dim3(kSfxThreads), 0, (hipStream_t)s, c,
         v->d_lut, n, d_org, d_dir, d_grad_rgba, d_ckpt, d_grad_density, (float *)partials);
   float *slices = (float *)partials + (size_t)nb * 1024;
   hipLaunchKernelGGL(k_reduce_lut_slices, dim3(4, kLutSlices), dim3(256), 0, (hipStream_t)s,
