@@ -20,9 +20,8 @@
 //  * suffix (default): the march is replayed forward, in the forward's own
 //    order, and T_{k+1} R_k = sum_{j>k} w_j v_j is the forward's FP64 total
 //    minus the FP64 prefix, so no division by tau and no checkpoint reads.
-// Both accumulate the density gradient per trilinear cell in registers and
-// flush it when the ray leaves the cell; LUT gradients are pre-reduced in LDS
-// per workgroup.
+// Both add each sample's eight trilinear corner gradients to a per-wave LDS
+// window of the grid; LUT gradients are pre-reduced in LDS per workgroup.
 #include <float.h>
 #include <limits.h>
 #include <string.h>
@@ -276,16 +275,15 @@ __device__ __forceinline__ void lut_enter(Flush flush, LutAcc &L, int i)
 // a face seen at a grazing angle; then the rays are aligned by distance from
 // the camera.  Every chunk of up to kSfxChunk steps, the wave takes the box of
 // voxels its samples touch; if the box fits its LDS window (kWinCap
-// entries), the carried corner gradients it flushes are added there, as
-// 64-bit fixed-point integers -- LDS integer atomics are cheap, LDS float
-// atomics are not -- and the window is added to the grid at the end of the
-// chunk with row-contiguous float atomics.  A box that does not fit is
-// halved in steps, down to one step; beyond that the flushes go to the grid.
-// The fixed-point scale of each workgroup comes from a bound on any single
-// flush built from its rays' largest output gradient and the LUT range: a
-// flush is below 2^50 units, so sums of any number of flushes that can meet
-// in one word stay far from overflow, and the resolution is 2^-50 of the
-// bound.
+// entries), each sample's eight corner gradients are added there, as 64-bit
+// fixed-point integers -- LDS integer atomics are cheap, LDS float atomics
+// are not -- and the window is added to the grid at the end of the chunk
+// with row-contiguous float atomics.  A box that does not fit is halved in
+// steps, down to one step; beyond that the corners go to the grid.  The
+// fixed-point scale of each workgroup comes from a bound on any single
+// addition built from its rays' largest output gradient and the LUT range:
+// an addition is below 2^50 units, so the sums that can meet in one word
+// stay far from overflow, and the resolution is 2^-50 of the bound.
 // ---------------------------------------------------------------------------
 constexpr int kSfxWaves = 8;
 constexpr int kSfxThreads = 64 * kSfxWaves;
@@ -310,127 +308,14 @@ __device__ __forceinline__ float from_fix(unsigned long long q, double inv_scale
 
 struct WinSink {
   unsigned long long *win; // this wave's window
-  float *gd;
-  double scale;            // 2^e of the fixed-point words
   int ox, oy, oz;          // box origin (voxel coordinates)
   int sx, sxy;             // box row and slice pitch
   bool lds;                // the box fits the window
 };
 
-struct WinCarry {
-  float a[8];     // corner q = x + 2y + 4z of the current cell
-  uint32_t base;  // grid index of corner 0
-  int lbase;      // window index of corner 0 (while the sink is the window)
-  int ci, cj, ck; // the cell
-  bool live;
-};
-
 __device__ __forceinline__ int win_index(const WinSink &S, int i, int j, int k)
 {
   return (i - S.ox) + (j - S.oy) * S.sx + (k - S.oz) * S.sxy;
-}
-
-template <bool LDS>
-__device__ __forceinline__ void win_put(const WinSink &S, int lidx, uint32_t gidx, float x)
-{
-  if (LDS)
-    __hip_atomic_fetch_add(S.win + lidx, to_fix(x, S.scale), __ATOMIC_RELAXED,
-        __HIP_MEMORY_SCOPE_WORKGROUP);
-  else if (x != 0.0f)
-    unsafeAtomicAdd(S.gd + gidx, x);
-}
-
-// move the cell one step along AXIS (UP: +1, else -1), flushing the face left
-// behind; rays of a tile are nearly parallel, so a wave rarely takes both
-template <int AXIS, bool UP, bool LDS>
-__device__ __forceinline__ void win_shift(const DvCtx &c, const WinSink &S, WinCarry &A)
-{
-  constexpr int bit = 1 << AXIS;
-  const uint32_t stride = AXIS == 0 ? 1u : (AXIS == 1 ? c.sy : c.sz);
-  const int lstride = AXIS == 0 ? 1 : (AXIS == 1 ? S.sx : S.sxy);
-#pragma unroll
-  for (int q0 = 0; q0 < 8; q0++) {
-    if (q0 & bit)
-      continue;
-    const int q1 = q0 | bit;
-    const int qf = UP ? q0 : q1; // the corner left behind
-    const uint32_t go = (uint32_t)(qf & 1) + (uint32_t)((qf >> 1) & 1) * c.sy
-        + (uint32_t)((qf >> 2) & 1) * c.sz;
-    const int lo = (qf & 1) + ((qf >> 1) & 1) * S.sx + ((qf >> 2) & 1) * S.sxy;
-    win_put<LDS>(S, A.lbase + lo, A.base + go, A.a[qf]);
-    if (UP) {
-      A.a[q0] = A.a[q1];
-      A.a[q1] = 0.0f;
-    } else {
-      A.a[q1] = A.a[q0];
-      A.a[q0] = 0.0f;
-    }
-  }
-  A.base = UP ? A.base + stride : A.base - stride;
-  A.lbase = UP ? A.lbase + lstride : A.lbase - lstride;
-}
-
-template <int AXIS, bool LDS>
-__device__ __forceinline__ void win_step(const DvCtx &c, const WinSink &S, WinCarry &A, int d)
-{
-  if (d > 0)
-    win_shift<AXIS, true, LDS>(c, S, A);
-  else if (d < 0)
-    win_shift<AXIS, false, LDS>(c, S, A);
-}
-
-template <bool LDS>
-__device__ __forceinline__ void win_flush_cell(const DvCtx &c, const WinSink &S, WinCarry &A)
-{
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const uint32_t go = (uint32_t)(q & 1) + (uint32_t)((q >> 1) & 1) * c.sy
-        + (uint32_t)((q >> 2) & 1) * c.sz;
-    const int lo = (q & 1) + ((q >> 1) & 1) * S.sx + ((q >> 2) & 1) * S.sxy;
-    win_put<LDS>(S, A.lbase + lo, A.base + go, A.a[q]);
-    A.a[q] = 0.0f;
-  }
-}
-
-__device__ __forceinline__ void win_flush(const DvCtx &c, const WinSink &S, WinCarry &A)
-{
-  if (!A.live)
-    return;
-  if (S.lds)
-    win_flush_cell<true>(c, S, A);
-  else
-    win_flush_cell<false>(c, S, A);
-  A.live = false;
-}
-
-template <bool LDS>
-__device__ __forceinline__ void win_move(const DvCtx &c, const WinSink &S, WinCarry &A, const Sample &sm)
-{
-  const int dx = sm.ci - A.ci, dy = sm.cj - A.cj, dz = sm.ck - A.ck;
-  if (A.live && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
-    win_step<0, LDS>(c, S, A, dx);
-    win_step<1, LDS>(c, S, A, dy);
-    win_step<2, LDS>(c, S, A, dz);
-  } else {
-    if (A.live)
-      win_flush_cell<LDS>(c, S, A);
-    A.base = sm.base;
-    A.lbase = win_index(S, sm.ci, sm.cj, sm.ck);
-    A.live = true;
-  }
-  A.ci = sm.ci;
-  A.cj = sm.cj;
-  A.ck = sm.ck;
-}
-
-__device__ __forceinline__ void win_enter(const DvCtx &c, const WinSink &S, WinCarry &A, const Sample &sm)
-{
-  if (A.live && sm.ci == A.ci && sm.cj == A.cj && sm.ck == A.ck)
-    return;
-  if (S.lds)
-    win_move<true>(c, S, A, sm);
-  else
-    win_move<false>(c, S, A, sm);
 }
 
 // the trilinear cell of a sample position, exactly as dv_sample finds it
@@ -596,11 +481,11 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd(DvCtx c, const floa
   // bounds the transmittance, |v| <= Vm = Gm (3 Lm + 1), |suffix|, |T R|
   // and |T v| <= Wm Vm, |da| <= Dm = 2 expo Wm Vm / max(1 - am, 1e-8); a sample
   // adds at most Pm = max(Gm Wm, Dm) to a LUT scalar and at most
-  // 256/range 2 Lm (3 Gm Wm + Dm) to ds.  A carried corner holds at most the
-  // samples of the 8 cells around it, each cell at most spc samples; a
-  // carried LUT scalar at most the ray's kmax samples.  A window word gets
-  // one flush per ray of the wave, a LUT word at most kmax Pm per ray of the
-  // workgroup: with flushes below 2^50 units, no word exceeds 2^60.
+  // 256/range 2 Lm (3 Gm Wm + Dm) to ds, so to any corner.
+  // A carried LUT scalar holds at most the ray's kmax samples.  A window word
+  // gets at most kSfxChunk additions per ray of the wave in a chunk, a LUT
+  // word at most kmax Pm per ray of the workgroup: with additions below 2^50
+  // units, no word exceeds 2^60.
   double dscale, lscale;
   double dinv, linv;
   {
@@ -613,9 +498,7 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd(DvCtx c, const floa
     const float Dm = 2.0f * c.expo * Wm * Vm / fmaxf(1.0f - am, 1e-8f);
     const float Pm = fmaxf(Gm * Wm, Dm);
     const float dsm = 256.0f * fabsf(c.inv_range) * 2.0f * Lm * (3.0f * Gm * Wm + Dm);
-    const float stepg = c.step * fminf(fminf(c.inv_sp[0], c.inv_sp[1]), c.inv_sp[2]);
-    const float spc = floorf(1.7320508f / stepg) + 2.0f;
-    const int ed = fix_exponent(dsm * 8.0f * spc);
+    const int ed = fix_exponent(dsm);
     const int el = fix_exponent(Pm * kmax);
     dscale = ldexp(1.0, ed);
     lscale = ldexp(1.0, el);
@@ -652,14 +535,6 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd(DvCtx c, const floa
   }
   const int gbeg = wave_min(N > 0 ? off : INT_MAX), gend = wave_max(N > 0 ? off + (int)N : INT_MIN);
 
-  WinCarry A;
-  A.live = false;
-  A.base = 0;
-  A.lbase = 0;
-  A.ci = A.cj = A.ck = 0;
-#pragma unroll
-  for (int q = 0; q < 8; q++)
-    A.a[q] = 0.0f;
   LutAcc L;
   L.i = -1;
 #pragma unroll
@@ -676,15 +551,11 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd(DvCtx c, const floa
   for (int g0 = gbeg; g0 < gend; g0 += CL) {
     WinSink S;
     S.win = win;
-    S.gd = grad_density;
-    S.scale = dscale;
     int vol;
     for (CL = kSfxChunk;; CL >>= 1) {
       const int g1 = min(gend, g0 + CL);
-      // cells this ray's samples of the chunk touch (replay indices j), with
-      // the sample before the chunk (the carried corners it flushes in this
-      // chunk are there)
-      const int ja = max(g0 - 1 - off, 0), jb = min((int)N - 1, g1 - 1 - off);
+      // cells this ray's samples of the chunk touch (replay indices j)
+      const int ja = max(g0 - off, 0), jb = min((int)N - 1, g1 - 1 - off);
       int bl[3] = {INT_MAX, INT_MAX, INT_MAX}, bh[3] = {INT_MIN, INT_MIN, INT_MIN};
       if (N > 0 && jb >= ja) {
         const int ka = REPLAY ? (int)N - 1 - jb : ja, kb = REPLAY ? (int)N - 1 - ja : jb;
@@ -711,8 +582,6 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd(DvCtx c, const floa
         break;
     }
     const int g1 = min(gend, g0 + CL);
-    if (A.live)
-      A.lbase = win_index(S, A.ci, A.cj, A.ck);
     for (int g = max(g0, off); g < g1 && g < off + (int)N; g++) {
       const int j = g - off;
       const int k = REPLAY ? (int)N - 1 - j : j;
@@ -769,20 +638,23 @@ __global__ __launch_bounds__(kSfxThreads) void k_diffvol_bwd(DvCtx c, const floa
       // density: d(channel)/ds through the LUT slope, then trilinear weights
       const float4 sl = cl.slope;
       const float ds = fmaf(dr, sl.x, fmaf(dg, sl.y, fmaf(db, sl.z, da * sl.w))) * cl.du_ds;
-      win_enter(c, S, A, sm);
+      // the eight trilinear corners
       const float ux = 1.0f - sm.fx, uy = 1.0f - sm.fy, uz = 1.0f - sm.fz;
       const float d0 = ds * ux, d1 = ds * sm.fx;
       const float w00 = uy * uz, w10 = sm.fy * uz, w01 = uy * sm.fz, w11 = sm.fy * sm.fz;
-      A.a[0] = fmaf(d0, w00, A.a[0]);
-      A.a[1] = fmaf(d1, w00, A.a[1]);
-      A.a[2] = fmaf(d0, w10, A.a[2]);
-      A.a[3] = fmaf(d1, w10, A.a[3]);
-      A.a[4] = fmaf(d0, w01, A.a[4]);
-      A.a[5] = fmaf(d1, w01, A.a[5]);
-      A.a[6] = fmaf(d0, w11, A.a[6]);
-      A.a[7] = fmaf(d1, w11, A.a[7]);
-      if (j == (int)N - 1)
-        win_flush(c, S, A); // the ray ends in this chunk
+      const float wq[8] = {d0 * w00, d1 * w00, d0 * w10, d1 * w10, d0 * w01, d1 * w01, d0 * w11, d1 * w11};
+      if (S.lds) {
+        const int lb = win_index(S, sm.ci, sm.cj, sm.ck);
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+          __hip_atomic_fetch_add(S.win + lb + (q & 1) + ((q >> 1) & 1) * S.sx + ((q >> 2) & 1) * S.sxy,
+              to_fix(wq[q], dscale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else if (ds != 0.0f) {
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+          unsafeAtomicAdd(grad_density + sm.base + (uint32_t)(q & 1) + (uint32_t)((q >> 1) & 1) * c.sy
+              + (uint32_t)((q >> 2) & 1) * c.sz, wq[q]);
+      }
     }
     if (S.lds) {
       // add the window to the grid, row by row (x fastest)
