@@ -805,20 +805,11 @@ int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, c
   const uint32_t threads = (uint32_t)kSfxThreads;
   const uint32_t nb = c.img_w > 0 ? (uint32_t)(n / ((size_t)c.img_w * c.img_h)) * c.tiles_per_img
                                   : (n + threads - 1) / threads;
-  // per-workgroup LUT partials live in a grow-only buffer owned by this call site
-  static thread_local void *partials = nullptr;
-  static thread_local size_t cap = 0;
+  // per-workgroup LUT partials: stream-ordered memory, so calls on other
+  // streams or devices never share it
   const size_t need = ((size_t)nb + kLutSlices) * 1024 * 4;
-  if (cap < need) {
-    if (partials) {
-      VRX_HIP(hipStreamSynchronize((hipStream_t)s));
-      VRX_HIP(hipFree(partials));
-    }
-    partials = nullptr;
-    cap = 0;
-    VRX_HIP(hipMalloc(&partials, need));
-    cap = need;
-  }
+  void *partials = nullptr;
+  VRX_HIP(hipMallocAsync(&partials, need, (hipStream_t)s));
   if (suffix)
     hipLaunchKernelGGL(k_diffvol_bwd<false>, dim3(nb), dim3(kSfxThreads), 0, (hipStream_t)s, c,
         v->d_lut, n, d_org, d_dir, d_grad_rgba, d_ckpt, d_grad_density, (float *)partials);
@@ -830,7 +821,9 @@ int vrx_diffvol_backward(const vrx_diffvol *v, uint32_t n, const float *d_org, c
       (const float *)partials, nb, slices);
   hipLaunchKernelGGL(k_reduce_lut, dim3(4), dim3(256), 0, (hipStream_t)s, (const float *)slices,
       d_grad_lut);
-  VRX_LAUNCH_CHECK();
+  const hipError_t launched = hipGetLastError();
+  VRX_HIP(hipFreeAsync(partials, (hipStream_t)s));
+  VRX_HIP(launched);
   return VRX_OK;
 }
 
