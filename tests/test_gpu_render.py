@@ -215,3 +215,60 @@ def test_volume_with_surface_in_front_and_inside(require_gpu):
     np.testing.assert_array_equal(g["primitiveId"], o["primitiveId"])
     np.testing.assert_array_equal(g["objectId"], o["objectId"])
     np.testing.assert_allclose(g["depth"], o["depth"], rtol=1e-5)
+
+
+def _quad_grid(n=33, seed=4):
+    """A heightfield as an indexed quad mesh: quad (a, b, c, d) per grid cell."""
+    verts, _ = S.gen_heightfield(n, seed)
+    i, j = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    a = (i * n + j).reshape(-1)
+    quads = np.stack([a, a + 1, a + n + 1, a + n], 1).astype(np.uint32)
+    return verts, quads
+
+
+def test_quads_trace_as_the_reference_triangle_split(require_gpu):
+    """Quad geometry (Quad.cpp:124-145): each quad is traced as the triangles
+    (x, y, w) and (z, w, y); primitiveId is the triangle index.  The quad
+    frame equals the frame of that triangle mesh bit for bit, and the oracle's
+    render of it; the soup form (4 vertices per quad) too."""
+    verts, quads = _quad_grid()
+    cam = S.CameraDesc((0.5, -1.5, 1.2), S.look_at((0.5, -1.5, 1.2), (0.5, 0.5, 0.0)), (0, 0, 1),
+                       math.radians(45.0), 4 / 3)
+    tris = S.quads_to_triangles(quads, len(verts))
+    sc_q = S.SceneDesc(geoms=[S.GeomDesc("quad", verts, quads, surface_id=3)], world_geoms=[0])
+    sc_t = S.SceneDesc(geoms=[S.GeomDesc("triangle", verts, tris, surface_id=3)], world_geoms=[0])
+    gq = render_gpu(sc_q, cam, 160, 120)
+    gt = render_gpu(sc_t, cam, 160, 120)
+    o = orc.OracleScene(sc_t).render(cam, 160, 120)
+    compare(gq, o)
+    for ch in ["color", "depth", "primitiveId", "objectId", "instanceId"]:
+        np.testing.assert_array_equal(gq[ch], gt[ch], err_msg=ch)
+    hit = gq["depth"] < 1e29
+    assert hit.mean() > 0.05 and len(np.unique(gq["primitiveId"][hit])) > 200
+    # soup: quad q is vertices 4q .. 4q+3
+    vs = verts[quads.reshape(-1)]
+    sc_qs = S.SceneDesc(geoms=[S.GeomDesc("quad", vs, surface_id=3)], world_geoms=[0])
+    sc_ts = S.SceneDesc(geoms=[S.GeomDesc("triangle", vs, S.quads_to_triangles(None, len(vs)),
+                                          surface_id=3)], world_geoms=[0])
+    gs = render_gpu(sc_qs, cam, 160, 120)
+    compare(gs, orc.OracleScene(sc_ts).render(cam, 160, 120))
+
+
+def test_quad_soup_must_be_a_multiple_of_four(require_gpu):
+    dev = A.Device()
+    s = S.AnariScene(dev)
+    g = s.keep(dev.new("Geometry", "quad"))
+    pos = np.zeros((6, 3), np.float32)
+    dev.set(g, "vertex.position", s.keep(dev.array(pos, A.FLOAT32_VEC3)), A.ARRAY1D)
+    dev.commit(g)
+    w = s.keep(dev.new("World"))
+    sf = s.keep(dev.new("Surface"))
+    dev.set(sf, "geometry", g, A.GEOMETRY)
+    dev.commit(sf)
+    dev.set(w, "surface", s.keep(dev.array([sf], A.SURFACE)), A.ARRAY1D)
+    dev.commit(w)
+    dev.get_property(w, "bounds", A.FLOAT32_BOX3)
+    errs = " ".join(str(e) for e in dev.errors())
+    assert "non-multiple of 4" in errs, errs
+    s.release()
+    dev.close()

@@ -348,3 +348,13 @@ def test_diffvol_forward_equals_volume_march():
     # stay at the level of that one rounding
     np.testing.assert_allclose(got, img.reshape(-1, 4), rtol=1e-7, atol=1e-9)
     assert img[..., 3].max() > 0.3
+
+
+def test_quad_split_matches_reference_rule():
+    """Quad.cpp:124-145: indexed quad (x, y, z, w) -> triangles (x, y, w) and
+    (z, w, y); soup quad q -> 4q + (0, 1, 3) and 4q + (2, 3, 1)."""
+    q = np.array([[10, 11, 12, 13], [7, 3, 5, 1]], np.uint32)
+    t = S.quads_to_triangles(q, 20)
+    assert t.tolist() == [[10, 11, 13], [12, 13, 11], [7, 3, 1], [5, 1, 3]]
+    ts = S.quads_to_triangles(None, 8)
+    assert ts.tolist() == [[0, 1, 3], [2, 3, 1], [4, 5, 7], [6, 7, 5]]

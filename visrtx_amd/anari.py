@@ -27,7 +27,7 @@ CAMERA, FRAME, GEOMETRY, GROUP, INSTANCE = 507, 508, 509, 510, 511
 LIGHT, MATERIAL, RENDERER, SURFACE, SAMPLER = 512, 513, 514, 515, 516
 SPATIAL_FIELD, VOLUME, WORLD = 517, 518, 519
 INT32 = 1016
-UINT32, UINT32_VEC2, UINT32_VEC3 = 1020, 1021, 1022
+UINT32, UINT32_VEC2, UINT32_VEC3, UINT32_VEC4 = 1020, 1021, 1022, 1023
 UINT64 = 1028
 UFIXED8 = 1036
 UFIXED8_VEC4 = 1039
@@ -43,7 +43,8 @@ _OBJ_TYPES = {ARRAY1D, ARRAY2D, ARRAY3D, CAMERA, FRAME, GEOMETRY, GROUP, INSTANC
 _ELEM = {  # element type -> (numpy dtype, components)
     FLOAT32: (np.float32, 1), FLOAT32_VEC2: (np.float32, 2), FLOAT32_VEC3: (np.float32, 3),
     FLOAT32_VEC4: (np.float32, 4), UINT32: (np.uint32, 1), UINT32_VEC2: (np.uint32, 2),
-    UINT32_VEC3: (np.uint32, 3), UFIXED8: (np.uint8, 1), INT32: (np.int32, 1),
+    UINT32_VEC3: (np.uint32, 3), UINT32_VEC4: (np.uint32, 4), UFIXED8: (np.uint8, 1),
+    INT32: (np.int32, 1),
 }
 
 STATUS_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,

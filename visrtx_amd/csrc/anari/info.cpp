@@ -77,6 +77,13 @@ std::vector<ObjDef> &tables()
               {"vertex.normal", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "FLOAT32_VEC3 shading normals"},
               {"primitive.id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32 primitiveId values"},
               {"cullBackfaces", ANARI_BOOL, false, &kFalse, nullptr, nullptr, "skip back-facing hits"}}},
+      {ANARI_GEOMETRY, "quad", "quads, each traced as two triangles (Quad.cpp)",
+          {{"vertex.position", ANARI_ARRAY1D, true, nullptr, nullptr, nullptr, "FLOAT32_VEC3 vertices"},
+              {"primitive.index", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32_VEC4 quads"},
+              {"vertex.normal", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "FLOAT32_VEC3 shading normals"},
+              {"primitive.id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr,
+                  "UINT32 primitiveId values, indexed by triangle (2 per quad)"},
+              {"cullBackfaces", ANARI_BOOL, false, &kFalse, nullptr, nullptr, "skip back-facing hits"}}},
       {ANARI_MATERIAL, "matte", "constant-colour matte material (Matte.cpp)",
           {{"color", ANARI_FLOAT32_VEC3, false, kGrey3, nullptr, nullptr, "base colour"},
               {"opacity", ANARI_FLOAT32, false, &kOne, nullptr, nullptr, "opacity"},

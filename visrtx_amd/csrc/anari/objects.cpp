@@ -296,13 +296,16 @@ const void *Array::device()
 }
 
 // ---------------------------------------------------------------------------
-// Geometry (scene/surface/geometry/{Geometry,Triangle,Sphere,Cylinder})
+// Geometry (scene/surface/geometry/{Geometry,Triangle,Quad,Sphere,Cylinder})
 // ---------------------------------------------------------------------------
 Geometry::Geometry(Device *d, const std::string &sub) : Object(d, ANARI_GEOMETRY, sub)
 {
   if (sub == "triangle")
     ptype = VRX_TRIANGLE;
-  else if (sub == "sphere")
+  else if (sub == "quad") {
+    ptype = VRX_TRIANGLE;
+    quad = true;
+  } else if (sub == "sphere")
     ptype = VRX_SPHERE;
   else if (sub == "cylinder")
     ptype = VRX_CYLINDER;
@@ -310,7 +313,33 @@ Geometry::Geometry(Device *d, const std::string &sub) : Object(d, ANARI_GEOMETRY
     globalRadius = 1.0f; // Cylinder.cpp:49
 }
 
-Geometry::~Geometry() { freeBVH(); }
+Geometry::~Geometry()
+{
+  freeBVH();
+  vrx_free(dQuadTri);
+}
+
+const uint32_t *Geometry::deviceIndex()
+{
+  if (!quad)
+    return index ? (const uint32_t *)index->device() : nullptr;
+  if (quadTriDirty) {
+    const size_t bytes = quadTri.size() * sizeof(uint32_t);
+    if (bytes > capQuadTri) {
+      vrx_free(dQuadTri);
+      dQuadTri = nullptr;
+      capQuadTri = 0;
+      if (!dev->check(vrx_malloc(&dQuadTri, bytes), this, "quad index allocation"))
+        return nullptr;
+      capQuadTri = bytes;
+    }
+    if (bytes && !dev->check(vrx_memcpy_sync(dQuadTri, quadTri.data(), bytes, VRX_H2D), this,
+                     "quad index upload"))
+      return nullptr;
+    quadTriDirty = false;
+  }
+  return (const uint32_t *)dQuadTri;
+}
 
 void Geometry::freeBVH()
 {
@@ -343,6 +372,8 @@ void Geometry::commit()
   index = arrayParam(this, "primitive.index");
   primId = arrayParam(this, "primitive.id");
   normal = ptype == VRX_TRIANGLE ? arrayParam(this, "vertex.normal") : nullptr;
+  quadTri.clear();
+  quadTriDirty = true;
   cull = get<int>("cullBackfaces", ANARI_BOOL, 0) != 0;
   if (ptype == VRX_SPHERE) {
     radius = arrayParam(this, "vertex.radius");
@@ -365,7 +396,53 @@ void Geometry::commit()
     return;
   }
   const uint64_t nv = pos->count();
-  if (ptype == VRX_TRIANGLE) {
+  if (quad) {
+    // Quad.cpp:57-63 (soup must be a multiple of 4), :124-145 (the split)
+    if (index && index->elem != ANARI_UINT32_VEC4) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+          "'primitive.index' must be ANARI_UINT32_VEC4 on quad geometry");
+      return;
+    }
+    if (!index && nv % 4 != 0) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+          "'vertex.position' on quad geometry is a non-multiple of 4 without 'primitive.index' "
+          "present");
+      return;
+    }
+    const uint64_t nq = index ? index->count() : nv / 4;
+    if (2 * nq >= 0x7FFFFFFFull) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT, "too many quads");
+      return;
+    }
+    const uint32_t *ix = index ? (const uint32_t *)index->host() : nullptr;
+    uint32_t mx = 0;
+    quadTri.resize(6 * nq);
+    for (uint64_t q = 0; q < nq; q++) {
+      uint32_t x, y, z, w;
+      if (ix) {
+        x = ix[4 * q];
+        y = ix[4 * q + 1];
+        z = ix[4 * q + 2];
+        w = ix[4 * q + 3];
+      } else {
+        x = (uint32_t)(4 * q);
+        y = x + 1;
+        z = x + 2;
+        w = x + 3;
+      }
+      mx = std::max(std::max(mx, std::max(x, y)), std::max(z, w));
+      const uint32_t t[6] = {x, y, w, z, w, y};
+      for (int k = 0; k < 6; k++)
+        quadTri[6 * q + k] = t[k];
+    }
+    if (nq && (uint64_t)mx >= nv) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+          "'primitive.index' refers to vertex %u but only %llu vertices exist", mx,
+          (unsigned long long)nv);
+      return;
+    }
+    n = (uint32_t)(2 * nq);
+  } else if (ptype == VRX_TRIANGLE) {
     if (index) {
       if (index->elem != ANARI_UINT32_VEC3) {
         report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
@@ -397,7 +474,7 @@ void Geometry::commit()
     }
     n = (uint32_t)(index ? index->count() : nv / 2);
   }
-  if (index) {
+  if (index && !quad) {
     // every index must address an existing vertex (checked once per commit)
     const uint32_t *ix = (const uint32_t *)index->host();
     const size_t ni = index->count() * (ptype == VRX_TRIANGLE ? 3 : (ptype == VRX_CYLINDER ? 2 : 1));
@@ -458,7 +535,9 @@ int Geometry::buildBVH(vrx_stream s)
   p.type = ptype;
   p.n = n;
   p.d_position = (const float *)pos->device();
-  p.d_index = index ? (const uint32_t *)index->device() : nullptr;
+  p.d_index = deviceIndex();
+  if ((quad || index) && !p.d_index && n)
+    return -1;
   p.d_radius = radius ? (const float *)radius->device() : nullptr;
   p.global_radius = globalRadius;
   if (primId)
@@ -946,7 +1025,7 @@ int World::rebuild()
       vg.bvh = g->bvh;
       vg.bvh.d_wide = g->wideValid ? g->dWide : nullptr;
       vg.d_position = (const float *)g->pos->device();
-      vg.d_index = g->index ? (const uint32_t *)g->index->device() : nullptr;
+      vg.d_index = g->deviceIndex();
       vg.d_radius = g->radius ? (const float *)g->radius->device() : nullptr;
       vg.global_radius = g->globalRadius;
       vg.d_primitive_id = g->primId ? (const uint32_t *)g->primId->device() : nullptr;

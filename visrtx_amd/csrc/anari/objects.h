@@ -118,6 +118,7 @@ struct Geometry : Object {
   bool isValid() const override { return valid; }
 
   int ptype{-1};
+  bool quad{false}; // "quad": built and traced as two triangles per quad
   bool valid{false};
   Array *pos{nullptr}, *index{nullptr}, *radius{nullptr}, *primId{nullptr}, *normal{nullptr};
   float globalRadius{0.01f};
@@ -134,6 +135,13 @@ struct Geometry : Object {
   void *dWide{nullptr};
   size_t capWide{0};
   bool wideValid{false};
+  // triangle indices generated from the quads (Quad.cpp:124-145)
+  std::vector<uint32_t> quadTri;
+  void *dQuadTri{nullptr};
+  size_t capQuadTri{0};
+  bool quadTriDirty{true};
+  // device triangle/segment/centre indices (the generated ones for quads)
+  const uint32_t *deviceIndex();
   int buildBVH(vrx_stream s);
   int allocWide();
   int buildWide(vrx_stream s);

@@ -24,7 +24,7 @@ from . import anari as A
 # ---------------------------------------------------------------------------
 @dataclass
 class GeomDesc:
-    type: str                       # 'triangle' | 'sphere' | 'cylinder'
+    type: str                       # 'triangle' | 'quad' | 'sphere' | 'cylinder'
     position: np.ndarray            # (V, 3) float32
     index: np.ndarray | None = None
     radius: np.ndarray | None = None
@@ -41,6 +41,8 @@ class GeomDesc:
     def num_prims(self) -> int:
         if self.type == "triangle":
             return len(self.index) if self.index is not None else len(self.position) // 3
+        if self.type == "quad":   # two triangles per quad
+            return 2 * (len(self.index) if self.index is not None else len(self.position) // 4)
         if self.type == "sphere":
             return len(self.index) if self.index is not None else len(self.position)
         return len(self.index) if self.index is not None else len(self.position) // 2
@@ -227,7 +229,19 @@ def batched_workload(views: int = 64, res: int = 512, instances: int = 256, mesh
 # ---------------------------------------------------------------------------
 # ANARI construction
 # ---------------------------------------------------------------------------
-_GEOM_SUB = {"triangle": "triangle", "sphere": "sphere", "cylinder": "cylinder"}
+_GEOM_SUB = {"triangle": "triangle", "quad": "quad", "sphere": "sphere", "cylinder": "cylinder"}
+
+
+def quads_to_triangles(index: np.ndarray | None, num_vertices: int) -> np.ndarray:
+    """The two triangles of each quad, as the device splits them
+    (scene/surface/geometry/Quad.cpp:124-145): (x, y, w) and (z, w, y) of
+    quad (x, y, z, w); a soup uses quad q = vertices 4q .. 4q+3."""
+    q = (np.asarray(index, np.uint32).reshape(-1, 4) if index is not None
+         else np.arange(num_vertices, dtype=np.uint32).reshape(-1, 4))
+    t = np.empty((len(q), 2, 3), np.uint32)
+    t[:, 0] = q[:, [0, 1, 3]]
+    t[:, 1] = q[:, [2, 3, 1]]
+    return t.reshape(-1, 3)
 
 
 class AnariScene:
@@ -259,7 +273,8 @@ def _surface(dev: A.Device, s: AnariScene, g: GeomDesc):
     s.geoms.append(geom)
     dev.set(geom, "vertex.position", s.keep(dev.array(g.position, A.FLOAT32_VEC3)), A.ARRAY1D)
     if g.index is not None:
-        et = {"triangle": A.UINT32_VEC3, "sphere": A.UINT32, "cylinder": A.UINT32_VEC2}[g.type]
+        et = {"triangle": A.UINT32_VEC3, "quad": A.UINT32_VEC4, "sphere": A.UINT32,
+              "cylinder": A.UINT32_VEC2}[g.type]
         dev.set(geom, "primitive.index", s.keep(dev.array(g.index, et)), A.ARRAY1D)
     if g.radius is not None:
         name = "vertex.radius" if g.type == "sphere" else "primitive.radius"
