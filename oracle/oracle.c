@@ -72,7 +72,8 @@ void orc_pcg_seed(orc_pcg *r, uint64_t initstate, uint64_t initseq)
 float orc_pcg_float(orc_pcg *r) { return (float)(orc_pcg_next(r) >> 8) * 0x1p-24f; }
 
 /* ------------------------------------------------------------------------- */
-/* Primitive bounds (Sphere.cu:82-110, Cylinder.cpp:53-102, Triangle.cpp)    */
+/* Primitive bounds (Sphere.cu:82-110, Cylinder.cpp:53-102, Cone.cpp:98-105, */
+/* Triangle.cpp)                                                             */
 /* ------------------------------------------------------------------------- */
 static void prim_box(int type, uint32_t i, const float *pos, const uint32_t *index,
     const float *radius, float global_radius, v3 *lo, v3 *hi)
@@ -88,6 +89,14 @@ static void prim_box(int type, uint32_t i, const float *pos, const uint32_t *ind
     uint32_t b = index ? index[2 * (size_t)i + 1] : 2 * i + 1;
     v3 p0 = ld3(pos + 3 * (size_t)a), p1 = ld3(pos + 3 * (size_t)b);
     float r = fabsf(radius ? radius[i] : global_radius);
+    v3 mn = vmin(p0, p1), mx = vmax(p0, p1);
+    *lo = mk(mn.x - r, mn.y - r, mn.z - r);
+    *hi = mk(mx.x + r, mx.y + r, mx.z + r);
+  } else if (type == ORC_CONE) {
+    uint32_t a = index ? index[2 * (size_t)i] : 2 * i;
+    uint32_t b = index ? index[2 * (size_t)i + 1] : 2 * i + 1;
+    v3 p0 = ld3(pos + 3 * (size_t)a), p1 = ld3(pos + 3 * (size_t)b);
+    float r = fmaxf(radius[a], radius[b]);
     v3 mn = vmin(p0, p1), mx = vmax(p0, p1);
     *lo = mk(mn.x - r, mn.y - r, mn.z - r);
     *hi = mk(mx.x + r, mx.y + r, mx.z + r);
@@ -433,6 +442,67 @@ static inline int isect_cylinder(v3 ro, v3 rd, v3 p0, v3 p1, float radius, int c
   return valid[0] | valid[1] | valid[2];
 }
 
+/* Intersectors_ptx.cu:192-246: the cap disc at the end beyond which the ray
+ * origin projects (m1 < 0: the p0 end, m2 > 0: the p1 end), then the lateral
+ * surface of the capped cone (I. Quilez's closed form; the quadratic
+ * k2 t^2 + 2 k1 t + k0 = 0 at its smaller root, accepted for 0 < y < m0).
+ * Up to two reports in order (cap, side); normals unnormalised. */
+static inline int isect_cone(v3 ro, v3 rd, v3 p0, v3 p1, float ra, float rb, float ts[2],
+    v3 ns[2], int valid[2])
+{
+  valid[0] = valid[1] = 0;
+  const v3 ba = sub(p1, p0), oa = sub(ro, p0), ob = sub(ro, p1);
+  const float m0 = dot(ba, ba), m1 = dot(oa, ba), m2 = dot(ob, ba), m3 = dot(rd, ba);
+  const float rinv = 1.0f / sqrtf(m0);
+  if (m1 < 0.0f) {
+    const v3 q = sub(fmul(m3, oa), fmul(m1, rd));
+    if (dot(q, q) < (ra * ra) * (m3 * m3)) {
+      ts[0] = -m1 / m3;
+      ns[0] = fmul(-rinv, ba);
+      valid[0] = 1;
+    }
+  } else if (m2 > 0.0f) {
+    const v3 q = sub(fmul(m3, ob), fmul(m2, rd));
+    if (dot(q, q) < (rb * rb) * (m3 * m3)) {
+      ts[0] = -m2 / m3;
+      ns[0] = fmul(rinv, ba);
+      valid[0] = 1;
+    }
+  }
+  const float m4 = dot(rd, oa), m5 = dot(oa, oa);
+  const float rr = ra - rb;
+  const float hy = m0 + rr * rr;
+  const float m00 = m0 * m0;
+  const float k2 = m00 - (m3 * m3) * hy;
+  const float k1 = (m00 * m4 - (m1 * m3) * hy) + (m0 * ra) * (rr * m3);
+  const float k0 = (m00 * m5 - (m1 * m1) * hy) + (m0 * ra) * ((rr * m1) * 2.0f - m0 * ra);
+  const float h = k1 * k1 - k2 * k0;
+  if (h >= 0.0f) {
+    const float t = (-k1 - sqrtf(h)) / k2;
+    const float y = m1 + t * m3;
+    if (y > 0.0f && y < m0) {
+      const v3 pt = add(oa, fmul(t, rd));
+      ts[1] = t;
+      ns[1] = sub(fmul(m0, add(fmul(m0, pt), fmul(rr * ra, ba))), fmul(hy * y, ba));
+      valid[1] = 1;
+    }
+  }
+  return valid[0] | valid[1];
+}
+
+int orc_intersect_cone(const float o[3], const float d[3], const float p0[3], const float p1[3],
+    float ra, float rb, float *ts, float *ns, int *valid)
+{
+  v3 n[2];
+  int r = isect_cone(ld3(o), ld3(d), ld3(p0), ld3(p1), ra, rb, ts, n, valid);
+  for (int i = 0; i < 2; i++) {
+    if (valid[i]) {
+      ns[3 * i] = n[i].x; ns[3 * i + 1] = n[i].y; ns[3 * i + 2] = n[i].z;
+    }
+  }
+  return r;
+}
+
 int orc_intersect_cylinder(const float o[3], const float d[3], const float p0[3],
     const float p1[3], float radius, int caps, float *ts, float *ns, int *valid)
 {
@@ -693,6 +763,29 @@ static inline int test_prim(tctx *c, uint32_t p, float tmin, float *tmax, best_t
     if (isect_cylinder(c->o, c->d, ld3(G->position + 3 * (size_t)a),
             ld3(G->position + 3 * (size_t)b), r, G->caps, ts, ns, valid)) {
       for (int k = 0; k < 3; k++) {
+        if (!valid[k])
+          continue;
+        float t = ts[k];
+        if (t >= tmin && t <= *tmax) {
+          if (c->any)
+            return 1;
+          if (better(best, t, key)) {
+            best->found = 1; best->t = t; best->key = key; best->entry = c->e; best->prim = p;
+            best->a0 = ns[k].x; best->a1 = ns[k].y; best->a2 = ns[k].z;
+            *tmax = t;
+          }
+        }
+      }
+    }
+  } else if (G->type == ORC_CONE) {
+    uint32_t a = G->index ? G->index[2 * (size_t)p] : 2 * p;
+    uint32_t b = G->index ? G->index[2 * (size_t)p + 1] : 2 * p + 1;
+    float ts[2];
+    v3 ns[2];
+    int valid[2];
+    if (isect_cone(c->o, c->d, ld3(G->position + 3 * (size_t)a), ld3(G->position + 3 * (size_t)b),
+            G->radius[a], G->radius[b], ts, ns, valid)) {
+      for (int k = 0; k < 2; k++) {
         if (!valid[k])
           continue;
         float t = ts[k];

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-ORC_TRIANGLE, ORC_SPHERE, ORC_CYLINDER = 0, 1, 2
+ORC_TRIANGLE, ORC_SPHERE, ORC_CYLINDER, ORC_CONE = 0, 1, 2, 4
 ORC_RAYCAST, ORC_AO = 0, 1
 ORC_COLOR_F32, ORC_COLOR_SRGB8, ORC_COLOR_UNORM8 = 0, 1, 2
 LEAF = 0x80000000
@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
         L.orc_pack_unorm4x8.argtypes = [vp]
         L.orc_linear_to_srgb.argtypes = [vp, vp]
         L.orc_intersect_triangle.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_intersect_cone.argtypes = [vp, vp, vp, vp, C.c_float, C.c_float, vp, vp, vp]
         L.orc_intersect_triangle_f64.argtypes = [vp, vp, vp, vp, vp]
         L.orc_sample_field.argtypes = [vp, vp]
         L.orc_classify.argtypes = [vp, C.c_double, vp]
@@ -200,7 +201,8 @@ def expand_bits10(v: int) -> int:
 # ---------------------------------------------------------------------------
 # scenes
 # ---------------------------------------------------------------------------
-_TYPES = {"triangle": ORC_TRIANGLE, "sphere": ORC_SPHERE, "cylinder": ORC_CYLINDER}
+_TYPES = {"triangle": ORC_TRIANGLE, "sphere": ORC_SPHERE, "cylinder": ORC_CYLINDER,
+          "cone": ORC_CONE}
 IDENTITY12 = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], np.float32)
 
 

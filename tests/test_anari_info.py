@@ -62,7 +62,7 @@ def subtypes(L, typ):
 
 
 def test_subtypes(lib):
-    assert set(subtypes(lib, A.GEOMETRY)) >= {"triangle", "quad", "sphere", "cylinder"}
+    assert set(subtypes(lib, A.GEOMETRY)) >= {"triangle", "quad", "sphere", "cylinder", "cone"}
     assert set(subtypes(lib, A.RENDERER)) >= {"raycast", "ao", "default"}
 
 

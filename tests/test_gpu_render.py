@@ -272,3 +272,48 @@ def test_quad_soup_must_be_a_multiple_of_four(require_gpu):
     assert "non-multiple of 4" in errs, errs
     s.release()
     dev.close()
+
+
+def _cone_scene(indexed):
+    """3,000 capped cones with per-vertex radii (Cone.cpp, Intersectors_ptx.cu:192-246)."""
+    rng = np.random.default_rng(6)
+    n = 3000
+    p0 = rng.random((n, 3), dtype=np.float32)
+    p1 = p0 + rng.normal(0, 0.05, (n, 3)).astype(np.float32)
+    pos = np.stack([p0, p1], 1).reshape(-1, 3)
+    rad = rng.uniform(0.0, 0.015, 2 * n).astype(np.float32)
+    idx = None
+    if indexed:   # reversed pairs: cone i joins vertices 2i+1 -> 2i
+        idx = np.stack([2 * np.arange(n) + 1, 2 * np.arange(n)], 1).astype(np.uint32)
+    g = S.GeomDesc("cone", pos, idx, radius=rad, surface_id=5)
+    sc = S.SceneDesc(geoms=[g], world_geoms=[0], world_id=2)
+    cam = S.CameraDesc((0.5, 0.5, -1.0), (0, 0, 1), (0, 1, 0), math.radians(50), 4 / 3)
+    return sc, cam
+
+
+@pytest.mark.parametrize("indexed", [False, True])
+def test_cones_raycast(require_gpu, indexed):
+    """Cones vs the oracle: ids bit-exact, depth and colour within 1e-4."""
+    sc, cam = _cone_scene(indexed)
+    g = render_gpu(sc, cam, 320, 240)
+    o = orc.OracleScene(sc).render(cam, 320, 240)
+    compare(g, o)
+    assert (g["depth"] < 1e29).mean() > 0.05
+
+
+def test_cones_with_other_types_and_ao(require_gpu):
+    """A world mixing cones, quads and spheres (generic kernel) under the AO
+    renderer, against the oracle."""
+    sc_c, cam = _cone_scene(False)
+    verts, quads = _quad_grid(17, seed=8)
+    c, r = S.gen_spheres(2000, seed=3, rmin=2e-3, rmax=6e-3)
+    geoms = [sc_c.geoms[0], S.GeomDesc("quad", verts * 0.5 + 0.25, quads, surface_id=7),
+             S.GeomDesc("sphere", c, radius=r, surface_id=8)]
+    sc = S.SceneDesc(geoms=geoms, world_geoms=[0, 1, 2], world_id=1)
+    tris = S.quads_to_triangles(quads, len(verts))
+    sc_o = S.SceneDesc(geoms=[geoms[0], S.GeomDesc("triangle", verts * 0.5 + 0.25, tris, surface_id=7),
+                              geoms[2]], world_geoms=[0, 1, 2], world_id=1)
+    rp = dict(pixelSamples=2, ambientSamples=2, ambientOcclusionDistance=0.05)
+    g = render_gpu(sc, cam, 160, 120, kind="ao", **rp)
+    o = orc.OracleScene(sc_o).render(cam, 160, 120, kind="ao", **rp)
+    compare(g, o)

@@ -358,3 +358,46 @@ def test_quad_split_matches_reference_rule():
     assert t.tolist() == [[10, 11, 13], [12, 13, 11], [7, 3, 1], [5, 1, 3]]
     ts = S.quads_to_triangles(None, 8)
     assert ts.tolist() == [[0, 1, 3], [2, 3, 1], [4, 5, 7], [6, 7, 5]]
+
+
+def _cone(o, d, p0, p1, ra, rb):
+    ts = np.zeros(2, np.float32)
+    ns = np.zeros(6, np.float32)
+    valid = np.zeros(2, np.int32)
+    keep = [np.asarray(v, np.float32) for v in (o, d, p0, p1)]
+    r = orc.lib().orc_intersect_cone(*[k.ctypes.data for k in keep], ra, rb,
+                                     ts.ctypes.data, ns.ctypes.data, valid.ctypes.data)
+    return r, ts, ns.reshape(2, 3), valid
+
+
+def test_cone_intersection_closed_forms():
+    """Intersectors_ptx.cu:192-246 restated: equal radii reduce to a cylinder,
+    a ray down the axis meets the near cap, a pointed cone's side is hit where
+    its radius equals the ray's distance from the axis, and misses miss."""
+    # ra == rb: side hit at distance 0.5 from the x axis
+    r, ts, ns, v = _cone((0, 0, -5), (0, 0, 1), (-1, 0, 0), (1, 0, 0), 0.5, 0.5)
+    assert v[1] and not v[0] and ts[1] == pytest.approx(4.5, rel=1e-6)
+    n = ns[1] / np.linalg.norm(ns[1])
+    np.testing.assert_allclose(n, [0, 0, -1], atol=1e-6)
+    # down the axis from beyond p0: the p0 cap, normal -axis
+    r, ts, ns, v = _cone((-3, 0, 0), (1, 0, 0), (-1, 0, 0), (1, 0, 0), 0.5, 0.25)
+    assert v[0] and ts[0] == pytest.approx(2.0, rel=1e-6)
+    np.testing.assert_allclose(ns[0] / np.linalg.norm(ns[0]), [-1, 0, 0], atol=1e-6)
+    # from beyond p1: the p1 cap
+    r, ts, ns, v = _cone((4, 0.1, 0), (-1, 0, 0), (-1, 0, 0), (1, 0, 0), 0.5, 0.25)
+    assert v[0] and ts[0] == pytest.approx(3.0, rel=1e-6)
+    # pointed cone along z (radius 1 at z=0, 0 at z=2): at z = 1 the radius is 0.5
+    r, ts, ns, v = _cone((5, 0, 1), (-1, 0, 0), (0, 0, 0), (0, 0, 2), 1.0, 0.0)
+    assert v[1] and ts[1] == pytest.approx(4.5, rel=1e-6)
+    n = ns[1] / np.linalg.norm(ns[1])
+    np.testing.assert_allclose(n, np.array([2.0, 0.0, 1.0]) / math.sqrt(5.0), atol=1e-6)
+    # beside the pointed cone: miss
+    r, ts, ns, v = _cone((5, 0.5, 1.0), (-1, 0, 0), (0, 0, 0), (0, 0, 2), 0.2, 0.0)
+    assert not v.any()
+
+
+def test_cone_bounds_use_the_larger_end_radius():
+    """Cone.cpp:98-105: box = min/max of the end points -/+ max(r0, r1)."""
+    pos = np.array([[0, 0, 0], [1, 2, 3]], np.float32)
+    box = orc.bounds(orc.ORC_CONE, pos, radius=np.array([0.5, 0.25], np.float32))
+    np.testing.assert_allclose(box.reshape(-1), [-0.5, -0.5, -0.5, 1.5, 2.5, 3.5])

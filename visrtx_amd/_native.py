@@ -41,7 +41,7 @@ def _load(path: str) -> C.CDLL:
 # ---------------------------------------------------------------------------
 # C structs of include/vrx_hip.h
 # ---------------------------------------------------------------------------
-VRX_TRIANGLE, VRX_SPHERE, VRX_CYLINDER, VRX_BOX = 0, 1, 2, 3
+VRX_TRIANGLE, VRX_SPHERE, VRX_CYLINDER, VRX_BOX, VRX_CONE = 0, 1, 2, 3, 4
 VRX_H2D, VRX_D2H, VRX_D2D = 0, 1, 2
 VRX_RENDER_RAYCAST, VRX_RENDER_AO = 0, 1
 VRX_COLOR_F32, VRX_COLOR_SRGB8, VRX_COLOR_UNORM8 = 0, 1, 2

@@ -77,6 +77,12 @@ std::vector<ObjDef> &tables()
               {"vertex.normal", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "FLOAT32_VEC3 shading normals"},
               {"primitive.id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32 primitiveId values"},
               {"cullBackfaces", ANARI_BOOL, false, &kFalse, nullptr, nullptr, "skip back-facing hits"}}},
+      {ANARI_GEOMETRY, "cone", "capped cones between vertex pairs (Cone.cpp)",
+          {{"vertex.position", ANARI_ARRAY1D, true, nullptr, nullptr, nullptr, "FLOAT32_VEC3 end points"},
+              {"vertex.radius", ANARI_ARRAY1D, true, nullptr, nullptr, nullptr, "FLOAT32 per-vertex radii"},
+              {"primitive.index", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32_VEC2 end point pairs"},
+              {"caps", ANARI_STRING, false, "none", nullptr, nullptr, "read, as the reference does; end discs are always tested"},
+              {"primitive.id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32 primitiveId values"}}},
       {ANARI_GEOMETRY, "quad", "quads, each traced as two triangles (Quad.cpp)",
           {{"vertex.position", ANARI_ARRAY1D, true, nullptr, nullptr, nullptr, "FLOAT32_VEC3 vertices"},
               {"primitive.index", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32_VEC4 quads"},

@@ -296,7 +296,7 @@ const void *Array::device()
 }
 
 // ---------------------------------------------------------------------------
-// Geometry (scene/surface/geometry/{Geometry,Triangle,Quad,Sphere,Cylinder})
+// Geometry (scene/surface/geometry/{Geometry,Triangle,Quad,Sphere,Cylinder,Cone})
 // ---------------------------------------------------------------------------
 Geometry::Geometry(Device *d, const std::string &sub) : Object(d, ANARI_GEOMETRY, sub)
 {
@@ -309,6 +309,8 @@ Geometry::Geometry(Device *d, const std::string &sub) : Object(d, ANARI_GEOMETRY
     ptype = VRX_SPHERE;
   else if (sub == "cylinder")
     ptype = VRX_CYLINDER;
+  else if (sub == "cone")
+    ptype = VRX_CONE;
   if (ptype == VRX_CYLINDER)
     globalRadius = 1.0f; // Cylinder.cpp:49
 }
@@ -382,6 +384,17 @@ void Geometry::commit()
     radius = arrayParam(this, "primitive.radius");
     globalRadius = get<float>("radius", ANARI_FLOAT32, 1.0f);
     caps = getString("caps", "none") != "none"; // Cylinder.cpp:47
+  } else if (ptype == VRX_CONE) {
+    // Cone.cpp:42-62: per-vertex radii are required.  "caps" is read there
+    // but the intersector always tests the end discs (Intersectors_ptx.cu:
+    // 215-227), so it changes nothing here either.
+    radius = arrayParam(this, "vertex.radius");
+    caps = getString("caps", "none") != "none";
+    if (!radius) {
+      report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+          "missing required parameter 'vertex.radius' on cone geometry");
+      return;
+    }
   } else {
     radius = nullptr;
   }
@@ -469,7 +482,7 @@ void Geometry::commit()
   } else {
     if (index && index->elem != ANARI_UINT32_VEC2) {
       report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
-          "'primitive.index' must be ANARI_UINT32_VEC2 on cylinder geometry");
+          "'primitive.index' must be ANARI_UINT32_VEC2 on %s geometry", subtype.c_str());
       return;
     }
     n = (uint32_t)(index ? index->count() : nv / 2);
@@ -477,7 +490,8 @@ void Geometry::commit()
   if (index && !quad) {
     // every index must address an existing vertex (checked once per commit)
     const uint32_t *ix = (const uint32_t *)index->host();
-    const size_t ni = index->count() * (ptype == VRX_TRIANGLE ? 3 : (ptype == VRX_CYLINDER ? 2 : 1));
+    const size_t ni = index->count()
+        * (ptype == VRX_TRIANGLE ? 3 : (ptype == VRX_CYLINDER || ptype == VRX_CONE ? 2 : 1));
     uint32_t mx = 0;
     for (size_t i = 0; i < ni; i++)
       mx = ix[i] > mx ? ix[i] : mx;
@@ -489,7 +503,7 @@ void Geometry::commit()
     }
   }
   if (radius) {
-    const uint64_t need = ptype == VRX_SPHERE ? nv : n;
+    const uint64_t need = ptype == VRX_SPHERE || ptype == VRX_CONE ? nv : n;
     if (radius->count() < need) {
       report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
           "radius array has %llu entries, %llu needed", (unsigned long long)radius->count(),

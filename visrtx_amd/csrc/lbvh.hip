@@ -88,6 +88,7 @@ size_t leaf_record_bytes(int type)
   case VRX_SPHERE:
     return 16;
   case VRX_CYLINDER:
+  case VRX_CONE:
     return 32;
   default:
     return 0;
@@ -121,6 +122,15 @@ __device__ __forceinline__ void prim_box(const vrx_prims &p, uint32_t i, v3 &lo,
     uint32_t b = p.d_index ? p.d_index[2 * (size_t)i + 1] : 2 * i + 1;
     v3 p0 = ld3(p.d_position + 3 * (size_t)a), p1 = ld3(p.d_position + 3 * (size_t)b);
     float r = fabsf(p.d_radius ? p.d_radius[i] : p.global_radius);
+    v3 mn = vmin(p0, p1), mx = vmax(p0, p1);
+    lo = mk(mn.x - r, mn.y - r, mn.z - r);
+    hi = mk(mx.x + r, mx.y + r, mx.z + r);
+  } else if constexpr (T == VRX_CONE) {
+    // Cone.cpp:98-105: the end points' box grown by the larger end radius
+    uint32_t a = p.d_index ? p.d_index[2 * (size_t)i] : 2 * i;
+    uint32_t b = p.d_index ? p.d_index[2 * (size_t)i + 1] : 2 * i + 1;
+    v3 p0 = ld3(p.d_position + 3 * (size_t)a), p1 = ld3(p.d_position + 3 * (size_t)b);
+    float r = fmaxf(p.d_radius[a], p.d_radius[b]);
     v3 mn = vmin(p0, p1), mx = vmax(p0, p1);
     lo = mk(mn.x - r, mn.y - r, mn.z - r);
     hi = mk(mx.x + r, mx.y + r, mx.z + r);
@@ -565,6 +575,13 @@ __device__ __forceinline__ void write_leaf(
     float4 *L = (float4 *)leaves + 2 * (size_t)k;
     L[0] = make_float4(p0.x, p0.y, p0.z, r);
     L[1] = make_float4(p1.x, p1.y, p1.z, 0.0f);
+  } else if constexpr (T == VRX_CONE) {
+    uint32_t a = p.d_index ? p.d_index[2 * (size_t)orig] : 2 * orig;
+    uint32_t b = p.d_index ? p.d_index[2 * (size_t)orig + 1] : 2 * orig + 1;
+    v3 p0 = ld3(p.d_position + 3 * (size_t)a), p1 = ld3(p.d_position + 3 * (size_t)b);
+    float4 *L = (float4 *)leaves + 2 * (size_t)k;
+    L[0] = make_float4(p0.x, p0.y, p0.z, p.d_radius[a]);
+    L[1] = make_float4(p1.x, p1.y, p1.z, p.d_radius[b]);
   } else if constexpr (T == VRX_TRIANGLE) {
     uint32_t a = p.d_index ? p.d_index[3 * (size_t)orig] : 3 * orig;
     uint32_t b = p.d_index ? p.d_index[3 * (size_t)orig + 1] : 3 * orig + 1;
@@ -593,6 +610,13 @@ __device__ __forceinline__ void leaf_box(
     v3 mn = vmin(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z)), mx = vmax(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
     lo = mk(mn.x - a.w, mn.y - a.w, mn.z - a.w);
     hi = mk(mx.x + a.w, mx.y + a.w, mx.z + a.w);
+  } else if constexpr (T == VRX_CONE) {
+    const float4 *L = (const float4 *)leaves + 2 * (size_t)k;
+    float4 a = L[0], b = L[1];
+    v3 mn = vmin(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z)), mx = vmax(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
+    float r = fmaxf(a.w, b.w);
+    lo = mk(mn.x - r, mn.y - r, mn.z - r);
+    hi = mk(mx.x + r, mx.y + r, mx.z + r);
   } else if constexpr (T == VRX_TRIANGLE) {
     const float4 *L = (const float4 *)leaves + 3 * (size_t)k;
     float4 a = L[0], b = L[1], c = L[2];
@@ -616,7 +640,8 @@ __global__ __launch_bounds__(256) void k_leaves(
   // gather the record the Morton pass wrote for primitive `orig`: one
   // contiguous 16/32/48-byte read instead of separate index, vertex and
   // radius gathers
-  constexpr int nq = T == VRX_SPHERE ? 1 : (T == VRX_CYLINDER ? 2 : (T == VRX_TRIANGLE ? 3 : 0));
+  constexpr int nq = T == VRX_SPHERE ? 1
+      : (T == VRX_CYLINDER || T == VRX_CONE) ? 2 : (T == VRX_TRIANGLE ? 3 : 0);
   const float4 *src = (const float4 *)rec + (size_t)nq * orig;
   float4 *dst = (float4 *)leaves + (size_t)nq * k;
 #pragma unroll
@@ -873,6 +898,9 @@ int bvh_build_impl(const vrx_prims *p, const vrx_bvh *out, void *d_ws, size_t ws
     return build_typed<VRX_CYLINDER>(*p, *out, ws, codes_out, cbox_out, s);
   case VRX_BOX:
     return build_typed<VRX_BOX>(*p, *out, ws, codes_out, cbox_out, s);
+  case VRX_CONE:
+    VRX_ARG(p->d_radius, "cones need per-vertex radii (d_radius)");
+    return build_typed<VRX_CONE>(*p, *out, ws, codes_out, cbox_out, s);
   default:
     set_error("unknown primitive type %d", p->type);
     return VRX_ERR_ARG;
@@ -897,7 +925,7 @@ extern "C" {
 int vrx_bvh_query_sizes(int32_t type, uint32_t n, vrx_bvh_sizes *out)
 {
   VRX_ARG(out, "out");
-  VRX_ARG(type >= 0 && type <= VRX_BOX, "type");
+  VRX_ARG(type >= 0 && type <= VRX_CONE, "type");
   bvh_sizes(type, n, out);
   return VRX_OK;
 }

@@ -279,6 +279,56 @@ __device__ __forceinline__ void load_blas(
   }
 }
 
+// Capped cone between p0 (radius ra) and p1 (radius rb), as the reference's
+// custom intersector tests it (Intersectors_ptx.cu:192-246): the cap disc
+// at the end the ray origin lies beyond (its plane side by the projection
+// onto the axis), then the lateral surface -- the published closed form for
+// a ray against a capped cone (I. Quilez), with the conic's quadratic
+// k2 t^2 + 2 k1 t + k0 = 0 taken at its smaller root.  Up to two candidate
+// hits (ts/ns: t and unnormalised normal; bit q of the result marks q).
+__device__ __forceinline__ int isect_cone(
+    v3 ro, v3 rd, v3 p0, v3 p1, float ra, float rb, float ts[2], v3 ns[2])
+{
+  int valid = 0;
+  const v3 ba = sub(p1, p0), oa = sub(ro, p0), ob = sub(ro, p1);
+  const float m0 = dot(ba, ba), m1 = dot(oa, ba), m2 = dot(ob, ba), m3 = dot(rd, ba);
+  const float rinv = 1.0f / sqrtf(m0);
+  if (m1 < 0.0f) {
+    const v3 q = sub(fmul(m3, oa), fmul(m1, rd));
+    if (dot(q, q) < (ra * ra) * (m3 * m3)) {
+      ts[0] = -m1 / m3;
+      ns[0] = fmul(-rinv, ba);
+      valid |= 1;
+    }
+  } else if (m2 > 0.0f) {
+    const v3 q = sub(fmul(m3, ob), fmul(m2, rd));
+    if (dot(q, q) < (rb * rb) * (m3 * m3)) {
+      ts[0] = -m2 / m3;
+      ns[0] = fmul(rinv, ba);
+      valid |= 1;
+    }
+  }
+  const float m4 = dot(rd, oa), m5 = dot(oa, oa);
+  const float rr = ra - rb;
+  const float hy = m0 + rr * rr;
+  const float m00 = m0 * m0;
+  const float k2 = m00 - (m3 * m3) * hy;
+  const float k1 = (m00 * m4 - (m1 * m3) * hy) + (m0 * ra) * (rr * m3);
+  const float k0 = (m00 * m5 - (m1 * m1) * hy) + (m0 * ra) * ((rr * m1) * 2.0f - m0 * ra);
+  const float h = k1 * k1 - k2 * k0;
+  if (h >= 0.0f) {
+    const float t = (-k1 - sqrtf(h)) / k2;
+    const float y = m1 + t * m3;
+    if (y > 0.0f && y < m0) {
+      const v3 pt = add(oa, fmul(t, rd));
+      ts[1] = t;
+      ns[1] = sub(fmul(m0, add(fmul(m0, pt), fmul(rr * ra, ba))), fmul(hy * y, ba));
+      valid |= 2;
+    }
+  }
+  return valid;
+}
+
 template <int PT>
 __device__ __forceinline__ int blas_type(const BlasCtx &b)
 {
@@ -351,6 +401,33 @@ __device__ __forceinline__ bool test_leaf(
         b.caps, ts, ns);
 #pragma unroll
     for (int q = 0; q < 3; q++) {
+      if (!(vm & (1 << q)))
+        continue;
+      float t = ts[q];
+      if (t >= tmin && t <= tmax) {
+        if (ANY)
+          return true;
+        uint32_t prim = gld(b.ids, k);
+        if (take(h, t, tmax, b.entry, prim)) {
+          h.found = true;
+          h.t = t;
+          h.entry = b.entry;
+          h.prim = prim;
+          h.a0 = ns[q].x;
+          h.a1 = ns[q].y;
+          h.a2 = ns[q].z;
+          tmax = t;
+        }
+      }
+    }
+  } else if (type == VRX_CONE) {
+    const float4 *L = b.leaves + 2 * (size_t)k;
+    float4 P0 = gld(L, 0), P1 = gld(L, 1);
+    float ts[2];
+    v3 ns[2];
+    int vm = isect_cone(o, d, mk(P0.x, P0.y, P0.z), mk(P1.x, P1.y, P1.z), P0.w, P1.w, ts, ns);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
       if (!(vm & (1 << q)))
         continue;
       float t = ts[q];
