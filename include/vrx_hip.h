@@ -50,7 +50,8 @@ enum vrx_prim_type {
   VRX_SPHERE = 1,
   VRX_CYLINDER = 2,
   VRX_BOX = 3, /* plain AABB input (used for the instance level) */
-  VRX_CONE = 4 /* capped cones between vertex pairs, per-vertex radii */
+  VRX_CONE = 4, /* capped cones between vertex pairs, per-vertex radii */
+  VRX_CURVE = 5 /* round linear curve segments: vertex index[i] to index[i]+1 */
 };
 
 enum vrx_memcpy_kind { VRX_H2D = 0, VRX_D2H = 1, VRX_D2D = 2 };
@@ -89,15 +90,17 @@ typedef struct vrx_prims {
   int32_t type;           /* vrx_prim_type */
   uint32_t n;             /* number of primitives */
   const float *d_position;/* float3 vertices / centers (VRX_BOX: float[6] boxes) */
-  const uint32_t *d_index;/* uint3 / uint2 / uint primitive.index or NULL */
-  const float *d_radius;  /* sphere, cone: per-vertex radius; cylinder: per-primitive; or NULL */
+  const uint32_t *d_index;/* uint3 / uint2 / uint primitive.index or NULL (curve: required,
+                             one first-vertex index per segment) */
+  const float *d_radius;  /* sphere, cone, curve: per-vertex radius; cylinder: per-primitive;
+                             or NULL */
   float global_radius;
 } vrx_prims;
 
 /* Output buffers of one BVH.  Sizes (bytes) from vrx_bvh_sizes. */
 typedef struct vrx_bvh {
   void *d_nodes;          /* (n-1) * 64 B: both child boxes + children + heights */
-  void *d_leaves;         /* n * leaf record (48/16/32/0/32 B for tri/sphere/cyl/box/cone) */
+  void *d_leaves;         /* n * leaf record (48/16/32/0/32/32 B: tri/sphere/cyl/box/cone/curve) */
   uint32_t *d_leaf_ids;   /* n: sorted leaf -> original primitive index */
   void *d_header;         /* 64 B: root box (6 f32), height, root, n, type */
   void *d_wide;           /* optional (n-1) * 128 B 4-wide nodes (vrx_bvh_widen), or NULL */

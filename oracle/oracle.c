@@ -100,6 +100,13 @@ static void prim_box(int type, uint32_t i, const float *pos, const uint32_t *ind
     v3 mn = vmin(p0, p1), mx = vmax(p0, p1);
     *lo = mk(mn.x - r, mn.y - r, mn.z - r);
     *hi = mk(mx.x + r, mx.y + r, mx.z + r);
+  } else if (type == ORC_CURVE) {
+    /* Curve.cpp:83 (round linear): a sphere swept from (p0, r0) to (p1, r1) */
+    uint32_t a = index[i];
+    v3 p0 = ld3(pos + 3 * (size_t)a), p1 = ld3(pos + 3 * (size_t)a + 3);
+    float r0 = radius ? radius[a] : global_radius, r1 = radius ? radius[a + 1] : global_radius;
+    *lo = vmin(mk(p0.x - r0, p0.y - r0, p0.z - r0), mk(p1.x - r1, p1.y - r1, p1.z - r1));
+    *hi = vmax(mk(p0.x + r0, p0.y + r0, p0.z + r0), mk(p1.x + r1, p1.y + r1, p1.z + r1));
   } else {
     uint32_t a = index ? index[3 * (size_t)i] : 3 * i;
     uint32_t b = index ? index[3 * (size_t)i + 1] : 3 * i + 1;
@@ -503,6 +510,64 @@ int orc_intersect_cone(const float o[3], const float d[3], const float p0[3], co
   return r;
 }
 
+/* Round linear curve segment (Curve.cpp:83): the convex hull of the spheres
+ * (pa, ra) and (pb, rb).  The ray enters it on the tangent cone (rounded-cone
+ * closed form, I. Quilez) when the entry lies between the tangent circles,
+ * else on the nearer end sphere; one sphere inside the other leaves that
+ * sphere.  Same operation order as the device. */
+static inline int isect_round_cone(v3 ro, v3 rd, v3 pa, v3 pb, float ra, float rb, float *t, v3 *n)
+{
+  const v3 ba = sub(pb, pa), oa = sub(ro, pa), ob = sub(ro, pb);
+  const float rr = ra - rb;
+  const float m0 = dot(ba, ba), m1 = dot(ba, oa), m2 = dot(ba, rd), m3 = dot(rd, oa);
+  const float m5 = dot(oa, oa), m6 = dot(ob, rd), m7 = dot(ob, ob);
+  const float d2 = m0 - rr * rr;
+  if (d2 > 0.0f) {
+    const float k2 = d2 - m2 * m2;
+    const float k1 = (d2 * m3 - m1 * m2) + (m2 * rr) * ra;
+    const float k0 = ((d2 * m5 - m1 * m1) + ((m1 * rr) * ra) * 2.0f) - (m0 * ra) * ra;
+    const float h = k1 * k1 - k0 * k2;
+    if (h < 0.0f)
+      return 0;
+    const float tc = (-sqrtf(h) - k1) / k2;
+    const float y = (m1 - ra * rr) + tc * m2;
+    if (y > 0.0f && y < d2) {
+      *t = tc;
+      *n = sub(fmul(d2, add(oa, fmul(tc, rd))), fmul(y, ba));
+      return 1;
+    }
+  }
+  const float h1 = (m3 * m3 - m5) + ra * ra;
+  const float h2 = (m6 * m6 - m7) + rb * rb;
+  const int use1 = d2 > 0.0f || ra >= rb, use2 = d2 > 0.0f || rb > ra;
+  int hit = 0;
+  if (use1 && h1 > 0.0f) {
+    *t = -m3 - sqrtf(h1);
+    *n = add(oa, fmul(*t, rd));
+    hit = 1;
+  }
+  if (use2 && h2 > 0.0f) {
+    const float t2 = -m6 - sqrtf(h2);
+    if (!hit || t2 < *t) {
+      *t = t2;
+      *n = add(ob, fmul(t2, rd));
+      hit = 1;
+    }
+  }
+  return hit;
+}
+
+int orc_intersect_round_cone(const float o[3], const float d[3], const float pa[3],
+    const float pb[3], float ra, float rb, float *t, float n[3])
+{
+  v3 nn;
+  int r = isect_round_cone(ld3(o), ld3(d), ld3(pa), ld3(pb), ra, rb, t, &nn);
+  if (r) {
+    n[0] = nn.x; n[1] = nn.y; n[2] = nn.z;
+  }
+  return r;
+}
+
 int orc_intersect_cylinder(const float o[3], const float d[3], const float p0[3],
     const float p1[3], float radius, int caps, float *ts, float *ns, int *valid)
 {
@@ -775,6 +840,23 @@ static inline int test_prim(tctx *c, uint32_t p, float tmin, float *tmax, best_t
             *tmax = t;
           }
         }
+      }
+    }
+  } else if (G->type == ORC_CURVE) {
+    uint32_t a = G->index[p];
+    float r0 = G->radius ? G->radius[a] : G->global_radius;
+    float r1 = G->radius ? G->radius[a + 1] : G->global_radius;
+    float t;
+    v3 nn;
+    if (isect_round_cone(c->o, c->d, ld3(G->position + 3 * (size_t)a),
+            ld3(G->position + 3 * (size_t)a + 3), r0, r1, &t, &nn)
+        && t >= tmin && t <= *tmax) {
+      if (c->any)
+        return 1;
+      if (better(best, t, key)) {
+        best->found = 1; best->t = t; best->key = key; best->entry = c->e; best->prim = p;
+        best->a0 = nn.x; best->a1 = nn.y; best->a2 = nn.z;
+        *tmax = t;
       }
     }
   } else if (G->type == ORC_CONE) {

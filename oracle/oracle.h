@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-enum { ORC_TRIANGLE = 0, ORC_SPHERE = 1, ORC_CYLINDER = 2, ORC_CONE = 4 };
+enum { ORC_TRIANGLE = 0, ORC_SPHERE = 1, ORC_CYLINDER = 2, ORC_CONE = 4, ORC_CURVE = 5 };
 
 /* ---- PCG32 (O'Neill 2014) ------------------------------------------------ */
 typedef struct { uint64_t state, inc; } orc_pcg;
@@ -88,7 +88,8 @@ typedef struct {
   int32_t type;
   uint32_t num_prims;
   const float *position;        /* vertex.position float3 */
-  const uint32_t *index;        /* primitive.index (uint3 / uint2 / uint) or NULL */
+  const uint32_t *index;        /* primitive.index (uint3 / uint2 / uint) or NULL; curves:
+                                   first vertex of each segment (required) */
   const float *radius;          /* sphere, cone vertex.radius / cylinder primitive.radius or NULL */
   float global_radius;
   int32_t caps;
@@ -198,6 +199,8 @@ int orc_intersect_cylinder(const float o[3], const float d[3], const float p0[3]
     const float p1[3], float radius, int caps, float *ts /*3*/, float *ns /*9*/, int *valid /*3*/);
 int orc_intersect_cone(const float o[3], const float d[3], const float p0[3], const float p1[3],
     float ra, float rb, float *ts /*2*/, float *ns /*6*/, int *valid /*2*/);
+int orc_intersect_round_cone(const float o[3], const float d[3], const float pa[3],
+    const float pb[3], float ra, float rb, float *t, float n[3]);
 
 void orc_camera_ray(const orc_camera *c, uint32_t w, uint32_t h, float px, float py,
     float org[3], float dir[3]);

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-ORC_TRIANGLE, ORC_SPHERE, ORC_CYLINDER, ORC_CONE = 0, 1, 2, 4
+ORC_TRIANGLE, ORC_SPHERE, ORC_CYLINDER, ORC_CONE, ORC_CURVE = 0, 1, 2, 4, 5
 ORC_RAYCAST, ORC_AO = 0, 1
 ORC_COLOR_F32, ORC_COLOR_SRGB8, ORC_COLOR_UNORM8 = 0, 1, 2
 LEAF = 0x80000000
@@ -106,6 +106,7 @@ def lib() -> C.CDLL:
         L.orc_linear_to_srgb.argtypes = [vp, vp]
         L.orc_intersect_triangle.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_intersect_cone.argtypes = [vp, vp, vp, vp, C.c_float, C.c_float, vp, vp, vp]
+        L.orc_intersect_round_cone.argtypes = [vp, vp, vp, vp, C.c_float, C.c_float, vp, vp]
         L.orc_intersect_triangle_f64.argtypes = [vp, vp, vp, vp, vp]
         L.orc_sample_field.argtypes = [vp, vp]
         L.orc_classify.argtypes = [vp, C.c_double, vp]
@@ -152,7 +153,7 @@ def bounds(type_: int, pos, index=None, radius=None, global_radius=0.0):
 def _num_prims(type_, pos, index):
     if type_ == ORC_TRIANGLE:
         return len(index) if index is not None else len(pos) // 3
-    if type_ == ORC_SPHERE:
+    if type_ in (ORC_SPHERE, ORC_CURVE):
         return len(index) if index is not None else len(pos)
     return len(index) if index is not None else len(pos) // 2
 
@@ -202,7 +203,7 @@ def expand_bits10(v: int) -> int:
 # scenes
 # ---------------------------------------------------------------------------
 _TYPES = {"triangle": ORC_TRIANGLE, "sphere": ORC_SPHERE, "cylinder": ORC_CYLINDER,
-          "cone": ORC_CONE}
+          "cone": ORC_CONE, "curve": ORC_CURVE}
 IDENTITY12 = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], np.float32)
 
 
@@ -233,14 +234,17 @@ class OracleScene:
             pos = np.ascontiguousarray(g.position, np.float32)
             self._keep.append(pos)
             G.position = pos.ctypes.data
-            for name, arr, dt in [("index", g.index, np.uint32), ("radius", g.radius, np.float32),
+            index = g.index
+            if g.type == "curve" and index is None:   # Curve.cpp:107-118
+                index = np.arange(len(pos) // 2, dtype=np.uint32)
+            for name, arr, dt in [("index", index, np.uint32), ("radius", g.radius, np.float32),
                                   ("primitive_id", g.primitive_id, np.uint32),
                                   ("vertex_normal", g.vertex_normal, np.float32)]:
                 if arr is not None:
                     a = np.ascontiguousarray(arr, dt)
                     self._keep.append(a)
                     setattr(G, name, a.ctypes.data)
-            default_r = 0.01 if g.type == "sphere" else 1.0
+            default_r = 0.01 if g.type == "sphere" else 1.0   # Curve.cpp:49 too
             G.global_radius = g.global_radius if g.global_radius is not None else default_r
             G.caps = 1 if g.caps else 0
             G.cull_backfaces = 1 if g.cull else 0

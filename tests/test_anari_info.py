@@ -62,7 +62,7 @@ def subtypes(L, typ):
 
 
 def test_subtypes(lib):
-    assert set(subtypes(lib, A.GEOMETRY)) >= {"triangle", "quad", "sphere", "cylinder", "cone"}
+    assert set(subtypes(lib, A.GEOMETRY)) >= {"triangle", "quad", "sphere", "cylinder", "cone", "curve"}
     assert set(subtypes(lib, A.RENDERER)) >= {"raycast", "ao", "default"}
 
 
@@ -110,5 +110,5 @@ def test_defaults_and_clamps(lib):
     # wrong parameter type or unknown names answer null
     assert not lib.anariGetParameterInfo(None, A.GEOMETRY, b"sphere", b"radius", A.INT32,
                                          b"default", A.INT32)
-    assert not lib.anariGetObjectInfo(None, A.GEOMETRY, b"curve", b"parameter",
+    assert not lib.anariGetObjectInfo(None, A.GEOMETRY, b"isosurface", b"parameter",
                                       ANARI_PARAMETER_LIST)

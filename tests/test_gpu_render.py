@@ -317,3 +317,76 @@ def test_cones_with_other_types_and_ao(require_gpu):
     g = render_gpu(sc, cam, 160, 120, kind="ao", **rp)
     o = orc.OracleScene(sc_o).render(cam, 160, 120, kind="ao", **rp)
     compare(g, o)
+
+
+def _curve_scene(indexed):
+    """400 random polylines of 8 segments (round linear curves, Curve.cpp)
+    with per-vertex radii; or, as a soup, segments 0 .. nv/2 - 1."""
+    rng = np.random.default_rng(9)
+    lines, per = 400, 9
+    start = rng.random((lines, 1, 3), dtype=np.float32)
+    steps = rng.normal(0, 0.02, (lines, per, 3)).astype(np.float32)
+    pos = (start + np.cumsum(steps, 1)).reshape(-1, 3)
+    rad = rng.uniform(0.001, 0.008, len(pos)).astype(np.float32)
+    idx = None
+    if indexed:   # each polyline's 8 segments start at its first 8 vertices
+        idx = (np.arange(lines)[:, None] * per + np.arange(per - 1)[None]).reshape(-1).astype(np.uint32)
+    g = S.GeomDesc("curve", pos, idx, radius=rad, surface_id=4)
+    sc = S.SceneDesc(geoms=[g], world_geoms=[0], world_id=6)
+    cam = S.CameraDesc((0.5, 0.5, -1.0), (0, 0, 1), (0, 1, 0), math.radians(50), 4 / 3)
+    return sc, cam
+
+
+def _segment_t(g, s, org, d):
+    a = int(g.index[s]) if g.index is not None else s
+    t = np.zeros(1, np.float32)
+    n = np.zeros(3, np.float32)
+    keep = [np.ascontiguousarray(v, np.float32) for v in (org, d, g.position[a], g.position[a + 1])]
+    hit = orc.lib().orc_intersect_round_cone(*[k.ctypes.data for k in keep], float(g.radius[a]),
+                                             float(g.radius[a + 1]), t.ctypes.data, n.ctypes.data)
+    return float(t[0]) if hit else None
+
+
+@pytest.mark.parametrize("indexed", [False, True])
+def test_curves_raycast(require_gpu, indexed):
+    """Round linear curves vs the oracle: depth/colour within 1e-4, ids
+    bit-exact except where two adjacent segments meet the ray within 4 ulps
+    of each other (the shared joint sphere seen through two formulas); there
+    either id is a correct closest hit."""
+    sc, cam = _curve_scene(indexed)
+    g = render_gpu(sc, cam, 320, 240)
+    o = orc.OracleScene(sc).render(cam, 320, 240)
+    diff = np.argwhere(g["primitiveId"] != o["primitiveId"])
+    geom = sc.geoms[0]
+    for (y, x) in diff:
+        pg, po = int(g["primitiveId"][y, x]), int(o["primitiveId"][y, x])
+        org, d = orc.camera_ray(cam, 320, 240, float(x), float(y))
+        tg, to = _segment_t(geom, pg, org, d), _segment_t(geom, po, org, d)
+        assert abs(pg - po) == 1 and tg is not None and to is not None, (x, y, pg, po)
+        assert abs(tg - to) <= 4 * np.spacing(np.float32(to)), (x, y, tg, to)
+        g["primitiveId"][y, x] = po
+    assert len(diff) <= 4
+    compare(g, o)
+    assert (g["depth"] < 1e29).mean() > 0.02
+
+
+def test_curve_index_out_of_range_is_rejected(require_gpu):
+    dev = A.Device()
+    s = S.AnariScene(dev)
+    g = s.keep(dev.new("Geometry", "curve"))
+    dev.set(g, "vertex.position", s.keep(dev.array(np.zeros((4, 3), np.float32), A.FLOAT32_VEC3)),
+            A.ARRAY1D)
+    dev.set(g, "primitive.index", s.keep(dev.array(np.array([0, 3], np.uint32), A.UINT32)),
+            A.ARRAY1D)
+    dev.commit(g)
+    sf = s.keep(dev.new("Surface"))
+    dev.set(sf, "geometry", g, A.GEOMETRY)
+    dev.commit(sf)
+    w = s.keep(dev.new("World"))
+    dev.set(w, "surface", s.keep(dev.array([sf], A.SURFACE)), A.ARRAY1D)
+    dev.commit(w)
+    dev.get_property(w, "bounds", A.FLOAT32_BOX3)   # flushes the deferred commits
+    errs = " ".join(str(e) for e in dev.errors())
+    assert "needs vertex 4" in errs, errs
+    s.release()
+    dev.close()

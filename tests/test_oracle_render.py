@@ -401,3 +401,78 @@ def test_cone_bounds_use_the_larger_end_radius():
     pos = np.array([[0, 0, 0], [1, 2, 3]], np.float32)
     box = orc.bounds(orc.ORC_CONE, pos, radius=np.array([0.5, 0.25], np.float32))
     np.testing.assert_allclose(box.reshape(-1), [-0.5, -0.5, -0.5, 1.5, 2.5, 3.5])
+
+
+def _round_cone(o, d, pa, pb, ra, rb):
+    t = np.zeros(1, np.float32)
+    n = np.zeros(3, np.float32)
+    keep = [np.asarray(v, np.float32) for v in (o, d, pa, pb)]
+    r = orc.lib().orc_intersect_round_cone(*[k.ctypes.data for k in keep], ra, rb,
+                                           t.ctypes.data, n.ctypes.data)
+    return r, float(t[0]), n
+
+
+def _swept_sphere_entry(o, d, pa, pb, ra, rb, steps=20001):
+    """FP64 brute force: the first entry of the ray into any sphere of the
+    sweep c(s) = pa + s (pb - pa), r(s) = ra + s (rb - ra), s in [0, 1]."""
+    o, d, pa, pb = (np.asarray(v, np.float64) for v in (o, d, pa, pb))
+    s = np.linspace(0.0, 1.0, steps)
+    c = pa[None] + s[:, None] * (pb - pa)[None]
+    r = ra + s * (rb - ra)
+    oc = o[None] - c
+    b = oc @ d
+    h = b * b - (np.sum(oc * oc, 1) - r * r)
+    ok = h >= 0
+    if not ok.any():
+        return None
+    return float(np.min(-b[ok] - np.sqrt(h[ok])))
+
+
+def test_round_linear_curve_against_a_swept_sphere():
+    """The closed form (tangent cone between the end spheres, else the nearer
+    end sphere) against an FP64 brute force over 20,001 spheres of the sweep,
+    on random rays and segments; hit/miss agree away from grazing rays."""
+    rng = np.random.default_rng(12)
+    checked = disagree = 0
+    for _ in range(300):
+        pa = rng.uniform(-0.5, 0.5, 3)
+        pb = pa + rng.normal(0, 0.6, 3)
+        ra, rb = rng.uniform(0.05, 0.4, 2)
+        o = rng.uniform(-3, 3, 3)
+        tgt = (pa + pb) / 2 + rng.normal(0, 0.4, 3)
+        d = tgt - o
+        d /= np.linalg.norm(d)
+        hit, t, n = _round_cone(o, d, pa, pb, ra, rb)
+        tb = _swept_sphere_entry(o, d, pa, pb, ra, rb)
+        if (tb is None) != (not hit):
+            disagree += 1           # only a ray grazing the surface may differ
+            continue
+        if tb is None:
+            continue
+        assert t == pytest.approx(tb, rel=2e-4, abs=2e-4)
+        checked += 1
+    assert checked > 80 and disagree <= 2
+
+
+def test_round_linear_curve_special_cases():
+    # equal radii: a capsule; side hit at distance r from the axis
+    hit, t, n = _round_cone((0, 0, -5), (0, 0, 1), (-1, 0, 0), (1, 0, 0), 0.5, 0.5)
+    assert hit and t == pytest.approx(4.5, rel=1e-6)
+    np.testing.assert_allclose(n / np.linalg.norm(n), [0, 0, -1], atol=1e-6)
+    # down the axis: the end sphere of p0 (rounded end, not a flat cap)
+    hit, t, n = _round_cone((-3, 0, 0), (1, 0, 0), (-1, 0, 0), (1, 0, 0), 0.5, 0.25)
+    assert hit and t == pytest.approx(1.5, rel=1e-6)
+    # the small sphere inside the large one: the large sphere alone
+    hit, t, n = _round_cone((0, 0, -5), (0, 0, 1), (0, 0, 0), (0.1, 0, 0), 1.0, 0.2)
+    assert hit and t == pytest.approx(4.0, rel=1e-6)
+    # a miss
+    hit, t, n = _round_cone((0, 3, -5), (0, 0, 1), (-1, 0, 0), (1, 0, 0), 0.5, 0.5)
+    assert not hit
+
+
+def test_curve_bounds_cover_both_end_spheres():
+    pos = np.array([[0, 0, 0], [1, 2, 3], [4, 4, 4]], np.float32)
+    box = orc.bounds(orc.ORC_CURVE, pos, index=np.array([0, 1], np.uint32),
+                     radius=np.array([0.5, 0.25, 1.0], np.float32))
+    np.testing.assert_allclose(box, [[-0.5, -0.5, -0.5, 1.25, 2.25, 3.25],
+                                     [0.75, 1.75, 2.75, 5.0, 5.0, 5.0]])

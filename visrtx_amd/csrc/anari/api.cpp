@@ -34,7 +34,7 @@ inline Object *O(ANARIObject o) { return (Object *)o; }
     return ret;
 
 const char *g_deviceSubtypes[] = {"default", "visrtx_amd", nullptr};
-const char *g_geomSubtypes[] = {"triangle", "quad", "sphere", "cylinder", "cone", nullptr};
+const char *g_geomSubtypes[] = {"triangle", "quad", "sphere", "cylinder", "cone", "curve", nullptr};
 const char *g_rendererSubtypes[] = {"default", "raycast", "ao", nullptr};
 const char *g_cameraSubtypes[] = {"perspective", nullptr};
 const char *g_materialSubtypes[] = {"matte", nullptr};

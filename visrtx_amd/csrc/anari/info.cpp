@@ -83,6 +83,13 @@ std::vector<ObjDef> &tables()
               {"primitive.index", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32_VEC2 end point pairs"},
               {"caps", ANARI_STRING, false, "none", nullptr, nullptr, "read, as the reference does; end discs are always tested"},
               {"primitive.id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32 primitiveId values"}}},
+      {ANARI_GEOMETRY, "curve", "round linear curve segments (Curve.cpp)",
+          {{"vertex.position", ANARI_ARRAY1D, true, nullptr, nullptr, nullptr, "FLOAT32_VEC3 control points"},
+              {"vertex.radius", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "FLOAT32 per-vertex radii"},
+              {"radius", ANARI_FLOAT32, false, &kOne, nullptr, nullptr, "radius when no vertex.radius"},
+              {"primitive.index", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr,
+                  "UINT32 first vertex of each segment"},
+              {"primitive.id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32 primitiveId values"}}},
       {ANARI_GEOMETRY, "quad", "quads, each traced as two triangles (Quad.cpp)",
           {{"vertex.position", ANARI_ARRAY1D, true, nullptr, nullptr, nullptr, "FLOAT32_VEC3 vertices"},
               {"primitive.index", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "UINT32_VEC4 quads"},

@@ -296,7 +296,7 @@ const void *Array::device()
 }
 
 // ---------------------------------------------------------------------------
-// Geometry (scene/surface/geometry/{Geometry,Triangle,Quad,Sphere,Cylinder,Cone})
+// Geometry (scene/surface/geometry/{Geometry,Triangle,Quad,Sphere,Cylinder,Cone,Curve})
 // ---------------------------------------------------------------------------
 Geometry::Geometry(Device *d, const std::string &sub) : Object(d, ANARI_GEOMETRY, sub)
 {
@@ -311,36 +311,38 @@ Geometry::Geometry(Device *d, const std::string &sub) : Object(d, ANARI_GEOMETRY
     ptype = VRX_CYLINDER;
   else if (sub == "cone")
     ptype = VRX_CONE;
-  if (ptype == VRX_CYLINDER)
-    globalRadius = 1.0f; // Cylinder.cpp:49
+  else if (sub == "curve")
+    ptype = VRX_CURVE;
+  if (ptype == VRX_CYLINDER || ptype == VRX_CURVE)
+    globalRadius = 1.0f; // Cylinder.cpp:49, Curve.cpp:49
 }
 
 Geometry::~Geometry()
 {
   freeBVH();
-  vrx_free(dQuadTri);
+  vrx_free(dGenIdx);
 }
 
 const uint32_t *Geometry::deviceIndex()
 {
-  if (!quad)
+  if (!useGenIdx)
     return index ? (const uint32_t *)index->device() : nullptr;
-  if (quadTriDirty) {
-    const size_t bytes = quadTri.size() * sizeof(uint32_t);
-    if (bytes > capQuadTri) {
-      vrx_free(dQuadTri);
-      dQuadTri = nullptr;
-      capQuadTri = 0;
-      if (!dev->check(vrx_malloc(&dQuadTri, bytes), this, "quad index allocation"))
+  if (genIdxDirty) {
+    const size_t bytes = genIdx.size() * sizeof(uint32_t);
+    if (bytes > capGenIdx) {
+      vrx_free(dGenIdx);
+      dGenIdx = nullptr;
+      capGenIdx = 0;
+      if (!dev->check(vrx_malloc(&dGenIdx, bytes), this, "quad index allocation"))
         return nullptr;
-      capQuadTri = bytes;
+      capGenIdx = bytes;
     }
-    if (bytes && !dev->check(vrx_memcpy_sync(dQuadTri, quadTri.data(), bytes, VRX_H2D), this,
+    if (bytes && !dev->check(vrx_memcpy_sync(dGenIdx, genIdx.data(), bytes, VRX_H2D), this,
                      "quad index upload"))
       return nullptr;
-    quadTriDirty = false;
+    genIdxDirty = false;
   }
-  return (const uint32_t *)dQuadTri;
+  return (const uint32_t *)dGenIdx;
 }
 
 void Geometry::freeBVH()
@@ -374,8 +376,9 @@ void Geometry::commit()
   index = arrayParam(this, "primitive.index");
   primId = arrayParam(this, "primitive.id");
   normal = ptype == VRX_TRIANGLE ? arrayParam(this, "vertex.normal") : nullptr;
-  quadTri.clear();
-  quadTriDirty = true;
+  genIdx.clear();
+  genIdxDirty = true;
+  useGenIdx = quad;
   cull = get<int>("cullBackfaces", ANARI_BOOL, 0) != 0;
   if (ptype == VRX_SPHERE) {
     radius = arrayParam(this, "vertex.radius");
@@ -395,6 +398,10 @@ void Geometry::commit()
           "missing required parameter 'vertex.radius' on cone geometry");
       return;
     }
+  } else if (ptype == VRX_CURVE) {
+    // Curve.cpp:45-50: per-vertex radii, else "radius" (default 1)
+    radius = arrayParam(this, "vertex.radius");
+    globalRadius = get<float>("radius", ANARI_FLOAT32, 1.0f);
   } else {
     radius = nullptr;
   }
@@ -429,7 +436,7 @@ void Geometry::commit()
     }
     const uint32_t *ix = index ? (const uint32_t *)index->host() : nullptr;
     uint32_t mx = 0;
-    quadTri.resize(6 * nq);
+    genIdx.resize(6 * nq);
     for (uint64_t q = 0; q < nq; q++) {
       uint32_t x, y, z, w;
       if (ix) {
@@ -446,7 +453,7 @@ void Geometry::commit()
       mx = std::max(std::max(mx, std::max(x, y)), std::max(z, w));
       const uint32_t t[6] = {x, y, w, z, w, y};
       for (int k = 0; k < 6; k++)
-        quadTri[6 * q + k] = t[k];
+        genIdx[6 * q + k] = t[k];
     }
     if (nq && (uint64_t)mx >= nv) {
       report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
@@ -455,6 +462,33 @@ void Geometry::commit()
       return;
     }
     n = (uint32_t)(2 * nq);
+  } else if (ptype == VRX_CURVE) {
+    // segment i joins vertices index[i] and index[i] + 1 (round linear
+    // curves); without an index, segments start at 0 .. nv/2 - 1
+    // (Curve.cpp:107-118)
+    if (index && index->elem != ANARI_UINT32) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+          "'primitive.index' must be ANARI_UINT32 on curve geometry");
+      return;
+    }
+    const uint64_t ns = index ? index->count() : nv / 2;
+    const uint32_t *ix = index ? (const uint32_t *)index->host() : nullptr;
+    uint32_t mx = 0;
+    for (uint64_t i = 0; ix && i < ns; i++)
+      mx = std::max(mx, ix[i]);
+    if (ns && (uint64_t)(ix ? mx : (uint32_t)(ns - 1)) + 1 >= nv) {
+      report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+          "curve segment starting at vertex %u needs vertex %u but only %llu vertices exist",
+          ix ? mx : (uint32_t)(ns - 1), (ix ? mx : (uint32_t)(ns - 1)) + 1, (unsigned long long)nv);
+      return;
+    }
+    if (!ix) {
+      genIdx.resize(ns);
+      for (uint64_t i = 0; i < ns; i++)
+        genIdx[i] = (uint32_t)i;
+      useGenIdx = true;
+    }
+    n = (uint32_t)ns;
   } else if (ptype == VRX_TRIANGLE) {
     if (index) {
       if (index->elem != ANARI_UINT32_VEC3) {
@@ -487,7 +521,7 @@ void Geometry::commit()
     }
     n = (uint32_t)(index ? index->count() : nv / 2);
   }
-  if (index && !quad) {
+  if (index && !quad && ptype != VRX_CURVE) {
     // every index must address an existing vertex (checked once per commit)
     const uint32_t *ix = (const uint32_t *)index->host();
     const size_t ni = index->count()
@@ -503,7 +537,7 @@ void Geometry::commit()
     }
   }
   if (radius) {
-    const uint64_t need = ptype == VRX_SPHERE || ptype == VRX_CONE ? nv : n;
+    const uint64_t need = ptype == VRX_SPHERE || ptype == VRX_CONE || ptype == VRX_CURVE ? nv : n;
     if (radius->count() < need) {
       report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
           "radius array has %llu entries, %llu needed", (unsigned long long)radius->count(),
@@ -550,7 +584,7 @@ int Geometry::buildBVH(vrx_stream s)
   p.n = n;
   p.d_position = (const float *)pos->device();
   p.d_index = deviceIndex();
-  if ((quad || index) && !p.d_index && n)
+  if ((useGenIdx || index) && !p.d_index && n)
     return -1;
   p.d_radius = radius ? (const float *)radius->device() : nullptr;
   p.global_radius = globalRadius;

@@ -135,11 +135,13 @@ struct Geometry : Object {
   void *dWide{nullptr};
   size_t capWide{0};
   bool wideValid{false};
-  // triangle indices generated from the quads (Quad.cpp:124-145)
-  std::vector<uint32_t> quadTri;
-  void *dQuadTri{nullptr};
-  size_t capQuadTri{0};
-  bool quadTriDirty{true};
+  // generated indices: the triangles of quads (Quad.cpp:124-145), the
+  // segments of a curve soup (Curve.cpp:107-118)
+  bool useGenIdx{false};
+  std::vector<uint32_t> genIdx;
+  void *dGenIdx{nullptr};
+  size_t capGenIdx{0};
+  bool genIdxDirty{true};
   // device triangle/segment/centre indices (the generated ones for quads)
   const uint32_t *deviceIndex();
   int buildBVH(vrx_stream s);

@@ -89,6 +89,7 @@ size_t leaf_record_bytes(int type)
     return 16;
   case VRX_CYLINDER:
   case VRX_CONE:
+  case VRX_CURVE:
     return 32;
   default:
     return 0;
@@ -134,6 +135,15 @@ __device__ __forceinline__ void prim_box(const vrx_prims &p, uint32_t i, v3 &lo,
     v3 mn = vmin(p0, p1), mx = vmax(p0, p1);
     lo = mk(mn.x - r, mn.y - r, mn.z - r);
     hi = mk(mx.x + r, mx.y + r, mx.z + r);
+  } else if constexpr (T == VRX_CURVE) {
+    // a sphere swept from (p0, r0) to (p1, r1): its box is the two end
+    // spheres' box, since p -/+ r is linear along the segment
+    uint32_t a = p.d_index[i];
+    v3 p0 = ld3(p.d_position + 3 * (size_t)a), p1 = ld3(p.d_position + 3 * (size_t)a + 3);
+    float r0 = p.d_radius ? p.d_radius[a] : p.global_radius;
+    float r1 = p.d_radius ? p.d_radius[a + 1] : p.global_radius;
+    lo = vmin(mk(p0.x - r0, p0.y - r0, p0.z - r0), mk(p1.x - r1, p1.y - r1, p1.z - r1));
+    hi = vmax(mk(p0.x + r0, p0.y + r0, p0.z + r0), mk(p1.x + r1, p1.y + r1, p1.z + r1));
   } else if constexpr (T == VRX_TRIANGLE) {
     uint32_t a = p.d_index ? p.d_index[3 * (size_t)i] : 3 * i;
     uint32_t b = p.d_index ? p.d_index[3 * (size_t)i + 1] : 3 * i + 1;
@@ -582,6 +592,12 @@ __device__ __forceinline__ void write_leaf(
     float4 *L = (float4 *)leaves + 2 * (size_t)k;
     L[0] = make_float4(p0.x, p0.y, p0.z, p.d_radius[a]);
     L[1] = make_float4(p1.x, p1.y, p1.z, p.d_radius[b]);
+  } else if constexpr (T == VRX_CURVE) {
+    uint32_t a = p.d_index[orig];
+    v3 p0 = ld3(p.d_position + 3 * (size_t)a), p1 = ld3(p.d_position + 3 * (size_t)a + 3);
+    float4 *L = (float4 *)leaves + 2 * (size_t)k;
+    L[0] = make_float4(p0.x, p0.y, p0.z, p.d_radius ? p.d_radius[a] : p.global_radius);
+    L[1] = make_float4(p1.x, p1.y, p1.z, p.d_radius ? p.d_radius[a + 1] : p.global_radius);
   } else if constexpr (T == VRX_TRIANGLE) {
     uint32_t a = p.d_index ? p.d_index[3 * (size_t)orig] : 3 * orig;
     uint32_t b = p.d_index ? p.d_index[3 * (size_t)orig + 1] : 3 * orig + 1;
@@ -617,6 +633,11 @@ __device__ __forceinline__ void leaf_box(
     float r = fmaxf(a.w, b.w);
     lo = mk(mn.x - r, mn.y - r, mn.z - r);
     hi = mk(mx.x + r, mx.y + r, mx.z + r);
+  } else if constexpr (T == VRX_CURVE) {
+    const float4 *L = (const float4 *)leaves + 2 * (size_t)k;
+    float4 a = L[0], b = L[1];
+    lo = vmin(mk(a.x - a.w, a.y - a.w, a.z - a.w), mk(b.x - b.w, b.y - b.w, b.z - b.w));
+    hi = vmax(mk(a.x + a.w, a.y + a.w, a.z + a.w), mk(b.x + b.w, b.y + b.w, b.z + b.w));
   } else if constexpr (T == VRX_TRIANGLE) {
     const float4 *L = (const float4 *)leaves + 3 * (size_t)k;
     float4 a = L[0], b = L[1], c = L[2];
@@ -641,7 +662,7 @@ __global__ __launch_bounds__(256) void k_leaves(
   // contiguous 16/32/48-byte read instead of separate index, vertex and
   // radius gathers
   constexpr int nq = T == VRX_SPHERE ? 1
-      : (T == VRX_CYLINDER || T == VRX_CONE) ? 2 : (T == VRX_TRIANGLE ? 3 : 0);
+      : (T == VRX_CYLINDER || T == VRX_CONE || T == VRX_CURVE) ? 2 : (T == VRX_TRIANGLE ? 3 : 0);
   const float4 *src = (const float4 *)rec + (size_t)nq * orig;
   float4 *dst = (float4 *)leaves + (size_t)nq * k;
 #pragma unroll
@@ -901,6 +922,9 @@ int bvh_build_impl(const vrx_prims *p, const vrx_bvh *out, void *d_ws, size_t ws
   case VRX_CONE:
     VRX_ARG(p->d_radius, "cones need per-vertex radii (d_radius)");
     return build_typed<VRX_CONE>(*p, *out, ws, codes_out, cbox_out, s);
+  case VRX_CURVE:
+    VRX_ARG(p->d_index, "curves need segment indices (d_index)");
+    return build_typed<VRX_CURVE>(*p, *out, ws, codes_out, cbox_out, s);
   default:
     set_error("unknown primitive type %d", p->type);
     return VRX_ERR_ARG;
@@ -925,7 +949,7 @@ extern "C" {
 int vrx_bvh_query_sizes(int32_t type, uint32_t n, vrx_bvh_sizes *out)
 {
   VRX_ARG(out, "out");
-  VRX_ARG(type >= 0 && type <= VRX_CONE, "type");
+  VRX_ARG(type >= 0 && type <= VRX_CURVE, "type");
   bvh_sizes(type, n, out);
   return VRX_OK;
 }

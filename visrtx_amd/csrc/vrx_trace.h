@@ -329,6 +329,58 @@ __device__ __forceinline__ int isect_cone(
   return valid;
 }
 
+// Round linear curve segment: the sphere swept from (pa, ra) to (pb, rb),
+// the reference's curve primitive type (Curve.cpp:83, OptiX round linear
+// curves).  The surface is the convex hull of the two end spheres, so the
+// ray enters it once: on the tangent cone (the published closed form for a
+// rounded cone, I. Quilez), when the entry lies between the two tangent
+// circles, else on the nearer end sphere.  When one end sphere contains the
+// other the segment is that sphere.  Returns hit; t is the entry distance,
+// n an unnormalised outward normal.
+__device__ __forceinline__ bool isect_round_cone(
+    v3 ro, v3 rd, v3 pa, v3 pb, float ra, float rb, float &t, v3 &n)
+{
+  const v3 ba = sub(pb, pa), oa = sub(ro, pa), ob = sub(ro, pb);
+  const float rr = ra - rb;
+  const float m0 = dot(ba, ba), m1 = dot(ba, oa), m2 = dot(ba, rd), m3 = dot(rd, oa);
+  const float m5 = dot(oa, oa), m6 = dot(ob, rd), m7 = dot(ob, ob);
+  const float d2 = m0 - rr * rr;
+  if (d2 > 0.0f) {
+    const float k2 = d2 - m2 * m2;
+    const float k1 = (d2 * m3 - m1 * m2) + (m2 * rr) * ra;
+    const float k0 = ((d2 * m5 - m1 * m1) + ((m1 * rr) * ra) * 2.0f) - (m0 * ra) * ra;
+    const float h = k1 * k1 - k0 * k2;
+    if (h < 0.0f)
+      return false; // the ray misses the tangent cone, so both spheres
+    const float tc = (-sqrtf(h) - k1) / k2;
+    const float y = (m1 - ra * rr) + tc * m2;
+    if (y > 0.0f && y < d2) {
+      t = tc;
+      n = sub(fmul(d2, add(oa, fmul(tc, rd))), fmul(y, ba));
+      return true;
+    }
+  }
+  // end spheres (for d2 <= 0 only the larger one bounds the segment)
+  const float h1 = (m3 * m3 - m5) + ra * ra;
+  const float h2 = (m6 * m6 - m7) + rb * rb;
+  const bool use1 = d2 > 0.0f || ra >= rb, use2 = d2 > 0.0f || rb > ra;
+  bool hit = false;
+  if (use1 && h1 > 0.0f) {
+    t = -m3 - sqrtf(h1);
+    n = add(oa, fmul(t, rd));
+    hit = true;
+  }
+  if (use2 && h2 > 0.0f) {
+    const float t2 = -m6 - sqrtf(h2);
+    if (!hit || t2 < t) {
+      t = t2;
+      n = add(ob, fmul(t2, rd));
+      hit = true;
+    }
+  }
+  return hit;
+}
+
 template <int PT>
 __device__ __forceinline__ int blas_type(const BlasCtx &b)
 {
@@ -418,6 +470,27 @@ __device__ __forceinline__ bool test_leaf(
           h.a2 = ns[q].z;
           tmax = t;
         }
+      }
+    }
+  } else if (type == VRX_CURVE) {
+    const float4 *L = b.leaves + 2 * (size_t)k;
+    float4 P0 = gld(L, 0), P1 = gld(L, 1);
+    float t;
+    v3 nn;
+    if (isect_round_cone(o, d, mk(P0.x, P0.y, P0.z), mk(P1.x, P1.y, P1.z), P0.w, P1.w, t, nn)
+        && t >= tmin && t <= tmax) {
+      if (ANY)
+        return true;
+      uint32_t prim = gld(b.ids, k);
+      if (take(h, t, tmax, b.entry, prim)) {
+        h.found = true;
+        h.t = t;
+        h.entry = b.entry;
+        h.prim = prim;
+        h.a0 = nn.x;
+        h.a1 = nn.y;
+        h.a2 = nn.z;
+        tmax = t;
       }
     }
   } else if (type == VRX_CONE) {

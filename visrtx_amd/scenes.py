@@ -24,7 +24,7 @@ from . import anari as A
 # ---------------------------------------------------------------------------
 @dataclass
 class GeomDesc:
-    type: str                       # 'triangle' | 'quad' | 'sphere' | 'cylinder' | 'cone'
+    type: str                       # 'triangle' | 'quad' | 'sphere' | 'cylinder' | 'cone' | 'curve'
     position: np.ndarray            # (V, 3) float32
     index: np.ndarray | None = None
     radius: np.ndarray | None = None
@@ -45,6 +45,8 @@ class GeomDesc:
             return 2 * (len(self.index) if self.index is not None else len(self.position) // 4)
         if self.type == "sphere":
             return len(self.index) if self.index is not None else len(self.position)
+        if self.type == "curve":    # segment i: vertices index[i], index[i] + 1
+            return len(self.index) if self.index is not None else len(self.position) // 2
         return len(self.index) if self.index is not None else len(self.position) // 2
 
 
@@ -230,7 +232,7 @@ def batched_workload(views: int = 64, res: int = 512, instances: int = 256, mesh
 # ANARI construction
 # ---------------------------------------------------------------------------
 _GEOM_SUB = {"triangle": "triangle", "quad": "quad", "sphere": "sphere", "cylinder": "cylinder",
-             "cone": "cone"}
+             "cone": "cone", "curve": "curve"}
 
 
 def quads_to_triangles(index: np.ndarray | None, num_vertices: int) -> np.ndarray:
@@ -275,10 +277,10 @@ def _surface(dev: A.Device, s: AnariScene, g: GeomDesc):
     dev.set(geom, "vertex.position", s.keep(dev.array(g.position, A.FLOAT32_VEC3)), A.ARRAY1D)
     if g.index is not None:
         et = {"triangle": A.UINT32_VEC3, "quad": A.UINT32_VEC4, "sphere": A.UINT32,
-              "cylinder": A.UINT32_VEC2, "cone": A.UINT32_VEC2}[g.type]
+              "cylinder": A.UINT32_VEC2, "cone": A.UINT32_VEC2, "curve": A.UINT32}[g.type]
         dev.set(geom, "primitive.index", s.keep(dev.array(g.index, et)), A.ARRAY1D)
     if g.radius is not None:
-        name = "vertex.radius" if g.type in ("sphere", "cone") else "primitive.radius"
+        name = "vertex.radius" if g.type in ("sphere", "cone", "curve") else "primitive.radius"
         dev.set(geom, name, s.keep(dev.array(g.radius, A.FLOAT32)), A.ARRAY1D)
     if g.global_radius is not None:
         dev.set(geom, "radius", g.global_radius, A.FLOAT32)
