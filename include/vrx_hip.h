@@ -178,6 +178,27 @@ typedef struct vrx_volume {
   uint32_t pad;
 } vrx_volume;
 
+/* One light in world space, as the directLight renderer samples it
+ * (gpu/sampleLight.h:58-100; scene/light/{Directional,Point,Spot}.cpp).
+ * The host applies the owning instance's transform:
+ *   directional: dir = xfm (linear part) * -direction (direction normalised
+ *                at commit), the direction towards the light, not
+ *                renormalised; radiance = color * irradiance
+ *   point:       pos = xfm * position; radiance = color * intensity
+ *   spot:        pos = xfm * position; dir = direction as given (the
+ *                reference does not transform it); radiance = color *
+ *                intensity; cos_outer = cos(openingAngle), cos_inner =
+ *                cos(openingAngle - 2 falloffAngle) */
+enum vrx_light_type { VRX_LIGHT_DIRECTIONAL = 0, VRX_LIGHT_POINT = 1, VRX_LIGHT_SPOT = 2 };
+
+typedef struct vrx_light {
+  int32_t type;
+  float radiance[3];
+  float pos[3];
+  float dir[3];
+  float cos_outer, cos_inner;
+} vrx_light;
+
 typedef struct vrx_scene {
   const vrx_geom *d_geoms;      /* device array */
   uint32_t num_geoms;
@@ -191,7 +212,8 @@ typedef struct vrx_scene {
   int32_t single_type;          /* prim type if every geom shares it, else -1 */
   uint64_t max_volume_voxels;   /* largest field's voxel count (>= 2^31 selects 64-bit gathers) */
   uint32_t wide;                /* 1 = every BLAS (and the TLAS if any) has d_wide */
-  uint32_t pad;
+  uint32_t num_lights;
+  const vrx_light *d_lights;    /* device array (directLight renderer) */
 } vrx_scene;
 
 typedef struct vrx_camera {
@@ -202,7 +224,7 @@ typedef struct vrx_camera {
   float region[4];
 } vrx_camera;
 
-enum vrx_renderer_kind { VRX_RENDER_RAYCAST = 0, VRX_RENDER_AO = 1 };
+enum vrx_renderer_kind { VRX_RENDER_RAYCAST = 0, VRX_RENDER_AO = 1, VRX_RENDER_DIRECT = 2 };
 enum vrx_color_format { VRX_COLOR_F32 = 0, VRX_COLOR_SRGB8 = 1, VRX_COLOR_UNORM8 = 2 };
 
 typedef struct vrx_renderer {
@@ -242,7 +264,9 @@ size_t vrx_render_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r,
     uint32_t width, uint32_t height);
 /* newFrame clears (accumulation, depth = FLT_MAX, ids = 0). */
 int vrx_frame_clear(const vrx_frame *f, vrx_stream s);
-/* One frame (all pixel samples) of the raycast or AO renderer. */
+/* One frame (all pixel samples) of the raycast, AO or directLight renderer
+ * (Raycast_ptx.cu:60-179, AmbientOcclusion_ptx.cu:86-220,
+ * DirectLight_ptx.cu:228-420).  directLight reads scene->d_lights. */
 int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer *r,
     const vrx_frame *f, vrx_stream s);
 /* Host-side camera setup (camera/Perspective.cpp:40-72). */

@@ -777,6 +777,7 @@ typedef struct {
   float t;
   uint64_t key;
   int found;
+  float acc; /* any-hit: accumulated opacity */
   uint32_t entry, prim;
   float a0, a1, a2; /* tri: u, v ; cylinder: normal */
 } best_t;
@@ -799,6 +800,15 @@ typedef struct {
   int any; /* any-hit mode */
 } tctx;
 
+/* Any-hit of shadow and AO rays (AmbientOcclusion_ptx.cu:53-69,
+   DirectLight_ptx.cu:255-280): every reported intersection adds its
+   surface's opacity, o += a (1 - o); the ray ends once o reaches 0.99. */
+static inline int any_hit(tctx *c, best_t *best)
+{
+  best->acc = best->acc + c->G->opacity * (1.0f - best->acc);
+  return !(best->acc < 0.99f);
+}
+
 /* test primitive p (original index).  returns 1 if any-hit terminated. */
 static inline int test_prim(tctx *c, uint32_t p, float tmin, float *tmax, best_t *best)
 {
@@ -811,7 +821,7 @@ static inline int test_prim(tctx *c, uint32_t p, float tmin, float *tmax, best_t
     if (isect_sphere(c->o, c->d, ld3(G->position + 3 * (size_t)vi), r, &t)) {
       if (t >= tmin && t <= *tmax) {
         if (c->any)
-          return 1;
+          return any_hit(c, best);
         if (better(best, t, key)) {
           best->found = 1; best->t = t; best->key = key; best->entry = c->e; best->prim = p;
           *tmax = t;
@@ -832,8 +842,11 @@ static inline int test_prim(tctx *c, uint32_t p, float tmin, float *tmax, best_t
           continue;
         float t = ts[k];
         if (t >= tmin && t <= *tmax) {
-          if (c->any)
-            return 1;
+          if (c->any) {
+            if (any_hit(c, best))
+              return 1;
+            continue;
+          }
           if (better(best, t, key)) {
             best->found = 1; best->t = t; best->key = key; best->entry = c->e; best->prim = p;
             best->a0 = ns[k].x; best->a1 = ns[k].y; best->a2 = ns[k].z;
@@ -852,7 +865,7 @@ static inline int test_prim(tctx *c, uint32_t p, float tmin, float *tmax, best_t
             ld3(G->position + 3 * (size_t)a + 3), r0, r1, &t, &nn)
         && t >= tmin && t <= *tmax) {
       if (c->any)
-        return 1;
+        return any_hit(c, best);
       if (better(best, t, key)) {
         best->found = 1; best->t = t; best->key = key; best->entry = c->e; best->prim = p;
         best->a0 = nn.x; best->a1 = nn.y; best->a2 = nn.z;
@@ -872,8 +885,11 @@ static inline int test_prim(tctx *c, uint32_t p, float tmin, float *tmax, best_t
           continue;
         float t = ts[k];
         if (t >= tmin && t <= *tmax) {
-          if (c->any)
-            return 1;
+          if (c->any) {
+            if (any_hit(c, best))
+              return 1;
+            continue;
+          }
           if (better(best, t, key)) {
             best->found = 1; best->t = t; best->key = key; best->entry = c->e; best->prim = p;
             best->a0 = ns[k].x; best->a1 = ns[k].y; best->a2 = ns[k].z;
@@ -897,7 +913,7 @@ static inline int test_prim(tctx *c, uint32_t p, float tmin, float *tmax, best_t
             return 0;
         }
         if (c->any)
-          return 1;
+          return any_hit(c, best);
         if (better(best, t, key)) {
           best->found = 1; best->t = t; best->key = key; best->entry = c->e; best->prim = p;
           best->a0 = u; best->a1 = v;
@@ -1079,6 +1095,21 @@ int orc_trace_any(const orc_scene *s, const float org[3], const float dir[3], fl
 {
   best_t b;
   return trace_impl(s, ld3(org), ld3(dir), tmin, tmax, 0, 1, &b, stats);
+}
+
+static float surface_atten(const orc_scene *s, v3 o, v3 d, float tmin, float tmax,
+    uint64_t *stats)
+{
+  best_t b;
+  if (trace_impl(s, o, d, tmin, tmax, 0, 1, &b, stats))
+    return 1.0f; /* reached 0.99: fully blocked */
+  return b.acc;
+}
+
+float orc_surface_attenuation(const orc_scene *s, const float org[3], const float dir[3],
+    float tmin, float tmax)
+{
+  return surface_atten(s, ld3(org), ld3(dir), tmin, tmax, NULL);
 }
 
 int orc_trace_brute(const orc_scene *s, const float org[3], const float dir[3], float tmin,
@@ -1280,6 +1311,8 @@ void orc_classify(const orc_volume *v, double s, double rgba[4])
 
 static inline int near_rel(double a, double b)
 {
+  if (isinf(a) || isinf(b))
+    return 0; /* an unbounded ray end is never a tie */
   double m = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
   return fabs(a - b) <= AMB_T_REL * (m > 1.0 ? m : 1.0);
 }
@@ -1616,12 +1649,127 @@ static inline v4 inv_tonemap4(v4 v)
   return r;
 }
 
+/* computeAO (gpu/computeAO.h:38-60): n cosine-weighted hemisphere rays from
+   the geometric-normal offset point, four PCG draws each, each weighted by
+   the surface attenuation along it. */
+static float ao_factor(const orc_scene *s, const shit *h, float dist, int n, orc_pcg *rs,
+    uint64_t *rays, uint64_t *st)
+{
+  float weights = 0.0f, hits = 0.0f;
+  for (int k = 0; k < n; k++) {
+    v3 ao_o = add(h->hp, fmul(h->eps, h->ng));
+    float q0 = orc_pcg_float(rs), q1 = orc_pcg_float(rs), q2 = orc_pcg_float(rs);
+    (void)orc_pcg_float(rs);
+    v3 dir = normalize(mk(2.0f * q0 - 1.0f, 2.0f * q1 - 1.0f, 2.0f * q2 - 1.0f));
+    if (!(dot(dir, h->ns) > 0.0f))
+      dir = neg(dir);
+    float wgt = fmaxf(0.0f, dot(dir, h->ns));
+    weights += wgt;
+    if (wgt != 0.0f) {
+      (*rays)++;
+      hits += wgt * surface_atten(s, ao_o, dir, 0.0f, dist, st);
+    }
+  }
+  return weights > 0.0f ? 1.0f - hits / weights : 0.0f;
+}
+
+/* volumeAttenuation (DirectLight_ptx.cu:62-69, any-hit :270-279): every
+   volume the shadow ray crosses on [0, tmax] marched into one opacity
+   (FP64, no colour), volumes in index order, stopping at 0.99. */
+static float volume_atten(const orc_scene *s, v3 o, v3 d, float tmax, float inv_rate,
+    uint64_t *ns, int *amb)
+{
+  double op = 0.0, c[3] = {0.0, 0.0, 0.0};
+  double od[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+  for (uint32_t i = 0; i < s->num_volumes && op < 0.99; i++) {
+    double t0, t1;
+    if (!vol_slab_amb(&s->volumes[i], od, dd, 0.0, (double)tmax, &t0, &t1, amb))
+      continue;
+    march_amb(&s->volumes[i], od, dd, t0, t1, (double)inv_rate, 1, c, &op, ns, amb);
+  }
+  return (float)op;
+}
+
+/* sampleLight (gpu/sampleLight.h:58-100), lights in world space. */
+typedef struct {
+  v3 radiance, dir;
+  float dist;
+} lsample;
+
+static lsample sample_light(const orc_light *L, v3 hp)
+{
+  lsample ls;
+  v3 rad = ld3(L->radiance);
+  if (L->type == ORC_LIGHT_DIRECTIONAL) {
+    ls.dir = ld3(L->dir);
+    ls.dist = INFINITY;
+    ls.radiance = rad;
+  } else if (L->type == ORC_LIGHT_POINT) {
+    v3 v = sub(ld3(L->pos), hp);
+    ls.dist = sqrtf(dot(v, v));
+    ls.dir = normalize(v);
+    ls.radiance = rad;
+  } else {
+    /* spot: the distance is the length of the normalised direction, as the
+       reference computes it (sampleLight.h:81-82) */
+    ls.dir = normalize(sub(ld3(L->pos), hp));
+    ls.dist = sqrtf(dot(ls.dir, ls.dir));
+    float spot = dot(normalize(ld3(L->dir)), neg(ls.dir));
+    if (spot < L->cos_outer)
+      spot = 0.0f;
+    else if (spot > L->cos_inner)
+      spot = 1.0f;
+    else {
+      spot = (spot - L->cos_outer) / (L->cos_inner - L->cos_outer);
+      spot = spot * spot * (3.0f - 2.0f * spot);
+    }
+    ls.radiance = mulf(rad, spot);
+  }
+  return ls;
+}
+
+/* shadeSurface of the directLight renderer (DirectLight_ptx.cu:71-225) for
+   matte surfaces (MatteShader_ptx.cu:71-79): AO factor, ambient times tint,
+   and per light base/pi * max(0, Ns.L) * radiance times the surface and
+   volume transmission of the shadow ray from hp + eps*Ns.  Matte surfaces
+   spawn no bounce ray. */
+static v4 shade_direct(const orc_scene *s, const orc_renderer *r, const shit *h, int aos,
+    float inv_rate, orc_pcg *rs, uint64_t *rays, uint64_t *vs, int *amb, uint64_t *st)
+{
+  v3 sp = add(h->hp, fmul(h->eps, h->ns));
+  float aof = aos > 0 ? ao_factor(s, h, r->ao_distance, aos, rs, rays, st) : 1.0f;
+  v3 base = ld3(h->G->color);
+  v3 contrib = mk(0, 0, 0);
+  if (r->ambient_radiance > 0.0f)
+    contrib = mulv(mulf(ld3(r->ambient_color), r->ambient_radiance), base);
+  v3 kd = mulf(base, (float)0.318309886183790671538); /* 1/pi */
+  for (uint32_t l = 0; l < s->num_lights; l++) {
+    lsample ls = sample_light(&s->lights[l], h->hp);
+    float ndl = fmaxf(0.0f, dot(h->ns, ls.dir));
+    v3 c = mulv(mulf(kd, ndl), ls.radiance);
+    if (isnan(c.x) || isnan(c.y) || isnan(c.z))
+      continue;
+    if (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)
+      continue; /* adds exactly zero whatever the shadow ray finds */
+    (*rays)++;
+    float so = 1.0f - surface_atten(s, sp, ls.dir, 0.0f, ls.dist, st);
+    float vo = 1.0f;
+    if (s->num_volumes)
+      vo = 1.0f - volume_atten(s, sp, ls.dir, ls.dist, inv_rate, vs, amb);
+    contrib = add(contrib, mulf(c, so * vo));
+  }
+  contrib = mulf(contrib, aof);
+  v4 out = {contrib, h->G->opacity};
+  return out;
+}
+
 int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r, orc_frame *f)
 {
   cam_x cx = cam_setup(cam);
   const uint32_t W = f->width, H = f->height;
   float iw = 1.0f / (float)W, ih = 1.0f / (float)H;
-  int spp = r->kind == ORC_AO ? (r->pixel_samples > 1 ? r->pixel_samples : 1) : 1;
+  const int sampled = r->kind == ORC_AO || r->kind == ORC_DIRECT;
+  int spp = sampled ? (r->pixel_samples > 1 ? r->pixel_samples : 1) : 1;
   int aos = r->ambient_samples < 0 ? 0 : (r->ambient_samples > 256 ? 256 : r->ambient_samples);
   float inv_rate = 1.0f / r->volume_sampling_rate;
   uint64_t s_nodes = 0, s_prims = 0, s_rays = 0, s_vs = 0;
@@ -1642,7 +1790,7 @@ int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r,
       uint64_t st[2] = {0, 0};
       for (int i = 0; i < spp; i++) {
         v3 o, d;
-        if (r->kind == ORC_AO) {
+        if (sampled) {
           float r0 = orc_pcg_float(&rs), r1 = orc_pcg_float(&rs);
           (void)orc_pcg_float(&rs);
           (void)orc_pcg_float(&rs);
@@ -1680,39 +1828,39 @@ int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r,
               }
               first = 0;
             }
-            v3 base = ld3(h.G->color);
-            float mop = h.G->opacity;
-            v3 lighting;
-            if (r->kind == ORC_AO) {
-              float ao = 1.0f;
-              if (aos > 0) {
-                float weights = 0.0f, hits = 0.0f;
-                for (int k = 0; k < aos; k++) {
-                  v3 ao_o = add(h.hp, fmul(h.eps, h.ng));
-                  float q0 = orc_pcg_float(&rs), q1 = orc_pcg_float(&rs), q2 = orc_pcg_float(&rs);
-                  (void)orc_pcg_float(&rs);
-                  v3 dir = normalize(mk(2.0f * q0 - 1.0f, 2.0f * q1 - 1.0f, 2.0f * q2 - 1.0f));
-                  if (!(dot(dir, h.ns) > 0.0f))
-                    dir = neg(dir);
-                  float wgt = fmaxf(0.0f, dot(dir, h.ns));
-                  weights += wgt;
-                  if (wgt != 0.0f) {
-                    best_t bb;
-                    s_rays++;
-                    int occ = trace_impl(s, ao_o, dir, 0.0f, r->ao_distance, 0, 1, &bb, st);
-                    hits += wgt * (occ ? 1.0f : 0.0f);
-                  }
-                }
-                ao = weights > 0.0f ? 1.0f - hits / weights : 0.0f;
+            if (r->kind == ORC_DIRECT) {
+              uint64_t nr = 0;
+              v4 sr = shade_direct(s, r, &h, aos, inv_rate, &rs, &nr, &s_vs, &amb, st);
+              s_rays += nr;
+              /* DirectLight_ptx.cu:372-381: the shading result replaces the
+                 volume colour/opacity in front of the surface and is then
+                 accumulated onto itself */
+              if (isnan(sr.c.x) || isnan(sr.c.y) || isnan(sr.c.z)) {
+                color = mk(0, 0, 0);
+                opacity = 0.0f;
+              } else {
+                color = sr.c;
+                opacity = sr.a;
               }
-              float sc = ao * r->ambient_radiance;
-              lighting = fmul(sc, ld3(r->ambient_color));
+              acc3(&color, sr.c, opacity);
+              acc1(&opacity, sr.a, opacity);
             } else {
-              float sc = fabsf(dot(d, h.ns));
-              lighting = fmul(sc, ld3(r->ambient_color));
+              v3 base = ld3(h.G->color);
+              float mop = h.G->opacity;
+              v3 lighting;
+              if (r->kind == ORC_AO) {
+                uint64_t nr = 0;
+                float ao = aos > 0 ? ao_factor(s, &h, r->ao_distance, aos, &rs, &nr, st) : 1.0f;
+                s_rays += nr;
+                float sc = ao * r->ambient_radiance;
+                lighting = fmul(sc, ld3(r->ambient_color));
+              } else {
+                float sc = fabsf(dot(d, h.ns));
+                lighting = fmul(sc, ld3(r->ambient_color));
+              }
+              acc3(&color, mulv(base, lighting), opacity);
+              acc1(&opacity, mop, opacity);
             }
-            acc3(&color, mulv(base, lighting), opacity);
-            acc1(&opacity, mop, opacity);
             color = mulf(color, opacity);
             acc3(&outc, color, outo);
             acc1(&outo, opacity, outo);

@@ -119,6 +119,21 @@ typedef struct {
   uint32_t inst_id;             /* instance id (world id for world-level) */
 } orc_volume;
 
+/* A light in world space (gpu/sampleLight.h:58-100): the host has applied
+ * the instance transform.  directional: dir = direction towards the light,
+ * radiance = color*irradiance; point: pos, radiance = color*intensity;
+ * spot: pos, dir = cone axis (untransformed), radiance = color*intensity,
+ * cos_outer/cos_inner of the opening angle and of the inner angle. */
+enum { ORC_LIGHT_DIRECTIONAL = 0, ORC_LIGHT_POINT = 1, ORC_LIGHT_SPOT = 2 };
+
+typedef struct {
+  int32_t type;
+  float radiance[3];
+  float pos[3];
+  float dir[3];
+  float cos_outer, cos_inner;
+} orc_light;
+
 typedef struct {
   uint32_t num_geoms;
   const orc_geom *geoms;
@@ -127,6 +142,8 @@ typedef struct {
   const orc_entry *entries;
   uint32_t num_volumes;
   const orc_volume *volumes;
+  uint32_t num_lights;
+  const orc_light *lights;
 } orc_scene;
 
 int orc_scene_build(orc_scene *s);
@@ -138,7 +155,7 @@ typedef struct {
   float region[4];
 } orc_camera;
 
-enum { ORC_RAYCAST = 0, ORC_AO = 1 };
+enum { ORC_RAYCAST = 0, ORC_AO = 1, ORC_DIRECT = 2 };
 enum { ORC_COLOR_F32 = 0, ORC_COLOR_SRGB8 = 1, ORC_COLOR_UNORM8 = 2 };
 
 typedef struct {
@@ -183,6 +200,10 @@ int orc_trace_brute(const orc_scene *s, const float org[3], const float dir[3],
     float tmin, float tmax, int cull, orc_hit *hit);
 int orc_trace_any(const orc_scene *s, const float org[3], const float dir[3],
     float tmin, float tmax, uint64_t *stats);
+/* surfaceAttenuation (intersectRay.h:105-111): opacity accumulated over every
+ * surface hit on [tmin, tmax], o += a (1 - o), 1 once it reaches 0.99. */
+float orc_surface_attenuation(const orc_scene *s, const float org[3], const float dir[3],
+    float tmin, float tmax);
 
 /* Batched closest-hit for arrays of rays (OpenMP). Out arrays n each. */
 void orc_trace_rays(const orc_scene *s, uint32_t n, const float *org, const float *dir,

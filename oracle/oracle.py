@@ -7,6 +7,7 @@ oracle.h for which reference files each oracle function restates.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -15,7 +16,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 ORC_TRIANGLE, ORC_SPHERE, ORC_CYLINDER, ORC_CONE, ORC_CURVE = 0, 1, 2, 4, 5
-ORC_RAYCAST, ORC_AO = 0, 1
+ORC_RAYCAST, ORC_AO, ORC_DIRECT = 0, 1, 2
+ORC_LIGHT_DIRECTIONAL, ORC_LIGHT_POINT, ORC_LIGHT_SPOT = 0, 1, 2
 ORC_COLOR_F32, ORC_COLOR_SRGB8, ORC_COLOR_UNORM8 = 0, 1, 2
 LEAF = 0x80000000
 
@@ -44,10 +46,15 @@ class orc_volume(C.Structure):
                 ("unit_distance", C.c_float), ("id", C.c_uint32), ("inst_id", C.c_uint32)]
 
 
+class orc_light(C.Structure):
+    _fields_ = [("type", C.c_int32), ("radiance", C.c_float * 3), ("pos", C.c_float * 3),
+                ("dir", C.c_float * 3), ("cos_outer", C.c_float), ("cos_inner", C.c_float)]
+
+
 class orc_scene(C.Structure):
     _fields_ = [("num_geoms", C.c_uint32), ("geoms", vp), ("bvhs", vp),
                 ("num_entries", C.c_uint32), ("entries", vp), ("num_volumes", C.c_uint32),
-                ("volumes", vp)]
+                ("volumes", vp), ("num_lights", C.c_uint32), ("lights", vp)]
 
 
 class orc_camera(C.Structure):
@@ -120,6 +127,8 @@ def lib() -> C.CDLL:
         L.orc_trace_closest.argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_int, vp, vp]
         L.orc_trace_brute.argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_int, vp]
         L.orc_trace_any.argtypes = [vp, vp, vp, C.c_float, C.c_float, vp]
+        L.orc_surface_attenuation.argtypes = [vp, vp, vp, C.c_float, C.c_float]
+        L.orc_surface_attenuation.restype = C.c_float
         L.orc_trace_rays.argtypes = [vp, C.c_uint32, vp, vp, C.c_float, C.c_float, vp, vp, vp, vp]
         L.orc_intersect_sphere.argtypes = [vp, vp, vp, C.c_float, vp]
         L.orc_intersect_cylinder.argtypes = [vp, vp, vp, vp, C.c_float, C.c_int, vp, vp, vp]
@@ -221,6 +230,46 @@ def tf_discretize(color, opacity, value_range=(0.0, 1.0), uniform_color=(1, 1, 1
     return lut
 
 
+def light_record(L, xfm=None):
+    """World-space light record of a scenes.LightDesc under a column-major 3x4
+    instance transform, in fp32 with the device host's operation order
+    (scene/light/{Directional,Point,Spot}.cpp commit rules; gpu/sampleLight.h
+    xfmVec / xfmPoint)."""
+    f = np.float32
+    m = IDENTITY12 if xfm is None else np.asarray(xfm, np.float32)
+    rec = orc_light()
+    col = [f(c) for c in L.color]
+
+    def vec(v):
+        return [f(f(f(m[i] * v[0]) + f(m[3 + i] * v[1])) + f(m[6 + i] * v[2])) for i in range(3)]
+
+    if L.type == "directional":
+        rec.type = ORC_LIGHT_DIRECTIONAL
+        d = [f(x) for x in L.direction]
+        inv = f(1.0) / np.sqrt(f(f(f(d[0] * d[0]) + f(d[1] * d[1])) + f(d[2] * d[2])))
+        nd = [-f(x * inv) for x in d]
+        rec.dir[:] = vec(nd)
+        irr = f(max(float(L.irradiance), 0.0))
+        rec.radiance[:] = [f(c * irr) for c in col]
+    else:
+        p = [f(x) for x in L.position]
+        rec.pos[:] = [f(v + m[9 + i]) for i, v in enumerate(vec(p))]
+        inten = f(max(float(L.intensity), 0.0))
+        rec.radiance[:] = [f(c * inten) for c in col]
+        if L.type == "point":
+            rec.type = ORC_LIGHT_POINT
+        elif L.type == "spot":
+            rec.type = ORC_LIGHT_SPOT
+            rec.dir[:] = [f(x) for x in L.direction]
+            opening = f(L.opening_angle)
+            inner = f(opening - f(f(2.0) * f(L.falloff_angle)))
+            rec.cos_outer = float(f(math.cos(float(opening))))
+            rec.cos_inner = float(f(math.cos(float(inner))))
+        else:
+            raise ValueError(L.type)
+    return rec
+
+
 class OracleScene:
     """An orc_scene built from a visrtx_amd.scenes.SceneDesc (same arrays)."""
 
@@ -279,6 +328,11 @@ class OracleScene:
             V.unit_distance = v.unit_distance
             V.id = v.id
             V.inst_id = sc.world_id
+        recs = [light_record(L) for L in getattr(sc, "lights", [])]
+        for inst in sc.instances:
+            recs += [light_record(L, inst.xfm) for L in getattr(inst, "lights", [])]
+        lights = (orc_light * max(len(recs), 1))(*recs)
+        self._lights = lights
         self._geoms, self._entries, self._vols = geoms, entries, vols
         self.s = orc_scene()
         self.s.num_geoms = len(sc.geoms)
@@ -287,6 +341,8 @@ class OracleScene:
         self.s.entries = C.addressof(entries)
         self.s.num_volumes = len(sc.volumes)
         self.s.volumes = C.addressof(vols)
+        self.s.num_lights = len(recs)
+        self.s.lights = C.addressof(lights)
         if lib().orc_scene_build(C.byref(self.s)) != 0:
             raise RuntimeError("orc_scene_build failed")
 
@@ -330,6 +386,12 @@ class OracleScene:
         return bool(lib().orc_trace_any(C.byref(self.s), o.ctypes.data, d.ctypes.data, tmin, tmax,
                                         None))
 
+    def surface_attenuation(self, org, dir_, tmin=0.0, tmax=3.4e38) -> float:
+        o = np.asarray(org, np.float32)
+        d = np.asarray(dir_, np.float32)
+        return float(lib().orc_surface_attenuation(C.byref(self.s), o.ctypes.data, d.ctypes.data,
+                                                   tmin, tmax))
+
     def trace_rays(self, org, dirs, tmin=0.0, tmax=3.4e38):
         n = len(org)
         o = np.ascontiguousarray(org, np.float32)
@@ -345,10 +407,11 @@ class OracleScene:
     def render(self, cam, width, height, kind="raycast", color_format=ORC_COLOR_F32,
                threads=0, **rp):
         r = orc_renderer()
-        r.kind = ORC_AO if kind in ("ao", "default") else ORC_RAYCAST
+        r.kind = {"ao": ORC_AO, "directLight": ORC_DIRECT, "default": ORC_DIRECT}.get(
+            kind, ORC_RAYCAST)
         r.background[:] = list(rp.get("background", (0.0, 0.0, 0.0, 1.0)))
         r.ambient_color[:] = list(rp.get("ambientColor", (1.0, 1.0, 1.0)))
-        r.ambient_radiance = rp.get("ambientRadiance", 1.0)
+        r.ambient_radiance = rp.get("ambientRadiance", 1.0 if r.kind == ORC_AO else 0.0)
         r.pixel_samples = rp.get("pixelSamples", 1)
         r.ambient_samples = rp.get("ambientSamples", 1)
         r.ao_distance = rp.get("ambientOcclusionDistance", 1e20)

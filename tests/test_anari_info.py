@@ -4,6 +4,7 @@ object's parameters, and the defaults and clamps it applies at commit
 (renderer defaults Renderer.cpp:152-170, sphere radius Sphere.cu:52,
 camera fovy 60 degrees and direction +z Camera.cpp:68-76)."""
 import ctypes as C
+import math
 
 import pytest
 
@@ -63,11 +64,12 @@ def subtypes(L, typ):
 
 def test_subtypes(lib):
     assert set(subtypes(lib, A.GEOMETRY)) >= {"triangle", "quad", "sphere", "cylinder", "cone", "curve"}
-    assert set(subtypes(lib, A.RENDERER)) >= {"raycast", "ao", "default"}
+    assert set(subtypes(lib, A.RENDERER)) >= {"raycast", "ao", "directLight", "default"}
+    assert set(subtypes(lib, A.LIGHT)) == {"directional", "point", "spot"}
 
 
 def test_every_listed_subtype_has_a_parameter_table(lib):
-    for typ in (A.GEOMETRY, A.RENDERER):
+    for typ in (A.GEOMETRY, A.RENDERER, A.LIGHT):
         for sub in subtypes(lib, typ):
             assert params(lib, typ, sub.encode()), (typ, sub)
 
@@ -107,6 +109,20 @@ def test_defaults_and_clamps(lib):
     assert default(lib, A.CAMERA, b"perspective", "fovy", A.FLOAT32, C.c_float) == \
         pytest.approx(1.0471976)
     assert default(lib, A.SURFACE, None, "id", A.UINT32, C.c_uint32) == 0xFFFFFFFF
+    # ambientRadiance defaults: 1 for ao, 0 for directLight / default
+    # (AmbientOcclusion.cpp:44, DirectLight.cpp:46)
+    assert default(lib, A.RENDERER, b"ao", "ambientRadiance", A.FLOAT32, C.c_float) == 1.0
+    for sub in (b"directLight", b"default"):
+        assert default(lib, A.RENDERER, sub, "ambientRadiance", A.FLOAT32, C.c_float) == 0.0
+        assert default(lib, A.RENDERER, sub, "lightFalloff", A.FLOAT32, C.c_float) == 1.0
+    assert default(lib, A.LIGHT, b"spot", "openingAngle", A.FLOAT32, C.c_float) == \
+        pytest.approx(math.pi)
+    assert default(lib, A.LIGHT, b"spot", "falloffAngle", A.FLOAT32, C.c_float) == \
+        pytest.approx(0.1)
+    assert default(lib, A.LIGHT, b"directional", "direction", A.FLOAT32_VEC3, C.c_float, 3) == \
+        [0.0, 0.0, -1.0]
+    assert params(lib, A.WORLD, None)["light"] == A.ARRAY1D
+    assert params(lib, A.GROUP, None)["light"] == A.ARRAY1D
     # wrong parameter type or unknown names answer null
     assert not lib.anariGetParameterInfo(None, A.GEOMETRY, b"sphere", b"radius", A.INT32,
                                          b"default", A.INT32)

@@ -43,7 +43,8 @@ def _load(path: str) -> C.CDLL:
 # ---------------------------------------------------------------------------
 VRX_TRIANGLE, VRX_SPHERE, VRX_CYLINDER, VRX_BOX, VRX_CONE = 0, 1, 2, 3, 4
 VRX_H2D, VRX_D2H, VRX_D2D = 0, 1, 2
-VRX_RENDER_RAYCAST, VRX_RENDER_AO = 0, 1
+VRX_RENDER_RAYCAST, VRX_RENDER_AO, VRX_RENDER_DIRECT = 0, 1, 2
+VRX_LIGHT_DIRECTIONAL, VRX_LIGHT_POINT, VRX_LIGHT_SPOT = 0, 1, 2
 VRX_COLOR_F32, VRX_COLOR_SRGB8, VRX_COLOR_UNORM8 = 0, 1, 2
 
 f3 = C.c_float * 3
@@ -80,7 +81,13 @@ class vrx_scene(C.Structure):
                 ("num_entries", C.c_uint32), ("tlas", vrx_bvh), ("tlas_height", C.c_uint32),
                 ("max_blas_height", C.c_uint32), ("d_volumes", C.c_void_p),
                 ("num_volumes", C.c_uint32), ("single_type", C.c_int32),
-                ("max_volume_voxels", C.c_uint64), ("wide", C.c_uint32), ("pad", C.c_uint32)]
+                ("max_volume_voxels", C.c_uint64), ("wide", C.c_uint32),
+                ("num_lights", C.c_uint32), ("d_lights", C.c_void_p)]
+
+
+class vrx_light(C.Structure):
+    _fields_ = [("type", C.c_int32), ("radiance", f3), ("pos", f3), ("dir", f3),
+                ("cos_outer", C.c_float), ("cos_inner", C.c_float)]
 
 
 class vrx_renderer(C.Structure):

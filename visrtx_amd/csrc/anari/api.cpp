@@ -35,7 +35,8 @@ inline Object *O(ANARIObject o) { return (Object *)o; }
 
 const char *g_deviceSubtypes[] = {"default", "visrtx_amd", nullptr};
 const char *g_geomSubtypes[] = {"triangle", "quad", "sphere", "cylinder", "cone", "curve", nullptr};
-const char *g_rendererSubtypes[] = {"default", "raycast", "ao", nullptr};
+const char *g_rendererSubtypes[] = {"default", "raycast", "ao", "directLight", nullptr};
+const char *g_lightSubtypes[] = {"directional", "point", "spot", nullptr};
 const char *g_cameraSubtypes[] = {"perspective", nullptr};
 const char *g_materialSubtypes[] = {"matte", nullptr};
 const char *g_fieldSubtypes[] = {"structuredRegular", nullptr};
@@ -161,7 +162,7 @@ void anariUnmapArray(ANARIDevice d, ANARIArray a)
 ANARILight anariNewLight(ANARIDevice d, const char *type)
 {
   GUARD(d, nullptr);
-  return create<Object>(d, ANARI_LIGHT, std::string(type ? type : ""));
+  return create<Light>(d, std::string(type ? type : ""));
 }
 
 ANARICamera anariNewCamera(ANARIDevice d, const char *type)
@@ -374,6 +375,8 @@ const char **anariGetObjectSubtypes(ANARIDevice, ANARIDataType t)
     return g_volumeSubtypes;
   case ANARI_INSTANCE:
     return g_instanceSubtypes;
+  case ANARI_LIGHT:
+    return g_lightSubtypes;
   default:
     return g_none;
   }

@@ -43,7 +43,9 @@ const float kUpY[3] = {0.0f, 1.0f, 0.0f};
 const float kBlack4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
 const float kRegion[4] = {0.0f, 0.0f, 1.0f, 1.0f};
 const float kRange01[2] = {0.0f, 1.0f};
-const float kOne = 1.0f;
+const float kOne = 1.0f, kZero = 0.0f;
+const float kPi = 3.14159265358979f, kFalloff = 0.1f;
+const float kDirNegZ[3] = {0.0f, 0.0f, -1.0f};
 const float kSphereRadius = 0.01f;
 const float kFovy = 1.04719755f; // 60 degrees
 const float kRate = 0.125f, kRateMin = 1e-3f, kRateMax = 10.0f;
@@ -118,9 +120,27 @@ std::vector<ObjDef> &tables()
               {"valueRange", ANARI_FLOAT32_BOX1, false, kRange01, nullptr, nullptr, "field values mapped to the maps"},
               {"unitDistance", ANARI_FLOAT32, false, &kOne, nullptr, nullptr, "distance of one opacity unit"},
               {"id", ANARI_UINT32, false, &kNoId, nullptr, nullptr, "objectId value"}}},
-      {ANARI_GROUP, nullptr, "surfaces and volumes instanced together",
+      {ANARI_LIGHT, "directional", "light from one direction (Directional.cpp)",
+          {{"color", ANARI_FLOAT32_VEC3, false, kOne3, nullptr, nullptr, "light colour"},
+              {"direction", ANARI_FLOAT32_VEC3, false, kDirNegZ, nullptr, nullptr, "direction the light travels"},
+              {"irradiance", ANARI_FLOAT32, false, &kOne, &kZero, nullptr, "irradiance"}}},
+      {ANARI_LIGHT, "point", "point light (Point.cpp)",
+          {{"color", ANARI_FLOAT32_VEC3, false, kOne3, nullptr, nullptr, "light colour"},
+              {"position", ANARI_FLOAT32_VEC3, false, kZero3, nullptr, nullptr, "light position"},
+              {"intensity", ANARI_FLOAT32, false, &kOne, &kZero, nullptr, "intensity (else power)"},
+              {"power", ANARI_FLOAT32, false, &kOne, &kZero, nullptr, "used when no intensity is set"}}},
+      {ANARI_LIGHT, "spot", "spot light with a smooth edge (Spot.cpp)",
+          {{"color", ANARI_FLOAT32_VEC3, false, kOne3, nullptr, nullptr, "light colour"},
+              {"position", ANARI_FLOAT32_VEC3, false, kZero3, nullptr, nullptr, "light position"},
+              {"direction", ANARI_FLOAT32_VEC3, false, kDirNegZ, nullptr, nullptr, "cone axis"},
+              {"openingAngle", ANARI_FLOAT32, false, &kPi, nullptr, nullptr, "full cone angle (radians)"},
+              {"falloffAngle", ANARI_FLOAT32, false, &kFalloff, nullptr, nullptr, "width of the smooth edge (radians)"},
+              {"intensity", ANARI_FLOAT32, false, &kOne, &kZero, nullptr, "intensity (else power)"},
+              {"power", ANARI_FLOAT32, false, &kOne, &kZero, nullptr, "used when no intensity is set"}}},
+      {ANARI_GROUP, nullptr, "surfaces, volumes and lights instanced together",
           {{"surface", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "SURFACE array"},
-              {"volume", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "VOLUME array"}}},
+              {"volume", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "VOLUME array"},
+              {"light", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "LIGHT array"}}},
       {ANARI_INSTANCE, "transform", "a group under an affine transform (Instance.cpp)",
           {{"group", ANARI_GROUP, true, nullptr, nullptr, nullptr, "the group"},
               {"transform", ANARI_FLOAT32_MAT4, false, kIdentity4, nullptr, nullptr, "object to world"},
@@ -129,6 +149,7 @@ std::vector<ObjDef> &tables()
           {{"surface", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "SURFACE array"},
               {"volume", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "VOLUME array"},
               {"instance", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "INSTANCE array"},
+              {"light", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "LIGHT array"},
               {"id", ANARI_UINT32, false, &kNoId, nullptr, nullptr, "instanceId of world-level objects"}}},
       {ANARI_CAMERA, "perspective", "pinhole camera (Perspective.cpp)",
           {{"position", ANARI_FLOAT32_VEC3, false, kZero3, nullptr, nullptr, "eye position"},
@@ -153,12 +174,16 @@ std::vector<ObjDef> &tables()
   return t;
 }
 
-std::vector<ParamDef> rendererParams(bool ao)
+enum { kRaycast = 0, kAO = 1, kDirect = 2 };
+
+std::vector<ParamDef> rendererParams(int kind)
 {
+  const bool ao = kind != kRaycast;
   std::vector<ParamDef> p = {
       {"background", ANARI_FLOAT32_VEC4, false, kBlack4, nullptr, nullptr, "colour of rays that miss"},
       {"ambientColor", ANARI_FLOAT32_VEC3, false, kOne3, nullptr, nullptr, "ambient light colour"},
-      {"ambientRadiance", ANARI_FLOAT32, false, &kOne, nullptr, nullptr, "ambient light intensity"},
+      {"ambientRadiance", ANARI_FLOAT32, false, kind == kAO ? &kOne : &kZero, nullptr, nullptr,
+          "ambient light intensity"},
       {"cullTriangleBackfaces", ANARI_BOOL, false, &kFalse, nullptr, nullptr, "skip back-facing triangles"},
       {"volumeSamplingRate", ANARI_FLOAT32, false, &kRate, &kRateMin, &kRateMax, "samples per half voxel"},
       {"sampleLimit", ANARI_INT32, false, &kLimit, &kInt0, nullptr, "accumulation limit (raycast: 1)"},
@@ -170,6 +195,8 @@ std::vector<ParamDef> rendererParams(bool ao)
     p.push_back({"ambientSamples", ANARI_INT32, false, &kInt1, &kInt0, &kInt256, "occlusion rays per hit"});
     p.push_back({"ambientOcclusionDistance", ANARI_FLOAT32, false, &kAoDist, nullptr, nullptr, "occlusion ray length"});
   }
+  if (kind == kDirect)
+    p.push_back({"lightFalloff", ANARI_FLOAT32, false, &kOne, &kZero, &kOne, "accepted; not used in shading"});
   return p;
 }
 
@@ -178,9 +205,11 @@ ObjDef *find(ANARIDataType type, const char *subtype)
   static std::vector<ObjDef> renderers;
   if (type == ANARI_RENDERER) {
     if (renderers.empty()) {
-      renderers.push_back({ANARI_RENDERER, "raycast", "first-hit shading (Raycast_ptx.cu)", rendererParams(false), {}});
-      renderers.push_back({ANARI_RENDERER, "ao", "ambient occlusion (AmbientOcclusion_ptx.cu)", rendererParams(true), {}});
-      renderers.push_back({ANARI_RENDERER, "default", "ambient occlusion", rendererParams(true), {}});
+      renderers.push_back({ANARI_RENDERER, "raycast", "first-hit shading (Raycast_ptx.cu)", rendererParams(kRaycast), {}});
+      renderers.push_back({ANARI_RENDERER, "ao", "ambient occlusion (AmbientOcclusion_ptx.cu)", rendererParams(kAO), {}});
+      renderers.push_back({ANARI_RENDERER, "directLight",
+          "lights with shadow rays and ambient occlusion (DirectLight_ptx.cu)", rendererParams(kDirect), {}});
+      renderers.push_back({ANARI_RENDERER, "default", "the directLight renderer", rendererParams(kDirect), {}});
     }
     for (auto &o : renderers)
       if (subtype && std::strcmp(subtype, o.subtype) == 0)

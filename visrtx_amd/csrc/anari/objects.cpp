@@ -890,6 +890,88 @@ void Group::commit()
 {
   surfaces = objectList<Surface>(this, "surface", ANARI_SURFACE);
   volumes = objectList<Volume>(this, "volume", ANARI_VOLUME);
+  lights = objectList<Light>(this, "light", ANARI_LIGHT); // Group.cpp:92
+}
+
+// ---------------------------------------------------------------------------
+// Light (scene/light/Light.cpp:47-54, Directional.cpp:40-47, Point.cpp:40-48,
+// Spot.cpp:40-68)
+// ---------------------------------------------------------------------------
+void Light::commit()
+{
+  type = -1;
+  float c[3] = {1, 1, 1};
+  getRaw("color", ANARI_FLOAT32_VEC3, c);
+  memcpy(color, c, 12);
+  const float fmax = std::numeric_limits<float>::max();
+  if (subtype == "directional") {
+    type = VRX_LIGHT_DIRECTIONAL;
+    float dv[3] = {0, 0, -1};
+    getRaw("direction", ANARI_FLOAT32_VEC3, dv);
+    const float inv = 1.0f / std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    for (int k = 0; k < 3; k++)
+      direction[k] = dv[k] * inv;
+    irradiance = std::clamp(get<float>("irradiance", ANARI_FLOAT32, 1.0f), 0.0f, fmax);
+  } else if (subtype == "point" || subtype == "spot") {
+    type = subtype == "point" ? VRX_LIGHT_POINT : VRX_LIGHT_SPOT;
+    float pv[3] = {0, 0, 0};
+    getRaw("position", ANARI_FLOAT32_VEC3, pv);
+    memcpy(position, pv, 12);
+    // "intensity", else "power", else 1
+    const float power = get<float>("power", ANARI_FLOAT32, 1.0f);
+    intensity = std::clamp(get<float>("intensity", ANARI_FLOAT32, power), 0.0f, fmax);
+    if (type == VRX_LIGHT_SPOT) {
+      float dv[3] = {0, 0, -1};
+      getRaw("direction", ANARI_FLOAT32_VEC3, dv);
+      memcpy(direction, dv, 12);
+      const float opening = get<float>("openingAngle", ANARI_FLOAT32, 3.14159265358979f);
+      const float falloff = get<float>("falloffAngle", ANARI_FLOAT32, 0.1f);
+      const float inner = opening - 2.0f * falloff;
+      if (inner < 0.0f)
+        report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+            "falloffAngle should be smaller than half of openingAngle");
+      cosOuter = std::cos(opening);
+      cosInner = std::cos(inner);
+    }
+  } else {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "light subtype '%s' is not supported (directional, point, spot)", subtype.c_str());
+  }
+}
+
+vrx_light Light::record(const float *m) const
+{
+  static const float id3[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+  if (!m)
+    m = id3;
+  vrx_light l{};
+  l.type = type;
+  // gpu/sampleLight.h:58-100: xfmVec / xfmPoint of the instance's matrix,
+  // summed column by column
+  auto vec = [&](const float *v, float *out) {
+    for (int i = 0; i < 3; i++)
+      out[i] = m[i] * v[0] + m[3 + i] * v[1] + m[6 + i] * v[2];
+  };
+  auto point = [&](const float *v, float *out) {
+    for (int i = 0; i < 3; i++)
+      out[i] = m[i] * v[0] + m[3 + i] * v[1] + m[6 + i] * v[2] + m[9 + i];
+  };
+  if (type == VRX_LIGHT_DIRECTIONAL) {
+    const float nd[3] = {-direction[0], -direction[1], -direction[2]};
+    vec(nd, l.dir);
+    for (int k = 0; k < 3; k++)
+      l.radiance[k] = color[k] * irradiance;
+  } else {
+    point(position, l.pos);
+    for (int k = 0; k < 3; k++)
+      l.radiance[k] = color[k] * intensity;
+    if (type == VRX_LIGHT_SPOT) {
+      memcpy(l.dir, direction, 12); // not transformed by the reference either
+      l.cos_outer = cosOuter;
+      l.cos_inner = cosInner;
+    }
+  }
+  return l;
 }
 
 void Instance::commit()
@@ -920,6 +1002,8 @@ void World::freeScene()
   vrx_free(dGeoms);
   vrx_free(dEntries);
   vrx_free(dVolumes);
+  vrx_free(dLights);
+  dLights = nullptr;
   vrx_free(dTlasNodes);
   vrx_free(dTlasLeaves);
   vrx_free(dTlasIds);
@@ -934,6 +1018,7 @@ void World::commit()
   instances = objectList<Instance>(this, "instance", ANARI_INSTANCE);
   surfaces = objectList<Surface>(this, "surface", ANARI_SURFACE);
   volumes = objectList<Volume>(this, "volume", ANARI_VOLUME);
+  lights = objectList<Light>(this, "light", ANARI_LIGHT); // World.cpp:124
   id = get<uint32_t>("id", ANARI_UINT32, ~0u); // World.cpp:159 (zero instance id)
   sceneDirty = true;
 }
@@ -1144,6 +1229,19 @@ int World::rebuild()
       wb[3 + k] = std::max(wb[3 + k], hi[k]);
     }
   }
+  // light instances (World.cpp:462-486): the world's own lights, then each
+  // instance's group lights under that instance's transform
+  std::vector<vrx_light> hl;
+  for (Light *l : lights)
+    if (l && l->isValid())
+      hl.push_back(l->record(nullptr));
+  for (Instance *in : instances) {
+    if (!in || !in->group)
+      continue;
+    for (Light *l : in->group->lights)
+      if (l && l->isValid())
+        hl.push_back(l->record(in->xfm));
+  }
   freeScene();
   auto upload = [&](void *&dst, const void *src, size_t bytes) -> bool {
     if (!bytes)
@@ -1154,7 +1252,8 @@ int World::rebuild()
   };
   if (!upload(dGeoms, hg.data(), hg.size() * sizeof(vrx_geom))
       || !upload(dEntries, he.data(), he.size() * sizeof(vrx_entry))
-      || !upload(dVolumes, hv.data(), hv.size() * sizeof(vrx_volume)))
+      || !upload(dVolumes, hv.data(), hv.size() * sizeof(vrx_volume))
+      || !upload(dLights, hl.data(), hl.size() * sizeof(vrx_light)))
     return -1;
   memset(&scene, 0, sizeof(scene));
   scene.d_geoms = (const vrx_geom *)dGeoms;
@@ -1163,6 +1262,8 @@ int World::rebuild()
   scene.num_entries = (uint32_t)he.size();
   scene.d_volumes = (const vrx_volume *)dVolumes;
   scene.num_volumes = (uint32_t)hv.size();
+  scene.d_lights = (const vrx_light *)dLights;
+  scene.num_lights = (uint32_t)hl.size();
   scene.max_blas_height = maxH;
   scene.single_type = single >= 0 ? single : -1;
   scene.max_volume_voxels = maxVox;
@@ -1306,14 +1407,18 @@ void Camera::commit()
 }
 
 // ---------------------------------------------------------------------------
-// Renderer (renderer/Renderer.cpp:97-207, Raycast.cpp:45-50, AmbientOcclusion.cpp:44-56)
+// Renderer (renderer/Renderer.cpp:97-207, Raycast.cpp:45-50, AmbientOcclusion.cpp:44-56,
+// DirectLight.cpp:46-64)
 // ---------------------------------------------------------------------------
 Renderer::Renderer(Device *d, const std::string &sub) : Object(d, ANARI_RENDERER, sub)
 {
+  // Renderer.cpp:114-121: "default" is the directLight renderer
   if (sub == "raycast")
     kind = VRX_RENDER_RAYCAST;
-  else if (sub == "ao" || sub == "default")
+  else if (sub == "ao")
     kind = VRX_RENDER_AO;
+  else if (sub == "directLight" || sub == "default")
+    kind = VRX_RENDER_DIRECT;
 }
 
 void Renderer::commit()
@@ -1326,7 +1431,12 @@ void Renderer::commit()
   float ac[3] = {1, 1, 1};
   getRaw("ambientColor", ANARI_FLOAT32_VEC3, ac);
   memcpy(r.ambient_color, ac, 12);
-  r.ambient_radiance = get<float>("ambientRadiance", ANARI_FLOAT32, 1.0f);
+  // default ambient radiance: 1 for ao, 0 for directLight and raycast
+  // (AmbientOcclusion.cpp:44, DirectLight.cpp:46, Renderer.h:56)
+  r.ambient_radiance =
+      get<float>("ambientRadiance", ANARI_FLOAT32, kind == VRX_RENDER_AO ? 1.0f : 0.0f);
+  // directLight's lightFalloff is clamped to [0, 1] and not used in shading
+  lightFalloff = std::clamp(get<float>("lightFalloff", ANARI_FLOAT32, 1.0f), 0.0f, 1.0f);
   r.pixel_samples = get<int>("pixelSamples", ANARI_INT32, 1);
   r.ambient_samples = std::clamp(get<int>("ambientSamples", ANARI_INT32, 1), 0, 256);
   r.ao_distance = get<float>("ambientOcclusionDistance", ANARI_FLOAT32, 1e20f);
@@ -1345,7 +1455,7 @@ void Renderer::commit()
     sampleLimit = 1; // single-shot renderer
   if (kind < 0)
     report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
-        "renderer subtype '%s' is not supported (raycast, ao)", subtype.c_str());
+        "renderer subtype '%s' is not supported (raycast, ao, directLight)", subtype.c_str());
 }
 
 bool Renderer::getProperty(
@@ -1782,6 +1892,7 @@ static int commitOrder(ANARIDataType t)
     return 1;
   case ANARI_SURFACE:
   case ANARI_VOLUME:
+  case ANARI_LIGHT:
     return 2;
   case ANARI_GROUP:
     return 3;

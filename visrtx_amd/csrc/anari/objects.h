@@ -196,11 +196,28 @@ struct Volume : Object {
   void *dLut{nullptr};
 };
 
+// directional / point / spot lights (scene/light/*.cpp); the light record
+// in world space is made by the world from these and the instance transform
+struct Light : Object {
+  Light(Device *d, const std::string &sub) : Object(d, ANARI_LIGHT, sub) {}
+  void commit() override;
+  bool isValid() const override { return type >= 0; }
+  int type{-1};
+  float color[3]{1, 1, 1};
+  float direction[3]{0, 0, -1};
+  float position[3]{0, 0, 0};
+  float irradiance{1.0f}, intensity{1.0f};
+  float cosOuter{-1.0f}, cosInner{-1.0f};
+  // the world-space record under the 3x4 column-major transform xfm (or identity)
+  vrx_light record(const float *xfm) const;
+};
+
 struct Group : Object {
   Group(Device *d) : Object(d, ANARI_GROUP, "") {}
   void commit() override;
   std::vector<Surface *> surfaces;
   std::vector<Volume *> volumes;
+  std::vector<Light *> lights;
 };
 
 struct Instance : Object {
@@ -220,13 +237,14 @@ struct World : Object {
   std::vector<Instance *> instances;
   std::vector<Surface *> surfaces;
   std::vector<Volume *> volumes;
+  std::vector<Light *> lights;
   uint32_t id{~0u};
 
   // device scene
   bool sceneDirty{true};
   uint64_t builtAt{0};
   vrx_scene scene{};
-  void *dGeoms{nullptr}, *dEntries{nullptr}, *dVolumes{nullptr};
+  void *dGeoms{nullptr}, *dEntries{nullptr}, *dVolumes{nullptr}, *dLights{nullptr};
   void *dTlasNodes{nullptr}, *dTlasLeaves{nullptr}, *dTlasIds{nullptr}, *dTlasHeader{nullptr};
   void *dTlasWide{nullptr};
   float bounds[6]{0, 0, 0, 0, 0, 0};
@@ -256,9 +274,13 @@ struct Renderer : Object {
   int kind{-1};
   vrx_renderer r{};
   int sampleLimit{128};
+  float lightFalloff{1.0f};
   // launch parameters left to the frame's autotuner (not set explicitly)
   bool tuneWidth{true}, tuneStack{true};
-  int spp() const { return r.kind == VRX_RENDER_AO ? (r.pixel_samples > 1 ? r.pixel_samples : 1) : 1; }
+  int spp() const
+  {
+    return r.kind == VRX_RENDER_RAYCAST ? 1 : (r.pixel_samples > 1 ? r.pixel_samples : 1);
+  }
 };
 
 struct Frame : Object {

@@ -1,9 +1,10 @@
-// render.hip -- raycast and ambient-occlusion renderers, volume marching,
+// render.hip -- raycast, ambient-occlusion and directLight renderers, volume marching,
 // frame clears, camera setup and the instance-level BVH build.
 //
 // Per-pixel programs restated from the reference raygen programs:
 //   renderer/Raycast_ptx.cu:60-179          (raycast: first-hit ids, facing-ratio shading)
 //   renderer/AmbientOcclusion_ptx.cu:86-220 (spp loop, AO any-hit rays, gpu/computeAO.h:38-60)
+//   renderer/DirectLight_ptx.cu:62-420      (lights, shadow rays, gpu/sampleLight.h:58-100)
 //   gpu/volumeIntegration.h:46-165,317-350  (emission-absorption march, all volumes)
 //   gpu/gpu_util.h:323-443                  (tonemap accumulation, channel writes)
 //   gpu/cameraCreateRay.h:38-81             (pinhole rays; raycast uses the pixel corner)
@@ -216,6 +217,15 @@ static void launch_kw(const vrx_scene &sc, const vrx_camera &cam, const vrx_rend
 {
   dim3 grid((f.width + kTile - 1) / kTile, (f.height + kTile - 1) / kTile);
   size_t lds = (size_t)sd * kBlock * 4;
+  if constexpr (PT < 0) {
+    // directLight runs in the generic kernel only (lights, any primitive
+    // type, volume transmission of shadow rays)
+    if (r.kind == VRX_RENDER_DIRECT) {
+      hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_DIRECT, WIDE>), grid, dim3(kBlock), lds, s,
+          sc, cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
+      return;
+    }
+  }
   if (r.kind == VRX_RENDER_AO)
     hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_AO, WIDE>), grid, dim3(kBlock), lds, s, sc,
         cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
@@ -283,6 +293,8 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
   VRX_ARG(scene->num_entries == 0 || (scene->d_entries && scene->d_geoms), "scene arrays");
   VRX_ARG(scene->num_entries <= 1 || scene->tlas.d_leaf_ids, "instance BVH not built");
   VRX_ARG(scene->num_volumes == 0 || scene->d_volumes, "volume array");
+  VRX_ARG(r->kind >= VRX_RENDER_RAYCAST && r->kind <= VRX_RENDER_DIRECT, "renderer kind");
+  VRX_ARG(scene->num_lights == 0 || scene->d_lights, "light array");
   VRX_ARG(r->node_width == 0 || r->node_width == 2 || r->node_width == 4, "node width 2 or 4");
   VRX_ARG(!wide_of(r) || scene->num_entries == 0 || scene->wide,
       "node width 4 needs 4-wide nodes for every BVH (vrx_bvh_widen, scene->wide)");
@@ -316,7 +328,7 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
   }
   // the primitive-specialised kernels carry no volume code: scenes with
   // volumes take the generic one
-  switch (scene->num_volumes ? -1 : scene->single_type) {
+  switch (scene->num_volumes || r->kind == VRX_RENDER_DIRECT ? -1 : scene->single_type) {
   case VRX_TRIANGLE:
     return launch_pt<VRX_TRIANGLE>(*scene, *cam, *r, *f, sd, cap, st);
   case VRX_SPHERE:

@@ -358,22 +358,171 @@ __device__ __forceinline__ void wave_count_add(uint64_t *ctr, uint32_t v)
 }
 
 // --------------------------------------------------------------------------
+// shadow rays, ambient occlusion, lights
+// --------------------------------------------------------------------------
+// surfaceAttenuation (intersectRay.h:105-111) through the any-hit in trace():
+// the opacity accumulated by the surfaces on [0, tmax].  A ray whose opacity
+// reached 0.99 counts as fully blocked: which hits the reference visits before
+// it stops depends on its traversal order, so its value then lies anywhere in
+// [0.99, 1]; below 0.99 every hit is visited and the value is the same.
+template <int PT, bool TWO, bool WIDE>
+__device__ __forceinline__ float surface_atten(
+    const vrx_scene &sc, v3 o, v3 d, float tmax, Stack &st, uint32_t &rays)
+{
+  Hit hh;
+  rays++;
+  const bool term = TWO ? trace<PT, true, true, WIDE>(sc, o, d, 0.0f, tmax, false, st, hh)
+                        : trace<PT, false, true, WIDE>(sc, o, d, 0.0f, tmax, false, st, hh);
+  return term ? 1.0f : hh.a0;
+}
+
+// computeAO (gpu/computeAO.h:38-60): n cosine-weighted hemisphere rays from
+// the geometric-normal offset point, four PCG draws each.
+template <int PT, bool TWO, bool WIDE>
+__device__ float ao_factor(const vrx_scene &sc, const SurfHit &s, float dist, int n, Pcg &rs,
+    Stack &st, uint32_t &rays)
+{
+  float weights = 0.0f, hits = 0.0f;
+  for (int k = 0; k < n; k++) {
+    v3 ao_o = add(s.hp, fmul(s.eps, s.ng));
+    float q0 = pcg_float(rs), q1 = pcg_float(rs), q2 = pcg_float(rs);
+    (void)pcg_float(rs);
+    v3 dir = normalize(mk(2.0f * q0 - 1.0f, 2.0f * q1 - 1.0f, 2.0f * q2 - 1.0f));
+    if (!(dot(dir, s.ns) > 0.0f))
+      dir = neg(dir);
+    float wgt = fmaxf(0.0f, dot(dir, s.ns));
+    weights += wgt;
+    if (wgt != 0.0f)
+      hits += wgt * surface_atten<PT, TWO, WIDE>(sc, ao_o, dir, dist, st, rays);
+  }
+  return weights > 0.0f ? 1.0f - hits / weights : 0.0f;
+}
+
+// volumeAttenuation (DirectLight_ptx.cu:62-69 with the shadow any-hit at
+// :270-279): every volume the shadow ray crosses on [0, tmax] is marched into
+// one opacity (rayMarchVolume without colour, volumeIntegration.h:64-103),
+// volumes in index order, stopping at 0.99.
+__device__ float volume_atten(
+    const vrx_scene &sc, v3 o, v3 d, float tmax, float inv_rate, uint32_t &ns)
+{
+  float opacity = 0.0f;
+  for (uint32_t i = 0; i < sc.num_volumes && opacity < 0.99f; i++) {
+    const vrx_volume &V = sc.d_volumes[i];
+    float t0, t1;
+    if (!vol_slab(V, o, d, 0.0f, tmax, t0, t1))
+      continue;
+    v3 unused = mk(0.0f, 0.0f, 0.0f);
+    const VolCtx c = vol_ctx(V, inv_rate);
+    if (vol_is_large(V))
+      march<true, 2, true>(c, (const float4 *)V.d_lut, o, d, t0, t1, unused, opacity, ns);
+    else
+      march<false, 2, true>(c, (const float4 *)V.d_lut, o, d, t0, t1, unused, opacity, ns);
+  }
+  return opacity;
+}
+
+// sampleLight (gpu/sampleLight.h:58-100) for a light already in world space.
+struct LightSample {
+  v3 radiance, dir;
+  float dist;
+};
+
+__device__ __forceinline__ LightSample sample_light(const vrx_light *L, v3 hp)
+{
+  LightSample ls;
+  const int type = gld(&L->type);
+  const v3 rad = mk(gld(&L->radiance[0]), gld(&L->radiance[1]), gld(&L->radiance[2]));
+  if (type == VRX_LIGHT_DIRECTIONAL) {
+    ls.dir = mk(gld(&L->dir[0]), gld(&L->dir[1]), gld(&L->dir[2]));
+    ls.dist = INFINITY;
+    ls.radiance = rad;
+  } else {
+    const v3 pos = mk(gld(&L->pos[0]), gld(&L->pos[1]), gld(&L->pos[2]));
+    if (type == VRX_LIGHT_POINT) {
+      const v3 v = sub(pos, hp);
+      ls.dist = sqrtf(dot(v, v));
+      ls.dir = normalize(v);
+      ls.radiance = rad;
+    } else {
+      // spot: the reference measures the distance after normalising the
+      // direction, so its shadow rays end about one unit from the surface
+      ls.dir = normalize(sub(pos, hp));
+      ls.dist = sqrtf(dot(ls.dir, ls.dir));
+      const v3 sd = normalize(mk(gld(&L->dir[0]), gld(&L->dir[1]), gld(&L->dir[2])));
+      const float co = gld(&L->cos_outer), ci = gld(&L->cos_inner);
+      float spot = dot(sd, neg(ls.dir));
+      if (spot < co)
+        spot = 0.0f;
+      else if (spot > ci)
+        spot = 1.0f;
+      else {
+        spot = (spot - co) / (ci - co);
+        spot = spot * spot * (3.0f - 2.0f * spot);
+      }
+      ls.radiance = mulf(rad, spot);
+    }
+  }
+  return ls;
+}
+
+// shadeSurface of the directLight renderer (DirectLight_ptx.cu:71-225) for
+// matte surfaces: AO factor, ambient term times the tint, and per light the
+// Lambertian term (MatteShader_ptx.cu:71-79: base/pi * max(0, Ns.L) *
+// radiance / pdf, pdf = 1) times the surface and volume transmission of the
+// shadow ray from the shading point.  Matte surfaces spawn no bounce ray
+// (their nextRay is zero), so the bounce loop ends at once.  Returns the
+// colour and the material opacity.
+template <int PT, bool TWO, bool WIDE>
+__device__ float4 shade_direct(const vrx_scene &sc, const vrx_renderer &r, const SurfHit &s,
+    int aos, Pcg &rs, Stack &st, uint32_t &rays, uint32_t &vsamples)
+{
+  const v3 sp = add(s.hp, fmul(s.eps, s.ns));
+  const float aof =
+      aos > 0 ? ao_factor<PT, TWO, WIDE>(sc, s, r.ao_distance, aos, rs, st, rays) : 1.0f;
+  v3 contrib = mk(0.0f, 0.0f, 0.0f);
+  if (r.ambient_radiance > 0.0f)
+    contrib = mulv(mulf(ld3(r.ambient_color), r.ambient_radiance), s.color);
+  const v3 kd = mulf(s.color, 0.318309886183790672f);
+  for (uint32_t l = 0; l < sc.num_lights; l++) {
+    const LightSample ls = sample_light(sc.d_lights + l, s.hp);
+    const float ndl = fmaxf(0.0f, dot(s.ns, ls.dir));
+    const v3 c = mulv(mulf(kd, ndl), ls.radiance);
+    if (c.x != c.x || c.y != c.y || c.z != c.z)
+      continue;
+    // a light at grazing or back-facing incidence adds exactly zero whatever
+    // its shadow ray finds, so that ray is not traced
+    if (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)
+      continue;
+    const float so = 1.0f - surface_atten<PT, TWO, WIDE>(sc, sp, ls.dir, ls.dist, st, rays);
+    float vo = 1.0f;
+    if (PT < 0 && sc.num_volumes)
+      vo = 1.0f - volume_atten(sc, sp, ls.dir, ls.dist, r.inv_volume_sampling_rate, vsamples);
+    const float att = so * vo;
+    contrib = add(contrib, mulf(c, att));
+  }
+  contrib = mulf(contrib, aof);
+  return make_float4(contrib.x, contrib.y, contrib.z, s.opacity);
+}
+
+// --------------------------------------------------------------------------
 // the per-pixel program
 // --------------------------------------------------------------------------
-// One pixel of the raycast / AO program (Raycast_ptx.cu:60-179,
-// AmbientOcclusion_ptx.cu:86-220): all pixel samples, the surface/volume
-// layer loop, accumulation and channel writes at output index idx.
+// One pixel of the raycast / AO / directLight program (Raycast_ptx.cu:60-179,
+// AmbientOcclusion_ptx.cu:86-220, DirectLight_ptx.cu:311-418): all pixel
+// samples, the surface/volume layer loop, accumulation and channel writes at
+// output index idx.
 template <int PT, bool TWO, int KIND, bool WIDE = false>
 __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_camera &cam,
     const vrx_renderer &r, const vrx_frame &f, uint32_t px, uint32_t py, size_t idx, Stack &st,
     uint32_t &rays, uint32_t &vsamples)
 {
   const float iw = 1.0f / (float)f.width, ih = 1.0f / (float)f.height;
-  const int spp = KIND == VRX_RENDER_AO ? max(r.pixel_samples, 1) : 1;
+  constexpr bool kSampled = KIND == VRX_RENDER_AO || KIND == VRX_RENDER_DIRECT;
+  const int spp = kSampled ? max(r.pixel_samples, 1) : 1;
   const int aos = min(max(r.ambient_samples, 0), 256);
   const bool cull = r.cull_backfaces != 0;
   Pcg rs;
-  if (KIND == VRX_RENDER_AO)
+  if (kSampled)
     pcg_seed(rs, (uint64_t)idx, (uint64_t)f.frame_id);
 
   float4 accum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -387,7 +536,7 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
   }
   for (int i = 0; i < spp; i++) {
     v3 o, d;
-    if (KIND == VRX_RENDER_AO) {
+    if (kSampled) {
       float r0 = pcg_float(rs), r1 = pcg_float(rs);
       (void)pcg_float(rs);
       (void)pcg_float(rs);
@@ -430,40 +579,37 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
           }
           first = false;
         }
-        v3 lighting;
-        if (KIND == VRX_RENDER_AO) {
-          float ao = 1.0f;
-          if (aos > 0) {
-            float weights = 0.0f, hits = 0.0f;
-            for (int k = 0; k < aos; k++) {
-              v3 ao_o = add(s.hp, fmul(s.eps, s.ng));
-              float q0 = pcg_float(rs), q1 = pcg_float(rs), q2 = pcg_float(rs);
-              (void)pcg_float(rs);
-              v3 dir = normalize(mk(2.0f * q0 - 1.0f, 2.0f * q1 - 1.0f, 2.0f * q2 - 1.0f));
-              if (!(dot(dir, s.ns) > 0.0f))
-                dir = neg(dir);
-              float wgt = fmaxf(0.0f, dot(dir, s.ns));
-              weights += wgt;
-              if (wgt != 0.0f) {
-                Hit hh;
-                rays++;
-                bool occ = TWO ? trace<PT, true, true, WIDE>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh)
-                               : trace<PT, false, true, WIDE>(sc, ao_o, dir, 0.0f, r.ao_distance, false, st, hh);
-                hits += wgt * (occ ? 1.0f : 0.0f);
-              }
-            }
-            ao = weights > 0.0f ? 1.0f - hits / weights : 0.0f;
+        if (KIND == VRX_RENDER_DIRECT) {
+          const float4 sr = shade_direct<PT, TWO, WIDE>(sc, r, s, aos, rs, st, rays, vsamples);
+          // DirectLight_ptx.cu:372-381: the shading result replaces the
+          // colour and opacity marched through volumes in front of the
+          // surface, and is then accumulated onto itself
+          if (sr.x != sr.x || sr.y != sr.y || sr.z != sr.z) {
+            color = mk(0.0f, 0.0f, 0.0f);
+            opacity = 0.0f;
+          } else {
+            color = mk(sr.x, sr.y, sr.z);
+            opacity = sr.w;
           }
-          float scl = ao * r.ambient_radiance;
-          lighting = fmul(scl, ld3(r.ambient_color));
+          float w = 1.0f - opacity;
+          color = mk(color.x + sr.x * w, color.y + sr.y * w, color.z + sr.z * w);
+          opacity = opacity + sr.w * (1.0f - opacity);
         } else {
-          float scl = fabsf(dot(d, s.ns));
-          lighting = fmul(scl, ld3(r.ambient_color));
+          v3 lighting;
+          if (KIND == VRX_RENDER_AO) {
+            const float ao =
+                aos > 0 ? ao_factor<PT, TWO, WIDE>(sc, s, r.ao_distance, aos, rs, st, rays) : 1.0f;
+            float scl = ao * r.ambient_radiance;
+            lighting = fmul(scl, ld3(r.ambient_color));
+          } else {
+            float scl = fabsf(dot(d, s.ns));
+            lighting = fmul(scl, ld3(r.ambient_color));
+          }
+          v3 bl = mulv(s.color, lighting);
+          float w = 1.0f - opacity;
+          color = mk(color.x + bl.x * w, color.y + bl.y * w, color.z + bl.z * w);
+          opacity = opacity + s.opacity * (1.0f - opacity);
         }
-        v3 bl = mulv(s.color, lighting);
-        float w = 1.0f - opacity;
-        color = mk(color.x + bl.x * w, color.y + bl.y * w, color.z + bl.z * w);
-        opacity = opacity + s.opacity * (1.0f - opacity);
         color = mulf(color, opacity);
         float wo = 1.0f - outo;
         outc = mk(outc.x + color.x * wo, outc.y + color.y * wo, outc.z + color.z * wo);

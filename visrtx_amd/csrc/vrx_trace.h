@@ -247,6 +247,7 @@ struct BlasCtx {
   bool cull;
   bool caps;
   uint32_t entry;
+  float opacity; // surface opacity (any-hit rays only)
 };
 
 __device__ __forceinline__ void load_blas(
@@ -264,6 +265,7 @@ __device__ __forceinline__ void load_blas(
   b.cull = !any && (cull_all || gld(&G->cull_backfaces) != 0);
   b.caps = gld(&G->caps) != 0;
   b.entry = e;
+  b.opacity = any ? gld(&G->opacity) : 1.0f;
   n = gld(&G->n);
   root = n == 1 ? kLeafBit : 0u;
   if (gld(&Ep->identity)) {
@@ -399,6 +401,17 @@ __device__ __forceinline__ bool take(Hit &h, float t, float tmax, uint32_t entry
   return entry < h.entry || (entry == h.entry && prim < h.prim);
 }
 
+// Any-hit of shadow and AO rays (DirectLight_ptx.cu:255-280,
+// AmbientOcclusion_ptx.cu:53-69): every reported intersection in the ray
+// interval adds its surface's opacity, o += a (1 - o), and the ray ends once
+// o reaches 0.99.  h.a0 carries o (trace() starts it at 0); an opaque surface
+// ends the ray at its first hit.
+__device__ __forceinline__ bool any_hit(const BlasCtx &b, Hit &h)
+{
+  h.a0 = h.a0 + b.opacity * (1.0f - h.a0);
+  return !(h.a0 < 0.99f);
+}
+
 // Test leaf k of the current BLAS.  Returns true for an any-hit termination.
 template <int PT, bool ANY>
 __device__ __forceinline__ bool test_leaf(
@@ -410,7 +423,7 @@ __device__ __forceinline__ bool test_leaf(
     float t;
     if (isect_sphere(o, d, mk(L.x, L.y, L.z), L.w, t) && t >= tmin && t <= tmax) {
       if (ANY)
-        return true;
+        return any_hit(b, h);
       uint32_t prim = gld(b.ids, k);
       if (take(h, t, tmax, b.entry, prim)) {
         h.found = true;
@@ -432,7 +445,7 @@ __device__ __forceinline__ bool test_leaf(
           return false;
       }
       if (ANY)
-        return true;
+        return any_hit(b, h);
       uint32_t prim = gld(b.ids, k);
       if (take(h, t, tmax, b.entry, prim)) {
         h.found = true;
@@ -457,8 +470,11 @@ __device__ __forceinline__ bool test_leaf(
         continue;
       float t = ts[q];
       if (t >= tmin && t <= tmax) {
-        if (ANY)
-          return true;
+        if (ANY) {
+          if (any_hit(b, h))
+            return true;
+          continue;
+        }
         uint32_t prim = gld(b.ids, k);
         if (take(h, t, tmax, b.entry, prim)) {
           h.found = true;
@@ -480,7 +496,7 @@ __device__ __forceinline__ bool test_leaf(
     if (isect_round_cone(o, d, mk(P0.x, P0.y, P0.z), mk(P1.x, P1.y, P1.z), P0.w, P1.w, t, nn)
         && t >= tmin && t <= tmax) {
       if (ANY)
-        return true;
+        return any_hit(b, h);
       uint32_t prim = gld(b.ids, k);
       if (take(h, t, tmax, b.entry, prim)) {
         h.found = true;
@@ -505,8 +521,11 @@ __device__ __forceinline__ bool test_leaf(
         continue;
       float t = ts[q];
       if (t >= tmin && t <= tmax) {
-        if (ANY)
-          return true;
+        if (ANY) {
+          if (any_hit(b, h))
+            return true;
+          continue;
+        }
         uint32_t prim = gld(b.ids, k);
         if (take(h, t, tmax, b.entry, prim)) {
           h.found = true;

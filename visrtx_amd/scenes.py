@@ -51,10 +51,24 @@ class GeomDesc:
 
 
 @dataclass
+class LightDesc:
+    """A directional, point or spot light (scene/light/*.cpp parameters)."""
+    type: str                       # "directional" | "point" | "spot"
+    color: tuple = (1.0, 1.0, 1.0)
+    direction: tuple = (0.0, 0.0, -1.0)
+    position: tuple = (0.0, 0.0, 0.0)
+    irradiance: float = 1.0         # directional
+    intensity: float = 1.0          # point / spot
+    opening_angle: float = math.pi  # spot
+    falloff_angle: float = 0.1      # spot
+
+
+@dataclass
 class InstanceDesc:
     geoms: list
     xfm: np.ndarray | None = None   # 12 floats, column-major 3x4
     id: int = 0xFFFFFFFF
+    lights: list = field(default_factory=list)   # LightDesc in the instance's group
 
 
 @dataclass
@@ -85,6 +99,7 @@ class SceneDesc:
     instances: list = field(default_factory=list)
     volumes: list = field(default_factory=list)
     world_id: int = 0xFFFFFFFF
+    lights: list = field(default_factory=list)   # world-level LightDesc
 
 
 def look_at(pos, target):
@@ -331,6 +346,23 @@ def _volume(dev: A.Device, s: AnariScene, v: VolumeDesc):
     return vol
 
 
+def _light(dev: A.Device, s: AnariScene, L: LightDesc):
+    h = s.keep(dev.new("Light", L.type))
+    dev.set(h, "color", L.color, A.FLOAT32_VEC3)
+    if L.type == "directional":
+        dev.set(h, "direction", L.direction, A.FLOAT32_VEC3)
+        dev.set(h, "irradiance", L.irradiance, A.FLOAT32)
+    else:
+        dev.set(h, "position", L.position, A.FLOAT32_VEC3)
+        dev.set(h, "intensity", L.intensity, A.FLOAT32)
+        if L.type == "spot":
+            dev.set(h, "direction", L.direction, A.FLOAT32_VEC3)
+            dev.set(h, "openingAngle", L.opening_angle, A.FLOAT32)
+            dev.set(h, "falloffAngle", L.falloff_angle, A.FLOAT32)
+    dev.commit(h)
+    return h
+
+
 def build_anari(dev: A.Device, sc: SceneDesc) -> AnariScene:
     s = AnariScene(dev)
     surfs = [None] * len(sc.geoms)
@@ -351,8 +383,12 @@ def build_anari(dev: A.Device, sc: SceneDesc) -> AnariScene:
         insts = []
         for inst in sc.instances:
             grp = s.keep(dev.new("Group"))
-            arr = s.keep(dev.array([surf_of(i) for i in inst.geoms], A.SURFACE))
-            dev.set(grp, "surface", arr, A.ARRAY1D)
+            if inst.geoms:
+                arr = s.keep(dev.array([surf_of(i) for i in inst.geoms], A.SURFACE))
+                dev.set(grp, "surface", arr, A.ARRAY1D)
+            if inst.lights:
+                la = s.keep(dev.array([_light(dev, s, L) for L in inst.lights], A.LIGHT))
+                dev.set(grp, "light", la, A.ARRAY1D)
             dev.commit(grp)
             ih = s.keep(dev.new("Instance", "transform"))
             dev.set(ih, "group", grp, A.GROUP)
@@ -363,6 +399,9 @@ def build_anari(dev: A.Device, sc: SceneDesc) -> AnariScene:
             dev.commit(ih)
             insts.append(ih)
         dev.set(world, "instance", s.keep(dev.array(insts, A.INSTANCE)), A.ARRAY1D)
+    if sc.lights:
+        la = s.keep(dev.array([_light(dev, s, L) for L in sc.lights], A.LIGHT))
+        dev.set(world, "light", la, A.ARRAY1D)
     if sc.world_id != 0xFFFFFFFF:
         dev.set(world, "id", sc.world_id, A.UINT32)
     dev.commit(world)
@@ -387,7 +426,7 @@ def make_renderer(dev: A.Device, s: AnariScene, kind: str = "raycast", **params)
              "ambientRadiance": A.FLOAT32, "pixelSamples": A.INT32, "ambientSamples": A.INT32,
              "ambientOcclusionDistance": A.FLOAT32, "cullTriangleBackfaces": A.BOOL,
              "volumeSamplingRate": A.FLOAT32, "sampleLimit": A.INT32, "stackDepthLDS": A.INT32,
-             "volumeUnroll": A.INT32, "nodeWidth": A.INT32}
+             "volumeUnroll": A.INT32, "nodeWidth": A.INT32, "lightFalloff": A.FLOAT32}
     for k, v in params.items():
         dev.set(r, k, v, types[k])
     dev.commit(r)
