@@ -382,7 +382,7 @@ def cpu_baseline(threads):
                       f"{build_s * 1000:.0f} ms"}
 
 
-SECTIONS = ("headline", "ao", "heightfield", "volume", "batched", "diffvol")
+SECTIONS = ("headline", "ao", "direct", "heightfield", "volume", "batched", "diffvol")
 
 
 def main():
@@ -417,7 +417,7 @@ def main():
     W, H = 1920, 1080
     value = sec_per_step = roofline = None
     rays = W * H
-    if "headline" in sec or "ao" in sec:
+    if "headline" in sec or "ao" in sec or "direct" in sec:
         # ---- headline: 10M spheres, 1920x1080 raycast ---------------------
         log("generating 10M spheres")
         sc, cam = S.spheres_workload(width=W, height=H)
@@ -484,7 +484,25 @@ def main():
         log(f"AO 16spp, {ow}-wide nodes: {sps2 * 1000:.2f} ms/frame -> "
             f"{extra[f'ao16_mrays_s_width{ow}']} Mrays/s")
         ao.release()
-    if "headline" in sec or "ao" in sec:
+    if "direct" in sec:
+        # directLight renderer on the same spheres: a directional and a point
+        # light (one shadow ray each per hit), one AO ray, 4 spp
+        sc.lights = [S.LightDesc("directional", direction=(0.3, -1.0, 0.5), irradiance=2.0),
+                     S.LightDesc("point", position=(0.5, 1.5, -0.5), intensity=1.0)]
+        dl = Workload(dev, sc, cam, W, H, "directLight", ids=True, pixelSamples=4,
+                      ambientSamples=1, ambientOcclusionDistance=0.02, ambientRadiance=0.2)
+        sps, _ = dl.timed(3, 1)
+        nr = dl.rays()
+        extra["direct_light_4spp"] = {
+            "mrays_s": round(nr / sps / 1e6, 2), "rays_per_frame": int(nr),
+            "ms_per_frame": round(sps * 1000, 3),
+            "tuned_launch": dict(zip(("node_width", "stack_lds"), dl.tuned())),
+            "lights": "directional + point", "ambient_samples": 1}
+        log(f"directLight 4spp: {sps * 1000:.2f} ms/frame, {nr} rays -> "
+            f"{extra['direct_light_4spp']['mrays_s']} Mrays/s")
+        dl.release()
+        sc.lights = []
+    if "headline" in sec or "ao" in sec or "direct" in sec:
         work.release()
         del sc
 
