@@ -382,7 +382,8 @@ def cpu_baseline(threads):
                       f"{build_s * 1000:.0f} ms"}
 
 
-SECTIONS = ("headline", "ao", "direct", "heightfield", "volume", "batched", "diffvol")
+SECTIONS = ("headline", "ao", "direct", "instances", "heightfield", "volume", "batched",
+            "diffvol")
 
 
 def main():
@@ -506,6 +507,26 @@ def main():
         work.release()
         del sc
 
+    if "instances" in sec:
+        # one Instance with a 1M-element transform array over a sphere group,
+        # 1920x1080 primary raycast (two-level traversal)
+        sci, cami = S.instanced_spheres_workload(1_000_000, width=W, height=H)
+        t = time.perf_counter()
+        iw = Workload(dev, sci, cami, W, H, "raycast")
+        dev.get_property(iw.s.world, "bounds", A.FLOAT32_BOX3)
+        build_s = time.perf_counter() - t
+        sps, di = iw.timed(10, 2)
+        extra["instances_1m_transform_array"] = {
+            "scene_build_s": round(build_s, 3),
+            "bvh_build_ms": round(1000.0 * dev.get_property(iw.s.world, "bvhBuildDuration",
+                                                             A.FLOAT32), 3),
+            "mrays_s": round(W * H / sps / 1e6, 2),
+            "kernel_ms": round(statistics.mean(di) * 1000, 4),
+            "tuned_launch": dict(zip(("node_width", "stack_lds"), iw.tuned()))}
+        log(f"1M instances: build {build_s:.2f}s, "
+            f"{extra['instances_1m_transform_array']['mrays_s']} Mrays/s")
+        iw.release()
+        del sci
     if "heightfield" in sec:
         sc1, cam1 = S.heightfield_workload()
         hf = Workload(dev, sc1, cam1, 1024, 768, "raycast")

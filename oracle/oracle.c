@@ -323,7 +323,10 @@ int orc_bvh_build(int type, uint32_t n, const float *pos, const uint32_t *index,
   uint32_t *sc = (uint32_t *)malloc(sizeof(uint32_t) * n);
   if (!out->leaf_boxes || !out->order || !out->child || !out->node_boxes || !codes || !sc)
     return -1;
-  orc_bounds(type, n, pos, index, radius, global_radius, out->leaf_boxes);
+  if (type == ORC_BOXES)
+    memcpy(out->leaf_boxes, pos, sizeof(float) * 6 * (size_t)n);
+  else
+    orc_bounds(type, n, pos, index, radius, global_radius, out->leaf_boxes);
   orc_morton(out->leaf_boxes, n, codes, NULL);
   orc_sort(codes, n, sc, out->order);
   orc_karras(sc, n, out->child, NULL);
@@ -737,11 +740,55 @@ static inline v3 xfm_normal(const float *inv, v3 n)
 int orc_scene_build(orc_scene *s)
 {
   s->bvhs = (orc_bvh *)calloc(s->num_geoms ? s->num_geoms : 1, sizeof(orc_bvh));
+  s->tlas = NULL;
   for (uint32_t g = 0; g < s->num_geoms; g++) {
     const orc_geom *G = &s->geoms[g];
     if (orc_bvh_build(G->type, G->num_prims, G->position, G->index, G->radius,
             G->global_radius, &s->bvhs[g]))
       return -1;
+  }
+  if (s->num_entries > 1) {
+    /* An acceleration structure over the entries, for the oracle's own speed
+       on scenes of many instances: world boxes of each entry's BLAS root box
+       (8 corners transformed in FP64, padded outwards), an LBVH over them.
+       It only prunes entries; every hit is still found by the exact leaf
+       tests, and the closest-hit order key is independent of it. */
+    float *boxes = (float *)malloc(sizeof(float) * 6 * (size_t)s->num_entries);
+    if (!boxes)
+      return -1;
+    for (uint32_t e = 0; e < s->num_entries; e++) {
+      const orc_entry *E = &s->entries[e];
+      const orc_bvh *B = &s->bvhs[E->geom];
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (int c = 0; c < 8 && B->n > 0; c++) {
+        double p[3] = {B->root_box[(c & 1) ? 3 : 0], B->root_box[(c & 2) ? 4 : 1],
+            B->root_box[(c & 4) ? 5 : 2]};
+        for (int k = 0; k < 3; k++) {
+          double w = (double)E->xfm[k] * p[0] + (double)E->xfm[3 + k] * p[1] +
+              (double)E->xfm[6 + k] * p[2] + (double)E->xfm[9 + k];
+          lo[k] = w < lo[k] ? w : lo[k];
+          hi[k] = w > hi[k] ? w : hi[k];
+        }
+      }
+      float *b = boxes + 6 * (size_t)e;
+      for (int k = 0; k < 3; k++) {
+        if (B->n == 0) {
+          b[k] = b[3 + k] = 0.0f; /* empty BLAS: its traversal returns at once */
+          continue;
+        }
+        double pad = 1e-5 * (hi[k] - lo[k]) + 1e-5 * (fabs(lo[k]) + fabs(hi[k])) + 1e-30;
+        b[k] = (float)(lo[k] - pad);
+        b[3 + k] = (float)(hi[k] + pad);
+      }
+    }
+    orc_bvh *t = (orc_bvh *)calloc(1, sizeof(orc_bvh));
+    if (!t || orc_bvh_build(ORC_BOXES, s->num_entries, boxes, NULL, NULL, 0.0f, t)) {
+      free(boxes);
+      free(t);
+      return -1;
+    }
+    free(boxes);
+    s->tlas = t;
   }
   return 0;
 }
@@ -754,6 +801,11 @@ void orc_scene_free(orc_scene *s)
     orc_bvh_free(&s->bvhs[g]);
   free(s->bvhs);
   s->bvhs = NULL;
+  if (s->tlas) {
+    orc_bvh_free((orc_bvh *)s->tlas);
+    free(s->tlas);
+    s->tlas = NULL;
+  }
 }
 
 /* ray box slab test, shared rule with the device traversal */
@@ -1006,15 +1058,65 @@ static int trace_impl(const orc_scene *s, v3 o, v3 d, float tmin, float tmax, in
 {
   memset(best, 0, sizeof(*best));
   float tcur = tmax;
-  for (uint32_t e = 0; e < s->num_entries; e++) {
-    tctx c;
-    float inv[12];
-    int ident;
-    setup_entry(&c, s, e, o, d, inv, &ident);
-    c.cull = !any && (cull_all || c.G->cull_backfaces);
-    c.any = any;
-    if (traverse_entry(&c, tmin, &tcur, best, stats))
-      return 1;
+  const orc_bvh *T = (const orc_bvh *)s->tlas;
+  if (!T) {
+    for (uint32_t e = 0; e < s->num_entries; e++) {
+      tctx c;
+      float inv[12];
+      int ident;
+      setup_entry(&c, s, e, o, d, inv, &ident);
+      c.cull = !any && (cull_all || c.G->cull_backfaces);
+      c.any = any;
+      if (traverse_entry(&c, tmin, &tcur, best, stats))
+        return 1;
+    }
+    return any ? 0 : best->found;
+  }
+  /* entries through the oracle's own instance-level BVH (near first) */
+  const v3 winv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  uint32_t stack[256];
+  int sp = 0;
+  uint32_t cur = T->n == 1 ? LEAF_BIT : 0u;
+  float tn;
+  if (!slab(T->root_box, o, winv, tmin, tcur, &tn))
+    return 0;
+  for (;;) {
+    if (!(cur & LEAF_BIT)) {
+      const float *nb = T->node_boxes + 12 * (size_t)cur;
+      float tn0, tn1;
+      int h0 = slab(nb, o, winv, tmin, tcur, &tn0);
+      int h1 = slab(nb + 6, o, winv, tmin, tcur, &tn1);
+      uint32_t c0 = T->child[2 * (size_t)cur], c1 = T->child[2 * (size_t)cur + 1];
+      if (h0 && h1) {
+        if (tn1 < tn0) {
+          uint32_t tmp = c0;
+          c0 = c1;
+          c1 = tmp;
+        }
+        stack[sp++] = c1;
+        cur = c0;
+        continue;
+      } else if (h0) {
+        cur = c0;
+        continue;
+      } else if (h1) {
+        cur = c1;
+        continue;
+      }
+    } else {
+      uint32_t e = T->order[cur & ~LEAF_BIT];
+      tctx c;
+      float inv[12];
+      int ident;
+      setup_entry(&c, s, e, o, d, inv, &ident);
+      c.cull = !any && (cull_all || c.G->cull_backfaces);
+      c.any = any;
+      if (traverse_entry(&c, tmin, &tcur, best, stats))
+        return 1;
+    }
+    if (sp == 0)
+      break;
+    cur = stack[--sp];
   }
   return any ? 0 : best->found;
 }

@@ -44,7 +44,8 @@
 extern "C" {
 #endif
 
-enum { ORC_TRIANGLE = 0, ORC_SPHERE = 1, ORC_CYLINDER = 2, ORC_CONE = 4, ORC_CURVE = 5 };
+enum { ORC_TRIANGLE = 0, ORC_SPHERE = 1, ORC_CYLINDER = 2, ORC_BOXES = 3, ORC_CONE = 4,
+       ORC_CURVE = 5 }; /* ORC_BOXES: pos holds 6 floats (lo, hi) per primitive */
 
 /* ---- PCG32 (O'Neill 2014) ------------------------------------------------ */
 typedef struct { uint64_t state, inc; } orc_pcg;
@@ -144,6 +145,7 @@ typedef struct {
   const orc_volume *volumes;
   uint32_t num_lights;
   const orc_light *lights;
+  void *tlas;                   /* filled by orc_scene_build when num_entries > 1 */
 } orc_scene;
 
 int orc_scene_build(orc_scene *s);

@@ -46,6 +46,9 @@ class orc_volume(C.Structure):
                 ("unit_distance", C.c_float), ("id", C.c_uint32), ("inst_id", C.c_uint32)]
 
 
+_ENTRY = np.dtype([("xfm", np.float32, 12), ("id", np.uint32), ("geom", np.uint32)])
+
+
 class orc_light(C.Structure):
     _fields_ = [("type", C.c_int32), ("radiance", C.c_float * 3), ("pos", C.c_float * 3),
                 ("dir", C.c_float * 3), ("cos_outer", C.c_float), ("cos_inner", C.c_float)]
@@ -54,7 +57,7 @@ class orc_light(C.Structure):
 class orc_scene(C.Structure):
     _fields_ = [("num_geoms", C.c_uint32), ("geoms", vp), ("bvhs", vp),
                 ("num_entries", C.c_uint32), ("entries", vp), ("num_volumes", C.c_uint32),
-                ("volumes", vp), ("num_lights", C.c_uint32), ("lights", vp)]
+                ("volumes", vp), ("num_lights", C.c_uint32), ("lights", vp), ("tlas", vp)]
 
 
 class orc_camera(C.Structure):
@@ -300,18 +303,33 @@ class OracleScene:
             G.color[:] = list(g.color)
             G.opacity = g.opacity
             G.surface_id = g.surface_id
-        ents = []
-        for gi in sc.world_geoms:
-            ents.append((IDENTITY12, sc.world_id, gi))
+        # entries (orc_entry: 12 floats, id, geom) as one structured array:
+        # world-level surfaces, then each instance's transforms x its surfaces
+        parts = []
+        if sc.world_geoms:
+            e = np.zeros(len(sc.world_geoms), _ENTRY)
+            e["xfm"] = IDENTITY12
+            e["id"] = sc.world_id
+            e["geom"] = sc.world_geoms
+            parts.append(e)
         for inst in sc.instances:
-            x = IDENTITY12 if inst.xfm is None else np.asarray(inst.xfm, np.float32)
-            for gi in inst.geoms:
-                ents.append((x, inst.id, gi))
-        entries = (orc_entry * max(len(ents), 1))()
-        for i, (x, iid, gi) in enumerate(ents):
-            entries[i].xfm[:] = [float(v) for v in x]
-            entries[i].id = iid
-            entries[i].geom = gi
+            if not inst.geoms:
+                continue
+            if inst.xfm is not None and np.ndim(inst.xfm) == 2:
+                x = np.asarray(inst.xfm, np.float32).reshape(-1, 12)
+                ids = (np.full(len(x), inst.id, np.uint32) if inst.ids is None
+                       else np.asarray(inst.ids, np.uint32))
+            else:
+                x = (IDENTITY12 if inst.xfm is None else np.asarray(inst.xfm, np.float32))[None]
+                ids = np.array([inst.id], np.uint32)
+            e = np.zeros((len(x), len(inst.geoms)), _ENTRY)
+            e["xfm"] = x[:, None, :]
+            e["id"] = ids[:, None]
+            e["geom"] = np.asarray(inst.geoms, np.uint32)[None, :]
+            parts.append(e.reshape(-1))
+        ents = np.concatenate(parts) if parts else np.zeros(1, _ENTRY)
+        n_ent = sum(len(p) for p in parts)
+        entries = ents
         vols = (orc_volume * max(len(sc.volumes), 1))()
         for i, v in enumerate(sc.volumes):
             V = vols[i]
@@ -330,15 +348,16 @@ class OracleScene:
             V.inst_id = sc.world_id
         recs = [light_record(L) for L in getattr(sc, "lights", [])]
         for inst in sc.instances:
-            recs += [light_record(L, inst.xfm) for L in getattr(inst, "lights", [])]
+            for xfm, _ in inst.transforms():
+                recs += [light_record(L, xfm) for L in getattr(inst, "lights", [])]
         lights = (orc_light * max(len(recs), 1))(*recs)
         self._lights = lights
         self._geoms, self._entries, self._vols = geoms, entries, vols
         self.s = orc_scene()
         self.s.num_geoms = len(sc.geoms)
         self.s.geoms = C.addressof(geoms)
-        self.s.num_entries = len(ents)
-        self.s.entries = C.addressof(entries)
+        self.s.num_entries = n_ent
+        self.s.entries = entries.ctypes.data
         self.s.num_volumes = len(sc.volumes)
         self.s.volumes = C.addressof(vols)
         self.s.num_lights = len(recs)

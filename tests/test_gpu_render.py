@@ -390,3 +390,20 @@ def test_curve_index_out_of_range_is_rejected(require_gpu):
     assert "needs vertex 4" in errs, errs
     s.release()
     dev.close()
+
+
+def test_transform_array_million_instances(require_gpu):
+    """A million instances from one transform array, raycast and AO, against
+    the oracle (which walks the entries through its own instance BVH)."""
+    sc, cam = S.instanced_spheres_workload(width=320, height=240)
+    g = render_gpu(sc, cam, 320, 240)
+    os_ = orc.OracleScene(sc)
+    o = os_.render(cam, 320, 240)
+    compare(g, o)
+    hit = g["depth"] < 1e29
+    assert hit.mean() > 0.5
+    assert ((g["instanceId"][hit] - 7) % 3 == 0).all()
+    rp = dict(pixelSamples=1, ambientSamples=2, ambientOcclusionDistance=0.02)
+    g = render_gpu(sc, cam, 160, 120, kind="ao", **rp)
+    o = os_.render(cam, 160, 120, kind="ao", **rp)
+    compare(g, o)

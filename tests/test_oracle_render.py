@@ -178,6 +178,55 @@ def test_bvh_closest_hit_equals_brute_force(seed):
             assert (a.t, a.entry, a.prim) == (b.t, b.entry, b.prim)
 
 
+def _transform_array_scene(n=500, seed=3):
+    """One instance whose transform array places a small sphere mesh n times
+    (rotation, scale, translation per element) with per-element ids."""
+    rng = np.random.default_rng(seed)
+    v, idx = S.gen_sphere_mesh(80, seed=seed)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w),
+                  2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w),
+                  2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)], 1)
+    xf = np.concatenate([R * rng.uniform(0.01, 0.04, (n, 1)), rng.random((n, 3))], 1)
+    inst = S.InstanceDesc(geoms=[0], xfm=xf.astype(np.float32),
+                          ids=rng.permutation(n).astype(np.uint32) + 1000)
+    sc = S.SceneDesc(geoms=[S.GeomDesc("triangle", v, idx, surface_id=3)], instances=[inst])
+    return sc, rng
+
+
+def test_transform_array_instances_and_entry_bvh():
+    """Instance transform arrays (Instance.cpp:115-128): n entries with their
+    own ids.  The oracle's entry-level BVH gives the brute force's closest
+    hits, and frames equal those of a linear walk over the entries."""
+    sc, rng = _transform_array_scene()
+    os_ = orc.OracleScene(sc)
+    assert os_.s.num_entries == 500 and os_.s.tlas
+    for _ in range(400):
+        o = rng.uniform(-0.5, 1.5, 3).astype(np.float32)
+        d = _unit(rng.uniform(0, 1, 3) - o).astype(np.float32)
+        a = os_.trace_closest(o, d)
+        b = os_.trace_brute(o, d)
+        assert (a is None) == (b is None)
+        if a is not None:
+            assert (a.t, a.entry, a.prim) == (b.t, b.entry, b.prim)
+    cam = S.CameraDesc((0.5, 0.5, -1.0), (0, 0, 1), (0, 1, 0), math.radians(50), 1.0)
+    with_bvh = os_.render(cam, 64, 64)
+    with_bvh_ao = os_.render(cam, 64, 64, kind="ao", ambientSamples=2,
+                             ambientOcclusionDistance=0.1)
+    os_.s.tlas = None                     # linear walk (the structure stays allocated)
+    linear = os_.render(cam, 64, 64)
+    linear_ao = os_.render(cam, 64, 64, kind="ao", ambientSamples=2,
+                           ambientOcclusionDistance=0.1)
+    for a, b in ((with_bvh, linear), (with_bvh_ao, linear_ao)):
+        for ch in ("color", "depth", "primitiveId", "objectId", "instanceId"):
+            np.testing.assert_array_equal(a[ch], b[ch], err_msg=ch)
+    hit = with_bvh["depth"] < 1e29
+    ids = set(np.unique(with_bvh["instanceId"][hit]).tolist())
+    assert len(ids) > 50 and ids <= set(range(1000, 1500))
+
+
 def test_tie_rule_lowest_entry_then_primitive():
     c = np.array([[0, 0, 0]] * 8, np.float32)
     c[3] = c[7] = (0.5, 0.5, 0.5)

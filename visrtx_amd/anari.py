@@ -44,7 +44,7 @@ _ELEM = {  # element type -> (numpy dtype, components)
     FLOAT32: (np.float32, 1), FLOAT32_VEC2: (np.float32, 2), FLOAT32_VEC3: (np.float32, 3),
     FLOAT32_VEC4: (np.float32, 4), UINT32: (np.uint32, 1), UINT32_VEC2: (np.uint32, 2),
     UINT32_VEC3: (np.uint32, 3), UINT32_VEC4: (np.uint32, 4), UFIXED8: (np.uint8, 1),
-    INT32: (np.int32, 1),
+    INT32: (np.int32, 1), FLOAT32_MAT4: (np.float32, 16),
 }
 
 STATUS_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,

@@ -991,6 +991,32 @@ void Instance::commit()
     const float id3[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
     memcpy(xfm, id3, sizeof(xfm));
   }
+  xfms.clear();
+  ids.clear();
+  // transform / id arrays (Instance.cpp:40-47, 80-97, 115-128)
+  if (Array *xa = arrayParam(this, "transform")) {
+    if (xa->elem != ANARI_FLOAT32_MAT4 || !xa->host()) {
+      report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+          "'transform' array elements must be FLOAT32_MAT4; array ignored");
+    } else {
+      const float *m = (const float *)xa->host();
+      const size_t n = xa->count();
+      xfms.resize(12 * n);
+      for (size_t i = 0; i < n; i++)
+        for (int c = 0; c < 4; c++)
+          for (int r = 0; r < 3; r++)
+            xfms[12 * i + 3 * c + r] = m[16 * i + 4 * c + r];
+      if (Array *ia = arrayParam(this, "id")) {
+        if (ia->elem != ANARI_UINT32 || ia->count() < n || !ia->host())
+          report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+              "'id' array must hold one UINT32 per transform; array ignored");
+        else {
+          const uint32_t *v = (const uint32_t *)ia->host();
+          ids.assign(v, v + n);
+        }
+      }
+    }
+  }
   if (!group)
     report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT, "missing 'group' on ANARIInstance");
 }
@@ -1089,9 +1115,10 @@ int World::rebuild()
   for (Instance *in : instances) {
     if (!in || !in->group)
       continue;
-    for (Surface *sf : in->group->surfaces)
-      if (sf && sf->isValid())
-        occ.push_back({sf, in->xfm, in->id});
+    for (size_t t = 0; t < in->numTransforms(); t++)
+      for (Surface *sf : in->group->surfaces)
+        if (sf && sf->isValid())
+          occ.push_back({sf, in->xfmAt(t), in->idAt(t)});
   }
   std::vector<Volume *> vols;
   std::vector<uint32_t> volInst;
@@ -1103,17 +1130,18 @@ int World::rebuild()
   for (Instance *in : instances) {
     if (!in || !in->group)
       continue;
-    for (Volume *v : in->group->volumes) {
-      if (!v || !v->isValid())
-        continue;
-      if (!isIdentity(in->xfm)) {
-        report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
-            "instanced volumes with a non-identity transform are not supported; skipped");
-        continue;
+    for (size_t t = 0; t < in->numTransforms(); t++)
+      for (Volume *v : in->group->volumes) {
+        if (!v || !v->isValid())
+          continue;
+        if (!isIdentity(in->xfmAt(t))) {
+          report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
+              "instanced volumes with a non-identity transform are not supported; skipped");
+          continue;
+        }
+        vols.push_back(v);
+        volInst.push_back(in->idAt(t));
       }
-      vols.push_back(v);
-      volInst.push_back(in->id);
-    }
   }
 
   vrx_event e0 = nullptr, e1 = nullptr;
@@ -1238,9 +1266,10 @@ int World::rebuild()
   for (Instance *in : instances) {
     if (!in || !in->group)
       continue;
-    for (Light *l : in->group->lights)
-      if (l && l->isValid())
-        hl.push_back(l->record(in->xfm));
+    for (size_t t = 0; t < in->numTransforms(); t++)
+      for (Light *l : in->group->lights)
+        if (l && l->isValid())
+          hl.push_back(l->record(in->xfmAt(t)));
   }
   freeScene();
   auto upload = [&](void *&dst, const void *src, size_t bytes) -> bool {

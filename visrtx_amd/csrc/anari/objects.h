@@ -226,6 +226,13 @@ struct Instance : Object {
   Group *group{nullptr};
   float xfm[12]{1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
   uint32_t id{~0u};
+  // a `transform` array instances the group once per element, each with the
+  // matching element of an `id` array if one is given (Instance.cpp:115-128)
+  std::vector<float> xfms; // 12 floats (column-major 3x4) per transform
+  std::vector<uint32_t> ids;
+  size_t numTransforms() const { return xfms.empty() ? 1 : xfms.size() / 12; }
+  const float *xfmAt(size_t i) const { return xfms.empty() ? xfm : xfms.data() + 12 * i; }
+  uint32_t idAt(size_t i) const { return !xfms.empty() && !ids.empty() ? ids[i] : id; }
 };
 
 struct World : Object {

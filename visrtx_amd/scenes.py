@@ -66,9 +66,18 @@ class LightDesc:
 @dataclass
 class InstanceDesc:
     geoms: list
-    xfm: np.ndarray | None = None   # 12 floats, column-major 3x4
+    xfm: np.ndarray | None = None   # 12 floats, column-major 3x4; or (n, 12): a transform array
     id: int = 0xFFFFFFFF
     lights: list = field(default_factory=list)   # LightDesc in the instance's group
+    ids: np.ndarray | None = None   # with a transform array: one id per transform
+
+    def transforms(self):
+        """(xfm or None, id) per instanced copy (Instance.cpp:115-128)."""
+        if self.xfm is None or np.ndim(self.xfm) == 1:
+            return [(self.xfm, self.id)]
+        x = np.asarray(self.xfm, np.float32).reshape(-1, 12)
+        ids = [self.id] * len(x) if self.ids is None else [int(v) for v in self.ids]
+        return list(zip(x, ids))
 
 
 @dataclass
@@ -213,6 +222,24 @@ def orbit_poses(n: int, center, radius: float, seed: int, fovy_deg: float = 45.0
                                                               r * math.sin(phi)]))
         cams.append(CameraDesc(pos, look_at(pos, c), (0, 1, 0), math.radians(fovy_deg), aspect))
     return cams
+
+
+def instanced_spheres_workload(n: int = 1_000_000, seed: int = 21, width: int = 1920,
+                               height: int = 1080):
+    """One Instance whose 'transform' array holds n elements (Instance.cpp:
+    115-128): a unit sphere scaled to radius 2e-3..6e-3 and moved into the
+    unit cube, with ids 3k + 7 from an 'id' array."""
+    rng = np.random.default_rng(seed)
+    s = rng.uniform(2e-3, 6e-3, n).astype(np.float32)
+    xf = np.zeros((n, 12), np.float32)
+    xf[:, 0] = xf[:, 4] = xf[:, 8] = s
+    xf[:, 9:12] = rng.random((n, 3), dtype=np.float32)
+    g = GeomDesc("sphere", np.zeros((1, 3), np.float32), radius=np.ones(1, np.float32),
+                 surface_id=5)
+    inst = InstanceDesc(geoms=[0], xfm=xf, ids=np.arange(n, dtype=np.uint32) * 3 + 7)
+    sc = SceneDesc(geoms=[g], instances=[inst], world_id=1)
+    cam = CameraDesc((0.5, 0.5, -1.2), (0, 0, 1), (0, 1, 0), math.radians(45), width / height)
+    return sc, cam
 
 
 def batched_workload(views: int = 64, res: int = 512, instances: int = 256, meshes: int = 16,
@@ -392,7 +419,17 @@ def build_anari(dev: A.Device, sc: SceneDesc) -> AnariScene:
             dev.commit(grp)
             ih = s.keep(dev.new("Instance", "transform"))
             dev.set(ih, "group", grp, A.GROUP)
-            if inst.xfm is not None:
+            if inst.xfm is not None and np.ndim(inst.xfm) == 2:
+                # a transform array: column-major 4x4 per element
+                x = np.asarray(inst.xfm, np.float32).reshape(-1, 4, 3)
+                m4 = np.zeros((len(x), 4, 4), np.float32)
+                m4[:, :, :3] = x
+                m4[:, 3, 3] = 1.0
+                dev.set(ih, "transform", s.keep(dev.array(m4, A.FLOAT32_MAT4)), A.ARRAY1D)
+                if inst.ids is not None:
+                    dev.set(ih, "id", s.keep(dev.array(np.asarray(inst.ids, np.uint32), A.UINT32)),
+                            A.ARRAY1D)
+            elif inst.xfm is not None:
                 dev.set(ih, "transform", np.asarray(inst.xfm, np.float32), A.FLOAT32_MAT3x4)
             if inst.id != 0xFFFFFFFF:
                 dev.set(ih, "id", inst.id, A.UINT32)
