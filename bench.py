@@ -382,8 +382,8 @@ def cpu_baseline(threads):
                       f"{build_s * 1000:.0f} ms"}
 
 
-SECTIONS = ("headline", "ao", "direct", "instances", "heightfield", "volume", "batched",
-            "diffvol")
+SECTIONS = ("headline", "ao", "direct", "instances", "heightfield", "volume", "sparse",
+            "batched", "diffvol")
 
 
 def main():
@@ -560,6 +560,27 @@ def main():
             f"{extra['volume_gsamples_s']} Gsamples/s")
         vol.release()
         del sc3
+    if "sparse" in sec:
+        # a sparse 512^3 field (12 blobs): the march with and without skipping
+        # of empty 16^3 macro cells (identical frames)
+        log("generating sparse 512^3 volume")
+        sc4, cam4 = S.sparse_volume_workload(512, width=W, height=H)
+        res = {}
+        for skip in (False, True):
+            sv = Workload(dev, sc4, cam4, W, H, "raycast", volumeSamplingRate=1.0,
+                          emptySpaceSkipping=skip)
+            _, d4 = sv.timed(5, 1)
+            res[skip] = (statistics.mean(d4) * 1000, sv.samples())
+            sv.release()
+        full = res[False][1]
+        extra["volume_sparse_512"] = {
+            "kernel_ms_no_skip": round(res[False][0], 3),
+            "kernel_ms_skip": round(res[True][0], 3),
+            "speedup": round(res[False][0] / res[True][0], 2),
+            "samples_no_skip": int(full), "samples_skip": int(res[True][1]),
+            "effective_gsamples_s_skip": round(full / (res[True][0] / 1000) / 1e9, 2)}
+        log(f"sparse volume: {res[False][0]:.3f} ms -> {res[True][0]:.3f} ms with skipping")
+        del sc4
 
     if "batched" in sec:
         log("batched 64 views")

@@ -176,6 +176,10 @@ typedef struct vrx_volume {
   uint32_t id;
   uint32_t inst_id;
   uint32_t pad;
+  /* NULL, or the macro-cell majorant grid from vrx_volume_majorant_grid: the
+   * march skips the samples inside cells whose majorant is 0 (they add
+   * nothing), when the renderer's skip_empty is set */
+  const float *d_majorant;
 } vrx_volume;
 
 /* One light in world space, as the directLight renderer samples it
@@ -242,6 +246,7 @@ typedef struct vrx_renderer {
   int32_t block_pixels;   /* pixels per workgroup: 64/128/256 */
   int32_t vol_unroll;     /* volume samples gathered per step: 1/2/4/8 */
   int32_t node_width;     /* BVH node width traversed: 2 (default) or 4 (needs scene->wide) */
+  int32_t skip_empty;     /* 1: skip empty macro cells of volumes that carry d_majorant */
 } vrx_renderer;
 
 typedef struct vrx_frame {
@@ -269,6 +274,17 @@ int vrx_frame_clear(const vrx_frame *f, vrx_stream s);
  * DirectLight_ptx.cu:228-420).  directLight reads scene->d_lights. */
 int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer *r,
     const vrx_frame *f, vrx_stream s);
+/* Macro-cell majorant grid of a volume (scene/volume/space_skipping/
+ * UniformGrid.cu:44-258: cells of 16^3 voxels, ceil(dims/16) per axis, the
+ * largest transfer-function opacity over each cell's value range).  This
+ * grid is conservative, where the reference's is sampled: cell c's value
+ * range covers every voxel a trilinear sample inside the cell (or within one
+ * voxel of it) can read, and its majorant is the largest LUT alpha of the
+ * entries those values interpolate between, so a 0 majorant means no sample
+ * in the cell adds colour or opacity.  d_majorant holds
+ * vrx_volume_grid_cells(v) floats, x fastest; v->d_data and v->d_lut are read. */
+size_t vrx_volume_grid_cells(const vrx_volume *v);
+int vrx_volume_majorant_grid(const vrx_volume *v, float *d_majorant, vrx_stream s);
 /* Host-side camera setup (camera/Perspective.cpp:40-72). */
 int vrx_camera_perspective(const float pos[3], const float dir[3], const float up[3],
     float fovy, float aspect, const float region[4], vrx_camera *out);

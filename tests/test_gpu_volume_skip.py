@@ -1,0 +1,141 @@
+"""Macro-cell majorant grid (UniformGrid.cu:44-258: 16^3-voxel cells) and the
+empty-space skipping it drives in the volume march.
+
+The grid is checked bit for bit against a numpy restatement of its rule
+(each cell's value range over its voxels plus one on every side, then the
+largest LUT alpha over the entries that range classifies to).  Skipping is
+exact: a sample whose LUT alpha is 0 has step transmittance exactly 1 and
+adds nothing, so frames with skipping equal frames without it bit for bit,
+and both match the FP64 oracle as the other volume tests do; only the
+volumeSamples counter drops."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.devmem import DeviceBuffer
+from tests.test_gpu_render import render_gpu
+from visrtx_amd import _native as N
+from visrtx_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid_reference(data, lut, vr):
+    """numpy restatement of the conservative majorant grid."""
+    nz, ny, nx = data.shape
+    m = [(n + 15) // 16 for n in (nx, ny, nz)]
+    out = np.zeros((m[2], m[1], m[0]), np.float32)
+    f = np.float32
+    lo_v, hi_v = f(vr[0]), f(vr[1])
+    inv = f(1.0) / f(hi_v - lo_v)
+
+    def entry(v):
+        cl = min(max(f(v), lo_v), hi_v)
+        x = min(max(f(f(f(cl - lo_v) * inv) * f(256.0)) - f(0.5), f(0.0)), f(255.0))
+        return min(int(x), 254)
+
+    for cz in range(m[2]):
+        for cy in range(m[1]):
+            for cx in range(m[0]):
+                blk = data[max(16 * cz - 1, 0):min(16 * cz + 17, nz - 1) + 1,
+                           max(16 * cy - 1, 0):min(16 * cy + 17, ny - 1) + 1,
+                           max(16 * cx - 1, 0):min(16 * cx + 17, nx - 1) + 1]
+                fin = blk[~np.isnan(blk)]
+                if fin.size == 0:
+                    continue
+                e0, e1 = entry(fin.min()), entry(fin.max())
+                i0, i1 = min(e0, e1), max(e0, e1) + 1
+                out[cz, cy, cx] = lut.reshape(-1, 4)[i0:i1 + 1, 3].max()
+    return out
+
+
+@pytest.mark.parametrize("shape", [(40, 33, 50), (16, 16, 16), (1, 20, 17)])
+def test_majorant_grid_matches_restatement(require_gpu, shape):
+    rng = np.random.default_rng(sum(shape))
+    data = rng.random(shape, dtype=np.float32)
+    data[data < 0.6] = 0.0
+    data[:18, :18, :18] = 0.0                    # cell (0, 0, 0) and its halo: empty
+    data[tuple(rng.integers(0, s, 5) for s in shape)] = np.nan
+    lut = orc.tf_discretize(S.COLORMAP5, np.array([0.0, 0.0, 0.3, 1.0], np.float32), (0.0, 1.0))
+    vr = (0.0, 1.0)
+    want = _grid_reference(data, lut, vr)
+    L = N.hip()
+    db, lb = DeviceBuffer.from_array(data), DeviceBuffer.from_array(lut)
+    v = N.vrx_volume()
+    v.d_data, v.d_lut = db.addr, lb.addr
+    v.dims[:] = [shape[2], shape[1], shape[0]]
+    v.spacing[:] = [1.0, 1.0, 1.0]
+    v.value_range[:] = list(vr)
+    v.unit_distance = 1.0
+    cells = L.vrx_volume_grid_cells(C.byref(v))
+    assert cells == want.size
+    out = DeviceBuffer(4 * cells)
+    print(f"[grid] shape {shape} cells {cells}", flush=True)
+    N.check(L.vrx_volume_majorant_grid(C.byref(v), out.ptr, None), "grid")
+    N.check(L.vrx_device_sync(), "sync")
+    got = out.to_array(np.float32, cells).reshape(want.shape)
+    np.testing.assert_array_equal(got, want)
+    assert want.reshape(-1)[0] == 0 and (want.size == 1 or (want > 0).any())
+    for b in (db, lb, out):
+        b.free()
+
+
+def _render_both(sc, cam, w, h, kind, **rp):
+    on = render_gpu(sc, cam, w, h, kind=kind, emptySpaceSkipping=True, **rp)
+    off = render_gpu(sc, cam, w, h, kind=kind, emptySpaceSkipping=False, **rp)
+    for ch in ("color", "depth", "primitiveId", "objectId", "instanceId"):
+        np.testing.assert_array_equal(on[ch], off[ch], err_msg=ch)
+    return on, off
+
+
+def _vol_samples(sc, cam, w, h, kind, skip, **rp):
+    from visrtx_amd import anari as A
+    dev = A.Device()
+    s = S.build_anari(dev, sc)
+    c = S.make_camera(dev, s, cam)
+    r = S.make_renderer(dev, s, kind, emptySpaceSkipping=skip, **rp)
+    f = S.make_frame(dev, s, s.world, c, r, w, h)
+    dev.render(f)
+    dev.ready(f)
+    n = dev.get_property(f, "volumeSamples", A.UINT64)
+    s.release()
+    dev.close()
+    return n
+
+
+def test_skipping_is_exact_volume_only(require_gpu):
+    """Volume-only raycast (the LDS-LUT kernel): same frame with and without
+    skipping, equal to the FP64 oracle, with far fewer samples taken."""
+    sc, cam = S.sparse_volume_workload(128, width=200, height=120)
+    on, _ = _render_both(sc, cam, 200, 120, "raycast", volumeSamplingRate=0.5)
+    o = orc.OracleScene(sc).render(cam, 200, 120, volumeSamplingRate=0.5)
+    ok = o["ambiguous"] == 0
+    assert ok.mean() > 0.99
+    np.testing.assert_allclose(on["color"][ok], o["color"][ok], rtol=0, atol=1e-4)
+    np.testing.assert_array_equal(on["objectId"], o["objectId"])
+    n_on = _vol_samples(sc, cam, 200, 120, "raycast", True, volumeSamplingRate=0.5)
+    n_off = _vol_samples(sc, cam, 200, 120, "raycast", False, volumeSamplingRate=0.5)
+    print(f"volume samples: {n_on} with skipping, {n_off} without")
+    assert n_on < 0.5 * n_off
+
+
+def test_skipping_is_exact_with_surfaces_and_lights(require_gpu):
+    """The generic kernel (march_all before surfaces) and directLight's
+    volume shadow rays skip too, with identical frames."""
+    sc, cam = S.sparse_volume_workload(96, width=160, height=100)
+    c, r = S.gen_spheres(400, seed=4, rmin=0.01, rmax=0.03)
+    sc.geoms = [S.GeomDesc("sphere", c, radius=r, surface_id=3)]
+    sc.world_geoms = [0]
+    sc.lights = [S.LightDesc("directional", direction=(0.2, -1.0, 0.3), irradiance=2.0)]
+    _render_both(sc, cam, 160, 100, "ao", pixelSamples=1, ambientSamples=1,
+                 volumeSamplingRate=0.5)
+    on, _ = _render_both(sc, cam, 160, 100, "directLight", pixelSamples=1, ambientSamples=1,
+                         ambientRadiance=0.2, volumeSamplingRate=0.5)
+    o = orc.OracleScene(sc).render(cam, 160, 100, kind="directLight", pixelSamples=1,
+                                   ambientSamples=1, ambientRadiance=0.2, volumeSamplingRate=0.5)
+    ok = o["ambiguous"] == 0
+    assert ok.mean() > 0.98
+    np.testing.assert_allclose(on["color"][ok], o["color"][ok], rtol=0, atol=1e-4)
+    np.testing.assert_array_equal(on["primitiveId"], o["primitiveId"])

@@ -85,6 +85,13 @@ class vrx_scene(C.Structure):
                 ("num_lights", C.c_uint32), ("d_lights", C.c_void_p)]
 
 
+class vrx_volume(C.Structure):
+    _fields_ = [("d_data", C.c_void_p), ("dims", C.c_uint32 * 3), ("origin", f3),
+                ("spacing", f3), ("d_lut", C.c_void_p), ("value_range", C.c_float * 2),
+                ("unit_distance", C.c_float), ("id", C.c_uint32), ("inst_id", C.c_uint32),
+                ("pad", C.c_uint32), ("d_majorant", C.c_void_p)]
+
+
 class vrx_light(C.Structure):
     _fields_ = [("type", C.c_int32), ("radiance", f3), ("pos", f3), ("dir", f3),
                 ("cos_outer", C.c_float), ("cos_inner", C.c_float)]
@@ -96,7 +103,7 @@ class vrx_renderer(C.Structure):
                 ("ambient_samples", C.c_int32), ("ao_distance", C.c_float),
                 ("cull_backfaces", C.c_int32), ("inv_volume_sampling_rate", C.c_float),
                 ("stack_lds", C.c_int32), ("block_pixels", C.c_int32), ("vol_unroll", C.c_int32),
-                ("node_width", C.c_int32)]
+                ("node_width", C.c_int32), ("skip_empty", C.c_int32)]
 
 
 class vrx_batch(C.Structure):
@@ -167,6 +174,8 @@ def hip() -> C.CDLL:
             "vrx_frame_clear": (i32, [vp, vp]),
             "vrx_render": (i32, [vp, vp, vp, vp, vp]),
             "vrx_camera_perspective": (i32, [vp, vp, vp, C.c_float, C.c_float, vp, vp]),
+            "vrx_volume_grid_cells": (sz, [vp]),
+            "vrx_volume_majorant_grid": (i32, [vp, vp, vp]),
             "vrx_tlas_query_sizes": (i32, [u32, vp]),
             "vrx_tlas_build": (i32, [vp, vp, vp, sz, vp]),
             "vrx_diffvol_forward": (i32, [vp, u32, vp, vp, vp, vp, vp]),

@@ -52,7 +52,7 @@ const float kRate = 0.125f, kRateMin = 1e-3f, kRateMax = 10.0f;
 const float kAoDist = 1e20f;
 const float kHalf = 0.5f;
 const int32_t kInt1 = 1, kInt0 = 0, kInt2 = 2, kInt4 = 4, kInt256 = 256, kLimit = 128;
-const int32_t kFalse = 0;
+const int32_t kFalse = 0, kTrue1 = 1;
 const uint32_t kNoId = 0xFFFFFFFFu;
 const ANARIDataType kSrgb = ANARI_UFIXED8_RGBA_SRGB;
 const float kIdentity4[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
@@ -189,7 +189,9 @@ std::vector<ParamDef> rendererParams(int kind)
       {"sampleLimit", ANARI_INT32, false, &kLimit, &kInt0, nullptr, "accumulation limit (raycast: 1)"},
       {"stackDepthLDS", ANARI_INT32, false, &kInt0, &kInt0, nullptr, "traversal stack entries kept in LDS (0: default)"},
       {"volumeUnroll", ANARI_INT32, false, &kInt0, &kInt0, nullptr, "volume samples gathered per step (0: default)"},
-      {"nodeWidth", ANARI_INT32, false, &kInt4, &kInt2, &kInt4, "BVH node width traversed: 2 or 4"}};
+      {"nodeWidth", ANARI_INT32, false, &kInt4, &kInt2, &kInt4, "BVH node width traversed: 2 or 4"},
+      {"emptySpaceSkipping", ANARI_BOOL, false, &kTrue1, nullptr, nullptr,
+          "skip empty 16^3 macro cells in volume marches (same image)"}};
   if (ao) {
     p.push_back({"pixelSamples", ANARI_INT32, false, &kInt1, &kInt1, nullptr, "samples per pixel per frame"});
     p.push_back({"ambientSamples", ANARI_INT32, false, &kInt1, &kInt0, &kInt256, "occlusion rays per hit"});

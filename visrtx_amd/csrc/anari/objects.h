@@ -194,6 +194,11 @@ struct Volume : Object {
   float unitDistance{1.0f};
   float lut[256 * 4];
   void *dLut{nullptr};
+  // macro-cell majorant grid (rebuilt with the world; attached to the
+  // device volume when at least 1/8 of its cells are empty)
+  void *dGrid{nullptr};
+  size_t gridCells{0};
+  float emptyCellFraction{0.0f};
 };
 
 // directional / point / spot lights (scene/light/*.cpp); the light record

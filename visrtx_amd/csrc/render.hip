@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_render_volume(vrx_scene sc, vrx_came
       vo = V.id;
       vi = V.inst_id;
       vdepth = t0;
-      march<I64, U, false>(vol_ctx(V, r.inv_volume_sampling_rate), s_lut, o, d, t0, t1, color, opacity,
+      march<I64, U, false>(vol_ctx(V, r.inv_volume_sampling_rate, r.skip_empty != 0), s_lut, o, d, t0, t1, color, opacity,
           vsamples);
     }
     const float depth = fminf(1e30f, vdepth);
@@ -152,6 +152,69 @@ __global__ void k_clear_frame(vrx_frame f)
 // --------------------------------------------------------------------------
 // instance level: world boxes of entries
 // --------------------------------------------------------------------------
+// Macro-cell majorants: one 256-lane workgroup per 16^3-voxel cell reduces
+// the min / max over the cell's voxels extended by one voxel on every side
+// (every voxel a trilinear sample within a voxel of the cell can read), then
+// takes the largest LUT alpha over the entries that range classifies to
+// (the same clamp / scale / floor as classify()).  NaN voxels are ignored
+// (the march skips NaN samples); a cell of only NaN gets majorant 0.
+__global__ __launch_bounds__(256) void k_majorants(vrx_volume v, float *out)
+{
+  __shared__ float s_lo[4], s_hi[4];
+  const int mx = ((int)v.dims[0] + 15) >> 4, my = ((int)v.dims[1] + 15) >> 4;
+  const int cell = blockIdx.x;
+  const int cx = cell % mx, cy = (cell / mx) % my, cz = cell / (mx * my);
+  int b0[3], n[3];
+  const int cc[3] = {cx, cy, cz};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    b0[a] = max(16 * cc[a] - 1, 0);
+    const int b1 = min(16 * cc[a] + 17, (int)v.dims[a] - 1);
+    n[a] = b1 - b0[a] + 1;
+  }
+  float lo = INFINITY, hi = -INFINITY;
+  const int total = n[0] * n[1] * n[2];
+  for (int i = threadIdx.x; i < total; i += 256) {
+    const int x = i % n[0], y = (i / n[0]) % n[1], z = i / (n[0] * n[1]);
+    const float val = gld(v.d_data, (size_t)(b0[0] + x) + (size_t)v.dims[0] *
+        ((size_t)(b0[1] + y) + (size_t)v.dims[1] * (size_t)(b0[2] + z)));
+    lo = fminf(lo, val); // fminf / fmaxf return the other operand for NaN
+    hi = fmaxf(hi, val);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, off, 64));
+    hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0)
+    return;
+  lo = fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3]));
+  hi = fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3]));
+  float m = 0.0f;
+  if (lo <= hi) {
+    const float vlo = v.value_range[0], vhi = v.value_range[1];
+    const float inv = 1.0f / (vhi - vlo);
+    int e[2];
+    const float ends[2] = {lo, hi};
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const float cl = fminf(fmaxf(ends[q], vlo), vhi);
+      const float x = fminf(fmaxf((cl - vlo) * inv * 256.0f - 0.5f, 0.0f), 255.0f);
+      e[q] = min((int)x, 254);
+    }
+    const int i0 = min(e[0], e[1]), i1 = max(e[0], e[1]) + 1;
+    const float4 *lut = (const float4 *)v.d_lut;
+    for (int i = i0; i <= i1; i++)
+      m = fmaxf(m, gld(lut, i).w);
+  }
+  out[cell] = m;
+}
+
 __global__ void k_entry_boxes(vrx_scene sc, float *boxes)
 {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -338,6 +401,26 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
   default:
     return launch_pt<-1>(*scene, *cam, *r, *f, sd, cap, st);
   }
+}
+
+size_t vrx_volume_grid_cells(const vrx_volume *v)
+{
+  if (!v)
+    return 0;
+  return (size_t)((v->dims[0] + 15) / 16) * ((v->dims[1] + 15) / 16) * ((v->dims[2] + 15) / 16);
+}
+
+int vrx_volume_majorant_grid(const vrx_volume *v, float *d_majorant, vrx_stream s)
+{
+  VRX_ARG(v && d_majorant, "null argument");
+  VRX_ARG(v->d_data && v->d_lut, "volume data / LUT");
+  VRX_ARG(v->dims[0] > 0 && v->dims[1] > 0 && v->dims[2] > 0, "volume dims");
+  const size_t cells = vrx_volume_grid_cells(v);
+  VRX_ARG(cells < (1u << 31), "too many macro cells");
+  hipLaunchKernelGGL(k_majorants, dim3((unsigned)cells), dim3(256), 0, (hipStream_t)s, *v,
+      d_majorant);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
 }
 
 int vrx_camera_perspective(const float pos[3], const float dir[3], const float up[3], float fovy,

@@ -67,6 +67,8 @@ struct VolCtx {
   float origin[3], inv_sp[3];
   uint32_t sy, sz;
   float step, expo, lo, hi, inv_range;
+  const float *mc; // macro-cell majorants (16^3 voxels per cell) or null
+  int mcd[3];
 };
 
 __device__ __forceinline__ float vol_step(const vrx_volume &v)
@@ -75,10 +77,14 @@ __device__ __forceinline__ float vol_step(const vrx_volume &v)
   return fminf(fminf(v.spacing[0] / 2.0f, v.spacing[1] / 2.0f), v.spacing[2] / 2.0f);
 }
 
-__device__ __forceinline__ VolCtx vol_ctx(const vrx_volume &v, float inv_rate)
+__device__ __forceinline__ VolCtx vol_ctx(const vrx_volume &v, float inv_rate, bool skip = false)
 {
   VolCtx c;
   c.data = v.d_data;
+  c.mc = skip ? v.d_majorant : nullptr;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    c.mcd[k] = ((int)v.dims[k] + 15) >> 4;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     c.dims[k] = (int)v.dims[k];
@@ -205,6 +211,38 @@ __device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, 
   const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
   float T = 1.0f;
   for (uint32_t k = 0; k < kMaxSamplesPerSegment; k += U) {
+    if (c.mc) {
+      // empty-space skipping: when the macro cell of sample k has a 0
+      // majorant, every sample up to the ray's exit from that cell adds
+      // nothing (alpha 0 gives a step transmittance of exactly 1), so jump
+      // to the first sample past the exit.  The cell's value range covers
+      // one voxel beyond its box, which absorbs the rounding of the exit.
+      const float t = t0 + (float)k * c.step;
+      if (!(t <= t1))
+        break;
+      const float g[3] = {gox + gdx * t, goy + gdy * t, goz + gdz * t};
+      const float gd[3] = {gdx, gdy, gdz}, go[3] = {gox, goy, goz};
+      int ci[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        ci[a] = min(max((int)floorf(g[a] * 0.0625f), 0), c.mcd[a] - 1);
+      if (gld(c.mc + ((size_t)ci[2] * c.mcd[1] + ci[1]) * c.mcd[0] + ci[0]) == 0.0f) {
+        float te = INFINITY;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          if (gd[a] > 0.0f)
+            te = fminf(te, (16.0f * (float)(ci[a] + 1) - go[a]) / gd[a]);
+          else if (gd[a] < 0.0f)
+            te = fminf(te, (16.0f * (float)ci[a] - go[a]) / gd[a]);
+        }
+        const float kn = ceilf((te - t0) / c.step);
+        if (!(kn < (float)kMaxSamplesPerSegment))
+          break;
+        // k advances by at least one sample; the loop adds U
+        k = max((uint32_t)kn, k + 1) - U;
+        continue;
+      }
+    }
     float sv[U];
     bool ok[U];
 #pragma unroll
@@ -247,7 +285,8 @@ __device__ __forceinline__ bool vol_is_large(const vrx_volume &v)
 
 // rayMarchAllVolumes (volumeIntegration.h:317-350)
 __device__ float march_all(const vrx_scene &sc, v3 o, v3 d, float tlower, float tfar,
-    float inv_rate, v3 &color, float &opacity, uint32_t &obj, uint32_t &inst, uint32_t &ns)
+    float inv_rate, v3 &color, float &opacity, uint32_t &obj, uint32_t &inst, uint32_t &ns,
+    bool skip)
 {
   float depth = tfar;
   bool first = true;
@@ -277,7 +316,7 @@ __device__ float march_all(const vrx_scene &sc, v3 o, v3 d, float tlower, float 
       first = false;
     }
     depth = fminf(depth, bt0);
-    const VolCtx c = vol_ctx(V, inv_rate);
+    const VolCtx c = vol_ctx(V, inv_rate, skip);
     if (vol_is_large(V))
       march<true, 2, true>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
     else
@@ -403,7 +442,7 @@ __device__ float ao_factor(const vrx_scene &sc, const SurfHit &s, float dist, in
 // one opacity (rayMarchVolume without colour, volumeIntegration.h:64-103),
 // volumes in index order, stopping at 0.99.
 __device__ float volume_atten(
-    const vrx_scene &sc, v3 o, v3 d, float tmax, float inv_rate, uint32_t &ns)
+    const vrx_scene &sc, v3 o, v3 d, float tmax, float inv_rate, bool skip, uint32_t &ns)
 {
   float opacity = 0.0f;
   for (uint32_t i = 0; i < sc.num_volumes && opacity < 0.99f; i++) {
@@ -412,7 +451,7 @@ __device__ float volume_atten(
     if (!vol_slab(V, o, d, 0.0f, tmax, t0, t1))
       continue;
     v3 unused = mk(0.0f, 0.0f, 0.0f);
-    const VolCtx c = vol_ctx(V, inv_rate);
+    const VolCtx c = vol_ctx(V, inv_rate, skip);
     if (vol_is_large(V))
       march<true, 2, true>(c, (const float4 *)V.d_lut, o, d, t0, t1, unused, opacity, ns);
     else
@@ -496,7 +535,8 @@ __device__ float4 shade_direct(const vrx_scene &sc, const vrx_renderer &r, const
     const float so = 1.0f - surface_atten<PT, TWO, WIDE>(sc, sp, ls.dir, ls.dist, st, rays);
     float vo = 1.0f;
     if (PT < 0 && sc.num_volumes)
-      vo = 1.0f - volume_atten(sc, sp, ls.dir, ls.dist, r.inv_volume_sampling_rate, vsamples);
+      vo = 1.0f - volume_atten(sc, sp, ls.dir, ls.dist, r.inv_volume_sampling_rate,
+                          r.skip_empty != 0, vsamples);
     const float att = so * vo;
     contrib = add(contrib, mulf(c, att));
   }
@@ -564,7 +604,7 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
         float vdepth = s.t;
         if (PT < 0 && sc.num_volumes) // scenes with volumes use the generic kernel
           vdepth = march_all(sc, o, d, tlower, s.t, r.inv_volume_sampling_rate, color, opacity,
-              vo, vi, vsamples);
+              vo, vi, vsamples, r.skip_empty != 0);
         if (first) {
           if (vdepth < s.t) {
             depth = vdepth;
@@ -620,7 +660,7 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
         float vdepth = tmax;
         if (PT < 0 && sc.num_volumes) // scenes with volumes use the generic kernel
           vdepth = march_all(sc, o, d, tlower, tmax, r.inv_volume_sampling_rate, color, opacity,
-              vo, vi, vsamples);
+              vo, vi, vsamples, r.skip_empty != 0);
         if (first) {
           depth = fminf(depth, vdepth);
           prim = 0;

@@ -199,6 +199,37 @@ def volume_workload(n: int = 512, width: int = 1920, height: int = 1080, seed: i
     return sc, cam
 
 
+def sparse_volume_workload(n: int = 512, blobs: int = 12, width: int = 1920,
+                           height: int = 1080, seed: int = 5):
+    """n^3 field, bounds [0,1]^3: value 0 except inside ``blobs`` balls
+    (radius 0.06..0.14), where it is a smooth bump times fractal noise.  The
+    transfer function's opacity is exactly 0 below value 1/3, so most 16^3
+    macro cells are empty."""
+    rng = np.random.default_rng(seed)
+    noise = gen_noise_volume(n, seed=seed)
+    ax = np.linspace(0.0, 1.0, n, dtype=np.float32)
+    data = np.zeros((n, n, n), np.float32)
+    for _ in range(blobs):
+        c = rng.uniform(0.15, 0.85, 3)
+        r = rng.uniform(0.06, 0.14)
+        i0 = [max(int((c[a] - r) * (n - 1)) - 1, 0) for a in range(3)]
+        i1 = [min(int((c[a] + r) * (n - 1)) + 2, n) for a in range(3)]
+        z, y, x = np.meshgrid(ax[i0[2]:i1[2]], ax[i0[1]:i1[1]], ax[i0[0]:i1[0]], indexing="ij")
+        q = ((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / (r * r)
+        bump = np.where(q < 1.0, (1.0 - q) ** 2, 0.0).astype(np.float32)
+        sl = (slice(i0[2], i1[2]), slice(i0[1], i1[1]), slice(i0[0], i1[0]))
+        data[sl] = np.maximum(data[sl], bump * (0.5 + noise[sl]))
+    sp = 1.0 / (n - 1)
+    vol = VolumeDesc(data=data, origin=(0.0, 0.0, 0.0), spacing=(sp, sp, sp),
+                     color=COLORMAP5, opacity=np.array([0.0, 0.0, 0.4, 0.8], np.float32),
+                     value_range=(0.0, 1.0), unit_distance=sp * 4, id=0)
+    sc = SceneDesc(volumes=[vol], world_id=0)
+    pos = (1.9, 1.5, -1.0)
+    cam = CameraDesc(pos, look_at(pos, (0.5, 0.5, 0.5)), (0, 1, 0), math.radians(35.0),
+                     width / height)
+    return sc, cam
+
+
 def random_rotation(rng) -> np.ndarray:
     """Uniformly distributed rotation (normalised Gaussian quaternion), 3x3 float64."""
     q = rng.normal(size=4)
@@ -463,7 +494,8 @@ def make_renderer(dev: A.Device, s: AnariScene, kind: str = "raycast", **params)
              "ambientRadiance": A.FLOAT32, "pixelSamples": A.INT32, "ambientSamples": A.INT32,
              "ambientOcclusionDistance": A.FLOAT32, "cullTriangleBackfaces": A.BOOL,
              "volumeSamplingRate": A.FLOAT32, "sampleLimit": A.INT32, "stackDepthLDS": A.INT32,
-             "volumeUnroll": A.INT32, "nodeWidth": A.INT32, "lightFalloff": A.FLOAT32}
+             "volumeUnroll": A.INT32, "nodeWidth": A.INT32, "lightFalloff": A.FLOAT32,
+             "emptySpaceSkipping": A.BOOL}
     for k, v in params.items():
         dev.set(r, k, v, types[k])
     dev.commit(r)
