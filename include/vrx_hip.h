@@ -218,6 +218,8 @@ typedef struct vrx_scene {
   uint32_t wide;                /* 1 = every BLAS (and the TLAS if any) has d_wide */
   uint32_t num_lights;
   const vrx_light *d_lights;    /* device array (directLight renderer) */
+  uint32_t num_majorant_grids;  /* volumes that carry d_majorant */
+  uint32_t pad;
 } vrx_scene;
 
 typedef struct vrx_camera {

@@ -82,7 +82,8 @@ class vrx_scene(C.Structure):
                 ("max_blas_height", C.c_uint32), ("d_volumes", C.c_void_p),
                 ("num_volumes", C.c_uint32), ("single_type", C.c_int32),
                 ("max_volume_voxels", C.c_uint64), ("wide", C.c_uint32),
-                ("num_lights", C.c_uint32), ("d_lights", C.c_void_p)]
+                ("num_lights", C.c_uint32), ("d_lights", C.c_void_p),
+                ("num_majorant_grids", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class vrx_volume(C.Structure):

@@ -1239,6 +1239,7 @@ int World::rebuild()
   }
   std::vector<vrx_volume> hv;
   uint64_t maxVox = 0;
+  uint32_t nGrids = 0;
   for (size_t i = 0; i < vols.size(); i++) {
     Volume *v = vols[i];
     vrx_volume vv{};
@@ -1274,6 +1275,7 @@ int World::rebuild()
           vv.d_majorant = (const float *)v->dGrid;
       }
     }
+    nGrids += vv.d_majorant ? 1u : 0u;
     hv.push_back(vv);
     maxVox = std::max(maxVox, (uint64_t)vv.dims[0] * vv.dims[1] * vv.dims[2]);
     float lo[3], hi[3];
@@ -1319,6 +1321,7 @@ int World::rebuild()
   scene.num_volumes = (uint32_t)hv.size();
   scene.d_lights = (const vrx_light *)dLights;
   scene.num_lights = (uint32_t)hl.size();
+  scene.num_majorant_grids = nGrids;
   scene.max_blas_height = maxH;
   scene.single_type = single >= 0 ? single : -1;
   scene.max_volume_voxels = maxVox;

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam,
 // rayMarchAllVolumes, the renderer's colour-times-opacity, background), with
 // the transfer-function LUT staged in LDS and no traversal state, so the
 // kernel needs about half the registers of the general one.
-template <bool I64, int U>
+template <bool I64, int U, bool SKIP>
 __global__ __launch_bounds__(kBlock) void k_render_volume(vrx_scene sc, vrx_camera cam,
     vrx_renderer r, vrx_frame f)
 {
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_render_volume(vrx_scene sc, vrx_came
       vo = V.id;
       vi = V.inst_id;
       vdepth = t0;
-      march<I64, U, false>(vol_ctx(V, r.inv_volume_sampling_rate, r.skip_empty != 0), s_lut, o, d, t0, t1, color, opacity,
+      march<I64, U, false, SKIP>(vol_ctx(V, r.inv_volume_sampling_rate, SKIP), s_lut, o, d, t0, t1, color, opacity,
           vsamples);
     }
     const float depth = fminf(1e30f, vdepth);
@@ -373,8 +373,18 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
     dim3 grid((f->width + kTile - 1) / kTile, (f->height + kTile - 1) / kTile);
     const int u = r->vol_unroll > 0 ? r->vol_unroll : 4;
     const bool big = scene->max_volume_voxels >= (1ull << 31);
+    // the volume's macro-cell grid, when it has one, drives empty-space
+    // skipping in its own kernel instance (the dense march carries no check)
+    const bool skip = r->skip_empty && scene->num_majorant_grids > 0;
 #define VRX_VOL_LAUNCH(B, U)                                                                    \
-  hipLaunchKernelGGL((k_render_volume<B, U>), grid, dim3(kBlock), 0, st, *scene, *cam, *r, *f)
+  do {                                                                                          \
+    if (skip)                                                                                   \
+      hipLaunchKernelGGL((k_render_volume<B, U, true>), grid, dim3(kBlock), 0, st, *scene, *cam, \
+          *r, *f);                                                                              \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_render_volume<B, U, false>), grid, dim3(kBlock), 0, st, *scene,      \
+          *cam, *r, *f);                                                                        \
+  } while (0)
     if (big)
       VRX_VOL_LAUNCH(true, 2);
     else if (u <= 1)

@@ -199,7 +199,7 @@ __device__ __forceinline__ float4 classify(const float4 *lut, const VolCtx &c, f
 // positions do not depend on the accumulation), which keeps 8*U loads in
 // flight per lane; samples past the exit or the 0.99 cut-off are discarded,
 // their clamped gathers always stay inside the field.
-template <bool I64, int U, bool LUT_GLOBAL>
+template <bool I64, int U, bool LUT_GLOBAL, bool SKIP = true>
 __device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, v3 d, float t0,
     float t1, v3 &color, float &opacity, uint32_t &ns)
 {
@@ -211,7 +211,7 @@ __device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, 
   const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
   float T = 1.0f;
   for (uint32_t k = 0; k < kMaxSamplesPerSegment; k += U) {
-    if (c.mc) {
+    if (SKIP && c.mc) {
       // empty-space skipping: when the macro cell of sample k has a 0
       // majorant, every sample up to the ray's exit from that cell adds
       // nothing (alpha 0 gives a step transmittance of exactly 1), so jump
