@@ -129,6 +129,13 @@ def test_batched_rejects_bad_tiles_and_small_scratch(require_gpu):
     assert L.vrx_render_batch(C.byref(bf.scene), C.byref(bf.renderer), C.byref(bf.b), None) \
         == -2
     assert b"scratch" in L.vrx_last_error()
+    # the batched kernel runs the raycast program only: other renderers are
+    # refused, not silently rendered as raycast
+    rr = N.vrx_renderer.from_buffer_copy(bf.renderer)
+    rr.kind = N.VRX_RENDER_AO
+    bf.b.scratch_bytes = 0
+    assert L.vrx_render_batch(C.byref(bf.scene), C.byref(rr), C.byref(bf.b), None) == -2
+    assert b"raycast" in L.vrx_last_error()
     s.release()
     dev.close()
 

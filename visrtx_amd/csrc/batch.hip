@@ -188,6 +188,7 @@ static int validate(const vrx_scene *scene, const vrx_renderer *r, const vrx_bat
   VRX_ARG(scene->num_entries == 0 || (scene->d_entries && scene->d_geoms), "scene arrays");
   VRX_ARG(scene->num_entries <= 1 || scene->tlas.d_leaf_ids, "instance BVH not built");
   VRX_ARG(scene->num_volumes == 0 || scene->d_volumes, "volume array");
+  VRX_ARG(r->kind == VRX_RENDER_RAYCAST, "batched views run the raycast renderer only");
   VRX_ARG(r->node_width == 0 || r->node_width == 2 || r->node_width == 4, "node width 2 or 4");
   VRX_ARG(r->node_width != 4 || scene->num_entries == 0 || scene->wide,
       "node width 4 needs 4-wide nodes for every BVH (scene->wide)");
