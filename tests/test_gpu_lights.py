@@ -29,8 +29,8 @@ def _quad(y, half, down, opacity=1.0, sid=5, color=(0.5, 0.5, 0.5), cx=0.0, cz=0
 
 def _lit_scene(seed=3):
     """A plane, 3000 spheres, 200 cylinders, a translucent canopy, two
-    instances of a sphere mesh (one group carrying a point light), and world
-    directional, point and spot lights."""
+    instances of a sphere mesh whose groups carry a point light and a
+    directional + spot pair, and world directional, point and spot lights."""
     rng = np.random.default_rng(seed)
     n = 3000
     c = np.stack([rng.uniform(-1, 1, n), rng.uniform(0.05, 1.0, n),
@@ -55,7 +55,14 @@ def _lit_scene(seed=3):
     insts = [S.InstanceDesc(geoms=[4], xfm=np.concatenate([rot, [0.5, 0.3, -0.4]]).astype(
                  np.float32), id=100, lights=[pl]),
              S.InstanceDesc(geoms=[4], xfm=np.concatenate([rot, [-0.6, 0.35, 0.5]]).astype(
-                 np.float32), id=101)]
+                 np.float32), id=101, lights=[
+                     # under the instance's 0.3 scale: a directional light's
+                     # direction is not renormalised, a spot's axis is not
+                     # transformed (sampleLight.h:58-100)
+                     S.LightDesc("directional", direction=(0.2, -1.0, 0.1), irradiance=1.0),
+                     S.LightDesc("spot", position=(0.0, 3.0, 0.0), direction=(0.1, -1.0, 0.0),
+                                 intensity=0.9, opening_angle=math.radians(70.0),
+                                 falloff_angle=0.2)])]
     lights = [S.LightDesc("directional", color=(1.0, 0.95, 0.9), direction=(0.4, -1.0, 0.25),
                           irradiance=1.5),
               S.LightDesc("point", position=(-0.8, 1.2, -0.6), intensity=0.8),

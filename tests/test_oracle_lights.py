@@ -270,3 +270,32 @@ def test_instance_light_transform():
     o = _render(sc_r, cam)
     np.testing.assert_allclose(o["color"][..., :3],
                                np.broadcast_to(np.array(BASE) * INV_PI, (H, W, 3)), rtol=1e-5)
+
+
+def test_directional_light_under_scaling_instance_is_not_renormalised():
+    """xfmVec of the instance's matrix is applied to the direction with no
+    renormalisation (sampleLight.h:58-66): under a uniform scale of 2 the
+    Lambert term doubles."""
+    dl = S.LightDesc("directional", direction=(0.0, -1.0, 0.0), irradiance=1.0)
+    x2 = np.array([2, 0, 0, 0, 2, 0, 0, 0, 2, 0, 0, 0], np.float32)
+    sc, cam = _plane_scene(instances=[S.InstanceDesc(geoms=[], xfm=x2, lights=[dl])])
+    o = _render(sc, cam)
+    np.testing.assert_allclose(o["color"][..., :3],
+                               np.broadcast_to(2.0 * np.array(BASE) * INV_PI, (H, W, 3)),
+                               rtol=1e-5)
+
+
+def test_spot_light_under_instance_moves_but_keeps_its_axis():
+    """A spot light's position follows the instance transform, its axis does
+    not (sampleLight.h:81-86 reads ld.spot.direction untransformed): a group
+    spot at the origin pointing down, under a translation (0, 0.8, 0) and a
+    90-degree rotation about z, lights the plane as a world spot at
+    (0, 0.8, 0) pointing down."""
+    sp = S.LightDesc("spot", position=(0.0, 0.0, 0.0), direction=(0.0, -1.0, 0.0),
+                     intensity=2.0, opening_angle=math.radians(60.0), falloff_angle=0.2)
+    rz = np.array([0, 1, 0, -1, 0, 0, 0, 0, 1, 0.0, 0.8, 0.0], np.float32)
+    sc_i, cam = _plane_scene(instances=[S.InstanceDesc(geoms=[], xfm=rz, lights=[sp])])
+    ws = S.LightDesc("spot", position=(0.0, 0.8, 0.0), direction=(0.0, -1.0, 0.0),
+                     intensity=2.0, opening_angle=math.radians(60.0), falloff_angle=0.2)
+    sc_w, _ = _plane_scene(lights=[ws])
+    np.testing.assert_array_equal(_render(sc_i, cam)["color"], _render(sc_w, cam)["color"])
