@@ -124,3 +124,31 @@ def test_launch_autotuning_keeps_frames_identical(require_gpu):
     assert not dev.errors(), dev.errors()
     s.release()
     dev.close()
+
+
+def test_group_bounds_property(require_gpu):
+    """Group "bounds" (Group.cpp:65-86): union of the surfaces' BVH boxes and
+    the volumes' boxes in object space, available before any world uses the
+    group."""
+    dev = A.Device()
+    c, r = S.gen_spheres(5000, seed=9, rmin=2e-3, rmax=9e-3)
+    sc = S.SceneDesc(geoms=[S.GeomDesc("sphere", c, radius=r, surface_id=1)], world_geoms=[0])
+    s = S.build_anari(dev, sc)
+    surf = S._surface(dev, s, sc.geoms[0])
+    vol = S._volume(dev, s, S.VolumeDesc(np.zeros((4, 5, 6), np.float32),
+                                         origin=(-1.0, 0.25, 0.5), spacing=(0.5, 0.5, 0.5)))
+    grp = s.keep(dev.new("Group"))
+    dev.set(grp, "surface", s.keep(dev.array([surf], A.SURFACE)), A.ARRAY1D)
+    dev.commit(grp)
+    got = np.array(dev.get_property(grp, "bounds", A.FLOAT32_BOX3), np.float32)
+    want = np.concatenate([(c - r[:, None]).min(0), (c + r[:, None]).max(0)]).astype(np.float32)
+    np.testing.assert_array_equal(got, want)
+    dev.set(grp, "volume", s.keep(dev.array([vol], A.VOLUME)), A.ARRAY1D)
+    dev.commit(grp)
+    got = np.array(dev.get_property(grp, "bounds", A.FLOAT32_BOX3), np.float32)
+    vlo, vhi = np.array([-1.0, 0.25, 0.5]), np.array([-1.0, 0.25, 0.5]) + 0.5 * np.array([5, 4, 3])
+    np.testing.assert_allclose(got, np.concatenate([np.minimum(want[:3], vlo),
+                                                    np.maximum(want[3:], vhi)]), rtol=1e-6)
+    assert not dev.errors(), dev.errors()
+    s.release()
+    dev.close()

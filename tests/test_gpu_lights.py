@@ -160,3 +160,34 @@ def test_unsupported_light_is_skipped(require_gpu):
     o = orc.OracleScene(sc).render(cam, 64, 48, kind="directLight", pixelSamples=1,
                                    ambientSamples=0)
     np.testing.assert_allclose(col, o["color"], rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("prim", ["sphere", "triangle", "cylinder"])
+def test_direct_light_primitive_specialised_kernels(require_gpu, prim):
+    """Worlds of one primitive type take the kernel specialised for it (no
+    volume code); directLight there against the oracle."""
+    lights = [S.LightDesc("directional", direction=(0.3, -1.0, 0.4), irradiance=1.5),
+              S.LightDesc("point", position=(0.3, 1.4, -0.2), intensity=0.7),
+              S.LightDesc("spot", position=(0.5, 0.9, 0.5), direction=(0.0, -1.0, 0.0),
+                          intensity=1.0, opening_angle=math.radians(80.0), falloff_angle=0.3)]
+    if prim == "sphere":
+        sc, cam = S.spheres_workload(n=100_000, width=160, height=120, seed=8)
+    elif prim == "triangle":
+        sc, cam = S.heightfield_workload()
+        cam = S.CameraDesc(cam.position, cam.direction, cam.up, cam.fovy, 160 / 120)
+    else:
+        rng = np.random.default_rng(5)
+        p0 = rng.random((4000, 3), dtype=np.float32)
+        p1 = p0 + rng.normal(0, 0.05, (4000, 3)).astype(np.float32)
+        sc = S.SceneDesc(geoms=[S.GeomDesc("cylinder", np.stack([p0, p1], 1).reshape(-1, 3),
+                                           radius=rng.uniform(0.003, 0.01, 4000).astype(
+                                               np.float32), caps=True, surface_id=2)],
+                         world_geoms=[0], world_id=3)
+        cam = S.CameraDesc((0.5, 0.5, -1.0), (0, 0, 1), (0, 1, 0), math.radians(50), 4 / 3)
+    sc.lights = lights
+    rp = dict(pixelSamples=2, ambientSamples=1, ambientRadiance=0.2,
+              ambientOcclusionDistance=0.05)
+    g = render_gpu(sc, cam, 160, 120, kind="directLight", **rp)
+    o = orc.OracleScene(sc).render(cam, 160, 120, kind="directLight", **rp)
+    compare(g, o)
+    assert (g["color"][..., :3] > 0).any()

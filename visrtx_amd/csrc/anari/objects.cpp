@@ -897,6 +897,47 @@ void Group::commit()
   lights = objectList<Light>(this, "light", ANARI_LIGHT); // Group.cpp:92
 }
 
+// Group.cpp:65-86: "bounds" (object space) is the union of the group's
+// surface BVH boxes (built here if they are not yet) and volume boxes.
+bool Group::getProperty(
+    const std::string &name, ANARIDataType t, void *mem, uint64_t size, uint32_t mask)
+{
+  if (name == "bounds" && t == ANARI_FLOAT32_BOX3 && size >= 24) {
+    dev->flushCommits();
+    vrx_stream s = dev->stream;
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    auto extend = [&](const float *lo, const float *hi) {
+      for (int k = 0; k < 3; k++) {
+        b[k] = std::min(b[k], lo[k]);
+        b[3 + k] = std::max(b[3 + k], hi[k]);
+      }
+    };
+    for (Surface *sf : surfaces) {
+      if (!sf || !sf->isValid())
+        continue;
+      Geometry *g = sf->geom;
+      if (g->bvhDirty && g->buildBVH(s) != 0)
+        return false;
+      if (!dev->check(vrx_memcpy(&g->header, g->dHeader, sizeof(vrx_bvh_header), VRX_D2H, s),
+              this, "header readback")
+          || !dev->check(vrx_stream_sync(s), this, "stream sync"))
+        return false;
+      if (g->header.n > 0)
+        extend(g->header.lo, g->header.hi);
+    }
+    for (Volume *v : volumes) {
+      if (!v || !v->isValid())
+        continue;
+      float lo[3], hi[3];
+      v->field->bounds(lo, hi);
+      extend(lo, hi);
+    }
+    memcpy(mem, b, 24);
+    return true;
+  }
+  return Object::getProperty(name, t, mem, size, mask);
+}
+
 // ---------------------------------------------------------------------------
 // Light (scene/light/Light.cpp:47-54, Directional.cpp:40-47, Point.cpp:40-48,
 // Spot.cpp:40-68)

@@ -220,6 +220,8 @@ struct Light : Object {
 struct Group : Object {
   Group(Device *d) : Object(d, ANARI_GROUP, "") {}
   void commit() override;
+  bool getProperty(const std::string &name, ANARIDataType type, void *mem, uint64_t size,
+      uint32_t mask) override;
   std::vector<Surface *> surfaces;
   std::vector<Volume *> volumes;
   std::vector<Light *> lights;

@@ -280,16 +280,10 @@ static void launch_kw(const vrx_scene &sc, const vrx_camera &cam, const vrx_rend
 {
   dim3 grid((f.width + kTile - 1) / kTile, (f.height + kTile - 1) / kTile);
   size_t lds = (size_t)sd * kBlock * 4;
-  if constexpr (PT < 0) {
-    // directLight runs in the generic kernel only (lights, any primitive
-    // type, volume transmission of shadow rays)
-    if (r.kind == VRX_RENDER_DIRECT) {
-      hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_DIRECT, WIDE>), grid, dim3(kBlock), lds, s,
-          sc, cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
-      return;
-    }
-  }
-  if (r.kind == VRX_RENDER_AO)
+  if (r.kind == VRX_RENDER_DIRECT)
+    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_DIRECT, WIDE>), grid, dim3(kBlock), lds, s, sc,
+        cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
+  else if (r.kind == VRX_RENDER_AO)
     hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_AO, WIDE>), grid, dim3(kBlock), lds, s, sc,
         cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
   else
@@ -401,7 +395,7 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
   }
   // the primitive-specialised kernels carry no volume code: scenes with
   // volumes take the generic one
-  switch (scene->num_volumes || r->kind == VRX_RENDER_DIRECT ? -1 : scene->single_type) {
+  switch (scene->num_volumes ? -1 : scene->single_type) {
   case VRX_TRIANGLE:
     return launch_pt<VRX_TRIANGLE>(*scene, *cam, *r, *f, sd, cap, st);
   case VRX_SPHERE:
