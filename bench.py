@@ -253,6 +253,12 @@ def bench_diffvol(iters=200, log_every=50):
     iter_ms = (time.perf_counter() - t0) * 1000.0 / iters
     with torch.no_grad():
         last = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean().item()
+        # the same timings at the state Adam reached (denser field, rays cross
+        # transfer-function intervals at almost every sample)
+        rgba1 = D.render(dens, lut, org, dirs, spec)
+        g_up = (2.0 / rgba1.numel()) * (rgba1 - target)
+    fwd_ms_final = timeit(fwd, 10)
+    fbi_ms_final = timeit(fwdbwd_integral, 10)
     return {"fwd_ms": round(fwd_ms, 3), "adjoint": "suffix",
             "fwd_bwd_integral_ms": round(fbi_ms, 3),
             "fwd_bwd_integral_over_fwd": round(fbi_ms / fwd_ms, 3),
@@ -261,6 +267,9 @@ def bench_diffvol(iters=200, log_every=50):
             "replay_adjoint": {"fwd_bwd_integral_ms": round(fbi_replay_ms, 3),
                                "fwd_bwd_integral_over_fwd": round(fbi_replay_ms / fwd_ms, 3),
                                "fwd_bwd_with_mse_loss_ms": round(fb_replay_ms, 3)},
+            "after_adam": {"fwd_ms": round(fwd_ms_final, 3),
+                           "fwd_bwd_integral_ms": round(fbi_ms_final, 3),
+                           "fwd_bwd_integral_over_fwd": round(fbi_ms_final / fwd_ms_final, 3)},
             "adam_ms_per_iter": round(iter_ms, 3),
             "iters": iters, "mse_first": first, "mse_final": last, "rays": int(org.shape[0])}
 
