@@ -58,3 +58,33 @@ def test_cpp_client_matches_oracle(require_gpu, tmp_path):
         cs = (cs * 1099511628211 + v) & M64
     assert int(kv["hits"]) == int(hit.sum())
     assert kv["checksum"] == f"{cs:016x}"
+
+
+def test_cpp_client_direct_light(require_gpu, tmp_path):
+    """The same client with the directLight renderer and a directional light
+    created through the C API: its sRGB image against the oracle's, within
+    one code value."""
+    n, W, H = 20000, 160, 120
+    out = tmp_path / "lit.ppm"
+    print(f"[app] {APP} {n} {W}x{H} directLight", flush=True)
+    r = subprocess.run([APP, str(n), str(W), str(H), str(out), "directLight"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    kv = dict(f.split("=", 1) for f in r.stdout.split())
+    assert kv["errors"] == "0"
+    data = out.read_bytes()
+    head = f"P6\n{W} {H}\n255\n".encode()
+    assert data.startswith(head)
+    img = np.frombuffer(data[len(head):], np.uint8).reshape(H, W, 3)[::-1]   # bottom row first
+    pos, rad = _app_spheres(n)
+    sc = S.SceneDesc(geoms=[S.GeomDesc("sphere", pos, radius=rad, surface_id=7,
+                                       color=(0.9, 0.6, 0.3))], world_geoms=[0],
+                     lights=[S.LightDesc("directional", direction=(0.3, -1.0, 0.5),
+                                         irradiance=2.0)])
+    cam = S.CameraDesc((0.5, 0.5, -1.2), (0.0, 0.0, 1.0), (0, 1, 0), math.pi / 3.0, W / H)
+    o = orc.OracleScene(sc).render(cam, W, H, kind="directLight", color_format=orc.ORC_COLOR_SRGB8,
+                                   background=(0.1, 0.1, 0.1, 1.0), ambientRadiance=0.2)
+    ref = o["color"].view(np.uint8).reshape(H, W, 4)[..., :3]
+    assert np.abs(img.astype(np.int32) - ref.astype(np.int32)).max() <= 1
+    assert img.std() > 5

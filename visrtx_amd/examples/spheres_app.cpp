@@ -58,6 +58,10 @@ int main(int argc, char **argv)
   const uint32_t W = argc > 2 ? (uint32_t)std::strtoul(argv[2], nullptr, 10) : 800u;
   const uint32_t H = argc > 3 ? (uint32_t)std::strtoul(argv[3], nullptr, 10) : 600u;
   const char *out = argc > 4 ? argv[4] : "spheres.ppm";
+  // renderer subtype: "raycast" (default) or "directLight" (then the world
+  // gets a directional light and the renderer a little ambient light)
+  const char *rtype = argc > 5 ? argv[5] : "raycast";
+  const bool lit = std::strcmp(rtype, "directLight") == 0;
 
   ANARILibrary lib = anariLoadLibrary("visrtx_amd", status, nullptr);
   if (!lib) {
@@ -109,6 +113,17 @@ int main(int argc, char **argv)
   ANARIWorld world = anariNewWorld(dev);
   ANARIArray1D aSurf = anariNewArray1D(dev, &surf, nullptr, nullptr, ANARI_SURFACE, 1);
   anariSetParameter(dev, world, "surface", ANARI_ARRAY1D, &aSurf);
+  ANARILight light = nullptr;
+  if (lit) {
+    light = anariNewLight(dev, "directional");
+    setF3(dev, light, "direction", 0.3f, -1.0f, 0.5f);
+    float irradiance = 2.0f;
+    anariSetParameter(dev, light, "irradiance", ANARI_FLOAT32, &irradiance);
+    anariCommitParameters(dev, light);
+    ANARIArray1D aLight = anariNewArray1D(dev, &light, nullptr, nullptr, ANARI_LIGHT, 1);
+    anariSetParameter(dev, world, "light", ANARI_ARRAY1D, &aLight);
+    anariRelease(dev, aLight);
+  }
   anariCommitParameters(dev, world);
   anariRelease(dev, aSurf);
 
@@ -120,9 +135,13 @@ int main(int argc, char **argv)
   anariSetParameter(dev, cam, "aspect", ANARI_FLOAT32, &aspect);
   anariCommitParameters(dev, cam);
 
-  ANARIRenderer ren = anariNewRenderer(dev, "raycast");
+  ANARIRenderer ren = anariNewRenderer(dev, rtype);
   float bg[4] = {0.1f, 0.1f, 0.1f, 1.0f};
   anariSetParameter(dev, ren, "background", ANARI_FLOAT32_VEC4, bg);
+  if (lit) {
+    float ambient = 0.2f;
+    anariSetParameter(dev, ren, "ambientRadiance", ANARI_FLOAT32, &ambient);
+  }
   anariCommitParameters(dev, ren);
 
   ANARIFrame frame = anariNewFrame(dev);
@@ -181,6 +200,8 @@ int main(int argc, char **argv)
   for (ANARIObject o : {(ANARIObject)frame, (ANARIObject)ren, (ANARIObject)cam,
            (ANARIObject)world, (ANARIObject)surf, (ANARIObject)mat, (ANARIObject)geom})
     anariRelease(dev, o);
+  if (light)
+    anariRelease(dev, light);
   anariRelease(dev, (ANARIObject)dev);
   anariUnloadLibrary(lib);
   return (px && hits > 0 && g_errors == 0) ? 0 : 2;
