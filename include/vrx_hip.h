@@ -353,8 +353,9 @@ typedef struct vrx_batch {
   int32_t persistent_blocks; /* 0 = fill the device at the kernel's occupancy */
 } vrx_batch;
 
-/* Renders num_views views of one scene (raycast renderer: color, depth and
- * id channels) in ONE persistent launch: every wave pulls 64-pixel tiles of
+/* Renders num_views views of one scene (the raycast, AO or directLight
+ * program: color, depth and id channels; each view's random numbers are those
+ * of the same view rendered alone) in ONE persistent launch: every wave pulls 64-pixel tiles of
  * any view from the d_work_counter queue.  The counter is reset by a kernel
  * on stream s, so the call is graph-capturable.  Output index of pixel
  * (x, y) of view v is v*w*h + y*w + x, row 0 at the bottom. */

@@ -129,13 +129,11 @@ def test_batched_rejects_bad_tiles_and_small_scratch(require_gpu):
     assert L.vrx_render_batch(C.byref(bf.scene), C.byref(bf.renderer), C.byref(bf.b), None) \
         == -2
     assert b"scratch" in L.vrx_last_error()
-    # the batched kernel runs the raycast program only: other renderers are
-    # refused, not silently rendered as raycast
+    # an unknown renderer kind is refused
     rr = N.vrx_renderer.from_buffer_copy(bf.renderer)
-    rr.kind = N.VRX_RENDER_AO
-    bf.b.scratch_bytes = 0
+    rr.kind = 7
     assert L.vrx_render_batch(C.byref(bf.scene), C.byref(rr), C.byref(bf.b), None) == -2
-    assert b"raycast" in L.vrx_last_error()
+    assert b"renderer kind" in L.vrx_last_error()
     s.release()
     dev.close()
 
@@ -161,5 +159,33 @@ def test_batched_wide_nodes_equal_per_view(require_gpu):
     bf.render()
     torch.cuda.synchronize()
     _assert_same(bf.channels(), got, "autotuned")
+    s.release()
+    dev.close()
+
+
+@pytest.mark.parametrize("kind", ["ao", "directLight"])
+def test_batched_ao_and_direct_light_equal_per_view_frames(require_gpu, kind):
+    """The AO and directLight programs in the persistent batched launch: each
+    view equals the same view rendered alone (same per-pixel PCG32 sequence)."""
+    w, h = 48, 32
+    sc, cams = _small_batched(4, (w, h))
+    sc.lights = [S.LightDesc("directional", direction=(0.3, -1.0, 0.2), irradiance=1.5),
+                 S.LightDesc("point", position=(0.0, 3.0, 0.0), intensity=1.0)]
+    from visrtx_amd import batch as B
+    dev = A.Device()
+    s = S.build_anari(dev, sc)
+    r = S.make_renderer(dev, s, kind, pixelSamples=2, ambientSamples=2,
+                        ambientOcclusionDistance=0.5, ambientRadiance=0.3)
+    bf = B.BatchFrame(B.scene_of(dev, s.world), B.renderer_of(dev, r), cams, w, h)
+    print(f"[batch] {kind} {len(cams)} views {w}x{h}", flush=True)
+    bf.render()
+    torch.cuda.synchronize()
+    got = bf.channels()
+    for v, cam in enumerate(cams):
+        ref = _per_view(dev, s, r, cam, w, h)
+        _assert_same({k: got[k][v] for k in CHANNELS}, ref, f"{kind} view {v}")
+    assert not dev.errors(), dev.errors()
+    assert got["color"][..., :3].max() > 0.05
+    bf = None
     s.release()
     dev.close()

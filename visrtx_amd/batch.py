@@ -57,7 +57,6 @@ class BatchFrame:
                  height: int, srgb: bool = False, device="cuda"):
         self.scene = scene
         self.renderer = N.vrx_renderer.from_buffer_copy(renderer)
-        self.renderer.kind = N.VRX_RENDER_RAYCAST
         if self.renderer.node_width == 4 and not scene.wide:
             self.renderer.node_width = 2   # the scene was fetched without 4-wide nodes
         structs = [camera_struct(c) for c in cams]

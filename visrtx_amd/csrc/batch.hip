@@ -32,7 +32,7 @@ __device__ __forceinline__ vrx_camera load_camera(const vrx_camera *cams, uint32
   return c;
 }
 
-template <int PT, bool TWO, bool WIDE>
+template <int PT, bool TWO, bool WIDE, int KIND>
 __global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, vrx_renderer r,
     vrx_batch b, uint32_t sd, uint32_t cap, uint32_t *scratch, uint32_t tiles_x, uint32_t tiles_y)
 {
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, v
     if (px < b.width && py < b.height) {
       const vrx_camera cam = load_camera(b.d_cams, view);
       const size_t idx = (size_t)view * b.width * b.height + (size_t)py * b.width + px;
-      render_pixel<PT, TWO, VRX_RENDER_RAYCAST, WIDE>(sc, cam, r, f, px, py, idx, st, rays, vsamples);
+      render_pixel<PT, TWO, KIND, WIDE>(sc, cam, r, f, px, py, idx, st, rays, vsamples);
     }
   }
 }
@@ -120,32 +120,42 @@ static uint32_t cap_for(const vrx_scene &sc, uint32_t sd, bool wide)
   return m > sd ? m - sd : 0u;
 }
 
-template <int PT, bool WIDE>
-static const void *kernel_ptr(bool two)
+template <int PT, bool WIDE, int KIND>
+static const void *kernel_ptr_k(bool two)
 {
-  return two ? (const void *)&k_render_batch<PT, true, WIDE>
-             : (const void *)&k_render_batch<PT, false, WIDE>;
+  return two ? (const void *)&k_render_batch<PT, true, WIDE, KIND>
+             : (const void *)&k_render_batch<PT, false, WIDE, KIND>;
+}
+
+template <int PT, bool WIDE>
+static const void *kernel_ptr(bool two, int kind)
+{
+  if (kind == VRX_RENDER_AO)
+    return kernel_ptr_k<PT, WIDE, VRX_RENDER_AO>(two);
+  if (kind == VRX_RENDER_DIRECT)
+    return kernel_ptr_k<PT, WIDE, VRX_RENDER_DIRECT>(two);
+  return kernel_ptr_k<PT, WIDE, VRX_RENDER_RAYCAST>(two);
 }
 
 template <bool WIDE>
-static const void *pick_kernel_w(const vrx_scene &sc)
+static const void *pick_kernel_w(const vrx_scene &sc, int kind)
 {
   const bool two = sc.num_entries > 1;
   switch (sc.num_volumes ? -1 : sc.single_type) { // volumes: generic kernel
   case VRX_TRIANGLE:
-    return kernel_ptr<VRX_TRIANGLE, WIDE>(two);
+    return kernel_ptr<VRX_TRIANGLE, WIDE>(two, kind);
   case VRX_SPHERE:
-    return kernel_ptr<VRX_SPHERE, WIDE>(two);
+    return kernel_ptr<VRX_SPHERE, WIDE>(two, kind);
   case VRX_CYLINDER:
-    return kernel_ptr<VRX_CYLINDER, WIDE>(two);
+    return kernel_ptr<VRX_CYLINDER, WIDE>(two, kind);
   default:
-    return kernel_ptr<-1, WIDE>(two);
+    return kernel_ptr<-1, WIDE>(two, kind);
   }
 }
 
-static const void *pick_kernel(const vrx_scene &sc, bool wide)
+static const void *pick_kernel(const vrx_scene &sc, bool wide, int kind)
 {
-  return wide ? pick_kernel_w<true>(sc) : pick_kernel_w<false>(sc);
+  return wide ? pick_kernel_w<true>(sc, kind) : pick_kernel_w<false>(sc, kind);
 }
 
 __global__ void k_reset_counter(uint32_t *c)
@@ -161,7 +171,7 @@ static int launch_batch(const vrx_scene &sc, const vrx_renderer &r, const vrx_ba
   const uint32_t sd = pick_sd(r.stack_lds);
   const uint32_t cap = cap_for(sc, sd, wide);
   const size_t lds = (size_t)sd * block * 4;
-  const void *kern = pick_kernel(sc, wide);
+  const void *kern = pick_kernel(sc, wide, r.kind);
   int per_cu = 0;
   VRX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds));
   if (per_cu < 1)
@@ -188,7 +198,8 @@ static int validate(const vrx_scene *scene, const vrx_renderer *r, const vrx_bat
   VRX_ARG(scene->num_entries == 0 || (scene->d_entries && scene->d_geoms), "scene arrays");
   VRX_ARG(scene->num_entries <= 1 || scene->tlas.d_leaf_ids, "instance BVH not built");
   VRX_ARG(scene->num_volumes == 0 || scene->d_volumes, "volume array");
-  VRX_ARG(r->kind == VRX_RENDER_RAYCAST, "batched views run the raycast renderer only");
+  VRX_ARG(r->kind >= VRX_RENDER_RAYCAST && r->kind <= VRX_RENDER_DIRECT, "renderer kind");
+  VRX_ARG(scene->num_lights == 0 || scene->d_lights, "light array");
   VRX_ARG(r->node_width == 0 || r->node_width == 2 || r->node_width == 4, "node width 2 or 4");
   VRX_ARG(r->node_width != 4 || scene->num_entries == 0 || scene->wide,
       "node width 4 needs 4-wide nodes for every BVH (scene->wide)");

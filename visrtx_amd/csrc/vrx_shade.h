@@ -562,8 +562,10 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
   const int aos = min(max(r.ambient_samples, 0), 256);
   const bool cull = r.cull_backfaces != 0;
   Pcg rs;
+  // the sequence is keyed by the pixel within its view, so a view rendered
+  // in a batch draws the same numbers as the same view rendered alone
   if (kSampled)
-    pcg_seed(rs, (uint64_t)idx, (uint64_t)f.frame_id);
+    pcg_seed(rs, (uint64_t)px + (uint64_t)py * f.width, (uint64_t)f.frame_id);
 
   float4 accum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float dcur = FLT_MAX;
