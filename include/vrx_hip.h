@@ -287,6 +287,16 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
  * vrx_volume_grid_cells(v) floats, x fastest; v->d_data and v->d_lut are read. */
 size_t vrx_volume_grid_cells(const vrx_volume *v);
 int vrx_volume_majorant_grid(const vrx_volume *v, float *d_majorant, vrx_stream s);
+/* Woodcock (delta) tracking of n rays through v over the majorant grid
+ * v->d_majorant (volumeIntegration.h:167-228 _sampleDistance, dda.h:43-122):
+ * per ray (origin, direction, [tmin, tmax]) the collision distance d_t (tmax
+ * if none), d_tr = 0 on a collision else 1, d_albedo (3 per ray) the LUT
+ * colour at the last sample taken.  Random numbers: PCG32 keyed (ray index,
+ * seed).  Step = min(spacing)/2, extinction = LUT alpha per step.  Fields
+ * below 2^31 voxels with dims[0] >= 2. */
+int vrx_volume_sample_distance(const vrx_volume *v, uint32_t n, const float *d_org,
+    const float *d_dir, const float *d_tminmax, uint64_t seed, float *d_t, float *d_albedo,
+    float *d_tr, vrx_stream s);
 /* Host-side camera setup (camera/Perspective.cpp:40-72). */
 int vrx_camera_perspective(const float pos[3], const float dir[3], const float up[3],
     float fovy, float aspect, const float region[4], vrx_camera *out);

@@ -2123,3 +2123,153 @@ double orc_diffvol_loss(const uint32_t dims[3], const float origin[3], const flo
   }
   return total;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Macro-cell majorants and Woodcock tracking                                 */
+/* ------------------------------------------------------------------------- */
+/* Majorant grid of 16^3-voxel cells (scene/volume/space_skipping/
+   UniformGrid.cu:44-258 in layout; the value range is conservative here):
+   min / max over each cell's voxels plus a one-voxel halo (NaN ignored), then
+   the largest LUT alpha over the entries that range classifies to, with the
+   device's fp32 clamp / scale / floor. */
+void orc_majorant_grid(const orc_volume *v, float *out)
+{
+  const int m[3] = {((int)v->dims[0] + 15) / 16, ((int)v->dims[1] + 15) / 16,
+      ((int)v->dims[2] + 15) / 16};
+  for (int cz = 0; cz < m[2]; cz++)
+    for (int cy = 0; cy < m[1]; cy++)
+      for (int cx = 0; cx < m[0]; cx++) {
+        const int c[3] = {cx, cy, cz};
+        int b0[3], b1[3];
+        for (int a = 0; a < 3; a++) {
+          b0[a] = 16 * c[a] - 1 < 0 ? 0 : 16 * c[a] - 1;
+          b1[a] = 16 * c[a] + 17 > (int)v->dims[a] - 1 ? (int)v->dims[a] - 1 : 16 * c[a] + 17;
+        }
+        float lo = INFINITY, hi = -INFINITY;
+        for (int z = b0[2]; z <= b1[2]; z++)
+          for (int y = b0[1]; y <= b1[1]; y++)
+            for (int x = b0[0]; x <= b1[0]; x++) {
+              float val = v->data[(size_t)x + (size_t)v->dims[0] * ((size_t)y +
+                  (size_t)v->dims[1] * (size_t)z)];
+              lo = fminf(lo, val);
+              hi = fmaxf(hi, val);
+            }
+        float mj = 0.0f;
+        if (lo <= hi) {
+          const float vlo = v->value_range[0], vhi = v->value_range[1];
+          const float inv = 1.0f / (vhi - vlo);
+          int e[2];
+          const float ends[2] = {lo, hi};
+          for (int q = 0; q < 2; q++) {
+            float cl = fminf(fmaxf(ends[q], vlo), vhi);
+            float x = fminf(fmaxf((cl - vlo) * inv * 256.0f - 0.5f, 0.0f), 255.0f);
+            e[q] = (int)x < 254 ? (int)x : 254;
+          }
+          int i0 = e[0] < e[1] ? e[0] : e[1], i1 = (e[0] > e[1] ? e[0] : e[1]) + 1;
+          for (int i = i0; i <= i1; i++)
+            mj = fmaxf(mj, v->lut[4 * i + 3]);
+        }
+        out[((size_t)cz * m[1] + cy) * m[0] + cx] = mj;
+      }
+}
+
+/* Woodcock tracking (volumeIntegration.h:167-228 _sampleDistance over the
+   cells of dda.h:43-122): in a cell of majorant m, tentative collisions at
+   t -= log(1 - u) / m * step; real when the LUT alpha there >= u' m.  The
+   stepping is fp32 as on the device; field samples and classification are
+   FP64.  PCG32 keyed (ray index, seed). */
+void orc_sample_distance(const orc_volume *v, const float *majorant, uint32_t n,
+    const float *org, const float *dir, const float *tmm, uint64_t seed, float *t_out,
+    float *albedo, float *tr_out)
+{
+  const int m[3] = {((int)v->dims[0] + 15) / 16, ((int)v->dims[1] + 15) / 16,
+      ((int)v->dims[2] + 15) / 16};
+  float sp2 = v->spacing[0] / 2.0f;
+  if (v->spacing[1] / 2.0f < sp2)
+    sp2 = v->spacing[1] / 2.0f;
+  if (v->spacing[2] / 2.0f < sp2)
+    sp2 = v->spacing[2] / 2.0f;
+  const float step = sp2 * 1.0f;
+  double lutd[1024];
+  for (int i = 0; i < 1024; i++)
+    lutd[i] = v->lut[i];
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const float *o = org + 3 * i, *d = dir + 3 * i;
+    float tout = tmm[2 * i + 1], tr = 1.0f, alb[3] = {0.0f, 0.0f, 0.0f};
+    /* the interval clip in fp32, as the device's vol_slab */
+    float tn = -INFINITY, tf = INFINITY;
+    for (int a = 0; a < 3; a++) {
+      float lo = v->origin[a], hi = v->origin[a] + ((float)v->dims[a] - 1.0f) * v->spacing[a];
+      float inv = 1.0f / d[a];
+      float ta = (lo - o[a]) * inv, tb = (hi - o[a]) * inv;
+      tn = fmaxf(tn, fminf(ta, tb));
+      tf = fminf(tf, fmaxf(ta, tb));
+    }
+    const float tmin = tmm[2 * i], tmax = tmm[2 * i + 1];
+    if (tn < tf && !(tf < tmin || tn > tmax)) {
+      const float t0 = fminf(fmaxf(tn, tmin), tmax), t1 = fminf(fmaxf(tf, tmin), tmax);
+      float g0[3], gd[3];
+      for (int a = 0; a < 3; a++) {
+        float inv_sp = 1.0f / v->spacing[a];
+        g0[a] = (o[a] - v->origin[a]) * inv_sp;
+        gd[a] = d[a] * inv_sp;
+      }
+      orc_pcg rs;
+      orc_pcg_seed(&rs, (uint64_t)i, seed);
+      float t = t0;
+      const int max_cells = 3 * (m[0] + m[1] + m[2]) + 8;
+      int hit = 0;
+      for (int cell = 0; cell < max_cells && t < t1 && !hit; cell++) {
+        float tm = fminf(t + 1e-3f * step, 0.5f * (t + t1));
+        int ci[3];
+        for (int a = 0; a < 3; a++) {
+          int q = (int)floorf((g0[a] + gd[a] * tm) * 0.0625f);
+          ci[a] = q < 0 ? 0 : (q > m[a] - 1 ? m[a] - 1 : q);
+        }
+        float te = t1;
+        for (int a = 0; a < 3; a++) {
+          if (gd[a] > 0.0f)
+            te = fminf(te, (16.0f * (float)(ci[a] + 1) - g0[a]) / gd[a]);
+          else if (gd[a] < 0.0f)
+            te = fminf(te, (16.0f * (float)ci[a] - g0[a]) / gd[a]);
+        }
+        if (!(te > t))
+          te = tm;
+        const float mj = majorant[((size_t)ci[2] * m[1] + ci[1]) * m[0] + ci[0]];
+        if (mj > 0.0f) {
+          for (int k = 0; k < 4096; k++) {
+            t -= logf(1.0f - orc_pcg_float(&rs)) / mj * step;
+            if (t >= te)
+              break;
+            double p[3];
+            for (int a = 0; a < 3; a++)
+              p[a] = (double)v->origin[a] + ((double)g0[a] + (double)gd[a] * (double)t) *
+                  (double)v->spacing[a];
+            double s = orc_sample_field(v, p);
+            if (s == s) {
+              double rgba[4];
+              classify_lut(lutd, v->lut, v->value_range, s, rgba, NULL, NULL, NULL);
+              alb[0] = (float)rgba[0];
+              alb[1] = (float)rgba[1];
+              alb[2] = (float)rgba[2];
+              float u = orc_pcg_float(&rs);
+              if ((float)rgba[3] >= u * mj) {
+                tr = 0.0f;
+                tout = t;
+                hit = 1;
+                break;
+              }
+            }
+          }
+        }
+        t = te;
+      }
+    }
+    t_out[i] = tout;
+    tr_out[i] = tr;
+    albedo[3 * i] = alb[0];
+    albedo[3 * i + 1] = alb[1];
+    albedo[3 * i + 2] = alb[2];
+  }
+}

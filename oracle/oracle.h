@@ -189,6 +189,13 @@ typedef struct {
 
 int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r, orc_frame *f);
 
+/* Macro-cell majorant grid (ceil(dims/16) cells, x fastest) and Woodcock
+   distance sampling over it (see oracle.c). */
+void orc_majorant_grid(const orc_volume *v, float *out);
+void orc_sample_distance(const orc_volume *v, const float *majorant, uint32_t n,
+    const float *org, const float *dir, const float *tmm, uint64_t seed, float *t_out,
+    float *albedo, float *tr_out);
+
 /* Closest-hit query on a built scene (world space).  Returns 1 on hit. */
 typedef struct {
   float t;

@@ -130,6 +130,11 @@ def lib() -> C.CDLL:
         L.orc_trace_closest.argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_int, vp, vp]
         L.orc_trace_brute.argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_int, vp]
         L.orc_trace_any.argtypes = [vp, vp, vp, C.c_float, C.c_float, vp]
+        L.orc_majorant_grid.argtypes = [vp, vp]
+        L.orc_majorant_grid.restype = None
+        L.orc_sample_distance.argtypes = [vp, vp, C.c_uint32, vp, vp, vp, C.c_uint64, vp, vp,
+                                          vp]
+        L.orc_sample_distance.restype = None
         L.orc_surface_attenuation.argtypes = [vp, vp, vp, C.c_float, C.c_float]
         L.orc_surface_attenuation.restype = C.c_float
         L.orc_trace_rays.argtypes = [vp, C.c_uint32, vp, vp, C.c_float, C.c_float, vp, vp, vp, vp]
@@ -519,6 +524,49 @@ def volume_image(vol_desc, cam, width, height, inv_sampling_rate=1.0, early_stop
                            amb.ctypes.data)
     volume_image.last_ambiguous = amb
     return out, ns.value
+
+
+def _volume_struct(vol_desc):
+    """orc_volume of a scenes.VolumeDesc plus the arrays it points at."""
+    v = orc_volume()
+    nz, ny, nx = vol_desc.data.shape
+    v.dims[:] = [nx, ny, nz]
+    v.origin[:] = list(vol_desc.origin)
+    v.spacing[:] = list(vol_desc.spacing)
+    d = np.ascontiguousarray(vol_desc.data, np.float32)
+    lut = tf_discretize(vol_desc.color, vol_desc.opacity, vol_desc.value_range)
+    v.data = d.ctypes.data
+    v.lut = lut.ctypes.data
+    v.value_range[:] = list(vol_desc.value_range)
+    v.unit_distance = vol_desc.unit_distance
+    return v, (d, lut)
+
+
+def majorant_grid(vol_desc) -> np.ndarray:
+    """(mz, my, mx) float32 macro-cell majorants (16^3 voxels per cell)."""
+    v, keep = _volume_struct(vol_desc)
+    m = [(int(x) + 15) // 16 for x in v.dims]
+    out = np.zeros((m[2], m[1], m[0]), np.float32)
+    lib().orc_majorant_grid(C.byref(v), out.ctypes.data)
+    return out
+
+
+def sample_distance(vol_desc, org, dirs, tminmax, seed, majorant=None):
+    """Woodcock tracking per ray: (t, albedo (n, 3), tr)."""
+    v, keep = _volume_struct(vol_desc)
+    mg = majorant_grid(vol_desc) if majorant is None else np.ascontiguousarray(majorant,
+                                                                                np.float32)
+    o = np.ascontiguousarray(org, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    tm = np.ascontiguousarray(tminmax, np.float32)
+    n = len(o)
+    t = np.empty(n, np.float32)
+    alb = np.empty((n, 3), np.float32)
+    tr = np.empty(n, np.float32)
+    lib().orc_sample_distance(C.byref(v), mg.ctypes.data, n, o.ctypes.data, d.ctypes.data,
+                              tm.ctypes.data, C.c_uint64(seed), t.ctypes.data, alb.ctypes.data,
+                              tr.ctypes.data)
+    return t, alb, tr
 
 
 def pack_unorm4x8(v) -> int:

@@ -16,39 +16,11 @@ import pytest
 from oracle import oracle as orc
 from tests.devmem import DeviceBuffer
 from tests.test_gpu_render import render_gpu
+from tests.test_oracle_majorants import grid_reference as _grid_reference
 from visrtx_amd import _native as N
 from visrtx_amd import scenes as S
 
 pytestmark = pytest.mark.gpu
-
-
-def _grid_reference(data, lut, vr):
-    """numpy restatement of the conservative majorant grid."""
-    nz, ny, nx = data.shape
-    m = [(n + 15) // 16 for n in (nx, ny, nz)]
-    out = np.zeros((m[2], m[1], m[0]), np.float32)
-    f = np.float32
-    lo_v, hi_v = f(vr[0]), f(vr[1])
-    inv = f(1.0) / f(hi_v - lo_v)
-
-    def entry(v):
-        cl = min(max(f(v), lo_v), hi_v)
-        x = min(max(f(f(f(cl - lo_v) * inv) * f(256.0)) - f(0.5), f(0.0)), f(255.0))
-        return min(int(x), 254)
-
-    for cz in range(m[2]):
-        for cy in range(m[1]):
-            for cx in range(m[0]):
-                blk = data[max(16 * cz - 1, 0):min(16 * cz + 17, nz - 1) + 1,
-                           max(16 * cy - 1, 0):min(16 * cy + 17, ny - 1) + 1,
-                           max(16 * cx - 1, 0):min(16 * cx + 17, nx - 1) + 1]
-                fin = blk[~np.isnan(blk)]
-                if fin.size == 0:
-                    continue
-                e0, e1 = entry(fin.min()), entry(fin.max())
-                i0, i1 = min(e0, e1), max(e0, e1) + 1
-                out[cz, cy, cx] = lut.reshape(-1, 4)[i0:i1 + 1, 3].max()
-    return out
 
 
 @pytest.mark.parametrize("shape", [(40, 33, 50), (16, 16, 16), (1, 20, 17)])
@@ -139,3 +111,54 @@ def test_skipping_is_exact_with_surfaces_and_lights(require_gpu):
     assert ok.mean() > 0.98
     np.testing.assert_allclose(on["color"][ok], o["color"][ok], rtol=0, atol=1e-4)
     np.testing.assert_array_equal(on["primitiveId"], o["primitiveId"])
+
+
+def test_woodcock_sampling_matches_oracle(require_gpu):
+    """vrx_volume_sample_distance against the oracle's restatement on the same
+    rays and PCG32 streams: the collision flag and distance agree on all but
+    the rare rays whose fp32 / FP64 field samples fall on opposite sides of
+    an acceptance threshold."""
+    n = 80
+    data = S.gen_noise_volume(n, seed=3)
+    data[:, :, : n // 3] = 0.0                                 # an empty third
+    sp = 1.0 / (n - 1)
+    vol = S.VolumeDesc(data, spacing=(sp, sp, sp), color=S.COLORMAP5,
+                       opacity=np.array([0.0, 0.0, 0.05, 0.3], np.float32))
+    lut = orc.tf_discretize(vol.color, vol.opacity, vol.value_range)
+    rng = np.random.default_rng(4)
+    m = 20000
+    org = (rng.random((m, 3)) * 3.0 - 1.0).astype(np.float32)
+    tgt = rng.random((m, 3)).astype(np.float32)
+    dirs = tgt - org
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    tmm = np.tile(np.array([0.0, 1e30], np.float32), (m, 1))
+    tmm[::7, 1] = 1.5                                          # some bounded rays
+    L = N.hip()
+    db, lb = DeviceBuffer.from_array(data), DeviceBuffer.from_array(lut)
+    v = N.vrx_volume()
+    v.d_data, v.d_lut = db.addr, lb.addr
+    v.dims[:] = [n, n, n]
+    v.spacing[:] = [sp, sp, sp]
+    v.value_range[:] = [0.0, 1.0]
+    v.unit_distance = 1.0
+    cells = L.vrx_volume_grid_cells(C.byref(v))
+    gb = DeviceBuffer(4 * cells)
+    N.check(L.vrx_volume_majorant_grid(C.byref(v), gb.ptr, None), "grid")
+    v.d_majorant = gb.addr
+    bufs = [DeviceBuffer.from_array(a) for a in (org, dirs, tmm)]
+    out_t, out_a, out_tr = DeviceBuffer(4 * m), DeviceBuffer(12 * m), DeviceBuffer(4 * m)
+    print(f"[woodcock] {m} rays through {n}^3", flush=True)
+    N.check(L.vrx_volume_sample_distance(C.byref(v), m, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
+                                         C.c_uint64(9), out_t.ptr, out_a.ptr, out_tr.ptr, None),
+            "sample_distance")
+    N.check(L.vrx_device_sync(), "sync")
+    g_t, g_tr = out_t.to_array(np.float32, m), out_tr.to_array(np.float32, m)
+    g_grid = gb.to_array(np.float32, cells)
+    np.testing.assert_array_equal(g_grid, orc.majorant_grid(vol).reshape(-1))
+    o_t, o_a, o_tr = orc.sample_distance(vol, org, dirs, tmm, seed=9)
+    same = (g_tr == o_tr) & np.isclose(g_t, o_t, rtol=1e-4, atol=1e-6)
+    print(f"agree {same.mean():.5f}, collisions {(g_tr == 0).mean():.3f}")
+    assert same.mean() > 0.995
+    assert 0.05 < (g_tr == 0).mean() < 0.95
+    for b in [db, lb, gb, out_t, out_a, out_tr] + bufs:
+        b.free()

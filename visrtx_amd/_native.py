@@ -177,6 +177,7 @@ def hip() -> C.CDLL:
             "vrx_camera_perspective": (i32, [vp, vp, vp, C.c_float, C.c_float, vp, vp]),
             "vrx_volume_grid_cells": (sz, [vp]),
             "vrx_volume_majorant_grid": (i32, [vp, vp, vp]),
+            "vrx_volume_sample_distance": (i32, [vp, u32, vp, vp, vp, C.c_uint64, vp, vp, vp, vp]),
             "vrx_tlas_query_sizes": (i32, [u32, vp]),
             "vrx_tlas_build": (i32, [vp, vp, vp, sz, vp]),
             "vrx_diffvol_forward": (i32, [vp, u32, vp, vp, vp, vp, vp]),

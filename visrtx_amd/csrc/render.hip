@@ -215,6 +215,89 @@ __global__ __launch_bounds__(256) void k_majorants(vrx_volume v, float *out)
   out[cell] = m;
 }
 
+// Woodcock (delta) tracking of one ray per lane through a volume with a
+// majorant grid (volumeIntegration.h:167-228 with dda.h:43-122): the ray
+// walks the 16^3-voxel macro cells it crosses; in a cell of majorant m it
+// takes tentative collisions at exponential distances of mean step / m
+// (step = the field's step, min(spacing)/2), and a tentative collision is a
+// real one when the LUT alpha there is at least u * m.  PCG32 per ray,
+// keyed (ray index, seed).  t_out is the collision distance (or the end of
+// the ray's interval), tr 0 on a collision else 1, albedo the LUT colour at
+// the last sample taken.  Loops are bounded by the cell count and by a cap
+// on tentative collisions per cell.
+__global__ __launch_bounds__(256) void k_sample_distance(vrx_volume v, uint32_t n,
+    const float *org, const float *dir, const float *tmm, uint64_t seed, float *out_t,
+    float *out_albedo, float *out_tr)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const v3 o = mk(gld(org, 3 * (size_t)i), gld(org, 3 * (size_t)i + 1), gld(org, 3 * (size_t)i + 2));
+  const v3 d = mk(gld(dir, 3 * (size_t)i), gld(dir, 3 * (size_t)i + 1), gld(dir, 3 * (size_t)i + 2));
+  const float tmin = gld(tmm, 2 * (size_t)i), tmax = gld(tmm, 2 * (size_t)i + 1);
+  float t_out = tmax, tr = 1.0f;
+  float3 alb = make_float3(0.0f, 0.0f, 0.0f);
+  float t0, t1;
+  if (vol_slab(v, o, d, tmin, tmax, t0, t1)) {
+    const VolCtx c = vol_ctx(v, 1.0f, false);
+    const float step = c.step;
+    const float *mc = v.d_majorant;
+    const float g0[3] = {(o.x - c.origin[0]) * c.inv_sp[0], (o.y - c.origin[1]) * c.inv_sp[1],
+        (o.z - c.origin[2]) * c.inv_sp[2]};
+    const float gd[3] = {d.x * c.inv_sp[0], d.y * c.inv_sp[1], d.z * c.inv_sp[2]};
+    Pcg rs;
+    pcg_seed(rs, (uint64_t)i, seed);
+    float t = t0;
+    const int max_cells = 3 * (c.mcd[0] + c.mcd[1] + c.mcd[2]) + 8;
+    bool hit = false;
+    for (int cell = 0; cell < max_cells && t < t1 && !hit; cell++) {
+      // the cell of the segment starting at t: the point a little past t
+      const float tm = fminf(t + 1e-3f * step, 0.5f * (t + t1));
+      int ci[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        ci[a] = min(max((int)floorf((g0[a] + gd[a] * tm) * 0.0625f), 0), c.mcd[a] - 1);
+      float te = t1;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        if (gd[a] > 0.0f)
+          te = fminf(te, (16.0f * (float)(ci[a] + 1) - g0[a]) / gd[a]);
+        else if (gd[a] < 0.0f)
+          te = fminf(te, (16.0f * (float)ci[a] - g0[a]) / gd[a]);
+      }
+      if (!(te > t))
+        te = tm; // a cell whose exit rounds to t: step past it
+      const float m = gld(mc + ((size_t)ci[2] * c.mcd[1] + ci[1]) * c.mcd[0] + ci[0]);
+      if (m > 0.0f) {
+        for (int k = 0; k < 4096; k++) {
+          t -= logf(1.0f - pcg_float(rs)) / m * step;
+          if (t >= te)
+            break;
+          const float s = trilinear<false>(c, g0[0] + gd[0] * t, g0[1] + gd[1] * t,
+              g0[2] + gd[2] * t);
+          if (s == s) {
+            const float4 co = classify<true>((const float4 *)v.d_lut, c, s);
+            alb = make_float3(co.x, co.y, co.z);
+            const float u = pcg_float(rs);
+            if (co.w >= u * m) {
+              tr = 0.0f;
+              t_out = t;
+              hit = true;
+              break;
+            }
+          }
+        }
+      }
+      t = te;
+    }
+  }
+  out_t[i] = t_out;
+  out_tr[i] = tr;
+  out_albedo[3 * (size_t)i] = alb.x;
+  out_albedo[3 * (size_t)i + 1] = alb.y;
+  out_albedo[3 * (size_t)i + 2] = alb.z;
+}
+
 __global__ void k_entry_boxes(vrx_scene sc, float *boxes)
 {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -423,6 +506,22 @@ int vrx_volume_majorant_grid(const vrx_volume *v, float *d_majorant, vrx_stream 
   VRX_ARG(cells < (1u << 31), "too many macro cells");
   hipLaunchKernelGGL(k_majorants, dim3((unsigned)cells), dim3(256), 0, (hipStream_t)s, *v,
       d_majorant);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
+}
+
+int vrx_volume_sample_distance(const vrx_volume *v, uint32_t n, const float *d_org,
+    const float *d_dir, const float *d_tminmax, uint64_t seed, float *d_t, float *d_albedo,
+    float *d_tr, vrx_stream s)
+{
+  VRX_ARG(v && v->d_data && v->d_lut && v->d_majorant, "volume with data, LUT and majorant grid");
+  VRX_ARG(v->dims[0] >= 2 && v->dims[1] >= 1 && v->dims[2] >= 1, "volume dims");
+  VRX_ARG((uint64_t)v->dims[0] * v->dims[1] * v->dims[2] < (1ull << 31), "volume too large");
+  VRX_ARG(n == 0 || (d_org && d_dir && d_tminmax && d_t && d_albedo && d_tr), "ray arrays");
+  if (n == 0)
+    return VRX_OK;
+  hipLaunchKernelGGL(k_sample_distance, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)s, *v,
+      n, d_org, d_dir, d_tminmax, seed, d_t, d_albedo, d_tr);
   VRX_LAUNCH_CHECK();
   return VRX_OK;
 }
