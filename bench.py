@@ -274,6 +274,64 @@ def bench_diffvol(iters=200, log_every=50):
             "iters": iters, "mse_first": first, "mse_final": last, "rays": int(org.shape[0])}
 
 
+def bench_woodcock(vd, cam, w, h, reps=10):
+    """Woodcock distance sampling (vrx_volume_sample_distance) of one camera
+    ray per pixel through the volume, over its majorant grid."""
+    import ctypes as C
+    import torch
+    from visrtx_amd import _native as N
+    from visrtx_amd import batch as B
+    from oracle import oracle as orc
+    L = N.hip()
+    data = torch.from_numpy(np.ascontiguousarray(vd.data)).cuda()
+    lut = torch.from_numpy(orc.tf_discretize(vd.color, vd.opacity, vd.value_range)).cuda()
+    v = N.vrx_volume()
+    v.d_data, v.d_lut = data.data_ptr(), lut.data_ptr()
+    nz, ny, nx = vd.data.shape
+    v.dims[:] = [nx, ny, nz]
+    v.origin[:] = list(vd.origin)
+    v.spacing[:] = list(vd.spacing)
+    v.value_range[:] = list(vd.value_range)
+    v.unit_distance = vd.unit_distance
+    cells = L.vrx_volume_grid_cells(C.byref(v))
+    grid = torch.empty(cells, device="cuda", dtype=torch.float32)
+    N.check(L.vrx_volume_majorant_grid(C.byref(v), C.c_void_p(grid.data_ptr()), None), "grid")
+    v.d_majorant = grid.data_ptr()
+    n = w * h
+    cs = B.camera_struct(cam)
+    org = torch.tensor(list(cs.pos), device="cuda").expand(n, 3).contiguous()
+    ys, xs = torch.meshgrid(torch.arange(h, device="cuda"), torch.arange(w, device="cuda"),
+                            indexing="ij")
+    sx = (xs.reshape(-1).float() + 0.5) / w
+    sy = (ys.reshape(-1).float() + 0.5) / h
+    d00, du, dv = (torch.tensor(list(x), device="cuda") for x in (cs.d00, cs.du, cs.dv))
+    dirs = d00[None] + sx[:, None] * du[None] + sy[:, None] * dv[None]
+    dirs = (dirs / dirs.norm(dim=1, keepdim=True)).contiguous()
+    tmm = torch.tensor([0.0, 1e30], device="cuda").expand(n, 2).contiguous()
+    t = torch.empty(n, device="cuda")
+    alb = torch.empty((n, 3), device="cuda")
+    tr = torch.empty(n, device="cuda")
+
+    def run(seed):
+        N.check(L.vrx_volume_sample_distance(
+            C.byref(v), n, C.c_void_p(org.data_ptr()), C.c_void_p(dirs.data_ptr()),
+            C.c_void_p(tmm.data_ptr()), C.c_uint64(seed), C.c_void_p(t.data_ptr()),
+            C.c_void_p(alb.data_ptr()), C.c_void_p(tr.data_ptr()), None), "sample_distance")
+
+    run(0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for k in range(reps):
+        run(k + 1)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"rays": n, "ms": round(ms, 4), "mrays_s": round(n / (ms / 1000) / 1e6, 2),
+            "collision_fraction": round(float((tr == 0).float().mean()), 4),
+            "empty_cell_fraction": round(float((grid == 0).float().mean()), 4)}
+
+
 def bench_batched(dev, reps=5):
     """64 orbit views at 512x512 of 256 instances of 16 meshes (10k-2M
     triangles): one autotuned persistent launch (eager and HIP-graph replay)
@@ -589,6 +647,8 @@ def main():
             "samples_no_skip": int(full), "samples_skip": int(res[True][1]),
             "effective_gsamples_s_skip": round(full / (res[True][0] / 1000) / 1e9, 2)}
         log(f"sparse volume: {res[False][0]:.3f} ms -> {res[True][0]:.3f} ms with skipping")
+        extra["woodcock_sparse_512"] = bench_woodcock(sc4.volumes[0], cam4, W, H)
+        log(f"woodcock: {extra['woodcock_sparse_512']['mrays_s']} Mrays/s")
         del sc4
 
     if "batched" in sec:
