@@ -102,6 +102,13 @@ def test_woodcock_transmittance_expectation(split):
     hit = tr == 0.0
     assert (t[hit] >= 0.5 - 1e-5).all() and (t[hit] <= 1.5 + 1e-5).all()
     assert (t[~hit] == 1e30).all()
+    if not split:
+        # free paths in the homogeneous slab: exponential with rate
+        # sigma = alpha / step, truncated at the slab length 1
+        sig, x = tau, t[hit] - 0.5
+        mean = 1.0 / sig - math.exp(-sig) / (1.0 - math.exp(-sig))
+        var = 1.0 / sig ** 2 - math.exp(-sig) / (1.0 - math.exp(-sig)) ** 2
+        assert abs(x.mean() - mean) < 4.0 * math.sqrt(var / len(x)), (x.mean(), mean)
 
 
 def test_woodcock_skips_empty_cells_and_misses():
