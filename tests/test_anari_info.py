@@ -122,6 +122,9 @@ def test_defaults_and_clamps(lib):
     assert default(lib, A.LIGHT, b"directional", "direction", A.FLOAT32_VEC3, C.c_float, 3) == \
         [0.0, 0.0, -1.0]
     assert params(lib, A.WORLD, None)["light"] == A.ARRAY1D
+    for name in (b"transform", b"id"):   # transform / id arrays (Instance.cpp:115-128)
+        assert lib.anariGetParameterInfo(None, A.INSTANCE, b"transform", name, A.ARRAY1D,
+                                         b"description", ANARI_STRING)
     assert params(lib, A.GROUP, None)["light"] == A.ARRAY1D
     # wrong parameter type or unknown names answer null
     assert not lib.anariGetParameterInfo(None, A.GEOMETRY, b"sphere", b"radius", A.INT32,

@@ -144,7 +144,11 @@ std::vector<ObjDef> &tables()
       {ANARI_INSTANCE, "transform", "a group under an affine transform (Instance.cpp)",
           {{"group", ANARI_GROUP, true, nullptr, nullptr, nullptr, "the group"},
               {"transform", ANARI_FLOAT32_MAT4, false, kIdentity4, nullptr, nullptr, "object to world"},
-              {"id", ANARI_UINT32, false, &kNoId, nullptr, nullptr, "instanceId value"}}},
+              {"transform", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr,
+                  "FLOAT32_MAT4 array: one instance of the group per element"},
+              {"id", ANARI_UINT32, false, &kNoId, nullptr, nullptr, "instanceId value"},
+              {"id", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr,
+                  "UINT32 array: instanceId per transform-array element"}}},
       {ANARI_WORLD, nullptr, "the scene",
           {{"surface", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "SURFACE array"},
               {"volume", ANARI_ARRAY1D, false, nullptr, nullptr, nullptr, "VOLUME array"},
