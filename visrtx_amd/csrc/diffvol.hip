@@ -247,16 +247,12 @@ __device__ __forceinline__ void lut_enter(Flush flush, LutAcc &L, int i)
     return;
   const bool up = L.i >= 0 && i == L.i + 1;   // entry L.i+1 is kept as the new lo
   const bool down = L.i >= 0 && i == L.i - 1; // entry L.i is kept as the new hi
-  if (L.i >= 0) {
-    float flo[4], fhi[4];
-#pragma unroll
-    for (int ch = 0; ch < 4; ch++) {
-      flo[ch] = down ? 0.0f : L.lo[ch];
-      fhi[ch] = up ? 0.0f : L.hi[ch];
-    }
-    flush(L.i, flo);
-    flush(L.i + 1, fhi);
-  }
+  // the entry that moves with the interval is not flushed (its share would
+  // add zero); only lanes with something to add issue the LDS atomics
+  if (L.i >= 0 && !down)
+    flush(L.i, L.lo);
+  if (L.i >= 0 && !up)
+    flush(L.i + 1, L.hi);
 #pragma unroll
   for (int ch = 0; ch < 4; ch++) {
     const float nlo = up ? L.hi[ch] : 0.0f, nhi = down ? L.lo[ch] : 0.0f;
