@@ -5,11 +5,19 @@
  *
  * Function names, argument order and meaning follow the Khronos ANARI 1.0
  * specification, which is what VisRTX's device implements through the SDK's
- * helium::BaseDevice (devices/rtx/VisRTXDevice.h:56-121).  The numeric enum
- * values below are restated from the specification's type list; those for
- * the box/matrix types (2004+) could not be checked against an SDK install
- * offline.  An application compiled against the real SDK headers must use
- * the same values for the types it passes.
+ * helium::BaseDevice (devices/rtx/VisRTXDevice.h:56-121).
+ *
+ * Source of the numeric values.  Neither the VisRTX tree nor this image holds
+ * an ANARI-SDK header (anari_enums.h) or any file with the numbers, so none
+ * of them is checked against one.  Each group below is marked:
+ *   [spec]        restated from the ANARI 1.0 specification's enumerations
+ *                 (type list order, severity and status tables) from memory;
+ *   [unverified]  as [spec], but the position in the type list is uncertain
+ *                 (the box, region and matrix types after the sRGB types).
+ * An application compiled against the real SDK headers must pass the same
+ * values; a mismatch shows as "unsupported type" warnings or ignored
+ * parameters, never as memory corruption, because every parameter is
+ * checked against the type it is read as.
  */
 #ifndef VRX_ANARI_H
 #define VRX_ANARI_H
@@ -23,6 +31,12 @@ extern "C" {
 
 #define ANARI_INTERFACE __attribute__((visibility("default")))
 
+#ifdef __cplusplus
+/* as in the SDK's C++ view of the handles: a device handle converts to an
+   object handle (anariSetParameter(device, device, ...)) */
+struct _ANARIObject {};
+struct _ANARIDevice : _ANARIObject {};
+#endif
 typedef struct _ANARILibrary *ANARILibrary;
 typedef struct _ANARIDevice *ANARIDevice;
 typedef struct _ANARIObject *ANARIObject;
@@ -55,7 +69,7 @@ typedef int ANARIStatusSeverity;
 typedef int ANARIStatusCode;
 typedef int ANARIWaitMask;
 
-/* data types */
+/* data types [spec] */
 #define ANARI_UNKNOWN 0
 #define ANARI_DATA_TYPE 100
 #define ANARI_STRING 101
@@ -113,6 +127,7 @@ typedef int ANARIWaitMask;
 #define ANARI_UFIXED8_RA_SRGB 2001
 #define ANARI_UFIXED8_RGB_SRGB 2002
 #define ANARI_UFIXED8_RGBA_SRGB 2003
+/* [unverified] box and matrix types */
 #define ANARI_FLOAT32_BOX1 2004
 #define ANARI_FLOAT32_BOX2 2005
 #define ANARI_FLOAT32_BOX3 2006
@@ -121,7 +136,7 @@ typedef int ANARIWaitMask;
 #define ANARI_FLOAT32_MAT4 2026
 #define ANARI_FLOAT32_MAT3x4 2028
 
-/* status */
+/* status callback severities [spec]: numbered from 1 (fatal) to 6 (debug) */
 #define ANARI_SEVERITY_FATAL_ERROR 1
 #define ANARI_SEVERITY_ERROR 2
 #define ANARI_SEVERITY_WARNING 3
@@ -129,6 +144,7 @@ typedef int ANARIWaitMask;
 #define ANARI_SEVERITY_INFO 5
 #define ANARI_SEVERITY_DEBUG 6
 
+/* status codes [spec] */
 #define ANARI_STATUS_NO_ERROR 0
 #define ANARI_STATUS_UNKNOWN_ERROR 1
 #define ANARI_STATUS_INVALID_ARGUMENT 2
@@ -137,6 +153,7 @@ typedef int ANARIWaitMask;
 #define ANARI_STATUS_UNSUPPORTED_DEVICE 5
 #define ANARI_STATUS_VERSION_MISMATCH 6
 
+/* wait masks [spec] */
 #define ANARI_NO_WAIT 0
 #define ANARI_WAIT 1
 

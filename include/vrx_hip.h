@@ -67,6 +67,12 @@ int vrx_malloc(void **d_ptr, size_t bytes);
 int vrx_free(void *d_ptr);
 int vrx_host_malloc(void **h_ptr, size_t bytes);
 int vrx_host_free(void *h_ptr);
+/* Where an application pointer lives (array/Array.cpp:38-65
+ * getPointerAddressSpace): host (pageable or pinned), device memory of the
+ * current device, managed memory, or device memory of another device. */
+enum vrx_pointer_kind { VRX_PTR_HOST = 0, VRX_PTR_DEVICE = 1, VRX_PTR_MANAGED = 2,
+  VRX_PTR_FOREIGN = 3 };
+int vrx_pointer_kind(const void *p, int *kind);
 int vrx_memcpy(void *dst, const void *src, size_t bytes, int kind, vrx_stream s);
 int vrx_memcpy_sync(void *dst, const void *src, size_t bytes, int kind);
 int vrx_fill_u32(void *d_ptr, uint32_t value, size_t count, vrx_stream s);

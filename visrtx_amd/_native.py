@@ -153,6 +153,7 @@ def hip() -> C.CDLL:
             "vrx_host_free": (i32, [vp]),
             "vrx_memcpy": (i32, [vp, vp, sz, i32, vp]),
             "vrx_memcpy_sync": (i32, [vp, vp, sz, i32]),
+            "vrx_pointer_kind": (i32, [vp, C.POINTER(C.c_int)]),
             "vrx_fill_u32": (i32, [vp, u32, sz, vp]),
             "vrx_copy_kernel": (i32, [vp, vp, sz, vp]),
             "vrx_stream_create": (i32, [vp]),

@@ -152,6 +152,15 @@ class Device:
         self.L.anariUnmapArray(self.h, a)
         return a
 
+    def shared_array(self, ptr: int, elem: int, n: int, deleter=None, user=None):
+        """A shared (or, with a deleter, captured) 1D array over the caller's
+        memory at ``ptr``: host, device or managed memory
+        (array/Array.cpp:38-118).  The caller keeps the memory alive."""
+        a = self.L.anariNewArray1D(self.h, C.c_void_p(ptr), deleter, user, elem, n)
+        if not a:
+            raise _native.VrxError(f"anariNewArray1D(shared) failed: {self.errors()}")
+        return a
+
     def set(self, obj, name: str, value, typ: int):
         if typ == STRING:
             buf = C.c_char_p(value.encode())
@@ -167,6 +176,10 @@ class Device:
             return
         if typ in (UINT32,):
             v = C.c_uint32(int(value))
+            self.L.anariSetParameter(self.h, obj, name.encode(), typ, C.byref(v))
+            return
+        if typ == UINT64:
+            v = C.c_uint64(int(value))
             self.L.anariSetParameter(self.h, obj, name.encode(), typ, C.byref(v))
             return
         if typ in (FRAME_COMPLETION_CALLBACK, VOID_POINTER):

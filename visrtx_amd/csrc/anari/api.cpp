@@ -7,6 +7,8 @@
 // makeVisRTXAMDDevice (VisRTXLibrary.cpp:76-80 counterpart).  Every call takes
 // the device lock (object creation is thread-safe, as exercised by
 // tests/api/TestMultiThreadedObjectCreation.cpp:57-95) and never throws.
+// Waits for a frame (frameReady WAIT, mapFrame, blocking frame properties)
+// release the lock while they block.
 #include <string.h>
 
 #include <new>
@@ -61,6 +63,35 @@ void freeParam(Param &p)
   if (p.obj)
     releaseInternal(p.obj);
   p.obj = nullptr;
+}
+
+// Waits for a frame's last render and its completion callback WITHOUT the
+// device lock, so that other threads, and a callback that calls the API,
+// proceed meanwhile.  The frame is held by an internal reference while the
+// lock is released.  Returns false for a handle that is not a frame.
+bool waitFrameUnlocked(Device *dev, Object *o)
+{
+  vrx_event ev = nullptr;
+  std::shared_ptr<Frame::Sync> sync;
+  uint64_t gen = 0;
+  {
+    std::lock_guard<std::recursive_mutex> lock(dev->mutex);
+    if (!dev->init() || !o || o->type != ANARI_FRAME)
+      return false;
+    Frame *fr = (Frame *)o;
+    if (!fr->rendered)
+      return true;
+    retainInternal(fr);
+    ev = fr->evDone;
+    sync = fr->sync;
+    gen = fr->callbacksIssued;
+  }
+  vrx_event_sync(ev);
+  sync->waitFor(gen);
+  std::lock_guard<std::recursive_mutex> lock(dev->mutex);
+  ((Frame *)o)->wait(); // complete now: reports a traversal stack overflow
+  releaseInternal(o);
+  return true;
 }
 
 } // namespace
@@ -120,6 +151,13 @@ static ANARIArray newArray(ANARIDevice d, ANARIDataType at, const void *mem, ANA
   if (sizeOf(t) == 0) {
     D(d)->reportf(nullptr, ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
         "unsupported array element type %d", t);
+    return nullptr;
+  }
+  int space = VRX_PTR_HOST;
+  if (isObjectType(t) && mem && vrx_pointer_kind(mem, &space) == VRX_OK && space != VRX_PTR_HOST) {
+    // Array.cpp:79-82
+    D(d)->reportf(nullptr, ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_OPERATION,
+        "illegal operation: cannot create object arrays from GPU memory");
     return nullptr;
   }
   return create<Array>(d, at, mem, del, user, t, n1, n2, n3);
@@ -307,7 +345,9 @@ void anariCommitParameters(ANARIDevice d, ANARIObject o)
   if (!D(d)->init())
     return;
   Object *obj = O(o);
-  // snapshot staged parameters (internal references are duplicated)
+  // snapshot staged parameters (internal references are duplicated); the
+  // object observes the arrays of its new snapshot only
+  obj->observeParams(false);
   for (auto &kv : obj->params)
     freeParam(kv.second);
   obj->params.clear();
@@ -316,6 +356,7 @@ void anariCommitParameters(ANARIDevice d, ANARIObject o)
     retainInternal(p.obj);
     obj->params[kv.first] = std::move(p);
   }
+  obj->observeParams(true);
   D(d)->queueCommit(obj);
 }
 
@@ -399,6 +440,16 @@ int anariGetProperty(ANARIDevice d, ANARIObject o, const char *name, ANARIDataTy
 {
   if (!d || !name || !mem)
     return 0;
+  if (o && (void *)o != (void *)d && (mask & ANARI_WAIT)) {
+    // a blocking query waits for the object's frame outside the lock
+    bool isFrame;
+    {
+      std::lock_guard<std::recursive_mutex> lock(D(d)->mutex);
+      isFrame = O(o)->type == ANARI_FRAME;
+    }
+    if (isFrame)
+      waitFrameUnlocked(D(d), O(o));
+  }
   std::lock_guard<std::recursive_mutex> lock(D(d)->mutex);
   if (!o || (void *)o == (void *)d) {
     // device properties (VisRTXDevice.cpp:520-552)
@@ -450,9 +501,9 @@ const void *anariMapFrame(ANARIDevice d, ANARIFrame f, const char *channel, uint
     pixelType = &t0;
   *width = *height = 0;
   *pixelType = ANARI_UNKNOWN;
-  GUARD(d, nullptr);
-  if (!f || !channel || O(f)->type != ANARI_FRAME)
+  if (!d || !f || !channel || !waitFrameUnlocked(D(d), O(f)))
     return nullptr;
+  GUARD(d, nullptr);
   return ((Frame *)f)->map(channel, width, height, pixelType);
 }
 
@@ -468,14 +519,17 @@ void anariRenderFrame(ANARIDevice d, ANARIFrame f)
 
 int anariFrameReady(ANARIDevice d, ANARIFrame f, ANARIWaitMask m)
 {
-  GUARD(d, 0);
-  if (!f || O(f)->type != ANARI_FRAME)
+  if (!d || !f)
     return 0;
-  Frame *fr = (Frame *)f;
-  if (m == ANARI_NO_WAIT)
-    return fr->rendered ? (vrx_event_query(fr->evDone) == 1 ? 1 : 0) : 1;
-  fr->wait();
-  return 1;
+  if (m == ANARI_NO_WAIT) {
+    GUARD(d, 0);
+    if (O(f)->type != ANARI_FRAME)
+      return 0;
+    return ((Frame *)f)->ready() ? 1 : 0;
+  }
+  // Frame.cu:430-438: WAIT returns once the frame, and its completion
+  // callback, are done
+  return waitFrameUnlocked(D(d), O(f)) ? 1 : 0;
 }
 
 void anariDiscardFrame(ANARIDevice, ANARIFrame) {}

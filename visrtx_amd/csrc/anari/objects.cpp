@@ -140,8 +140,31 @@ static void dropParams(std::map<std::string, Param> &m)
 
 Object::~Object()
 {
+  observeParams(false);
   dropParams(staged);
   dropParams(params);
+}
+
+static bool isArrayType(ANARIDataType t)
+{
+  return t == ANARI_ARRAY1D || t == ANARI_ARRAY2D || t == ANARI_ARRAY3D;
+}
+
+void Object::observeParams(bool on)
+{
+  for (auto &kv : params) {
+    Object *o = kv.second.obj;
+    if (!o || !isArrayType(o->type) || o == this)
+      continue;
+    auto &v = ((Array *)o)->observers;
+    if (on) {
+      v.push_back(this);
+    } else {
+      auto it = std::find(v.begin(), v.end(), this);
+      if (it != v.end())
+        v.erase(it);
+    }
+  }
 }
 
 bool Object::getProperty(const std::string &name, ANARIDataType t, void *mem, uint64_t size, uint32_t)
@@ -211,16 +234,19 @@ void releaseInternal(Object *o)
 }
 
 // ---------------------------------------------------------------------------
-// Array (array/Array.cpp:69-119,164-197,248-297; MemoryAllocation.cpp)
+// Array (array/Array.cpp:38-197,248-297; Array1D.cpp:43-91; MemoryAllocation.cpp)
 // ---------------------------------------------------------------------------
 Array::Array(Device *d, ANARIDataType arrType, const void *appMem, ANARIMemoryDeleter del,
     const void *userData, ANARIDataType e, uint64_t n1, uint64_t n2, uint64_t n3)
     : Object(d, arrType, ""), elem(e), dims{n1, n2, n3}, elemSize(sizeOf(e)), app(appMem),
       deleter(del), deleterUser(userData)
 {
+  end = capacity();
   if (!appMem) {
-    owned.assign(bytes() ? bytes() : 1, 0); // managed: zeroed, app fills via map
+    owned.assign(capacityBytes() ? capacityBytes() : 1, 0); // managed: zeroed, app fills via map
     app = nullptr;
+  } else if (vrx_pointer_kind(appMem, &space) != VRX_OK) {
+    space = VRX_PTR_HOST;
   }
   if (isObjectType(elem))
     refreshObjects();
@@ -231,24 +257,110 @@ Array::~Array()
   for (Object *o : objects)
     releaseInternal(o);
   objects.clear();
-  if (app && deleter)
+  if (devOwned)
+    vrx_free(const_cast<void *>(app));
+  else if (app && deleter)
     deleter(deleterUser, app);
   if (dptr)
     vrx_free(dptr);
 }
 
+const void *Array::host()
+{
+  const uint8_t *base = nullptr;
+  if (space == VRX_PTR_DEVICE || space == VRX_PTR_FOREIGN) {
+    // device memory: a host copy made on demand (commit-time reads only)
+    if (mirrorVersion != version || mirror.size() != capacityBytes()) {
+      mirror.resize(capacityBytes());
+      if (!mirror.empty()
+          && !dev->check(vrx_memcpy_sync(mirror.data(), app, mirror.size(), VRX_D2H), this,
+              "array read-back"))
+        return nullptr;
+      mirrorVersion = version;
+    }
+    base = mirror.data();
+  } else {
+    base = owned.empty() ? (const uint8_t *)app : owned.data();
+  }
+  return base ? base + begin * elemSize : nullptr;
+}
+
 void *Array::map()
 {
+  if (mapped)
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
+        "array mapped again without being previously unmapped");
   mapped = true;
-  return (void *)host();
+  // the whole allocation, where the application's data lives (device
+  // memory for device-resident arrays)
+  return owned.empty() ? const_cast<void *>(app) : owned.data();
 }
 
 void Array::unmap()
 {
+  if (!mapped) {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
+        "array unmapped again without being previously mapped");
+    return;
+  }
   mapped = false;
   version++;
   if (isObjectType(elem))
     refreshObjects();
+  notifyObservers();
+}
+
+void Array::commit()
+{
+  if (type != ANARI_ARRAY1D)
+    return;
+  // Array1D::commitParameters / finalize (Array1D.cpp:43-67)
+  const uint64_t cap = capacity();
+  auto read = [&](const char *name, uint64_t def) -> uint64_t {
+    const Param *p = find(name);
+    if (!p || p->bytes.empty())
+      return def;
+    if (p->type == ANARI_UINT64 || p->type == ANARI_INT64) {
+      uint64_t v;
+      memcpy(&v, p->bytes.data(), 8);
+      return p->type == ANARI_INT64 && (int64_t)v < 0 ? 0 : v;
+    }
+    if (p->type == ANARI_UINT32) {
+      uint32_t v;
+      memcpy(&v, p->bytes.data(), 4);
+      return v;
+    }
+    if (p->type == ANARI_INT32) {
+      int32_t v;
+      memcpy(&v, p->bytes.data(), 4);
+      return v < 0 ? 0 : (uint64_t)v;
+    }
+    return def;
+  };
+  if (cap == 0) {
+    report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT, "array size must be greater than zero");
+    begin = end = 0;
+    return;
+  }
+  uint64_t b = std::min(read("begin", 0), cap - 1);
+  uint64_t e = std::clamp(read("end", cap), (uint64_t)1, cap);
+  if (b > e) {
+    report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+        "array 'begin' is not less than 'end', swapping values");
+    std::swap(b, e);
+  }
+  begin = b;
+  end = e;
+  version++;
+  if (isObjectType(elem))
+    refreshObjects();
+  notifyObservers();
+}
+
+void Array::notifyObservers()
+{
+  for (Object *o : observers)
+    dev->queueCommit(o);
 }
 
 void Array::refreshObjects()
@@ -267,18 +379,46 @@ void Array::refreshObjects()
 
 void Array::onPublicReleaseToZero()
 {
-  // Array.cpp:164-182: a shared array still referenced internally is copied
-  if (app && !deleter && intRef > 0) {
-    owned.assign((const uint8_t *)app, (const uint8_t *)app + bytes());
+  // Array.cpp:164-182 (privatize): a shared array still referenced
+  // internally is copied, host memory to host memory and device memory to
+  // a device allocation of its own
+  if (!app || deleter || devOwned || intRef <= 0)
+    return;
+  if (space == VRX_PTR_HOST || space == VRX_PTR_MANAGED) {
+    owned.assign((const uint8_t *)app, (const uint8_t *)app + capacityBytes());
     app = nullptr;
+    space = VRX_PTR_HOST;
+    return;
   }
+  report(ANARI_SEVERITY_PERFORMANCE_WARNING, ANARI_STATUS_NO_ERROR,
+      "making private copy of shared GPU array");
+  void *copy = nullptr;
+  if (!dev->check(vrx_malloc(&copy, capacityBytes()), this, "array privatization"))
+    return;
+  if (!dev->check(vrx_memcpy(copy, app, capacityBytes(), VRX_D2D, dev->stream), this,
+          "array privatization")
+      || !dev->check(vrx_stream_sync(dev->stream), this, "array privatization")) {
+    vrx_free(copy);
+    return;
+  }
+  app = copy;
+  devOwned = true;
+  space = VRX_PTR_DEVICE;
 }
 
 const void *Array::device()
 {
+  const size_t off = (size_t)begin * elemSize;
+  if (deviceResident())
+    return app ? (const uint8_t *)app + off : nullptr;
+  if (space == VRX_PTR_FOREIGN) {
+    report(ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_ARGUMENT,
+        "array memory belongs to another device");
+    return nullptr;
+  }
   if (dptr && dversion == version)
-    return dptr;
-  size_t b = bytes();
+    return (const uint8_t *)dptr + off;
+  size_t b = capacityBytes();
   if (b == 0)
     return nullptr;
   if (!dptr || dbytes < b) {
@@ -289,10 +429,11 @@ const void *Array::device()
       return nullptr;
     dbytes = b;
   }
-  if (!dev->check(vrx_memcpy(dptr, host(), b, VRX_H2D, dev->stream), this, "array upload"))
+  const void *src = owned.empty() ? app : owned.data();
+  if (!dev->check(vrx_memcpy(dptr, src, b, VRX_H2D, dev->stream), this, "array upload"))
     return nullptr;
   dversion = version;
-  return dptr;
+  return (const uint8_t *)dptr + off;
 }
 
 // ---------------------------------------------------------------------------
@@ -890,11 +1031,30 @@ static std::vector<T *> objectList(const Object *o, const char *name, ANARIDataT
   return out;
 }
 
+// replaces an owning list: the new entries are retained before the old ones
+// are released (an entry in both lists survives)
+template <typename T>
+static void holdList(std::vector<T *> &dst, std::vector<T *> src)
+{
+  for (T *o : src)
+    retainInternal(o);
+  for (T *o : dst)
+    releaseInternal(o);
+  dst.swap(src);
+}
+
+Group::~Group()
+{
+  holdList(surfaces, {});
+  holdList(volumes, {});
+  holdList(lights, {});
+}
+
 void Group::commit()
 {
-  surfaces = objectList<Surface>(this, "surface", ANARI_SURFACE);
-  volumes = objectList<Volume>(this, "volume", ANARI_VOLUME);
-  lights = objectList<Light>(this, "light", ANARI_LIGHT); // Group.cpp:92
+  holdList(surfaces, objectList<Surface>(this, "surface", ANARI_SURFACE));
+  holdList(volumes, objectList<Volume>(this, "volume", ANARI_VOLUME));
+  holdList(lights, objectList<Light>(this, "light", ANARI_LIGHT)); // Group.cpp:92
 }
 
 // Group.cpp:65-86: "bounds" (object space) is the union of the group's
@@ -1066,7 +1226,14 @@ void Instance::commit()
     report(ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT, "missing 'group' on ANARIInstance");
 }
 
-World::~World() { freeScene(); }
+World::~World()
+{
+  freeScene();
+  holdList(instances, {});
+  holdList(surfaces, {});
+  holdList(volumes, {});
+  holdList(lights, {});
+}
 
 void World::freeScene()
 {
@@ -1086,10 +1253,10 @@ void World::freeScene()
 
 void World::commit()
 {
-  instances = objectList<Instance>(this, "instance", ANARI_INSTANCE);
-  surfaces = objectList<Surface>(this, "surface", ANARI_SURFACE);
-  volumes = objectList<Volume>(this, "volume", ANARI_VOLUME);
-  lights = objectList<Light>(this, "light", ANARI_LIGHT); // World.cpp:124
+  holdList(instances, objectList<Instance>(this, "instance", ANARI_INSTANCE));
+  holdList(surfaces, objectList<Surface>(this, "surface", ANARI_SURFACE));
+  holdList(volumes, objectList<Volume>(this, "volume", ANARI_VOLUME));
+  holdList(lights, objectList<Light>(this, "light", ANARI_LIGHT)); // World.cpp:124
   id = get<uint32_t>("id", ANARI_UINT32, ~0u); // World.cpp:159 (zero instance id)
   sceneDirty = true;
 }
@@ -1671,6 +1838,25 @@ void Frame::wait()
   }
 }
 
+void Frame::Sync::waitFor(uint64_t gen)
+{
+  std::unique_lock<std::mutex> lk(m);
+  cv.wait(lk, [&] { return done >= gen; });
+}
+
+bool Frame::callbacksDone()
+{
+  std::lock_guard<std::mutex> lk(sync->m);
+  return sync->done >= callbacksIssued;
+}
+
+bool Frame::ready()
+{
+  if (!rendered)
+    return true;
+  return vrx_event_query(evDone) == 1 && callbacksDone();
+}
+
 void Frame::pickLaunch(vrx_renderer &rr)
 {
   uint64_t key = (uint64_t)(uintptr_t)world ^ (world->builtAt << 20) ^ (uint64_t)(uintptr_t)renderer
@@ -1784,14 +1970,17 @@ void Frame::render()
     vrx_memcpy(hReadback + 2, dScratch, 4, VRX_D2H, dev->stream);
   }
   if (callback) {
-    cbctx = CallbackCtx{callback, callbackUser, dev, this};
-    vrx_launch_host_func(
-        dev->stream,
-        [](void *p) {
-          CallbackCtx *c = (CallbackCtx *)p;
-          c->cb(c->user, (ANARIDevice)c->dev, (ANARIFrame)c->frame);
-        },
-        &cbctx);
+    // Frame.cu:295-304 calls the callback from cudaLaunchHostFunc; here the
+    // host function only hands the job to the device's callback thread
+    dev->startCallbackThread();
+    CallbackJob *job = new CallbackJob{dev, callback, callbackUser, this, sync, callbacksIssued + 1};
+    if (dev->check(vrx_launch_host_func(
+                       dev->stream, [](void *p) { ((CallbackJob *)p)->dev->postCallback((CallbackJob *)p); },
+                       job),
+            this, "completion callback"))
+      callbacksIssued++;
+    else
+      delete job;
   }
   vrx_event_record(evDone, dev->stream);
   rendered = true;
@@ -1906,12 +2095,62 @@ Device::~Device()
 {
   if (stream)
     vrx_stream_sync(stream);
+  // every host function has run: the queue holds all outstanding callbacks,
+  // which the thread runs before it ends
+  if (cbThread.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(cbMutex);
+      cbStop = true;
+    }
+    cbCv.notify_all();
+    if (cbThread.get_id() == std::this_thread::get_id())
+      cbThread.detach(); // released from inside a callback
+    else
+      cbThread.join();
+  }
   for (Object *o : commitQueue)
     releaseInternal(o);
   commitQueue.clear();
   vrx_free(dWorkspace);
   if (stream)
     vrx_stream_destroy(stream);
+}
+
+void Device::startCallbackThread()
+{
+  if (!cbThread.joinable())
+    cbThread = std::thread([this] { callbackLoop(); });
+}
+
+void Device::postCallback(CallbackJob *j)
+{
+  {
+    std::lock_guard<std::mutex> lk(cbMutex);
+    cbQueue.push_back(j);
+  }
+  cbCv.notify_one();
+}
+
+void Device::callbackLoop()
+{
+  for (;;) {
+    CallbackJob *j = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(cbMutex);
+      cbCv.wait(lk, [&] { return cbStop || !cbQueue.empty(); });
+      if (cbQueue.empty())
+        return;
+      j = cbQueue.front();
+      cbQueue.pop_front();
+    }
+    j->cb(j->user, (ANARIDevice)j->dev, (ANARIFrame)j->frame);
+    {
+      std::lock_guard<std::mutex> lk(j->sync->m);
+      j->sync->done = std::max(j->sync->done, j->gen);
+    }
+    j->sync->cv.notify_all();
+    delete j;
+  }
 }
 
 // initDevice (VisRTXDevice.cpp:435-458): lazy, latched on failure
@@ -2008,8 +2247,14 @@ static int commitOrder(ANARIDataType t)
 
 void Device::flushCommits()
 {
-  if (commitQueue.empty())
-    return;
+  // commits can queue more commits (an array's observers), so repeat until
+  // the queue stays empty
+  while (!commitQueue.empty())
+    flushOnce();
+}
+
+void Device::flushOnce()
+{
   std::vector<Object *> q;
   q.swap(commitQueue);
   std::stable_sort(q.begin(), q.end(),

@@ -14,9 +14,13 @@
 #include <stdint.h>
 
 #include <array>
+#include <condition_variable>
+#include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "anari/anari.h"
@@ -54,6 +58,10 @@ struct Object {
 
   // called by the device when the deferred commit is flushed
   virtual void commit() {}
+  // register (on) or unregister (off) this object as an observer of the
+  // arrays among its committed parameters: unmapping or re-committing such an
+  // array queues this object's commit again (helium ChangeObserverPtr)
+  void observeParams(bool on);
   virtual bool isValid() const { return true; }
   virtual bool getProperty(const std::string &name, ANARIDataType type, void *mem, uint64_t size,
       uint32_t mask);
@@ -91,23 +99,39 @@ struct Array : Object {
   const void *app{nullptr};
   ANARIMemoryDeleter deleter{nullptr};
   const void *deleterUser{nullptr};
-  std::vector<uint8_t> owned; // managed or privatized storage
+  std::vector<uint8_t> owned; // managed or privatized host storage
+  // where `app` lives (vrx_pointer_kind): shared and captured arrays may hold
+  // device or managed memory, which is used in place (Array.cpp:38-65)
+  int space{VRX_PTR_HOST};
+  bool devOwned{false}; // `app` is a device copy made by privatization
   bool mapped{false};
   uint64_t version{1};
-  // device copy
+  // Array1D region [begin, end) of the capacity (Array1D.cpp:43-91)
+  uint64_t begin{0}, end{0};
+  // device copy of host storage
   void *dptr{nullptr};
   size_t dbytes{0};
   uint64_t dversion{0};
+  // host copy of device storage, for commit-time reads (index checks, LUTs)
+  std::vector<uint8_t> mirror;
+  uint64_t mirrorVersion{0};
   std::vector<Object *> objects; // object arrays: internal references
+  std::vector<Object *> observers; // objects whose committed parameters hold this array
 
-  uint64_t count() const { return dims[0] * dims[1] * dims[2]; }
+  uint64_t capacity() const { return dims[0] * dims[1] * dims[2]; }
+  // elements in use: the region for Array1D, else the capacity
+  uint64_t count() const { return type == ANARI_ARRAY1D ? end - begin : capacity(); }
+  size_t capacityBytes() const { return capacity() * elemSize; }
   size_t bytes() const { return count() * elemSize; }
-  const void *host() const { return owned.empty() ? app : owned.data(); }
+  bool deviceResident() const { return space == VRX_PTR_DEVICE || space == VRX_PTR_MANAGED; }
+  const void *host(); // first element of the region, host readable
   void *map();
   void unmap();
-  const void *device(); // uploads on first use / after unmap
+  void commit() override; // Array1D "begin" / "end"
+  const void *device(); // first element of the region on the device (uploads host data)
   void onPublicReleaseToZero() override; // privatize shared memory
   void refreshObjects();
+  void notifyObservers();
 };
 
 // ---------------------------------------------------------------------------
@@ -219,9 +243,12 @@ struct Light : Object {
 
 struct Group : Object {
   Group(Device *d) : Object(d, ANARI_GROUP, "") {}
+  ~Group() override;
   void commit() override;
   bool getProperty(const std::string &name, ANARIDataType type, void *mem, uint64_t size,
       uint32_t mask) override;
+  // internal references, taken at commit and dropped at the next commit or
+  // destruction (the object array may change or die in between)
   std::vector<Surface *> surfaces;
   std::vector<Volume *> volumes;
   std::vector<Light *> lights;
@@ -248,6 +275,7 @@ struct World : Object {
   void commit() override;
   bool getProperty(const std::string &name, ANARIDataType type, void *mem, uint64_t size,
       uint32_t mask) override;
+  // internal references, as Group's
   std::vector<Instance *> instances;
   std::vector<Surface *> surfaces;
   std::vector<Volume *> volumes;
@@ -335,12 +363,17 @@ struct Frame : Object {
     float ms[4]{};
   } tune;
   void pickLaunch(vrx_renderer &rr);
-  struct CallbackCtx {
-    ANARIFrameCompletionCallback cb;
-    const void *user;
-    void *dev;
-    void *frame;
-  } cbctx{}; // one outstanding render per frame, so one context per frame
+  // completion callbacks: each render with a callback posts one job; the
+  // device's callback thread runs it and advances `sync`
+  struct Sync {
+    std::mutex m;
+    std::condition_variable cv;
+    uint64_t done{0};
+    void waitFor(uint64_t gen);
+  };
+  std::shared_ptr<Sync> sync{std::make_shared<Sync>()};
+  uint64_t callbacksIssued{0};
+  bool callbacksDone(); // every posted callback has returned
   bool nextReset{true};
   uint64_t lastSeenFinalize{0};
   uint32_t frameID{0};
@@ -353,8 +386,20 @@ struct Frame : Object {
   void allocate();
   void freeBuffers();
   void render();
-  void wait();
+  void wait(); // GPU work of the last render (not its callback)
+  bool ready(); // GPU work done and its callback returned
   const void *map(const std::string &channel, uint32_t *w, uint32_t *h, ANARIDataType *pt);
+};
+
+// A frame completion callback waiting to run (posted by the HIP host function
+// of Frame::render, run by the device's callback thread).
+struct CallbackJob {
+  Device *dev;
+  ANARIFrameCompletionCallback cb;
+  const void *user;
+  Frame *frame;
+  std::shared_ptr<Frame::Sync> sync;
+  uint64_t gen;
 };
 
 // ---------------------------------------------------------------------------
@@ -378,10 +423,23 @@ struct Device {
   void *dWorkspace{nullptr};
   size_t workspaceBytes{0};
   std::map<std::string, Param> params;
+  // Frame completion callbacks run on this thread, never on the HIP runtime's
+  // host-function thread: the stream then never waits for application code,
+  // and a callback may call the API (it takes the device lock like any
+  // caller) while another thread waits for the frame without the lock.
+  std::thread cbThread;
+  std::mutex cbMutex;
+  std::condition_variable cbCv;
+  std::deque<CallbackJob *> cbQueue;
+  bool cbStop{false};
+  void postCallback(CallbackJob *j); // from the HIP host function: enqueue only
+  void startCallbackThread();
+  void callbackLoop();
 
   bool init();
   void *workspace(size_t bytes);
   void flushCommits();
+  void flushOnce();
   void queueCommit(Object *o);
   uint64_t tick() { return ++finalizeClock; }
   void report(Object *src, int severity, int code, const char *fmt, va_list ap);

@@ -713,20 +713,28 @@ __global__ __launch_bounds__(256) void k_reduce_lut(const float *slices, float *
   grad_lut[j] += s;
 }
 
-__constant__ vrx_camera c_cams[256];
+// up to 32 cameras travel by value in the kernel arguments (2 KiB), so each
+// launch owns its cameras and concurrent calls on any streams cannot race
+constexpr uint32_t kCamsPerLaunch = 32;
+struct CamBlock {
+  vrx_camera c[kCamsPerLaunch];
+};
 
 // pinhole rays through the pixel corners (gpu/cameraCreateRay.h:74-81,
-// raycast variant), row 0 = bottom; ray index = view*w*h + y*w + x
-__global__ void k_make_rays(uint32_t nviews, uint32_t w, uint32_t h, float *org, float *dir)
+// raycast variant), row 0 = bottom; ray index = view*w*h + y*w + x, for the
+// views first_view .. first_view + nviews - 1 (cams.c[view - first_view])
+__global__ void k_make_rays(CamBlock cams, uint32_t first_view, uint32_t nviews, uint32_t w,
+    uint32_t h, float *org, float *dir)
 {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t per = (size_t)w * h;
-  if (i >= per * nviews)
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= per * nviews)
     return;
-  const uint32_t view = (uint32_t)(i / per);
-  const uint32_t rem = (uint32_t)(i - (size_t)view * per);
+  const uint32_t lv = (uint32_t)(j / per);
+  const uint32_t rem = (uint32_t)(j - (size_t)lv * per);
   const uint32_t y = rem / w, x = rem - y * w;
-  const vrx_camera &c = c_cams[view];
+  const size_t i = (size_t)(first_view + lv) * per + rem;
+  const vrx_camera &c = cams.c[lv];
   float sx = (float)x * (1.0f / (float)w), sy = (float)y * (1.0f / (float)h);
   sx = c.region[0] * (1.0f - sx) + c.region[2] * sx;
   sy = c.region[1] * (1.0f - sy) + c.region[3] * sy;
@@ -828,14 +836,19 @@ int vrx_make_rays(const vrx_camera *cams, uint32_t nviews, uint32_t w, uint32_t 
 {
   VRX_ARG(cams && d_org && d_dir, "null argument");
   VRX_ARG(nviews >= 1 && nviews <= 256, "1..256 views");
-  VRX_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_cams), cams, sizeof(vrx_camera) * nviews, 0,
-      hipMemcpyHostToDevice, (hipStream_t)s));
-  size_t total = (size_t)nviews * w * h;
-  if (total == 0)
+  const size_t per = (size_t)w * h;
+  if (per == 0)
     return VRX_OK;
-  hipLaunchKernelGGL(k_make_rays, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-      (hipStream_t)s, nviews, w, h, d_org, d_dir);
-  VRX_LAUNCH_CHECK();
+  for (uint32_t v0 = 0; v0 < nviews; v0 += kCamsPerLaunch) {
+    const uint32_t nv = nviews - v0 < kCamsPerLaunch ? nviews - v0 : kCamsPerLaunch;
+    CamBlock cb;
+    memset(&cb, 0, sizeof(cb));
+    memcpy(cb.c, cams + v0, sizeof(vrx_camera) * nv);
+    const size_t total = per * nv;
+    hipLaunchKernelGGL(k_make_rays, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+        (hipStream_t)s, cb, v0, nv, (uint32_t)w, (uint32_t)h, d_org, d_dir);
+    VRX_LAUNCH_CHECK();
+  }
   return VRX_OK;
 }
 

@@ -124,6 +124,30 @@ int vrx_host_free(void *h_ptr)
   return VRX_OK;
 }
 
+int vrx_pointer_kind(const void *p, int *kind)
+{
+  VRX_ARG(kind, "kind");
+  *kind = VRX_PTR_HOST;
+  if (!p)
+    return VRX_OK;
+  hipPointerAttribute_t a;
+  memset(&a, 0, sizeof(a));
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    // plain pageable host memory is unknown to the runtime: clear the error
+    // so that the next call does not report it
+    (void)hipGetLastError();
+    return VRX_OK;
+  }
+  int cur = 0;
+  VRX_HIP(hipGetDevice(&cur));
+  if (a.type == hipMemoryTypeManaged || a.isManaged)
+    *kind = VRX_PTR_MANAGED;
+  else if (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeArray)
+    *kind = a.device == cur ? VRX_PTR_DEVICE : VRX_PTR_FOREIGN;
+  return VRX_OK;
+}
+
 static hipMemcpyKind kind_of(int kind)
 {
   return kind == VRX_H2D ? hipMemcpyHostToDevice
