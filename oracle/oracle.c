@@ -1410,6 +1410,12 @@ void orc_classify(const orc_volume *v, double s, double rgba[4])
    pixel. */
 #define AMB_T_REL 2e-6
 #define AMB_OPACITY 1e-5
+/* Which decision is in doubt, as bits of the flag: a segment end or slab
+   tie (the sample count of that segment may differ by one) and an
+   early-termination tie (the march may stop one or more samples earlier or
+   later while the opacity stays within AMB_OPACITY of 0.99). */
+#define AMB_TIE_T 1
+#define AMB_TIE_STOP 2
 
 static inline int near_rel(double a, double b)
 {
@@ -1445,7 +1451,7 @@ static int vol_slab_amb(const orc_volume *v, const double o[3], const double d[3
     tf = f < tf ? f : tf;
   }
   if (amb && (near_rel(tn, tf) || near_rel(tf, tmin) || near_rel(tn, tmax)))
-    *amb = 1;
+    *amb |= AMB_TIE_T;
   if (!(tn < tf))
     return 0;
   if (tf < tmin || tn > tmax)
@@ -1483,11 +1489,11 @@ static double march_amb(const orc_volume *v, const double org[3], const double d
   for (uint64_t k = 0; k < MAX_SAMPLES_PER_SEGMENT; k++) {
     double t = t0 + (double)k * step;
     if (amb && near_rel(t, t1))
-      *amb = 1;
+      *amb |= AMB_TIE_T;
     if (!(t <= t1))
       break;
     if (amb && early_stop && fabs(*opacity - 0.99) < AMB_OPACITY)
-      *amb = 1;
+      *amb |= AMB_TIE_STOP;
     if (early_stop && !(*opacity < 0.99))
       break;
     double p[3] = {org[0] + dir[0] * t, org[1] + dir[1] * t, org[2] + dir[2] * t};
@@ -2060,11 +2066,11 @@ static void diff_march(const uint32_t dims[3], const float origin[3], const floa
   for (uint64_t k = 0;; k++) {
     double t = t0 + (double)k * step;
     if (amb && near_rel(t, t1))
-      *amb = 1;
+      *amb |= AMB_TIE_T;
     if (!(t <= t1))
       break;
     if (amb && early_stop && fabs(out[3] - 0.99) < AMB_OPACITY)
-      *amb = 1;
+      *amb |= AMB_TIE_STOP;
     if (early_stop && !(out[3] < 0.99))
       break;
     double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};

@@ -19,6 +19,9 @@ ORC_TRIANGLE, ORC_SPHERE, ORC_CYLINDER, ORC_CONE, ORC_CURVE = 0, 1, 2, 4, 5
 ORC_RAYCAST, ORC_AO, ORC_DIRECT = 0, 1, 2
 ORC_LIGHT_DIRECTIONAL, ORC_LIGHT_POINT, ORC_LIGHT_SPOT = 0, 1, 2
 ORC_COLOR_F32, ORC_COLOR_SRGB8, ORC_COLOR_UNORM8 = 0, 1, 2
+# bits of the per-pixel "ambiguous" flag of volume marches: a segment-end or
+# slab tie (one sample more or less), an early-termination tie at opacity 0.99
+AMB_TIE_T, AMB_TIE_STOP = 1, 2
 LEAF = 0x80000000
 
 vp = C.c_void_p
