@@ -378,6 +378,19 @@ typedef struct vrx_batch {
 size_t vrx_batch_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b);
 int vrx_render_batch(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b,
     vrx_stream s);
+/* No counterpart in the reference: vrx_render_batch for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  b->d_scratch, b->scratch_bytes and
+ * b->d_work_counter are ignored: the tile-queue counter and the stack-spill
+ * area live in a file-scope device array of VRX_BATCH_STATIC_SCRATCH_BYTES
+ * that each GPU the library runs on holds until the process ends.  The
+ * persistent grid is shrunk until the spill area of every resident lane fits;
+ * a scene whose spill for one workgroup does not fit is refused (VRX_ERR_ARG).
+ * Nothing is allocated.  Every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU (on any streams) race: serialise them. */
+#define VRX_BATCH_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+int vrx_render_batch_static(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b,
+    vrx_stream s);
 /* Captures vrx_render_batch on stream s (not the default stream) into an
  * instantiated HIP graph; the graph holds the structs' values at capture. */
 int vrx_batch_graph_create(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b,

@@ -187,6 +187,7 @@ def hip() -> C.CDLL:
             "vrx_make_rays": (i32, [vp, u32, u32, u32, vp, vp, vp]),
             "vrx_batch_scratch_bytes": (sz, [vp, vp, vp]),
             "vrx_render_batch": (i32, [vp, vp, vp, vp]),
+            "vrx_render_batch_static": (i32, [vp, vp, vp, vp]),
             "vrx_batch_graph_create": (i32, [vp, vp, vp, vp, vp]),
             "vrx_graph_launch": (i32, [vp, vp]),
             "vrx_graph_destroy": (i32, [vp]),

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "vrx_internal.h"
@@ -34,7 +35,7 @@ __device__ __forceinline__ vrx_camera load_camera(const vrx_camera *cams, uint32
 
 template <int PT, bool TWO, bool WIDE, int KIND>
 __global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, vrx_renderer r,
-    vrx_batch b, uint32_t sd, uint32_t cap, uint32_t *scratch, uint32_t tiles_x, uint32_t tiles_y)
+    vrx_batch b, uint32_t sd, uint32_t cap, uint32_t *spill, uint32_t tiles_x, uint32_t tiles_y)
 {
   extern __shared__ uint32_t s_stack[];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -45,10 +46,10 @@ __global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, v
   st.sd_mask = sd - 1;
   const uint32_t gthreads = gridDim.x * blockDim.x;
   const uint32_t gtid = blockIdx.x * blockDim.x + tid;
-  st.ovf = scratch + 64 + gtid;
+  st.ovf = spill + 64 + gtid;
   st.ovf_stride = gthreads;
   st.ovf_cap = cap;
-  st.err = scratch;
+  st.err = spill;
   st.cnt = 0;
 
   vrx_frame f = {};
@@ -164,7 +165,12 @@ __global__ void k_reset_counter(uint32_t *c)
     *c = 0u;
 }
 
-static int launch_batch(const vrx_scene &sc, const vrx_renderer &r, const vrx_batch &b, hipStream_t s)
+// Queue reset + persistent launch.  `spill` is the stack-spill area (error
+// word, then `cap` entries per resident lane from word 64 on) and `counter`
+// the tile queue; with spill_limit > 0 the grid is shrunk so that the spill
+// area of every resident lane fits in spill_limit bytes.
+static int launch_batch(const vrx_scene &sc, const vrx_renderer &r, const vrx_batch &b, hipStream_t s,
+    uint32_t *spill, uint32_t *counter, size_t spill_limit = 0)
 {
   const int block = std::min(std::max(b.block_threads > 0 ? b.block_threads : 256, 64), 256) & ~63;
   const bool wide = r.node_width == 4;
@@ -180,20 +186,37 @@ static int launch_batch(const vrx_scene &sc, const vrx_renderer &r, const vrx_ba
                                             : (uint32_t)(num_cus() * per_cu);
   const uint32_t max_blocks = (uint32_t)num_cus() * kMaxThreadsPerCU / (uint32_t)block;
   blocks = std::min(blocks, max_blocks);
+  if (spill_limit) {
+    const size_t per_block = (size_t)cap * 4 * block;
+    const size_t fit = per_block ? (spill_limit - 256) / per_block : (size_t)blocks;
+    if (fit < 1) {
+      set_error("stack spill of one workgroup (%zu B) exceeds the static scratch (%zu B)",
+          per_block + 256, spill_limit);
+      return VRX_ERR_ARG;
+    }
+    blocks = (uint32_t)std::min<size_t>(blocks, fit);
+  }
   const uint32_t tw = (uint32_t)b.tile_w, th = (uint32_t)b.tile_h;
   const uint32_t tiles_x = (b.width + tw - 1) / tw, tiles_y = (b.height + th - 1) / th;
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, b.d_work_counter);
-  void *args[] = {(void *)&sc, (void *)&r, (void *)&b, (void *)&sd, (void *)&cap,
-      (void *)&b.d_scratch, (void *)&tiles_x, (void *)&tiles_y};
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter);
+  vrx_batch bq = b;
+  bq.d_work_counter = counter;
+  void *args[] = {(void *)&sc, (void *)&r, (void *)&bq, (void *)&sd, (void *)&cap,
+      (void *)&spill, (void *)&tiles_x, (void *)&tiles_y};
   VRX_HIP(hipLaunchKernel(kern, dim3(blocks), dim3(block), args, lds, s));
   return VRX_OK;
 }
 
-static int validate(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b)
+static int launch_batch(const vrx_scene &sc, const vrx_renderer &r, const vrx_batch &b, hipStream_t s)
+{
+  return launch_batch(sc, r, b, s, (uint32_t *)b.d_scratch, b.d_work_counter);
+}
+
+static int validate_common(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b)
 {
   VRX_ARG(scene && r && b, "null argument");
   VRX_ARG(b->num_views > 0 && b->width > 0 && b->height > 0, "batch dimensions");
-  VRX_ARG(b->d_cams && b->d_work_counter, "cameras / work counter");
+  VRX_ARG(b->d_cams, "cameras");
   VRX_ARG(b->tile_w > 0 && b->tile_h > 0 && b->tile_w * b->tile_h == 64, "tile must hold 64 pixels");
   VRX_ARG(scene->num_entries == 0 || (scene->d_entries && scene->d_geoms), "scene arrays");
   VRX_ARG(scene->num_entries <= 1 || scene->tlas.d_leaf_ids, "instance BVH not built");
@@ -203,11 +226,47 @@ static int validate(const vrx_scene *scene, const vrx_renderer *r, const vrx_bat
   VRX_ARG(r->node_width == 0 || r->node_width == 2 || r->node_width == 4, "node width 2 or 4");
   VRX_ARG(r->node_width != 4 || scene->num_entries == 0 || scene->wide,
       "node width 4 needs 4-wide nodes for every BVH (scene->wide)");
+  return VRX_OK;
+}
+
+static int validate(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b)
+{
+  int st = validate_common(scene, r, b);
+  if (st)
+    return st;
+  VRX_ARG(b->d_work_counter, "work counter");
   size_t need = vrx_batch_scratch_bytes(scene, r, b);
   if (!b->d_scratch || b->scratch_bytes < need) {
     set_error("batch scratch too small: %zu < %zu", b->scratch_bytes, need);
     return VRX_ERR_ARG;
   }
+  return VRX_OK;
+}
+
+// ---- the allocation-free variant -------------------------------------------
+// One array per GPU, in device memory the code object reserves at load time:
+// 256 B for the tile-queue counter, then the stack-spill area.  In host code
+// `scratch` names a host-side stand-in, never a device address: kernels get
+// the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) uint8_t scratch[VRX_BATCH_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static void *g_scratch_addr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int static_scratch(uint8_t **out)
+{
+  int dev = 0;
+  VRX_HIP(hipGetDevice(&dev));
+  VRX_ARG(dev >= 0 && dev < kMaxDevices, "device index");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    VRX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    VRX_ARG(p, "static scratch address");
+    g_scratch_addr[dev] = p;
+  }
+  *out = (uint8_t *)g_scratch_addr[dev];
   return VRX_OK;
 }
 
@@ -237,6 +296,26 @@ int vrx_render_batch(const vrx_scene *scene, const vrx_renderer *r, const vrx_ba
   if (st)
     return st;
   st = launch_batch(*scene, *r, *b, (hipStream_t)s);
+  if (st)
+    return st;
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
+}
+
+int vrx_render_batch_static(const vrx_scene *scene, const vrx_renderer *r, const vrx_batch *b,
+    vrx_stream s)
+{
+  int st = validate_common(scene, r, b);
+  if (st)
+    return st;
+  uint8_t *base = nullptr;
+  if ((st = static_scratch(&base)))
+    return st;
+  // the queue counter is reset by a kernel on s; the spill area needs no
+  // clearing (entries are written before they are read, the error word is
+  // only ever set)
+  st = launch_batch(*scene, *r, *b, (hipStream_t)s, (uint32_t *)(base + 256), (uint32_t *)base,
+      VRX_BATCH_STATIC_SCRATCH_BYTES - 256);
   if (st)
     return st;
   VRX_LAUNCH_CHECK();
