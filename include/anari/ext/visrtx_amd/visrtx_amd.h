@@ -18,9 +18,12 @@ extern "C" {
 /* Same shape as makeVisRTXDevice(ANARIStatusCallback, const void *). */
 ANARI_INTERFACE ANARIDevice makeVisRTXAMDDevice(ANARIStatusCallback cb, const void *userPtr);
 
-/* Loader-style entry (VisRTXLibrary.cpp:70-74 ANARI_DEFINE_LIBRARY_ENTRYPOINT):
- * returns a device for anariNewDevice(library, subtype); subtype is ignored
- * as in VisRTXLibrary::newDevice (VisRTXLibrary.cpp:56-59). */
+/* The device behind this library's own anariNewDevice(library, subtype):
+ * subtype is ignored, as in VisRTXLibrary::newDevice (VisRTXLibrary.cpp:
+ * 56-59).  It is not the ANARI-SDK's loader entry point: that one
+ * (ANARI_DEFINE_LIBRARY_ENTRYPOINT, VisRTXLibrary.cpp:70-74) returns a
+ * library object whose devices implement the SDK's anari::DeviceImpl, which
+ * this library does not provide. */
 ANARI_INTERFACE ANARIDevice visrtx_amd_library_new_device(const char *subtype,
     ANARIStatusCallback cb, const void *userPtr);
 
