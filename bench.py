@@ -238,10 +238,18 @@ def bench_diffvol(iters=200, log_every=50):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     first = last = None
+    # per iteration: forward (the render) and backward (MSE loss, its
+    # gradient and the adjoint) bracketed by events on the current stream
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(iters)]
     for it in range(iters):
         opt.zero_grad()
-        loss = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean()
+        ev[it][0].record()
+        rgba = D.render(dens, lut, org, dirs, spec)
+        ev[it][1].record()
+        loss = ((rgba - target) ** 2).mean()
         loss.backward()
+        ev[it][2].record()
         opt.step()
         with torch.no_grad():
             lut.clamp_(0.0, 0.999)
@@ -251,6 +259,8 @@ def bench_diffvol(iters=200, log_every=50):
             log(f"diffvol iter {it + 1}: loss {loss.item():.4e}")
     torch.cuda.synchronize()
     iter_ms = (time.perf_counter() - t0) * 1000.0 / iters
+    per_it = [(a.elapsed_time(b), b.elapsed_time(c)) for a, b, c in ev]
+    ratio_mean = statistics.mean((f + b) / f for f, b in per_it)
     with torch.no_grad():
         last = ((D.render(dens, lut, org, dirs, spec) - target) ** 2).mean().item()
         # the same timings at the state Adam reached (denser field, rays cross
@@ -271,6 +281,9 @@ def bench_diffvol(iters=200, log_every=50):
                            "fwd_bwd_integral_ms": round(fbi_ms_final, 3),
                            "fwd_bwd_integral_over_fwd": round(fbi_ms_final / fwd_ms_final, 3)},
             "adam_ms_per_iter": round(iter_ms, 3),
+            "adam_fwd_bwd_over_fwd_mean": round(ratio_mean, 3),
+            "adam_fwd_ms_mean": round(statistics.mean(f for f, _ in per_it), 3),
+            "adam_bwd_ms_mean": round(statistics.mean(b for _, b in per_it), 3),
             "iters": iters, "mse_first": first, "mse_final": last, "rays": int(org.shape[0])}
 
 

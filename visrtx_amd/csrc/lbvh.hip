@@ -9,10 +9,15 @@
 //      is histogram -> per-digit row scan (256 workgroups) -> LDS-staged
 //      scatter with wave-ballot stable ranking (tile = 256 lanes x 16 keys)
 //   4. Karras (HPG 2012) hierarchy, ties broken with index-augmented keys
-//   5. refit without atomics: nodes are bucketed by the common-prefix length
-//      of their key range (a child's is always longer than its parent's) and
-//      refit one length at a time, longest first, one kernel per length, so
-//      every node is written once by one thread with plain stores
+//   5. refit without atomics, in three launches.  A node's common-prefix
+//      length (delta) is always shorter than its children's, so refitting in
+//      decreasing delta visits children first.  Nodes whose leaf range lies
+//      in one chunk of kRefitChunk sorted leaves are refit by that chunk's
+//      workgroup, delta by delta, with boxes in LDS; nodes crossing a chunk
+//      border but inside one span of kRefitSpan leaves by that span's
+//      workgroup; the few crossing a span border by one last workgroup.
+//      Every node is written once, with plain stores; workgroup barriers
+//      order the deltas inside a launch, kernel boundaries the launches.
 // Bounds inputs follow scene/surface/geometry/Sphere.cu:82-110 (c +- r),
 // Cylinder.cpp:53-102 (min/max(v0,v1) +- |r|) and Triangle.cpp:112-138.
 //
@@ -35,20 +40,42 @@ static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 static uint32_t sort_blocks(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
 
+// workgroups of the centroid-box reduction (one partial box each)
+constexpr uint32_t kCboxBlocks = 1024;
+
+// refit granularity: leaves per chunk (pass 1, LDS) and per span (pass 2)
+constexpr uint32_t kRefitChunk = 512;
+constexpr uint32_t kRefitSpan = 1u << 16;
+constexpr int kRefitThreads = 512;
+constexpr int kSpanThreads = 1024;
+// a Karras tree is at most 64 internal nodes deep (delta grows by at least
+// one per level and is below 64), so at most 64 nodes contain any two
+// neighbouring leaves: a span holds at most 64 (kRefitSpan / kRefitChunk - 1)
+// nodes that cross a chunk border, and the top pass at most 64 per span border
+constexpr uint32_t kSpanListCap = 64u * (kRefitSpan / kRefitChunk);
+
+// level byte of an internal node: delta in bits 0..5, refit pass in bits 6..7
+// (1: inside a chunk, 2: inside a span, 3: crossing a span border)
+constexpr uint32_t kPassShift = 6;
+
 struct Workspace {
   uint32_t *keys_a, *vals_a, *keys_b, *vals_b;
   uint32_t *hist;      // 256 x nblocks, digit-major
   uint32_t *row_total; // 256
   uint32_t *cbox;      // 6 ordered-uint floats
+  uint32_t *cbox_part; // 6 x kCboxBlocks per-workgroup partial bounds
   uint2 *child;        // n-1 child pairs from the hierarchy pass
-  uint8_t *level;      // n-1 common-prefix lengths (0..63)
-  uint32_t *bucket;    // n-1 internal nodes grouped by prefix length
-  uint32_t *lcount;    // 64 nodes per prefix length
-  uint32_t *loff;      // 64 bucket offsets
-  uint32_t *lfill;     // 64 bucket fill counters
+  uint8_t *level;      // n-1 level bytes (delta | pass << 6)
+  uint32_t *top;       // nodes of the top pass
+  uint32_t *top_count; // 1
   void *rec;           // n leaf records in input order (written coalesced by
                        // the Morton pass, gathered into sorted order later)
 };
+
+static uint32_t top_capacity(uint32_t n)
+{
+  return 64u * ((n + kRefitSpan - 1) / kRefitSpan + 1);
+}
 
 static size_t workspace_layout(int type, uint32_t n, void *base, Workspace *w)
 {
@@ -68,12 +95,11 @@ static size_t workspace_layout(int type, uint32_t n, void *base, Workspace *w)
   ws.hist = (uint32_t *)take(4 * (size_t)256 * sort_blocks((uint32_t)nn));
   ws.row_total = (uint32_t *)take(4 * 256);
   ws.cbox = (uint32_t *)take(4 * 8);
+  ws.cbox_part = (uint32_t *)take(4 * 6 * (size_t)kCboxBlocks);
   ws.child = (uint2 *)take(8 * nn);
   ws.level = (uint8_t *)take(nn);
-  ws.bucket = (uint32_t *)take(4 * nn);
-  ws.lcount = (uint32_t *)take(4 * 64);
-  ws.loff = (uint32_t *)take(4 * 64);
-  ws.lfill = (uint32_t *)take(4 * 64);
+  ws.top = (uint32_t *)take(4 * (size_t)top_capacity((uint32_t)nn));
+  ws.top_count = (uint32_t *)take(4 * 64);
   ws.rec = take(leaf_record_bytes(type) * nn);
   if (w)
     *w = ws;
@@ -164,53 +190,110 @@ __device__ __forceinline__ v3 centroid(v3 lo, v3 hi)
   return mk((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
 }
 
-__global__ void k_cbox_init(uint32_t *cbox)
-{
-  if (threadIdx.x < 3)
-    cbox[threadIdx.x] = 0xFFFFFFFFu; // min slots
-  else if (threadIdx.x < 6)
-    cbox[threadIdx.x] = 0u; // max slots
-}
 
-template <int T>
-__global__ __launch_bounds__(256) void k_centroid_bounds(vrx_prims p, uint32_t *cbox)
+__device__ __forceinline__ void cbox_add(v3 lo, v3 hi, uint32_t mn[3], uint32_t mx[3])
 {
-  __shared__ uint32_t red[6][256];
-  uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
-    v3 lo, hi;
-    prim_box<T>(p, i, lo, hi);
-    v3 c = centroid(lo, hi);
-    float cc[3] = {c.x, c.y, c.z};
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      if (cc[k] == cc[k]) { // NaN centroids are ignored, as fminf/fmaxf do
-        uint32_t o = f2o(cc[k]);
-        mn[k] = min(mn[k], o);
-        mx[k] = max(mx[k], o);
-      }
-    }
-  }
+  const v3 c = centroid(lo, hi);
+  const float cc[3] = {c.x, c.y, c.z};
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    red[k][threadIdx.x] = mn[k];
-    red[3 + k][threadIdx.x] = mx[k];
+    if (cc[k] == cc[k]) { // NaN centroids are ignored, as fminf/fmaxf do
+      const uint32_t o = f2o(cc[k]);
+      mn[k] = min(mn[k], o);
+      mx[k] = max(mx[k], o);
+    }
+  }
+}
+
+// Centroid box: grid-stride, four primitives in flight per lane, then a wave
+// reduction and one LDS step across the waves: one partial box per workgroup.
+template <int T>
+__global__ __launch_bounds__(256) void k_centroid_bounds(vrx_prims p, uint32_t *part)
+{
+  __shared__ uint32_t red[6][4];
+  uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; (uint64_t)i + 3ull * stride < p.n; i += 4 * stride) {
+    v3 lo[4], hi[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      prim_box<T>(p, i + (uint32_t)u * stride, lo[u], hi[u]);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      cbox_add(lo[u], hi[u], mn, mx);
+  }
+  for (; i < p.n; i += stride) {
+    v3 lo, hi;
+    prim_box<T>(p, i, lo, hi);
+    cbox_add(lo, hi, mn, mx);
+  }
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], m, 64));
+      mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], m, 64));
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      red[k][wave] = mn[k];
+      red[3 + k][wave] = mx[k];
+    }
   }
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    uint32_t v = red[k][0];
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        red[k][threadIdx.x] = min(red[k][threadIdx.x], red[k][threadIdx.x + s]);
-        red[3 + k][threadIdx.x] = max(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
-      }
-    }
-    __syncthreads();
+    for (int w = 1; w < 4; w++)
+      v = k < 3 ? min(v, red[k][w]) : max(v, red[k][w]);
+    part[(size_t)k * gridDim.x + blockIdx.x] = v;
   }
-  if (threadIdx.x < 3)
-    atomicMin(&cbox[threadIdx.x], red[threadIdx.x][0]);
-  else if (threadIdx.x < 6)
-    atomicMax(&cbox[threadIdx.x], red[threadIdx.x][0]);
+}
+
+// the partial boxes of the nblocks workgroups -> the centroid box (a second
+// launch instead of same-address atomics from every workgroup)
+__global__ __launch_bounds__(256) void k_cbox_final(const uint32_t *part, uint32_t nblocks,
+    uint32_t *cbox)
+{
+  __shared__ uint32_t red[6][4];
+  uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+  for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      mn[k] = min(mn[k], part[(size_t)k * nblocks + b]);
+      mx[k] = max(mx[k], part[(size_t)(3 + k) * nblocks + b]);
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], m, 64));
+      mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], m, 64));
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      red[k][wave] = mn[k];
+      red[3 + k][wave] = mx[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    uint32_t v = red[k][0];
+#pragma unroll
+    for (int w = 1; w < 4; w++)
+      v = k < 3 ? min(v, red[k][w]) : max(v, red[k][w]);
+    cbox[k] = v;
+  }
 }
 
 __device__ __forceinline__ uint32_t expand_bits10(uint32_t v)
@@ -462,110 +545,89 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *
 // --------------------------------------------------------------------------
 // Karras hierarchy
 // --------------------------------------------------------------------------
-__device__ __forceinline__ int kdelta(const uint32_t *c, int64_t n, int64_t i, int64_t j)
-{
-  if (j < 0 || j >= n)
-    return -1;
-  uint32_t a = c[i], b = c[j];
-  if (a != b)
-    return __clz(a ^ b);
-  return 32 + __clz((uint32_t)i ^ (uint32_t)j);
-}
 
-__global__ __launch_bounds__(256) void k_karras(
-    const uint32_t *sc, uint32_t n_, uint2 *child, uint8_t *level, uint32_t *lcount)
-{
-  __shared__ uint32_t hist[64];
-  if (threadIdx.x < 64)
-    hist[threadIdx.x] = 0u;
-  __syncthreads();
-  const int64_t n = n_;
-  // grid-stride: a bounded grid keeps the per-block level histograms few, so
-  // the contended global level counters see one add per block and level
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n - 1;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int d = (kdelta(sc, n, i, i + 1) - kdelta(sc, n, i, i - 1)) > 0 ? 1 : -1;
-    int dmin = kdelta(sc, n, i, i - d);
-    int64_t lmax = 2;
-    while (kdelta(sc, n, i, i + lmax * d) > dmin)
-      lmax <<= 1;
-    int64_t l = 0;
-    for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
-      if (kdelta(sc, n, i, i + (l + t) * d) > dmin)
-        l += t;
-    int64_t j = i + l * d;
-    int dnode = kdelta(sc, n, i, j);
-    int64_t s = 0, t = l;
-    do {
-      t = (t + 1) >> 1;
-      if (kdelta(sc, n, i, i + (s + t) * d) > dnode)
-        s += t;
-    } while (t > 1);
-    int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
-    int64_t lo = i < j ? i : j, hi = i < j ? j : i;
-    uint32_t left = (lo == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
-    uint32_t right = (hi == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
-    child[i] = make_uint2(left, right);
-    // common-prefix length of the node's key range: every child range has a
-    // strictly longer one, so refitting in decreasing prefix length visits
-    // children before parents
-    uint32_t lv = (uint32_t)min(max(dnode, 0), 63);
-    level[i] = (uint8_t)lv;
-    atomicAdd(&hist[lv], 1u);
-  }
-  __syncthreads();
-  if (threadIdx.x < 64 && hist[threadIdx.x])
-    atomicAdd(&lcount[threadIdx.x], hist[threadIdx.x]);
-}
+// One internal node per lane.  The sorted codes around the
+// workgroup's nodes (kKarrasHalo on either side) are staged in LDS: a node's
+// searches probe keys at most twice its range length away, so all but the
+// largest nodes search in LDS; other probes read memory.
+constexpr int kKarrasThreads = 1024;
+constexpr int64_t kKarrasHalo = 1024;
 
-__global__ void k_level_offsets(const uint32_t *lcount, uint32_t *loff, uint32_t *lfill)
-{
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int l = 0; l < 64; l++) {
-      loff[l] = run;
-      run += lcount[l];
-      lfill[l] = 0u;
-    }
+// delta(i, j) of the Karras paper with index-augmented keys; j is given as
+// the distance x from i in direction d (> 0: i + x, else i - x), so all index
+// arithmetic is 32-bit (n < 2^31): -1 when j is outside [0, n).
+struct KarrasProbe {
+  const uint32_t *sc;
+  const uint32_t *tile;
+  uint32_t t0, tn, n, i, ci;
+  int d;
+  __device__ __forceinline__ int operator()(uint32_t x) const
+  {
+    if (d > 0 ? x >= n - i : x > i)
+      return -1;
+    const uint32_t j = d > 0 ? i + x : i - x;
+    // the tile is always read (clamped index), memory only outside it: a
+    // select between the two addresses would become one slow flat load
+    const uint32_t tj = j - t0;
+    uint32_t b = tile[tj < tn ? tj : (j < t0 ? 0u : tn - 1u)];
+    if (tj >= tn)
+      b = gld(sc, j);
+    if (ci != b)
+      return __clz(ci ^ b);
+    return 32 + __clz(i ^ j);
   }
-}
+};
 
-// group internal nodes by prefix length (block-aggregated reservations)
-// Each workgroup buckets a chunk of kLevelChunk nodes: ranks within the chunk
-// come from LDS counters, and the chunk claims its slice of every level with
-// one global add per level, so the 64 level counters see few, large adds.
-constexpr int kLevelItems = 16;
-constexpr uint32_t kLevelChunk = 256 * kLevelItems;
-
-__global__ __launch_bounds__(256) void k_level_scatter(
-    const uint8_t *level, uint32_t n_internal, const uint32_t *loff, uint32_t *lfill, uint32_t *bucket)
+__global__ __launch_bounds__(kKarrasThreads) void k_karras(
+    const uint32_t *sc, uint32_t n, uint2 *child, uint8_t *level)
 {
-  __shared__ uint32_t cnt[64], base[64];
-  if (threadIdx.x < 64)
-    cnt[threadIdx.x] = 0u;
+  __shared__ uint32_t tile[kKarrasThreads + 2 * kKarrasHalo];
+  const uint32_t b0 = blockIdx.x * (uint32_t)kKarrasThreads;
+  const uint32_t t0 = b0 > (uint32_t)kKarrasHalo ? b0 - (uint32_t)kKarrasHalo : 0u;
+  const uint32_t t1 = (uint32_t)min((uint64_t)b0 + kKarrasThreads + kKarrasHalo, (uint64_t)n);
+  for (uint32_t q = t0 + threadIdx.x; q < t1; q += kKarrasThreads)
+    tile[q - t0] = sc[q];
   __syncthreads();
-  const uint32_t c0 = blockIdx.x * kLevelChunk;
-  uint32_t lv[kLevelItems], r[kLevelItems];
-#pragma unroll
-  for (int k = 0; k < kLevelItems; k++) {
-    const uint32_t i = c0 + (uint32_t)k * 256 + threadIdx.x;
-    lv[k] = 0;
-    r[k] = 0;
-    if (i < n_internal) {
-      lv[k] = level[i];
-      r[k] = atomicAdd(&cnt[lv[k]], 1u);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 64 && cnt[threadIdx.x])
-    base[threadIdx.x] = loff[threadIdx.x] + atomicAdd(&lfill[threadIdx.x], cnt[threadIdx.x]);
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kLevelItems; k++) {
-    const uint32_t i = c0 + (uint32_t)k * 256 + threadIdx.x;
-    if (i < n_internal)
-      bucket[base[lv[k]] + r[k]] = i;
-  }
+  const uint32_t i = b0 + threadIdx.x;
+  if (i >= n - 1)
+    return;
+  KarrasProbe kd{sc, tile, t0, t1 - t0, n, i, tile[i - t0], 1};
+  // direction: towards the neighbour sharing the longer prefix
+  const int up = kd(1u);
+  kd.d = -1;
+  const int down = kd(1u);
+  const int d = up - down > 0 ? 1 : -1;
+  kd.d = -d;
+  const int dmin = kd(1u);
+  kd.d = d;
+  uint32_t lmax = 2;
+  while (kd(lmax) > dmin)
+    lmax <<= 1;
+  uint32_t l = 0;
+  for (uint32_t t = lmax >> 1; t >= 1; t >>= 1)
+    if (kd(l + t) > dmin)
+      l += t;
+  const uint32_t j = d > 0 ? i + l : i - l;
+  const int dnode = kd(l);
+  uint32_t s = 0, t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (kd(s + t) > dnode)
+      s += t;
+  } while (t > 1);
+  const uint32_t gamma = d > 0 ? i + s : i - s - 1;
+  const uint32_t lo = min(i, j), hi = max(i, j);
+  const uint32_t left = (lo == gamma) ? (kLeafBit | gamma) : gamma;
+  const uint32_t right = (hi == gamma + 1) ? (kLeafBit | (gamma + 1)) : (gamma + 1);
+  child[i] = make_uint2(left, right);
+  // common-prefix length of the node's key range (0..63: distinct keys
+  // differ within 32 bits, equal keys within 31 index bits); every child
+  // range has a strictly longer one.  The refit pass follows from where
+  // the leaf range [lo, hi] lies.
+  const uint32_t lv = (uint32_t)min(max(dnode, 0), 63);
+  const uint32_t pass = (lo / kRefitChunk == hi / kRefitChunk) ? 1u
+      : (lo / kRefitSpan == hi / kRefitSpan) ? 2u : 3u;
+  level[i] = (uint8_t)(lv | (pass << kPassShift));
 }
 
 template <int T>
@@ -670,40 +732,381 @@ __global__ __launch_bounds__(256) void k_leaves(
     gst(dst, q, gld(src, q));
 }
 
-// One prefix-length level of the refit: every node of the level reads its
-// children (leaf records, or internal nodes finished by earlier levels --
-// kernel boundaries make them visible) and writes its whole 64-byte node.
+// the box and height of child link c of a node, from the leaf records or
+// from a finished internal node in memory
 template <int T>
-__global__ __launch_bounds__(256) void k_refit_level(vrx_prims p, const void *leaves,
-    const uint32_t *ids, const uint2 *child, const uint32_t *bucket, const uint32_t *loff,
-    const uint32_t *lcount, int lv, Node *nodes)
+__device__ __forceinline__ void child_box_global(const vrx_prims &p, const void *leaves,
+    const uint32_t *ids, const Node *nodes, uint32_t c, v3 &lo, v3 &hi, uint32_t &h)
 {
-  const uint32_t cnt = lcount[lv];
-  const uint32_t beg = loff[lv];
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
-    const uint32_t node = bucket[beg + q];
-    const uint2 ch = child[node];
-    v3 lo[2], hi[2];
-    uint32_t h[2];
+  if (c & kLeafBit) {
+    leaf_box<T>(p, leaves, ids, c & ~kLeafBit, lo, hi);
+    h = 1u;
+  } else {
+    const Node C = nodes[c];
+    lo = vmin(mk(C.a.x, C.a.y, C.a.z), mk(C.b.z, C.b.w, C.c.x));
+    hi = vmax(mk(C.a.w, C.b.x, C.b.y), mk(C.c.y, C.c.z, C.c.w));
+    h = 1u + max(C.d.z, C.d.w);
+  }
+}
+
+__device__ __forceinline__ Node make_node(uint2 ch, const v3 lo[2], const v3 hi[2], const uint32_t h[2])
+{
+  Node N;
+  N.a = make_float4(lo[0].x, lo[0].y, lo[0].z, hi[0].x);
+  N.b = make_float4(hi[0].y, hi[0].z, lo[1].x, lo[1].y);
+  N.c = make_float4(lo[1].z, hi[1].x, hi[1].y, hi[1].z);
+  N.d = make_uint4(ch.x, ch.y, h[0], h[1]);
+  return N;
+}
+
+// boxes and heights of an inside node's children, from the chunk's LDS
+__device__ __forceinline__ void chunk_children(uint2 ch, uint32_t base, const float (*s_leaf)[6],
+    const float (*s_box)[6], const uint8_t *s_h, v3 lo[2], v3 hi[2], uint32_t h[2])
+{
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
-      const uint32_t c = s ? ch.y : ch.x;
-      if (c & kLeafBit) {
-        leaf_box<T>(p, leaves, ids, c & ~kLeafBit, lo[s], hi[s]);
-        h[s] = 1u;
-      } else {
-        const Node C = nodes[c];
-        lo[s] = vmin(mk(C.a.x, C.a.y, C.a.z), mk(C.b.z, C.b.w, C.c.x));
-        hi[s] = vmax(mk(C.a.w, C.b.x, C.b.y), mk(C.c.y, C.c.z, C.c.w));
-        h[s] = 1u + max(C.d.z, C.d.w);
+  for (int sd = 0; sd < 2; sd++) {
+    const uint32_t c = sd ? ch.y : ch.x;
+    const uint32_t cl = (c & ~kLeafBit) - base;
+    const float *bx = (c & kLeafBit) ? s_leaf[cl] : s_box[cl];
+    lo[sd] = mk(bx[0], bx[1], bx[2]);
+    hi[sd] = mk(bx[3], bx[4], bx[5]);
+    h[sd] = (c & kLeafBit) ? 1u : (uint32_t)s_h[cl];
+  }
+}
+
+// Pass 1: one workgroup per chunk of kRefitChunk sorted leaves.  The chunk's
+// leaf boxes, child pairs and level bytes are loaded into LDS at once; each
+// inside node records itself as its children's parent, and every leaf then
+// walks up: at each inside parent the first of its two children to arrive
+// stops (LDS arrival counter), the second refits the parent from the two
+// boxes in LDS and goes on.  The walk stops below a parent that is not an
+// inside node.  No barrier between tree levels.
+template <int T>
+__global__ __launch_bounds__(kRefitThreads) void k_refit_chunk(vrx_prims p, const void *leaves,
+    const uint32_t *ids, const uint2 *child, const uint8_t *level, Node *nodes)
+{
+  __shared__ float s_leaf[kRefitChunk][6];
+  __shared__ float s_box[kRefitChunk][6];
+  __shared__ uint2 s_child[kRefitChunk];
+  __shared__ uint32_t s_arrive[kRefitChunk];
+  __shared__ uint16_t s_lpar[kRefitChunk], s_npar[kRefitChunk];
+  __shared__ uint8_t s_h[kRefitChunk], s_pass[kRefitChunk];
+  constexpr int kPer = (int)kRefitChunk / kRefitThreads;
+  constexpr uint16_t kNone = 0xFFFFu;
+  const uint32_t n = p.n;
+  const uint32_t base = blockIdx.x * kRefitChunk;
+  const uint32_t nleaf = min(kRefitChunk, n - base);
+  const uint32_t nnode = min(kRefitChunk, n - 1 - base); // candidates base .. base + nnode - 1
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < kPer; m++) {
+    const uint32_t q = (uint32_t)tid + (uint32_t)m * kRefitThreads;
+    s_lpar[q] = kNone;
+    s_npar[q] = kNone;
+    s_arrive[q] = 0u;
+    s_pass[q] = 0u;
+    if (q < nleaf) {
+      v3 lo, hi;
+      leaf_box<T>(p, leaves, ids, base + q, lo, hi);
+      s_leaf[q][0] = lo.x; s_leaf[q][1] = lo.y; s_leaf[q][2] = lo.z;
+      s_leaf[q][3] = hi.x; s_leaf[q][4] = hi.y; s_leaf[q][5] = hi.z;
+    }
+    if (q < nnode) {
+      s_child[q] = child[base + q];
+      s_pass[q] = (uint8_t)(level[base + q] >> kPassShift);
+    }
+  }
+  __syncthreads();
+  // an inside node's children are leaves or inside nodes of this chunk
+#pragma unroll
+  for (int m = 0; m < kPer; m++) {
+    const uint32_t q = (uint32_t)tid + (uint32_t)m * kRefitThreads;
+    if (q < nnode && s_pass[q] == 1u) {
+      const uint2 ch = s_child[q];
+#pragma unroll
+      for (int sd = 0; sd < 2; sd++) {
+        const uint32_t c = sd ? ch.y : ch.x;
+        const uint32_t cl = (c & ~kLeafBit) - base;
+        if (c & kLeafBit)
+          s_lpar[cl] = (uint16_t)q;
+        else
+          s_npar[cl] = (uint16_t)q;
       }
     }
-    Node N;
-    N.a = make_float4(lo[0].x, lo[0].y, lo[0].z, hi[0].x);
-    N.b = make_float4(hi[0].y, hi[0].z, lo[1].x, lo[1].y);
-    N.c = make_float4(lo[1].z, hi[1].x, hi[1].y, hi[1].z);
-    N.d = make_uint4(ch.x, ch.y, h[0], h[1]);
-    nodes[node] = N;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < kPer; m++) {
+    const uint32_t q = (uint32_t)tid + (uint32_t)m * kRefitThreads;
+    uint32_t par = q < nleaf ? s_lpar[q] : kNone;
+    while (par != kNone) {
+      // the box of the arriving child was stored before this add; the
+      // second arrival sees both
+      if (__hip_atomic_fetch_add(&s_arrive[par], 1u, __ATOMIC_ACQ_REL,
+              __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+        break;
+      v3 lo[2], hi[2];
+      uint32_t h[2];
+      chunk_children(s_child[par], base, s_leaf, s_box, s_h, lo, hi, h);
+      const v3 ulo = vmin(lo[0], lo[1]), uhi = vmax(hi[0], hi[1]);
+      s_box[par][0] = ulo.x; s_box[par][1] = ulo.y; s_box[par][2] = ulo.z;
+      s_box[par][3] = uhi.x; s_box[par][4] = uhi.y; s_box[par][5] = uhi.z;
+      s_h[par] = (uint8_t)(1u + max(h[0], h[1]));
+      par = s_npar[par];
+    }
+  }
+  __syncthreads();
+  // the chunk's inside nodes from LDS, written lane-contiguous (whole lines)
+#pragma unroll
+  for (int m = 0; m < kPer; m++) {
+    const uint32_t q = (uint32_t)tid + (uint32_t)m * kRefitThreads;
+    if (q < nnode && s_pass[q] == 1u) {
+      v3 lo[2], hi[2];
+      uint32_t h[2];
+      const uint2 ch = s_child[q];
+      chunk_children(ch, base, s_leaf, s_box, s_h, lo, hi, h);
+      nodes[base + q] = make_node(ch, lo, hi, h);
+    }
+  }
+}
+
+// refit one node from its children in memory (passes 2 and 3)
+template <int T>
+__device__ __forceinline__ void refit_node_global(const vrx_prims &p, const void *leaves,
+    const uint32_t *ids, uint2 ch, Node *nodes, uint32_t node)
+{
+  v3 lo[2], hi[2];
+  uint32_t h[2];
+  child_box_global<T>(p, leaves, ids, nodes, ch.x, lo[0], hi[0], h[0]);
+  child_box_global<T>(p, leaves, ids, nodes, ch.y, lo[1], hi[1], h[1]);
+  nodes[node] = make_node(ch, lo, hi, h);
+}
+
+// Passes 2 and 3 share this: `cnt` entries (key << 6 | delta) in s_in, node
+// node_of(key), are sorted
+// by delta into s_node (LDS counting sort), their child pairs loaded into
+// s_ch, and the nodes are refit delta by delta, longest first; a node's
+// children are finished nodes in memory (earlier launches, or this
+// workgroup's own nodes of a longer delta, visible after the barrier that
+// ends their delta) or leaves.
+template <int T, class NodeOf>
+__device__ __forceinline__ void refit_sorted(const vrx_prims &p, const void *leaves,
+    const uint32_t *ids, const uint2 *child, Node *nodes, const uint32_t *s_in, uint32_t cnt,
+    NodeOf node_of, uint32_t *s_node, uint2 *s_ch, uint32_t *s_cnt, uint32_t *s_off)
+{
+  const uint32_t nt = blockDim.x, tid = threadIdx.x;
+  if (tid < 64)
+    s_cnt[tid] = 0u;
+  __syncthreads();
+  for (uint32_t q = tid; q < cnt; q += nt)
+    atomicAdd(&s_cnt[s_in[q] & 63u], 1u);
+  __syncthreads();
+  if (tid < 64) {
+    const uint32_t c = s_cnt[tid];
+    uint32_t incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t v = __shfl_up(incl, off, 64);
+      if ((int)tid >= off)
+        incl += v;
+    }
+    s_off[tid] = incl - c;
+    s_off[64 + tid] = incl - c; // fill cursor
+  }
+  __syncthreads();
+  for (uint32_t q = tid; q < cnt; q += nt) {
+    const uint32_t slot = atomicAdd(&s_off[64 + (s_in[q] & 63u)], 1u);
+    const uint32_t node = node_of(s_in[q] >> 6);
+    s_node[slot] = node;
+    s_ch[slot] = child[node];
+  }
+  __syncthreads();
+  for (int lv = 63; lv >= 0; lv--) {
+    const uint32_t c = s_cnt[lv];
+    if (!c)
+      continue;
+    const uint32_t off = s_off[lv];
+    for (uint32_t q = tid; q < c; q += nt)
+      refit_node_global<T>(p, leaves, ids, s_ch[off + q], nodes, s_node[off + q]);
+    __syncthreads();
+  }
+}
+
+// Pass 2: one workgroup per span of kRefitSpan leaves refits the span's
+// nodes that cross a chunk border and lists those crossing a span border for
+// the top pass.  The span's nodes are listed in index order (ordered
+// compaction of the level bytes); each finds its parent's list slot by a
+// binary search, and the nodes are refit by bottom-up walks with LDS
+// arrival counters, as in pass 1, with the boxes in memory: a child node is
+// a pass-1 node of the previous launch, or a pass-2 node this workgroup
+// wrote before the arrival add that lets its parent go.
+template <int T>
+__global__ __launch_bounds__(kSpanThreads) void k_refit_span(vrx_prims p, const void *leaves,
+    const uint32_t *ids, const uint2 *child, const uint8_t *level, Node *nodes, uint32_t *top,
+    uint32_t *top_count, uint32_t top_cap)
+{
+  constexpr uint32_t kBytes = kRefitSpan / kSpanThreads; // level bytes per lane (128)
+  constexpr uint32_t kSlots = kSpanListCap / kSpanThreads; // list slots per lane (8)
+  constexpr uint16_t kNone = 0xFFFFu;
+  __shared__ uint32_t s_list[kSpanListCap];
+  __shared__ uint32_t s_need[kSpanListCap];
+  __shared__ uint16_t s_par[kSpanListCap];
+  __shared__ uint32_t s_wsum[kSpanThreads / 64];
+  __shared__ uint32_t s_total;
+  const uint32_t n = p.n;
+  const uint32_t base = blockIdx.x * kRefitSpan;
+  const uint32_t nnode = min(kRefitSpan, n - 1 - base);
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // 1. this lane's kBytes level bytes (16-byte loads; the span start and the
+  //    level array are 16-byte aligned)
+  const uint32_t b0 = tid * kBytes;
+  // 16 level bytes of this lane's range (zero past the span's last node)
+  auto bytes16 = [&](uint32_t v) {
+    const uint32_t q0 = b0 + v * 16;
+    if (q0 + 16 <= nnode)
+      return ((const uint4 *)(level + base))[(b0 >> 4) + v];
+    uint32_t t[4] = {0u, 0u, 0u, 0u};
+    for (uint32_t k = 0; k < 16 && q0 + k < nnode; k++)
+      t[k >> 2] |= (uint32_t)level[base + q0 + k] << (8 * (k & 3));
+    return make_uint4(t[0], t[1], t[2], t[3]);
+  };
+  auto pass_in = [](const uint4 &w, uint32_t k) {
+    const uint32_t x = k < 4 ? w.x : k < 8 ? w.y : k < 12 ? w.z : w.w;
+    return (x >> (8 * (k & 3) + kPassShift)) & 3u;
+  };
+  uint32_t mine = 0;
+  for (uint32_t v = 0; v < kBytes / 16; v++) {
+    const uint4 w = bytes16(v);
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++) {
+      const uint32_t pass = pass_in(w, k);
+      mine += pass == 2u ? 1u : 0u;
+      if (pass == 3u) {
+        const uint32_t t = atomicAdd(top_count, 1u);
+        if (t < top_cap)
+          top[t] = base + b0 + v * 16 + k;
+      }
+    }
+  }
+  // exclusive scan of the per-lane counts (wave scan, then wave totals)
+  uint32_t incl = mine;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = __shfl_up(incl, off, 64);
+    if ((int)lane >= off)
+      incl += v;
+  }
+  if (lane == 63)
+    s_wsum[wave] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (uint32_t w = 0; w < kSpanThreads / 64; w++) {
+      const uint32_t c = s_wsum[w];
+      s_wsum[w] = run;
+      run += c;
+    }
+    s_total = run;
+  }
+  __syncthreads();
+  uint32_t slot = s_wsum[wave] + incl - mine;
+  for (uint32_t v = 0; v < kBytes / 16; v++) {
+    const uint4 w = bytes16(v);
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++)
+      if (pass_in(w, k) == 2u) {
+        if (slot < kSpanListCap)
+          s_list[slot] = b0 + v * 16 + k;
+        slot++;
+      }
+  }
+  const uint32_t cnt = min(s_total, kSpanListCap);
+  for (uint32_t i = tid; i < kSpanListCap; i += kSpanThreads)
+    s_par[i] = kNone;
+  __syncthreads();
+  // 2. parent links among the listed nodes and the number of listed
+  //    children each waits for
+  uint32_t start = 0; // bit m: slot tid + m * kSpanThreads has no listed child
+#pragma unroll
+  for (uint32_t m = 0; m < kSlots; m++) {
+    const uint32_t i = tid + m * kSpanThreads;
+    if (i >= cnt)
+      continue;
+    const uint2 ch = child[base + s_list[i]];
+    uint32_t need = 0;
+#pragma unroll
+    for (int sd = 0; sd < 2; sd++) {
+      const uint32_t c = sd ? ch.y : ch.x;
+      if (c & kLeafBit)
+        continue;
+      const uint32_t cl = c - base; // in the span: the child's range is in its parent's
+      uint32_t lo = 0, hi = cnt;      // first slot with s_list >= cl
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_list[mid] < cl)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo < cnt && s_list[lo] == cl) {
+        s_par[lo] = (uint16_t)i;
+        need++;
+      }
+    }
+    s_need[i] = need;
+    start |= need == 0 ? (1u << m) : 0u;
+  }
+  __syncthreads();
+  // 3. walks from the nodes with no listed child
+#pragma unroll
+  for (uint32_t m = 0; m < kSlots; m++) {
+    if (!((start >> m) & 1u))
+      continue;
+    uint32_t i = tid + m * kSpanThreads;
+    for (;;) {
+      const uint32_t node = base + s_list[i];
+      refit_node_global<T>(p, leaves, ids, child[node], nodes, node);
+      const uint32_t par = s_par[i];
+      if (par == kNone)
+        break;
+      // the node store above happens before this add (release); the parent
+      // reads its children after the add that brings its count to zero
+      if (__hip_atomic_fetch_sub(&s_need[par], 1u, __ATOMIC_ACQ_REL,
+              __HIP_MEMORY_SCOPE_WORKGROUP) != 1u)
+        break;
+      i = par;
+    }
+  }
+}
+
+// Pass 3: one workgroup refits the listed nodes that cross a span border,
+// kSpanListCap at a time (a batch holds whole deltas only if the list fits;
+// beyond that the list is refit delta by delta from memory).
+template <int T>
+__global__ __launch_bounds__(kSpanThreads) void k_refit_top(vrx_prims p, const void *leaves,
+    const uint32_t *ids, const uint2 *child, const uint8_t *level, Node *nodes,
+    const uint32_t *top, const uint32_t *top_count, uint32_t top_cap)
+{
+  __shared__ uint32_t s_in[kSpanListCap];
+  __shared__ uint32_t s_node[kSpanListCap];
+  __shared__ uint2 s_ch[kSpanListCap];
+  __shared__ uint32_t s_cnt[64], s_off[128];
+  const uint32_t cnt = min(*top_count, top_cap);
+  if (cnt <= kSpanListCap) {
+    for (uint32_t q = threadIdx.x; q < cnt; q += kSpanThreads)
+      s_in[q] = (q << 6) | (level[top[q]] & 63u);
+    __syncthreads();
+    refit_sorted<T>(p, leaves, ids, child, nodes, s_in, cnt, [top](uint32_t k) { return top[k]; },
+        s_node, s_ch, s_cnt, s_off);
+    return;
+  }
+  for (int lv = 63; lv >= 0; lv--) {
+    for (uint32_t q = threadIdx.x; q < cnt; q += kSpanThreads) {
+      const uint32_t node = top[q];
+      if ((level[node] & 63u) == (uint32_t)lv)
+        refit_node_global<T>(p, leaves, ids, child[node], nodes, node);
+    }
+    __syncthreads();
   }
 }
 
@@ -840,9 +1243,9 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
 {
   const uint32_t n = p.n;
   const unsigned g256 = (n + 255) / 256;
-  hipLaunchKernelGGL(k_cbox_init, dim3(1), dim3(64), 0, s, ws.cbox);
-  unsigned gb = min(g256, 2048u);
-  hipLaunchKernelGGL(k_centroid_bounds<T>, dim3(gb), dim3(256), 0, s, p, ws.cbox);
+  const unsigned gb = min(g256, kCboxBlocks);
+  hipLaunchKernelGGL(k_centroid_bounds<T>, dim3(gb), dim3(256), 0, s, p, ws.cbox_part);
+  hipLaunchKernelGGL(k_cbox_final, dim3(1), dim3(256), 0, s, ws.cbox_part, gb, ws.cbox);
   hipLaunchKernelGGL(k_morton<T>, dim3(g256), dim3(256), 0, s, p, ws.cbox, ws.keys_a, ws.vals_a,
       cbox_out, ws.rec);
   VRX_LAUNCH_CHECK();
@@ -868,18 +1271,16 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
   if (n >= 2) {
     const uint32_t ni = n - 1;
     const unsigned gi = (ni + 255) / 256;
-    hipLaunchKernelGGL(k_fill_zero64, dim3(1), dim3(64), 0, s, ws.lcount);
-    hipLaunchKernelGGL(k_karras, dim3(min(gi, 4096u)), dim3(256), 0, s, ki, n, ws.child, ws.level,
-        ws.lcount);
-    hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, ws.lcount, ws.loff, ws.lfill);
-    hipLaunchKernelGGL(k_level_scatter, dim3((ni + kLevelChunk - 1) / kLevelChunk), dim3(256), 0, s,
-        ws.level, ni, ws.loff, ws.lfill, ws.bucket);
-    // grid-stride refit: most prefix-length levels are small or empty, so a
-    // modest grid keeps their launches cheap
-    const unsigned gl = min(gi, 1024u);
-    for (int lv = 63; lv >= 0; lv--)
-      hipLaunchKernelGGL((k_refit_level<T>), dim3(gl), dim3(256), 0, s, p, out.d_leaves,
-          out.d_leaf_ids, ws.child, ws.bucket, ws.loff, ws.lcount, lv, nodes);
+    hipLaunchKernelGGL(k_fill_zero64, dim3(1), dim3(64), 0, s, ws.top_count);
+    hipLaunchKernelGGL(k_karras, dim3((ni + kKarrasThreads - 1) / kKarrasThreads),
+        dim3(kKarrasThreads), 0, s, ki, n, ws.child, ws.level);
+    hipLaunchKernelGGL((k_refit_chunk<T>), dim3((ni + kRefitChunk - 1) / kRefitChunk),
+        dim3(kRefitThreads), 0, s, p, out.d_leaves, out.d_leaf_ids, ws.child, ws.level, nodes);
+    hipLaunchKernelGGL((k_refit_span<T>), dim3((ni + kRefitSpan - 1) / kRefitSpan),
+        dim3(kSpanThreads), 0, s, p, out.d_leaves, out.d_leaf_ids, ws.child, ws.level, nodes,
+        ws.top, ws.top_count, top_capacity(n));
+    hipLaunchKernelGGL((k_refit_top<T>), dim3(1), dim3(kSpanThreads), 0, s, p, out.d_leaves,
+        out.d_leaf_ids, ws.child, ws.level, nodes, ws.top, ws.top_count, top_capacity(n));
   }
   hipLaunchKernelGGL((k_root_header<T>), dim3(1), dim3(64), 0, s, p, out.d_leaves, out.d_leaf_ids,
       nodes, (vrx_bvh_header *)out.d_header);
