@@ -673,39 +673,57 @@ __device__ __forceinline__ void write_leaf(
   }
 }
 
-// box of sorted leaf k, from its leaf record (same bounds rule as prim_box)
+// float4s per leaf record
 template <int T>
-__device__ __forceinline__ void leaf_box(
-    const vrx_prims &p, const void *leaves, const uint32_t *ids, uint32_t k, v3 &lo, v3 &hi)
+constexpr int record_quads()
+{
+  return T == VRX_SPHERE ? 1
+      : (T == VRX_CYLINDER || T == VRX_CONE || T == VRX_CURVE) ? 2 : (T == VRX_TRIANGLE ? 3 : 0);
+}
+
+// box of a leaf record (same bounds rule as prim_box)
+template <int T>
+__device__ __forceinline__ void record_box(const float4 *L, v3 &lo, v3 &hi)
 {
   if constexpr (T == VRX_SPHERE) {
-    float4 L = ((const float4 *)leaves)[k];
-    lo = mk(L.x - L.w, L.y - L.w, L.z - L.w);
-    hi = mk(L.x + L.w, L.y + L.w, L.z + L.w);
+    const float4 a = L[0];
+    lo = mk(a.x - a.w, a.y - a.w, a.z - a.w);
+    hi = mk(a.x + a.w, a.y + a.w, a.z + a.w);
   } else if constexpr (T == VRX_CYLINDER) {
-    const float4 *L = (const float4 *)leaves + 2 * (size_t)k;
-    float4 a = L[0], b = L[1];
+    const float4 a = L[0], b = L[1];
     v3 mn = vmin(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z)), mx = vmax(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
     lo = mk(mn.x - a.w, mn.y - a.w, mn.z - a.w);
     hi = mk(mx.x + a.w, mx.y + a.w, mx.z + a.w);
   } else if constexpr (T == VRX_CONE) {
-    const float4 *L = (const float4 *)leaves + 2 * (size_t)k;
-    float4 a = L[0], b = L[1];
+    const float4 a = L[0], b = L[1];
     v3 mn = vmin(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z)), mx = vmax(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
     float r = fmaxf(a.w, b.w);
     lo = mk(mn.x - r, mn.y - r, mn.z - r);
     hi = mk(mx.x + r, mx.y + r, mx.z + r);
   } else if constexpr (T == VRX_CURVE) {
-    const float4 *L = (const float4 *)leaves + 2 * (size_t)k;
-    float4 a = L[0], b = L[1];
+    const float4 a = L[0], b = L[1];
     lo = vmin(mk(a.x - a.w, a.y - a.w, a.z - a.w), mk(b.x - b.w, b.y - b.w, b.z - b.w));
     hi = vmax(mk(a.x + a.w, a.y + a.w, a.z + a.w), mk(b.x + b.w, b.y + b.w, b.z + b.w));
   } else if constexpr (T == VRX_TRIANGLE) {
-    const float4 *L = (const float4 *)leaves + 3 * (size_t)k;
-    float4 a = L[0], b = L[1], c = L[2];
+    const float4 a = L[0], b = L[1], c = L[2];
     v3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
     lo = vmin(vmin(v0, v1), v2);
     hi = vmax(vmax(v0, v1), v2);
+  }
+}
+
+// box of sorted leaf k, from its leaf record
+template <int T>
+__device__ __forceinline__ void leaf_box(
+    const vrx_prims &p, const void *leaves, const uint32_t *ids, uint32_t k, v3 &lo, v3 &hi)
+{
+  if constexpr (record_quads<T>() > 0) {
+    const float4 *L = (const float4 *)leaves + (size_t)record_quads<T>() * k;
+    float4 r[record_quads<T>()];
+#pragma unroll
+    for (int q = 0; q < record_quads<T>(); q++)
+      r[q] = L[q];
+    record_box<T>(r, lo, hi);
   } else {
     prim_box<T>(p, ids[k], lo, hi);
   }
@@ -723,8 +741,7 @@ __global__ __launch_bounds__(256) void k_leaves(
   // gather the record the Morton pass wrote for primitive `orig`: one
   // contiguous 16/32/48-byte read instead of separate index, vertex and
   // radius gathers
-  constexpr int nq = T == VRX_SPHERE ? 1
-      : (T == VRX_CYLINDER || T == VRX_CONE || T == VRX_CURVE) ? 2 : (T == VRX_TRIANGLE ? 3 : 0);
+  constexpr int nq = record_quads<T>();
   const float4 *src = (const float4 *)rec + (size_t)nq * orig;
   float4 *dst = (float4 *)leaves + (size_t)nq * k;
 #pragma unroll
@@ -782,8 +799,8 @@ __device__ __forceinline__ void chunk_children(uint2 ch, uint32_t base, const fl
 // boxes in LDS and goes on.  The walk stops below a parent that is not an
 // inside node.  No barrier between tree levels.
 template <int T>
-__global__ __launch_bounds__(kRefitThreads) void k_refit_chunk(vrx_prims p, const void *leaves,
-    const uint32_t *ids, const uint2 *child, const uint8_t *level, Node *nodes)
+__global__ __launch_bounds__(kRefitThreads) void k_refit_chunk(vrx_prims p, const void *rec,
+    void *leaves, const uint32_t *ids, const uint2 *child, const uint8_t *level, Node *nodes)
 {
   __shared__ float s_leaf[kRefitChunk][6];
   __shared__ float s_box[kRefitChunk][6];
@@ -796,7 +813,8 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_chunk(vrx_prims p, cons
   const uint32_t n = p.n;
   const uint32_t base = blockIdx.x * kRefitChunk;
   const uint32_t nleaf = min(kRefitChunk, n - base);
-  const uint32_t nnode = min(kRefitChunk, n - 1 - base); // candidates base .. base + nnode - 1
+  // candidates base .. base + nnode - 1 (none in a last chunk of one leaf)
+  const uint32_t nnode = base + 1 < n ? min(kRefitChunk, n - 1 - base) : 0u;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int m = 0; m < kPer; m++) {
@@ -806,8 +824,22 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_chunk(vrx_prims p, cons
     s_arrive[q] = 0u;
     s_pass[q] = 0u;
     if (q < nleaf) {
+      // the leaf record of sorted leaf base + q: gathered from the record
+      // the Morton pass wrote for the primitive, stored in sorted order
+      const uint32_t orig = ids[base + q];
       v3 lo, hi;
-      leaf_box<T>(p, leaves, ids, base + q, lo, hi);
+      if constexpr (record_quads<T>() > 0) {
+        constexpr int nq = record_quads<T>();
+        float4 r[nq];
+#pragma unroll
+        for (int k = 0; k < nq; k++) {
+          r[k] = gld((const float4 *)rec + (size_t)nq * orig, k);
+          gst((float4 *)leaves + (size_t)nq * (base + q), k, r[k]);
+        }
+        record_box<T>(r, lo, hi);
+      } else {
+        prim_box<T>(p, orig, lo, hi);
+      }
       s_leaf[q][0] = lo.x; s_leaf[q][1] = lo.y; s_leaf[q][2] = lo.z;
       s_leaf[q][3] = hi.x; s_leaf[q][4] = hi.y; s_leaf[q][5] = hi.z;
     }
@@ -1253,6 +1285,8 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
   uint32_t *ki = ws.keys_a, *vi = ws.vals_a, *ko = ws.keys_b, *vo = ws.vals_b;
   for (int pass = 0; pass < 4; pass++) {
     uint32_t shift = 8u * (uint32_t)pass;
+    if (pass == 3)
+      vo = out.d_leaf_ids; // the sorted order is the leaf -> primitive map
     hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ki, n, shift, nb, ws.hist);
     hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(256), 0, s, ws.hist, nb, ws.row_total);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ki, vi, ko, vo, n,
@@ -1262,11 +1296,12 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
     t = vi; vi = vo; vo = t;
   }
   VRX_LAUNCH_CHECK();
-  // after 4 passes the sorted data is back in keys_a/vals_a
+  // after 4 passes the sorted keys are back in keys_a, the order in leaf_ids
   if (codes_out)
     hipLaunchKernelGGL(k_copy_u32, dim3(g256), dim3(256), 0, s, ki, codes_out, n);
-  hipLaunchKernelGGL((k_leaves<T>), dim3(g256), dim3(256), 0, s, p, vi, ws.rec, out.d_leaves,
-      out.d_leaf_ids);
+  if (n < 2) // otherwise the chunk refit gathers the leaf records
+    hipLaunchKernelGGL((k_leaves<T>), dim3(g256), dim3(256), 0, s, p, vi, ws.rec, out.d_leaves,
+        out.d_leaf_ids);
   Node *nodes = (Node *)out.d_nodes;
   if (n >= 2) {
     const uint32_t ni = n - 1;
@@ -1274,8 +1309,9 @@ static int build_typed(const vrx_prims &p, const vrx_bvh &out, const Workspace &
     hipLaunchKernelGGL(k_fill_zero64, dim3(1), dim3(64), 0, s, ws.top_count);
     hipLaunchKernelGGL(k_karras, dim3((ni + kKarrasThreads - 1) / kKarrasThreads),
         dim3(kKarrasThreads), 0, s, ki, n, ws.child, ws.level);
-    hipLaunchKernelGGL((k_refit_chunk<T>), dim3((ni + kRefitChunk - 1) / kRefitChunk),
-        dim3(kRefitThreads), 0, s, p, out.d_leaves, out.d_leaf_ids, ws.child, ws.level, nodes);
+    hipLaunchKernelGGL((k_refit_chunk<T>), dim3((n + kRefitChunk - 1) / kRefitChunk),
+        dim3(kRefitThreads), 0, s, p, ws.rec, out.d_leaves, out.d_leaf_ids, ws.child, ws.level,
+        nodes);
     hipLaunchKernelGGL((k_refit_span<T>), dim3((ni + kRefitSpan - 1) / kRefitSpan),
         dim3(kSpanThreads), 0, s, p, out.d_leaves, out.d_leaf_ids, ws.child, ws.level, nodes,
         ws.top, ws.top_count, top_capacity(n));
