@@ -255,6 +255,10 @@ typedef struct vrx_renderer {
   int32_t vol_unroll;     /* volume samples gathered per step: 1/2/4/8 */
   int32_t node_width;     /* BVH node width traversed: 2 (default) or 4 (needs scene->wide) */
   int32_t skip_empty;     /* 1: skip empty macro cells of volumes that carry d_majorant */
+  int32_t volume_jitter;  /* 1: each volume segment starts at a random offset, the two
+                             jitters of volumeIntegration.h:83,120 (PCG32 per pixel and
+                             frame, as the AO directions); 0: the first sample on the
+                             segment entry (deterministic) */
 } vrx_renderer;
 
 typedef struct vrx_frame {

@@ -1653,9 +1653,23 @@ static float eps_from(v3 P, v3 dir, float t)
 }
 
 /* rayMarchAllVolumes (volumeIntegration.h:317-350), marching in FP64. */
+/* The first sample of a segment (volumeIntegration.h:83,120): the entry t0
+   plus stepSize * u1 (the field's step, rayMarchVolume) plus the sampling
+   step * u2 (_rayMarchVolume), u1 and u2 two draws of the pixel's PCG32
+   sequence, when jit is given; the offsets are formed in fp32 as the device
+   forms them. */
+static double jittered_start(const orc_volume *v, double t0, float inv_rate, orc_pcg *jit)
+{
+  if (!jit)
+    return t0;
+  const float ss = fminf(fminf(v->spacing[0] / 2.0f, v->spacing[1] / 2.0f), v->spacing[2] / 2.0f);
+  const float u1 = orc_pcg_float(jit), u2 = orc_pcg_float(jit);
+  return t0 + (double)(ss * u1) + (double)((ss * inv_rate) * u2);
+}
+
 static float march_all(const orc_scene *s, v3 o, v3 d, float tlower, float tfar, int early,
     float inv_rate, v3 *color, float *opacity, uint32_t *obj, uint32_t *inst, uint64_t *ns,
-    int *amb)
+    int *amb, orc_pcg *jit)
 {
   float depth = tfar;
   int first = 1;
@@ -1688,7 +1702,8 @@ static float march_all(const orc_scene *s, v3 o, v3 d, float tlower, float tfar,
     }
     if ((float)bt0 < depth)
       depth = (float)bt0;
-    march_amb(&s->volumes[best], od, dd, bt0, bt1, (double)inv_rate, early, c, &op, ns, amb);
+    march_amb(&s->volumes[best], od, dd, jittered_start(&s->volumes[best], bt0, inv_rate, jit),
+        bt1, (double)inv_rate, early, c, &op, ns, amb);
     tl = bt1 + 1e-3;
     last = best;
     if (!(op < 0.99))
@@ -1785,7 +1800,7 @@ static float ao_factor(const orc_scene *s, const shit *h, float dist, int n, orc
    volume the shadow ray crosses on [0, tmax] marched into one opacity
    (FP64, no colour), volumes in index order, stopping at 0.99. */
 static float volume_atten(const orc_scene *s, v3 o, v3 d, float tmax, float inv_rate,
-    uint64_t *ns, int *amb)
+    uint64_t *ns, int *amb, orc_pcg *jit)
 {
   double op = 0.0, c[3] = {0.0, 0.0, 0.0};
   double od[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
@@ -1793,7 +1808,8 @@ static float volume_atten(const orc_scene *s, v3 o, v3 d, float tmax, float inv_
     double t0, t1;
     if (!vol_slab_amb(&s->volumes[i], od, dd, 0.0, (double)tmax, &t0, &t1, amb))
       continue;
-    march_amb(&s->volumes[i], od, dd, t0, t1, (double)inv_rate, 1, c, &op, ns, amb);
+    march_amb(&s->volumes[i], od, dd, jittered_start(&s->volumes[i], t0, inv_rate, jit), t1,
+        (double)inv_rate, 1, c, &op, ns, amb);
   }
   return (float)op;
 }
@@ -1863,7 +1879,8 @@ static v4 shade_direct(const orc_scene *s, const orc_renderer *r, const shit *h,
     float so = 1.0f - surface_atten(s, sp, ls.dir, 0.0f, ls.dist, st);
     float vo = 1.0f;
     if (s->num_volumes)
-      vo = 1.0f - volume_atten(s, sp, ls.dir, ls.dist, inv_rate, vs, amb);
+      vo = 1.0f - volume_atten(s, sp, ls.dir, ls.dist, inv_rate, vs, amb,
+                          r->volume_jitter ? rs : NULL);
     contrib = add(contrib, mulf(c, so * vo));
   }
   contrib = mulf(contrib, aof);
@@ -1921,7 +1938,7 @@ int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r,
           if (h.found) {
             uint32_t vo = ~0u, vi = ~0u;
             float vdepth = march_all(s, o, d, tlower, h.t, 1, inv_rate, &color, &opacity, &vo,
-                &vi, &s_vs, &amb);
+                &vi, &s_vs, &amb, r->volume_jitter ? &rs : NULL);
             if (first) {
               if (vdepth < h.t) {
                 depth = vdepth;
@@ -1976,7 +1993,7 @@ int orc_render(const orc_scene *s, const orc_camera *cam, const orc_renderer *r,
           } else {
             uint32_t vo = ~0u, vi = ~0u;
             float vdepth = march_all(s, o, d, tlower, tmax, 1, inv_rate, &color, &opacity, &vo,
-                &vi, &s_vs, &amb);
+                &vi, &s_vs, &amb, r->volume_jitter ? &rs : NULL);
             if (first) {
               depth = fminf(depth, vdepth);
               prim = 0;

@@ -170,6 +170,7 @@ typedef struct {
   float ao_distance;
   int32_t cull_backfaces;
   float volume_sampling_rate;
+  int32_t volume_jitter; /* 1: jittered volume segment starts (two PCG draws each) */
 } orc_renderer;
 
 typedef struct {

@@ -73,7 +73,7 @@ class orc_renderer(C.Structure):
                 ("ambient_color", C.c_float * 3), ("ambient_radiance", C.c_float),
                 ("pixel_samples", C.c_int32), ("ambient_samples", C.c_int32),
                 ("ao_distance", C.c_float), ("cull_backfaces", C.c_int32),
-                ("volume_sampling_rate", C.c_float)]
+                ("volume_sampling_rate", C.c_float), ("volume_jitter", C.c_int32)]
 
 
 class orc_frame(C.Structure):
@@ -444,6 +444,7 @@ class OracleScene:
         r.ao_distance = rp.get("ambientOcclusionDistance", 1e20)
         r.cull_backfaces = int(rp.get("cullTriangleBackfaces", 0))
         r.volume_sampling_rate = min(max(rp.get("volumeSamplingRate", 0.125), 1e-3), 10.0)
+        r.volume_jitter = int(bool(rp.get("volumeJitter", False)))
         c = camera(cam)
         f = orc_frame()
         f.width, f.height = width, height

@@ -195,7 +195,9 @@ std::vector<ParamDef> rendererParams(int kind)
       {"volumeUnroll", ANARI_INT32, false, &kInt0, &kInt0, nullptr, "volume samples gathered per step (0: default)"},
       {"nodeWidth", ANARI_INT32, false, &kInt4, &kInt2, &kInt4, "BVH node width traversed: 2 or 4"},
       {"emptySpaceSkipping", ANARI_BOOL, false, &kTrue1, nullptr, nullptr,
-          "skip empty 16^3 macro cells in volume marches (same image)"}};
+          "skip empty 16^3 macro cells in volume marches (same image)"},
+      {"volumeJitter", ANARI_BOOL, false, &kFalse, nullptr, nullptr,
+          "start each volume segment at a random offset (per-pixel PCG32)"}};
   if (ao) {
     p.push_back({"pixelSamples", ANARI_INT32, false, &kInt1, &kInt1, nullptr, "samples per pixel per frame"});
     p.push_back({"ambientSamples", ANARI_INT32, false, &kInt1, &kInt0, &kInt256, "occlusion rays per hit"});

@@ -1716,6 +1716,9 @@ void Renderer::commit()
   r.node_width = get<int>("nodeWidth", ANARI_INT32, 4) == 2 ? 2 : 4;
   // extension: skip empty macro cells in volume marches (same image)
   r.skip_empty = get<int>("emptySpaceSkipping", ANARI_BOOL, 1) != 0 ? 1 : 0;
+  // extension: the reference always jitters volume samples
+  // (volumeIntegration.h:83,120); here it is opt-in, off by default
+  r.volume_jitter = get<int>("volumeJitter", ANARI_BOOL, 0) != 0 ? 1 : 0;
   tuneWidth = find("nodeWidth") == nullptr;
   tuneStack = find("stackDepthLDS") == nullptr;
   sampleLimit = get<int>("sampleLimit", ANARI_INT32, 128);

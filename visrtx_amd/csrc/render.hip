@@ -446,7 +446,10 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
     return VRX_ERR_ARG;
   }
   hipStream_t st = (hipStream_t)s;
-  if (scene->num_entries == 0 && scene->num_volumes == 1 && r->kind == VRX_RENDER_RAYCAST) {
+  // the volume-only kernel marches from the segment entry; jittered starts
+  // take the generic kernel
+  if (scene->num_entries == 0 && scene->num_volumes == 1 && r->kind == VRX_RENDER_RAYCAST
+      && !r->volume_jitter) {
     dim3 grid((f->width + kTile - 1) / kTile, (f->height + kTile - 1) / kTile);
     const int u = r->vol_unroll > 0 ? r->vol_unroll : 4;
     const bool big = scene->max_volume_voxels >= (1ull << 31);

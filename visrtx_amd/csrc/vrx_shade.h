@@ -284,9 +284,21 @@ __device__ __forceinline__ bool vol_is_large(const vrx_volume &v)
 }
 
 // rayMarchAllVolumes (volumeIntegration.h:317-350)
+// The first sample of a segment, bt0 plus the two jitters of rayMarchVolume
+// (volumeIntegration.h:120, the field's step) and _rayMarchVolume (:83, the
+// sampling step) when jit is given: two PCG draws per segment, in march order.
+__device__ __forceinline__ float jittered_start(const vrx_volume &V, const VolCtx &c, float t0,
+    Pcg *jit)
+{
+  if (!jit)
+    return t0;
+  t0 += vol_step(V) * pcg_float(*jit);
+  return t0 + c.step * pcg_float(*jit);
+}
+
 __device__ float march_all(const vrx_scene &sc, v3 o, v3 d, float tlower, float tfar,
     float inv_rate, v3 &color, float &opacity, uint32_t &obj, uint32_t &inst, uint32_t &ns,
-    bool skip)
+    bool skip, Pcg *jit = nullptr)
 {
   float depth = tfar;
   bool first = true;
@@ -317,10 +329,11 @@ __device__ float march_all(const vrx_scene &sc, v3 o, v3 d, float tlower, float 
     }
     depth = fminf(depth, bt0);
     const VolCtx c = vol_ctx(V, inv_rate, skip);
+    const float ts = jittered_start(V, c, bt0, jit);
     if (vol_is_large(V))
-      march<true, 2, true>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
+      march<true, 2, true>(c, (const float4 *)V.d_lut, o, d, ts, bt1, color, opacity, ns);
     else
-      march<false, 2, true>(c, (const float4 *)V.d_lut, o, d, bt0, bt1, color, opacity, ns);
+      march<false, 2, true>(c, (const float4 *)V.d_lut, o, d, ts, bt1, color, opacity, ns);
     tl = bt1 + 1e-3f;
     last = best;
     if (!(opacity < 0.99f))
@@ -441,8 +454,8 @@ __device__ float ao_factor(const vrx_scene &sc, const SurfHit &s, float dist, in
 // :270-279): every volume the shadow ray crosses on [0, tmax] is marched into
 // one opacity (rayMarchVolume without colour, volumeIntegration.h:64-103),
 // volumes in index order, stopping at 0.99.
-__device__ float volume_atten(
-    const vrx_scene &sc, v3 o, v3 d, float tmax, float inv_rate, bool skip, uint32_t &ns)
+__device__ float volume_atten(const vrx_scene &sc, v3 o, v3 d, float tmax, float inv_rate,
+    bool skip, uint32_t &ns, Pcg *jit = nullptr)
 {
   float opacity = 0.0f;
   for (uint32_t i = 0; i < sc.num_volumes && opacity < 0.99f; i++) {
@@ -452,10 +465,11 @@ __device__ float volume_atten(
       continue;
     v3 unused = mk(0.0f, 0.0f, 0.0f);
     const VolCtx c = vol_ctx(V, inv_rate, skip);
+    const float ts = jittered_start(V, c, t0, jit);
     if (vol_is_large(V))
-      march<true, 2, true>(c, (const float4 *)V.d_lut, o, d, t0, t1, unused, opacity, ns);
+      march<true, 2, true>(c, (const float4 *)V.d_lut, o, d, ts, t1, unused, opacity, ns);
     else
-      march<false, 2, true>(c, (const float4 *)V.d_lut, o, d, t0, t1, unused, opacity, ns);
+      march<false, 2, true>(c, (const float4 *)V.d_lut, o, d, ts, t1, unused, opacity, ns);
   }
   return opacity;
 }
@@ -536,7 +550,7 @@ __device__ float4 shade_direct(const vrx_scene &sc, const vrx_renderer &r, const
     float vo = 1.0f;
     if (PT < 0 && sc.num_volumes)
       vo = 1.0f - volume_atten(sc, sp, ls.dir, ls.dist, r.inv_volume_sampling_rate,
-                          r.skip_empty != 0, vsamples);
+                          r.skip_empty != 0, vsamples, r.volume_jitter ? &rs : nullptr);
     const float att = so * vo;
     contrib = add(contrib, mulf(c, att));
   }
@@ -564,8 +578,9 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
   Pcg rs;
   // the sequence is keyed by the pixel within its view, so a view rendered
   // in a batch draws the same numbers as the same view rendered alone
-  if (kSampled)
+  if (kSampled || r.volume_jitter)
     pcg_seed(rs, (uint64_t)px + (uint64_t)py * f.width, (uint64_t)f.frame_id);
+  Pcg *jit = r.volume_jitter ? &rs : nullptr;
 
   float4 accum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float dcur = FLT_MAX;
@@ -606,7 +621,7 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
         float vdepth = s.t;
         if (PT < 0 && sc.num_volumes) // scenes with volumes use the generic kernel
           vdepth = march_all(sc, o, d, tlower, s.t, r.inv_volume_sampling_rate, color, opacity,
-              vo, vi, vsamples, r.skip_empty != 0);
+              vo, vi, vsamples, r.skip_empty != 0, jit);
         if (first) {
           if (vdepth < s.t) {
             depth = vdepth;
@@ -662,7 +677,7 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
         float vdepth = tmax;
         if (PT < 0 && sc.num_volumes) // scenes with volumes use the generic kernel
           vdepth = march_all(sc, o, d, tlower, tmax, r.inv_volume_sampling_rate, color, opacity,
-              vo, vi, vsamples, r.skip_empty != 0);
+              vo, vi, vsamples, r.skip_empty != 0, jit);
         if (first) {
           depth = fminf(depth, vdepth);
           prim = 0;
