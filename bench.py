@@ -287,29 +287,26 @@ def bench_diffvol(iters=200, log_every=50):
             "iters": iters, "mse_first": first, "mse_final": last, "rays": int(org.shape[0])}
 
 
-def bench_woodcock(vd, cam, w, h, reps=10):
+def bench_woodcock(dev, work, cam, w, h, reps=10):
     """Woodcock distance sampling (vrx_volume_sample_distance) of one camera
-    ray per pixel through the volume, over its majorant grid."""
+    ray per pixel through the world's volume, over its majorant grid: the
+    volume record (field, transfer-function LUT, grid) is the one the ANARI
+    device built for the world."""
     import ctypes as C
     import torch
     from visrtx_amd import _native as N
     from visrtx_amd import batch as B
-    from oracle import oracle as orc
     L = N.hip()
-    data = torch.from_numpy(np.ascontiguousarray(vd.data)).cuda()
-    lut = torch.from_numpy(orc.tf_discretize(vd.color, vd.opacity, vd.value_range)).cuda()
+    scene = B.scene_of(dev, work.s.world)
     v = N.vrx_volume()
-    v.d_data, v.d_lut = data.data_ptr(), lut.data_ptr()
-    nz, ny, nx = vd.data.shape
-    v.dims[:] = [nx, ny, nz]
-    v.origin[:] = list(vd.origin)
-    v.spacing[:] = list(vd.spacing)
-    v.value_range[:] = list(vd.value_range)
-    v.unit_distance = vd.unit_distance
+    N.check(L.vrx_memcpy_sync(C.c_void_p(C.addressof(v)), C.c_void_p(scene.d_volumes),
+                              C.c_size_t(C.sizeof(v)), N.VRX_D2H), "volume record")
+    if not v.d_majorant:
+        raise RuntimeError("the world attached no majorant grid to the volume")
     cells = L.vrx_volume_grid_cells(C.byref(v))
-    grid = torch.empty(cells, device="cuda", dtype=torch.float32)
-    N.check(L.vrx_volume_majorant_grid(C.byref(v), C.c_void_p(grid.data_ptr()), None), "grid")
-    v.d_majorant = grid.data_ptr()
+    grid = np.empty(cells, np.float32)
+    N.check(L.vrx_memcpy_sync(C.c_void_p(grid.ctypes.data), C.c_void_p(v.d_majorant),
+                              C.c_size_t(grid.nbytes), N.VRX_D2H), "majorant grid")
     n = w * h
     cs = B.camera_struct(cam)
     org = torch.tensor(list(cs.pos), device="cuda").expand(n, 3).contiguous()
@@ -342,7 +339,8 @@ def bench_woodcock(vd, cam, w, h, reps=10):
     ms = e0.elapsed_time(e1) / reps
     return {"rays": n, "ms": round(ms, 4), "mrays_s": round(n / (ms / 1000) / 1e6, 2),
             "collision_fraction": round(float((tr == 0).float().mean()), 4),
-            "empty_cell_fraction": round(float((grid == 0).float().mean()), 4)}
+            "empty_cell_fraction": round(float((grid == 0).mean()), 4),
+            "lut": "built by the device's transferFunction1D volume"}
 
 
 def bench_batched(dev, reps=5):
@@ -438,28 +436,55 @@ def bench_batched(dev, reps=5):
 
 
 def cpu_baseline(threads):
-    """The oracle (C/OpenMP port of the same path) on a bounded sample of the
-    10M-sphere workload: same scene and camera, 640x360 rays."""
+    """The oracle (C/OpenMP port of the same path) on the host cores: the
+    headline 10M-sphere frame at full size (1920x1080 primary raycast rays,
+    same camera), median of 5 after a warm-up; plus the oracle's figures for
+    the 131k-triangle heightfield (full 1024x768 frame) and for the 512^3
+    volume march (FP64, a 480x270 frame of the same camera, Gsamples/s)."""
     from oracle import oracle as orc
     from visrtx_amd import scenes as S
+
+    def timed(fn, reps):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return statistics.median(ts)
+
     sc, cam = S.spheres_workload()
     t0 = time.perf_counter()
     o = orc.OracleScene(sc)
     build_s = time.perf_counter() - t0
-    w, h = 640, 360
-    o.render(cam, 160, 90, "raycast", threads=threads)  # warm-up
-    times = []
-    for _ in range(3):
-        t = time.perf_counter()
-        o.render(cam, w, h, "raycast", threads=threads)
-        times.append(time.perf_counter() - t)
+    w, h = 1920, 1080
+    med = timed(lambda: o.render(cam, w, h, "raycast", threads=threads), 5)
     o.close()
-    med = statistics.median(times)
-    return {"value": round(w * h / med / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+    del sc
+    sc1, cam1 = S.heightfield_workload()
+    o1 = orc.OracleScene(sc1)
+    med1 = timed(lambda: o1.render(cam1, 1024, 768, "raycast", threads=threads), 5)
+    o1.close()
+    sc3, cam3 = S.volume_workload(width=480, height=270)
+    o3 = orc.OracleScene(sc3)
+    box = {}
+
+    def vol():
+        box["out"] = o3.render(cam3, 480, 270, "raycast", threads=threads, volumeSamplingRate=1.0)
+
+    med3 = timed(vol, 3)
+    samples3 = int(box["out"]["stats"][3])
+    o3.close()
+    base = {"value": round(w * h / med / 1e6, 3), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
-            "sample": f"10M-sphere scene, {w}x{h} primary raycast rays (same camera), oracle "
-                      f"LBVH + BVH2 traversal in C/OpenMP, median of 3; oracle LBVH build "
-                      f"{build_s * 1000:.0f} ms"}
+            "sample": f"10M-sphere scene, the full {w}x{h} primary raycast frame (same camera), "
+                      f"oracle LBVH + BVH2 traversal in C/OpenMP, median of 5; oracle LBVH "
+                      f"build {build_s * 1000:.0f} ms"}
+    others = {"heightfield_1024x768_mrays_s": round(1024 * 768 / med1 / 1e6, 3),
+              "volume512_480x270_gsamples_s": round(samples3 / med3 / 1e9, 4),
+              "volume512_480x270_samples": samples3, "cores": threads,
+              "kind": "port (FP64 march for the volume)"}
+    return base, others
 
 
 SECTIONS = ("headline", "ao", "direct", "instances", "heightfield", "volume", "sparse",
@@ -510,6 +535,13 @@ def main():
     if "headline" in sec:
         lbvh_ms, lbvh_all = bench_lbvh(dev, work)
         extra["lbvh_build_ms_10m_spheres"] = round(lbvh_ms, 3)
+        # the survey's build bytes per primitive: AABB read 24, (key, index)
+        # write 8, 16 per radix pass (4 passes here), Karras node 32, refit 100
+        bpp = 24 + 8 + 16 * 4 + 32 + 100
+        gbs_b = 10_000_000 * bpp / (lbvh_ms / 1000) / 1e9
+        extra["lbvh_roofline"] = {"bound": "hbm", "algorithmic_bytes_per_prim": bpp,
+                                  "achieved_gbs": round(gbs_b, 1),
+                                  "frac": round(gbs_b / HBM_PEAK_GBS, 4)}
         log(f"LBVH build {lbvh_ms:.3f} ms (all {['%.3f' % x for x in lbvh_all]})")
         sec_per_step, durs = work.timed(args.steps, args.warmup)
         value = rays / sec_per_step / 1e6
@@ -617,6 +649,14 @@ def main():
         ow = hf.pin_other_width()
         _, d12 = hf.timed(20, 3)
         extra[f"heightfield_kernel_ms_width{ow}"] = round(statistics.mean(d12) * 1000, 4)
+        # roofline of the tuned kernel: the survey's traversal bytes per ray
+        # (ray in, hit out, 64-B nodes and 48-B triangles the oracle visits)
+        c1 = counts["heightfield_1024x768"]
+        bpr1 = 32 + 16 + 64 * c1["nodes_per_ray"] + 48 * c1["prims_per_ray"]
+        gbs1 = 1024 * 768 * bpr1 / (extra["heightfield_kernel_ms"] / 1000) / 1e9
+        extra["heightfield_roofline"] = {"bound": "hbm", "algorithmic_bytes_per_ray": round(bpr1, 2),
+                                         "achieved_gbs": round(gbs1, 1),
+                                         "frac": round(gbs1 / HBM_PEAK_GBS, 4)}
         log(f"heightfield: {sps * 1000:.3f} ms/step -> {extra['heightfield_mrays_s']} Mrays/s")
         hf.release()
 
@@ -651,6 +691,9 @@ def main():
                           emptySpaceSkipping=skip)
             _, d4 = sv.timed(5, 1)
             res[skip] = (statistics.mean(d4) * 1000, sv.samples())
+            if skip:
+                extra["woodcock_sparse_512"] = bench_woodcock(dev, sv, cam4, W, H)
+                log(f"woodcock: {extra['woodcock_sparse_512']['mrays_s']} Mrays/s")
             sv.release()
         full = res[False][1]
         extra["volume_sparse_512"] = {
@@ -660,13 +703,19 @@ def main():
             "samples_no_skip": int(full), "samples_skip": int(res[True][1]),
             "effective_gsamples_s_skip": round(full / (res[True][0] / 1000) / 1e9, 2)}
         log(f"sparse volume: {res[False][0]:.3f} ms -> {res[True][0]:.3f} ms with skipping")
-        extra["woodcock_sparse_512"] = bench_woodcock(sc4.volumes[0], cam4, W, H)
-        log(f"woodcock: {extra['woodcock_sparse_512']['mrays_s']} Mrays/s")
         del sc4
 
     if "batched" in sec:
         log("batched 64 views")
         bt = bench_batched(dev)
+        # roofline of the graph-replayed launch: traversal bytes per ray from
+        # the oracle's counts on views 0 and 37 (BLAS nodes and triangles)
+        c4 = counts["batched_views0_37_512"]
+        bpr4 = 32 + 16 + 64 * c4["nodes_per_ray"] + 48 * c4["prims_per_ray"]
+        gbs4 = bt["rays"] * bpr4 / (bt["batched_graph_event_ms"] / 1000) / 1e9
+        bt["roofline"] = {"bound": "hbm", "algorithmic_bytes_per_ray": round(bpr4, 1),
+                          "achieved_gbs": round(gbs4, 1), "frac": round(gbs4 / HBM_PEAK_GBS, 4),
+                          "counts": "oracle, views 0 and 37"}
         extra["batched_64x512"] = bt
         log(f"batched: plain {bt['plain_ms']} ms, graph {bt['batched_graph_ms']} ms "
             f"-> {bt['speedup_vs_plain']}x ({bt['batched_mrays_s']} Mrays/s)")
@@ -686,9 +735,15 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        # the host cores this GPU's share of the machine has (16 per GPU),
+        # or fewer where the host has fewer
         threads = min(os.cpu_count() or 1, 16)
         log(f"cpu baseline with {threads} threads")
-        cpu = cpu_baseline(threads)
+        cpu, cpu_other = cpu_baseline(threads)
+        extra["cpu_oracle_other"] = cpu_other
+        log(f"cpu baseline: {cpu['value']} Mrays/s; heightfield "
+            f"{cpu_other['heightfield_1024x768_mrays_s']} Mrays/s; volume "
+            f"{cpu_other['volume512_480x270_gsamples_s']} Gsamples/s")
 
     out = {
         "metric": "Mrays/s closest-hit, volume Gsamples/s, fwd+bwd ms/iter on 1 MI355X",

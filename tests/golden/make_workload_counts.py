@@ -37,6 +37,21 @@ def main():
     sc, cam = S.spheres_workload()
     res["spheres10m_1920x1080"] = traversal_counts(sc, cam, 1920, 1080)
     print(res["spheres10m_1920x1080"], flush=True)
+    # the batched scene: views 0 and 37 of the 64 (BLAS nodes and triangles;
+    # instance-level nodes and transforms are not counted)
+    sc, cams = S.batched_workload()
+    o = orc.OracleScene(sc)
+    t = time.time()
+    nodes = prims = rays = 0
+    for v in (0, 37):
+        st = o.render(cams[v], 512, 512, "raycast")["stats"]
+        nodes, prims, rays = nodes + int(st[0]), prims + int(st[1]), rays + int(st[2])
+    res["batched_views0_37_512"] = {"width": 512, "height": 512, "views": [0, 37], "rays": rays,
+                                    "internal_nodes": nodes, "prims_tested": prims,
+                                    "nodes_per_ray": nodes / rays, "prims_per_ray": prims / rays,
+                                    "oracle_seconds": time.time() - t}
+    print(res["batched_views0_37_512"], flush=True)
+    o.close()
     sc, cam = S.volume_workload()
     t = time.time()
     _, ns = orc.volume_image(sc.volumes[0], cam, 1920, 1080, inv_sampling_rate=1.0,
