@@ -274,8 +274,18 @@ typedef struct vrx_frame {
   uint32_t *d_inst_id;
   void *d_scratch;        /* traversal stack overflow; size from vrx_render_scratch_bytes */
   size_t scratch_bytes;
-  uint64_t *d_counters;   /* optional [0]=rays traced, [1]=volume samples (atomics) */
+  uint64_t *d_counters;   /* optional: VRX_COUNTER_SLOTS slots of VRX_COUNTER_STRIDE u64;
+                             slot s holds [s*VRX_COUNTER_STRIDE + 0] rays traced and
+                             [... + 1] volume samples, added by atomics (the caller zeroes
+                             them); the frame's totals are the sums over the slots */
 } vrx_frame;
+
+/* Frame counters are spread over slots (a workgroup adds into slot
+   workgroup-index mod VRX_COUNTER_SLOTS, one 128-byte line per slot) so that
+   the per-wave atomics of a launch do not all meet at one address. */
+#define VRX_COUNTER_SLOTS 64
+#define VRX_COUNTER_STRIDE 16
+#define VRX_COUNTER_BYTES (VRX_COUNTER_SLOTS * VRX_COUNTER_STRIDE * 8)
 
 size_t vrx_render_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r,
     uint32_t width, uint32_t height);

@@ -1746,15 +1746,25 @@ bool Renderer::getProperty(
 // ---------------------------------------------------------------------------
 // Frame (frame/Frame.cu:75-310, 312-443, 559-660)
 // ---------------------------------------------------------------------------
+uint64_t Frame::counterTotal(int c) const
+{
+  uint64_t t = 0;
+  if (hReadback)
+    for (int s = 0; s < VRX_COUNTER_SLOTS; s++)
+      t += hReadback[s * VRX_COUNTER_STRIDE + c];
+  return t;
+}
+
 Frame::Frame(Device *d) : Object(d, ANARI_FRAME, "")
 {
   vrx_event_create(&evStart);
   vrx_event_create(&evEnd);
   vrx_event_create(&evDone);
   void *p = nullptr;
-  if (vrx_host_malloc(&p, 32) == 0) {
+  // the counter slots, then the traversal stack-overflow word
+  if (vrx_host_malloc(&p, kReadbackBytes) == 0) {
     hReadback = (uint64_t *)p;
-    memset(hReadback, 0, 32);
+    memset(hReadback, 0, kReadbackBytes);
   }
 }
 
@@ -1815,7 +1825,7 @@ void Frame::allocate()
   bool f32 = colorType == ANARI_FLOAT32_VEC4;
   bool ok = dev->check(vrx_malloc(&dAccum, np * 16), this, "frame buffer")
       && dev->check(vrx_malloc(&dColor, np * (f32 ? 16 : 4)), this, "frame buffer")
-      && dev->check(vrx_malloc(&dCounters, 16), this, "frame counters");
+      && dev->check(vrx_malloc(&dCounters, VRX_COUNTER_BYTES), this, "frame counters");
   if (ok && depthType == ANARI_FLOAT32)
     ok = dev->check(vrx_malloc(&dDepth, np * 4), this, "depth buffer");
   if (ok && primType == ANARI_UINT32)
@@ -1834,7 +1844,7 @@ void Frame::wait()
   vrx_event_sync(evDone);
   // the traversal stacks are sized from the BVH heights, so this flag stays
   // clear; it is checked so that a violated bound can never pass silently
-  if (hReadback && (uint32_t)hReadback[2] != 0 && !stackReported) {
+  if (hReadback && (uint32_t)hReadback[kStackWord] != 0 && !stackReported) {
     report(ANARI_SEVERITY_ERROR, ANARI_STATUS_UNKNOWN_ERROR,
         "traversal stack overflow in the last frame: results are incomplete");
     stackReported = true;
@@ -1964,13 +1974,13 @@ void Frame::render()
   f.d_scratch = dScratch;
   f.scratch_bytes = scratchBytes;
   f.d_counters = (uint64_t *)dCounters;
-  vrx_fill_u32(dCounters, 0u, 4, dev->stream);
+  vrx_fill_u32(dCounters, 0u, VRX_COUNTER_BYTES / 4, dev->stream);
   vrx_event_record(evStart, dev->stream);
   dev->check(vrx_render(&world->scene, &camera->cam, &rr, &f, dev->stream), this, "render");
   vrx_event_record(evEnd, dev->stream);
   if (hReadback) {
-    vrx_memcpy(hReadback, dCounters, 16, VRX_D2H, dev->stream);
-    vrx_memcpy(hReadback + 2, dScratch, 4, VRX_D2H, dev->stream);
+    vrx_memcpy(hReadback, dCounters, VRX_COUNTER_BYTES, VRX_D2H, dev->stream);
+    vrx_memcpy(hReadback + kStackWord, dScratch, 4, VRX_D2H, dev->stream);
   }
   if (callback) {
     // Frame.cu:295-304 calls the callback from cudaLaunchHostFunc; here the
@@ -2043,7 +2053,6 @@ const void *Frame::map(const std::string &channel, uint32_t *w, uint32_t *h, ANA
 
 bool Frame::getProperty(const std::string &name, ANARIDataType t, void *mem, uint64_t sz, uint32_t mask)
 {
-  static const uint64_t kZero64 = 0;
   if (name == "duration" && t == ANARI_FLOAT32 && sz >= 4) {
     if (mask & ANARI_WAIT)
       wait();
@@ -2078,12 +2087,14 @@ bool Frame::getProperty(const std::string &name, ANARIDataType t, void *mem, uin
   }
   if (name == "raysTraced" && t == ANARI_UINT64 && sz >= 8) {
     wait();
-    memcpy(mem, hReadback ? &hReadback[0] : &kZero64, 8);
+    const uint64_t v = counterTotal(0);
+    memcpy(mem, &v, 8);
     return true;
   }
   if (name == "volumeSamples" && t == ANARI_UINT64 && sz >= 8) {
     wait();
-    memcpy(mem, hReadback ? &hReadback[1] : &kZero64, 8);
+    const uint64_t v = counterTotal(1);
+    memcpy(mem, &v, 8);
     return true;
   }
   return Object::getProperty(name, t, mem, sz, mask);

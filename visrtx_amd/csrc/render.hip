@@ -54,8 +54,8 @@ __global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam,
     render_pixel<PT, TWO, KIND, WIDE>(sc, cam, r, f, px, py, (size_t)px + (size_t)py * f.width, st, rays,
         vsamples);
   if (f.d_counters) {
-    wave_count_add(f.d_counters, rays);
-    wave_count_add(f.d_counters + 1, vsamples);
+    wave_count_add(f.d_counters, 0, rays);
+    wave_count_add(f.d_counters, 1, vsamples);
   }
 }
 
@@ -63,9 +63,12 @@ __global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam,
 // the same per-pixel program as k_render's miss branch (the march of
 // rayMarchAllVolumes, the renderer's colour-times-opacity, background), with
 // the transfer-function LUT staged in LDS and no traversal state, so the
-// kernel needs about half the registers of the general one.
+// kernel needs about half the registers of the general one.  One lane per ray
+// (march()): the kernel for volumes with a macro-cell grid, whose
+// empty-space skipping walks cells one dependent lookup at a time, so the
+// launch wants as many rays in flight as it can hold.
 template <bool I64, int U, bool SKIP>
-__global__ __launch_bounds__(kBlock) void k_render_volume(vrx_scene sc, vrx_camera cam,
+__global__ __launch_bounds__(kBlock) void k_render_volume_seq(vrx_scene sc, vrx_camera cam,
     vrx_renderer r, vrx_frame f)
 {
   __shared__ float4 s_lut[256];
@@ -126,8 +129,209 @@ __global__ __launch_bounds__(kBlock) void k_render_volume(vrx_scene sc, vrx_came
     write_pixel(f, idx, accum, 1, dcur, ids_changed, pid, oid, iid);
   }
   if (f.d_counters) {
-    wave_count_add(f.d_counters, px < f.width && py < f.height ? 1u : 0u);
-    wave_count_add(f.d_counters + 1, vsamples);
+    wave_count_add(f.d_counters, 0, px < f.width && py < f.height ? 1u : 0u);
+    wave_count_add(f.d_counters, 1, vsamples);
+  }
+}
+
+// Quad-lane helpers: lanes 4q..4q+3 of a wave hold the four consecutive
+// samples of one ray; quad_perm DPP moves read within the quad.
+#define VRX_QUAD_MOV(v, ctrl)                                                                    \
+  __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, (v)), (ctrl), 0xF, \
+                                0xF, false))
+constexpr int kQuadShr1 = 0x90;  // lane j reads lane max(j-1, 0)
+constexpr int kQuadShr2 = 0x44;  // lane j reads lane j-2 (lanes 0, 1 read themselves)
+constexpr int kQuadLast = 0xFF;  // every lane reads lane 3
+constexpr int kQuadXor1 = 0xB1;  // lane j reads lane j^1
+constexpr int kQuadXor2 = 0x4E;  // lane j reads lane j^2
+constexpr uint32_t kVolTile = 8; // pixels per side of a workgroup's tile (four 4x4 waves)
+
+// _rayMarchVolume (volumeIntegration.h:64-103), jitter off, with four lanes
+// per ray: lane j of a ray's quad takes samples 4m + j (m = 0, 1, ...), so a
+// wave's 64 lanes gather for 16 neighbouring rays at 4 consecutive positions
+// each, about half the distinct cache lines per gather instruction of one lane
+// per ray (the march is bound by the L1 tag rate of its gathers).  Sample
+// positions, the trilinear gather and the classification are those of
+// march(); each block of four consecutive samples is composited at once:
+//   P_j = st_0 ... st_j (the block's step transmittances, a quad scan);
+//   the opacity before sample j is opacity + T (1 - P_{j-1}), and sample j is
+//   taken while every sample up to it lies inside the segment and found that
+//   opacity below 0.99 (the sequential march's test);
+//   a taken sample adds w_j = T P_{j-1} (1 - st_j) to its lane's colour and
+//   opacity partials (summed over the quad once, at the end);
+//   a fully taken block moves T to T P_3 and opacity to opacity + T (1 - P_3).
+// This regroups the sequential products and sums of march() (rounding-level
+// differences, bounded by the FP64 oracle like march() itself).  Sixteen rays
+// per wave suit dense fields, where a ray's cost is its samples; volumes with
+// a macro-cell grid take the one-lane-per-ray kernel.
+template <bool I64, int U, int XP>
+__device__ __forceinline__ void march_quad_xp(const VolCtx &c, const float4 *lut, v3 o, v3 d,
+    float t0, float t1, uint32_t j, uint32_t lane, v3 &color, float &opacity, uint32_t &ns)
+{
+  const float gox = (o.x - c.origin[0]) * c.inv_sp[0], gdx = d.x * c.inv_sp[0];
+  const float goy = (o.y - c.origin[1]) * c.inv_sp[1], gdy = d.y * c.inv_sp[1];
+  const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
+  float T = 1.0f, op = 0.0f; // the ray's transmittance and opacity before the block
+  for (uint32_t k = 0; k < kMaxSamplesPerSegment; k += 4 * U) {
+    float sv[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const float t = __builtin_fmaf((float)(k + 4 * u + j), c.step, t0);
+      ok[u] = t <= t1;
+      const float gx = __builtin_fmaf(gdx, t, gox), gy = __builtin_fmaf(gdy, t, goy),
+                  gz = __builtin_fmaf(gdz, t, goz);
+      if constexpr (XP == 2)
+        sv[u] = trilinear_b32(c, gx, gy, gz);
+      else
+        sv[u] = trilinear<I64, XP>(c, gx, gy, gz);
+    }
+    bool stop = false;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!stop) {
+        const float s = sv[u];
+        float st = 1.0f;
+        float4 co = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ok[u] && s == s) {
+          co = classify_fma<false>(lut, c, s);
+          st = __builtin_amdgcn_exp2f(c.expo * __builtin_amdgcn_logf(1.0f - co.w));
+        }
+        // inclusive and exclusive products of st over the quad
+        float p = st;
+        {
+          const float y = VRX_QUAD_MOV(p, kQuadShr1);
+          p = j >= 1 ? p * y : p;
+        }
+        {
+          const float y = VRX_QUAD_MOV(p, kQuadShr2);
+          p = j >= 2 ? p * y : p;
+        }
+        const float y = VRX_QUAD_MOV(p, kQuadShr1);
+        const float pe = j >= 1 ? y : 1.0f;
+        const float tb = T * pe; // transmittance before this sample
+        const bool cont = ok[u] && op + (T - tb) < 0.99f;
+        // a sample is taken when it and every earlier one of its block continue
+        const uint64_t m = __ballot(cont);
+        uint64_t a = m & ((m << 1) | 0x1111111111111111ull);
+        a &= (a << 2) | 0x3333333333333333ull;
+        if ((a >> lane) & 1ull) {
+          ns++;
+          const float w = tb * (1.0f - st);
+          color.x = __builtin_fmaf(w, co.x, color.x);
+          color.y = __builtin_fmaf(w, co.y, color.y);
+          color.z = __builtin_fmaf(w, co.z, color.z);
+          opacity += w;
+        }
+        if (((a >> (lane & ~3u)) & 0xFull) != 0xFull) {
+          stop = true;
+        } else {
+          const float pl = VRX_QUAD_MOV(p, kQuadLast);
+          op = op + (T - T * pl);
+          T = T * pl;
+        }
+      }
+    }
+    if (stop)
+      break;
+  }
+}
+
+template <bool I64, int U>
+__device__ __forceinline__ void march_quad(const VolCtx &c, const float4 *lut, v3 o, v3 d, float t0,
+    float t1, uint32_t j, uint32_t lane, v3 &color, float &opacity, uint32_t &ns)
+{
+  if (!(c.step > 0.0f))
+    return;
+  // XP: 2 the b32 gathers, 1 x-pair gathers with 64-bit offsets, 0 a field
+  // one voxel wide in x (one branch per segment, outside the sample loop)
+  if (!I64 && c.b32)
+    march_quad_xp<I64, U, 2>(c, lut, o, d, t0, t1, j, lane, color, opacity, ns);
+  else if (c.dims[0] >= 2)
+    march_quad_xp<I64, U, 1>(c, lut, o, d, t0, t1, j, lane, color, opacity, ns);
+  else
+    march_quad_xp<I64, U, 0>(c, lut, o, d, t0, t1, j, lane, color, opacity, ns);
+}
+
+__device__ __forceinline__ float quad_sum(float v)
+{
+  v = v + VRX_QUAD_MOV(v, kQuadXor1);
+  return v + VRX_QUAD_MOV(v, kQuadXor2);
+}
+
+// The same program for a volume without a macro-cell grid (dense fields): a
+// 256-lane workgroup covers an 8x8 pixel tile, each wave a 4x4 tile with four
+// lanes per ray (march_quad); lane 0 of each quad writes the pixel.
+template <bool I64, int U>
+__global__ __launch_bounds__(kBlock) void k_render_volume_quad(vrx_scene sc, vrx_camera cam,
+    vrx_renderer r, vrx_frame f)
+{
+  __shared__ float4 s_lut[256];
+  const vrx_volume &V = sc.d_volumes[0];
+  s_lut[threadIdx.x] = gld((const float4 *)V.d_lut, threadIdx.x);
+  __syncthreads();
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wave = tid >> 6, lane = tid & 63, q = lane >> 2, j = lane & 3;
+  const uint32_t px = blockIdx.x * kVolTile + (wave & 1) * 4 + (q & 3);
+  const uint32_t py = blockIdx.y * kVolTile + (wave >> 1) * 4 + (q >> 2);
+  const bool inside = px < f.width && py < f.height;
+  uint32_t vsamples = 0;
+  if (inside) {
+    const size_t idx = (size_t)px + (size_t)py * f.width;
+    const float iw = 1.0f / (float)f.width, ih = 1.0f / (float)f.height;
+    v3 o, d;
+    cam_ray(cam, (float)px * iw, (float)py * ih, o, d);
+    v3 color = mk(0.0f, 0.0f, 0.0f);
+    float opacity = 0.0f;
+    uint32_t vo = ~0u, vi = ~0u;
+    float vdepth = FLT_MAX;
+    float t0, t1;
+    if (vol_slab(V, o, d, 0.0f, FLT_MAX, t0, t1)) {
+      vo = V.id;
+      vi = V.inst_id;
+      vdepth = t0;
+      march_quad<I64, U>(vol_ctx(V, r.inv_volume_sampling_rate), s_lut, o, d, t0, t1, j, lane,
+          color, opacity, vsamples);
+    }
+    // the quad's partial sums (every lane of a quad reaches this point)
+    color = mk(quad_sum(color.x), quad_sum(color.y), quad_sum(color.z));
+    opacity = quad_sum(opacity);
+    if (j == 0) {
+      const float depth = fminf(1e30f, vdepth);
+      color = mulf(color, opacity);
+      float w = 1.0f - opacity;
+      color = mk(color.x + r.background[0] * w, color.y + r.background[1] * w,
+          color.z + r.background[2] * w);
+      opacity = opacity + r.background[3] * (1.0f - opacity);
+      v3 outc = mk(0.0f + color.x * 1.0f, 0.0f + color.y * 1.0f, 0.0f + color.z * 1.0f);
+      float outo = 0.0f + opacity * 1.0f;
+      float4 accum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      float dcur = FLT_MAX;
+      if (f.frame_id != 0) {
+        accum = ((const float4 *)f.d_accum)[idx];
+        if (f.d_depth)
+          dcur = f.d_depth[idx];
+      }
+      float sc_ = 1.0f + fmaxf(0.0f, cmax3(outc));
+      accum.x = accum.x + outc.x / sc_;
+      accum.y = accum.y + outc.y / sc_;
+      accum.z = accum.z + outc.z / sc_;
+      accum.w = accum.w + outo;
+      bool ids_changed = f.frame_id == 0;
+      uint32_t pid = 0, oid = 0, iid = 0;
+      if (depth < dcur) {
+        dcur = depth;
+        pid = 0;
+        oid = vo;
+        iid = vi;
+        ids_changed = true;
+      }
+      write_pixel(f, idx, accum, 1, dcur, ids_changed, pid, oid, iid);
+    }
+  }
+  if (f.d_counters) {
+    wave_count_add(f.d_counters, 0, inside && j == 0 ? 1u : 0u);
+    wave_count_add(f.d_counters, 1, vsamples);
   }
 }
 
@@ -450,32 +654,51 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
   // take the generic kernel
   if (scene->num_entries == 0 && scene->num_volumes == 1 && r->kind == VRX_RENDER_RAYCAST
       && !r->volume_jitter) {
-    dim3 grid((f->width + kTile - 1) / kTile, (f->height + kTile - 1) / kTile);
-    const int u = r->vol_unroll > 0 ? r->vol_unroll : 4;
     const bool big = scene->max_volume_voxels >= (1ull << 31);
-    // the volume's macro-cell grid, when it has one, drives empty-space
-    // skipping in its own kernel instance (the dense march carries no check)
-    const bool skip = r->skip_empty && scene->num_majorant_grids > 0;
+    if (scene->num_majorant_grids > 0) {
+      // a volume with a macro-cell grid: one lane per ray, with empty-space
+      // skipping when the renderer asks for it (the frame is the same)
+      dim3 grid((f->width + kTile - 1) / kTile, (f->height + kTile - 1) / kTile);
+      const int u = r->vol_unroll > 0 ? r->vol_unroll : 4;
+      const bool skip = r->skip_empty != 0;
 #define VRX_VOL_LAUNCH(B, U)                                                                    \
   do {                                                                                          \
     if (skip)                                                                                   \
-      hipLaunchKernelGGL((k_render_volume<B, U, true>), grid, dim3(kBlock), 0, st, *scene, *cam, \
-          *r, *f);                                                                              \
+      hipLaunchKernelGGL((k_render_volume_seq<B, U, true>), grid, dim3(kBlock), 0, st, *scene,   \
+          *cam, *r, *f);                                                                        \
     else                                                                                        \
-      hipLaunchKernelGGL((k_render_volume<B, U, false>), grid, dim3(kBlock), 0, st, *scene,      \
+      hipLaunchKernelGGL((k_render_volume_seq<B, U, false>), grid, dim3(kBlock), 0, st, *scene,  \
           *cam, *r, *f);                                                                        \
   } while (0)
-    if (big)
-      VRX_VOL_LAUNCH(true, 2);
-    else if (u <= 1)
-      VRX_VOL_LAUNCH(false, 1);
-    else if (u == 2)
-      VRX_VOL_LAUNCH(false, 2);
-    else if (u <= 4)
-      VRX_VOL_LAUNCH(false, 4);
-    else
-      VRX_VOL_LAUNCH(false, 8);
+      if (big)
+        VRX_VOL_LAUNCH(true, 2);
+      else if (u <= 1)
+        VRX_VOL_LAUNCH(false, 1);
+      else if (u == 2)
+        VRX_VOL_LAUNCH(false, 2);
+      else if (u <= 4)
+        VRX_VOL_LAUNCH(false, 4);
+      else
+        VRX_VOL_LAUNCH(false, 8);
 #undef VRX_VOL_LAUNCH
+    } else {
+      // dense field: four lanes per ray, volumeUnroll blocks of four samples
+      // per iteration (default 2)
+      dim3 grid((f->width + kVolTile - 1) / kVolTile, (f->height + kVolTile - 1) / kVolTile);
+      const int u = r->vol_unroll > 0 ? r->vol_unroll : 2;
+#define VRX_VOL_LAUNCH(B, U)                                                                    \
+  hipLaunchKernelGGL((k_render_volume_quad<B, U>), grid, dim3(kBlock), 0, st, *scene, *cam, *r, \
+      *f)
+      if (big)
+        VRX_VOL_LAUNCH(true, 2);
+      else if (u <= 1)
+        VRX_VOL_LAUNCH(false, 1);
+      else if (u <= 2)
+        VRX_VOL_LAUNCH(false, 2);
+      else
+        VRX_VOL_LAUNCH(false, 4);
+#undef VRX_VOL_LAUNCH
+    }
     VRX_LAUNCH_CHECK();
     return VRX_OK;
   }

@@ -69,6 +69,14 @@ struct VolCtx {
   float step, expo, lo, hi, inv_range;
   const float *mc; // macro-cell majorants (16^3 voxels per cell) or null
   int mcd[3];
+  // the 32-bit gather form (b32): the field has 2 to 2^24 - 1 voxels along x,
+  // under 2^30 voxels and under 2^24 x-rows, so a voxel's byte offset fits
+  // 32 bits, its row index z * dims1 + y is exact in fp32 and row * dims0 is
+  // a 24-bit multiply
+  bool b32;
+  uint32_t oy, oz;        // byte offsets of the +1 neighbour in y / z (0 on a one-voxel axis)
+  float fimax[3], fdims1; // imax and dims[1] as floats
+  float tf_scale, tf_bias; // LUT texel coordinate s * tf_scale + tf_bias (0, 0 on an empty range)
 };
 
 __device__ __forceinline__ float vol_step(const vrx_volume &v)
@@ -100,6 +108,19 @@ __device__ __forceinline__ VolCtx vol_ctx(const vrx_volume &v, float inv_rate, b
   c.lo = v.value_range[0];
   c.hi = v.value_range[1];
   c.inv_range = 1.0f / (c.hi - c.lo);
+  const uint64_t rows = (uint64_t)v.dims[1] * v.dims[2];
+  const uint64_t nvox = rows * v.dims[0];
+  c.b32 = v.dims[0] >= 2 && v.dims[0] < (1u << 24) && nvox < (1ull << 30) && rows < (1ull << 24);
+  c.oy = v.dims[1] > 1 ? v.dims[0] * 4u : 0u;
+  c.oz = v.dims[2] > 1 ? v.dims[0] * v.dims[1] * 4u : 0u;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    c.fimax[k] = (float)c.imax[k];
+  c.fdims1 = (float)v.dims[1];
+  const float sc = 256.0f * c.inv_range;
+  const bool fin = sc - sc == 0.0f; // finite
+  c.tf_scale = fin ? sc : 0.0f;
+  c.tf_bias = fin ? -c.lo * sc - 0.5f : 0.0f;
   return c;
 }
 
@@ -107,7 +128,12 @@ __device__ __forceinline__ VolCtx vol_ctx(const vrx_volume &v, float inv_rate, b
 // clamp: the structured-regular field's clamp-to-edge linear texture
 // (StructuredRegularField.cpp:98-159, sampleSpatialField.h:54-78) with full
 // fp32 weights.  I64 selects 64-bit gather offsets for fields of >= 2^31 voxels.
-template <bool I64>
+// XP: 1 when the field has at least two voxels along x (one 8-byte gather per
+// x-pair), 0 when it has one, -1 to decide per call.  The march decides once
+// per segment: a uniform branch between the two forms inside the unrolled
+// sample loop makes the compiler wait for each sample's gathers before it
+// issues the next sample's.
+template <bool I64, int XP = -1>
 __device__ __forceinline__ float trilinear(const VolCtx &c, float gx, float gy, float gz)
 {
   float g[3] = {gx, gy, gz};
@@ -122,7 +148,7 @@ __device__ __forceinline__ float trilinear(const VolCtx &c, float gx, float gy, 
     f[k] = x - (float)i;
   }
   float c000, c100, c010, c110, c001, c101, c011, c111;
-  if (c.dims[0] >= 2) {
+  if (XP == 1 || (XP < 0 && c.dims[0] >= 2)) {
     // x-neighbours are adjacent floats (i0 <= dims-2): one 8-byte gather each
     const VRX_GLOBAL float *D = (const VRX_GLOBAL float *)c.data;
     float2 p00, p10, p01, p11;
@@ -191,6 +217,64 @@ __device__ __forceinline__ float4 classify(const float4 *lut, const VolCtx &c, f
       a.w + (b.w - a.w) * f);
 }
 
+// Forms of trilinear() and classify() for marches compared with the FP64
+// oracle within a tolerance (same clamps, same voxel and texel selection):
+// lerps as one fused multiply-add each; the LUT coordinate
+// (clamp(s) - lo) / range * 256 - 0.5 as s * tf_scale + tf_bias before the
+// texel clamp (a sample outside the range lands outside [0, 255] and clamps to
+// the same end texel).
+__device__ __forceinline__ float fma_lerp(float a, float b, float t)
+{
+  return __builtin_fmaf(b - a, t, a);
+}
+
+// trilinear() on a b32 field: voxel indices kept as floats (x >= 0, so floor
+// is the truncation), the row index as one exact FMA, and one 32-bit byte
+// offset from the field's base for the four 8-byte x-pair gathers (the y / z
+// neighbours are constant offsets: 0 on a one-voxel axis, where the edge clamp
+// repeats the voxel), which saves the per-gather 64-bit address arithmetic.
+__device__ __forceinline__ float trilinear_b32(const VolCtx &c, float gx, float gy, float gz)
+{
+  const float g[3] = {gx, gy, gz};
+  float i0[3], f[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float x = fminf(fmaxf(g[k], 0.0f), c.gmax[k]);
+    i0[k] = fminf(floorf(x), c.fimax[k]);
+    f[k] = x - i0[k];
+  }
+  const uint32_t row = (uint32_t)__builtin_fmaf(i0[2], c.fdims1, i0[1]);
+  const uint32_t bo = ((row & 0xffffffu) * ((uint32_t)c.dims[0] & 0xffffffu) + (uint32_t)i0[0]) * 4u;
+  const VRX_GLOBAL char *B = (const VRX_GLOBAL char *)c.data;
+  const float2 p00 = gld_pair((const VRX_GLOBAL float *)(B + bo));
+  const float2 p10 = gld_pair((const VRX_GLOBAL float *)(B + (bo + c.oy)));
+  const float2 p01 = gld_pair((const VRX_GLOBAL float *)(B + (bo + c.oz)));
+  const float2 p11 = gld_pair((const VRX_GLOBAL float *)(B + (bo + (c.oy + c.oz))));
+  const float c00 = fma_lerp(p00.x, p00.y, f[0]);
+  const float c10 = fma_lerp(p10.x, p10.y, f[0]);
+  const float c01 = fma_lerp(p01.x, p01.y, f[0]);
+  const float c11 = fma_lerp(p11.x, p11.y, f[0]);
+  return fma_lerp(fma_lerp(c00, c10, f[1]), fma_lerp(c01, c11, f[1]), f[2]);
+}
+
+template <bool LUT_GLOBAL>
+__device__ __forceinline__ float4 classify_fma(const float4 *lut, const VolCtx &c, float s)
+{
+  const float x = fminf(fmaxf(__builtin_fmaf(s, c.tf_scale, c.tf_bias), 0.0f), 255.0f);
+  const int i = min((int)x, 254);
+  const float f = x - (float)i;
+  float4 a, b;
+  if constexpr (LUT_GLOBAL) {
+    a = gld(lut, i);
+    b = gld(lut, i + 1);
+  } else {
+    a = lut[i];
+    b = lut[i + 1];
+  }
+  return make_float4(fma_lerp(a.x, b.x, f), fma_lerp(a.y, b.y, f), fma_lerp(a.z, b.z, f),
+      fma_lerp(a.w, b.w, f));
+}
+
 // _rayMarchVolume (volumeIntegration.h:64-103), jitter off.  Sample k sits at
 // t0 + k*step (not an accumulated sum, which keeps positions within an ulp of
 // the exact ones); the per-step transmittance pow(1-a, step/unitDistance) is
@@ -199,12 +283,10 @@ __device__ __forceinline__ float4 classify(const float4 *lut, const VolCtx &c, f
 // positions do not depend on the accumulation), which keeps 8*U loads in
 // flight per lane; samples past the exit or the 0.99 cut-off are discarded,
 // their clamped gathers always stay inside the field.
-template <bool I64, int U, bool LUT_GLOBAL, bool SKIP = true>
-__device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, v3 d, float t0,
+template <bool I64, int U, bool LUT_GLOBAL, bool SKIP, int XP>
+__device__ __forceinline__ void march_xp(const VolCtx &c, const float4 *lut, v3 o, v3 d, float t0,
     float t1, v3 &color, float &opacity, uint32_t &ns)
 {
-  if (!(c.step > 0.0f))
-    return;
   // voxel-space ray: g(t) = go + gd * t
   const float gox = (o.x - c.origin[0]) * c.inv_sp[0], gdx = d.x * c.inv_sp[0];
   const float goy = (o.y - c.origin[1]) * c.inv_sp[1], gdy = d.y * c.inv_sp[1];
@@ -249,7 +331,7 @@ __device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, 
     for (int u = 0; u < U; u++) {
       const float t = t0 + (float)(k + u) * c.step;
       ok[u] = t <= t1;
-      sv[u] = trilinear<I64>(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
+      sv[u] = trilinear<I64, XP>(c, gox + gdx * t, goy + gdy * t, goz + gdz * t);
     }
     bool done = false;
 #pragma unroll
@@ -276,6 +358,18 @@ __device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, 
     if (done)
       break;
   }
+}
+
+template <bool I64, int U, bool LUT_GLOBAL, bool SKIP = true>
+__device__ __forceinline__ void march(const VolCtx &c, const float4 *lut, v3 o, v3 d, float t0,
+    float t1, v3 &color, float &opacity, uint32_t &ns)
+{
+  if (!(c.step > 0.0f))
+    return;
+  if (c.dims[0] >= 2)
+    march_xp<I64, U, LUT_GLOBAL, SKIP, 1>(c, lut, o, d, t0, t1, color, opacity, ns);
+  else
+    march_xp<I64, U, LUT_GLOBAL, SKIP, 0>(c, lut, o, d, t0, t1, color, opacity, ns);
 }
 
 __device__ __forceinline__ bool vol_is_large(const vrx_volume &v)
@@ -398,15 +492,19 @@ __device__ __forceinline__ void write_pixel(const vrx_frame &f, size_t idx, floa
   }
 }
 
-__device__ __forceinline__ void wave_count_add(uint64_t *ctr, uint32_t v)
+// Adds a wave's sum of v to frame counter c (0 rays, 1 volume samples): one
+// atomic per wave, into the counter slot of the workgroup (slots spread the
+// atomics of a launch over VRX_COUNTER_SLOTS cache lines).
+__device__ __forceinline__ void wave_count_add(uint64_t *ctrs, uint32_t c, uint32_t v)
 {
-  // one atomic per wave
   uint32_t s = v;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
     s += __shfl_xor(s, off, 64);
-  if ((threadIdx.x & 63) == 0 && s)
-    atomicAdd((unsigned long long *)ctr, (unsigned long long)s);
+  if ((threadIdx.x & 63) == 0 && s) {
+    const uint32_t slot = (blockIdx.x + blockIdx.y * gridDim.x) % VRX_COUNTER_SLOTS;
+    atomicAdd((unsigned long long *)(ctrs + slot * VRX_COUNTER_STRIDE + c), (unsigned long long)s);
+  }
 }
 
 // --------------------------------------------------------------------------
