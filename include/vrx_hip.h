@@ -67,6 +67,8 @@ int vrx_malloc(void **d_ptr, size_t bytes);
 int vrx_free(void *d_ptr);
 int vrx_host_malloc(void **h_ptr, size_t bytes);
 int vrx_host_free(void *h_ptr);
+/* The device-side address of pinned host memory from vrx_host_malloc. */
+int vrx_host_device_pointer(void *h_ptr, void **d_ptr);
 /* Where an application pointer lives (array/Array.cpp:38-65
  * getPointerAddressSpace): host (pageable or pinned), device memory of the
  * current device, managed memory, or device memory of another device. */
@@ -286,6 +288,16 @@ typedef struct vrx_frame {
 #define VRX_COUNTER_SLOTS 64
 #define VRX_COUNTER_STRIDE 16
 #define VRX_COUNTER_BYTES (VRX_COUNTER_SLOTS * VRX_COUNTER_STRIDE * 8)
+
+/* After a frame: writes out[0] = rays traced and out[1] = volume samples
+   (the sums over f->d_counters' slots) and out[2] = the first 32-bit word of
+   f->d_scratch (the traversal stack-overflow flag, 0 when clean), then zeroes
+   the counter slots for the next frame, in one small kernel on stream s.
+   out must be device-accessible: device memory or pinned host memory from
+   vrx_host_malloc (read it after the stream has passed this call).  No
+   reference counterpart (Frame.cu reads its counters with copies); it
+   replaces a clear and two read-back copies per frame. */
+int vrx_frame_collect_counters(const vrx_frame *f, uint64_t *out, vrx_stream s);
 
 size_t vrx_render_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r,
     uint32_t width, uint32_t height);

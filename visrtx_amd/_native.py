@@ -152,6 +152,7 @@ def hip() -> C.CDLL:
             "vrx_free": (i32, [vp]),
             "vrx_host_malloc": (i32, [vp, sz]),
             "vrx_host_free": (i32, [vp]),
+            "vrx_host_device_pointer": (i32, [vp, vp]),
             "vrx_memcpy": (i32, [vp, vp, sz, i32, vp]),
             "vrx_memcpy_sync": (i32, [vp, vp, sz, i32]),
             "vrx_pointer_kind": (i32, [vp, C.POINTER(C.c_int)]),
@@ -176,6 +177,7 @@ def hip() -> C.CDLL:
             "vrx_render_scratch_bytes": (sz, [vp, vp, u32, u32]),
             "vrx_frame_clear": (i32, [vp, vp]),
             "vrx_render": (i32, [vp, vp, vp, vp, vp]),
+            "vrx_frame_collect_counters": (i32, [vp, vp, vp]),
             "vrx_camera_perspective": (i32, [vp, vp, vp, C.c_float, C.c_float, vp, vp]),
             "vrx_volume_grid_cells": (sz, [vp]),
             "vrx_volume_majorant_grid": (i32, [vp, vp, vp]),

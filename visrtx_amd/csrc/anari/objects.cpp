@@ -1746,25 +1746,20 @@ bool Renderer::getProperty(
 // ---------------------------------------------------------------------------
 // Frame (frame/Frame.cu:75-310, 312-443, 559-660)
 // ---------------------------------------------------------------------------
-uint64_t Frame::counterTotal(int c) const
-{
-  uint64_t t = 0;
-  if (hReadback)
-    for (int s = 0; s < VRX_COUNTER_SLOTS; s++)
-      t += hReadback[s * VRX_COUNTER_STRIDE + c];
-  return t;
-}
-
 Frame::Frame(Device *d) : Object(d, ANARI_FRAME, "")
 {
   vrx_event_create(&evStart);
   vrx_event_create(&evEnd);
   vrx_event_create(&evDone);
   void *p = nullptr;
-  // the counter slots, then the traversal stack-overflow word
-  if (vrx_host_malloc(&p, kReadbackBytes) == 0) {
+  // rays traced, volume samples and the traversal stack-overflow word,
+  // written by vrx_frame_collect_counters after each frame
+  if (vrx_host_malloc(&p, 32) == 0) {
     hReadback = (uint64_t *)p;
-    memset(hReadback, 0, kReadbackBytes);
+    memset(hReadback, 0, 32);
+    void *d = nullptr;
+    if (vrx_host_device_pointer(p, &d) == 0)
+      dReadback = (uint64_t *)d;
   }
 }
 
@@ -1825,7 +1820,9 @@ void Frame::allocate()
   bool f32 = colorType == ANARI_FLOAT32_VEC4;
   bool ok = dev->check(vrx_malloc(&dAccum, np * 16), this, "frame buffer")
       && dev->check(vrx_malloc(&dColor, np * (f32 ? 16 : 4)), this, "frame buffer")
-      && dev->check(vrx_malloc(&dCounters, VRX_COUNTER_BYTES), this, "frame counters");
+      && dev->check(vrx_malloc(&dCounters, VRX_COUNTER_BYTES), this, "frame counters")
+      && dev->check(vrx_fill_u32(dCounters, 0u, VRX_COUNTER_BYTES / 4, dev->stream), this,
+          "frame counters");
   if (ok && depthType == ANARI_FLOAT32)
     ok = dev->check(vrx_malloc(&dDepth, np * 4), this, "depth buffer");
   if (ok && primType == ANARI_UINT32)
@@ -1844,7 +1841,7 @@ void Frame::wait()
   vrx_event_sync(evDone);
   // the traversal stacks are sized from the BVH heights, so this flag stays
   // clear; it is checked so that a violated bound can never pass silently
-  if (hReadback && (uint32_t)hReadback[kStackWord] != 0 && !stackReported) {
+  if (hReadback && (uint32_t)hReadback[2] != 0 && !stackReported) {
     report(ANARI_SEVERITY_ERROR, ANARI_STATUS_UNKNOWN_ERROR,
         "traversal stack overflow in the last frame: results are incomplete");
     stackReported = true;
@@ -1974,14 +1971,12 @@ void Frame::render()
   f.d_scratch = dScratch;
   f.scratch_bytes = scratchBytes;
   f.d_counters = (uint64_t *)dCounters;
-  vrx_fill_u32(dCounters, 0u, VRX_COUNTER_BYTES / 4, dev->stream);
   vrx_event_record(evStart, dev->stream);
   dev->check(vrx_render(&world->scene, &camera->cam, &rr, &f, dev->stream), this, "render");
   vrx_event_record(evEnd, dev->stream);
-  if (hReadback) {
-    vrx_memcpy(hReadback, dCounters, VRX_COUNTER_BYTES, VRX_D2H, dev->stream);
-    vrx_memcpy(hReadback + kStackWord, dScratch, 4, VRX_D2H, dev->stream);
-  }
+  // totals to the pinned read-back words, slots cleared for the next frame
+  if (dReadback)
+    dev->check(vrx_frame_collect_counters(&f, dReadback, dev->stream), this, "frame counters");
   if (callback) {
     // Frame.cu:295-304 calls the callback from cudaLaunchHostFunc; here the
     // host function only hands the job to the device's callback thread
@@ -2087,13 +2082,13 @@ bool Frame::getProperty(const std::string &name, ANARIDataType t, void *mem, uin
   }
   if (name == "raysTraced" && t == ANARI_UINT64 && sz >= 8) {
     wait();
-    const uint64_t v = counterTotal(0);
+    const uint64_t v = hReadback ? hReadback[0] : 0;
     memcpy(mem, &v, 8);
     return true;
   }
   if (name == "volumeSamples" && t == ANARI_UINT64 && sz >= 8) {
     wait();
-    const uint64_t v = counterTotal(1);
+    const uint64_t v = hReadback ? hReadback[1] : 0;
     memcpy(mem, &v, 8);
     return true;
   }

@@ -380,12 +380,8 @@ struct Frame : Object {
   float duration{0.0f};
   // pinned read-back slots: [0] rays traced, [1] volume samples, [2] the
   // stack overflow flag (pinned, so the copies stay asynchronous)
-  // counter slots (VRX_COUNTER_SLOTS x VRX_COUNTER_STRIDE words), then the
-  // traversal stack-overflow word
-  static constexpr size_t kStackWord = VRX_COUNTER_SLOTS * VRX_COUNTER_STRIDE;
-  static constexpr size_t kReadbackBytes = (kStackWord + 8) * 8;
-  uint64_t *hReadback{nullptr};
-  uint64_t counterTotal(int c) const;
+  uint64_t *hReadback{nullptr}; // pinned: [0] rays, [1] volume samples, [2] stack flag
+  uint64_t *dReadback{nullptr}; // its device-side address
   bool allocated{false};
 
   void allocate();

@@ -723,6 +723,41 @@ size_t vrx_volume_grid_cells(const vrx_volume *v)
   return (size_t)((v->dims[0] + 15) / 16) * ((v->dims[1] + 15) / 16) * ((v->dims[2] + 15) / 16);
 }
 
+// One wave: lane i sums slot i of the two frame counters, a wave reduction
+// gives the totals, and the slots are cleared for the next frame.
+__global__ __launch_bounds__(64) void k_frame_counters(uint64_t *ctrs, const uint32_t *flag,
+    uint64_t *out)
+{
+  const uint32_t lane = threadIdx.x;
+  uint64_t r = 0, v = 0;
+  if (ctrs && lane < VRX_COUNTER_SLOTS) {
+    r = ctrs[lane * VRX_COUNTER_STRIDE + 0];
+    v = ctrs[lane * VRX_COUNTER_STRIDE + 1];
+    ctrs[lane * VRX_COUNTER_STRIDE + 0] = 0;
+    ctrs[lane * VRX_COUNTER_STRIDE + 1] = 0;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    r += __shfl_xor(r, off, 64);
+    v += __shfl_xor(v, off, 64);
+  }
+  if (lane == 0) {
+    out[0] = r;
+    out[1] = v;
+    out[2] = flag ? (uint64_t)*flag : 0ull;
+  }
+}
+
+int vrx_frame_collect_counters(const vrx_frame *f, uint64_t *out, vrx_stream s)
+{
+  VRX_ARG(f && out, "frame and output");
+  static_assert(VRX_COUNTER_SLOTS <= 64, "one lane per counter slot");
+  hipLaunchKernelGGL(k_frame_counters, dim3(1), dim3(64), 0, (hipStream_t)s, f->d_counters,
+      (const uint32_t *)f->d_scratch, out);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
+}
+
 int vrx_volume_majorant_grid(const vrx_volume *v, float *d_majorant, vrx_stream s)
 {
   VRX_ARG(v && d_majorant, "null argument");

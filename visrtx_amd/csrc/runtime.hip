@@ -107,6 +107,13 @@ int vrx_free(void *d_ptr)
   return VRX_OK;
 }
 
+int vrx_host_device_pointer(void *h_ptr, void **d_ptr)
+{
+  VRX_ARG(h_ptr && d_ptr, "pointers");
+  VRX_HIP(hipHostGetDevicePointer(d_ptr, h_ptr, 0));
+  return VRX_OK;
+}
+
 int vrx_host_malloc(void **h_ptr, size_t bytes)
 {
   VRX_ARG(h_ptr, "h_ptr");
