@@ -188,6 +188,12 @@ typedef struct vrx_volume {
    * march skips the samples inside cells whose majorant is 0 (they add
    * nothing), when the renderer's skip_empty is set */
   const float *d_majorant;
+  /* NULL, or the field as y-pairs from vrx_volume_build_ypairs: per voxel the
+   * float2 (v(x,y,z), v(x,min(y+1,ny-1),z)), x fastest, 8 bytes per voxel.
+   * The dense volume-only march then reads a cell's four corners of one z
+   * plane with one 16-byte gather (fields of 2 to 2^24-1 voxels along x,
+   * under 2^29 voxels and under 2^24 rows); same values, same frame */
+  const float *d_ypairs;
 } vrx_volume;
 
 /* One light in world space, as the directLight renderer samples it
@@ -319,6 +325,12 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
  * vrx_volume_grid_cells(v) floats, x fastest; v->d_data and v->d_lut are read. */
 size_t vrx_volume_grid_cells(const vrx_volume *v);
 int vrx_volume_majorant_grid(const vrx_volume *v, float *d_majorant, vrx_stream s);
+/* Writes v's field as y-pairs (see vrx_volume.d_ypairs) to d_ypairs, which
+ * holds dims[0]*dims[1]*dims[2] float2.  No reference counterpart (the
+ * reference samples a CUDA 3D texture, whose layout the driver chooses); the
+ * layout halves the dense march's gather instructions.  Fields of 2^29
+ * voxels or more are refused (their offsets would not fit 32 bits). */
+int vrx_volume_build_ypairs(const vrx_volume *v, float *d_ypairs, vrx_stream s);
 /* Woodcock (delta) tracking of n rays through v over the majorant grid
  * v->d_majorant (volumeIntegration.h:167-228 _sampleDistance, dda.h:43-122):
  * per ray (origin, direction, [tmin, tmax]) the collision distance d_t (tmax

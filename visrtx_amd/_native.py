@@ -90,7 +90,7 @@ class vrx_volume(C.Structure):
     _fields_ = [("d_data", C.c_void_p), ("dims", C.c_uint32 * 3), ("origin", f3),
                 ("spacing", f3), ("d_lut", C.c_void_p), ("value_range", C.c_float * 2),
                 ("unit_distance", C.c_float), ("id", C.c_uint32), ("inst_id", C.c_uint32),
-                ("pad", C.c_uint32), ("d_majorant", C.c_void_p)]
+                ("pad", C.c_uint32), ("d_majorant", C.c_void_p), ("d_ypairs", C.c_void_p)]
 
 
 class vrx_light(C.Structure):
@@ -181,6 +181,7 @@ def hip() -> C.CDLL:
             "vrx_camera_perspective": (i32, [vp, vp, vp, C.c_float, C.c_float, vp, vp]),
             "vrx_volume_grid_cells": (sz, [vp]),
             "vrx_volume_majorant_grid": (i32, [vp, vp, vp]),
+            "vrx_volume_build_ypairs": (i32, [vp, vp, vp]),
             "vrx_volume_sample_distance": (i32, [vp, u32, vp, vp, vp, C.c_uint64, vp, vp, vp, vp]),
             "vrx_tlas_query_sizes": (i32, [u32, vp]),
             "vrx_tlas_build": (i32, [vp, vp, vp, sz, vp]),

@@ -899,6 +899,7 @@ Volume::~Volume()
 {
   vrx_free(dLut);
   vrx_free(dGrid);
+  vrx_free(dPairs);
 }
 
 static float posf(float v, float lo, float hi)
@@ -1482,6 +1483,23 @@ int World::rebuild()
         if (8 * empty >= cells && empty > 0)
           vv.d_majorant = (const float *)v->dGrid;
       }
+    }
+    // a dense field alone in its world is marched by the four-lane kernel,
+    // which reads the y-pair copy of the field (8 B per voxel)
+    vv.d_ypairs = nullptr;
+    const uint64_t nvox = (uint64_t)vv.dims[0] * vv.dims[1] * vv.dims[2];
+    if (vv.d_data && !vv.d_majorant && vols.size() == 1 && he.empty() && vv.dims[0] >= 2
+        && nvox < (1ull << 29)) {
+      if (v->pairBytes != nvox * 8) {
+        vrx_free(v->dPairs);
+        v->dPairs = nullptr;
+        v->pairBytes = 0;
+        if (dev->check(vrx_malloc(&v->dPairs, nvox * 8), this, "volume y-pairs"))
+          v->pairBytes = nvox * 8;
+      }
+      if (v->dPairs && dev->check(vrx_volume_build_ypairs(&vv, (float *)v->dPairs, s), this,
+                          "volume y-pairs"))
+        vv.d_ypairs = (const float *)v->dPairs;
     }
     nGrids += vv.d_majorant ? 1u : 0u;
     hv.push_back(vv);

@@ -222,6 +222,8 @@ struct Volume : Object {
   // device volume when at least 1/8 of its cells are empty)
   void *dGrid{nullptr};
   size_t gridCells{0};
+  void *dPairs{nullptr}; // the y-pair copy of the field for the dense march
+  size_t pairBytes{0};
   float emptyCellFraction{0.0f};
 };
 

@@ -181,7 +181,9 @@ __device__ __forceinline__ void march_quad_xp(const VolCtx &c, const float4 *lut
       ok[u] = t <= t1;
       const float gx = __builtin_fmaf(gdx, t, gox), gy = __builtin_fmaf(gdy, t, goy),
                   gz = __builtin_fmaf(gdz, t, goz);
-      if constexpr (XP == 2)
+      if constexpr (XP == 3)
+        sv[u] = trilinear_ypairs(c, gx, gy, gz);
+      else if constexpr (XP == 2)
         sv[u] = trilinear_b32(c, gx, gy, gz);
       else
         sv[u] = trilinear<I64, XP>(c, gx, gy, gz);
@@ -243,9 +245,12 @@ __device__ __forceinline__ void march_quad(const VolCtx &c, const float4 *lut, v
 {
   if (!(c.step > 0.0f))
     return;
-  // XP: 2 the b32 gathers, 1 x-pair gathers with 64-bit offsets, 0 a field
-  // one voxel wide in x (one branch per segment, outside the sample loop)
-  if (!I64 && c.b32)
+  // XP: 3 the y-pair gathers, 2 the b32 gathers, 1 x-pair gathers with
+  // 64-bit offsets, 0 a field one voxel wide in x (one branch per segment,
+  // outside the sample loop)
+  if (!I64 && c.yp)
+    march_quad_xp<I64, U, 3>(c, lut, o, d, t0, t1, j, lane, color, opacity, ns);
+  else if (!I64 && c.b32)
     march_quad_xp<I64, U, 2>(c, lut, o, d, t0, t1, j, lane, color, opacity, ns);
   else if (c.dims[0] >= 2)
     march_quad_xp<I64, U, 1>(c, lut, o, d, t0, t1, j, lane, color, opacity, ns);
@@ -754,6 +759,32 @@ int vrx_frame_collect_counters(const vrx_frame *f, uint64_t *out, vrx_stream s)
   static_assert(VRX_COUNTER_SLOTS <= 64, "one lane per counter slot");
   hipLaunchKernelGGL(k_frame_counters, dim3(1), dim3(64), 0, (hipStream_t)s, f->d_counters,
       (const uint32_t *)f->d_scratch, out);
+  VRX_LAUNCH_CHECK();
+  return VRX_OK;
+}
+
+// y-pairs: out[2i] = v[i], out[2i+1] = v[i + nx] (the voxel one row up, or
+// itself on the last row), as one 8-byte store per voxel.
+__global__ __launch_bounds__(256) void k_ypairs(const float *__restrict__ v,
+    float2 *__restrict__ out, uint32_t nx, uint32_t ny, uint64_t n)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t y = (uint32_t)((i / nx) % ny);
+    const float a = gld(v, i);
+    const float b = y + 1 < ny ? gld(v, i + nx) : a;
+    out[i] = make_float2(a, b);
+  }
+}
+
+int vrx_volume_build_ypairs(const vrx_volume *v, float *d_ypairs, vrx_stream s)
+{
+  VRX_ARG(v && v->d_data && d_ypairs, "volume with data and output");
+  const uint64_t n = (uint64_t)v->dims[0] * v->dims[1] * v->dims[2];
+  VRX_ARG(n > 0 && n < (1ull << 29), "y-pairs need 0 < voxels < 2^29");
+  const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_ypairs, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s, v->d_data,
+      (float2 *)d_ypairs, v->dims[0], v->dims[1], n);
   VRX_LAUNCH_CHECK();
   return VRX_OK;
 }

@@ -55,6 +55,13 @@ __device__ __forceinline__ void gst(float4 *p, size_t i, const float4 &v)
   vrx_f4v w = {v.x, v.y, v.z, v.w};
   ((VRX_GLOBAL vrx_f4v *)p)[i] = w;
 }
+// four consecutive floats at 8-byte alignment (one 16-byte gather)
+typedef float vrx_f4v_a8 __attribute__((ext_vector_type(4))) __attribute__((aligned(8)));
+__device__ __forceinline__ float4 gld_quad8(const VRX_GLOBAL float *p)
+{
+  vrx_f4v_a8 v = *(const VRX_GLOBAL vrx_f4v_a8 *)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
 // two adjacent floats at 4-byte alignment (one 8-byte gather)
 __device__ __forceinline__ float2 gld_pair(const VRX_GLOBAL float *p)
 {

@@ -74,6 +74,8 @@ struct VolCtx {
   // 32 bits, its row index z * dims1 + y is exact in fp32 and row * dims0 is
   // a 24-bit multiply
   bool b32;
+  const float *yp; // the y-pair field (b32 fields under 2^29 voxels) or null
+  uint32_t ozp;    // y-pair byte offset of the +1 neighbour in z (0 on a one-voxel axis)
   uint32_t oy, oz;        // byte offsets of the +1 neighbour in y / z (0 on a one-voxel axis)
   float fimax[3], fdims1; // imax and dims[1] as floats
   float tf_scale, tf_bias; // LUT texel coordinate s * tf_scale + tf_bias (0, 0 on an empty range)
@@ -111,6 +113,8 @@ __device__ __forceinline__ VolCtx vol_ctx(const vrx_volume &v, float inv_rate, b
   const uint64_t rows = (uint64_t)v.dims[1] * v.dims[2];
   const uint64_t nvox = rows * v.dims[0];
   c.b32 = v.dims[0] >= 2 && v.dims[0] < (1u << 24) && nvox < (1ull << 30) && rows < (1ull << 24);
+  c.yp = c.b32 && nvox < (1ull << 29) ? v.d_ypairs : nullptr;
+  c.ozp = v.dims[2] > 1 ? v.dims[0] * v.dims[1] * 8u : 0u;
   c.oy = v.dims[1] > 1 ? v.dims[0] * 4u : 0u;
   c.oz = v.dims[2] > 1 ? v.dims[0] * v.dims[1] * 4u : 0u;
 #pragma unroll
@@ -254,6 +258,31 @@ __device__ __forceinline__ float trilinear_b32(const VolCtx &c, float gx, float 
   const float c10 = fma_lerp(p10.x, p10.y, f[0]);
   const float c01 = fma_lerp(p01.x, p01.y, f[0]);
   const float c11 = fma_lerp(p11.x, p11.y, f[0]);
+  return fma_lerp(fma_lerp(c00, c10, f[1]), fma_lerp(c01, c11, f[1]), f[2]);
+}
+
+// trilinear_b32() on the y-pair field: the 16 bytes at voxel (x, y, z)
+// hold v(x,y), v(x,y+1), v(x+1,y), v(x+1,y+1) of plane z, so two gathers (z
+// and z+1) read the cell; the same values and lerps as trilinear_b32().
+__device__ __forceinline__ float trilinear_ypairs(const VolCtx &c, float gx, float gy, float gz)
+{
+  const float g[3] = {gx, gy, gz};
+  float i0[3], f[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float x = fminf(fmaxf(g[k], 0.0f), c.gmax[k]);
+    i0[k] = fminf(floorf(x), c.fimax[k]);
+    f[k] = x - i0[k];
+  }
+  const uint32_t row = (uint32_t)__builtin_fmaf(i0[2], c.fdims1, i0[1]);
+  const uint32_t bo = ((row & 0xffffffu) * ((uint32_t)c.dims[0] & 0xffffffu) + (uint32_t)i0[0]) * 8u;
+  const VRX_GLOBAL char *B = (const VRX_GLOBAL char *)c.yp;
+  const float4 a = gld_quad8((const VRX_GLOBAL float *)(B + bo));
+  const float4 b = gld_quad8((const VRX_GLOBAL float *)(B + (bo + c.ozp)));
+  const float c00 = fma_lerp(a.x, a.z, f[0]);
+  const float c10 = fma_lerp(a.y, a.w, f[0]);
+  const float c01 = fma_lerp(b.x, b.z, f[0]);
+  const float c11 = fma_lerp(b.y, b.w, f[0]);
   return fma_lerp(fma_lerp(c00, c10, f[1]), fma_lerp(c01, c11, f[1]), f[2]);
 }
 
