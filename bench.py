@@ -676,6 +676,11 @@ def main():
                                                         HBM_PEAK_GBS, 4)
         vt = load_traffic("render_volume512_1920x1080")
         extra["volume_traffic_bytes_per_launch"] = vt["hbm_bytes_per_launch"] if vt else None
+        # the profiled HBM bytes over this run's kernel time: what the
+        # launch really moves (neighbouring samples share cache lines, so the
+        # algorithmic fraction above can exceed 1)
+        extra["volume_hbm_frac_profiled_traffic"] = (
+            round(vt["hbm_bytes_per_launch"] / (kms / 1000) / 1e9 / HBM_PEAK_GBS, 4) if vt else None)
         log(f"volume: kernel {kms:.3f} ms, {ns} samples (oracle {ref_ns}) -> "
             f"{extra['volume_gsamples_s']} Gsamples/s")
         vol.release()
