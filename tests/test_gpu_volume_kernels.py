@@ -1,0 +1,160 @@
+"""The volume-only raycast kernels on the GPU against the FP64 oracle march
+(volumeIntegration.h:64-103 _rayMarchVolume, Raycast_ptx.cu:60-179): the
+four-lanes-per-ray kernel for dense fields on the y-pair gathers (fields two
+voxels thick along x, y or z, where every cell sits at the edge clamp, and
+one-voxel boxes no ray enters), each unroll depth, image sizes that are not
+multiples of its 8x8 tile, accumulated frames and the frame counters; and
+the y-pair layout itself against numpy.
+
+Bars as for the other volume tests: ids exact, entry depth within 1e-5
+relative, colour within 1e-4 absolute on pixels the FP64 oracle does not
+flag as tie cases, and at least 97 % of pixels unflagged.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from visrtx_amd import _native as N
+from visrtx_amd import anari as A
+from visrtx_amd import scenes as S
+from tests.test_gpu_render import render_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(g, o, min_ok=0.97):
+    assert not g["errors"], g["errors"]
+    ok = o["ambiguous"] == 0
+    assert ok.mean() >= min_ok, f"ambiguous fraction {1 - ok.mean():.4f}"
+    np.testing.assert_allclose(g["color"][ok], o["color"][ok], rtol=0, atol=1e-4)
+    for ch in ("primitiveId", "objectId", "instanceId"):
+        np.testing.assert_array_equal(g[ch], o[ch])
+    np.testing.assert_allclose(g["depth"], o["depth"], rtol=1e-5)
+
+
+def _slab_scene(nx, ny, nz, seed, w, h):
+    """A noise field of nx x ny x nz voxels spanning [0,1] along each axis
+    with two or more voxels (the field's box is origin + (dims - 1) *
+    spacing, StructuredRegularField.cpp:98-159, so an axis of one voxel is a
+    box of zero thickness at 0.4 that no ray enters)."""
+    data = S.gen_noise_volume(nx, ny, nz, seed=seed)
+    dims = (nx, ny, nz)
+    sp = tuple(1.0 / (d - 1) if d > 1 else 0.2 for d in dims)
+    org = tuple(0.0 if d > 1 else 0.4 for d in dims)
+    vol = S.VolumeDesc(data, origin=org, spacing=sp, color=S.COLORMAP5,
+                       opacity=np.array([0.0, 0.3], np.float32), value_range=(0.0, 1.0),
+                       unit_distance=min(s for s in sp), id=7)
+    sc = S.SceneDesc(volumes=[vol], world_id=0)
+    pos = (1.7, 1.3, -1.1)
+    cam = S.CameraDesc(pos, S.look_at(pos, (0.5, 0.5, 0.5)), (0, 1, 0), math.radians(40.0),
+                       w / h)
+    return sc, cam
+
+
+@pytest.mark.parametrize("dims", [(40, 36, 32), (2, 36, 32), (40, 2, 32), (40, 36, 2),
+                                  (2, 2, 2), (2, 37, 3), (1, 36, 32), (40, 36, 1)])
+def test_dense_kernel_gather_forms_vs_oracle(require_gpu, dims):
+    """Thin fields put every sample's cell at the edge clamp (imax = 0 on a
+    two-voxel axis); a one-voxel axis gives a box no ray enters."""
+    sc, cam = _slab_scene(*dims, seed=21, w=96, h=72)
+    print(f"[volkern] dims {dims}", flush=True)
+    g = render_gpu(sc, cam, 96, 72, volumeSamplingRate=1.0)
+    o = orc.OracleScene(sc).render(cam, 96, 72, volumeSamplingRate=1.0)
+    hit = g["depth"] < 1e29
+    if min(dims) > 1:
+        _check(g, o)
+        assert hit.mean() > 0.05
+        assert (g["color"][hit][:, 3] > 0).mean() > 0.5
+    else:
+        # the oracle flags the rays through the plane (entry == exit), and
+        # both sides leave every pixel at the background
+        _check(g, o, min_ok=0.0)
+        assert not hit.any()
+        np.testing.assert_array_equal(g["color"], o["color"])
+    assert g["rays"] == 96 * 72
+
+
+def test_unroll_depths_give_one_frame(require_gpu):
+    """volumeUnroll 1, 2 and 4 group the same blocks of four samples: the
+    frames are bit-identical, and match the oracle."""
+    data = S.gen_noise_volume(56, seed=22)
+    sc, cam = S.volume_workload(n=56, width=120, height=90, data=data)
+    frames = []
+    for u in (1, 2, 4):
+        print(f"[volkern] volumeUnroll {u}", flush=True)
+        frames.append(render_gpu(sc, cam, 120, 90, volumeSamplingRate=1.0, volumeUnroll=u))
+    for f in frames[1:]:
+        for ch in ("color", "depth", "objectId"):
+            np.testing.assert_array_equal(f[ch], frames[0][ch])
+    o = orc.OracleScene(sc).render(cam, 120, 90, volumeSamplingRate=1.0)
+    _check(frames[0], o)
+
+
+@pytest.mark.parametrize("size", [(37, 23), (9, 130), (1, 1)])
+def test_ragged_image_sizes(require_gpu, size):
+    w, h = size
+    data = S.gen_noise_volume(32, seed=23)
+    sc, cam = S.volume_workload(n=32, width=w, height=h, data=data)
+    print(f"[volkern] {w}x{h}", flush=True)
+    g = render_gpu(sc, cam, w, h, volumeSamplingRate=0.5)
+    o = orc.OracleScene(sc).render(cam, w, h, volumeSamplingRate=0.5)
+    _check(g, o, min_ok=0.9 if w * h < 4 else 0.97)
+    assert g["rays"] == w * h
+
+
+def test_accumulated_frames_equal_one_frame(require_gpu):
+    """Raycast frames accumulate (gpu_util.h:323-389): four frames of the same
+    view give the single frame's colour (within the tonemap round trip) and
+    its depth and ids; the sample counter counts each frame's samples."""
+    data = S.gen_noise_volume(40, seed=24)
+    sc, cam = S.volume_workload(n=40, width=64, height=48, data=data)
+    dev = A.Device()
+    s = S.build_anari(dev, sc)
+    c = S.make_camera(dev, s, cam)
+    r = S.make_renderer(dev, s, "raycast", volumeSamplingRate=1.0)
+    f = S.make_frame(dev, s, s.world, c, r, 64, 48, color=A.FLOAT32_VEC4)
+    print("[volkern] accumulation, 4 frames", flush=True)
+    dev.render(f)
+    dev.ready(f)
+    one = dev.map_frame(f, "channel.color")[0].copy()
+    n1 = dev.get_property(f, "volumeSamples", A.UINT64)
+    for _ in range(3):
+        dev.render(f)
+        dev.ready(f)
+        assert dev.get_property(f, "volumeSamples", A.UINT64) == n1
+    four = dev.map_frame(f, "channel.color")[0]
+    depth = dev.map_frame(f, "channel.depth")[0]
+    assert not dev.errors(), dev.errors()
+    np.testing.assert_allclose(four, one, rtol=0, atol=2e-6)
+    o = orc.OracleScene(sc).render(cam, 64, 48, volumeSamplingRate=1.0)
+    np.testing.assert_allclose(depth, o["depth"], rtol=1e-5)
+    assert n1 > 0
+    s.release()
+    dev.close()
+
+
+def test_ypairs_layout(require_gpu):
+    """vrx_volume_build_ypairs: voxel (x, y, z) holds v(x,y,z) and
+    v(x, min(y+1, ny-1), z); bit-exact against numpy, on odd dims and a
+    field one voxel tall; fields of 2^29 voxels are refused."""
+    torch = pytest.importorskip("torch")
+    L = N.hip()
+    for nx, ny, nz in ((7, 5, 3), (33, 1, 4), (1, 9, 2)):
+        v = np.random.default_rng(nx * ny * nz).standard_normal((nz, ny, nx)).astype(np.float32)
+        dv = torch.from_numpy(v).cuda()
+        out = torch.empty(nz * ny * nx * 2, dtype=torch.float32, device="cuda")
+        vol = N.vrx_volume()
+        vol.d_data = dv.data_ptr()
+        vol.dims[:] = (nx, ny, nz)
+        assert L.vrx_volume_build_ypairs(C.byref(vol), C.c_void_p(out.data_ptr()), None) == 0
+        torch.cuda.synchronize()
+        up = np.concatenate([v[:, 1:, :], v[:, -1:, :]], axis=1)
+        want = np.stack([v, up], axis=-1).reshape(-1)
+        np.testing.assert_array_equal(out.cpu().numpy(), want)
+    vol = N.vrx_volume()
+    vol.d_data = 16
+    vol.dims[:] = (1024, 1024, 512)
+    assert L.vrx_volume_build_ypairs(C.byref(vol), C.c_void_p(16), None) != 0
