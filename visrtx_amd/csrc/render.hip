@@ -196,7 +196,7 @@ __device__ __forceinline__ void march_quad_xp(const VolCtx &c, const float4 *lut
         float st = 1.0f;
         float4 co = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (ok[u] && s == s) {
-          co = classify_fma<false>(lut, c, s);
+          co = classify_delta(lut, c, s);
           st = __builtin_amdgcn_exp2f(c.expo * __builtin_amdgcn_logf(1.0f - co.w));
         }
         // inclusive and exclusive products of st over the quad
@@ -271,9 +271,15 @@ template <bool I64, int U>
 __global__ __launch_bounds__(kBlock) void k_render_volume_quad(vrx_scene sc, vrx_camera cam,
     vrx_renderer r, vrx_frame f)
 {
-  __shared__ float4 s_lut[256];
+  // the LUT as (entry, next - entry) pairs (classify_delta)
+  __shared__ float4 s_lut[512];
   const vrx_volume &V = sc.d_volumes[0];
-  s_lut[threadIdx.x] = gld((const float4 *)V.d_lut, threadIdx.x);
+  {
+    const float4 a = gld((const float4 *)V.d_lut, threadIdx.x);
+    const float4 b = gld((const float4 *)V.d_lut, min(threadIdx.x + 1u, 255u));
+    s_lut[2 * threadIdx.x] = a;
+    s_lut[2 * threadIdx.x + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, b.w - a.w);
+  }
   __syncthreads();
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid >> 6, lane = tid & 63, q = lane >> 2, j = lane & 3;

@@ -270,7 +270,9 @@ __device__ __forceinline__ float trilinear_ypairs(const VolCtx &c, float gx, flo
   float i0[3], f[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    const float x = fminf(fmaxf(g[k], 0.0f), c.gmax[k]);
+    // one v_med3 for the clamp (a NaN coordinate, from a NaN ray only, still
+    // gives an index inside the field: fminf below returns fimax)
+    const float x = __builtin_amdgcn_fmed3f(g[k], 0.0f, c.gmax[k]);
     i0[k] = fminf(floorf(x), c.fimax[k]);
     f[k] = x - i0[k];
   }
@@ -302,6 +304,20 @@ __device__ __forceinline__ float4 classify_fma(const float4 *lut, const VolCtx &
   }
   return make_float4(fma_lerp(a.x, b.x, f), fma_lerp(a.y, b.y, f), fma_lerp(a.z, b.z, f),
       fma_lerp(a.w, b.w, f));
+}
+
+// classify_fma() on a LUT staged as (entry, next entry - entry) pairs: each
+// channel's lerp is one FMA on the stored difference (the same operands as
+// fma_lerp, so the same result).  lut2[2i] = lut[i], lut2[2i+1] = lut[i+1] -
+// lut[i] (i < 255).
+__device__ __forceinline__ float4 classify_delta(const float4 *lut2, const VolCtx &c, float s)
+{
+  const float x = fminf(fmaxf(__builtin_fmaf(s, c.tf_scale, c.tf_bias), 0.0f), 255.0f);
+  const int i = min((int)x, 254);
+  const float f = x - (float)i;
+  const float4 a = lut2[2 * i], d = lut2[2 * i + 1];
+  return make_float4(__builtin_fmaf(d.x, f, a.x), __builtin_fmaf(d.y, f, a.y),
+      __builtin_fmaf(d.z, f, a.z), __builtin_fmaf(d.w, f, a.w));
 }
 
 // _rayMarchVolume (volumeIntegration.h:64-103), jitter off.  Sample k sits at
