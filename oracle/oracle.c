@@ -1364,11 +1364,16 @@ static void classify_lut(const double *lutd, const float *lutf, const float vr[2
     double rgba[4], int *i0_out, double *f_out, double *du_ds)
 {
   double lo = vr[0], hi = vr[1];
-  double sc = clampd(s, lo < hi ? lo : hi, lo < hi ? hi : lo);
+  /* position() (gpu_math.h:185-189): glm::clamp(s, lo, hi) = min(max(s, lo),
+     hi), then (s - lo) / (hi - lo).  A reversed range (lo > hi) clamps every
+     sample to hi, so u = 1 (the last texel); an empty range (lo == hi) gives
+     0/0, a NaN texture coordinate in the reference, read here as texel 0
+     (the device does the same; parity unpinned for that case). */
+  double sc = fmin(fmax(s, lo), hi);
   double u = (sc - lo) / (hi - lo);
   double x = u * 256.0 - 0.5;
   int clamped = 0;
-  if (x < 0.0) {
+  if (!(x >= 0.0)) {
     x = 0.0;
     clamped = 1;
   }
@@ -1390,7 +1395,7 @@ static void classify_lut(const double *lutd, const float *lutf, const float vr[2
   if (f_out)
     *f_out = f;
   if (du_ds) {
-    int in_range = (s > (lo < hi ? lo : hi)) && (s < (lo < hi ? hi : lo));
+    int in_range = s > lo && s < hi;
     *du_ds = (clamped || !in_range) ? 0.0 : 256.0 / (hi - lo);
   }
 }

@@ -546,6 +546,15 @@ def _volume_struct(vol_desc):
     return v, (d, lut)
 
 
+def classify(vol_desc, s: float) -> np.ndarray:
+    """FP64 RGBA of the volume's transfer function at field value s
+    (classifySample, volumeIntegration.h:46-62, position() gpu_math.h:185-189)."""
+    v, keep = _volume_struct(vol_desc)
+    out = np.zeros(4, np.float64)
+    lib().orc_classify(C.byref(v), float(s), out.ctypes.data)
+    return out
+
+
 def majorant_grid(vol_desc) -> np.ndarray:
     """(mz, my, mx) float32 macro-cell majorants (16^3 voxels per cell)."""
     v, keep = _volume_struct(vol_desc)

@@ -158,3 +158,34 @@ def test_ypairs_layout(require_gpu):
     vol.d_data = 16
     vol.dims[:] = (1024, 1024, 512)
     assert L.vrx_volume_build_ypairs(C.byref(vol), C.c_void_p(16), None) != 0
+
+
+@pytest.mark.parametrize("kernel", ["quad", "generic"])
+@pytest.mark.parametrize("vr", [(0.3, 0.7), (0.7, 0.3)])
+def test_value_ranges_and_nan_voxels_vs_oracle(require_gpu, kernel, vr):
+    """The transfer function's value range through position() (glm::clamp,
+    gpu_math.h:185-189): a narrow range clamps values outside it to the end
+    texels, a reversed one reads the last texel everywhere.  A slab of NaN
+    voxels: samples whose cell touches one are skipped (volumeIntegration.h
+    :64-103).  The dense four-lane kernel, and the generic one (a sphere
+    behind the camera puts a surface in the world)."""
+    data = S.gen_noise_volume(40, seed=25)
+    data[:, 18:21, :] = np.nan
+    vol = S.VolumeDesc(data, origin=(0.0, 0.0, 0.0), spacing=(1 / 39,) * 3,
+                       color=S.COLORMAP5, opacity=np.array([0.0, 0.3], np.float32),
+                       value_range=vr, unit_distance=1 / 39, id=3)
+    sc = S.SceneDesc(volumes=[vol], world_id=0)
+    pos = (1.8, 1.4, -1.0)
+    cam = S.CameraDesc(pos, S.look_at(pos, (0.5, 0.5, 0.5)), (0, 1, 0), math.radians(40.0),
+                       4 / 3)
+    if kernel == "generic":
+        behind = np.array(pos, np.float32) + 2.0 * (np.array(pos, np.float32) - 0.5)
+        sc.geoms = [S.GeomDesc("sphere", behind[None].astype(np.float32),
+                               radius=np.array([0.05], np.float32), surface_id=5)]
+        sc.world_geoms = [0]
+    print(f"[volkern] range {vr}, {kernel} kernel", flush=True)
+    g = render_gpu(sc, cam, 96, 72, volumeSamplingRate=1.0)
+    o = orc.OracleScene(sc).render(cam, 96, 72, volumeSamplingRate=1.0)
+    _check(g, o)
+    hit = g["depth"] < 1e29
+    assert hit.mean() > 0.2

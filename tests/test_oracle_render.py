@@ -324,6 +324,35 @@ def test_tf_discretization_matches_interpolation():
         np.testing.assert_allclose(lut[:, k], np.interp(p, [0, 0.5, 1], colors[:, k]), atol=1e-6)
 
 
+def test_value_range_clamp_follows_position():
+    """position() (gpu_math.h:185-189) is glm::clamp(s, lo, hi) = min(max(s,
+    lo), hi), then (s - lo) / (hi - lo), read at texel u * 256 - 0.5: values
+    outside a range read the end texels, every value of a reversed range
+    (lo > hi) reads the last texel, and an empty range reads texel 0."""
+    from visrtx_amd import scenes as S
+    colors = np.array([[0, 0, 1], [0, 1, 0], [1, 0, 0]], np.float32)
+    op = np.array([0.1, 0.9], np.float32)
+    data = np.zeros((2, 2, 2), np.float32)
+    for vr, cases in (((0.2, 0.8), {-5.0: 0, 0.1: 0, 0.2: 0, 0.8: 255, 3.0: 255}),
+                      ((0.8, 0.2), {-5.0: 255, 0.1: 255, 0.5: 255, 0.9: 255}),
+                      ((0.5, 0.5), {-5.0: 0, 0.5: 0, 7.0: 0})):
+        vol = S.VolumeDesc(data, color=colors, opacity=op, value_range=vr)
+        lut = orc.tf_discretize(colors, op, vr)
+        for s, texel in cases.items():
+            np.testing.assert_allclose(orc.classify(vol, s), lut[texel], rtol=0, atol=1e-12,
+                                       err_msg=f"range {vr}, s = {s}")
+    vol = S.VolumeDesc(data, color=colors, opacity=op, value_range=(0.2, 0.8))
+    lut = orc.tf_discretize(colors, op, (0.2, 0.8))
+    # inside: texel coordinate (s - lo) / (hi - lo) * 256 - 0.5, linear (the
+    # range is stored as fp32)
+    lo, hi = float(np.float32(0.2)), float(np.float32(0.8))
+    x = (0.5 - lo) / (hi - lo) * 256 - 0.5
+    i = int(np.floor(x))
+    a, b = lut[i].astype(np.float64), lut[i + 1].astype(np.float64)
+    want = a + (b - a) * (x - i)
+    np.testing.assert_allclose(orc.classify(vol, 0.5), want, rtol=0, atol=1e-9)
+
+
 @pytest.mark.parametrize("seed", [3, 4])
 def test_any_hit_equals_closest_hit_within_range(seed):
     """Opaque geometry: the AO any-hit query is 'a closest hit exists in [tmin, tmax]'."""

@@ -78,7 +78,7 @@ struct VolCtx {
   uint32_t ozp;    // y-pair byte offset of the +1 neighbour in z (0 on a one-voxel axis)
   uint32_t oy, oz;        // byte offsets of the +1 neighbour in y / z (0 on a one-voxel axis)
   float fimax[3], fdims1; // imax and dims[1] as floats
-  float tf_scale, tf_bias; // LUT texel coordinate s * tf_scale + tf_bias (0, 0 on an empty range)
+  float tf_scale, tf_bias; // LUT texel coordinate s * tf_scale + tf_bias (clamped to [0, 255])
 };
 
 __device__ __forceinline__ float vol_step(const vrx_volume &v)
@@ -121,10 +121,14 @@ __device__ __forceinline__ VolCtx vol_ctx(const vrx_volume &v, float inv_rate, b
   for (int k = 0; k < 3; k++)
     c.fimax[k] = (float)c.imax[k];
   c.fdims1 = (float)v.dims[1];
+  // classify()'s clamp min(max(s, lo), hi) maps every sample of a reversed
+  // range (lo > hi) to hi, the last texel (x = 255); an empty range (lo ==
+  // hi) reads texel 0 there, and so does this form
   const float sc = 256.0f * c.inv_range;
   const bool fin = sc - sc == 0.0f; // finite
-  c.tf_scale = fin ? sc : 0.0f;
-  c.tf_bias = fin ? -c.lo * sc - 0.5f : 0.0f;
+  const bool rev = c.lo > c.hi;
+  c.tf_scale = fin && !rev ? sc : 0.0f;
+  c.tf_bias = rev ? 255.0f : fin ? -c.lo * sc - 0.5f : 0.0f;
   return c;
 }
 
