@@ -34,8 +34,9 @@ __device__ __forceinline__ vrx_camera load_camera(const vrx_camera *cams, uint32
 }
 
 template <int PT, bool TWO, bool WIDE, int KIND>
-__global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, vrx_renderer r,
-    vrx_batch b, uint32_t sd, uint32_t cap, uint32_t *spill, uint32_t tiles_x, uint32_t tiles_y)
+__device__ __forceinline__ void render_batch(const vrx_scene &sc, const vrx_renderer &r,
+    const vrx_batch &b, uint32_t sd, uint32_t cap, uint32_t *spill, uint32_t tiles_x,
+    uint32_t tiles_y)
 {
   extern __shared__ uint32_t s_stack[];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -87,6 +88,26 @@ __global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, v
   }
 }
 
+template <int PT, bool TWO, bool WIDE, int KIND>
+__global__ __launch_bounds__(kBatchMaxBlock) void k_render_batch(vrx_scene sc, vrx_renderer r,
+    vrx_batch b, uint32_t sd, uint32_t cap, uint32_t *spill, uint32_t tiles_x, uint32_t tiles_y)
+{
+  render_batch<PT, TWO, WIDE, KIND>(sc, r, b, sd, cap, spill, tiles_x, tiles_y);
+}
+
+// The raycast program on one primitive type, held to 96 VGPRs (5 waves per
+// SIMD): the traversal is latency-bound on L2 misses (every view reads most
+// of the scene), and on the 64-view scene the fifth wave outweighs the few
+// registers it spills (14.8 -> 13.6 ms).  AO and directLight would spill
+// 160-320 B per lane and keep the unconstrained kernel.
+template <int PT, bool TWO, bool WIDE>
+__global__ __launch_bounds__(kBatchMaxBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void
+k_render_batch_raycast(vrx_scene sc, vrx_renderer r, vrx_batch b, uint32_t sd, uint32_t cap,
+    uint32_t *spill, uint32_t tiles_x, uint32_t tiles_y)
+{
+  render_batch<PT, TWO, WIDE, VRX_RENDER_RAYCAST>(sc, r, b, sd, cap, spill, tiles_x, tiles_y);
+}
+
 static int g_num_cus = 0;
 
 static int num_cus()
@@ -124,8 +145,12 @@ static uint32_t cap_for(const vrx_scene &sc, uint32_t sd, bool wide)
 template <int PT, bool WIDE, int KIND>
 static const void *kernel_ptr_k(bool two)
 {
-  return two ? (const void *)&k_render_batch<PT, true, WIDE, KIND>
-             : (const void *)&k_render_batch<PT, false, WIDE, KIND>;
+  if constexpr (KIND == VRX_RENDER_RAYCAST && PT >= 0)
+    return two ? (const void *)&k_render_batch_raycast<PT, true, WIDE>
+               : (const void *)&k_render_batch_raycast<PT, false, WIDE>;
+  else
+    return two ? (const void *)&k_render_batch<PT, true, WIDE, KIND>
+               : (const void *)&k_render_batch<PT, false, WIDE, KIND>;
 }
 
 template <int PT, bool WIDE>

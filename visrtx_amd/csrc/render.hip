@@ -26,8 +26,8 @@
 namespace vrx {
 
 template <int PT, bool TWO, int KIND, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam, vrx_renderer r,
-    vrx_frame f, uint32_t sd, uint32_t ovf_cap, uint32_t *scratch)
+__device__ __forceinline__ void render_tile(const vrx_scene &sc, const vrx_camera &cam,
+    const vrx_renderer &r, const vrx_frame &f, uint32_t sd, uint32_t ovf_cap, uint32_t *scratch)
 {
   extern __shared__ uint32_t s_stack[];
   const int tid = threadIdx.x;
@@ -57,6 +57,28 @@ __global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam,
     wave_count_add(f.d_counters, 0, rays);
     wave_count_add(f.d_counters, 1, vsamples);
   }
+}
+
+template <int PT, bool TWO, int KIND, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_render(vrx_scene sc, vrx_camera cam, vrx_renderer r,
+    vrx_frame f, uint32_t sd, uint32_t ovf_cap, uint32_t *scratch)
+{
+  render_tile<PT, TWO, KIND, WIDE>(sc, cam, r, f, sd, ovf_cap, scratch);
+}
+
+// The same program held to 80 VGPRs (6 waves per SIMD), for the
+// primitive-specialised AO and directLight programs and two-level raycast:
+// their secondary rays and instance walks are latency-bound, and the extra
+// waves outweigh the registers spilled to scratch (10M spheres: AO 16 spp
+// 13.1 -> 11.9 ms, directLight 4 spp 6.4 -> 5.7 ms; 1M instances +7 %).
+// Single-level raycast (66-88 VGPRs) and the generic kernel (130-150 VGPRs)
+// keep k_render.
+template <int PT, bool TWO, int KIND, bool WIDE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_render_occ6(
+    vrx_scene sc, vrx_camera cam, vrx_renderer r, vrx_frame f, uint32_t sd, uint32_t ovf_cap,
+    uint32_t *scratch)
+{
+  render_tile<PT, TWO, KIND, WIDE>(sc, cam, r, f, sd, ovf_cap, scratch);
 }
 
 // Raycast renderer on a world holding exactly one volume and no surfaces:
@@ -572,21 +594,30 @@ static uint32_t ovf_cap_of(const vrx_scene *sc, uint32_t sd, bool wide)
   return m > sd ? m - sd : 0u;
 }
 
-template <int PT, bool TWO, bool WIDE>
-static void launch_kw(const vrx_scene &sc, const vrx_camera &cam, const vrx_renderer &r,
+template <int PT, bool TWO, int KIND, bool WIDE>
+static void launch_one(const vrx_scene &sc, const vrx_camera &cam, const vrx_renderer &r,
     const vrx_frame &f, uint32_t sd, uint32_t cap, hipStream_t s)
 {
   dim3 grid((f.width + kTile - 1) / kTile, (f.height + kTile - 1) / kTile);
   size_t lds = (size_t)sd * kBlock * 4;
-  if (r.kind == VRX_RENDER_DIRECT)
-    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_DIRECT, WIDE>), grid, dim3(kBlock), lds, s, sc,
-        cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
-  else if (r.kind == VRX_RENDER_AO)
-    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_AO, WIDE>), grid, dim3(kBlock), lds, s, sc,
-        cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
+  if constexpr (PT >= 0 && (KIND != VRX_RENDER_RAYCAST || TWO))
+    hipLaunchKernelGGL((k_render_occ6<PT, TWO, KIND, WIDE>), grid, dim3(kBlock), lds, s, sc, cam, r,
+        f, sd, cap, (uint32_t *)f.d_scratch);
   else
-    hipLaunchKernelGGL((k_render<PT, TWO, VRX_RENDER_RAYCAST, WIDE>), grid, dim3(kBlock), lds, s,
-        sc, cam, r, f, sd, cap, (uint32_t *)f.d_scratch);
+    hipLaunchKernelGGL((k_render<PT, TWO, KIND, WIDE>), grid, dim3(kBlock), lds, s, sc, cam, r, f,
+        sd, cap, (uint32_t *)f.d_scratch);
+}
+
+template <int PT, bool TWO, bool WIDE>
+static void launch_kw(const vrx_scene &sc, const vrx_camera &cam, const vrx_renderer &r,
+    const vrx_frame &f, uint32_t sd, uint32_t cap, hipStream_t s)
+{
+  if (r.kind == VRX_RENDER_DIRECT)
+    launch_one<PT, TWO, VRX_RENDER_DIRECT, WIDE>(sc, cam, r, f, sd, cap, s);
+  else if (r.kind == VRX_RENDER_AO)
+    launch_one<PT, TWO, VRX_RENDER_AO, WIDE>(sc, cam, r, f, sd, cap, s);
+  else
+    launch_one<PT, TWO, VRX_RENDER_RAYCAST, WIDE>(sc, cam, r, f, sd, cap, s);
 }
 
 template <int PT, bool TWO>
