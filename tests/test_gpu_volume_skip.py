@@ -23,15 +23,18 @@ from visrtx_amd import scenes as S
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("vr", [(0.0, 1.0), (0.7, 0.3)])
 @pytest.mark.parametrize("shape", [(40, 33, 50), (16, 16, 16), (1, 20, 17)])
-def test_majorant_grid_matches_restatement(require_gpu, shape):
+def test_majorant_grid_matches_restatement(require_gpu, shape, vr):
+    """A reversed value range classifies every value to the last texel
+    (position()'s glm::clamp, gpu_math.h:185-189), so every cell with a
+    finite voxel gets that texel's alpha."""
     rng = np.random.default_rng(sum(shape))
     data = rng.random(shape, dtype=np.float32)
     data[data < 0.6] = 0.0
     data[:18, :18, :18] = 0.0                    # cell (0, 0, 0) and its halo: empty
     data[tuple(rng.integers(0, s, 5) for s in shape)] = np.nan
     lut = orc.tf_discretize(S.COLORMAP5, np.array([0.0, 0.0, 0.3, 1.0], np.float32), (0.0, 1.0))
-    vr = (0.0, 1.0)
     want = _grid_reference(data, lut, vr)
     L = N.hip()
     db, lb = DeviceBuffer.from_array(data), DeviceBuffer.from_array(lut)
@@ -49,7 +52,10 @@ def test_majorant_grid_matches_restatement(require_gpu, shape):
     N.check(L.vrx_device_sync(), "sync")
     got = out.to_array(np.float32, cells).reshape(want.shape)
     np.testing.assert_array_equal(got, want)
-    assert want.reshape(-1)[0] == 0 and (want.size == 1 or (want > 0).any())
+    if vr[0] < vr[1]:
+        assert want.reshape(-1)[0] == 0 and (want.size == 1 or (want > 0).any())
+    else:
+        assert (want == lut.reshape(-1, 4)[254:, 3].max()).all()
     for b in (db, lb, out):
         b.free()
 
@@ -91,6 +97,30 @@ def test_skipping_is_exact_volume_only(require_gpu):
     n_off = _vol_samples(sc, cam, 200, 120, "raycast", False, volumeSamplingRate=0.5)
     print(f"volume samples: {n_on} with skipping, {n_off} without")
     assert n_on < 0.5 * n_off
+
+
+@pytest.mark.parametrize("vr,opacity", [((0.25, 0.6), None), ((0.6, 0.25), (0.8, 0.4, 0.0, 0.0))])
+def test_skipping_is_exact_for_value_ranges(require_gpu, vr, opacity):
+    """A narrow value range (values below it clamp to the first texel, alpha
+    0, so cells stay skippable) and a reversed one.  A reversed range
+    discretizes the transfer function to its first colour and opacity with a
+    NaN entry 0 (TransferFunction1D.cpp:101-150), and every sample reads the
+    last texel, so no cell is empty and the world marches the field with
+    the dense four-lane kernel instead.  Frames equal with and without
+    skipping and match the FP64 oracle."""
+    sc, cam = S.sparse_volume_workload(96, width=160, height=100)
+    sc.volumes[0].value_range = vr
+    if opacity is not None:
+        sc.volumes[0].opacity = np.array(opacity, np.float32)
+    on, _ = _render_both(sc, cam, 160, 100, "raycast", volumeSamplingRate=0.5)
+    o = orc.OracleScene(sc).render(cam, 160, 100, volumeSamplingRate=0.5)
+    ok = o["ambiguous"] == 0
+    assert ok.mean() > 0.99
+    np.testing.assert_allclose(on["color"][ok], o["color"][ok], rtol=0, atol=1e-4)
+    np.testing.assert_array_equal(on["objectId"], o["objectId"])
+    np.testing.assert_allclose(on["depth"], o["depth"], rtol=1e-5)
+    hit = on["depth"] < 1e29
+    assert hit.mean() > 0.05 and (on["color"][hit][:, :3].sum(axis=1) > 0).mean() > 0.05
 
 
 def test_skipping_is_exact_with_surfaces_and_lights(require_gpu):
