@@ -14,6 +14,7 @@ from visrtx_amd import anari as A
 ANARI_STRING = 101
 ANARI_BOOL = 103
 ANARI_PARAMETER_LIST = 152
+ANARI_STRING_LIST_T = 150
 
 
 class ANARIParameter(C.Structure):
@@ -131,3 +132,77 @@ def test_defaults_and_clamps(lib):
                                          b"default", A.INT32)
     assert not lib.anariGetObjectInfo(None, A.GEOMETRY, b"isosurface", b"parameter",
                                       ANARI_PARAMETER_LIST)
+
+
+# Parameters of the reference's renderers that this device does not
+# implement: the OptiX denoiser and checkerboarded sampling are outside the
+# ported path.
+_NOT_BUILT = {"denoise", "checkerboarding"}
+_TYPES = {"ANARI_INT32": (A.INT32, C.c_int32), "ANARI_FLOAT32": (A.FLOAT32, C.c_float),
+          "ANARI_BOOL": (ANARI_BOOL, C.c_int32)}
+
+
+def test_renderer_tables_match_the_reference_device_declaration(lib):
+    """Every parameter the reference declares for the default, ao and
+    raycast renderers (devices/rtx/visrtx_device.json, committed as
+    tests/golden/renderer_params.json) is listed here with the same type and
+    default, except the two features not built."""
+    import json
+    import os
+    spec = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "renderer_params.json")))
+    assert {"default", "ao", "raycast"} <= set(spec)
+    checked = 0
+    for sub in ("default", "ao", "raycast"):
+        mine = params(lib, A.RENDERER, sub.encode())
+        for p in spec[sub]:
+            if p["name"] in _NOT_BUILT:
+                continue
+            ptype, ctype = _TYPES[p["type"]]
+            assert mine.get(p["name"]) == ptype, (sub, p["name"])
+            got = default(lib, A.RENDERER, sub.encode(), p["name"], ptype, ctype)
+            if ptype == ANARI_BOOL:
+                assert bool(got) == p["default"], (sub, p["name"])
+            else:
+                assert got == pytest.approx(p["default"], rel=1e-6), (sub, p["name"], got)
+            checked += 1
+    assert checked == 16
+
+
+def _strings(p):
+    arr = C.cast(p, C.POINTER(C.c_char_p))
+    out, i = [], 0
+    while arr[i]:
+        out.append(arr[i].decode())
+        i += 1
+    return out
+
+
+def test_extension_and_feature_queries(lib):
+    """The device's "extension" and "subtypes.renderer" string-list
+    properties (VisRTXDevice.cpp:541-546) and each object's "feature" list
+    (read by visrtxGetObjectExtensions, VisRTXFeatureUtility.cpp:64-77): the
+    triangle back-face culling extension the reference reports is there, the
+    CUDA-only ones are not, and every object feature is a device extension."""
+    dev = A.Device()
+    p = C.c_void_p()
+    L = dev.L
+    assert L.anariGetProperty(dev.h, dev.h, b"extension", ANARI_STRING_LIST_T, C.byref(p),
+                              C.sizeof(p), A.WAIT) == 1
+    ext = _strings(p.value)
+    assert "ANARI_VISRTX_TRIANGLE_BACK_FACE_CULLING" in ext
+    assert "ANARI_KHR_GEOMETRY_SPHERE" in ext and "ANARI_KHR_ARRAY1D_REGION" in ext
+    assert "ANARI_VISRTX_ARRAY_CUDA" not in ext and "ANARI_NV_ARRAY_CUDA" not in ext
+    assert len(ext) == len(set(ext))
+    assert L.anariGetProperty(dev.h, dev.h, b"subtypes.renderer", ANARI_STRING_LIST_T, C.byref(p),
+                              C.sizeof(p), A.WAIT) == 1
+    assert set(_strings(p.value)) == set(subtypes(lib, A.RENDERER))
+    for typ in (A.GEOMETRY, A.RENDERER, A.LIGHT):
+        for sub in subtypes(lib, typ):
+            f = lib.anariGetObjectInfo(None, typ, sub.encode(), b"feature", ANARI_STRING_LIST_T)
+            assert f, (typ, sub)
+            feats = _strings(f)
+            assert feats and set(feats) <= set(ext), (typ, sub, feats)
+    tri = _strings(lib.anariGetObjectInfo(None, A.GEOMETRY, b"triangle", b"feature",
+                                          ANARI_STRING_LIST_T))
+    assert tri == ["ANARI_KHR_GEOMETRY_TRIANGLE", "ANARI_VISRTX_TRIANGLE_BACK_FACE_CULLING"]
+    dev.close()

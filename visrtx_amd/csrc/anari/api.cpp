@@ -100,6 +100,7 @@ namespace vrx_anari {
 const void *objectInfo(ANARIDataType type, const char *subtype, const char *info, ANARIDataType infoType);
 const void *parameterInfo(ANARIDataType type, const char *subtype, const char *param,
     ANARIDataType paramType, const char *info, ANARIDataType infoType);
+const char *const *deviceExtensions();
 } // namespace vrx_anari
 
 extern "C" {
@@ -472,6 +473,18 @@ int anariGetProperty(ANARIDevice d, ANARIObject o, const char *name, ANARIDataTy
     if (n == "version.patch" && t == ANARI_INT32 && size >= 4) {
       int v = 0;
       memcpy(mem, &v, 4);
+      return 1;
+    }
+    // the extension list and the renderer subtypes as string lists (a
+    // pointer to a null-terminated array of C strings)
+    if (n == "extension" && t == ANARI_STRING_LIST && size >= sizeof(void *)) {
+      const char *const *v = vrx_anari::deviceExtensions();
+      memcpy(mem, &v, sizeof(v));
+      return 1;
+    }
+    if (n == "subtypes.renderer" && t == ANARI_STRING_LIST && size >= sizeof(void *)) {
+      const char *const *v = g_rendererSubtypes;
+      memcpy(mem, &v, sizeof(v));
       return 1;
     }
     if (n == "hipDevice" && t == ANARI_INT32 && size >= 4) {

@@ -51,7 +51,7 @@ const float kFovy = 1.04719755f; // 60 degrees
 const float kRate = 0.125f, kRateMin = 1e-3f, kRateMax = 10.0f;
 const float kAoDist = 1e20f;
 const float kHalf = 0.5f;
-const int32_t kInt1 = 1, kInt0 = 0, kInt2 = 2, kInt4 = 4, kInt256 = 256, kLimit = 128;
+const int32_t kInt1 = 1, kInt0 = 0, kInt2 = 2, kInt4 = 4, kInt5 = 5, kInt256 = 256, kLimit = 128;
 const int32_t kFalse = 0, kTrue1 = 1;
 const uint32_t kNoId = 0xFFFFFFFFu;
 const ANARIDataType kSrgb = ANARI_UFIXED8_RGBA_SRGB;
@@ -203,8 +203,12 @@ std::vector<ParamDef> rendererParams(int kind)
     p.push_back({"ambientSamples", ANARI_INT32, false, &kInt1, &kInt0, &kInt256, "occlusion rays per hit"});
     p.push_back({"ambientOcclusionDistance", ANARI_FLOAT32, false, &kAoDist, nullptr, nullptr, "occlusion ray length"});
   }
-  if (kind == kDirect)
+  if (kind == kDirect) {
     p.push_back({"lightFalloff", ANARI_FLOAT32, false, &kOne, &kZero, &kOne, "accepted; not used in shading"});
+    // matte surfaces spawn no bounce ray, so the bounce loop ends after the
+    // first hit whatever the depth
+    p.push_back({"maxRayDepth", ANARI_INT32, false, &kInt5, &kInt0, nullptr, "accepted; matte surfaces do not bounce"});
+  }
   return p;
 }
 
@@ -235,11 +239,92 @@ ObjDef *find(ANARIDataType type, const char *subtype)
 
 } // namespace
 
+// The extensions this device implements, named as ANARI's query tables name
+// them ("ANARI_" + the extension's identifier in upper case); the reference
+// lists its set in visrtx_device.json "dependencies", and
+// VisRTXFeatureUtility.cpp:39-61 reads the same strings back.  CUDA arrays
+// and output buffers, HDRI lights, physically based materials, background
+// images, samplers, NanoVDB fields and the orthographic / depth-of-field
+// cameras are not built here.
+const char *const kDeviceExtensions[] = {"ANARI_KHR_ARRAY1D_REGION", "ANARI_KHR_CAMERA_PERSPECTIVE",
+    "ANARI_KHR_DEVICE_SYNCHRONIZATION", "ANARI_KHR_FRAME_ACCUMULATION",
+    "ANARI_KHR_FRAME_CHANNEL_PRIMITIVE_ID", "ANARI_KHR_FRAME_CHANNEL_OBJECT_ID",
+    "ANARI_KHR_FRAME_CHANNEL_INSTANCE_ID", "ANARI_KHR_FRAME_COMPLETION_CALLBACK",
+    "ANARI_KHR_GEOMETRY_CONE", "ANARI_KHR_GEOMETRY_CURVE", "ANARI_KHR_GEOMETRY_CYLINDER",
+    "ANARI_KHR_GEOMETRY_QUAD", "ANARI_KHR_GEOMETRY_SPHERE", "ANARI_KHR_GEOMETRY_TRIANGLE",
+    "ANARI_KHR_INSTANCE_TRANSFORM", "ANARI_KHR_INSTANCE_TRANSFORM_ARRAY",
+    "ANARI_KHR_LIGHT_DIRECTIONAL", "ANARI_KHR_LIGHT_POINT", "ANARI_KHR_LIGHT_SPOT",
+    "ANARI_KHR_MATERIAL_MATTE", "ANARI_KHR_RENDERER_AMBIENT_LIGHT",
+    "ANARI_KHR_RENDERER_BACKGROUND_COLOR", "ANARI_KHR_SPATIAL_FIELD_STRUCTURED_REGULAR",
+    "ANARI_KHR_VOLUME_TRANSFER_FUNCTION1D", "ANARI_VISRTX_TRIANGLE_BACK_FACE_CULLING", nullptr};
+
+const char *const *deviceExtensions()
+{
+  return kDeviceExtensions;
+}
+
+namespace {
+
+// anariGetObjectInfo(..., "feature", ANARI_STRING_LIST): the extensions an
+// object subtype belongs to (what visrtxGetObjectExtensions,
+// VisRTXFeatureUtility.cpp:64-77, reads)
+const char *const *features(ANARIDataType type, const char *sub)
+{
+  static const char *const kTri[] = {"ANARI_KHR_GEOMETRY_TRIANGLE", "ANARI_VISRTX_TRIANGLE_BACK_FACE_CULLING", nullptr};
+  static const char *const kQuad[] = {"ANARI_KHR_GEOMETRY_QUAD", nullptr};
+  static const char *const kSphere[] = {"ANARI_KHR_GEOMETRY_SPHERE", nullptr};
+  static const char *const kCyl[] = {"ANARI_KHR_GEOMETRY_CYLINDER", nullptr};
+  static const char *const kCone[] = {"ANARI_KHR_GEOMETRY_CONE", nullptr};
+  static const char *const kCurve[] = {"ANARI_KHR_GEOMETRY_CURVE", nullptr};
+  static const char *const kRen[] = {"ANARI_KHR_RENDERER_AMBIENT_LIGHT", "ANARI_KHR_RENDERER_BACKGROUND_COLOR",
+      "ANARI_VISRTX_TRIANGLE_BACK_FACE_CULLING", nullptr};
+  static const char *const kDir[] = {"ANARI_KHR_LIGHT_DIRECTIONAL", nullptr};
+  static const char *const kPoint[] = {"ANARI_KHR_LIGHT_POINT", nullptr};
+  static const char *const kSpot[] = {"ANARI_KHR_LIGHT_SPOT", nullptr};
+  static const char *const kCam[] = {"ANARI_KHR_CAMERA_PERSPECTIVE", nullptr};
+  static const char *const kMatte[] = {"ANARI_KHR_MATERIAL_MATTE", nullptr};
+  static const char *const kField[] = {"ANARI_KHR_SPATIAL_FIELD_STRUCTURED_REGULAR", nullptr};
+  static const char *const kVol[] = {"ANARI_KHR_VOLUME_TRANSFER_FUNCTION1D", nullptr};
+  static const char *const kInst[] = {"ANARI_KHR_INSTANCE_TRANSFORM", "ANARI_KHR_INSTANCE_TRANSFORM_ARRAY", nullptr};
+  static const char *const kFrame[] = {"ANARI_KHR_FRAME_ACCUMULATION", "ANARI_KHR_FRAME_CHANNEL_PRIMITIVE_ID",
+      "ANARI_KHR_FRAME_CHANNEL_OBJECT_ID", "ANARI_KHR_FRAME_CHANNEL_INSTANCE_ID",
+      "ANARI_KHR_FRAME_COMPLETION_CALLBACK", nullptr};
+  static const char *const kNone[] = {nullptr};
+  const auto is = [&](const char *n) { return sub && std::strcmp(sub, n) == 0; };
+  switch (type) {
+  case ANARI_GEOMETRY:
+    return is("triangle") ? kTri : is("quad") ? kQuad : is("sphere") ? kSphere
+        : is("cylinder") ? kCyl : is("cone") ? kCone : is("curve") ? kCurve : nullptr;
+  case ANARI_RENDERER:
+    return kRen;
+  case ANARI_LIGHT:
+    return is("directional") ? kDir : is("point") ? kPoint : is("spot") ? kSpot : nullptr;
+  case ANARI_CAMERA:
+    return is("perspective") ? kCam : nullptr;
+  case ANARI_MATERIAL:
+    return is("matte") ? kMatte : nullptr;
+  case ANARI_SPATIAL_FIELD:
+    return is("structuredRegular") ? kField : nullptr;
+  case ANARI_VOLUME:
+    return is("transferFunction1D") ? kVol : nullptr;
+  case ANARI_INSTANCE:
+    return kInst;
+  case ANARI_FRAME:
+    return kFrame;
+  default:
+    return kNone;
+  }
+}
+
+} // namespace
+
 const void *objectInfo(ANARIDataType type, const char *subtype, const char *info, ANARIDataType infoType)
 {
   ObjDef *o = find(type, subtype);
   if (!o || !info)
     return nullptr;
+  if (std::strcmp(info, "feature") == 0 && infoType == ANARI_STRING_LIST)
+    return features(type, subtype);
   if (std::strcmp(info, "description") == 0 && infoType == ANARI_STRING)
     return o->desc;
   if (std::strcmp(info, "parameter") == 0 && infoType == ANARI_PARAMETER_LIST) {
