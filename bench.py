@@ -645,10 +645,19 @@ def main():
         sps, d1 = hf.timed(20, 3)
         extra["heightfield_mrays_s"] = round(1024 * 768 / sps / 1e6, 2)
         extra["heightfield_kernel_ms"] = round(statistics.mean(d1) * 1000, 4)
-        extra["heightfield_tuned_launch"] = dict(zip(("node_width", "stack_lds"), hf.tuned()))
+        tw, tsd = hf.tuned()
+        extra["heightfield_tuned_launch"] = {
+            "node_width": tw, "stack_lds": tsd,
+            "long_ray_steps": dev.get_property(hf.f, "tunedLongRaySteps", A.INT32)}
         ow = hf.pin_other_width()
         _, d12 = hf.timed(20, 3)
         extra[f"heightfield_kernel_ms_width{ow}"] = round(statistics.mean(d12) * 1000, 4)
+        # the tuned launch without the long-ray pass
+        dev.set(hf.r, "nodeWidth", tw, A.INT32)
+        dev.set(hf.r, "longRaySteps", 0, A.INT32)
+        dev.commit(hf.r)
+        _, d13 = hf.timed(20, 3)
+        extra["heightfield_kernel_ms_no_long_ray_pass"] = round(statistics.mean(d13) * 1000, 4)
         # roofline of the tuned kernel: the survey's traversal bytes per ray
         # (ray in, hit out, 64-B nodes and 48-B triangles the oracle visits)
         c1 = counts["heightfield_1024x768"]

@@ -267,6 +267,11 @@ typedef struct vrx_renderer {
                              jitters of volumeIntegration.h:83,120 (PCG32 per pixel and
                              frame, as the AO directions); 0: the first sample on the
                              segment entry (deterministic) */
+  int32_t defer_steps;    /* raycast of a one-level scene of one primitive type: a primary
+                             walk still running after this many steps is finished by a
+                             second launch with 16 lanes per ray over t-segments (same
+                             pixels); 0 = off.  The frame scratch then also holds one u32
+                             per pixel (vrx_render_scratch_bytes) */
 } vrx_renderer;
 
 typedef struct vrx_frame {

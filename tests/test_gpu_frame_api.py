@@ -94,7 +94,8 @@ def test_accumulation_and_sample_limit(require_gpu):
 
 
 def test_launch_autotuning_keeps_frames_identical(require_gpu):
-    """The frame times node width x LDS stack depth candidates on its first
+    """The frame times node width x LDS stack depth (x the long-ray pass on a
+    one-level scene of one primitive type: eight candidates) on its first
     frames and keeps the fastest; every frame's channels stay bit-identical."""
     dev = A.Device()
     s, cam = _scene(dev)
@@ -102,7 +103,7 @@ def test_launch_autotuning_keeps_frames_identical(require_gpu):
     r = S.make_renderer(dev, s, "raycast")
     f = S.make_frame(dev, s, s.world, c, r, 96, 80)
     outs = []
-    for _ in range(7):
+    for _ in range(10):
         dev.commit(c)
         dev.render(f)
         dev.ready(f)
@@ -113,10 +114,11 @@ def test_launch_autotuning_keeps_frames_identical(require_gpu):
             np.testing.assert_array_equal(o[ch], outs[0][ch], err_msg=ch)
     assert dev.get_property(f, "tunedNodeWidth", A.INT32) in (2, 4)
     assert dev.get_property(f, "tunedStackDepth", A.INT32) in (16, 32)
+    assert dev.get_property(f, "tunedLongRaySteps", A.INT32) in (0, 32)
     # an explicit parameter is honoured, not tuned
     dev.set(r, "nodeWidth", 2, A.INT32)
     dev.commit(r)
-    for _ in range(4):
+    for _ in range(6):
         dev.commit(c)                   # raycast frames render once per camera commit
         dev.render(f)
         dev.ready(f)

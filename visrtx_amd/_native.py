@@ -105,7 +105,7 @@ class vrx_renderer(C.Structure):
                 ("cull_backfaces", C.c_int32), ("inv_volume_sampling_rate", C.c_float),
                 ("stack_lds", C.c_int32), ("block_pixels", C.c_int32), ("vol_unroll", C.c_int32),
                 ("node_width", C.c_int32), ("skip_empty", C.c_int32),
-                ("volume_jitter", C.c_int32)]
+                ("volume_jitter", C.c_int32), ("defer_steps", C.c_int32)]
 
 
 class vrx_batch(C.Structure):

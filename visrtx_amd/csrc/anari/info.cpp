@@ -197,7 +197,9 @@ std::vector<ParamDef> rendererParams(int kind)
       {"emptySpaceSkipping", ANARI_BOOL, false, &kTrue1, nullptr, nullptr,
           "skip empty 16^3 macro cells in volume marches (same image)"},
       {"volumeJitter", ANARI_BOOL, false, &kFalse, nullptr, nullptr,
-          "start each volume segment at a random offset (per-pixel PCG32)"}};
+          "start each volume segment at a random offset (per-pixel PCG32)"},
+      {"longRaySteps", ANARI_INT32, false, &kInt0, &kInt0, nullptr,
+          "raycast: primary walks past this many steps finish 16 lanes per ray (0: off; unset: tuned)"}};
   if (ao) {
     p.push_back({"pixelSamples", ANARI_INT32, false, &kInt1, &kInt1, nullptr, "samples per pixel per frame"});
     p.push_back({"ambientSamples", ANARI_INT32, false, &kInt1, &kInt0, &kInt256, "occlusion rays per hit"});

@@ -1737,8 +1737,12 @@ void Renderer::commit()
   // extension: the reference always jitters volume samples
   // (volumeIntegration.h:83,120); here it is opt-in, off by default
   r.volume_jitter = get<int>("volumeJitter", ANARI_BOOL, 0) != 0 ? 1 : 0;
+  // extension: raycast primary walks longer than this many steps finish in a
+  // second launch, 16 lanes per ray (same pixels); unset: tuned per frame
+  r.defer_steps = std::max(get<int>("longRaySteps", ANARI_INT32, 0), 0);
   tuneWidth = find("nodeWidth") == nullptr;
   tuneStack = find("stackDepthLDS") == nullptr;
+  tuneDefer = find("longRaySteps") == nullptr && r.kind == VRX_RENDER_RAYCAST;
   sampleLimit = get<int>("sampleLimit", ANARI_INT32, 128);
   if (kind == VRX_RENDER_RAYCAST)
     sampleLimit = 1; // single-shot renderer
@@ -1885,6 +1889,10 @@ bool Frame::ready()
   return vrx_event_query(evDone) == 1 && callbacksDone();
 }
 
+// step budget of the long-ray pass the frame autotuner tries (raycast of
+// one-level scenes with one primitive type)
+constexpr int kLongRaySteps = 32;
+
 void Frame::pickLaunch(vrx_renderer &rr)
 {
   uint64_t key = (uint64_t)(uintptr_t)world ^ (world->builtAt << 20) ^ (uint64_t)(uintptr_t)renderer
@@ -1892,14 +1900,22 @@ void Frame::pickLaunch(vrx_renderer &rr)
   if (key != tune.key) {
     tune = Tune{};
     tune.key = key;
-    const int widths[2] = {4, 2}, sds[2] = {16, 32};
+    const int widths[2] = {4, 2}, sds[2] = {16, 32}, defers[2] = {0, kLongRaySteps};
     const int nw = renderer->tuneWidth ? 2 : 1, ns = renderer->tuneStack ? 2 : 1;
-    for (int a = 0; a < ns; a++)
-      for (int b = 0; b < nw; b++) {
-        tune.width[tune.n] = renderer->tuneWidth ? widths[b] : rr.node_width;
-        tune.sd[tune.n] = renderer->tuneStack ? sds[a] : rr.stack_lds;
-        tune.n++;
-      }
+    // the long-ray pass applies to one-level scenes of one primitive type
+    const int st = world->scene.single_type;
+    const bool defer = renderer->tuneDefer && world->scene.num_entries == 1
+        && world->scene.num_volumes == 0
+        && (st == VRX_TRIANGLE || st == VRX_SPHERE || st == VRX_CYLINDER);
+    const int nd = defer ? 2 : 1;
+    for (int c = 0; c < nd; c++)
+      for (int a = 0; a < ns; a++)
+        for (int b = 0; b < nw; b++) {
+          tune.width[tune.n] = renderer->tuneWidth ? widths[b] : rr.node_width;
+          tune.sd[tune.n] = renderer->tuneStack ? sds[a] : rr.stack_lds;
+          tune.defer[tune.n] = defer ? defers[c] : rr.defer_steps;
+          tune.n++;
+        }
   } else if (tune.best < 0 && tune.last >= 0) {
     // the previous render used candidate `last`; wait() has made its events
     // complete
@@ -1919,6 +1935,7 @@ void Frame::pickLaunch(vrx_renderer &rr)
   tune.last = tune.best >= 0 ? -1 : use;
   rr.node_width = tune.width[use];
   rr.stack_lds = tune.sd[use];
+  rr.defer_steps = tune.defer[use];
 }
 
 void Frame::render()
@@ -2090,11 +2107,13 @@ bool Frame::getProperty(const std::string &name, ANARIDataType t, void *mem, uin
     memcpy(mem, &v, 4);
     return true;
   }
-  if ((name == "tunedNodeWidth" || name == "tunedStackDepth") && t == ANARI_INT32 && sz >= 4) {
+  if ((name == "tunedNodeWidth" || name == "tunedStackDepth" || name == "tunedLongRaySteps")
+      && t == ANARI_INT32 && sz >= 4) {
     // the launch configuration the autotuner settled on (0 while tuning)
     int v = 0;
     if (tune.best >= 0)
-      v = name == "tunedNodeWidth" ? tune.width[tune.best] : tune.sd[tune.best];
+      v = name == "tunedNodeWidth" ? tune.width[tune.best]
+          : name == "tunedStackDepth" ? tune.sd[tune.best] : tune.defer[tune.best];
     memcpy(mem, &v, 4);
     return true;
   }

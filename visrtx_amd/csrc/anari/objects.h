@@ -320,7 +320,7 @@ struct Renderer : Object {
   int sampleLimit{128};
   float lightFalloff{1.0f};
   // launch parameters left to the frame's autotuner (not set explicitly)
-  bool tuneWidth{true}, tuneStack{true};
+  bool tuneWidth{true}, tuneStack{true}, tuneDefer{false};
   int spp() const
   {
     return r.kind == VRX_RENDER_RAYCAST ? 1 : (r.pixel_samples > 1 ? r.pixel_samples : 1);
@@ -361,8 +361,8 @@ struct Frame : Object {
   struct Tune {
     uint64_t key{0};
     int n{0}, next{0}, best{-1}, last{-1};
-    int width[4]{}, sd[4]{};
-    float ms[4]{};
+    int width[8]{}, sd[8]{}, defer[8]{};
+    float ms[8]{};
   } tune;
   void pickLaunch(vrx_renderer &rr);
   // completion callbacks: each render with a callback posts one job; the

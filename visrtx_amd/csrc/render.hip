@@ -25,9 +25,10 @@
 
 namespace vrx {
 
-template <int PT, bool TWO, int KIND, bool WIDE>
+template <int PT, bool TWO, int KIND, bool WIDE, int LONG = 0>
 __device__ __forceinline__ void render_tile(const vrx_scene &sc, const vrx_camera &cam,
-    const vrx_renderer &r, const vrx_frame &f, uint32_t sd, uint32_t ovf_cap, uint32_t *scratch)
+    const vrx_renderer &r, const vrx_frame &f, uint32_t sd, uint32_t ovf_cap, uint32_t *scratch,
+    const LongRays *lr = nullptr)
 {
   extern __shared__ uint32_t s_stack[];
   const int tid = threadIdx.x;
@@ -51,8 +52,8 @@ __device__ __forceinline__ void render_tile(const vrx_scene &sc, const vrx_camer
 
   uint32_t rays = 0, vsamples = 0;
   if (inside)
-    render_pixel<PT, TWO, KIND, WIDE>(sc, cam, r, f, px, py, (size_t)px + (size_t)py * f.width, st, rays,
-        vsamples);
+    render_pixel<PT, TWO, KIND, WIDE, LONG>(sc, cam, r, f, px, py, (size_t)px + (size_t)py * f.width,
+        st, rays, vsamples, lr);
   if (f.d_counters) {
     wave_count_add(f.d_counters, 0, rays);
     wave_count_add(f.d_counters, 1, vsamples);
@@ -79,6 +80,52 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
     uint32_t *scratch)
 {
   render_tile<PT, TWO, KIND, WIDE>(sc, cam, r, f, sd, ovf_cap, scratch);
+}
+
+// The one-level primitive raycast with a step budget: walks longer than
+// lr.budget loop iterations queue their pixel for k_render_long.
+template <int PT, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_render_defer(vrx_scene sc, vrx_camera cam,
+    vrx_renderer r, vrx_frame f, uint32_t sd, uint32_t ovf_cap, uint32_t *scratch, LongRays lr)
+{
+  render_tile<PT, false, VRX_RENDER_RAYCAST, WIDE, 1>(sc, cam, r, f, sd, ovf_cap, scratch, &lr);
+}
+
+// The queued pixels, 16 lanes each (four per wave), over a resident grid.
+// Dynamic LDS: the lanes' private short stacks (sd entries each), then one
+// shared stack of lr.gcap entries per lane group.
+template <int PT, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_render_long(vrx_scene sc, vrx_camera cam,
+    vrx_renderer r, vrx_frame f, uint32_t sd, uint32_t ovf_cap, uint32_t *scratch, LongRays lr)
+{
+  extern __shared__ uint32_t s_stack[];
+  const int tid = threadIdx.x;
+  lr.gs = s_stack + sd * kBlock + (uint32_t)(tid >> 4) * lr.gcap;
+  Stack st;
+  st.lds = s_stack + tid;
+  st.stride = kBlock;
+  st.sd = sd;
+  st.sd_mask = sd - 1;
+  const uint32_t gthreads = gridDim.x * kBlock;
+  st.ovf = scratch + 64 + blockIdx.x * kBlock + tid;
+  st.ovf_stride = gthreads;
+  st.ovf_cap = ovf_cap;
+  st.err = scratch;
+  st.cnt = 0;
+  const uint32_t n = gld(lr.count, 0);
+  const uint32_t groups = gridDim.x * (kBlock / 16);
+  uint32_t rays = 0, vsamples = 0;
+  for (uint32_t i = blockIdx.x * (kBlock / 16) + ((uint32_t)tid >> 4); i < n; i += groups) {
+    const uint32_t idx = gld(lr.queue, i);
+    const uint32_t py = idx / f.width, px = idx - py * f.width;
+    uint32_t rr = 0;
+    render_pixel<PT, false, VRX_RENDER_RAYCAST, WIDE, 2>(sc, cam, r, f, px, py, idx, st, rr,
+        vsamples, &lr);
+    if ((tid & 15) == 0)
+      rays += rr;
+  }
+  if (f.d_counters)
+    wave_count_add(f.d_counters, 0, rays);
 }
 
 // Raycast renderer on a world holding exactly one volume and no surfaces:
@@ -594,12 +641,62 @@ static uint32_t ovf_cap_of(const vrx_scene *sc, uint32_t sd, bool wide)
   return m > sd ? m - sd : 0u;
 }
 
+static int num_cus()
+{
+  static int n = 0;
+  if (!n) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess)
+      return 256;
+    n = p.multiProcessorCount;
+  }
+  return n;
+}
+
+// byte offset of the long-ray queue in the frame scratch: after the header
+// and the stack-spill area of one lane per pixel of the 16x16-tile grid
+static size_t long_queue_offset(uint32_t width, uint32_t height, uint32_t cap)
+{
+  const size_t tiles = (size_t)((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
+  return 256 + tiles * kBlock * (size_t)cap * 4;
+}
+
 template <int PT, bool TWO, int KIND, bool WIDE>
 static void launch_one(const vrx_scene &sc, const vrx_camera &cam, const vrx_renderer &r,
     const vrx_frame &f, uint32_t sd, uint32_t cap, hipStream_t s)
 {
   dim3 grid((f.width + kTile - 1) / kTile, (f.height + kTile - 1) / kTile);
   size_t lds = (size_t)sd * kBlock * 4;
+  if constexpr (PT >= 0 && !TWO && KIND == VRX_RENDER_RAYCAST) {
+    if (r.defer_steps > 0) {
+      LongRays lr;
+      lr.budget = (uint32_t)r.defer_steps;
+      // shared-stack capacity of a lane group: each step adds at most three
+      // entries per lane (one with BVH2 nodes) above the entries of the
+      // levels still open; pushes past it go to the lanes' private stacks
+      const uint32_t levels = WIDE ? (sc.max_blas_height + 1) / 2 : sc.max_blas_height;
+      const uint32_t room = (uint32_t)((size_t(128) << 10) - lds) / (4u * (kBlock / 16)); // 128 KiB of LDS
+      lr.gcap = std::min(std::max(((WIDE ? 48u : 16u) * (levels + 2) + 63u) & ~63u, 64u), room & ~63u);
+      lr.gs = nullptr;
+      lr.count = (uint32_t *)f.d_scratch + 1; // header word 1
+      lr.queue = (uint32_t *)((char *)f.d_scratch + long_queue_offset(f.width, f.height, cap));
+      (void)vrx_fill_u32(lr.count, 0u, 1, (vrx_stream)s);
+      hipLaunchKernelGGL((k_render_defer<PT, WIDE>), grid, dim3(kBlock), lds, s, sc, cam, r, f, sd,
+          cap, (uint32_t *)f.d_scratch, lr);
+      // a resident grid for the queued rays, no larger than the first pass
+      // (whose lanes the spill area is sized for)
+      const size_t lds2 = lds + (size_t)(kBlock / 16) * lr.gcap * 4;
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&k_render_long<PT, WIDE>,
+              kBlock, lds2) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+      const uint32_t blocks = std::min((uint32_t)(per_cu * num_cus()), grid.x * grid.y);
+      hipLaunchKernelGGL((k_render_long<PT, WIDE>), dim3(blocks), dim3(kBlock), lds2, s, sc, cam,
+          r, f, sd, cap, (uint32_t *)f.d_scratch, lr);
+      return;
+    }
+  }
   if constexpr (PT >= 0 && (KIND != VRX_RENDER_RAYCAST || TWO))
     hipLaunchKernelGGL((k_render_occ6<PT, TWO, KIND, WIDE>), grid, dim3(kBlock), lds, s, sc, cam, r,
         f, sd, cap, (uint32_t *)f.d_scratch);
@@ -654,8 +751,8 @@ size_t vrx_render_scratch_bytes(const vrx_scene *scene, const vrx_renderer *r, u
     return 0;
   uint32_t sd = pick_sd(r);
   uint32_t cap = ovf_cap_of(scene, sd, wide_of(r));
-  size_t tiles = (size_t)((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
-  return 256 + tiles * kBlock * (size_t)cap * 4;
+  const size_t queue = r && r->defer_steps > 0 ? (size_t)width * height * 4 : 0;
+  return long_queue_offset(width, height, cap) + queue;
 }
 
 int vrx_frame_clear(const vrx_frame *f, vrx_stream s)
@@ -682,6 +779,7 @@ int vrx_render(const vrx_scene *scene, const vrx_camera *cam, const vrx_renderer
   VRX_ARG(r->kind >= VRX_RENDER_RAYCAST && r->kind <= VRX_RENDER_DIRECT, "renderer kind");
   VRX_ARG(scene->num_lights == 0 || scene->d_lights, "light array");
   VRX_ARG(r->node_width == 0 || r->node_width == 2 || r->node_width == 4, "node width 2 or 4");
+  VRX_ARG(r->defer_steps >= 0, "defer_steps");
   VRX_ARG(!wide_of(r) || scene->num_entries == 0 || scene->wide,
       "node width 4 needs 4-wide nodes for every BVH (vrx_bvh_widen, scene->wide)");
   uint32_t sd = pick_sd(r);

@@ -706,17 +706,214 @@ __device__ float4 shade_direct(const vrx_scene &sc, const vrx_renderer &r, const
 }
 
 // --------------------------------------------------------------------------
+// long primary rays (raycast renderer, one-level scenes of one primitive type)
+// --------------------------------------------------------------------------
+// A launch waits for its slowest wave, and on scenes seen at grazing angles
+// (a heightfield's horizon) a few rays walk far more nodes than the rest,
+// one dependent fetch at a time.  With a step budget, the first pass gives
+// up on a ray after `budget` loop iterations and queues its pixel; a second
+// launch gives each queued ray 16 lanes that walk its BVH together
+// (trace_group) and takes the closest hit over the lanes with take()'s rule
+// (smaller t, then lower entry and primitive).  Every intersection test is
+// the same, so the pixel comes out as the single walk would give it.
+struct LongRays {
+  uint32_t budget;  // loop iterations before a primary walk is deferred
+  uint32_t *count;  // queued pixels
+  uint32_t *queue;  // one slot per pixel
+  uint32_t *gs;     // second pass: this lane group's shared stack (LDS)
+  uint32_t gcap;    // its capacity in entries
+};
+
+// queue pixel idx (one atomic per wave: the lanes deferring together)
+__device__ __forceinline__ void defer_pixel(const LongRays &lr, uint32_t idx)
+{
+  const uint64_t m = __ballot(1);
+  const uint32_t n = (uint32_t)__popcll(m);
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (rank == 0)
+    base = atomicAdd(lr.count, n);
+  base = __shfl(base, leader, 64);
+  gst(lr.queue, base + rank, idx);
+}
+
+// Closest hit of one ray walked by the 16 lanes of an aligned lane group
+// that share one LIFO stack in LDS (gs, gcap entries): each step, every lane
+// without a node takes one from its private stack or the shared one, visits
+// it, keeps its nearest hit child and pushes the other hit children to the
+// shared stack; boxes are culled against the group's closest hit so far.
+// Each lane keeps its own closest hit by take() (so ties resolve as in one
+// walk) and the group takes the closest over its lanes.  A push that would
+// overflow the shared stack goes to the lanes' private stacks instead.
+template <int PT, bool WIDE>
+__device__ __forceinline__ bool trace_group(const vrx_scene &sc, v3 wo, v3 wd, float tmin,
+    float tmax, bool cull, Stack &st, Hit &h, uint32_t *gs, uint32_t gcap)
+{
+  h.found = false;
+  h.t = tmax;
+  h.entry = 0;
+  h.prim = 0;
+  h.a0 = h.a1 = h.a2 = 0.0f;
+  if (sc.num_entries == 0)
+    return false;
+  st.cnt = 0;
+  BlasCtx b;
+  v3 o, d;
+  uint32_t root, n;
+  load_blas(sc, 0u, wo, wd, cull, false, b, o, d, root, n);
+  if (n == 0)
+    return false;
+  const v3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  WT wt;
+  if (blas_type<PT>(b) == VRX_TRIANGLE)
+    wt = wt_setup(d);
+  const uint32_t lane = threadIdx.x & 63u, s = lane & 15u;
+  const uint64_t gmask = 0xFFFFull << (lane & ~15u);
+  const uint64_t below = (1ull << lane) - 1ull;
+  bool has = s == 0;
+  uint32_t node = root;
+  uint32_t top = 0;  // shared stack entries (the same in all 16 lanes)
+  float tb = tmax;   // the group's closest hit so far
+  for (;;) {
+    if (!has && st.cnt > 0) {
+      node = st.pop();
+      has = true;
+    }
+    const uint64_t need = __ballot(!has) & gmask;
+    const uint32_t take = min((uint32_t)__popcll(need), top);
+    if (!has) {
+      const uint32_t rank = (uint32_t)__popcll(need & below);
+      if (rank < take) {
+        node = gs[top - 1u - rank];
+        has = true;
+      }
+    }
+    top -= take;
+    if ((__ballot(has) & gmask) == 0ull)
+      break;
+    // visit: up to three children to hand out (the nearest is kept)
+    uint32_t out[3];
+    uint32_t nout = 0;
+    if (has) {
+      if (WIDE && !(node & kLeafBit)) {
+        const Node4 nd = gld_node4(b.wnodes, node);
+        const float lx[4] = {nd.lox.x, nd.lox.y, nd.lox.z, nd.lox.w};
+        const float ly[4] = {nd.loy.x, nd.loy.y, nd.loy.z, nd.loy.w};
+        const float lz[4] = {nd.loz.x, nd.loz.y, nd.loz.z, nd.loz.w};
+        const float hx[4] = {nd.hix.x, nd.hix.y, nd.hix.z, nd.hix.w};
+        const float hy[4] = {nd.hiy.x, nd.hiy.y, nd.hiy.z, nd.hiy.w};
+        const float hz[4] = {nd.hiz.x, nd.hiz.y, nd.hiz.z, nd.hiz.w};
+        uint32_t l[4] = {nd.link.x, nd.link.y, nd.link.z, nd.link.w};
+        float t[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          float tn;
+          const bool hit = (uint32_t)c < nd.meta.x
+              && slab(lx[c], ly[c], lz[c], hx[c], hy[c], hz[c], o, inv, tmin, tb, tn);
+          t[c] = hit ? tn : INFINITY;
+        }
+        cswap(t[0], l[0], t[1], l[1]);
+        cswap(t[2], l[2], t[3], l[3]);
+        cswap(t[0], l[0], t[2], l[2]);
+        cswap(t[1], l[1], t[3], l[3]);
+        cswap(t[1], l[1], t[2], l[2]);
+        has = t[0] < INFINITY;
+        node = l[0];
+        if (t[3] < INFINITY)
+          out[nout++] = l[3];
+        if (t[2] < INFINITY)
+          out[nout++] = l[2];
+        if (t[1] < INFINITY)
+          out[nout++] = l[1];
+      } else if (!(node & kLeafBit)) {
+        const Node nd = gld_node(b.nodes, node);
+        float tn0, tn1;
+        const bool h0 = slab(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, tmin, tb, tn0);
+        const bool h1 = slab(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, tmin, tb, tn1);
+        has = h0 || h1;
+        if (h0 && h1) {
+          const bool swap = tn1 < tn0;
+          node = swap ? nd.d.y : nd.d.x;
+          out[nout++] = swap ? nd.d.x : nd.d.y;
+        } else {
+          node = h0 ? nd.d.x : nd.d.y;
+        }
+      } else {
+        float tcur = h.t;
+        (void)test_leaf<PT, false>(b, node & ~kLeafBit, o, d, wt, tmin, tcur, h);
+        has = false;
+      }
+    }
+    // hand the other children to the group: exclusive prefix sum over the 16 lanes
+    uint32_t incl = nout;
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) {
+      const uint32_t v = (uint32_t)__shfl_up((int)incl, (unsigned)m, 16);
+      if (s >= (uint32_t)m)
+        incl += v;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, 15, 16);
+    if (top + total <= gcap) {
+      const uint32_t at = top + incl - nout;
+      for (uint32_t j = 0; j < nout; j++)
+        gs[at + j] = out[j];
+      top += total;
+    } else {
+      for (uint32_t j = 0; j < nout; j++)
+        st.push(out[j]);
+    }
+    // the group's culling bound: the closest hit of any lane
+    float t = h.found ? h.t : tmax;
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1)
+      t = fminf(t, __shfl_xor(t, m, 64));
+    tb = t;
+  }
+  bool found = h.found;
+#pragma unroll
+  for (int m = 1; m < 16; m <<= 1) {
+    const bool of = __shfl_xor((int)found, m, 64) != 0;
+    const float ot = __shfl_xor(h.t, m, 64);
+    const uint32_t oe = (uint32_t)__shfl_xor((int)h.entry, m, 64);
+    const uint32_t op = (uint32_t)__shfl_xor((int)h.prim, m, 64);
+    const float oa0 = __shfl_xor(h.a0, m, 64), oa1 = __shfl_xor(h.a1, m, 64),
+                oa2 = __shfl_xor(h.a2, m, 64);
+    const bool better = of
+        && (!found || ot < h.t || (ot == h.t && (oe < h.entry || (oe == h.entry && op < h.prim))));
+    if (better) {
+      found = true;
+      h.t = ot;
+      h.entry = oe;
+      h.prim = op;
+      h.a0 = oa0;
+      h.a1 = oa1;
+      h.a2 = oa2;
+    }
+  }
+  h.found = found;
+  return found;
+}
+
+// --------------------------------------------------------------------------
 // the per-pixel program
 // --------------------------------------------------------------------------
 // One pixel of the raycast / AO / directLight program (Raycast_ptx.cu:60-179,
 // AmbientOcclusion_ptx.cu:86-220, DirectLight_ptx.cu:311-418): all pixel
 // samples, the surface/volume layer loop, accumulation and channel writes at
 // output index idx.
-template <int PT, bool TWO, int KIND, bool WIDE = false>
+// LONG (raycast, one-level scenes of one primitive type only): 1 = walks
+// over lr.budget iterations queue their pixel and write nothing; 2 = the
+// pixel's walks are shared by its 16-lane group (trace_group), and the
+// group's first lane writes it.
+template <int PT, bool TWO, int KIND, bool WIDE = false, int LONG = 0>
 __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_camera &cam,
     const vrx_renderer &r, const vrx_frame &f, uint32_t px, uint32_t py, size_t idx, Stack &st,
-    uint32_t &rays, uint32_t &vsamples)
+    uint32_t &rays, uint32_t &vsamples, const LongRays *lr = nullptr)
 {
+  static_assert(LONG == 0 || (KIND == VRX_RENDER_RAYCAST && !TWO && PT >= 0),
+      "long-ray passes are for the one-level primitive raycast");
   const float iw = 1.0f / (float)f.width, ih = 1.0f / (float)f.height;
   constexpr bool kSampled = KIND == VRX_RENDER_AO || KIND == VRX_RENDER_DIRECT;
   const int spp = kSampled ? max(r.pixel_samples, 1) : 1;
@@ -757,8 +954,23 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
     for (int layer = 0; layer < kMaxSurfaceLayers && outo < 0.99f; layer++) {
       Hit h;
       rays++;
-      bool found = TWO ? trace<PT, true, false, WIDE>(sc, o, d, tlower, tmax, cull, st, h)
-                       : trace<PT, false, false, WIDE>(sc, o, d, tlower, tmax, cull, st, h);
+      bool found;
+      if constexpr (LONG == 1) {
+        bool deferred = false;
+        found = trace<PT, false, false, WIDE, true>(sc, o, d, tlower, tmax, cull, st, h, lr->budget,
+            &deferred);
+        if (deferred) {
+          // the second pass renders the whole pixel and counts its rays
+          defer_pixel(*lr, (uint32_t)idx);
+          rays -= (uint32_t)(layer + 1);
+          return;
+        }
+      } else if constexpr (LONG == 2) {
+        found = trace_group<PT, WIDE>(sc, o, d, tlower, tmax, cull, st, h, lr->gs, lr->gcap);
+      } else {
+        found = TWO ? trace<PT, true, false, WIDE>(sc, o, d, tlower, tmax, cull, st, h)
+                    : trace<PT, false, false, WIDE>(sc, o, d, tlower, tmax, cull, st, h);
+      }
       v3 color = mk(0.0f, 0.0f, 0.0f);
       float opacity = 0.0f;
       if (found) {
@@ -856,7 +1068,8 @@ __device__ __forceinline__ void render_pixel(const vrx_scene &sc, const vrx_came
       ids_changed = true;
     }
   }
-  write_pixel(f, idx, accum, spp, dcur, ids_changed, pid, oid, iid);
+  if (LONG != 2 || (threadIdx.x & 15u) == 0)
+    write_pixel(f, idx, accum, spp, dcur, ids_changed, pid, oid, iid);
 }
 
 } // namespace vrx

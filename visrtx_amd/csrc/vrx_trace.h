@@ -596,9 +596,11 @@ __device__ __forceinline__ bool visit4(const Node4 &nd, v3 o, v3 inv, float tmin
   return true;
 }
 
-template <int PT, bool TWO_LEVEL, bool ANY, bool WIDE = false>
+// BUDGET: the walk gives up after `budget` loop iterations and sets
+// *deferred (the caller finishes that ray another way); h is then partial.
+template <int PT, bool TWO_LEVEL, bool ANY, bool WIDE = false, bool BUDGET = false>
 __device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax, bool cull_all,
-    Stack &st, Hit &h)
+    Stack &st, Hit &h, uint32_t budget = 0, bool *deferred = nullptr)
 {
   h.found = false;
   h.t = tmax;
@@ -637,7 +639,14 @@ __device__ bool trace(const vrx_scene &sc, v3 wo, v3 wd, float tmin, float tmax,
     wnodes = b.wnodes;
     node = root;
   }
+  [[maybe_unused]] uint32_t iters = 0;
   for (;;) {
+    if constexpr (BUDGET) {
+      if (++iters > budget) {
+        *deferred = true;
+        return false;
+      }
+    }
     bool do_pop = true;
     if (WIDE && !(node & kLeafBit)) {
       const Node4 nd = gld_node4(wnodes, node);

@@ -495,7 +495,7 @@ def make_renderer(dev: A.Device, s: AnariScene, kind: str = "raycast", **params)
              "ambientOcclusionDistance": A.FLOAT32, "cullTriangleBackfaces": A.BOOL,
              "volumeSamplingRate": A.FLOAT32, "sampleLimit": A.INT32, "stackDepthLDS": A.INT32,
              "volumeUnroll": A.INT32, "nodeWidth": A.INT32, "lightFalloff": A.FLOAT32,
-             "emptySpaceSkipping": A.BOOL, "volumeJitter": A.BOOL}
+             "emptySpaceSkipping": A.BOOL, "volumeJitter": A.BOOL, "longRaySteps": A.INT32}
     for k, v in params.items():
         dev.set(r, k, v, types[k])
     dev.commit(r)
