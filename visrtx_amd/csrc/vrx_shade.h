@@ -18,6 +18,7 @@ constexpr int kBlock = kTile * kTile;
 constexpr int kMaxSurfaceLayers = 4096;
 constexpr uint32_t kMaxSamplesPerSegment = 1u << 24;
 constexpr int kMaxVolumeSegments = 256;
+constexpr int kLookCells = 3; // macro cells looked up together by the skipping march
 
 // --------------------------------------------------------------------------
 // camera
@@ -341,32 +342,70 @@ __device__ __forceinline__ void march_xp(const VolCtx &c, const float4 *lut, v3 
   const float goy = (o.y - c.origin[1]) * c.inv_sp[1], gdy = d.y * c.inv_sp[1];
   const float goz = (o.z - c.origin[2]) * c.inv_sp[2], gdz = d.z * c.inv_sp[2];
   float T = 1.0f;
+  // macro-cell walk: per axis the reciprocal direction in voxel units
+  [[maybe_unused]] float ig[3] = {0.0f, 0.0f, 0.0f};
+  if constexpr (SKIP) {
+    ig[0] = gdx != 0.0f ? 1.0f / gdx : 0.0f;
+    ig[1] = gdy != 0.0f ? 1.0f / gdy : 0.0f;
+    ig[2] = gdz != 0.0f ? 1.0f / gdz : 0.0f;
+  }
   for (uint32_t k = 0; k < kMaxSamplesPerSegment; k += U) {
     if (SKIP && c.mc) {
-      // empty-space skipping: when the macro cell of sample k has a 0
-      // majorant, every sample up to the ray's exit from that cell adds
-      // nothing (alpha 0 gives a step transmittance of exactly 1), so jump
-      // to the first sample past the exit.  The cell's value range covers
-      // one voxel beyond its box, which absorbs the rounding of the exit.
+      // empty-space skipping: a macro cell with a 0 majorant adds nothing
+      // between the ray's entry and exit (alpha 0 gives a step transmittance
+      // of exactly 1), so the march jumps to the first sample past the exit.
+      // When the cell of sample k is empty, the next kLookCells - 1 cells
+      // along the ray come from a DDA over the macro grid (arithmetic only),
+      // so their majorant loads are issued together and a run of empty cells
+      // costs one more round trip per kLookCells - 1 cells.  A cell's value
+      // range covers one voxel beyond its box, which absorbs the rounding of
+      // the exits.
       const float t = t0 + (float)k * c.step;
       if (!(t <= t1))
         break;
       const float g[3] = {gox + gdx * t, goy + gdy * t, goz + gdz * t};
       const float gd[3] = {gdx, gdy, gdz}, go[3] = {gox, goy, goz};
-      int ci[3];
+      int cc[3];
 #pragma unroll
       for (int a = 0; a < 3; a++)
-        ci[a] = min(max((int)floorf(g[a] * 0.0625f), 0), c.mcd[a] - 1);
-      if (gld(c.mc + ((size_t)ci[2] * c.mcd[1] + ci[1]) * c.mcd[0] + ci[0]) == 0.0f) {
-        float te = INFINITY;
+        cc[a] = min(max((int)floorf(g[a] * 0.0625f), 0), c.mcd[a] - 1);
+      if (gld(c.mc + ((size_t)cc[2] * c.mcd[1] + cc[1]) * c.mcd[0] + cc[0]) == 0.0f) {
+        float mj[kLookCells], te[kLookCells];
+        bool in[kLookCells];
+        mj[0] = 0.0f;
+        in[0] = true;
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-          if (gd[a] > 0.0f)
-            te = fminf(te, (16.0f * (float)(ci[a] + 1) - go[a]) / gd[a]);
-          else if (gd[a] < 0.0f)
-            te = fminf(te, (16.0f * (float)ci[a] - go[a]) / gd[a]);
+        for (int q = 0; q < kLookCells; q++) {
+          if (q > 0) {
+            in[q] = cc[0] >= 0 && cc[0] < c.mcd[0] && cc[1] >= 0 && cc[1] < c.mcd[1] && cc[2] >= 0
+                && cc[2] < c.mcd[2];
+            mj[q] = in[q] ? gld(c.mc + ((size_t)cc[2] * c.mcd[1] + cc[1]) * c.mcd[0] + cc[0]) : 1.0f;
+          }
+          // exit of this cell and the axis it leaves through
+          float tq = INFINITY;
+          int ax = 0;
+#pragma unroll
+          for (int a = 0; a < 3; a++) {
+            if (gd[a] != 0.0f) {
+              const float e = (16.0f * (float)(cc[a] + (gd[a] > 0.0f ? 1 : 0)) - go[a]) * ig[a];
+              if (e < tq) {
+                tq = e;
+                ax = a;
+              }
+            }
+          }
+          te[q] = tq;
+          cc[ax] += gd[ax] > 0.0f ? 1 : -1;
         }
-        const float kn = ceilf((te - t0) / c.step);
+        int nz = 0;
+#pragma unroll
+        for (int q = 0; q < kLookCells; q++)
+          nz += (nz == q && in[q] && mj[q] == 0.0f) ? 1 : 0;
+        float tn = te[0];
+#pragma unroll
+        for (int q = 1; q < kLookCells; q++)
+          tn = q < nz ? te[q] : tn;
+        const float kn = ceilf((tn - t0) / c.step);
         if (!(kn < (float)kMaxSamplesPerSegment))
           break;
         // k advances by at least one sample; the loop adds U
