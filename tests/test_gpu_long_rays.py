@@ -98,3 +98,23 @@ def test_autotuner_frames(require_gpu):
     assert not dev.errors(), dev.errors()
     s.release()
     dev.close()
+
+
+@pytest.mark.parametrize("stack", [8, 64])
+def test_edge_cases_long_rays(require_gpu, stack):
+    """One primitive (a root leaf), two primitives, and a world with no
+    surfaces, at a budget of one step and at both ends of the LDS stack
+    depth: the same frames as without the pass."""
+    cam = S.CameraDesc((0.5, 0.5, -1.2), (0.0, 0.0, 1.0), (0, 1, 0), 0.8, 4 / 3)
+    for n in (1, 2):
+        c = np.array([[0.5, 0.5, 0.5], [0.55, 0.5, 0.6]], np.float32)[:n]
+        r = np.array([0.2, 0.15], np.float32)[:n]
+        sc = S.SceneDesc(geoms=[S.GeomDesc("sphere", c, radius=r, surface_id=2)], world_geoms=[0],
+                         world_id=0)
+        base = render_gpu(sc, cam, 64, 48, longRaySteps=0, stackDepthLDS=stack)
+        print(f"[long] {n} sphere(s), stack {stack}", flush=True)
+        _same(render_gpu(sc, cam, 64, 48, longRaySteps=1, stackDepthLDS=stack), base)
+        assert (base["depth"] < 1e29).any()
+    sc = S.SceneDesc(world_id=0)
+    base = render_gpu(sc, cam, 32, 24, longRaySteps=0)
+    _same(render_gpu(sc, cam, 32, 24, longRaySteps=1), base)
