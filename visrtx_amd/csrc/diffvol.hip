@@ -325,40 +325,45 @@ __device__ __forceinline__ void cell_of(const DvCtx &c, float gx, float gy, floa
   }
 }
 
-__device__ __forceinline__ void wave_box(int bl[3], int bh[3])
+// Wave-wide min / max with every lane active: within each 16-lane row by DPP
+// (quad_perm xor 1 and xor 2, then row_ror 4 and 8), then across the four rows
+// by readlane, so the reductions stay off the LDS pipe that carries the
+// scatter's atomics.  A source lane DPP cannot read leaves the lane's own
+// value, which min / max absorb.
+template <class Op>
+__device__ __forceinline__ int wave_reduce_i(int x, Op op)
 {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) {
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-      bl[q] = min(bl[q], __shfl_xor(bl[q], m));
-      bh[q] = max(bh[q], __shfl_xor(bh[q], m));
-    }
-  }
+  x = op(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+  x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+  x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x124, 0xF, 0xF, false)); // row_ror:4
+  x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x128, 0xF, 0xF, false)); // row_ror:8
+  return op(op(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16)),
+      op(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
 }
 
 __device__ __forceinline__ int wave_min(int x)
 {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1)
-    x = min(x, __shfl_xor(x, m));
-  return x;
+  return wave_reduce_i(x, [](int a, int b) { return min(a, b); });
 }
 
 __device__ __forceinline__ int wave_max(int x)
 {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1)
-    x = max(x, __shfl_xor(x, m));
-  return x;
+  return wave_reduce_i(x, [](int a, int b) { return max(a, b); });
 }
 
 __device__ __forceinline__ float wave_fmax(float x)
 {
+  return __int_as_float(wave_reduce_i(__float_as_int(x),
+      [](int a, int b) { return __float_as_int(fmaxf(__int_as_float(a), __int_as_float(b))); }));
+}
+
+__device__ __forceinline__ void wave_box(int bl[3], int bh[3])
+{
 #pragma unroll
-  for (int m = 1; m < 64; m <<= 1)
-    x = fmaxf(x, __shfl_xor(x, m));
-  return x;
+  for (int q = 0; q < 3; q++) {
+    bl[q] = wave_min(bl[q]);
+    bh[q] = wave_max(bh[q]);
+  }
 }
 
 // LDS operations of this wave complete before the next ones are issued
