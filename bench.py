@@ -535,7 +535,7 @@ def main():
     if "headline" in sec:
         lbvh_ms, lbvh_all = bench_lbvh(dev, work)
         extra["lbvh_build_ms_10m_spheres"] = round(lbvh_ms, 3)
-        # the survey's build bytes per primitive: AABB read 24, (key, index)
+        # algorithmic build bytes per primitive: AABB read 24, (key, index)
         # write 8, 16 per radix pass (4 passes here), Karras node 32, refit 100
         bpp = 24 + 8 + 16 * 4 + 32 + 100
         gbs_b = 10_000_000 * bpp / (lbvh_ms / 1000) / 1e9
@@ -658,7 +658,7 @@ def main():
         dev.commit(hf.r)
         _, d13 = hf.timed(20, 3)
         extra["heightfield_kernel_ms_no_long_ray_pass"] = round(statistics.mean(d13) * 1000, 4)
-        # roofline of the tuned kernel: the survey's traversal bytes per ray
+        # roofline of the tuned kernel: algorithmic traversal bytes per ray
         # (ray in, hit out, 64-B nodes and 48-B triangles the oracle visits)
         c1 = counts["heightfield_1024x768"]
         bpr1 = 32 + 16 + 64 * c1["nodes_per_ray"] + 48 * c1["prims_per_ray"]
