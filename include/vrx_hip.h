@@ -259,7 +259,6 @@ typedef struct vrx_renderer {
   float inv_volume_sampling_rate;
   /* launch tuning (autotuned; 0 = defaults) */
   int32_t stack_lds;      /* LDS short-stack entries per lane: 8/16/32/64 */
-  int32_t block_pixels;   /* pixels per workgroup: 64/128/256 */
   int32_t vol_unroll;     /* volume samples gathered per step: 1/2/4/8 */
   int32_t node_width;     /* BVH node width traversed: 2 (default) or 4 (needs scene->wide) */
   int32_t skip_empty;     /* 1: skip empty macro cells of volumes that carry d_majorant */

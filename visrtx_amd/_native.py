@@ -103,7 +103,7 @@ class vrx_renderer(C.Structure):
                 ("ambient_radiance", C.c_float), ("pixel_samples", C.c_int32),
                 ("ambient_samples", C.c_int32), ("ao_distance", C.c_float),
                 ("cull_backfaces", C.c_int32), ("inv_volume_sampling_rate", C.c_float),
-                ("stack_lds", C.c_int32), ("block_pixels", C.c_int32), ("vol_unroll", C.c_int32),
+                ("stack_lds", C.c_int32), ("vol_unroll", C.c_int32),
                 ("node_width", C.c_int32), ("skip_empty", C.c_int32),
                 ("volume_jitter", C.c_int32), ("defer_steps", C.c_int32)]
 
