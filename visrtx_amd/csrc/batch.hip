@@ -108,20 +108,9 @@ k_render_batch_raycast(vrx_scene sc, vrx_renderer r, vrx_batch b, uint32_t sd, u
   render_batch<PT, TWO, WIDE, VRX_RENDER_RAYCAST>(sc, r, b, sd, cap, spill, tiles_x, tiles_y);
 }
 
-static int g_num_cus = 0;
-
 static int num_cus()
 {
-  if (!g_num_cus) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess)
-      return 256;
-    hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, dev) != hipSuccess)
-      return 256;
-    g_num_cus = p.multiProcessorCount;
-  }
-  return g_num_cus;
+  return device_cus();
 }
 
 static uint32_t pick_sd(int want)

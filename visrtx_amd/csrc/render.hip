@@ -641,19 +641,6 @@ static uint32_t ovf_cap_of(const vrx_scene *sc, uint32_t sd, bool wide)
   return m > sd ? m - sd : 0u;
 }
 
-static int num_cus()
-{
-  static int n = 0;
-  if (!n) {
-    int dev = 0;
-    hipDeviceProp_t p;
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess)
-      return 256;
-    n = p.multiProcessorCount;
-  }
-  return n;
-}
-
 // byte offset of the long-ray queue in the frame scratch: after the header
 // and the stack-spill area of one lane per pixel of the 16x16-tile grid
 static size_t long_queue_offset(uint32_t width, uint32_t height, uint32_t cap)
@@ -691,7 +678,7 @@ static void launch_one(const vrx_scene &sc, const vrx_camera &cam, const vrx_ren
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&k_render_long<PT, WIDE>,
               kBlock, lds2) != hipSuccess || per_cu < 1)
         per_cu = 1;
-      const uint32_t blocks = std::min((uint32_t)(per_cu * num_cus()), grid.x * grid.y);
+      const uint32_t blocks = std::min((uint32_t)(per_cu * device_cus()), grid.x * grid.y);
       hipLaunchKernelGGL((k_render_long<PT, WIDE>), dim3(blocks), dim3(kBlock), lds2, s, sc, cam,
           r, f, sd, cap, (uint32_t *)f.d_scratch, lr);
       return;

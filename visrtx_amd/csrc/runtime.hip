@@ -9,10 +9,31 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "vrx_internal.h"
 
 namespace vrx {
+
+int device_cus()
+{
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return 256;
+  if (dev >= 0 && dev < 64) {
+    const int n = cache[dev].load(std::memory_order_relaxed);
+    if (n > 0)
+      return n;
+  }
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess || p.multiProcessorCount < 1)
+    return 256;
+  if (dev >= 0 && dev < 64)
+    cache[dev].store(p.multiProcessorCount, std::memory_order_relaxed);
+  return p.multiProcessorCount;
+}
+
 
 static thread_local char g_err[512] = "";
 

@@ -9,6 +9,8 @@
 namespace vrx {
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
+// compute units of the current device (cached per device; 256 if unknown)
+int device_cus();
 } // namespace vrx
 
 #define VRX_HIP(call)                                                   \
