@@ -112,11 +112,15 @@ __global__ __launch_bounds__(kBlock) void k_render_long(vrx_scene sc, vrx_camera
   st.ovf_cap = ovf_cap;
   st.err = scratch;
   st.cnt = 0;
-  const uint32_t n = gld(lr.count, 0);
+  // the queue holds at most one entry per pixel; entries are pixel indices
+  const uint32_t npix = f.width * f.height;
+  const uint32_t n = min(gld(lr.count, 0), npix);
   const uint32_t groups = gridDim.x * (kBlock / 16);
   uint32_t rays = 0, vsamples = 0;
   for (uint32_t i = blockIdx.x * (kBlock / 16) + ((uint32_t)tid >> 4); i < n; i += groups) {
     const uint32_t idx = gld(lr.queue, i);
+    if (idx >= npix)
+      continue;
     const uint32_t py = idx / f.width, px = idx - py * f.width;
     uint32_t rr = 0;
     render_pixel<PT, false, VRX_RENDER_RAYCAST, WIDE, 2>(sc, cam, r, f, px, py, idx, st, rr,
@@ -668,6 +672,8 @@ static void launch_one(const vrx_scene &sc, const vrx_camera &cam, const vrx_ren
       lr.gs = nullptr;
       lr.count = (uint32_t *)f.d_scratch + 1; // header word 1
       lr.queue = (uint32_t *)((char *)f.d_scratch + long_queue_offset(f.width, f.height, cap));
+      // the queue count is reset by a kernel on the stream (a failed launch
+      // is reported by the caller's launch check)
       (void)vrx_fill_u32(lr.count, 0u, 1, (vrx_stream)s);
       hipLaunchKernelGGL((k_render_defer<PT, WIDE>), grid, dim3(kBlock), lds, s, sc, cam, r, f, sd,
           cap, (uint32_t *)f.d_scratch, lr);
