@@ -192,3 +192,21 @@ def test_woodcock_sampling_matches_oracle(require_gpu):
     assert 0.05 < (g_tr == 0).mean() < 0.95
     for b in [db, lb, gb, out_t, out_a, out_tr] + bufs:
         b.free()
+
+
+def test_skipping_march_unroll_depths_give_one_frame(require_gpu):
+    """The grid kernel at volumeUnroll 1, 2, 4 and 8, with and without
+    skipping: unrolling only groups the gathers and a skip lands on the same
+    sample whatever the group, so every frame is bit-identical (and the
+    sample counter drops only with skipping)."""
+    sc, cam = S.sparse_volume_workload(96, width=160, height=100)
+    frames = []
+    for skip in (True, False):
+        for u in (1, 2, 4, 8):
+            print(f"[grid] skip {skip} volumeUnroll {u}", flush=True)
+            frames.append(render_gpu(sc, cam, 160, 100, emptySpaceSkipping=skip,
+                                     volumeSamplingRate=0.5, volumeUnroll=u))
+    for f in frames[1:]:
+        assert not f["errors"], f["errors"]
+        for ch in ("color", "depth", "primitiveId", "objectId", "instanceId"):
+            np.testing.assert_array_equal(f[ch], frames[0][ch], err_msg=ch)
