@@ -94,9 +94,10 @@ def test_accumulation_and_sample_limit(require_gpu):
 
 
 def test_launch_autotuning_keeps_frames_identical(require_gpu):
-    """The frame times node width x LDS stack depth (x the long-ray pass on a
-    one-level scene of one primitive type: eight candidates) on its first
-    frames and keeps the fastest; every frame's channels stay bit-identical."""
+    """The frame times node width x LDS stack depth on its first frames (on a
+    one-level scene of one primitive type, then the fastest of those with the
+    long-ray pass: five candidates) and keeps the fastest; every frame's
+    channels stay bit-identical."""
     dev = A.Device()
     s, cam = _scene(dev)
     c = S.make_camera(dev, s, cam)

@@ -76,9 +76,9 @@ def test_translucent_layers_long_rays(require_gpu):
 
 def test_autotuner_frames(require_gpu):
     """With longRaySteps unset the frame autotuner times each candidate
-    launch (node width x LDS stack x long-ray pass on or off) on its first
-    frames: every frame is the same frame, equal to the one without the
-    pass, and the tuner reports its choice."""
+    launch (node width x LDS stack, then the fastest with the long-ray
+    pass) on its first frames: every frame is the same frame, equal to the
+    one without the pass, and the tuner reports its choice."""
     sc, cam = S.heightfield_workload(width=256, height=192)
     ref = render_gpu(sc, cam, 256, 192, longRaySteps=0)
     dev = A.Device()

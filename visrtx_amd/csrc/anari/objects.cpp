@@ -1900,22 +1900,21 @@ void Frame::pickLaunch(vrx_renderer &rr)
   if (key != tune.key) {
     tune = Tune{};
     tune.key = key;
-    const int widths[2] = {4, 2}, sds[2] = {16, 32}, defers[2] = {0, kLongRaySteps};
+    const int widths[2] = {4, 2}, sds[2] = {16, 32};
     const int nw = renderer->tuneWidth ? 2 : 1, ns = renderer->tuneStack ? 2 : 1;
-    // the long-ray pass applies to one-level scenes of one primitive type
+    // the long-ray pass applies to one-level scenes of one primitive type;
+    // it is tried once, on the fastest node width and stack depth
     const int st = world->scene.single_type;
-    const bool defer = renderer->tuneDefer && world->scene.num_entries == 1
+    tune.deferPending = renderer->tuneDefer && world->scene.num_entries == 1
         && world->scene.num_volumes == 0
         && (st == VRX_TRIANGLE || st == VRX_SPHERE || st == VRX_CYLINDER);
-    const int nd = defer ? 2 : 1;
-    for (int c = 0; c < nd; c++)
-      for (int a = 0; a < ns; a++)
-        for (int b = 0; b < nw; b++) {
-          tune.width[tune.n] = renderer->tuneWidth ? widths[b] : rr.node_width;
-          tune.sd[tune.n] = renderer->tuneStack ? sds[a] : rr.stack_lds;
-          tune.defer[tune.n] = defer ? defers[c] : rr.defer_steps;
-          tune.n++;
-        }
+    for (int a = 0; a < ns; a++)
+      for (int b = 0; b < nw; b++) {
+        tune.width[tune.n] = renderer->tuneWidth ? widths[b] : rr.node_width;
+        tune.sd[tune.n] = renderer->tuneStack ? sds[a] : rr.stack_lds;
+        tune.defer[tune.n] = tune.deferPending ? 0 : rr.defer_steps;
+        tune.n++;
+      }
   } else if (tune.best < 0 && tune.last >= 0) {
     // the previous render used candidate `last`; wait() has made its events
     // complete
@@ -1923,10 +1922,20 @@ void Frame::pickLaunch(vrx_renderer &rr)
     if (vrx_event_elapsed_ms(evStart, evEnd, &ms) == 0)
       tune.ms[tune.last] = ms;
     if (tune.next >= tune.n) {
-      tune.best = 0;
+      int b = 0;
       for (int i = 1; i < tune.n; i++)
-        if (tune.ms[i] < tune.ms[tune.best])
-          tune.best = i;
+        if (tune.ms[i] < tune.ms[b])
+          b = i;
+      if (tune.deferPending) {
+        // one more candidate: the fastest launch with the long-ray pass
+        tune.deferPending = false;
+        tune.width[tune.n] = tune.width[b];
+        tune.sd[tune.n] = tune.sd[b];
+        tune.defer[tune.n] = kLongRaySteps;
+        tune.n++;
+      } else {
+        tune.best = b;
+      }
     }
   }
   const int use = tune.best >= 0 ? tune.best : tune.next;

@@ -361,6 +361,7 @@ struct Frame : Object {
   struct Tune {
     uint64_t key{0};
     int n{0}, next{0}, best{-1}, last{-1};
+    bool deferPending{false}; // the long-ray candidate is still to be added
     int width[8]{}, sd[8]{}, defer[8]{};
     float ms[8]{};
   } tune;
